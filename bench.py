@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""H_eff*psi throughput of the two-site DMRG inner loop on one MI355X.
+
+Workload: the centre bond of the SU(2) J1-J2 Heisenberg cylinder (width 6,
+length 24, snake ordering, 12 open bond operators across the cut) at
+D = 2048 kept multiplets.  Sector dimensions follow a rotor histogram
+D_J ~ (2J+1) exp(-J(J+1) / (2 sigma^2)), sigma = 1.8, fused with the
+spin-1/2 site on each side; operator and wave-function entries are seeded
+Gaussians.  A step is one H_eff*psi application (all 12 bond terms plus the
+two block Hamiltonians) with every input resident in HBM.
+
+Reports the mixed-precision path (FP32 operands on FP32 MFMA, FP64 output)
+as `value`, the FP64 path beside it, per-kernel HIP-event timings on the
+engine's stream, and the reference library's CPU path on the same tensors.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+PEAK_FP64 = 78.6    # TFLOP/s, MI355X dense FP64 matrix
+PEAK_FP32 = 157.3   # TFLOP/s, MI355X dense FP32 matrix
+PEAK_HBM = 8000.0   # GB/s
+
+
+def rotor_dims(D, sigma, two_j_parity=0, jmax2=16):
+    js = list(range(two_j_parity, jmax2 + 1, 2))
+    w = np.array([(j + 1) * np.exp(-(j / 2) * (j / 2 + 1) / (2 * sigma ** 2)) for j in js])
+    d = np.maximum(1, np.round(D * w / w.sum())).astype(int)
+    keep = d >= 2
+    return [j for j, k in zip(js, keep) if k], [int(x) for x, k in zip(d, keep) if k]
+
+
+def fused(js, ds):
+    """fuse(block bond, spin-1/2) sector dimensions (triangle rule)."""
+    out = {}
+    for j, d in zip(js, ds):
+        for jj in (j - 1, j + 1):
+            if jj >= 0:
+                out[jj] = out.get(jj, 0) + d
+    ks = sorted(out)
+    return ks, [out[k] for k in ks]
+
+
+def build_workload(D=2048, n_op=12, sigma=1.8, seed=1):
+    import sadmrg_testlib as tl
+    rng = np.random.default_rng(seed)
+    js, ds = rotor_dims(D, sigma)
+    fj, fd = fused(js, ds)
+    b0 = (1, fj, fd)
+    b1 = (-1, fj, fd)
+    ch = (-1, [2], [1])
+    psi = tl.random_tensor(rng, 1, 0, 0, [b0, b1])
+    psi["data"] /= np.linalg.norm(psi["data"])
+    scale_l = 1.0 / np.sqrt(max(fd))
+    hl = tl.random_tensor(rng, 1, 0, 0, [(-1, fj, fd), b0], scale=scale_l)
+    hr = tl.random_tensor(rng, 1, 0, 0, [(1, fj, fd), b1], scale=scale_l)
+    # symmetrise the block Hamiltonians: (A + A^T) / 2 per diagonal block
+    for h in (hl, hr):
+        off = 0
+        _, _, bonds = tl.parse_header(h["header"])
+        for b in range(tl.nten(bonds)):
+            if not h["present"][b]:
+                continue
+            dims = tl.block_dims(bonds, b)
+            n = dims[0] * dims[1]
+            m = h["data"][off:off + n].reshape(dims, order="F")
+            h["data"][off:off + n] = (0.5 * (m + m.T)).ravel(order="F")
+            off += n
+    o1 = [tl.random_tensor(rng, 1, 0, 0, [(-1, fj, fd), ch, b0], scale=scale_l) for _ in range(n_op)]
+    o2 = [tl.random_tensor(rng, 1, 0, 0, [(1, fj, fd), ch, b1], scale=scale_l) for _ in range(n_op)]
+    info = {"two_j_block": js, "dims_block": ds, "two_j_fused": fj, "dims_fused": fd,
+            "n": int(sum(d * d for d in fd)), "n_op": n_op, "D": D, "sigma": sigma}
+    return dict(psi=psi, hl=hl, hr=hr, o1=o1, o2=o2, info=info)
+
+
+class HipTimer:
+    """HIP events recorded on the engine's own stream."""
+
+    def __init__(self, stream):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        self.stream = ctypes.c_void_p(stream)
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e):
+        assert self.hip.hipEventRecord(e, self.stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        t = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(t), a, b) == 0
+        return t.value
+
+
+def time_stage(plan, timer, d_in, d_out, prec, stage, reps):
+    e0, e1 = timer.event(), timer.event()
+    timer.record(e0)
+    for _ in range(reps):
+        plan.apply_stage(d_in, d_out, prec, stage)
+    timer.record(e1)
+    return timer.elapsed_ms(e0, e1) / reps
+
+
+def cpu_baseline(wl, budget_s=20.0):
+    """The reference's hamiltonian_vector_multiplication_idmrg (dmrg_inst.cpp:283-371)
+    on the same tensors, host cores, reference OpenMP over bond terms, sequential MKL."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refbind as rb
+    import sadmrg_testlib as tl
+    if not rb.available():
+        return None
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    n_op = wl["info"]["n_op"]
+    threads = max(1, min(n_op, ncpu))
+    rb.init(physpn=1, max_angm=20, max_dcut=wl["info"]["D"], psize=threads)
+    ns = 2 * n_op
+    d = rb.RefDMRG(1, 0, 4, ns // 4, wl["info"]["D"], 1, 0)
+    hmap = np.zeros((ns, ns), dtype=np.int32)
+    for j in range(n_op):
+        hmap[j, n_op + j] = hmap[n_op + j, j] = 1
+    coup = np.zeros(ns + 1)
+    coup[1] = 1.0
+    d.set_couplings(hmap, coup)
+    il, ir = n_op - 1, n_op
+    d.set_tensor("hh", il, 0, tl.to_ref(wl["hl"]))
+    d.set_tensor("hh", ir, 0, tl.to_ref(wl["hr"]))
+    for j in range(n_op):
+        d.set_tensor("opr", il, j, tl.to_ref(wl["o1"][j]))
+        d.set_tensor("opr", ir, n_op + j, tl.to_ref(wl["o2"][j]))
+    vin = tl.to_ref(wl["psi"], 1)
+    t0 = time.time()
+    sec = d.time_heff(il, ir, vin, 1)
+    reps = 1
+    if sec < budget_s / 3:
+        reps = max(1, int(budget_s / sec) - 1)
+        sec = d.time_heff(il, ir, vin, reps)
+    out = tl.from_ref(d.heff(il, ir, vin))
+    return {"seconds_per_apply": sec, "reps": reps + 1, "threads": threads, "wall": time.time() - t0,
+            "output": out}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--D", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the engine has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import sadmrg_testlib as tl
+    sad = tl.load_pkg()
+    print(f"[bench] building C3 workload D={args.D}", file=sys.stderr, flush=True)
+    wl = build_workload(D=args.D)
+    ctx = sad.Context(0)
+    cgc = sad.Cgc(1)
+    psi = tl.to_dev(ctx, wl["psi"], 1)
+    hl, hr = tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"])
+    o1 = [tl.to_dev(ctx, t) for t in wl["o1"]]
+    o2 = [tl.to_dev(ctx, t) for t in wl["o2"]]
+    plan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
+    st = plan.stats()
+    out = psi.copy()
+    d_in, d_out = psi.device_ptr(), out.device_ptr()
+    timer = HipTimer(ctx.stream())
+    flops = st["flops"]
+    f1, f2 = plan.stage_flops(1), plan.stage_flops(2)
+    print(f"[bench] n={st['n']} flops/apply={flops:.4e} tiles={st['tiles1']}+{st['tiles2']}", file=sys.stderr,
+          flush=True)
+
+    res = {}
+    for name, prec in (("fp64", sad.FP64), ("fp32", sad.FP32)):
+        for _ in range(args.warmup):
+            plan.apply_device(d_in, d_out, prec)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.apply_device(d_in, d_out, prec)
+        ctx.sync()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        s1 = time_stage(plan, timer, d_in, d_out, prec, 1, max(3, args.steps // 2))
+        s2 = time_stage(plan, timer, d_in, d_out, prec, 2, max(3, args.steps // 2))
+        res[name] = {"ms_per_step": wall_ms, "tflops": flops / (wall_ms * 1e-3) / 1e12,
+                     "stage1_ms": s1, "stage2_ms": s2,
+                     "stage1_tflops": f1 / (s1 * 1e-3) / 1e12, "stage2_tflops": f2 / (s2 * 1e-3) / 1e12}
+        print(f"[bench] {name}: {wall_ms:.3f} ms/apply {res[name]['tflops']:.2f} TFLOP/s "
+              f"(stage1 {s1:.3f} ms, stage2 {s2:.3f} ms)", file=sys.stderr, flush=True)
+
+    # accuracy of the mixed path against FP64 on the same input
+    o64 = tl.from_dev(plan.apply(psi, sad.FP64))
+    o32 = tl.from_dev(plan.apply(psi, sad.FP32))
+    rel32, _ = tl.compare(o32, o64)
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] timing the reference CPU path", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(wl, args.cpu_budget)
+        if cpu is not None:
+            relc, _ = tl.compare(o64, cpu["output"])
+            cpu["parity_rel"] = relc
+            print(f"[bench] cpu: {cpu['seconds_per_apply']:.3f} s/apply on {cpu['threads']} threads, "
+                  f"parity {relc:.2e}", file=sys.stderr, flush=True)
+
+    dom = "stage2" if res["fp32"]["stage2_ms"] >= res["fp32"]["stage1_ms"] else "stage1"
+    dom_flops = f2 if dom == "stage2" else f1
+    dom_ms = res["fp32"][dom + "_ms"]
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    line = {
+        "metric": "H_eff*psi TFLOP/s, SU(2) D=2048 centre bond (mixed FP32-MFMA vs FP64)",
+        "value": round(res["fp32"]["tflops"], 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["fp32"]["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 operands, fp32 MFMA accumulate, fp64 output",
+        "data": "synthetic",
+        "config": {"workload": f"SU(2) J1-J2 6x24 cylinder centre bond, D={args.D} multiplets, n_op=12, "
+                               f"N={st['n']} reduced elements, one H_eff*psi per step",
+                   "flops_per_step": flops, "sectors": wl["info"]["two_j_fused"],
+                   "sector_dims": wl["info"]["dims_fused"]},
+        "fp64": {k: round(v, 4) for k, v in res["fp64"].items()},
+        "fp32": {k: round(v, 4) for k, v in res["fp32"].items()},
+        "mixed_vs_fp64_speedup": round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3),
+        "fp32_rel_err_vs_fp64": rel32,
+        "roofline": {"bound": "mfma", "kernel": f"grouped_gemm {dom} (fp32)", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
+                     "traffic": None,
+                     "fp64_path": {"achieved": round(res["fp64"]["tflops"], 3), "peak": PEAK_FP64,
+                                   "frac": round(res["fp64"]["tflops"] / PEAK_FP64, 4)}},
+        "cpu_baseline": None if cpu is None else {
+            "value": round(flops / cpu["seconds_per_apply"] / 1e12, 5), "unit": "TFLOP/s",
+            "cores": cpu["threads"], "kind": "reference",
+            "sample": f"{cpu['reps']} applications of DMRG::hamiltonian_vector_multiplication_idmrg on the same "
+                      f"tensors ({cpu['seconds_per_apply']:.3f} s each, FP64, MKL sequential + reference OpenMP)",
+            "parity_rel_vs_gpu_fp64": cpu["parity_rel"]},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,184 @@
+/*
+ * sadmrg.h -- C ABI of the MI355X two-site DMRG inner-loop engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes.  Every entry point
+ * returns 0 on success or a nonzero status (SADMRG_ERR_*); the message of the
+ * last failure on the calling thread is returned by sadmrg_last_error().
+ * (The reference prints and calls exit() instead, e.g.
+ * src/block_tensor_inst.cpp:2191-2196; there is no C interface in the
+ * reference, whose boundary is its C++ class API: block_tensor.hpp:38-193,
+ * dense.hpp:26-156, lanczos.hpp:55-91, dmrg.hpp:29-133.)
+ *
+ * Block tensors cross the boundary in one format:
+ *   header  = [nbond, locspin, {dir, nmom, q[nmom], dim[nmom]} x nbond]
+ *             q = 2J (SU(2)) or charge (U(1)), sorted ascending per bond
+ *   present = one byte per block; blocks are numbered column-major over the
+ *             bonds' sectors, bond 0 fastest (src/symmetry/su2.cpp:432-450)
+ *   data    = the present blocks, concatenated in block order, each block
+ *             column-major (bond 0 fastest, dense_inst.cpp:31-45); complex
+ *             data is interleaved (re, im)
+ * Device data is FP64 (double or complex<double>).
+ */
+#ifndef SADMRG_H
+#define SADMRG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sadmrg_ctx sadmrg_ctx;       /* one GPU, one stream, caching allocator */
+typedef struct sadmrg_cgc sadmrg_cgc;       /* SU(2) recoupling tables */
+typedef struct sadmrg_tensor sadmrg_tensor; /* BlockTensor<Scalar,Sym> */
+typedef struct sadmrg_heff sadmrg_heff;     /* H_eff matvec plan of one two-site step */
+typedef struct sadmrg_dmrg sadmrg_dmrg;     /* DMRG<Scalar,Sym> driver */
+
+enum {
+  SADMRG_OK = 0,
+  SADMRG_ERR_INVALID = 1,
+  SADMRG_ERR_HIP = 2,
+  SADMRG_ERR_UNSUPPORTED = 3,
+  SADMRG_ERR_NUMERICS = 4,
+  SADMRG_ERR_NO_DEVICE = 5
+};
+enum { SADMRG_U1 = 0, SADMRG_SU2 = 1 };
+enum { SADMRG_REAL = 0, SADMRG_COMPLEX = 1 };
+enum { SADMRG_FP64 = 0, SADMRG_FP32 = 1 };
+
+/* ---- runtime ------------------------------------------------------------ */
+const char* sadmrg_last_error(void);
+int sadmrg_version(int* major, int* minor);
+int sadmrg_ctx_create(int device, sadmrg_ctx** out);
+int sadmrg_ctx_destroy(sadmrg_ctx* ctx);
+int sadmrg_ctx_sync(sadmrg_ctx* ctx);
+int sadmrg_ctx_memory(sadmrg_ctx* ctx, long long* bytes_in_use, long long* bytes_cached);
+/* the context's HIP stream (hipStream_t), for callers that time or order work on it */
+int sadmrg_ctx_stream(sadmrg_ctx* ctx, void** stream);
+
+/* ---- SU(2) tables (src/symmetry/su2.cpp:41-75, src/cgc_tables.cpp:102-357) */
+/* <j1 m1; j2 m2 | j3 m3>, all arguments doubled (CGCTable::cg) */
+int sadmrg_clebsch_gordan(int j1, int m1, int j2, int m2, int j3, int m3, double* out);
+int sadmrg_cgc_create(int physpn, sadmrg_cgc** out);
+int sadmrg_cgc_destroy(sadmrg_cgc* cgc);
+/* fac_hamilt_vec[i][j][k] (cgc_tables.hpp:27) */
+int sadmrg_cgc_hamilt_vec(sadmrg_cgc* cgc, int i, int j, int k, double* out);
+/* fac_operator_onsite_{left(side 0),rght(1)}[j][m][l][i][p] (cgc_tables.hpp:30-31) */
+int sadmrg_cgc_onsite(sadmrg_cgc* cgc, int side, int j, int m, int l, int i, int p, double* out);
+/* fac_operator_transformation_{left,rght}[j][m][l][i][k][p] (cgc_tables.hpp:34-35) */
+int sadmrg_cgc_transformation(sadmrg_cgc* cgc, int side, int j, int m, int l, int i, int k, int p, double* out);
+/* fac_operator_pairup_{left,rght}[j][i][k][m][p] (cgc_tables.hpp:38-39) */
+int sadmrg_cgc_pairup(sadmrg_cgc* cgc, int side, int j, int i, int k, int m, int p, double* out);
+
+/* ---- block tensors (BlockTensor<Scalar,Sym>, block_tensor.hpp:38-193) ---- */
+/* BlockTensor::set(StructType, data, cgc) (block_tensor.hpp:63); cgc2_kind
+   selects the CGC of 2-leg SU(2) blocks: 0 identity, 1 singlet */
+int sadmrg_tensor_create(sadmrg_ctx* ctx, int sym, int cplx, const int* header, int header_len,
+                         const unsigned char* present, const double* data, int cgc2_kind, sadmrg_tensor** out);
+/* empty (null) tensor, the state after BlockTensor::clean() (block_tensor.hpp:59) */
+int sadmrg_tensor_null(sadmrg_ctx* ctx, int sym, int cplx, sadmrg_tensor** out);
+int sadmrg_tensor_destroy(sadmrg_tensor* t);
+int sadmrg_tensor_header(const sadmrg_tensor* t, int* header, int cap, int* len);
+int sadmrg_tensor_nten(const sadmrg_tensor* t, int* nten);
+int sadmrg_tensor_nelem(const sadmrg_tensor* t, long long* nelem);
+/* present: nten bytes; data: nelem (x2 if complex) doubles (BlockTensor::get_elems, block_tensor.hpp:99) */
+int sadmrg_tensor_download(const sadmrg_tensor* t, unsigned char* present, double* data);
+/* BlockTensor::operator= (block_tensor.hpp:107) */
+int sadmrg_tensor_copy(sadmrg_tensor* dst, const sadmrg_tensor* src);
+int sadmrg_tensor_device_ptr(const sadmrg_tensor* t, void** ptr);
+
+/* BlockTensor::hamiltonian_vector_multiplication (block_tensor.hpp:179) */
+int sadmrg_hamiltonian_vector_multiplication(sadmrg_tensor* out, const sadmrg_tensor* vec, const sadmrg_tensor* op1,
+                                             const sadmrg_tensor* op2, sadmrg_cgc* cgc);
+/* BlockTensor::contract (block_tensor.hpp:142) */
+int sadmrg_contract(sadmrg_tensor* out, const sadmrg_tensor* t1, int i1, const sadmrg_tensor* t2, int i2);
+/* Environment updates (block_tensor.hpp:163-176).  kind: 0 overlap_initial,
+   1 overlap_transformation(uu,vv,op1), 2 operator_initial(uu,vv,op1),
+   3 operator_transformation(uu,vv,op1), 4 operator_pairup(uu,vv,op1,op2). */
+int sadmrg_env_update(sadmrg_tensor* out, int kind, const sadmrg_tensor* uu, const sadmrg_tensor* vv,
+                      const sadmrg_tensor* op1, const sadmrg_tensor* op2, int flag, sadmrg_cgc* cgc);
+/* BlockTensor::svd (block_tensor.hpp:135) with the global max_dcut as an
+   argument; wout receives kept then discarded singular values (*nw of them). */
+int sadmrg_svd(const sadmrg_tensor* vec, sadmrg_tensor* tu, double p1, sadmrg_tensor* tv, double p2, int max_dcut,
+               double* wout, int wout_cap, int* nw);
+/* BlockTensor::fuse(BondType, BondType) (block_tensor.hpp:129); bond = [dir, nmom, q..., dim...] */
+int sadmrg_fuse(sadmrg_tensor* out, const int* bond1, const int* bond2);
+/* BlockTensor::fuse_to_singlet (block_tensor.hpp:131) */
+int sadmrg_fuse_to_singlet(sadmrg_tensor* out, const int* bond1, const int* bond2);
+/* BlockTensor::make_spinor_start (block_tensor.hpp:156) */
+int sadmrg_make_spinor_start(sadmrg_tensor* out, int physpn);
+/* In-place transformations.  op: 0 left2right_vectran, 1 right2left_vectran
+   (block_tensor.hpp:124-125), 2 take_conjugate(bond=a) (:145), 3
+   makeup_input_vector (:187), 4 *= (a + i b) (:110-111), 5 normalize_vector
+   (:119), 6 random_initialize_tensor (:189), 7 shift(a, b) (:122),
+   8 operator=(double a) (:106). */
+int sadmrg_tensor_transform(sadmrg_tensor* t, int op, double a, double b);
+/* y += (re + i im) x  (operator+= / -=, block_tensor.hpp:108-109) */
+int sadmrg_tensor_axpy(sadmrg_tensor* y, double re, double im, const sadmrg_tensor* x);
+/* inner_prod_c (block_tensor.hpp:117): out2 = {re, im} of x^H y */
+int sadmrg_tensor_dot(const sadmrg_tensor* x, const sadmrg_tensor* y, double* out2);
+/* reseed the global generator of random_init (dense_inst.cpp:23, default 42) */
+int sadmrg_rng_seed(unsigned long long seed);
+
+/* ---- H_eff plan: DMRG::hamiltonian_vector_multiplication_idmrg
+        (dmrg.hpp:114-115) as one grouped launch pair ------------------------ */
+int sadmrg_heff_create(sadmrg_ctx* ctx, sadmrg_cgc* cgc, const sadmrg_tensor* hl, const sadmrg_tensor* hr, int nterm,
+                       const sadmrg_tensor* const* o1, const sadmrg_tensor* const* o2,
+                       const sadmrg_tensor* psi_layout, sadmrg_heff** out);
+int sadmrg_heff_destroy(sadmrg_heff* h);
+/* psi_out is (re)laid out like psi_layout */
+int sadmrg_heff_apply(sadmrg_heff* h, const sadmrg_tensor* psi_in, sadmrg_tensor* psi_out, int precision);
+/* device pointers in the layout of psi_layout; enqueued on the context stream */
+int sadmrg_heff_apply_device(sadmrg_heff* h, const void* d_in, void* d_out, int precision);
+/* one grouped launch of the pair (stage 1: X = psi^T O1, stage 2: output
+   blocks); stage 0 runs both.  For per-kernel timing. */
+int sadmrg_heff_apply_stage(sadmrg_heff* h, const void* d_in, void* d_out, int precision, int stage);
+int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops);
+/* info[8] = {psi blocks, stage-1 problems, stage-2 problems, stage-1 tiles,
+   stage-2 tiles, stage-1 tile cfg, stage-2 tile cfg, reserved} */
+int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n, int* info);
+/* Lanczos::lanczos1 (lanczos.hpp:70) on the plan; psi is the start vector on
+   entry and the Ritz vector on return.  counters[4] = {FP64 matvecs, FP32
+   matvecs, Lanczos steps, refinement restarts}. */
+int sadmrg_lanczos(sadmrg_heff* h, sadmrg_tensor* psi, int mlanc, int precision, int refine, double refine_tol,
+                   double* energy, int* counters, double* residual);
+
+/* ---- DMRG driver (DMRG<Scalar,Sym>, dmrg.hpp:29-133) --------------------- */
+/* DMRG(xx, yy, bondd, physpn, totspin, 0) (dmrg.hpp:99) */
+int sadmrg_dmrg_create(sadmrg_ctx* ctx, int sym, int cplx, int xx, int yy, int bondd, int physpn, int totspin,
+                       sadmrg_dmrg** out);
+int sadmrg_dmrg_destroy(sadmrg_dmrg* d);
+int sadmrg_dmrg_ns(sadmrg_dmrg* d, int* ns);
+/* replaces hmap (ns x ns, row-major) and coup (dmrg_inst.cpp:117-176) */
+int sadmrg_dmrg_set_couplings(sadmrg_dmrg* d, const int* hmap, const double* coup, int ncoup);
+int sadmrg_dmrg_get_couplings(sadmrg_dmrg* d, int* hmap, double* coup, int cap);
+/* the global max_dcut (main/main_dmrg_su2_real.cpp:32) */
+int sadmrg_dmrg_set_max_dcut(sadmrg_dmrg* d, int max_dcut);
+int sadmrg_dmrg_set_precision(sadmrg_dmrg* d, int precision, int refine, double refine_tol);
+int sadmrg_dmrg_idmrg(sadmrg_dmrg* d);         /* do_idmrg (dmrg.hpp:107) */
+int sadmrg_dmrg_prepare_sweep(sadmrg_dmrg* d); /* prepare_sweep (dmrg.hpp:110) */
+int sadmrg_dmrg_sweep(sadmrg_dmrg* d);         /* sweep (dmrg.hpp:111) */
+int sadmrg_dmrg_energy(sadmrg_dmrg* d, double* e); /* get_enr (dmrg.hpp:102) */
+/* per-step records since the last call: 12 doubles each {il, ir, energy,
+   truncation error, kept, FP64 matvecs, FP32 matvecs, t_env, t_lanczos, t_svd,
+   H_eff flops, 0}; *n = number written; records are consumed */
+int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n);
+/* which: 0 uu[i], 1 hh[i], 2 opr[i][j] */
+int sadmrg_dmrg_get_tensor(sadmrg_dmrg* d, int which, int i, int j, sadmrg_tensor* out);
+int sadmrg_dmrg_set_tensor(sadmrg_dmrg* d, int which, int i, int j, const sadmrg_tensor* t);
+/* step methods: 0 wavefunc_transformation(a,b), 1 prepare_site_operator_from_left(a,b),
+   2 prepare_site_operator_from_right(a,b), 3 prepare_site_operator_from_left(a),
+   4 prepare_site_operator_from_right(a), 5 prepare_sweep() (dmrg.hpp:90-121) */
+int sadmrg_dmrg_call(sadmrg_dmrg* d, int code, int a, int b);
+/* prepare_input_vector (dmrg.hpp:121) */
+int sadmrg_dmrg_input_vector(sadmrg_dmrg* d, int il, int ir, sadmrg_tensor* out);
+/* hamiltonian_vector_multiplication_idmrg (dmrg.hpp:114) */
+int sadmrg_dmrg_heff(sadmrg_dmrg* d, int il, int ir, const sadmrg_tensor* vin, sadmrg_tensor* vout, int precision);
+/* lanczos_solve_eigenvector_idmrg (dmrg.hpp:90); vec updated in place */
+int sadmrg_dmrg_lanczos(sadmrg_dmrg* d, int il, int ir, sadmrg_tensor* vec, double* energy);
+/* vec.svd(uu[a], p1, uu[b], p2, ww) as in DMRG::sweep (dmrg_inst.cpp:894) */
+int sadmrg_dmrg_svd(sadmrg_dmrg* d, const sadmrg_tensor* vec, int a, int b, double p1, double p2, double* wout,
+                    int cap, int* nw);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SADMRG_H */

@@ -1,0 +1,489 @@
+"""ctypes binding of libsadmrg.so (include/sadmrg.h).
+
+Host-side mirror of the reference's BlockTensor / Lanczos / DMRG interface
+for tests and benchmarks.  All arithmetic runs in the HIP library; this module
+only moves tensors across the C ABI.  If the shared library is missing, the
+import of any entry point raises: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsadmrg.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include", "sadmrg.h")
+
+U1, SU2 = 0, 1
+FP64, FP32 = 0, 1
+
+_lib = None
+
+
+class SadmrgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sadmrg error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp, bp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_ubyte)
+    pp = C.POINTER(C.c_void_p)
+    llp = C.POINTER(C.c_longlong)
+    i, d = C.c_int, C.c_double
+    sig = {
+        "sadmrg_last_error": (C.c_char_p, []),
+        "sadmrg_version": (i, [ip, ip]),
+        "sadmrg_ctx_create": (i, [i, pp]),
+        "sadmrg_ctx_destroy": (i, [vp]),
+        "sadmrg_ctx_sync": (i, [vp]),
+        "sadmrg_ctx_memory": (i, [vp, llp, llp]),
+        "sadmrg_ctx_stream": (i, [vp, pp]),
+        "sadmrg_clebsch_gordan": (i, [i, i, i, i, i, i, dp]),
+        "sadmrg_cgc_create": (i, [i, pp]),
+        "sadmrg_cgc_destroy": (i, [vp]),
+        "sadmrg_cgc_hamilt_vec": (i, [vp, i, i, i, dp]),
+        "sadmrg_cgc_onsite": (i, [vp, i, i, i, i, i, i, dp]),
+        "sadmrg_cgc_transformation": (i, [vp, i, i, i, i, i, i, i, dp]),
+        "sadmrg_cgc_pairup": (i, [vp, i, i, i, i, i, i, dp]),
+        "sadmrg_tensor_create": (i, [vp, i, i, ip, i, bp, dp, i, pp]),
+        "sadmrg_tensor_null": (i, [vp, i, i, pp]),
+        "sadmrg_tensor_destroy": (i, [vp]),
+        "sadmrg_tensor_header": (i, [vp, ip, i, ip]),
+        "sadmrg_tensor_nten": (i, [vp, ip]),
+        "sadmrg_tensor_nelem": (i, [vp, llp]),
+        "sadmrg_tensor_download": (i, [vp, bp, dp]),
+        "sadmrg_tensor_copy": (i, [vp, vp]),
+        "sadmrg_tensor_device_ptr": (i, [vp, pp]),
+        "sadmrg_hamiltonian_vector_multiplication": (i, [vp, vp, vp, vp, vp]),
+        "sadmrg_contract": (i, [vp, vp, i, vp, i]),
+        "sadmrg_env_update": (i, [vp, i, vp, vp, vp, vp, i, vp]),
+        "sadmrg_svd": (i, [vp, vp, d, vp, d, i, dp, i, ip]),
+        "sadmrg_fuse": (i, [vp, ip, ip]),
+        "sadmrg_fuse_to_singlet": (i, [vp, ip, ip]),
+        "sadmrg_make_spinor_start": (i, [vp, i]),
+        "sadmrg_tensor_transform": (i, [vp, i, d, d]),
+        "sadmrg_tensor_axpy": (i, [vp, d, d, vp]),
+        "sadmrg_tensor_dot": (i, [vp, vp, dp]),
+        "sadmrg_rng_seed": (i, [C.c_ulonglong]),
+        "sadmrg_heff_create": (i, [vp, vp, vp, vp, i, pp, pp, vp, pp]),
+        "sadmrg_heff_destroy": (i, [vp]),
+        "sadmrg_heff_apply": (i, [vp, vp, vp, i]),
+        "sadmrg_heff_apply_device": (i, [vp, vp, vp, i]),
+        "sadmrg_heff_apply_stage": (i, [vp, vp, vp, i, i]),
+        "sadmrg_heff_stage_flops": (i, [vp, i, dp]),
+        "sadmrg_heff_stats": (i, [vp, dp, dp, llp, ip]),
+        "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
+        "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
+        "sadmrg_dmrg_destroy": (i, [vp]),
+        "sadmrg_dmrg_ns": (i, [vp, ip]),
+        "sadmrg_dmrg_set_couplings": (i, [vp, ip, dp, i]),
+        "sadmrg_dmrg_get_couplings": (i, [vp, ip, dp, i]),
+        "sadmrg_dmrg_set_max_dcut": (i, [vp, i]),
+        "sadmrg_dmrg_set_precision": (i, [vp, i, i, d]),
+        "sadmrg_dmrg_idmrg": (i, [vp]),
+        "sadmrg_dmrg_prepare_sweep": (i, [vp]),
+        "sadmrg_dmrg_sweep": (i, [vp]),
+        "sadmrg_dmrg_energy": (i, [vp, dp]),
+        "sadmrg_dmrg_records": (i, [vp, dp, i, ip]),
+        "sadmrg_dmrg_get_tensor": (i, [vp, i, i, i, vp]),
+        "sadmrg_dmrg_set_tensor": (i, [vp, i, i, i, vp]),
+        "sadmrg_dmrg_call": (i, [vp, i, i, i]),
+        "sadmrg_dmrg_input_vector": (i, [vp, i, i, vp]),
+        "sadmrg_dmrg_heff": (i, [vp, i, i, vp, vp, i]),
+        "sadmrg_dmrg_lanczos": (i, [vp, i, i, vp, dp]),
+        "sadmrg_dmrg_svd": (i, [vp, vp, i, i, d, d, dp, i, ip]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names declared in include/sadmrg.h (parsed from the header)."""
+    import re
+    text = open(INCLUDE).read()
+    return sorted(set(re.findall(r"\b(sadmrg_[a-z0-9_]+)\s*\(", text)))
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().sadmrg_last_error().decode()
+        raise SadmrgError(rc, msg)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _bptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_ubyte))
+
+
+class Context:
+    """One device, one stream (sadmrg_ctx)."""
+
+    def __init__(self, device=0):
+        p = C.c_void_p()
+        check(lib().sadmrg_ctx_create(device, C.byref(p)))
+        self.ptr = p
+
+    def close(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.sadmrg_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def sync(self):
+        check(lib().sadmrg_ctx_sync(self.ptr))
+
+    def stream(self):
+        p = C.c_void_p()
+        check(lib().sadmrg_ctx_stream(self.ptr, C.byref(p)))
+        return p.value
+
+    def memory(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        check(lib().sadmrg_ctx_memory(self.ptr, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+class Cgc:
+    """SU(2) recoupling tables."""
+
+    def __init__(self, physpn=1):
+        p = C.c_void_p()
+        check(lib().sadmrg_cgc_create(physpn, C.byref(p)))
+        self.ptr = p
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.sadmrg_cgc_destroy(self.ptr)
+
+    def hamilt_vec(self, i, j, k):
+        o = C.c_double()
+        check(lib().sadmrg_cgc_hamilt_vec(self.ptr, i, j, k, C.byref(o)))
+        return o.value
+
+    def onsite(self, side, j, m, l, i, p):
+        o = C.c_double()
+        check(lib().sadmrg_cgc_onsite(self.ptr, side, j, m, l, i, p, C.byref(o)))
+        return o.value
+
+    def transformation(self, side, j, m, l, i, k, p):
+        o = C.c_double()
+        check(lib().sadmrg_cgc_transformation(self.ptr, side, j, m, l, i, k, p, C.byref(o)))
+        return o.value
+
+    def pairup(self, side, j, i, k, m, p):
+        o = C.c_double()
+        check(lib().sadmrg_cgc_pairup(self.ptr, side, j, i, k, m, p, C.byref(o)))
+        return o.value
+
+
+def clebsch_gordan(j1, m1, j2, m2, j3, m3):
+    o = C.c_double()
+    check(lib().sadmrg_clebsch_gordan(j1, m1, j2, m2, j3, m3, C.byref(o)))
+    return o.value
+
+
+@dataclass
+class Packed:
+    """Block tensor in the C ABI exchange format (host arrays)."""
+    sym: int
+    cplx: int
+    header: np.ndarray
+    present: np.ndarray
+    data: np.ndarray
+
+
+class Tensor:
+    """Device block tensor (sadmrg_tensor)."""
+
+    def __init__(self, ctx, ptr, sym, cplx):
+        self.ctx, self.ptr, self.sym, self.cplx = ctx, ptr, sym, cplx
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.sadmrg_tensor_destroy(self.ptr)
+            self.ptr = None
+
+    @classmethod
+    def null(cls, ctx, sym, cplx=0):
+        p = C.c_void_p()
+        check(lib().sadmrg_tensor_null(ctx.ptr, sym, cplx, C.byref(p)))
+        return cls(ctx, p, sym, cplx)
+
+    @classmethod
+    def from_packed(cls, ctx, pk, cgc2_kind=0):
+        hdr = np.ascontiguousarray(pk.header, dtype=np.int32)
+        pres = np.ascontiguousarray(pk.present, dtype=np.uint8)
+        data = np.ascontiguousarray(pk.data, dtype=np.float64)
+        if data.size == 0:
+            data = np.zeros(1)
+        if pres.size == 0:
+            pres = np.zeros(1, dtype=np.uint8)
+        p = C.c_void_p()
+        check(lib().sadmrg_tensor_create(ctx.ptr, pk.sym, pk.cplx, _iptr(hdr), len(hdr), _bptr(pres), _dptr(data),
+                                         cgc2_kind, C.byref(p)))
+        return cls(ctx, p, pk.sym, pk.cplx)
+
+    def header(self):
+        n = C.c_int()
+        check(lib().sadmrg_tensor_header(self.ptr, None, 0, C.byref(n)))
+        h = np.zeros(max(n.value, 1), dtype=np.int32)
+        check(lib().sadmrg_tensor_header(self.ptr, _iptr(h), len(h), C.byref(n)))
+        return h[:n.value]
+
+    def nten(self):
+        n = C.c_int()
+        check(lib().sadmrg_tensor_nten(self.ptr, C.byref(n)))
+        return n.value
+
+    def nelem(self):
+        n = C.c_longlong()
+        check(lib().sadmrg_tensor_nelem(self.ptr, C.byref(n)))
+        return n.value
+
+    def packed(self):
+        nt = self.nten()
+        ne = self.nelem()
+        pres = np.zeros(max(nt, 1), dtype=np.uint8)
+        w = 2 if self.cplx else 1
+        data = np.zeros(max(ne * w, 1))
+        check(lib().sadmrg_tensor_download(self.ptr, _bptr(pres), _dptr(data)))
+        return Packed(self.sym, self.cplx, self.header(), pres[:nt].copy(), data[:ne * w].copy())
+
+    def copy(self):
+        t = Tensor.null(self.ctx, self.sym, self.cplx)
+        check(lib().sadmrg_tensor_copy(t.ptr, self.ptr))
+        return t
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        check(lib().sadmrg_tensor_device_ptr(self.ptr, C.byref(p)))
+        return p.value
+
+    def transform(self, op, a=0.0, b=0.0):
+        check(lib().sadmrg_tensor_transform(self.ptr, op, a, b))
+        return self
+
+    def axpy(self, re, x, im=0.0):
+        check(lib().sadmrg_tensor_axpy(self.ptr, re, im, x.ptr))
+        return self
+
+    def dot(self, y):
+        o = np.zeros(2)
+        check(lib().sadmrg_tensor_dot(self.ptr, y.ptr, _dptr(o)))
+        return complex(o[0], o[1])
+
+
+def hvm(vec, op1, op2, cgc=None):
+    out = Tensor.null(vec.ctx, vec.sym, vec.cplx)
+    check(lib().sadmrg_hamiltonian_vector_multiplication(out.ptr, vec.ptr, op1.ptr, op2.ptr,
+                                                         cgc.ptr if cgc else None))
+    return out
+
+
+def contract(t1, i1, t2, i2):
+    out = Tensor.null(t1.ctx, t1.sym, t1.cplx)
+    check(lib().sadmrg_contract(out.ptr, t1.ptr, i1, t2.ptr, i2))
+    return out
+
+
+ENV_KIND = {"overlap_initial": 0, "overlap_transformation": 1, "operator_initial": 2,
+            "operator_transformation": 3, "operator_pairup": 4}
+
+
+def env(kind, uu, vv, op1=None, op2=None, flag=0, cgc=None):
+    out = Tensor.null(uu.ctx, uu.sym, uu.cplx)
+    check(lib().sadmrg_env_update(out.ptr, ENV_KIND[kind], uu.ptr, vv.ptr, op1.ptr if op1 else None,
+                                  op2.ptr if op2 else None, flag, cgc.ptr if cgc else None))
+    return out
+
+
+def svd(vec, tu, p1, tv, p2, max_dcut, cap=1 << 16):
+    w = np.zeros(cap)
+    n = C.c_int()
+    check(lib().sadmrg_svd(vec.ptr, tu.ptr, p1, tv.ptr, p2, max_dcut, _dptr(w), cap, C.byref(n)))
+    return w[:n.value].copy()
+
+
+def _bond_arr(b):
+    return np.asarray([b[0], len(b[1])] + list(b[1]) + list(b[2]), dtype=np.int32)
+
+
+def fuse(ctx, sym, cplx, b1, b2):
+    """b = (dir, labels, dims)"""
+    out = Tensor.null(ctx, sym, cplx)
+    a1, a2 = _bond_arr(b1), _bond_arr(b2)
+    check(lib().sadmrg_fuse(out.ptr, _iptr(a1), _iptr(a2)))
+    return out
+
+
+class Heff:
+    """Fused H_eff matvec plan (sadmrg_heff)."""
+
+    def __init__(self, ctx, cgc, hl, hr, o1, o2, psi_layout):
+        n = len(o1)
+        A = (C.c_void_p * max(n, 1))(*[t.ptr.value if isinstance(t.ptr, C.c_void_p) else t.ptr for t in o1])
+        B = (C.c_void_p * max(n, 1))(*[t.ptr.value if isinstance(t.ptr, C.c_void_p) else t.ptr for t in o2])
+        p = C.c_void_p()
+        check(lib().sadmrg_heff_create(ctx.ptr, cgc.ptr if cgc else None, hl.ptr if hl else None,
+                                       hr.ptr if hr else None, n, A, B, psi_layout.ptr, C.byref(p)))
+        self.ptr, self.ctx = p, ctx
+        self._keep = (hl, hr, list(o1), list(o2), psi_layout)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.sadmrg_heff_destroy(self.ptr)
+
+    def apply(self, psi, precision=FP64):
+        out = Tensor.null(psi.ctx, psi.sym, psi.cplx)
+        check(lib().sadmrg_heff_apply(self.ptr, psi.ptr, out.ptr, precision))
+        return out
+
+    def apply_device(self, d_in, d_out, precision=FP64):
+        check(lib().sadmrg_heff_apply_device(self.ptr, C.c_void_p(d_in), C.c_void_p(d_out), precision))
+
+    def apply_stage(self, d_in, d_out, precision=FP64, stage=0):
+        check(lib().sadmrg_heff_apply_stage(self.ptr, C.c_void_p(d_in), C.c_void_p(d_out), precision, stage))
+
+    def stage_flops(self, stage):
+        f = C.c_double()
+        check(lib().sadmrg_heff_stage_flops(self.ptr, stage, C.byref(f)))
+        return f.value
+
+    def stats(self):
+        f, b = C.c_double(), C.c_double()
+        n = C.c_longlong()
+        info = np.zeros(8, dtype=np.int32)
+        check(lib().sadmrg_heff_stats(self.ptr, C.byref(f), C.byref(b), C.byref(n), _iptr(info)))
+        return {"flops": f.value, "bytes": b.value, "n": n.value, "blocks": int(info[0]),
+                "stage1_problems": int(info[1]), "stage2_problems": int(info[2]), "tiles1": int(info[3]),
+                "tiles2": int(info[4]), "tile_cfg1": int(info[5]), "tile_cfg2": int(info[6])}
+
+    def lanczos(self, psi, mlanc=40, precision=FP64, refine=False, refine_tol=0.0):
+        e = C.c_double()
+        cnt = np.zeros(4, dtype=np.int32)
+        res = C.c_double()
+        check(lib().sadmrg_lanczos(self.ptr, psi.ptr, mlanc, precision, 1 if refine else 0, refine_tol,
+                                   C.byref(e), _iptr(cnt), C.byref(res)))
+        return e.value, {"matvec64": int(cnt[0]), "matvec32": int(cnt[1]), "steps": int(cnt[2]),
+                         "restarts": int(cnt[3]), "residual64": res.value}
+
+
+REC_FIELDS = ("il", "ir", "energy", "trunc_err", "kept", "matvec64", "matvec32", "t_env", "t_lanczos", "t_svd",
+              "heff_flops", "reserved")
+
+
+class DMRG:
+    """DMRG<Scalar,Sym> on the device engine (sadmrg_dmrg)."""
+
+    def __init__(self, ctx, sym, cplx, xx, yy, bondd, physpn=1, totspin=0):
+        p = C.c_void_p()
+        check(lib().sadmrg_dmrg_create(ctx.ptr, sym, cplx, xx, yy, bondd, physpn, totspin, C.byref(p)))
+        self.ptr, self.ctx, self.sym, self.cplx = p, ctx, sym, cplx
+        n = C.c_int()
+        check(lib().sadmrg_dmrg_ns(p, C.byref(n)))
+        self.ns = n.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.sadmrg_dmrg_destroy(self.ptr)
+
+    def set_couplings(self, hmap, coup):
+        h = np.ascontiguousarray(hmap, dtype=np.int32).reshape(-1)
+        c = np.ascontiguousarray(coup, dtype=np.float64)
+        check(lib().sadmrg_dmrg_set_couplings(self.ptr, _iptr(h), _dptr(c), len(c)))
+
+    def couplings(self):
+        h = np.zeros(self.ns * self.ns, dtype=np.int32)
+        c = np.zeros(self.ns + 1)
+        check(lib().sadmrg_dmrg_get_couplings(self.ptr, _iptr(h), _dptr(c), len(c)))
+        return h.reshape(self.ns, self.ns), c
+
+    def set_max_dcut(self, d):
+        check(lib().sadmrg_dmrg_set_max_dcut(self.ptr, d))
+
+    def set_precision(self, precision=FP64, refine=False, refine_tol=0.0):
+        check(lib().sadmrg_dmrg_set_precision(self.ptr, precision, 1 if refine else 0, refine_tol))
+
+    def idmrg(self):
+        check(lib().sadmrg_dmrg_idmrg(self.ptr))
+
+    def prepare_sweep(self):
+        check(lib().sadmrg_dmrg_prepare_sweep(self.ptr))
+
+    def sweep(self):
+        check(lib().sadmrg_dmrg_sweep(self.ptr))
+
+    def energy(self):
+        e = C.c_double()
+        check(lib().sadmrg_dmrg_energy(self.ptr, C.byref(e)))
+        return e.value
+
+    def records(self):
+        out = []
+        while True:
+            buf = np.zeros(12 * 256)
+            n = C.c_int()
+            check(lib().sadmrg_dmrg_records(self.ptr, _dptr(buf), 256, C.byref(n)))
+            for k in range(n.value):
+                out.append(dict(zip(REC_FIELDS, buf[12 * k:12 * k + 12].tolist())))
+            if n.value < 256:
+                return out
+
+    def tensor(self, which, i, j=0):
+        code = {"uu": 0, "hh": 1, "opr": 2}[which]
+        t = Tensor.null(self.ctx, self.sym, self.cplx)
+        check(lib().sadmrg_dmrg_get_tensor(self.ptr, code, i, j, t.ptr))
+        return t
+
+    def set_tensor(self, which, i, j, t):
+        code = {"uu": 0, "hh": 1, "opr": 2}[which]
+        check(lib().sadmrg_dmrg_set_tensor(self.ptr, code, i, j, t.ptr))
+
+    def call(self, name, a=0, b=0):
+        code = {"wavefunc_transformation": 0, "left2": 1, "right2": 2, "left1": 3, "right1": 4,
+                "prepare_sweep": 5}[name]
+        check(lib().sadmrg_dmrg_call(self.ptr, code, a, b))
+
+    def input_vector(self, il, ir):
+        t = Tensor.null(self.ctx, self.sym, self.cplx)
+        check(lib().sadmrg_dmrg_input_vector(self.ptr, il, ir, t.ptr))
+        return t
+
+    def heff(self, il, ir, vin, precision=FP64):
+        t = Tensor.null(self.ctx, self.sym, self.cplx)
+        check(lib().sadmrg_dmrg_heff(self.ptr, il, ir, vin.ptr, t.ptr, precision))
+        return t
+
+    def lanczos(self, il, ir, vec):
+        e = C.c_double()
+        check(lib().sadmrg_dmrg_lanczos(self.ptr, il, ir, vec.ptr, C.byref(e)))
+        return e.value
+
+    def svd(self, vec, a, b, p1, p2, cap=1 << 16):
+        w = np.zeros(cap)
+        n = C.c_int()
+        check(lib().sadmrg_dmrg_svd(self.ptr, vec.ptr, a, b, p1, p2, _dptr(w), cap, C.byref(n)))
+        return w[:n.value].copy()

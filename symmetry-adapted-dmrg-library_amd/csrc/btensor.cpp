@@ -1,0 +1,1183 @@
+// btensor.cpp -- see btensor.hpp.
+#include "btensor.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <rocsolver/rocsolver.h>
+#include "kernels.hpp"
+
+namespace sadmrg {
+
+namespace {
+
+constexpr int kMaxLeg = 8;
+
+GemmPrec prec_of(bool cplx) { return cplx ? GemmPrec::Z64 : GemmPrec::F64; }
+
+void labels_of(const BTensor& t, int b, int* lab) { t.st.labels(b, lab); }
+
+// Column-major matrix view of one block as an operand.
+Operand view(int arena, const BTensor& t, int b, int64_t ld, bool trans, bool conj = false) {
+  Operand o;
+  o.arena = arena;
+  o.off = t.off[b];
+  o.ld = ld;
+  o.trans = trans;
+  o.conj = conj;
+  return o;
+}
+
+struct Arenas {
+  ArenaTable at{};
+  int64_t n[kMaxArenas];
+  Arenas() {
+    std::memset(&at, 0, sizeof(at));
+    for (auto& x : n) x = -1;
+  }
+  // base pointer and element count (the count bounds every table access)
+  void set(int i, const void* p, int64_t elems) {
+    at.base[i] = p;
+    n[i] = p ? std::max<int64_t>(elems, 0) : 0;
+  }
+};
+
+void run_batch(GemmBatch& b, Context* ctx, bool cplx, const Arenas& a) {
+  if (b.empty()) return;
+  for (int i = 0; i < kMaxArenas; ++i)
+    if (a.n[i] >= 0) b.limit(i, a.n[i]);
+  b.finalize(ctx, cplx ? 0 : -1);
+  b.run(ctx, prec_of(cplx), a.at);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void BTensor::layout(Context* c, bool complex_, const Struct& s, const std::vector<uint8_t>& mask, uint8_t tag0) {
+  ctx = c;
+  cplx = complex_;
+  st = s;
+  const int n = s.nten();
+  off.assign(n, -1);
+  tag.assign(n, tag0);
+  nelem = 0;
+  for (int b = 0; b < n; ++b)
+    if (mask[b]) {
+      off[b] = nelem;
+      nelem += s.block_size(b);
+    }
+  buf = DevBuf(c, std::max<int64_t>(nelem, 1) * sizeof(double) * w());
+  SADMRG_HIP(launch_fill(std::max<int64_t>(nelem, 1) * w(), 0.0, dptr(), c->stream()));
+}
+
+void BTensor::clean() {
+  st = Struct{};
+  off.clear();
+  tag.clear();
+  buf.release();
+  nelem = 0;
+}
+
+Struct parse_header(SymKind sym, const int* h, int len) {
+  require(len >= 2, "tensor header too short");
+  Struct s;
+  s.sym = sym;
+  const int nb = h[0];
+  s.locspin = h[1];
+  require(nb >= 0 && nb <= kMaxLeg, "tensor header: bad nbond");
+  int p = 2;
+  for (int i = 0; i < nb; ++i) {
+    require(p + 2 <= len, "tensor header truncated");
+    const int dir = h[p], nm = h[p + 1];
+    require(nm >= 0 && p + 2 + 2 * nm <= len, "tensor header truncated");
+    std::vector<int> q(h + p + 2, h + p + 2 + nm), d(h + p + 2 + nm, h + p + 2 + 2 * nm);
+    for (int x : d) require(x >= 0, "tensor header: negative dimension");
+    s.bonds.push_back(Bond::make(dir, q, d));
+    p += 2 + 2 * nm;
+  }
+  return s;
+}
+
+std::vector<int> make_header(const Struct& s) {
+  std::vector<int> h{s.nbond(), s.locspin};
+  for (const auto& b : s.bonds) {
+    h.push_back(b.dir);
+    h.push_back(b.nmom());
+    h.insert(h.end(), b.q.begin(), b.q.end());
+    h.insert(h.end(), b.dim.begin(), b.dim.end());
+  }
+  return h;
+}
+
+void tensor_from_host(BTensor& t, Context* ctx, SymKind sym, bool cplx, const int* header, int len,
+                      const unsigned char* present, const double* data, int cgc2_kind) {
+  Struct s = parse_header(sym, header, len);
+  std::vector<uint8_t> mask(s.nten(), 0);
+  for (int b = 0; b < s.nten(); ++b) mask[b] = (present[b] && s.allowed(b)) ? 1 : 0;
+  const uint8_t tg = s.nbond() == 3 ? kCgcCanon : (cgc2_kind == 1 ? kCgcSinglet : kCgcIdent);
+  t.clean();
+  if (s.nbond() == 0) return;
+  t.layout(ctx, cplx, s, mask, tg);
+  if (t.nelem > 0) {
+    // input packs present blocks in block order: identical to the layout
+    SADMRG_HIP(hipMemcpyAsync(t.dptr(), data, t.nelem * sizeof(double) * t.w(), hipMemcpyHostToDevice,
+                              ctx->stream()));
+    ctx->sync();
+  }
+}
+
+void tensor_to_host(const BTensor& t, unsigned char* present, double* data) {
+  for (int b = 0; b < t.nten(); ++b) present[b] = t.present(b) ? 1 : 0;
+  if (t.nelem > 0 && data) {
+    SADMRG_HIP(hipMemcpyAsync(data, t.dptr(), t.nelem * sizeof(double) * t.w(), hipMemcpyDeviceToHost,
+                              t.ctx->stream()));
+    t.ctx->sync();
+  }
+}
+
+void copy_tensor(BTensor& dst, const BTensor& src) {
+  if (&dst == &src) return;
+  dst.clean();
+  if (src.is_null()) return;
+  std::vector<uint8_t> mask(src.nten());
+  for (int b = 0; b < src.nten(); ++b) mask[b] = src.present(b);
+  dst.layout(src.ctx, src.cplx, src.st, mask, kCgcIdent);
+  dst.tag = src.tag;
+  if (src.nelem > 0)
+    SADMRG_HIP(hipMemcpyAsync(dst.dptr(), src.dptr(), src.nelem * sizeof(double) * src.w(),
+                              hipMemcpyDeviceToDevice, src.ctx->stream()));
+}
+
+// ---------------------------------------------------------------------------
+// hamiltonian_vector_multiplication: out[a', b'] += fac * O1^T psi O2 per
+// (op1 block j, psi block k, op2 block l) triple; stage 1 = psi^T O1
+// (block_tensor_inst.cpp:2258), stage 2 = step1^T O2 (:2267), fac from
+// fac_hamilt_vec[J_in][chan][J_out] with |fac| < 1e-12 skipped (:2268-2272).
+void op_hvm(BTensor& out, const BTensor& vec, const BTensor& op1, const BTensor& op2, CgcTables* cgc) {
+  out.clean();
+  if (op1.is_null() || op2.is_null()) return;
+  const SymKind sym = vec.sym();
+  Context* ctx = vec.ctx;
+  require(vec.nbond() == 2 && op1.nbond() == 3 && op2.nbond() == 3, "hamiltonian_vector_multiplication: ranks");
+  const Bond ch2 = sym == SymKind::SU2 ? op2.st.bonds[1] : op2.st.bonds[1].inverted();
+  require(op1.st.bonds[0] == vec.st.bonds[0].inverted() && op2.st.bonds[0] == vec.st.bonds[1].inverted() &&
+              op1.st.bonds[1] == ch2,
+          "hamiltonian_vector_multiplication bb wrong");
+  Struct os;
+  os.sym = sym;
+  os.locspin = 0;
+  os.bonds = {op1.st.bonds[2], op2.st.bonds[2]};
+  const int nt1 = op1.nten(), nt2 = vec.nten(), nt3 = op2.nten(), nto = os.nten();
+  struct S1 { int j, k; int64_t off; int M, N, K; };
+  struct Con { int s1, l; double fac; };
+  std::vector<S1> s1;
+  std::vector<std::vector<Con>> per(nto);
+  int64_t scratch = 0;
+  int a1[3], a2[2], a3[3], d1[3], d2[2], d3[3];
+  for (int i = 0; i < nt1 * nt2; ++i) {
+    const int j = i % nt1, k = (i / nt1) % nt2;
+    if (!op1.st.allowed(j) || !vec.st.allowed(k)) continue;
+    labels_of(op1, j, a1);
+    labels_of(vec, k, a2);
+    if (a1[0] != a2[0]) continue;
+    if (!op1.present(j) || !vec.present(k)) continue;
+    op1.st.block_dims(j, d1);
+    vec.st.block_dims(k, d2);
+    int si = -1;
+    for (int l = 0; l < nt3; ++l) {
+      if (!op2.st.allowed(l)) continue;
+      labels_of(op2, l, a3);
+      if (a3[0] != a2[1] || a1[1] != a3[1]) continue;
+      const int lab[2] = {a1[2], a3[2]};
+      if (!os.allowed_labels(lab)) continue;
+      const int m = os.index_of_labels(lab);
+      if (m < 0) continue;
+      if (!op2.present(l)) continue;
+      double fac = 1.0;
+      if (sym == SymKind::SU2) {
+        fac = cgc->hamilt_vec(a1[0], a1[1], a1[2]);
+        if (std::fabs(fac) < 1e-12) continue;
+      }
+      if (si < 0) {
+        si = static_cast<int>(s1.size());
+        s1.push_back(S1{j, k, scratch, d2[1], d1[1] * d1[2], d2[0]});
+        scratch += int64_t(d2[1]) * d1[1] * d1[2];
+      }
+      per[m].push_back(Con{si, l, fac});
+    }
+  }
+  std::vector<uint8_t> mask(nto, 0);
+  for (int m = 0; m < nto; ++m) mask[m] = per[m].empty() ? 0 : 1;
+  out.layout(ctx, vec.cplx, os, mask, kCgcSinglet);
+  if (s1.empty()) return;
+  DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double) * vec.w());
+  Arenas ar;
+  ar.set(0, vec.dptr(), vec.nelem);
+  ar.set(1, op1.dptr(), op1.nelem);
+  ar.set(2, op2.dptr(), op2.nelem);
+  ar.set(3, sbuf.get(), scratch);
+  ar.set(4, out.dptr(), out.nelem);
+  GemmBatch g1, g2;
+  for (const auto& s : s1) {
+    const int p = g1.add_problem(3, s.off, s.M, s.M, s.N, false);
+    g1.add_seg(p, view(0, vec, s.k, s.K, true), view(1, op1, s.j, s.K, false), s.K, 1.0);
+  }
+  for (int m = 0; m < nto; ++m) {
+    if (per[m].empty()) continue;
+    int dm[2];
+    os.block_dims(m, dm);
+    const int p = g2.add_problem(4, out.off[m], dm[0], dm[0], dm[1], false);
+    for (const auto& c : per[m]) {
+      const S1& s = s1[c.s1];
+      op2.st.block_dims(c.l, d3);
+      const int kk = d3[0] * d3[1];
+      require(kk == s.M * (s.N / dm[0]), "hvm: inner dimension mismatch");
+      Operand A;
+      A.arena = 3;
+      A.off = s.off;
+      A.ld = kk;
+      A.trans = true;
+      g2.add_seg(p, A, view(2, op2, c.l, kk, false), kk, c.fac);
+    }
+  }
+  run_batch(g1, ctx, vec.cplx, ar);
+  run_batch(g2, ctx, vec.cplx, ar);
+  ctx->sync();
+}
+
+// ---------------------------------------------------------------------------
+// Generic contraction of bond i1 of t1 with bond i2 of t2.  Output bonds are
+// t1's remaining bonds cyclically after i1, then t2's after i2; each block is
+// one GEMM per shared sector (Dense::contract, dense_inst.cpp:369-408).  For
+// U(1) only the first matching shared sector contributes, as in the reference
+// (block_tensor_inst.cpp:1099-1117 has no U(1) accumulation branch).
+void op_contract(BTensor& out, const BTensor& t1, int i1, const BTensor& t2, int i2) {
+  Context* ctx = t1.ctx;
+  const SymKind sym = t1.sym();
+  const int n1 = t1.nbond(), n2 = t2.nbond();
+  require(i1 >= 0 && i1 < n1 && i2 >= 0 && i2 < n2, "contract: bond index out of range");
+  require(t1.st.bonds[i1] == t2.st.bonds[i2].inverted(),
+          "BlockTensor::contract, two bonds can not be contracted, check bond parameters");
+  Struct os;
+  os.sym = sym;
+  os.locspin = t1.st.locspin + t2.st.locspin;
+  for (int i = 0; i < n1 - 1; ++i) os.bonds.push_back(t1.st.bonds[(i1 + 1 + i) % n1]);
+  for (int i = 0; i < n2 - 1; ++i) os.bonds.push_back(t2.st.bonds[(i2 + 1 + i) % n2]);
+  const int nto = os.nten();
+  const Bond& sh = t1.st.bonds[i1];
+  std::vector<std::vector<std::pair<int, int>>> per(nto);
+  int lab[2 * kMaxLeg], l1[kMaxLeg], l2[kMaxLeg];
+  for (int i = 0; i < nto; ++i) {
+    if (!os.allowed(i)) continue;
+    os.labels(i, lab);
+    for (int k = 0; k < n1 - 1; ++k) l1[(i1 + 1 + k) % n1] = lab[k];
+    for (int k = 0; k < n2 - 1; ++k) l2[(i2 + 1 + k) % n2] = lab[k + n1 - 1];
+    for (int j = 0; j < sh.nmom(); ++j) {
+      l1[i1] = sh.q[j];
+      l2[i2] = sh.q[j];
+      if (!t1.st.allowed_labels(l1) || !t2.st.allowed_labels(l2)) continue;
+      const int kb = t1.st.index_of_labels(l1), lb = t2.st.index_of_labels(l2);
+      if (kb < 0 || lb < 0) continue;
+      if (!t1.present(kb) || !t2.present(lb)) continue;
+      if (sym == SymKind::U1 && !per[i].empty()) continue;
+      per[i].push_back({kb, lb});
+    }
+  }
+  std::vector<uint8_t> mask(nto, 0);
+  for (int i = 0; i < nto; ++i) mask[i] = per[i].empty() ? 0 : 1;
+  uint8_t tg = kCgcOther;
+  if (os.nbond() == 3) tg = kCgcCanon;
+  else if (n1 == 2 && n2 == 2) {
+    const uint8_t ta = t1.tag.empty() ? kCgcIdent : t1.tag[0], tb = t2.tag.empty() ? kCgcIdent : t2.tag[0];
+    tg = ta == kCgcIdent ? tb : (tb == kCgcIdent ? ta : kCgcOther);
+  }
+  BTensor res;
+  res.layout(ctx, t1.cplx, os, mask, tg);
+
+  // Interior contracted legs need a transposed copy (Dense::shift).
+  auto prod = [](const int* d, int a, int b) {
+    int64_t p = 1;
+    for (int i = a; i < b; ++i) p *= d[i];
+    return p;
+  };
+  int64_t scratch = 0;
+  struct Item { int out, kb, lb; int64_t aoff, boff; };
+  std::vector<Item> items;
+  for (int i = 0; i < nto; ++i)
+    for (auto& pr : per[i]) {
+      Item it{i, pr.first, pr.second, -1, -1};
+      int d[kMaxLeg];
+      if (i1 != 0 && i1 != n1 - 1) { it.aoff = scratch; scratch += t1.block_size(pr.first); }
+      if (i2 != 0 && i2 != n2 - 1) { it.boff = scratch; scratch += t2.block_size(pr.second); }
+      (void)d;
+      items.push_back(it);
+    }
+  DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double) * t1.w());
+  for (const auto& it : items) {
+    int d[kMaxLeg];
+    if (it.aoff >= 0) {
+      t1.st.block_dims(it.kb, d);
+      const int64_t B = prod(d, 0, i1 + 1), At = prod(d, i1 + 1, n1);
+      SADMRG_HIP(launch_copy2d(static_cast<int>(At), static_cast<int>(B), t1.block_ptr(it.kb), B, 1,
+                               sbuf.as<double>() + it.aoff * t1.w(), At, t1.w(), 1.0, nullptr, 0, ctx->stream()));
+    }
+    if (it.boff >= 0) {
+      t2.st.block_dims(it.lb, d);
+      const int64_t B = prod(d, 0, i2), At = prod(d, i2, n2);
+      SADMRG_HIP(launch_copy2d(static_cast<int>(At), static_cast<int>(B), t2.block_ptr(it.lb), B, 1,
+                               sbuf.as<double>() + it.boff * t1.w(), At, t1.w(), 1.0, nullptr, 0, ctx->stream()));
+    }
+  }
+  Arenas ar;
+  ar.set(0, t1.dptr(), t1.nelem);
+  ar.set(1, t2.dptr(), t2.nelem);
+  ar.set(2, sbuf.get(), scratch);
+  ar.set(3, res.dptr(), res.nelem);
+  GemmBatch g;
+  for (int i = 0; i < nto; ++i) {
+    if (per[i].empty()) continue;
+    int dout[2 * kMaxLeg];
+    os.block_dims(i, dout);
+    int64_t M = 1, N = 1;
+    for (int k = 0; k < n1 - 1; ++k) M *= dout[k];
+    for (int k = 0; k < n2 - 1; ++k) N *= dout[n1 - 1 + k];
+    const int p = g.add_problem(3, res.off[i], M, static_cast<int>(M), static_cast<int>(N), false);
+    for (const auto& it : items) {
+      if (it.out != i) continue;
+      int da[kMaxLeg], db[kMaxLeg];
+      t1.st.block_dims(it.kb, da);
+      t2.st.block_dims(it.lb, db);
+      const int K = da[i1];
+      Operand A, B;
+      if (i1 == 0) { A = view(0, t1, it.kb, K, true); }
+      else if (i1 == n1 - 1) { A = view(0, t1, it.kb, M, false); }
+      else { A.arena = 2; A.off = it.aoff; A.ld = M; A.trans = false; }
+      if (i2 == 0) { B = view(1, t2, it.lb, K, false); }
+      else if (i2 == n2 - 1) { B = view(1, t2, it.lb, N, true); }
+      else { B.arena = 2; B.off = it.boff; B.ld = K; B.trans = false; }
+      g.add_seg(p, A, B, K, 1.0);
+    }
+  }
+  run_batch(g, ctx, t1.cplx, ar);
+  ctx->sync();
+  out = std::move(res);
+}
+
+// ---------------------------------------------------------------------------
+// Environment updates.  The reduced physical leg (spin-1/2: dimension 1 per
+// sector) lets every block chain reduce to plain GEMMs; a larger physical or
+// channel dimension is rejected as unsupported.
+namespace {
+
+int dims3(const BTensor& t, int b, int k) {
+  int d[kMaxLeg];
+  t.st.block_dims(b, d);
+  return d[k];
+}
+
+}  // namespace
+
+void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const BTensor* op1, const BTensor* op2,
+            int flag, CgcTables* cgc) {
+  Context* ctx = uu.ctx;
+  const SymKind sym = uu.sym();
+  const bool su2 = sym == SymKind::SU2;
+  const bool cx = uu.cplx;
+  require(flag == 0 || flag == 1, "environment update: flag must be 0 or 1");
+  require(uu.nbond() == 3 && vv.nbond() == 3, "environment update: uu/vv must be 3-leg");
+  Struct os;
+  os.sym = sym;
+  os.locspin = 0;
+  int a1[3], a2[3], a3[3], a4[3];
+  Arenas ar;
+  ar.set(0, uu.dptr(), uu.nelem);
+  ar.set(1, vv.dptr(), vv.nelem);
+  ar.set(2, op1 ? op1->dptr() : nullptr, op1 ? op1->nelem : 0);
+  ar.set(3, op2 ? op2->dptr() : nullptr, op2 ? op2->nelem : 0);
+  GemmBatch g1, g2;
+  int64_t scratch = 0;
+  std::vector<std::vector<std::pair<int, int>>> dummy;
+
+  if (kind == kOverlapInitial) {
+    require(uu.st.bonds[flag] == vv.st.bonds[flag] && uu.st.bonds[1 - flag] == vv.st.bonds[1 - flag],
+            "overlap initial wrong bb");
+    os.bonds = {uu.st.bonds[2], vv.st.bonds[2].inverted()};
+    const int nto = os.nten();
+    struct C { int j, k; };
+    std::vector<std::vector<C>> per(nto);
+    for (int j = 0; j < uu.nten(); ++j) {
+      if (!uu.st.allowed(j) || !uu.present(j)) continue;
+      labels_of(uu, j, a1);
+      for (int k = 0; k < vv.nten(); ++k) {
+        if (!vv.st.allowed(k) || !vv.present(k)) continue;
+        labels_of(vv, k, a2);
+        if (a1[0] != a2[0] || a1[1] != a2[1]) continue;
+        const int lab[2] = {a1[2], a2[2]};
+        const int m = os.index_of_labels(lab);
+        // blocks outside the symmetry-allowed set are never read back
+        if (m < 0 || !os.allowed(m)) continue;
+        per[m].push_back(C{j, k});
+      }
+    }
+    std::vector<uint8_t> mask(nto);
+    for (int m = 0; m < nto; ++m) mask[m] = !per[m].empty();
+    out.clean();
+    out.layout(ctx, cx, os, mask, kCgcIdent);
+    ar.set(5, out.dptr(), out.nelem);
+    for (int m = 0; m < nto; ++m) {
+      if (per[m].empty()) continue;
+      int dm[2];
+      os.block_dims(m, dm);
+      const int p = g2.add_problem(5, out.off[m], dm[0], dm[0], dm[1], false);
+      for (auto& c : per[m]) {
+        const int K = dims3(uu, c.j, 0) * dims3(uu, c.j, 1);
+        g2.add_seg(p, view(0, uu, c.j, K, true), view(1, vv, c.k, K, false, true), K, 1.0);
+      }
+    }
+    run_batch(g2, ctx, cx, ar);
+    ctx->sync();
+    return;
+  }
+
+  require(op1 != nullptr, "environment update: missing operator");
+  const BTensor& op = *op1;
+  // Stage-1 product of one (op block j, uu block k) pair, reused by every vv
+  // block that matches it.
+  struct S1 { int j, k; int64_t off; int M, N, K; };
+  std::vector<S1> s1;
+  struct Con { int s1, l; double scale; int K; };
+  std::vector<std::vector<Con>> per;
+
+  auto stage1_uu_op = [&](int j, int k) -> S1 {
+    // step1(c, y, z) = sum_a uu(a@flag, c) op(a, y, z): uu is (l x r) once
+    // the unit physical leg is dropped.
+    const int l = dims3(uu, k, flag), r = dims3(uu, k, 2);
+    require(dims3(uu, k, 1 - flag) == 1, "environment update: physical leg dimension > 1 is unsupported");
+    int d[3];
+    op.st.block_dims(j, d);
+    require(d[0] == l, "environment update: operator / site tensor dimension mismatch");
+    S1 s{j, k, scratch, r, d[1] * d[2], l};
+    scratch += int64_t(r) * d[1] * d[2];
+    return s;
+  };
+
+  if (kind == kOverlapTransformation) {
+    require(uu.st.bonds[flag] == op.st.bonds[0].inverted() && vv.st.bonds[flag] == op.st.bonds[1],
+            "overlap transformation wrong bb");
+    os.bonds = {uu.st.bonds[2], vv.st.bonds[2].inverted()};
+    per.resize(os.nten());
+    const int nt1 = op.nten(), nt2 = uu.nten();
+    for (int i = 0; i < nt1 * nt2; ++i) {
+      const int j = i % nt1, k = (i / nt1) % nt2;
+      if (!op.st.allowed(j) || !uu.st.allowed(k)) continue;
+      labels_of(op, j, a1);
+      labels_of(uu, k, a2);
+      if (a1[0] != a2[flag]) continue;
+      if (!op.present(j) || !uu.present(k)) continue;
+      int si = -1;
+      for (int l = 0; l < vv.nten(); ++l) {
+        if (!vv.st.allowed(l) || !vv.present(l)) continue;
+        labels_of(vv, l, a3);
+        if (a1[1] != a3[flag] || a2[1 - flag] != a3[1 - flag]) continue;
+        const int lab[2] = {a2[2], a3[2]};
+        if (!os.allowed_labels(lab)) continue;
+        const int m = os.index_of_labels(lab);
+        if (si < 0) {
+          // step1(y, c) = op^T (y x l) uu (l x r)
+          const int l0 = dims3(uu, k, flag), r = dims3(uu, k, 2);
+          require(dims3(uu, k, 1 - flag) == 1, "overlap transformation: physical leg dimension > 1 is unsupported");
+          const int y = dims3(op, j, 1);
+          si = static_cast<int>(s1.size());
+          s1.push_back(S1{j, k, scratch, y, r, l0});
+          scratch += int64_t(y) * r;
+        }
+        per[m].push_back(Con{si, l, 1.0, s1[si].M});
+      }
+    }
+    std::vector<uint8_t> mask(os.nten());
+    for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
+    out.clean();
+    out.layout(ctx, cx, os, mask, kCgcIdent);
+    DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double) * uu.w());
+    ar.set(4, sbuf.get(), scratch);
+    ar.set(5, out.dptr(), out.nelem);
+    for (const auto& s : s1) {
+      const int p = g1.add_problem(4, s.off, s.M, s.M, s.N, false);
+      g1.add_seg(p, view(2, op, s.j, s.K, true), view(0, uu, s.k, s.K, false), s.K, 1.0);
+    }
+    for (int m = 0; m < os.nten(); ++m) {
+      if (per[m].empty()) continue;
+      int dm[2];
+      os.block_dims(m, dm);
+      const int p = g2.add_problem(5, out.off[m], dm[0], dm[0], dm[1], false);
+      for (const auto& c : per[m]) {
+        const S1& s = s1[c.s1];
+        require(dims3(vv, c.l, 1 - flag) == 1, "overlap transformation: physical leg dimension > 1");
+        Operand A;
+        A.arena = 4;
+        A.off = s.off;
+        A.ld = s.M;
+        A.trans = true;
+        g2.add_seg(p, A, view(1, vv, c.l, s.M, false, true), s.M, 1.0);
+      }
+    }
+    run_batch(g1, ctx, cx, ar);
+    run_batch(g2, ctx, cx, ar);
+    ctx->sync();
+    return;
+  }
+
+  if (kind == kOperatorInitial) {
+    require(uu.st.bonds[1 - flag] == op.st.bonds[0].inverted() && vv.st.bonds[1 - flag] == op.st.bonds[2],
+            "operator initial wrong bb");
+    os.bonds = {uu.st.bonds[2], op.st.bonds[1], vv.st.bonds[2].inverted()};
+    per.resize(os.nten());
+    // operator values are needed on the host as scalar weights
+    std::vector<double> hop(op.nelem * op.w());
+    if (op.nelem) {
+      SADMRG_HIP(hipMemcpyAsync(hop.data(), op.dptr(), hop.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream()));
+      ctx->sync();
+    }
+    const int nt1 = op.nten(), nt2 = uu.nten();
+    const int P2 = cgc ? cgc->p2() : 3;
+    for (int i = 0; i < nt1 * nt2; ++i) {
+      const int j = i % nt1, k = (i / nt1) % nt2;
+      if (!op.st.allowed(j) || !uu.st.allowed(k)) continue;
+      labels_of(op, j, a1);
+      labels_of(uu, k, a2);
+      if (a1[0] != a2[1 - flag]) continue;
+      if (!op.present(j) || !uu.present(k)) continue;
+      require(op.block_size(j) == 1, "operator initial: multi-element site operators are unsupported");
+      for (int l = 0; l < vv.nten(); ++l) {
+        if (!vv.st.allowed(l) || !vv.present(l)) continue;
+        labels_of(vv, l, a3);
+        if (a1[2] != a3[1 - flag] || a2[flag] != a3[flag]) continue;
+        const int lab[3] = {a2[2], a1[1], a3[2]};
+        if (!os.allowed_labels(lab)) continue;
+        const int m = os.index_of_labels(lab);
+        double fac = 1.0;
+        if (su2) {
+          fac = cgc->onsite(flag, a2[2], a1[1], a3[2], a2[flag], a1[0] + a1[2] * P2);
+          if (std::fabs(fac) < 1e-8) continue;
+        }
+        require(!cx || hop[op.off[j] * 2 + 1] == 0.0, "operator initial: complex site operator unsupported");
+        const double sv = hop[op.off[j] * op.w()];
+        const int K = dims3(uu, k, 0) * dims3(uu, k, 1);
+        per[m].push_back(Con{k, l, fac * sv, K});
+      }
+    }
+    std::vector<uint8_t> mask(os.nten());
+    for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
+    out.clean();
+    out.layout(ctx, cx, os, mask, kCgcCanon);
+    ar.set(5, out.dptr(), out.nelem);
+    for (int m = 0; m < os.nten(); ++m) {
+      if (per[m].empty()) continue;
+      int dm[3];
+      os.block_dims(m, dm);
+      const int p = g2.add_problem(5, out.off[m], int64_t(dm[0]) * dm[1], dm[0] * dm[1], dm[2], false);
+      for (const auto& c : per[m])
+        g2.add_seg(p, view(0, uu, c.s1, c.K, true), view(1, vv, c.l, c.K, false, true), c.K, c.scale);
+    }
+    run_batch(g2, ctx, cx, ar);
+    ctx->sync();
+    return;
+  }
+
+  if (kind == kOperatorTransformation) {
+    require(uu.st.bonds[flag] == op.st.bonds[0].inverted() && vv.st.bonds[flag] == op.st.bonds[2],
+            "operator transformation wrong bb");
+    os.bonds = {uu.st.bonds[2], op.st.bonds[1], vv.st.bonds[2].inverted()};
+    per.resize(os.nten());
+    const int nt1 = op.nten(), nt2 = uu.nten();
+    for (int i = 0; i < nt1 * nt2; ++i) {
+      const int j = i % nt1, k = (i / nt1) % nt2;
+      if (!op.st.allowed(j) || !uu.st.allowed(k)) continue;
+      labels_of(op, j, a1);
+      labels_of(uu, k, a2);
+      if (a1[0] != a2[flag]) continue;
+      if (!op.present(j) || !uu.present(k)) continue;
+      int si = -1;
+      for (int l = 0; l < vv.nten(); ++l) {
+        if (!vv.st.allowed(l) || !vv.present(l)) continue;
+        labels_of(vv, l, a3);
+        if (a1[2] != a3[flag] || a2[1 - flag] != a3[1 - flag]) continue;
+        const int lab[3] = {a2[2], a1[1], a3[2]};
+        if (!os.allowed_labels(lab)) continue;
+        const int m = os.index_of_labels(lab);
+        require(dims3(vv, l, 1 - flag) == 1, "operator transformation: physical leg dimension > 1 is unsupported");
+        double fac = 1.0;
+        if (su2) {
+          fac = cgc->transformation(flag, a2[2], a1[1], a3[2], a2[flag], a3[flag], a2[1 - flag]);
+          if (std::fabs(fac) < 1e-8) continue;
+        }
+        if (si < 0) {
+          si = static_cast<int>(s1.size());
+          s1.push_back(stage1_uu_op(j, k));
+        }
+        per[m].push_back(Con{si, l, fac, dims3(op, j, 2)});
+      }
+    }
+    std::vector<uint8_t> mask(os.nten());
+    for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
+    out.clean();
+    out.layout(ctx, cx, os, mask, kCgcCanon);
+    DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double) * uu.w());
+    ar.set(4, sbuf.get(), scratch);
+    ar.set(5, out.dptr(), out.nelem);
+    for (const auto& s : s1) {
+      const int p = g1.add_problem(4, s.off, s.M, s.M, s.N, false);
+      g1.add_seg(p, view(0, uu, s.k, s.K, true), view(2, op, s.j, s.K, false), s.K, 1.0);
+    }
+    for (int m = 0; m < os.nten(); ++m) {
+      if (per[m].empty()) continue;
+      int dm[3];
+      os.block_dims(m, dm);
+      const int M = dm[0] * dm[1];
+      const int p = g2.add_problem(5, out.off[m], M, M, dm[2], false);
+      for (const auto& c : per[m]) {
+        const S1& s = s1[c.s1];
+        // step1 as (r*chan) x z, times conj(vv) (z x r')
+        Operand A;
+        A.arena = 4;
+        A.off = s.off;
+        A.ld = M;
+        A.trans = false;
+        g2.add_seg(p, A, view(1, vv, c.l, c.K, false, true), c.K, c.scale);
+      }
+    }
+    run_batch(g1, ctx, cx, ar);
+    run_batch(g2, ctx, cx, ar);
+    ctx->sync();
+    return;
+  }
+
+  if (kind == kOperatorPairup) {
+    require(op2 != nullptr, "operator pairup: missing site operator");
+    const BTensor& sg = *op2;
+    const Bond chk = su2 ? sg.st.bonds[1] : sg.st.bonds[1].inverted();
+    require(uu.st.bonds[flag] == op.st.bonds[0].inverted() && vv.st.bonds[flag] == op.st.bonds[2] &&
+                op.st.bonds[1] == chk,
+            "operator pairup wrong bb");
+    os.bonds = {uu.st.bonds[2], vv.st.bonds[2].inverted()};
+    per.resize(os.nten());
+    std::vector<double> hsg(sg.nelem * sg.w());
+    if (sg.nelem) {
+      SADMRG_HIP(hipMemcpyAsync(hsg.data(), sg.dptr(), hsg.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream()));
+      ctx->sync();
+    }
+    const int P2 = cgc ? cgc->p2() : 3;
+    const int nt1 = op.nten(), nt2 = uu.nten();
+    for (int i = 0; i < nt1 * nt2; ++i) {
+      const int j = i % nt1, k = (i / nt1) % nt2;
+      if (!op.st.allowed(j) || !uu.st.allowed(k)) continue;
+      labels_of(op, j, a1);
+      labels_of(uu, k, a2);
+      if (a1[0] != a2[flag]) continue;
+      if (!op.present(j) || !uu.present(k)) continue;
+      int si = -1;
+      for (int l = 0; l < vv.nten(); ++l) {
+        if (!vv.st.allowed(l) || !vv.present(l)) continue;
+        labels_of(vv, l, a3);
+        if (a1[2] != a3[flag]) continue;
+        for (int n = 0; n < sg.nten(); ++n) {
+          if (!sg.st.allowed(n)) continue;
+          labels_of(sg, n, a4);
+          if (a4[0] != a2[1 - flag] || a4[2] != a3[1 - flag] || a4[1] != a1[1]) continue;
+          if (!sg.present(n)) continue;
+          const int lab[2] = {a2[2], a3[2]};
+          if (!os.allowed_labels(lab)) continue;
+          const int m = os.index_of_labels(lab);
+          require(sg.block_size(n) == 1 && dims3(op, j, 1) == 1 && dims3(vv, l, 1 - flag) == 1,
+                  "operator pairup: physical/channel leg dimension > 1 is unsupported");
+          double fac = 1.0;
+          if (su2) {
+            fac = cgc->pairup(flag, a2[2], a2[1 - flag == 1 ? 0 : 1], a3[1 - flag == 1 ? 0 : 1], a1[1],
+                              a4[0] + a4[2] * P2);
+            if (std::fabs(fac) < 1e-8) continue;
+          }
+          require(!cx || hsg[sg.off[n] * 2 + 1] == 0.0, "operator pairup: complex site operator unsupported");
+          const double sv = hsg[sg.off[n] * sg.w()];
+          if (si < 0) {
+            si = static_cast<int>(s1.size());
+            s1.push_back(stage1_uu_op(j, k));
+          }
+          per[m].push_back(Con{si, l, fac * sv, dims3(op, j, 2)});
+        }
+      }
+    }
+    std::vector<uint8_t> mask(os.nten());
+    for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
+    out.clean();
+    out.layout(ctx, cx, os, mask, kCgcIdent);
+    DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double) * uu.w());
+    ar.set(4, sbuf.get(), scratch);
+    ar.set(5, out.dptr(), out.nelem);
+    for (const auto& s : s1) {
+      const int p = g1.add_problem(4, s.off, s.M, s.M, s.N, false);
+      g1.add_seg(p, view(0, uu, s.k, s.K, true), view(2, op, s.j, s.K, false), s.K, 1.0);
+    }
+    for (int m = 0; m < os.nten(); ++m) {
+      if (per[m].empty()) continue;
+      int dm[2];
+      os.block_dims(m, dm);
+      const int p = g2.add_problem(5, out.off[m], dm[0], dm[0], dm[1], false);
+      for (const auto& c : per[m]) {
+        const S1& s = s1[c.s1];
+        Operand A;
+        A.arena = 4;
+        A.off = s.off;
+        A.ld = dm[0];
+        A.trans = false;
+        g2.add_seg(p, A, view(1, vv, c.l, c.K, false, true), c.K, c.scale);
+      }
+    }
+    run_batch(g1, ctx, cx, ar);
+    run_batch(g2, ctx, cx, ar);
+    ctx->sync();
+    return;
+  }
+  fail(kErrInvalid, "environment update: unknown kind");
+}
+
+// ---------------------------------------------------------------------------
+void op_fuse(BTensor& out, Context* ctx, bool cplx, SymKind sym, const Bond& b1, const Bond& b2) {
+  Struct s;
+  s.sym = sym;
+  s.locspin = 0;
+  s.bonds = {b1, b2, fuse_bonds(sym, b1, b2)};
+  std::vector<uint8_t> mask(s.nten());
+  for (int b = 0; b < s.nten(); ++b) mask[b] = s.allowed(b);
+  out.clean();
+  out.layout(ctx, cplx, s, mask, kCgcCanon);
+  std::vector<int64_t> pos(s.bonds[2].nmom(), 0);
+  int idx[3], d[3];
+  for (int b = 0; b < s.nten(); ++b) {
+    if (!mask[b]) continue;
+    s.decode(b, idx);
+    s.block_dims(b, d);
+    const int64_t n = int64_t(d[0]) * d[1];
+    require(pos[idx[2]] + n <= d[2], "BlockTensor::fuse(bondtype, bondtype) wrong");
+    const int w = out.w();
+    SADMRG_HIP(launch_strided_fill(n, 1.0, out.block_ptr(b) + pos[idx[2]] * n * w, (1 + n) * w, ctx->stream()));
+    pos[idx[2]] += n;
+  }
+}
+
+void op_fuse_to_singlet(BTensor& out, Context* ctx, bool cplx, SymKind sym, const Bond& b1, const Bond& b2) {
+  Struct s;
+  s.sym = sym;
+  s.locspin = 0;
+  s.bonds = {b1, b2};
+  std::vector<uint8_t> mask(s.nten(), 0);
+  int lab[2];
+  for (int b = 0; b < s.nten(); ++b) {
+    s.labels(b, lab);
+    mask[b] = sym == SymKind::SU2 ? (lab[0] == lab[1]) : (lab[0] * b1.dir + lab[1] * b2.dir == 0);
+  }
+  out.clean();
+  out.layout(ctx, cplx, s, mask, kCgcSinglet);
+  for (int b = 0; b < s.nten(); ++b)
+    if (mask[b]) SADMRG_HIP(launch_strided_fill(s.block_size(b), 1.0, out.block_ptr(b), out.w(), ctx->stream()));
+}
+
+// Cyclic bond shift (Dense::shift is a 2-D transpose of a B x A split).
+static void shift_impl(BTensor& t, int i0, int i1, bool vectran_dirs) {
+  if (t.is_null()) return;
+  const int n = t.nbond();
+  const int ishift = (i1 > i0) ? (i1 - i0) : (n - (i0 - i1));
+  Struct ns = t.st;
+  for (int i = 0; i < n; ++i) ns.bonds[(i + ishift) % n] = t.st.bonds[i];
+  if (vectran_dirs && t.sym() == SymKind::SU2) {
+    ns.bonds[0].dir = 1;
+    ns.bonds[1].dir = 1;
+    ns.bonds[2].dir = -1;
+  }
+  std::vector<uint8_t> mask(ns.nten(), 0);
+  std::vector<int> map(t.nten(), -1);
+  int lab[kMaxLeg], lab1[kMaxLeg];
+  for (int b = 0; b < t.nten(); ++b) {
+    if (!t.st.allowed(b) || !t.present(b)) continue;
+    t.st.labels(b, lab);
+    for (int k = 0; k < n; ++k) lab1[(k + ishift) % n] = lab[k];
+    const int j = ns.index_of_labels(lab1);
+    if (j < 0) continue;
+    map[b] = j;
+    mask[j] = 1;
+  }
+  BTensor r;
+  r.layout(t.ctx, t.cplx, ns, mask, n == 3 ? kCgcCanon : (t.tag.empty() ? kCgcIdent : t.tag[0]));
+  const int p = n - ishift;
+  for (int b = 0; b < t.nten(); ++b) {
+    if (map[b] < 0) continue;
+    int d[kMaxLeg];
+    t.st.block_dims(b, d);
+    int64_t B = 1, A = 1;
+    for (int i = 0; i < p; ++i) B *= d[i];
+    for (int i = p; i < n; ++i) A *= d[i];
+    // new[a + A*b] = old[b + B*a]
+    SADMRG_HIP(launch_copy2d(static_cast<int>(A), static_cast<int>(B), t.block_ptr(b), B, 1, r.block_ptr(map[b]), A,
+                             t.w(), 1.0, nullptr, 0, t.ctx->stream()));
+  }
+  t = std::move(r);
+}
+
+void op_vectran(BTensor& t, bool left_to_right) {
+  require(t.nbond() == 3, "vectran: 3-leg tensor expected");
+  if (left_to_right) shift_impl(t, 1, 0, true);
+  else shift_impl(t, 2, 0, true);
+}
+
+void op_shift(BTensor& t, int i0, int i1) {
+  if (i0 == i1) return;
+  shift_impl(t, i0, i1, false);
+}
+
+void op_take_conjugate(BTensor& t, int bond) {
+  require(bond >= 0 && bond < t.nbond(), "take_conjugate: bad bond");
+  t.st.bonds[bond].dir = -t.st.bonds[bond].dir;
+  if (t.sym() == SymKind::SU2 && t.nbond() == 2)
+    for (auto& g : t.tag) g = (g == kCgcIdent) ? kCgcSinglet : kCgcOther;
+}
+
+void op_makeup_input_vector(BTensor& t) {
+  require(t.nbond() == 2, "BlockTensor::makeup_input_vector called by wrong tensor");
+  const int n0 = t.st.bonds[0].nmom(), n1 = t.st.bonds[1].nmom();
+  std::vector<uint8_t> mask(t.nten(), 0);
+  bool change = false;
+  for (int b = 0; b < t.nten(); ++b) mask[b] = t.present(b);
+  for (int i = 0; i < n0; ++i)
+    for (int j = 0; j < n1; ++j)
+      if (t.st.bonds[0].q[i] == t.st.bonds[1].q[j] && !mask[i + j * n0]) {
+        mask[i + j * n0] = 1;
+        change = true;
+      }
+  if (!change) return;
+  BTensor r;
+  r.layout(t.ctx, t.cplx, t.st, mask, kCgcSinglet);
+  for (int b = 0; b < t.nten(); ++b)
+    if (t.present(b) && t.block_size(b) > 0)
+      SADMRG_HIP(hipMemcpyAsync(r.block_ptr(b), t.block_ptr(b), t.block_size(b) * sizeof(double) * t.w(),
+                                hipMemcpyDeviceToDevice, t.ctx->stream()));
+  t = std::move(r);
+}
+
+void op_make_spinor_start(BTensor& t, Context* ctx, bool cplx, SymKind sym, int physpn) {
+  Struct s;
+  s.sym = sym;
+  s.locspin = 0;
+  std::vector<uint8_t> mask;
+  std::vector<double> vals;
+  if (sym == SymKind::SU2) {
+    s.bonds = {Bond::make(-1, {physpn}, {1}), Bond::make(-1, {2}, {1}), Bond::make(1, {physpn}, {1})};
+    mask.assign(s.nten(), 0);
+    mask[0] = 1;
+    vals = {1.0};
+  } else {
+    std::vector<int> q, one;
+    for (int i = 0; i <= physpn; ++i) { q.push_back(-physpn + 2 * i); one.push_back(1); }
+    s.bonds = {Bond::make(-1, q, one), Bond::make(-1, {-2, 0, 2}, {1, 1, 1}), Bond::make(1, q, one)};
+    mask.assign(s.nten(), 0);
+    int lab[3];
+    for (int b = 0; b < s.nten(); ++b) {
+      if (!s.allowed(b)) continue;
+      s.labels(b, lab);
+      if (lab[0] == lab[2] && lab[1] == 0) { mask[b] = 1; vals.push_back(lab[0] / 2.0); }
+      else if (lab[0] != lab[2]) { mask[b] = 1; vals.push_back(1.0 / std::sqrt(2.0)); }
+    }
+  }
+  t.clean();
+  t.layout(ctx, cplx, s, mask, kCgcCanon);
+  std::vector<double> h(t.nelem * t.w(), 0.0);
+  for (size_t i = 0; i < vals.size(); ++i) h[i * t.w()] = vals[i];
+  if (!h.empty()) {
+    SADMRG_HIP(hipMemcpyAsync(t.dptr(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream()));
+    ctx->sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+void op_scale(BTensor& t, double re, double im) {
+  if (t.is_null() || t.nelem == 0) return;
+  if (!t.cplx) {
+    SADMRG_HIP(launch_axpby(t.nelem, re, 0.0, nullptr, t.dptr(), t.ctx->stream()));
+  } else {
+    SADMRG_HIP(launch_zscal(t.nelem, re, im, t.dptr(), t.dptr(), t.ctx->stream()));
+  }
+}
+
+void op_fill(BTensor& t, double v) {
+  if (t.is_null() || t.nelem == 0) return;
+  if (!t.cplx) SADMRG_HIP(launch_fill(t.nelem, v, t.dptr(), t.ctx->stream()));
+  else {
+    SADMRG_HIP(launch_fill(t.nelem * 2, 0.0, t.dptr(), t.ctx->stream()));
+    SADMRG_HIP(launch_strided_fill(t.nelem, v, t.dptr(), 2, t.ctx->stream()));
+  }
+}
+
+// y += (re, im) * x over the blocks of y's structure; a null block of y that
+// is present in x becomes a scaled copy (block_tensor_inst.cpp:225-278).
+void op_axpy(BTensor& y, double re, double im, const BTensor& x) {
+  if (x.is_null()) return;
+  if (y.is_null()) {
+    copy_tensor(y, x);
+    op_scale(y, re, im);
+    return;
+  }
+  require(y.st == x.st, "BlockTensor::operator+=: structures differ");
+  std::vector<uint8_t> mask(y.nten());
+  bool grow = false;
+  for (int b = 0; b < y.nten(); ++b) {
+    mask[b] = y.present(b) || (x.present(b) && y.st.allowed(b));
+    if (mask[b] && !y.present(b)) grow = true;
+  }
+  if (grow) {
+    BTensor r;
+    r.layout(y.ctx, y.cplx, y.st, mask, y.tag.empty() ? kCgcIdent : y.tag[0]);
+    for (int b = 0; b < y.nten(); ++b)
+      if (y.present(b) && y.block_size(b) > 0)
+        SADMRG_HIP(hipMemcpyAsync(r.block_ptr(b), y.block_ptr(b), y.block_size(b) * sizeof(double) * y.w(),
+                                  hipMemcpyDeviceToDevice, y.ctx->stream()));
+    y = std::move(r);
+  }
+  Context* ctx = y.ctx;
+  // identical layouts: one fused update
+  if (y.off == x.off) {
+    if (!y.cplx) SADMRG_HIP(launch_axpby(y.nelem, 1.0, re, x.dptr(), y.dptr(), ctx->stream()));
+    else {
+      double* c = ctx->dscal();
+      const double h[2] = {re, im};
+      SADMRG_HIP(hipMemcpyAsync(c, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream()));
+      SADMRG_HIP(launch_gemv_n_update(x.dptr(), 0, 1, c, y.dptr(), y.nelem, 1, ctx->stream()));
+      ctx->sync();
+    }
+    return;
+  }
+  for (int b = 0; b < y.nten(); ++b) {
+    if (!y.st.allowed(b) || !x.present(b) || !y.present(b)) continue;
+    const int64_t n = y.block_size(b);
+    if (!y.cplx) {
+      SADMRG_HIP(launch_axpby(n, 1.0, re, x.block_ptr(b), y.block_ptr(b), ctx->stream()));
+    } else {
+      double* c = ctx->dscal();
+      const double h[2] = {re, im};
+      SADMRG_HIP(hipMemcpyAsync(c, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream()));
+      SADMRG_HIP(launch_gemv_n_update(x.block_ptr(b), 0, 1, c, y.block_ptr(b), n, 1, ctx->stream()));
+      ctx->sync();
+    }
+  }
+}
+
+dcmplex op_dot(const BTensor& x, const BTensor& y) {
+  if (x.is_null() || y.is_null()) return dcmplex(0, 0);
+  Context* ctx = x.ctx;
+  std::vector<int> blocks;
+  for (int b = 0; b < x.nten() && b < y.nten(); ++b)
+    if (x.st.allowed(b) && x.present(b) && y.present(b)) blocks.push_back(b);
+  const int nb = static_cast<int>(blocks.size());
+  require(nb <= Context::kMaxDscal / 2, "inner_prod_c: too many blocks");
+  for (int i = 0; i < nb; ++i) {
+    const int b = blocks[i];
+    SADMRG_HIP(launch_gemv_t(x.block_ptr(b), 0, 1, y.block_ptr(b), x.block_size(b), x.cplx ? 1 : 0, ctx->partial(),
+                             1.0, ctx->dscal() + 2 * i, ctx->stream()));
+  }
+  std::vector<double> h(2 * std::max(nb, 1), 0.0);
+  if (nb) {
+    SADMRG_HIP(hipMemcpyAsync(h.data(), ctx->dscal(), 2 * nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream()));
+  }
+  ctx->sync();
+  dcmplex s(0, 0);
+  for (int i = 0; i < nb; ++i) s += dcmplex(h[2 * i], h[2 * i + 1]);
+  return s;
+}
+
+double op_normalize(BTensor& t) {
+  double nor = std::sqrt(std::fabs(op_dot(t, t).real()));
+  if (nor > 1e-12) op_scale(t, 1.0 / nor);
+  return nor;
+}
+
+std::mt19937_64& global_rng() {
+  static std::mt19937_64 g(42);
+  return g;
+}
+
+void reset_global_rng(uint64_t seed) { global_rng().seed(seed); }
+
+void op_random_init(BTensor& t) {
+  if (t.is_null() || t.nelem == 0) return;
+  std::uniform_real_distribution<double> dist(-1.0, 1.0);
+  auto& g = global_rng();
+  std::vector<double> h(t.nelem * t.w());
+  for (int b = 0; b < t.nten(); ++b) {
+    if (!t.st.allowed(b) || !t.present(b)) continue;
+    const int64_t n = t.block_size(b);
+    double* p = h.data() + t.off[b] * t.w();
+    for (int64_t e = 0; e < n; ++e) {
+      if (!t.cplx) p[e] = dist(g);
+      else {
+        // dcmplex(dist(g), dist(g)) as compiled by GCC evaluates the
+        // imaginary argument first
+        const double im = dist(g);
+        const double re = dist(g);
+        p[2 * e] = re;
+        p[2 * e + 1] = im;
+      }
+    }
+  }
+  SADMRG_HIP(hipMemcpyAsync(t.dptr(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, t.ctx->stream()));
+  t.ctx->sync();
+}
+
+// ---------------------------------------------------------------------------
+// Block SVD truncation (block_tensor_inst.cpp:501-637, dense_inst.cpp:447-541).
+void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, int max_dcut,
+            std::vector<double>& wout, std::vector<int>* kept_out) {
+  Context* ctx = vec.ctx;
+  const SymKind sym = vec.sym();
+  const bool cx = vec.cplx;
+  require(vec.nbond() == 2 && tu.nbond() == 3 && tv.nbond() == 3, "svd: ranks");
+  const Bond bb2_0 = tu.st.bonds[0], bb2_1 = tu.st.bonds[1], bb2_2 = tv.st.bonds[0], bb2_3 = tv.st.bonds[1];
+  const Bond bb0 = fuse_bonds(sym, bb2_0, bb2_1);
+  const Bond bb1 = fuse_bonds(sym, bb2_2, bb2_3);
+  const int nm1 = bb0.nmom(), nm2 = bb1.nmom();
+  require(vec.st.bonds[0].q == bb0.q && vec.st.bonds[1].q == bb1.q, "svd: vector structure does not match tu/tv");
+  struct Sec { int i, j; int m, n, mn; int64_t u_off, vt_off, s_off; };
+  std::vector<Sec> secs;
+  std::vector<int> sec_of(nm1, -1);
+  int64_t u_tot = 0, vt_tot = 0, s_tot = 0, a_max = 0;
+  for (int i = 0; i < nm1; ++i)
+    for (int j = 0; j < nm2; ++j) {
+      if (bb0.q[i] != bb1.q[j]) continue;
+      const int b = i + j * nm1;
+      if (!vec.present(b)) continue;
+      Sec s{i, j, bb0.dim[i], bb1.dim[j], std::min(bb0.dim[i], bb1.dim[j]), u_tot, vt_tot, s_tot};
+      sec_of[i] = static_cast<int>(secs.size());
+      secs.push_back(s);
+      u_tot += int64_t(s.m) * s.mn;
+      vt_tot += int64_t(s.mn) * s.n;
+      s_tot += s.mn;
+      a_max = std::max<int64_t>(a_max, int64_t(s.m) * s.n);
+    }
+  const int w = vec.w();
+  DevBuf du(ctx, std::max<int64_t>(u_tot, 1) * 8 * w), dvt(ctx, std::max<int64_t>(vt_tot, 1) * 8 * w);
+  DevBuf ds(ctx, std::max<int64_t>(s_tot, 1) * 8), dwork(ctx, std::max<int64_t>(a_max, 1) * 8 * w);
+  DevBuf de(ctx, std::max<int64_t>(s_tot, 1) * 8 + 64), dinfo(ctx, sizeof(int) * std::max<size_t>(secs.size(), 1));
+  rocblas_handle h = static_cast<rocblas_handle>(ctx->solver());
+  for (size_t si = 0; si < secs.size(); ++si) {
+    const Sec& s = secs[si];
+    const int b = s.i + s.j * nm1;
+    SADMRG_HIP(hipMemcpyAsync(dwork.get(), vec.block_ptr(b), int64_t(s.m) * s.n * 8 * w, hipMemcpyDeviceToDevice,
+                              ctx->stream()));
+    rocblas_status st;
+    if (!cx)
+      st = rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, s.m, s.n, dwork.as<double>(), s.m,
+                            ds.as<double>() + s.s_off, du.as<double>() + s.u_off, s.m, dvt.as<double>() + s.vt_off,
+                            s.mn, de.as<double>(), rocblas_outofplace, dinfo.as<int>() + si);
+    else
+      st = rocsolver_zgesvd(h, rocblas_svect_singular, rocblas_svect_singular, s.m, s.n,
+                            dwork.as<rocblas_double_complex>(), s.m, ds.as<double>() + s.s_off,
+                            du.as<rocblas_double_complex>() + s.u_off, s.m,
+                            dvt.as<rocblas_double_complex>() + s.vt_off, s.mn, de.as<double>(), rocblas_outofplace,
+                            dinfo.as<int>() + si);
+    if (st != rocblas_status_success) fail(kErrNumerics, "rocsolver gesvd failed");
+  }
+  std::vector<double> hs(std::max<int64_t>(s_tot, 1));
+  std::vector<int> hinfo(std::max<size_t>(secs.size(), 1));
+  SADMRG_HIP(hipMemcpyAsync(hs.data(), ds.get(), s_tot * 8, hipMemcpyDeviceToHost, ctx->stream()));
+  SADMRG_HIP(hipMemcpyAsync(hinfo.data(), dinfo.get(), secs.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
+  ctx->sync();
+  for (size_t si = 0; si < secs.size(); ++si)
+    if (hinfo[si] != 0) fail(kErrNumerics, "Dense::svd: gesvd info=" + std::to_string(hinfo[si]));
+  // kept per sector: S/S_max > 1e-10, at least one, at most max_dcut
+  std::vector<int> dc(nm1, 0);
+  struct SV { double v; int sec; int idx; };
+  std::vector<SV> all;
+  for (size_t si = 0; si < secs.size(); ++si) {
+    const Sec& s = secs[si];
+    const double* S = hs.data() + s.s_off;
+    const double smax = S[0] > 1e-16 ? S[0] : 1.0;
+    int kept = 0;
+    for (int k = 0; k < s.mn; ++k)
+      if (S[k] / smax > 1e-10) ++kept;
+    if (kept == 0) kept = 1;
+    if (max_dcut > 0 && kept > max_dcut) kept = max_dcut;
+    dc[s.i] = kept;
+    for (int k = 0; k < kept; ++k) all.push_back(SV{S[k], s.i, k});
+  }
+  // global descending order; ties broken by sector then index (deterministic)
+  std::stable_sort(all.begin(), all.end(), [](const SV& a, const SV& b) { return a.v > b.v; });
+  const int n = static_cast<int>(all.size());
+  const int dcut = n > max_dcut ? max_dcut : n;
+  std::vector<int> dim(nm1, 0);
+  for (int k = 0; k < dcut; ++k) dim[all[k].sec]++;
+  wout.assign(n, 0.0);
+  int mk = 0, md = dcut;
+  std::vector<int> kept_q, kept_d;
+  for (int i = 0; i < nm1; ++i) {
+    if (sec_of[i] < 0) continue;
+    const double* S = hs.data() + secs[sec_of[i]].s_off;
+    for (int k = 0; k < dim[i]; ++k) wout[mk + k] = S[k];
+    for (int k = dim[i]; k < dc[i]; ++k) wout[md + k - dim[i]] = S[k];
+    mk += dim[i];
+    md += dc[i] - dim[i];
+    if (dim[i] > 0) { kept_q.push_back(bb0.q[i]); kept_d.push_back(dim[i]); }
+  }
+  if (kept_out) *kept_out = dim;
+  // projectors: proju = (bb0^-1, new), projv = (bb1^-1, new')
+  const Bond nb = Bond::make(bb0.dir, kept_q, kept_d);
+  Bond nbv = nb;
+  if (sym == SymKind::U1) nbv.dir = -nbv.dir;
+  Struct su, sv;
+  su.sym = sv.sym = sym;
+  su.locspin = sv.locspin = 0;
+  su.bonds = {bb0.inverted(), nb};
+  sv.bonds = {bb1.inverted(), nbv};
+  std::vector<uint8_t> mu(su.nten(), 0), mv(sv.nten(), 0);
+  // the k-th allowed block of each projector takes the k-th kept sector
+  std::vector<int> ublk, vblk;
+  for (int b = 0; b < su.nten(); ++b)
+    if (su.allowed(b)) ublk.push_back(b);
+  for (int b = 0; b < sv.nten(); ++b)
+    if (sv.allowed(b)) vblk.push_back(b);
+  std::vector<int> kept_secs;
+  for (int i = 0; i < nm1; ++i)
+    if (dim[i] > 0) kept_secs.push_back(i);
+  require(ublk.size() >= kept_secs.size() && vblk.size() >= kept_secs.size(), "svd: projector blocks");
+  for (size_t k = 0; k < kept_secs.size(); ++k) { mu[ublk[k]] = 1; mv[vblk[k]] = 1; }
+  BTensor pu, pv;
+  pu.layout(ctx, cx, su, mu, kCgcIdent);
+  pv.layout(ctx, cx, sv, mv, kCgcIdent);
+  // column scales S^p
+  std::vector<double> cs;
+  std::vector<int64_t> cs_off_u(nm1, 0), cs_off_v(nm1, 0);
+  for (int i : kept_secs) {
+    const double* S = hs.data() + secs[sec_of[i]].s_off;
+    cs_off_u[i] = static_cast<int64_t>(cs.size());
+    for (int k = 0; k < dim[i]; ++k) cs.push_back(p1 != 0.0 ? std::pow(S[k], p1) : 1.0);
+    cs_off_v[i] = static_cast<int64_t>(cs.size());
+    for (int k = 0; k < dim[i]; ++k) cs.push_back(p2 != 0.0 ? std::pow(S[k], p2) : 1.0);
+  }
+  DevBuf dcs(ctx, std::max<size_t>(cs.size(), 1) * 8);
+  if (!cs.empty())
+    SADMRG_HIP(hipMemcpyAsync(dcs.get(), cs.data(), cs.size() * 8, hipMemcpyHostToDevice, ctx->stream()));
+  for (size_t k = 0; k < kept_secs.size(); ++k) {
+    const int i = kept_secs[k];
+    const Sec& s = secs[sec_of[i]];
+    // U block (m x dim): first dim columns of U scaled by S^p1
+    SADMRG_HIP(launch_copy2d(s.m, dim[i], du.as<double>() + s.u_off * w, 1, s.m, pu.block_ptr(ublk[k]), s.m, w, 1.0,
+                             dcs.as<double>() + cs_off_u[i], 0, ctx->stream()));
+    // V block (n x dim): V(j, c) = VT(c, j) * S_c^p2 (no conjugation, dense_inst.cpp:531-535)
+    SADMRG_HIP(launch_copy2d(s.n, dim[i], dvt.as<double>() + s.vt_off * w, s.mn, 1, pv.block_ptr(vblk[k]), s.n, w,
+                             1.0, dcs.as<double>() + cs_off_v[i], 0, ctx->stream()));
+  }
+  BTensor left, rght;
+  op_fuse(left, ctx, cx, sym, bb2_0, bb2_1);
+  op_fuse(rght, ctx, cx, sym, bb2_2, bb2_3);
+  op_contract(tu, left, 2, pu, 0);
+  op_contract(tv, rght, 2, pv, 0);
+  ctx->sync();
+}
+
+}  // namespace sadmrg

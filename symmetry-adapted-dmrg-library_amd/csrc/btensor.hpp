@@ -1,0 +1,118 @@
+// btensor.hpp -- device-resident block-sparse symmetric tensor.
+//
+// BTensor is the runtime-typed core behind BlockTensor<Scalar,Sym>
+// (block_tensor.hpp:38-193 of the reference).  Sector metadata lives on the
+// host; the reduced-element data of all present blocks lives in one device
+// allocation, block after block in block-index order, each block column-major.
+// A block that is allowed but absent is "null", exactly like a Dense with
+// nbond == 0 in the reference (dense_inst.cpp:755); null blocks own no memory.
+//
+// Operations mirror the reference's BlockTensor methods: the host enumerates
+// the same loop nests with the same matching predicates, fac tables and skip
+// thresholds, and lowers the per-block GEMMs into grouped block tables
+// (gemm_plan.hpp) executed by one or two device launches.
+#pragma once
+#include <cstdint>
+#include <random>
+#include <vector>
+#include "cgc.hpp"
+#include "core.hpp"
+#include "gemm_plan.hpp"
+#include "symmetry.hpp"
+
+namespace sadmrg {
+
+// Symbolic CGC block for SU(2) 2-leg tensors (3-leg tensors always carry the
+// canonical <j0 j1 | j2> block in the paths used here).
+enum CgcTag : uint8_t { kCgcIdent = 0, kCgcSinglet = 1, kCgcOther = 2, kCgcCanon = 3 };
+
+struct BTensor {
+  Context* ctx = nullptr;
+  bool cplx = false;
+  Struct st;                   // st.bonds empty => null tensor
+  std::vector<int64_t> off;    // per block element offset, -1 = null block
+  std::vector<uint8_t> tag;    // per block CGC tag (SU(2))
+  DevBuf buf;                  // nelem elements of (cplx ? 16 : 8) bytes
+  int64_t nelem = 0;
+
+  BTensor() = default;
+  BTensor(BTensor&&) = default;
+  BTensor& operator=(BTensor&&) = default;
+  BTensor(const BTensor&) = delete;
+  BTensor& operator=(const BTensor&) = delete;
+
+  bool is_null() const { return st.bonds.empty(); }
+  SymKind sym() const { return st.sym; }
+  int nbond() const { return st.nbond(); }
+  int nten() const { return st.nten(); }
+  bool present(int b) const { return b >= 0 && b < static_cast<int>(off.size()) && off[b] >= 0; }
+  int w() const { return cplx ? 2 : 1; }
+  double* dptr() const { return buf.as<double>(); }
+  double* block_ptr(int b) const { return dptr() + off[b] * w(); }
+  int64_t block_size(int b) const { return st.block_size(b); }
+
+  // Lays out the blocks flagged in mask (zero-filled device memory).
+  void layout(Context* c, bool complex_, const Struct& s, const std::vector<uint8_t>& mask, uint8_t tag0);
+  void clean();
+};
+
+// ---- exchange with host (the C ABI tensor format) ------------------------
+// header = [nbond, locspin, {dir, nmom, q..., dim...} per bond]
+Struct parse_header(SymKind sym, const int* header, int len);
+std::vector<int> make_header(const Struct& s);
+void tensor_from_host(BTensor& t, Context* ctx, SymKind sym, bool cplx, const int* header, int len,
+                      const unsigned char* present, const double* data, int cgc2_kind);
+// data must hold nelem * w doubles
+void tensor_to_host(const BTensor& t, unsigned char* present, double* data);
+void copy_tensor(BTensor& dst, const BTensor& src);
+
+// ---- reference-method mirrors --------------------------------------------
+// BlockTensor::hamiltonian_vector_multiplication (block_tensor_inst.cpp:2170-2318)
+void op_hvm(BTensor& out, const BTensor& vec, const BTensor& op1, const BTensor& op2, CgcTables* cgc);
+// BlockTensor::contract (block_tensor_inst.cpp:1046-1126)
+void op_contract(BTensor& out, const BTensor& t1, int i1, const BTensor& t2, int i2);
+
+enum EnvKind : int {
+  kOverlapInitial = 0,         // block_tensor_inst.cpp:1661-1788
+  kOverlapTransformation = 1,  // :1790-1951
+  kOperatorInitial = 2,        // :1282-1469
+  kOperatorTransformation = 3, // :1471-1658
+  kOperatorPairup = 4,         // :1953-2167
+};
+void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const BTensor* op1, const BTensor* op2,
+            int flag, CgcTables* cgc);
+
+// BlockTensor::svd (block_tensor_inst.cpp:501-637).  tu/tv are overwritten;
+// wout receives kept values (sector order) followed by discarded values.
+void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, int max_dcut,
+            std::vector<double>& wout, std::vector<int>* kept_per_sector = nullptr);
+
+// BlockTensor::fuse(BondType, BondType) (block_tensor_inst.cpp:708-743)
+void op_fuse(BTensor& out, Context* ctx, bool cplx, SymKind sym, const Bond& b1, const Bond& b2);
+// BlockTensor::fuse_to_singlet (block_tensor_inst.cpp:780-816)
+void op_fuse_to_singlet(BTensor& out, Context* ctx, bool cplx, SymKind sym, const Bond& b1, const Bond& b2);
+// BlockTensor::left2right_vectran / right2left_vectran (:429-497)
+void op_vectran(BTensor& t, bool left_to_right);
+// BlockTensor::shift (:913-948)
+void op_shift(BTensor& t, int i0, int i1);
+// BlockTensor::take_conjugate(int) (:684-701): bond direction flip + CGC
+void op_take_conjugate(BTensor& t, int bond);
+// BlockTensor::makeup_input_vector (:1231-1257)
+void op_makeup_input_vector(BTensor& t);
+// BlockTensor::make_spinor_start (:1145-1217)
+void op_make_spinor_start(BTensor& t, Context* ctx, bool cplx, SymKind sym, int physpn);
+
+// ---- vector algebra (block_tensor_inst.cpp:197-376) ----------------------
+void op_scale(BTensor& t, double re, double im = 0.0);            // *=, /=
+void op_axpy(BTensor& y, double re, double im, const BTensor& x); // += (1,0), -= (-1,0)
+dcmplex op_dot(const BTensor& x, const BTensor& y);               // inner_prod_c
+double op_normalize(BTensor& t);                                  // normalize_vector
+void op_fill(BTensor& t, double v);                               // operator=(double)
+
+// Global generator of the reference (dense_inst.cpp:23-24): mt19937_64(42),
+// uniform(-1, 1), consumed element by element in block order.
+std::mt19937_64& global_rng();
+void reset_global_rng(uint64_t seed);
+void op_random_init(BTensor& t);  // random_initialize_tensor (:1221-1229)
+
+}  // namespace sadmrg

@@ -1,0 +1,782 @@
+// capi.cpp -- C ABI (include/sadmrg.h) over the device engine.
+#include "../../include/sadmrg.h"
+#include <cstring>
+#include <exception>
+#include <memory>
+#include "btensor.hpp"
+#include "dmrg.hpp"
+#include "heff.hpp"
+#include "lanczos.hpp"
+
+using namespace sadmrg;
+
+struct sadmrg_ctx {
+  std::unique_ptr<Context> c;
+};
+struct sadmrg_cgc {
+  std::unique_ptr<CgcTables> t;
+};
+struct sadmrg_tensor {
+  Context* ctx;
+  SymKind sym;
+  bool cplx;
+  BTensor t;
+};
+struct sadmrg_heff {
+  std::unique_ptr<HeffPlan> p;
+};
+struct sadmrg_dmrg {
+  Context* ctx;
+  std::unique_ptr<Dmrg> d;
+};
+
+namespace {
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return SADMRG_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return SADMRG_ERR_INVALID;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return SADMRG_ERR_INVALID;
+  }
+}
+
+void need(const void* p, const char* what) {
+  if (!p) fail(kErrInvalid, std::string("null argument: ") + what);
+}
+
+SymKind sym_of(int s) {
+  if (s != SADMRG_U1 && s != SADMRG_SU2) fail(kErrInvalid, "symmetry must be SADMRG_U1 or SADMRG_SU2");
+  return s == SADMRG_SU2 ? SymKind::SU2 : SymKind::U1;
+}
+
+Prec prec_of(int p) {
+  if (p != SADMRG_FP64 && p != SADMRG_FP32) fail(kErrInvalid, "precision must be SADMRG_FP64 or SADMRG_FP32");
+  return p == SADMRG_FP32 ? Prec::FP32 : Prec::FP64;
+}
+
+Bond bond_of(const int* b) {
+  need(b, "bond");
+  const int nm = b[1];
+  if (nm < 0) fail(kErrInvalid, "bond: negative sector count");
+  return Bond::make(b[0], std::vector<int>(b + 2, b + 2 + nm), std::vector<int>(b + 2 + nm, b + 2 + 2 * nm));
+}
+
+void same_kind(const sadmrg_tensor* a, const sadmrg_tensor* b) {
+  if (a->sym != b->sym || a->cplx != b->cplx || a->ctx != b->ctx)
+    fail(kErrInvalid, "tensors differ in symmetry, scalar type or context");
+}
+
+CgcTables* cg_of(sadmrg_cgc* c, SymKind s) {
+  if (s == SymKind::SU2) {
+    need(c, "cgc (required for SU(2))");
+    return c->t.get();
+  }
+  return c ? c->t.get() : nullptr;
+}
+
+void fix_kind(sadmrg_tensor* t) {
+  // ops rebuild the BTensor; keep the handle's symmetry consistent
+  if (!t->t.is_null()) t->t.st.sym = t->sym;
+  t->t.ctx = t->ctx;
+  t->t.cplx = t->cplx;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sadmrg_last_error(void) { return last_error(); }
+
+int sadmrg_version(int* major, int* minor) {
+  return guard([&] {
+    need(major, "major");
+    need(minor, "minor");
+    *major = 0;
+    *minor = 1;
+  });
+}
+
+int sadmrg_ctx_create(int device, sadmrg_ctx** out) {
+  return guard([&] {
+    need(out, "out");
+    auto c = std::make_unique<sadmrg_ctx>();
+    c->c = std::make_unique<Context>(device);
+    *out = c.release();
+  });
+}
+
+int sadmrg_ctx_destroy(sadmrg_ctx* ctx) {
+  return guard([&] { delete ctx; });
+}
+
+int sadmrg_ctx_sync(sadmrg_ctx* ctx) {
+  return guard([&] {
+    need(ctx, "ctx");
+    ctx->c->sync();
+  });
+}
+
+int sadmrg_ctx_memory(sadmrg_ctx* ctx, long long* in_use, long long* cached) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (in_use) *in_use = static_cast<long long>(ctx->c->bytes_in_use());
+    if (cached) *cached = static_cast<long long>(ctx->c->bytes_cached());
+  });
+}
+
+int sadmrg_ctx_stream(sadmrg_ctx* ctx, void** stream) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(stream, "stream");
+    *stream = static_cast<void*>(ctx->c->stream());
+  });
+}
+
+int sadmrg_clebsch_gordan(int j1, int m1, int j2, int m2, int j3, int m3, double* out) {
+  return guard([&] {
+    need(out, "out");
+    *out = clebsch_gordan(j1, m1, j2, m2, j3, m3);
+  });
+}
+
+int sadmrg_cgc_create(int physpn, sadmrg_cgc** out) {
+  return guard([&] {
+    need(out, "out");
+    auto c = std::make_unique<sadmrg_cgc>();
+    c->t = std::make_unique<CgcTables>(physpn);
+    *out = c.release();
+  });
+}
+
+int sadmrg_cgc_destroy(sadmrg_cgc* cgc) {
+  return guard([&] { delete cgc; });
+}
+
+int sadmrg_cgc_hamilt_vec(sadmrg_cgc* c, int i, int j, int k, double* out) {
+  return guard([&] {
+    need(c, "cgc");
+    need(out, "out");
+    *out = c->t->hamilt_vec(i, j, k);
+  });
+}
+
+int sadmrg_cgc_onsite(sadmrg_cgc* c, int side, int j, int m, int l, int i, int p, double* out) {
+  return guard([&] {
+    need(c, "cgc");
+    need(out, "out");
+    *out = c->t->onsite(side, j, m, l, i, p);
+  });
+}
+
+int sadmrg_cgc_transformation(sadmrg_cgc* c, int side, int j, int m, int l, int i, int k, int p, double* out) {
+  return guard([&] {
+    need(c, "cgc");
+    need(out, "out");
+    *out = c->t->transformation(side, j, m, l, i, k, p);
+  });
+}
+
+int sadmrg_cgc_pairup(sadmrg_cgc* c, int side, int j, int i, int k, int m, int p, double* out) {
+  return guard([&] {
+    need(c, "cgc");
+    need(out, "out");
+    *out = c->t->pairup(side, j, i, k, m, p);
+  });
+}
+
+int sadmrg_tensor_create(sadmrg_ctx* ctx, int sym, int cplx, const int* header, int header_len,
+                         const unsigned char* present, const double* data, int cgc2_kind, sadmrg_tensor** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(header, "header");
+    need(out, "out");
+    auto t = std::make_unique<sadmrg_tensor>();
+    t->ctx = ctx->c.get();
+    t->sym = sym_of(sym);
+    t->cplx = cplx != 0;
+    Struct s = parse_header(t->sym, header, header_len);
+    if (s.nbond() > 0) {
+      need(present, "present");
+      int64_t n = 0;
+      for (int b = 0; b < s.nten(); ++b)
+        if (present[b] && s.allowed(b)) n += s.block_size(b);
+      if (n > 0) need(data, "data");
+    }
+    tensor_from_host(t->t, t->ctx, t->sym, t->cplx, header, header_len, present, data, cgc2_kind);
+    fix_kind(t.get());
+    *out = t.release();
+  });
+}
+
+int sadmrg_tensor_null(sadmrg_ctx* ctx, int sym, int cplx, sadmrg_tensor** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    auto t = std::make_unique<sadmrg_tensor>();
+    t->ctx = ctx->c.get();
+    t->sym = sym_of(sym);
+    t->cplx = cplx != 0;
+    fix_kind(t.get());
+    *out = t.release();
+  });
+}
+
+int sadmrg_tensor_destroy(sadmrg_tensor* t) {
+  return guard([&] { delete t; });
+}
+
+int sadmrg_tensor_header(const sadmrg_tensor* t, int* header, int cap, int* len) {
+  return guard([&] {
+    need(t, "tensor");
+    const std::vector<int> h = make_header(t->t.st);
+    if (len) *len = static_cast<int>(h.size());
+    if (header) {
+      if (cap < static_cast<int>(h.size())) fail(kErrInvalid, "header buffer too small");
+      std::memcpy(header, h.data(), h.size() * sizeof(int));
+    }
+  });
+}
+
+int sadmrg_tensor_nten(const sadmrg_tensor* t, int* nten) {
+  return guard([&] {
+    need(t, "tensor");
+    need(nten, "nten");
+    *nten = t->t.nten();
+  });
+}
+
+int sadmrg_tensor_nelem(const sadmrg_tensor* t, long long* n) {
+  return guard([&] {
+    need(t, "tensor");
+    need(n, "n");
+    *n = t->t.nelem;
+  });
+}
+
+int sadmrg_tensor_download(const sadmrg_tensor* t, unsigned char* present, double* data) {
+  return guard([&] {
+    need(t, "tensor");
+    if (t->t.is_null()) return;
+    need(present, "present");
+    tensor_to_host(t->t, present, data);
+  });
+}
+
+int sadmrg_tensor_copy(sadmrg_tensor* dst, const sadmrg_tensor* src) {
+  return guard([&] {
+    need(dst, "dst");
+    need(src, "src");
+    same_kind(dst, src);
+    copy_tensor(dst->t, src->t);
+    fix_kind(dst);
+  });
+}
+
+int sadmrg_tensor_device_ptr(const sadmrg_tensor* t, void** ptr) {
+  return guard([&] {
+    need(t, "tensor");
+    need(ptr, "ptr");
+    *ptr = t->t.dptr();
+  });
+}
+
+int sadmrg_hamiltonian_vector_multiplication(sadmrg_tensor* out, const sadmrg_tensor* vec, const sadmrg_tensor* op1,
+                                             const sadmrg_tensor* op2, sadmrg_cgc* cgc) {
+  return guard([&] {
+    need(out, "out");
+    need(vec, "vec");
+    need(op1, "op1");
+    need(op2, "op2");
+    same_kind(out, vec);
+    same_kind(vec, op1);
+    same_kind(vec, op2);
+    BTensor r;
+    op_hvm(r, vec->t, op1->t, op2->t, cg_of(cgc, vec->sym));
+    out->t = std::move(r);
+    fix_kind(out);
+  });
+}
+
+int sadmrg_contract(sadmrg_tensor* out, const sadmrg_tensor* t1, int i1, const sadmrg_tensor* t2, int i2) {
+  return guard([&] {
+    need(out, "out");
+    need(t1, "t1");
+    need(t2, "t2");
+    same_kind(t1, t2);
+    same_kind(out, t1);
+    if (t1->t.is_null() || t2->t.is_null()) fail(kErrInvalid, "contract: null tensor");
+    BTensor r;
+    op_contract(r, t1->t, i1, t2->t, i2);
+    out->t = std::move(r);
+    fix_kind(out);
+  });
+}
+
+int sadmrg_env_update(sadmrg_tensor* out, int kind, const sadmrg_tensor* uu, const sadmrg_tensor* vv,
+                      const sadmrg_tensor* op1, const sadmrg_tensor* op2, int flag, sadmrg_cgc* cgc) {
+  return guard([&] {
+    need(out, "out");
+    need(uu, "uu");
+    need(vv, "vv");
+    same_kind(out, uu);
+    same_kind(uu, vv);
+    if (kind != 0) {
+      need(op1, "op1");
+      same_kind(uu, op1);
+    }
+    if (kind == 4) {
+      need(op2, "op2");
+      same_kind(uu, op2);
+    }
+    if (kind < 0 || kind > 4) fail(kErrInvalid, "env_update: kind must be 0..4");
+    BTensor r;
+    op_env(r, kind, uu->t, vv->t, op1 ? &op1->t : nullptr, op2 ? &op2->t : nullptr, flag, cg_of(cgc, uu->sym));
+    out->t = std::move(r);
+    fix_kind(out);
+  });
+}
+
+int sadmrg_svd(const sadmrg_tensor* vec, sadmrg_tensor* tu, double p1, sadmrg_tensor* tv, double p2, int max_dcut,
+               double* wout, int wout_cap, int* nw) {
+  return guard([&] {
+    need(vec, "vec");
+    need(tu, "tu");
+    need(tv, "tv");
+    same_kind(vec, tu);
+    same_kind(vec, tv);
+    if (max_dcut < 1) fail(kErrInvalid, "svd: max_dcut must be >= 1");
+    std::vector<double> w;
+    op_svd(vec->t, tu->t, p1, tv->t, p2, max_dcut, w);
+    fix_kind(tu);
+    fix_kind(tv);
+    if (nw) *nw = static_cast<int>(w.size());
+    if (wout) {
+      for (int k = 0; k < wout_cap && k < static_cast<int>(w.size()); ++k) wout[k] = w[k];
+    }
+  });
+}
+
+int sadmrg_fuse(sadmrg_tensor* out, const int* b1, const int* b2) {
+  return guard([&] {
+    need(out, "out");
+    op_fuse(out->t, out->ctx, out->cplx, out->sym, bond_of(b1), bond_of(b2));
+    fix_kind(out);
+  });
+}
+
+int sadmrg_fuse_to_singlet(sadmrg_tensor* out, const int* b1, const int* b2) {
+  return guard([&] {
+    need(out, "out");
+    op_fuse_to_singlet(out->t, out->ctx, out->cplx, out->sym, bond_of(b1), bond_of(b2));
+    fix_kind(out);
+  });
+}
+
+int sadmrg_make_spinor_start(sadmrg_tensor* out, int physpn) {
+  return guard([&] {
+    need(out, "out");
+    op_make_spinor_start(out->t, out->ctx, out->cplx, out->sym, physpn);
+    fix_kind(out);
+  });
+}
+
+int sadmrg_tensor_transform(sadmrg_tensor* t, int op, double a, double b) {
+  return guard([&] {
+    need(t, "tensor");
+    BTensor& x = t->t;
+    if (x.is_null() && op != 6 && op != 4 && op != 8) fail(kErrInvalid, "transform: null tensor");
+    switch (op) {
+      case 0: op_vectran(x, true); break;
+      case 1: op_vectran(x, false); break;
+      case 2: op_take_conjugate(x, static_cast<int>(a)); break;
+      case 3: op_makeup_input_vector(x); break;
+      case 4: op_scale(x, a, b); break;
+      case 5: op_normalize(x); break;
+      case 6: op_random_init(x); break;
+      case 7: op_shift(x, static_cast<int>(a), static_cast<int>(b)); break;
+      case 8: op_fill(x, a); break;
+      default: fail(kErrInvalid, "transform: unknown op");
+    }
+    t->ctx->sync();
+    fix_kind(t);
+  });
+}
+
+int sadmrg_tensor_axpy(sadmrg_tensor* y, double re, double im, const sadmrg_tensor* x) {
+  return guard([&] {
+    need(y, "y");
+    need(x, "x");
+    same_kind(y, x);
+    op_axpy(y->t, re, im, x->t);
+    y->ctx->sync();
+    fix_kind(y);
+  });
+}
+
+int sadmrg_tensor_dot(const sadmrg_tensor* x, const sadmrg_tensor* y, double* out2) {
+  return guard([&] {
+    need(x, "x");
+    need(y, "y");
+    need(out2, "out");
+    same_kind(x, y);
+    const dcmplex v = op_dot(x->t, y->t);
+    out2[0] = v.real();
+    out2[1] = v.imag();
+  });
+}
+
+int sadmrg_rng_seed(unsigned long long seed) {
+  return guard([&] { reset_global_rng(seed); });
+}
+
+int sadmrg_heff_create(sadmrg_ctx* ctx, sadmrg_cgc* cgc, const sadmrg_tensor* hl, const sadmrg_tensor* hr, int nterm,
+                       const sadmrg_tensor* const* o1, const sadmrg_tensor* const* o2,
+                       const sadmrg_tensor* psi_layout, sadmrg_heff** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(psi_layout, "psi_layout");
+    need(out, "out");
+    if (nterm < 0) fail(kErrInvalid, "heff: negative term count");
+    if (nterm > 0) {
+      need(o1, "o1");
+      need(o2, "o2");
+    }
+    static const BTensor kNull;
+    std::vector<const BTensor*> a, b;
+    for (int t = 0; t < nterm; ++t) {
+      need(o1[t], "o1[t]");
+      need(o2[t], "o2[t]");
+      same_kind(o1[t], psi_layout);
+      same_kind(o2[t], psi_layout);
+      a.push_back(&o1[t]->t);
+      b.push_back(&o2[t]->t);
+    }
+    if (hl) same_kind(hl, psi_layout);
+    if (hr) same_kind(hr, psi_layout);
+    auto h = std::make_unique<sadmrg_heff>();
+    h->p = std::make_unique<HeffPlan>(ctx->c.get(), cg_of(cgc, psi_layout->sym), hl ? hl->t : kNull,
+                                      hr ? hr->t : kNull, a, b, psi_layout->t);
+    *out = h.release();
+  });
+}
+
+int sadmrg_heff_destroy(sadmrg_heff* h) {
+  return guard([&] { delete h; });
+}
+
+int sadmrg_heff_apply(sadmrg_heff* h, const sadmrg_tensor* in, sadmrg_tensor* out, int precision) {
+  return guard([&] {
+    need(h, "heff");
+    need(in, "psi_in");
+    need(out, "psi_out");
+    const BTensor& L = h->p->layout();
+    if (in->t.st != L.st || in->t.off != L.off) fail(kErrInvalid, "heff_apply: psi_in layout differs from the plan");
+    if (out->t.st != L.st || out->t.off != L.off) {
+      std::vector<uint8_t> mask(L.nten());
+      for (int b = 0; b < L.nten(); ++b) mask[b] = L.present(b);
+      out->t.clean();
+      out->t.layout(L.ctx, L.cplx, L.st, mask, kCgcSinglet);
+      fix_kind(out);
+    }
+    h->p->apply(in->t.dptr(), out->t.dptr(), prec_of(precision));
+    h->p->ctx()->sync();
+  });
+}
+
+int sadmrg_heff_apply_device(sadmrg_heff* h, const void* d_in, void* d_out, int precision) {
+  return guard([&] {
+    need(h, "heff");
+    need(d_in, "d_in");
+    need(d_out, "d_out");
+    h->p->apply(static_cast<const double*>(d_in), static_cast<double*>(d_out), prec_of(precision));
+  });
+}
+
+int sadmrg_heff_apply_stage(sadmrg_heff* h, const void* d_in, void* d_out, int precision, int stage) {
+  return guard([&] {
+    need(h, "heff");
+    need(d_in, "d_in");
+    need(d_out, "d_out");
+    if (stage < 0 || stage > 2) fail(kErrInvalid, "heff_apply_stage: stage must be 0, 1 or 2");
+    h->p->apply(static_cast<const double*>(d_in), static_cast<double*>(d_out), prec_of(precision), stage);
+  });
+}
+
+int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops) {
+  return guard([&] {
+    need(h, "heff");
+    need(flops, "flops");
+    *flops = h->p->stage_flops(stage);
+  });
+}
+
+int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n, int* info) {
+  return guard([&] {
+    need(h, "heff");
+    const HeffStats& s = h->p->stats();
+    if (flops) *flops = s.flops;
+    if (bytes) *bytes = s.bytes64;
+    if (n) *n = s.n;
+    if (info) {
+      info[0] = s.nsec_in;
+      info[1] = s.stage1_probs;
+      info[2] = s.stage2_probs;
+      info[3] = s.tiles1;
+      info[4] = s.tiles2;
+      info[5] = s.tile_cfg1;
+      info[6] = s.tile_cfg2;
+      info[7] = 0;
+    }
+  });
+}
+
+int sadmrg_lanczos(sadmrg_heff* h, sadmrg_tensor* psi, int mlanc, int precision, int refine, double refine_tol,
+                   double* energy, int* counters, double* residual) {
+  return guard([&] {
+    need(h, "heff");
+    need(psi, "psi");
+    const BTensor& L = h->p->layout();
+    if (psi->t.st != L.st || psi->t.off != L.off) fail(kErrInvalid, "lanczos: psi layout differs from the plan");
+    LanczosOpts o;
+    o.mlanc = mlanc;
+    o.prec = prec_of(precision);
+    o.refine = refine != 0;
+    if (refine_tol > 0) o.refine_tol = refine_tol;
+    const LanczosOut r = lanczos_solve(*h->p, psi->t.dptr(), o);
+    h->p->ctx()->sync();
+    if (energy) *energy = r.energy;
+    if (counters) {
+      counters[0] = r.matvec64;
+      counters[1] = r.matvec32;
+      counters[2] = r.steps;
+      counters[3] = r.restarts;
+    }
+    if (residual) *residual = r.residual64;
+  });
+}
+
+int sadmrg_dmrg_create(sadmrg_ctx* ctx, int sym, int cplx, int xx, int yy, int bondd, int physpn, int totspin,
+                       sadmrg_dmrg** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    if (xx < 1 || yy < 1 || bondd < 1) fail(kErrInvalid, "dmrg: lattice sizes and bondd must be positive");
+    auto d = std::make_unique<sadmrg_dmrg>();
+    d->ctx = ctx->c.get();
+    d->d = std::make_unique<Dmrg>(ctx->c.get(), sym_of(sym), cplx != 0, xx, yy, bondd, physpn, totspin);
+    *out = d.release();
+  });
+}
+
+int sadmrg_dmrg_destroy(sadmrg_dmrg* d) {
+  return guard([&] { delete d; });
+}
+
+int sadmrg_dmrg_ns(sadmrg_dmrg* d, int* ns) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(ns, "ns");
+    *ns = d->d->ns();
+  });
+}
+
+int sadmrg_dmrg_set_couplings(sadmrg_dmrg* d, const int* hmap, const double* coup, int ncoup) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(hmap, "hmap");
+    need(coup, "coup");
+    const int ns = d->d->ns();
+    if (ncoup < 1) fail(kErrInvalid, "set_couplings: ncoup must be >= 1");
+    d->d->set_couplings(std::vector<int>(hmap, hmap + ns * ns), std::vector<double>(coup, coup + ncoup));
+  });
+}
+
+int sadmrg_dmrg_get_couplings(sadmrg_dmrg* d, int* hmap, double* coup, int cap) {
+  return guard([&] {
+    need(d, "dmrg");
+    const auto& h = d->d->hmap();
+    const auto& c = d->d->coup();
+    if (hmap) std::memcpy(hmap, h.data(), h.size() * sizeof(int));
+    if (coup)
+      for (int i = 0; i < cap && i < static_cast<int>(c.size()); ++i) coup[i] = c[i];
+  });
+}
+
+int sadmrg_dmrg_set_max_dcut(sadmrg_dmrg* d, int max_dcut) {
+  return guard([&] {
+    need(d, "dmrg");
+    if (max_dcut < 1) fail(kErrInvalid, "max_dcut must be >= 1");
+    d->d->set_max_dcut(max_dcut);
+  });
+}
+
+int sadmrg_dmrg_set_precision(sadmrg_dmrg* d, int precision, int refine, double refine_tol) {
+  return guard([&] {
+    need(d, "dmrg");
+    LanczosOpts o = d->d->lanczos_opts();
+    o.prec = prec_of(precision);
+    o.refine = refine != 0;
+    if (refine_tol > 0) o.refine_tol = refine_tol;
+    d->d->set_lanczos(o);
+  });
+}
+
+int sadmrg_dmrg_idmrg(sadmrg_dmrg* d) {
+  return guard([&] {
+    need(d, "dmrg");
+    d->d->do_idmrg();
+  });
+}
+
+int sadmrg_dmrg_prepare_sweep(sadmrg_dmrg* d) {
+  return guard([&] {
+    need(d, "dmrg");
+    d->d->prepare_sweep();
+  });
+}
+
+int sadmrg_dmrg_sweep(sadmrg_dmrg* d) {
+  return guard([&] {
+    need(d, "dmrg");
+    d->d->sweep();
+  });
+}
+
+int sadmrg_dmrg_energy(sadmrg_dmrg* d, double* e) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(e, "e");
+    *e = d->d->energy();
+  });
+}
+
+int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n) {
+  return guard([&] {
+    need(d, "dmrg");
+    auto& r = d->d->records();
+    const int m = std::min<int>(cap, static_cast<int>(r.size()));
+    for (int i = 0; i < m; ++i) {
+      const StepRecord& s = r[i];
+      double* p = rec + 12 * i;
+      p[0] = s.il;
+      p[1] = s.ir;
+      p[2] = s.energy;
+      p[3] = s.trunc_err;
+      p[4] = s.kept;
+      p[5] = s.matvec64;
+      p[6] = s.matvec32;
+      p[7] = s.t_env;
+      p[8] = s.t_lanczos;
+      p[9] = s.t_svd;
+      p[10] = s.heff_flops;
+      p[11] = 0;
+    }
+    r.erase(r.begin(), r.begin() + m);
+    if (n) *n = m;
+  });
+}
+
+static BTensor& dmrg_slot(sadmrg_dmrg* d, int which, int i, int j) {
+  const int ns = d->d->ns();
+  if (i < 0 || i >= ns || j < 0 || j >= ns) fail(kErrInvalid, "dmrg tensor index out of range");
+  if (which == 0) return d->d->uu(i);
+  if (which == 1) return d->d->hh(i);
+  if (which == 2) return d->d->opr(i, j);
+  fail(kErrInvalid, "dmrg tensor: which must be 0 (uu), 1 (hh) or 2 (opr)");
+}
+
+int sadmrg_dmrg_get_tensor(sadmrg_dmrg* d, int which, int i, int j, sadmrg_tensor* out) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(out, "out");
+    copy_tensor(out->t, dmrg_slot(d, which, i, j));
+    d->ctx->sync();
+    fix_kind(out);
+  });
+}
+
+int sadmrg_dmrg_set_tensor(sadmrg_dmrg* d, int which, int i, int j, const sadmrg_tensor* t) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(t, "tensor");
+    copy_tensor(dmrg_slot(d, which, i, j), t->t);
+    d->ctx->sync();
+  });
+}
+
+int sadmrg_dmrg_call(sadmrg_dmrg* d, int code, int a, int b) {
+  return guard([&] {
+    need(d, "dmrg");
+    switch (code) {
+      case 0: d->d->wavefunc_transformation(a, b); break;
+      case 1: d->d->prepare_site_operator_from_left(a, b); break;
+      case 2: d->d->prepare_site_operator_from_right(a, b); break;
+      case 3: d->d->prepare_site_operator_from_left(a); break;
+      case 4: d->d->prepare_site_operator_from_right(a); break;
+      case 5: d->d->prepare_sweep(); break;
+      default: fail(kErrInvalid, "dmrg_call: unknown code");
+    }
+    d->ctx->sync();
+  });
+}
+
+int sadmrg_dmrg_input_vector(sadmrg_dmrg* d, int il, int ir, sadmrg_tensor* out) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(out, "out");
+    BTensor v;
+    d->d->prepare_input_vector(il, ir, v);
+    out->t = std::move(v);
+    fix_kind(out);
+  });
+}
+
+int sadmrg_dmrg_heff(sadmrg_dmrg* d, int il, int ir, const sadmrg_tensor* vin, sadmrg_tensor* vout, int precision) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(vin, "vin");
+    need(vout, "vout");
+    auto plan = d->d->make_heff(il, ir, vin->t, nullptr);
+    std::vector<uint8_t> mask(vin->t.nten());
+    for (int b = 0; b < vin->t.nten(); ++b) mask[b] = vin->t.present(b);
+    BTensor r;
+    r.layout(d->ctx, vin->t.cplx, vin->t.st, mask, kCgcSinglet);
+    plan->apply(vin->t.dptr(), r.dptr(), prec_of(precision));
+    d->ctx->sync();
+    vout->t = std::move(r);
+    fix_kind(vout);
+  });
+}
+
+int sadmrg_dmrg_lanczos(sadmrg_dmrg* d, int il, int ir, sadmrg_tensor* vec, double* energy) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(vec, "vec");
+    const double e = d->d->lanczos_solve(il, ir, vec->t);
+    if (energy) *energy = e;
+  });
+}
+
+int sadmrg_dmrg_svd(sadmrg_dmrg* d, const sadmrg_tensor* vec, int a, int b, double p1, double p2, double* wout,
+                    int cap, int* nw) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(vec, "vec");
+    d->d->svd(vec->t, a, b, p1, p2);
+    const auto& w = d->d->last_wout();
+    if (nw) *nw = static_cast<int>(w.size());
+    if (wout)
+      for (int k = 0; k < cap && k < static_cast<int>(w.size()); ++k) wout[k] = w[k];
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// core.cpp -- device context and caching allocator.
+#include "core.hpp"
+#include <rocsolver/rocsolver.h>
+#include "kernels.hpp"
+
+namespace sadmrg {
+
+namespace {
+thread_local std::string t_last_error;
+}
+
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+const char* last_error() { return t_last_error.c_str(); }
+
+void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+DevBuf::DevBuf(Context* c, size_t bytes) : ctx_(c), bytes_(bytes) {
+  if (bytes > 0) ptr_ = c->raw_alloc(bytes, &cap_);
+}
+
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::release() {
+  if (ptr_ && ctx_) ctx_->raw_free(ptr_, cap_);
+  ptr_ = nullptr;
+  bytes_ = cap_ = 0;
+}
+
+Context::Context(int device) : device_(device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device)
+    fail(kErrNoDevice, "no HIP device " + std::to_string(device) + " (device count " + std::to_string(n) + ")");
+  SADMRG_HIP(hipSetDevice(device));
+  // default priority stream (non-default priorities are avoided on purpose)
+  SADMRG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  partial_ = DevBuf(this, sizeof(double) * 2 * kPartialGrid * 64);
+  dscal_ = DevBuf(this, sizeof(double) * kMaxDscal);
+}
+
+Context::~Context() {
+  if (stream_) hipStreamSynchronize(stream_);
+  if (solver_) rocblas_destroy_handle(static_cast<rocblas_handle>(solver_));
+  partial_.release();
+  dscal_.release();
+  for (auto& kv : free_) hipFree(kv.second);
+  free_.clear();
+  if (stream_) hipStreamDestroy(stream_);
+}
+
+void Context::sync() { SADMRG_HIP(hipStreamSynchronize(stream_)); }
+
+static size_t round_cap(size_t bytes) {
+  const size_t a = 256;
+  size_t c = (bytes + a - 1) / a * a;
+  if (c >= (size_t(1) << 20)) {  // above 1 MiB: round to 1/8 power-of-two steps
+    size_t p = 1;
+    while (p < c) p <<= 1;
+    const size_t step = p / 8;
+    c = (c + step - 1) / step * step;
+  }
+  return c;
+}
+
+void* Context::raw_alloc(size_t bytes, size_t* cap) {
+  const size_t c = round_cap(bytes);
+  auto it = free_.lower_bound(c);
+  if (it != free_.end() && it->first <= c + c / 4) {
+    void* p = it->second;
+    *cap = it->first;
+    cached_ -= it->first;
+    in_use_ += it->first;
+    free_.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, c);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim();
+    SADMRG_HIP(hipMalloc(&p, c));
+  }
+  *cap = c;
+  in_use_ += c;
+  return p;
+}
+
+void Context::raw_free(void* p, size_t cap) {
+  in_use_ -= cap;
+  cached_ += cap;
+  free_.emplace(cap, p);
+}
+
+void Context::trim() {
+  SADMRG_HIP(hipStreamSynchronize(stream_));
+  for (auto& kv : free_) hipFree(kv.second);
+  free_.clear();
+  cached_ = 0;
+}
+
+void* Context::solver() {
+  if (!solver_) {
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) fail(kErrNumerics, "rocblas_create_handle failed");
+    rocblas_set_stream(h, stream_);
+    solver_ = h;
+  }
+  return solver_;
+}
+
+}  // namespace sadmrg

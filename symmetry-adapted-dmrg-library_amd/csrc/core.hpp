@@ -1,0 +1,128 @@
+// core.hpp -- device context, caching allocator, error handling, scalar traits.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <complex>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sadmrg {
+
+using dcmplex = std::complex<double>;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status : int {
+  kOk = 0,
+  kErrInvalid = 1,     // bad arguments / inconsistent bonds (the reference exits here)
+  kErrHip = 2,         // HIP runtime failure
+  kErrUnsupported = 3, // a case the reference supports that this engine rejects
+  kErrNumerics = 4,    // SVD / eigensolver failure
+  kErrNoDevice = 5,
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+inline void require(bool ok, const std::string& msg) {
+  if (!ok) fail(kErrInvalid, msg);
+}
+
+#define SADMRG_HIP(x)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      ::sadmrg::fail(::sadmrg::kErrHip, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class S> struct Traits;
+template <> struct Traits<double> {
+  static constexpr bool is_complex = false;
+  static constexpr int width = 1;  // doubles per element
+};
+template <> struct Traits<dcmplex> {
+  static constexpr bool is_complex = true;
+  static constexpr int width = 2;
+};
+
+class Context;
+
+// Owning device allocation returned to the context cache on destruction.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(Context* c, size_t bytes);
+  ~DevBuf();
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept { swap(o); }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); swap(o); }
+    return *this;
+  }
+  void release();
+  void* get() const { return ptr_; }
+  template <class T> T* as() const { return static_cast<T*>(ptr_); }
+  size_t bytes() const { return bytes_; }
+  Context* ctx() const { return ctx_; }
+
+ private:
+  void swap(DevBuf& o) noexcept {
+    std::swap(ctx_, o.ctx_);
+    std::swap(ptr_, o.ptr_);
+    std::swap(bytes_, o.bytes_);
+    std::swap(cap_, o.cap_);
+  }
+  Context* ctx_ = nullptr;
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+  size_t cap_ = 0;
+};
+
+// One device, one in-order stream.  All work is enqueued on `stream`, so a
+// cached allocation can be recycled as soon as the host releases it: later
+// users are ordered after earlier ones by the stream.
+class Context {
+ public:
+  explicit Context(int device);
+  ~Context();
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  void sync();
+
+  void* raw_alloc(size_t bytes, size_t* cap);
+  void raw_free(void* p, size_t cap);
+  void trim();
+
+  // scratch for reductions (2 * kPartialGrid * 64 doubles) and device scalars
+  double* partial() const { return partial_.as<double>(); }
+  double* dscal() const { return dscal_.as<double>(); }
+  static constexpr int kMaxDscal = 4096;
+
+  // rocSOLVER handle (created lazily)
+  void* solver();
+
+  size_t bytes_in_use() const { return in_use_; }
+  size_t bytes_cached() const { return cached_; }
+
+ private:
+  int device_;
+  hipStream_t stream_ = nullptr;
+  std::multimap<size_t, void*> free_;
+  size_t in_use_ = 0, cached_ = 0;
+  DevBuf partial_, dscal_;
+  void* solver_ = nullptr;
+};
+
+void set_last_error(const std::string& msg);
+const char* last_error();
+
+}  // namespace sadmrg

@@ -1,0 +1,383 @@
+// dmrg.cpp -- see dmrg.hpp.
+#include "dmrg.hpp"
+#include <cmath>
+#include <iostream>
+
+namespace sadmrg {
+
+namespace {
+using Clock = std::chrono::steady_clock;
+double since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+}  // namespace
+
+Dmrg::Dmrg(Context* ctx, SymKind sym, bool cplx, int xx, int yy, int bondd, int physpn, int totspin)
+    : ctx_(ctx), sym_(sym), cplx_(cplx), lx_(xx), ly_(yy), ns_(xx * yy), physpn_(physpn),
+      totspin_(totspin), max_dcut_(bondd) {
+  require(ns_ >= 4, "DMRG: at least 4 sites are required");
+  require(physpn == 1, "DMRG: only spin-1/2 sites (physpn = 1) are supported");
+  hmap_.assign(ns_ * ns_, 0);
+  fll_.assign(ns_ * ns_, 0);
+  frr_.assign(ns_ * ns_, 0);
+  // snake ordering, nearest neighbours, periodic in both directions
+  // (dmrg_inst.cpp:117-144)
+  for (int k = 0; k < 2; ++k)
+    for (int i = 0; i < ns_; ++i) {
+      const int x1 = i / ly_;
+      const int y1 = (x1 % 2 == 0) ? i % ly_ : ly_ - 1 - (i % ly_);
+      int x2, y2;
+      if (k == 0) { x2 = (x1 + 1) % lx_; y2 = y1; }
+      else { x2 = x1; y2 = (y1 + 1) % ly_; }
+      const int j = (x2 % 2 == 0) ? x2 * ly_ + y2 : x2 * ly_ + (ly_ - 1 - y2);
+      if (i != j) {
+        hmap_[i * ns_ + j] = 1;
+        hmap_[j * ns_ + i] = 1;
+      }
+    }
+  coup_.assign(ns_ + 1, 0.0);
+  coup_[1] = 1.0;
+  set_couplings(hmap_, coup_);
+  uu_.resize(ns_);
+  hh_.resize(ns_);
+  opr_.resize(ns_);
+  for (auto& r : opr_) r.resize(ns_);
+  if (sym_ == SymKind::SU2) cgc_ = std::make_unique<CgcTables>(physpn_);
+  initialize_local_operators();
+}
+
+void Dmrg::set_couplings(const std::vector<int>& hmap, const std::vector<double>& coup) {
+  require(static_cast<int>(hmap.size()) == ns_ * ns_, "set_couplings: hmap must be ns x ns");
+  for (int v : hmap) require(v >= 0 && v < static_cast<int>(coup.size()), "set_couplings: coupling index out of range");
+  hmap_ = hmap;
+  coup_ = coup;
+  for (int i = 0; i < ns_; ++i) {
+    for (int j = 0; j < ns_; ++j) { fll_[i * ns_ + j] = 0; frr_[i * ns_ + j] = 0; }
+    for (int j = 0; j < i; ++j)
+      for (int k = i + 1; k < ns_; ++k)
+        if (hm(j, k) != 0) fll_[i * ns_ + j] = 1;
+    for (int j = i + 1; j < ns_; ++j)
+      for (int k = 0; k < i; ++k)
+        if (hm(j, k) != 0) frr_[i * ns_ + j] = 1;
+  }
+}
+
+// dmrg_inst.cpp:222-260
+void Dmrg::initialize_local_operators() {
+  op_make_spinor_start(sigma_[0], ctx_, cplx_, sym_, physpn_);
+  op_make_spinor_start(sigma_[1], ctx_, cplx_, sym_, physpn_);
+  if (sym_ == SymKind::SU2) {
+    op_scale(sigma_[0], std::sqrt(3.0) / 2.0);
+    op_scale(sigma_[1], -std::sqrt(3.0) / 2.0);
+  } else {
+    const Bond b = sigma_[0].st.bonds[1].inverted();
+    BTensor t1, t2;
+    op_fuse_to_singlet(t1, ctx_, cplx_, sym_, b, b);
+    op_contract(t2, sigma_[1], 1, t1, 1);
+    op_shift(t2, 1, 0);
+    sigma_[1] = std::move(t2);
+  }
+  ctx_->sync();
+}
+
+void Dmrg::boundary_tensors() {
+  if (sym_ == SymKind::SU2) {
+    const Bond b0 = Bond::make(1, {0}, {1}), b1 = Bond::make(1, {physpn_}, {1});
+    const Bond b2 = Bond::make(1, {physpn_}, {1}), b3 = Bond::make(1, {totspin_}, {1});
+    op_fuse(uu_[0], ctx_, cplx_, sym_, b0, b1);
+    op_fuse(uu_[ns_ - 1], ctx_, cplx_, sym_, b2, b3);
+  } else {
+    const Bond b0 = Bond::make(1, {0}, {1}), b1 = Bond::make(1, {-1, 1}, {1, 1});
+    const Bond b2 = Bond::make(1, {-1, 1}, {1, 1}), b3 = Bond::make(-1, {totspin_}, {1});
+    op_fuse(uu_[0], ctx_, cplx_, sym_, b0, b1);
+    op_fuse(uu_[ns_ - 1], ctx_, cplx_, sym_, b2, b3);
+  }
+}
+
+// ---- environment updates ---------------------------------------------------
+// dmrg_inst.cpp:440-511 (truncated basis, exci = 0)
+void Dmrg::prepare_site_operator_from_left(int i, int /*flag*/) {
+  const auto t0 = Clock::now();
+  CgcTables* cg = cgc_.get();
+  if (i == 0) {
+    op_env(opr_[0][0], kOperatorInitial, uu_[0], uu_[0], &sigma_[0], nullptr, 0, cg);
+    hh_[0].clean();
+    cur_.t_env += since(t0);
+    return;
+  }
+  if (!hh_[i - 1].is_null()) op_env(hh_[i], kOverlapTransformation, uu_[i], uu_[i], &hh_[i - 1], nullptr, 0, cg);
+  else hh_[i].clean();
+  for (int k = 0; k < i; ++k)
+    if (fll_[i * ns_ + k] == 1)
+      op_env(opr_[i][k], kOperatorTransformation, uu_[i], uu_[i], &opr_[i - 1][k], nullptr, 0, cg);
+  BTensor op;
+  for (int k = 0; k < i; ++k)
+    if (hm(i, k) > 0 && !opr_[i - 1][k].is_null()) op_axpy(op, coup_[hm(i, k)], 0.0, opr_[i - 1][k]);
+  if (!op.is_null()) {
+    BTensor ht;
+    op_env(ht, kOperatorPairup, uu_[i], uu_[i], &op, &sigma_[1], 0, cg);
+    op_axpy(hh_[i], 1.0, 0.0, ht);
+  }
+  op_env(opr_[i][i], kOperatorInitial, uu_[i], uu_[i], &sigma_[0], nullptr, 0, cg);
+  cur_.t_env += since(t0);
+}
+
+// dmrg_inst.cpp:515-583
+void Dmrg::prepare_site_operator_from_right(int i, int /*flag*/) {
+  const auto t0 = Clock::now();
+  CgcTables* cg = cgc_.get();
+  if (i == ns_ - 1) {
+    op_env(opr_[i][i], kOperatorInitial, uu_[i], uu_[i], &sigma_[1], nullptr, 1, cg);
+    hh_[i].clean();
+    cur_.t_env += since(t0);
+    return;
+  }
+  if (!hh_[i + 1].is_null()) op_env(hh_[i], kOverlapTransformation, uu_[i], uu_[i], &hh_[i + 1], nullptr, 1, cg);
+  else hh_[i].clean();
+  for (int m0 = 0; m0 < ns_ - 1 - i; ++m0) {
+    const int k = i + 1 + m0;
+    if (frr_[i * ns_ + k] == 1)
+      op_env(opr_[i][k], kOperatorTransformation, uu_[i], uu_[i], &opr_[i + 1][k], nullptr, 1, cg);
+  }
+  BTensor op;
+  for (int m = 0; m < ns_ - 1 - i; ++m) {
+    const int k = i + 1 + m;
+    if (hm(i, k) > 0 && !opr_[i + 1][k].is_null()) op_axpy(op, coup_[hm(i, k)], 0.0, opr_[i + 1][k]);
+  }
+  if (!op.is_null()) {
+    BTensor ht;
+    op_env(ht, kOperatorPairup, uu_[i], uu_[i], &op, &sigma_[0], 1, cg);
+    op_axpy(hh_[i], 1.0, 0.0, ht);
+  }
+  op_env(opr_[i][i], kOperatorInitial, uu_[i], uu_[i], &sigma_[1], nullptr, 1, cg);
+  cur_.t_env += since(t0);
+}
+
+// dmrg_inst.cpp:587-639 (enlarged basis: identity fusion of uu[i]'s legs)
+void Dmrg::prepare_site_operator_from_left(int i) {
+  const auto t0 = Clock::now();
+  CgcTables* cg = cgc_.get();
+  BTensor u1;
+  op_fuse(u1, ctx_, cplx_, sym_, uu_[i].st.bonds[0], uu_[i].st.bonds[1]);
+  if (!hh_[i - 1].is_null()) op_env(hh_[i], kOverlapTransformation, u1, u1, &hh_[i - 1], nullptr, 0, cg);
+  else hh_[i].clean();
+  for (int k = 0; k < i; ++k)
+    if (fll_[i * ns_ + k] == 1) op_env(opr_[i][k], kOperatorTransformation, u1, u1, &opr_[i - 1][k], nullptr, 0, cg);
+  BTensor op;
+  for (int k = 0; k < i; ++k)
+    if (hm(i, k) > 0 && !opr_[i - 1][k].is_null()) op_axpy(op, coup_[hm(i, k)], 0.0, opr_[i - 1][k]);
+  if (!op.is_null()) {
+    BTensor ht;
+    op_env(ht, kOperatorPairup, u1, u1, &op, &sigma_[1], 0, cg);
+    op_axpy(hh_[i], 1.0, 0.0, ht);
+  }
+  op_env(opr_[i][i], kOperatorInitial, u1, u1, &sigma_[0], nullptr, 0, cg);
+  cur_.t_env += since(t0);
+}
+
+// dmrg_inst.cpp:643-695
+void Dmrg::prepare_site_operator_from_right(int i) {
+  const auto t0 = Clock::now();
+  CgcTables* cg = cgc_.get();
+  BTensor u1;
+  op_fuse(u1, ctx_, cplx_, sym_, uu_[i].st.bonds[0], uu_[i].st.bonds[1]);
+  if (!hh_[i + 1].is_null()) op_env(hh_[i], kOverlapTransformation, u1, u1, &hh_[i + 1], nullptr, 1, cg);
+  else hh_[i].clean();
+  for (int m0 = 0; m0 < ns_ - 1 - i; ++m0) {
+    const int k = i + 1 + m0;
+    if (frr_[i * ns_ + k] == 1) op_env(opr_[i][k], kOperatorTransformation, u1, u1, &opr_[i + 1][k], nullptr, 1, cg);
+  }
+  BTensor op;
+  for (int m = 0; m < ns_ - 1 - i; ++m) {
+    const int k = i + 1 + m;
+    if (hm(i, k) > 0 && !opr_[i + 1][k].is_null()) op_axpy(op, coup_[hm(i, k)], 0.0, opr_[i + 1][k]);
+  }
+  if (!op.is_null()) {
+    BTensor ht;
+    op_env(ht, kOperatorPairup, u1, u1, &op, &sigma_[0], 1, cg);
+    op_axpy(hh_[i], 1.0, 0.0, ht);
+  }
+  op_env(opr_[i][i], kOperatorInitial, u1, u1, &sigma_[1], nullptr, 1, cg);
+  cur_.t_env += since(t0);
+}
+
+// ---- wave-function preparation ---------------------------------------------
+// dmrg_inst.cpp:375-408
+void Dmrg::prepare_input_vector_initial(int il, int ir, BTensor& vec) {
+  BTensor t1, t2, left, rght, tmp, tmp3;
+  op_fuse(t1, ctx_, cplx_, sym_, uu_[il].st.bonds[0], uu_[il].st.bonds[1]);
+  op_fuse(t2, ctx_, cplx_, sym_, uu_[ir].st.bonds[0], uu_[ir].st.bonds[1]);
+  op_env(left, kOverlapInitial, uu_[il], t1, nullptr, nullptr, 0, cgc_.get());
+  op_env(rght, kOverlapInitial, uu_[ir], t2, nullptr, nullptr, 1, cgc_.get());
+  op_fuse_to_singlet(tmp, ctx_, cplx_, sym_, left.st.bonds[0].inverted(), rght.st.bonds[0].inverted());
+  op_contract(tmp3, left, 0, tmp, 0);
+  op_contract(vec, tmp3, 1, rght, 0);
+  op_makeup_input_vector(vec);
+}
+
+// dmrg_inst.cpp:412-437
+void Dmrg::prepare_input_vector(int il, int ir, BTensor& vec) {
+  const auto t0 = Clock::now();
+  BTensor t1, t2, left, rght;
+  op_fuse(t1, ctx_, cplx_, sym_, uu_[il].st.bonds[0], uu_[il].st.bonds[1]);
+  op_fuse(t2, ctx_, cplx_, sym_, uu_[ir].st.bonds[0], uu_[ir].st.bonds[1]);
+  op_env(left, kOverlapInitial, uu_[il], t1, nullptr, nullptr, 0, cgc_.get());
+  op_env(rght, kOverlapInitial, uu_[ir], t2, nullptr, nullptr, 1, cgc_.get());
+  if (sym_ == SymKind::SU2) op_take_conjugate(left, 0);
+  op_contract(vec, left, 0, rght, 0);
+  op_makeup_input_vector(vec);
+  cur_.t_env += since(t0);
+}
+
+void Dmrg::wavefunc_transformation(int i, int flag) { op_vectran(uu_[i], flag != 0); }
+
+// ---- H_eff and Lanczos ----------------------------------------------------------
+// Bond terms of DMRG::hamiltonian_vector_multiplication_idmrg (dmrg_inst.cpp:310-351):
+// on the left half each left operator opr[il][j] pairs with the coupling-
+// weighted sum of right operators; on the right half the roles swap.
+std::unique_ptr<HeffPlan> Dmrg::make_heff(int il, int ir, const BTensor& psi,
+                                          std::vector<std::unique_ptr<BTensor>>* keep) {
+  std::vector<std::unique_ptr<BTensor>> local;
+  auto& hold = keep ? *keep : local;
+  std::vector<const BTensor*> o1, o2;
+  const bool left_half = il < ns_ / 2;
+  const int m1 = left_half ? il + 1 : ns_ - ir;
+  const int m4 = left_half ? ns_ - ir : il + 1;
+  for (int m0 = 0; m0 < m1; ++m0) {
+    auto op = std::make_unique<BTensor>();
+    if (left_half) {
+      const int j = m0;
+      for (int m = 0; m < m4; ++m) {
+        const int k = ir + m;
+        if (hm(j, k) > 0 && !opr_[ir][k].is_null()) op_axpy(*op, coup_[hm(j, k)], 0.0, opr_[ir][k]);
+      }
+      if (op->is_null()) continue;
+      o1.push_back(&opr_[il][j]);
+      o2.push_back(op.get());
+    } else {
+      const int k = ir + m0;
+      for (int m = 0; m < m4; ++m) {
+        const int j = m;
+        if (hm(j, k) > 0 && !opr_[il][j].is_null()) op_axpy(*op, coup_[hm(j, k)], 0.0, opr_[il][j]);
+      }
+      if (op->is_null()) continue;
+      o1.push_back(op.get());
+      o2.push_back(&opr_[ir][k]);
+    }
+    hold.push_back(std::move(op));
+  }
+  return std::make_unique<HeffPlan>(ctx_, cgc_.get(), hh_[il], hh_[ir], o1, o2, psi);
+}
+
+// dmrg_inst.cpp:264-279
+double Dmrg::lanczos_solve(int il, int ir, BTensor& vec) {
+  const auto t0 = Clock::now();
+  int64_t ndata = 0;
+  for (int b = 0; b < vec.nten(); ++b)
+    if (vec.st.allowed(b)) ndata += vec.block_size(b);
+  int mlanc = 40;
+  if (mlanc > ndata) mlanc = static_cast<int>(ndata) + 1;
+  if (mlanc < 2) mlanc = 2;
+  for (int b = 0; b < vec.nten(); ++b)
+    require(!vec.st.allowed(b) || vec.present(b), "lanczos: input vector must hold every allowed block");
+  auto plan = make_heff(il, ir, vec, nullptr);
+  LanczosOpts o = lopt_;
+  o.mlanc = mlanc;
+  const LanczosOut r = ::sadmrg::lanczos_solve(*plan, vec.dptr(), o);
+  ctx_->sync();
+  gs_enr_ = r.energy;
+  cur_.il = il;
+  cur_.ir = ir;
+  cur_.energy = r.energy;
+  cur_.matvec64 = r.matvec64;
+  cur_.matvec32 = r.matvec32;
+  cur_.heff_flops = plan->stats().flops;
+  cur_.t_lanczos += since(t0);
+  return r.energy;
+}
+
+void Dmrg::svd(const BTensor& vec, int a, int b, double p1, double p2) {
+  const auto t0 = Clock::now();
+  op_svd(vec, uu_[a], p1, uu_[b], p2, max_dcut_, wout_);
+  const int n = static_cast<int>(wout_.size());
+  const int dcut = n > max_dcut_ ? max_dcut_ : n;
+  double te = 0;
+  for (int k = dcut; k < n; ++k) te += wout_[k] * wout_[k];
+  cur_.trunc_err = te;
+  cur_.kept = dcut;
+  cur_.t_svd += since(t0);
+  rec_.push_back(cur_);
+  cur_ = StepRecord{};
+}
+
+// ---- drivers ------------------------------------------------------------------
+// dmrg_inst.cpp:699-804
+void Dmrg::do_idmrg() {
+  boundary_tensors();
+  Bond pb1, pb2;
+  if (sym_ == SymKind::SU2) { pb1 = Bond::make(1, {physpn_}, {1}); pb2 = pb1; }
+  else { pb1 = Bond::make(1, {-1, 1}, {1, 1}); pb2 = pb1; }
+  for (int i = 0; i <= ns_ / 2 - 2; ++i) {
+    prepare_site_operator_from_left(i, 0);
+    prepare_site_operator_from_right(ns_ - 1 - i, i < ns_ / 2 - 2 ? 0 : 1);
+    const Bond b0 = uu_[i].st.bonds[2].inverted();
+    const Bond b1 = uu_[ns_ - 1 - i].st.bonds[2].inverted();
+    op_fuse(uu_[i + 1], ctx_, cplx_, sym_, b0, pb1);
+    op_fuse(uu_[ns_ - 2 - i], ctx_, cplx_, sym_, pb2, b1);
+    if (((i + 2) * 2 * physpn_ >= totspin_ && i >= 1) || i == ns_ / 2 - 2) {
+      BTensor vec;
+      prepare_input_vector_initial(i + 1, ns_ - 2 - i, vec);
+      prepare_site_operator_from_left(i + 1);
+      prepare_site_operator_from_right(ns_ - 2 - i);
+      op_random_init(vec);
+      op_normalize(vec);
+      lanczos_solve(i + 1, ns_ - 2 - i, vec);
+      if (i == ns_ / 2 - 2) svd(vec, i + 1, ns_ - 2 - i, 1.0, 0.0);
+      else svd(vec, i + 1, ns_ - 2 - i, 0.0, 0.0);
+    }
+  }
+}
+
+// dmrg_inst.cpp:808-854
+void Dmrg::prepare_sweep() {
+  boundary_tensors();
+  for (int i = 0; i < ns_ / 2 - 1; ++i) {
+    prepare_site_operator_from_left(i, 0);
+    prepare_site_operator_from_right(ns_ - 1 - i, i < ns_ / 2 - 2 ? 0 : 1);
+  }
+}
+
+// dmrg_inst.cpp:874-943 (checkpoint files are not written)
+void Dmrg::sweep() {
+  BTensor vec;
+  for (int i = ns_ / 2; i >= nfree_ + 2; --i) {
+    wavefunc_transformation(i - 1, 1);
+    prepare_site_operator_from_right(i, 1);
+    prepare_input_vector(i - 2, i - 1, vec);
+    prepare_site_operator_from_left(i - 2);
+    prepare_site_operator_from_right(i - 1);
+    lanczos_solve(i - 2, i - 1, vec);
+    if (i > nfree_ + 2) svd(vec, i - 2, i - 1, 1.0, 0.0);
+    else svd(vec, i - 2, i - 1, 0.0, 1.0);
+  }
+  for (int i = nfree_; i < ns_ - nfree_ - 2; ++i) {
+    wavefunc_transformation(i + 1, 0);
+    prepare_site_operator_from_left(i, 1);
+    prepare_input_vector(i + 1, i + 2, vec);
+    prepare_site_operator_from_left(i + 1);
+    prepare_site_operator_from_right(i + 2);
+    lanczos_solve(i + 1, i + 2, vec);
+    if (i < ns_ - nfree_ - 3) svd(vec, i + 1, i + 2, 0.0, 1.0);
+    else svd(vec, i + 1, i + 2, 1.0, 0.0);
+  }
+  for (int i = ns_ - nfree_ - 1; i >= ns_ / 2 + 1; --i) {
+    wavefunc_transformation(i - 1, 1);
+    prepare_site_operator_from_right(i, 1);
+    prepare_input_vector(i - 2, i - 1, vec);
+    prepare_site_operator_from_left(i - 2);
+    prepare_site_operator_from_right(i - 1);
+    lanczos_solve(i - 2, i - 1, vec);
+    svd(vec, i - 2, i - 1, 1.0, 0.0);
+  }
+  ctx_->sync();
+}
+
+}  // namespace sadmrg

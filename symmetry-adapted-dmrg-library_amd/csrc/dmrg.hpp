@@ -1,0 +1,89 @@
+// dmrg.hpp -- two-site DMRG driver on the device engine.
+//
+// Keeps the reference's DMRG<Scalar,Sym> state and step order (dmrg.hpp:29-133,
+// src/dmrg_inst.cpp): lattice maps hmap/coup/fll/frr, MPS tensors uu[ns],
+// block Hamiltonians hh[ns], open-bond operators opr[ns][ns], iDMRG warm-up,
+// three-pass finite sweep.  Every tensor is device resident; each step builds
+// one H_eff plan whose Lanczos solve runs entirely on the device.
+#pragma once
+#include <chrono>
+#include <memory>
+#include <vector>
+#include "btensor.hpp"
+#include "heff.hpp"
+#include "lanczos.hpp"
+
+namespace sadmrg {
+
+struct StepRecord {
+  int il, ir;
+  double energy;
+  double trunc_err;   // sum of squared discarded singular values
+  int kept;           // multiplets (SU(2)) / states (U(1)) kept
+  int matvec64, matvec32;
+  double t_env, t_lanczos, t_svd;  // seconds
+  double heff_flops;  // algorithmic flops per H_eff application at this step
+};
+
+class Dmrg {
+ public:
+  // xx, yy, bondd, physpn, totspin as DMRG(int,int,int,int,int,int) (dmrg_inst.cpp:63-187)
+  Dmrg(Context* ctx, SymKind sym, bool cplx, int xx, int yy, int bondd, int physpn, int totspin);
+
+  int ns() const { return ns_; }
+  // Replace the lattice map and recompute fll/frr (dmrg_inst.cpp:163-176).
+  void set_couplings(const std::vector<int>& hmap, const std::vector<double>& coup);
+  void set_max_dcut(int d) { max_dcut_ = d; }
+  void set_lanczos(const LanczosOpts& o) { lopt_ = o; }
+  const LanczosOpts& lanczos_opts() const { return lopt_; }
+
+  void do_idmrg();
+  void prepare_sweep();
+  void sweep();
+  double energy() const { return gs_enr_; }
+
+  // step-level entry points (private in the reference; exposed for parity)
+  void wavefunc_transformation(int i, int flag);
+  void prepare_site_operator_from_left(int i, int flag);
+  void prepare_site_operator_from_right(int i, int flag);
+  void prepare_site_operator_from_left(int i);
+  void prepare_site_operator_from_right(int i);
+  void prepare_input_vector(int il, int ir, BTensor& vec);
+  void prepare_input_vector_initial(int il, int ir, BTensor& vec);
+  std::unique_ptr<HeffPlan> make_heff(int il, int ir, const BTensor& psi_layout,
+                                      std::vector<std::unique_ptr<BTensor>>* keep);
+  double lanczos_solve(int il, int ir, BTensor& vec);
+  void svd(const BTensor& vec, int a, int b, double p1, double p2);
+
+  BTensor& uu(int i) { return uu_[i]; }
+  BTensor& hh(int i) { return hh_[i]; }
+  BTensor& opr(int i, int j) { return opr_[i][j]; }
+  const std::vector<double>& last_wout() const { return wout_; }
+  std::vector<StepRecord>& records() { return rec_; }
+  CgcTables* cgc() { return cgc_.get(); }
+  const std::vector<int>& hmap() const { return hmap_; }
+  const std::vector<double>& coup() const { return coup_; }
+
+ private:
+  void initialize_local_operators();
+  void boundary_tensors();
+  int hm(int i, int j) const { return hmap_[i * ns_ + j]; }
+
+  Context* ctx_;
+  SymKind sym_;
+  bool cplx_;
+  int lx_, ly_, ns_, nfree_ = 2, physpn_, totspin_, max_dcut_;
+  std::vector<int> hmap_, fll_, frr_;
+  std::vector<double> coup_;
+  std::vector<BTensor> uu_, hh_;
+  std::vector<std::vector<BTensor>> opr_;
+  BTensor sigma_[2];
+  std::unique_ptr<CgcTables> cgc_;
+  LanczosOpts lopt_;
+  double gs_enr_ = 0;
+  std::vector<double> wout_;
+  std::vector<StepRecord> rec_;
+  StepRecord cur_{};
+};
+
+}  // namespace sadmrg

@@ -1,0 +1,243 @@
+// heff.cpp -- see heff.hpp.
+#include "heff.hpp"
+#include <cmath>
+#include <cstring>
+#include <map>
+#include "kernels.hpp"
+
+namespace sadmrg {
+
+namespace {
+enum { kArIn = 0, kArOut = 1, kArScratch = 2, kArPack = 3 };
+}
+
+HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTensor& hr,
+                   const std::vector<const BTensor*>& o1, const std::vector<const BTensor*>& o2,
+                   const BTensor& psi)
+    : ctx_(ctx), cplx_(psi.cplx) {
+  require(o1.size() == o2.size(), "H_eff: term operator lists differ in length");
+  require(!psi.is_null() && psi.nbond() == 2, "H_eff: psi must be a 2-leg tensor");
+  const SymKind sym = psi.sym();
+  const bool su2 = sym == SymKind::SU2;
+  const int w = psi.w();
+  layout_.ctx = ctx;
+  layout_.cplx = psi.cplx;
+  layout_.st = psi.st;
+  layout_.off = psi.off;
+  layout_.tag = psi.tag;
+  layout_.nelem = psi.nelem;
+  n_ = psi.nelem;
+  const Struct& ps = psi.st;
+  const int ntp = ps.nten();
+
+  // ---- pack operator blocks -------------------------------------------------
+  std::vector<const BTensor*> tens;
+  auto tid_of = [&](const BTensor* t) {
+    for (size_t i = 0; i < tens.size(); ++i)
+      if (tens[i] == t) return static_cast<int>(i);
+    tens.push_back(t);
+    return static_cast<int>(tens.size()) - 1;
+  };
+  if (!hl.is_null()) tid_of(&hl);
+  if (!hr.is_null()) tid_of(&hr);
+  for (size_t t = 0; t < o1.size(); ++t) {
+    if (o1[t] && !o1[t]->is_null()) tid_of(o1[t]);
+    if (o2[t] && !o2[t]->is_null()) tid_of(o2[t]);
+  }
+  std::vector<int64_t> pbase(tens.size(), 0);
+  for (size_t i = 0; i < tens.size(); ++i) {
+    require(tens[i]->cplx == cplx_ && tens[i]->sym() == sym, "H_eff: operator scalar/symmetry mismatch");
+    pbase[i] = pack_elems_;
+    pack_elems_ += tens[i]->nelem;
+  }
+  pack64_ = DevBuf(ctx, std::max<int64_t>(pack_elems_, 1) * 8 * w);
+  for (size_t i = 0; i < tens.size(); ++i)
+    if (tens[i]->nelem)
+      SADMRG_HIP(hipMemcpyAsync(pack64_.as<double>() + pbase[i] * w, tens[i]->dptr(), tens[i]->nelem * 8 * w,
+                                hipMemcpyDeviceToDevice, ctx->stream()));
+  auto pk = [&](const BTensor* t, int b) -> Operand {
+    Operand o;
+    o.arena = kArPack;
+    o.off = pbase[tid_of(t)] + t->off[b];
+    return o;
+  };
+  auto pin = [&](int b) -> Operand {
+    Operand o;
+    o.arena = kArIn;
+    o.off = psi.off[b];
+    return o;
+  };
+
+  // ---- stage-2 problems: one per psi block ----------------------------------
+  std::vector<int> prob_of(ntp, -1);
+  int lab[4], d[4];
+  for (int b = 0; b < ntp; ++b) {
+    if (!psi.present(b)) continue;
+    ps.block_dims(b, d);
+    prob_of[b] = g2_.add_problem(kArOut, psi.off[b], d[0], d[0], d[1], false);
+    stats_.nsec_in++;
+  }
+  std::vector<int> contributions(ntp, 0);
+
+  // diagonal: contract(hh_L, 0, psi, 0) and contract(psi, 1, hh_R, 0)
+  if (!hl.is_null()) {
+    require(hl.nbond() == 2 && hl.st.bonds[0] == ps.bonds[0].inverted() && hl.st.bonds[1] == ps.bonds[0],
+            "H_eff: left block Hamiltonian bonds do not match psi");
+    for (int b = 0; b < ntp; ++b) {
+      if (!psi.present(b)) continue;
+      ps.labels(b, lab);
+      const int l1[2] = {lab[0], lab[0]};
+      if (!hl.st.allowed_labels(l1)) continue;
+      const int hb = hl.st.index_of_labels(l1);
+      if (hb < 0 || !hl.present(hb)) continue;
+      ps.block_dims(b, d);
+      Operand A = pk(&hl, hb), B = pin(b);
+      A.ld = d[0];
+      A.trans = true;
+      B.ld = d[0];
+      g2_.add_seg(prob_of[b], A, B, d[0], 1.0);
+      contributions[b]++;
+    }
+  }
+  if (!hr.is_null()) {
+    require(hr.nbond() == 2 && hr.st.bonds[0] == ps.bonds[1].inverted() && hr.st.bonds[1] == ps.bonds[1],
+            "H_eff: right block Hamiltonian bonds do not match psi");
+    for (int b = 0; b < ntp; ++b) {
+      if (!psi.present(b)) continue;
+      ps.labels(b, lab);
+      const int l2[2] = {lab[1], lab[1]};
+      if (!hr.st.allowed_labels(l2)) continue;
+      const int hb = hr.st.index_of_labels(l2);
+      if (hb < 0 || !hr.present(hb)) continue;
+      ps.block_dims(b, d);
+      Operand A = pin(b), B = pk(&hr, hb);
+      A.ld = d[0];
+      B.ld = d[1];
+      g2_.add_seg(prob_of[b], A, B, d[1], 1.0);
+      contributions[b]++;
+    }
+  }
+
+  // bond terms
+  for (size_t t = 0; t < o1.size(); ++t) {
+    const BTensor* op1 = o1[t];
+    const BTensor* op2 = o2[t];
+    if (!op1 || !op2 || op1->is_null() || op2->is_null()) continue;
+    const Bond ch2 = su2 ? op2->st.bonds[1] : op2->st.bonds[1].inverted();
+    require(op1->st.bonds[0] == ps.bonds[0].inverted() && op2->st.bonds[0] == ps.bonds[1].inverted() &&
+                op1->st.bonds[1] == ch2 && op1->st.bonds[2] == ps.bonds[0] && op2->st.bonds[2] == ps.bonds[1],
+            "H_eff: bond-term operator bonds do not match psi");
+    const int nt1 = op1->nten(), nt3 = op2->nten();
+    int a1[3], a3[3], d1[3], d3[3];
+    for (int i = 0; i < nt1 * ntp; ++i) {
+      const int j = i % nt1, k = (i / nt1) % ntp;
+      if (!op1->st.allowed(j) || !ps.allowed(k)) continue;
+      op1->st.labels(j, a1);
+      ps.labels(k, lab);
+      if (a1[0] != lab[0]) continue;
+      if (!op1->present(j) || !psi.present(k)) continue;
+      op1->st.block_dims(j, d1);
+      ps.block_dims(k, d);
+      int64_t xoff = -1;
+      int xm = 0;
+      for (int l = 0; l < nt3; ++l) {
+        if (!op2->st.allowed(l)) continue;
+        op2->st.labels(l, a3);
+        if (a3[0] != lab[1] || a1[1] != a3[1]) continue;
+        const int ol[2] = {a1[2], a3[2]};
+        if (!ps.allowed_labels(ol)) continue;
+        const int m = ps.index_of_labels(ol);
+        if (m < 0 || !op2->present(l)) continue;
+        double fac = 1.0;
+        if (su2) {
+          fac = cgc->hamilt_vec(a1[0], a1[1], a1[2]);
+          if (std::fabs(fac) < 1e-12) continue;
+        }
+        require(psi.present(m), "H_eff: output block missing from the psi layout");
+        if (xoff < 0) {
+          // X = psi_k^T O1_j : (b) x (c * a')
+          xoff = scratch_elems_;
+          xm = d[1];
+          scratch_elems_ += int64_t(d[1]) * d1[1] * d1[2];
+          const int p = g1_.add_problem(kArScratch, xoff, d[1], d[1], d1[1] * d1[2], false);
+          Operand A = pin(k), B = pk(op1, j);
+          A.ld = d[0];
+          A.trans = true;
+          B.ld = d[0];
+          g1_.add_seg(p, A, B, d[0], 1.0);
+        }
+        op2->st.block_dims(l, d3);
+        const int kk = d3[0] * d3[1];
+        require(kk == xm * d1[1], "H_eff: inner dimension mismatch");
+        Operand A, B = pk(op2, l);
+        A.arena = kArScratch;
+        A.off = xoff;
+        A.ld = kk;
+        A.trans = true;
+        B.ld = kk;
+        g2_.add_seg(prob_of[m], A, B, kk, fac);
+        contributions[m]++;
+      }
+    }
+  }
+  out_null_.assign(ntp, 0);
+  for (int b = 0; b < ntp; ++b) out_null_[b] = psi.present(b) && contributions[b] == 0;
+
+  for (GemmBatch* g : {&g1_, &g2_}) {
+    g->limit(kArIn, n_);
+    g->limit(kArOut, n_);
+    g->limit(kArScratch, scratch_elems_);
+    g->limit(kArPack, pack_elems_);
+  }
+  g1_.finalize(ctx, cplx_ ? 0 : -1);
+  g2_.finalize(ctx, cplx_ ? 0 : -1);
+  scratch64_ = DevBuf(ctx, std::max<int64_t>(scratch_elems_, 1) * 8 * w);
+  stats_.n = n_;
+  stats_.flops = g1_.flops(cplx_) + g2_.flops(cplx_);
+  stats_.bytes64 = 8.0 * w * (2.0 * n_ + pack_elems_);
+  stats_.stage1_probs = g1_.nprob();
+  stats_.stage2_probs = g2_.nprob();
+  stats_.stage1_segs = g1_.nseg();
+  stats_.stage2_segs = g2_.nseg();
+  stats_.tiles1 = g1_.ntiles();
+  stats_.tiles2 = g2_.ntiles();
+  stats_.tile_cfg1 = g1_.tile_cfg();
+  stats_.tile_cfg2 = g2_.tile_cfg();
+  ctx->sync();
+}
+
+double HeffPlan::stage_flops(int stage) const {
+  if (stage == 1) return g1_.flops(cplx_);
+  if (stage == 2) return g2_.flops(cplx_);
+  return g1_.flops(cplx_) + g2_.flops(cplx_);
+}
+
+void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage) {
+  ArenaTable at;
+  std::memset(&at, 0, sizeof(at));
+  if (prec == Prec::FP64 || cplx_) {
+    require(prec == Prec::FP64, "H_eff: FP32 path is real-only");
+    at.base[kArIn] = d_in;
+    at.base[kArOut] = d_out;
+    at.base[kArScratch] = scratch64_.get();
+    at.base[kArPack] = pack64_.get();
+    if (stage != 2) g1_.run(ctx_, cplx_ ? GemmPrec::Z64 : GemmPrec::F64, at);
+    if (stage != 1) g2_.run(ctx_, cplx_ ? GemmPrec::Z64 : GemmPrec::F64, at);
+    return;
+  }
+  if (!pack32_.get()) {
+    pack32_ = DevBuf(ctx_, std::max<int64_t>(pack_elems_, 1) * 4);
+    scratch32_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 4);
+    in32_ = DevBuf(ctx_, std::max<int64_t>(n_, 1) * 4);
+    SADMRG_HIP(launch_f64_to_f32(pack_elems_, pack64_.as<double>(), pack32_.as<float>(), ctx_->stream()));
+  }
+  if (stage != 2) SADMRG_HIP(launch_f64_to_f32(n_, d_in, in32_.as<float>(), ctx_->stream()));
+  at.base[kArIn] = in32_.get();
+  at.base[kArOut] = d_out;
+  at.base[kArScratch] = scratch32_.get();
+  at.base[kArPack] = pack32_.get();
+  if (stage != 2) g1_.run(ctx_, GemmPrec::F32, at);
+  if (stage != 1) g2_.run(ctx_, GemmPrec::F32_OUT64, at);
+}
+
+}  // namespace sadmrg

@@ -1,0 +1,67 @@
+// heff.hpp -- fused effective-Hamiltonian matvec plan.
+//
+// H_eff psi = H_L^T psi + psi H_R + sum_t fac * O1_t^T psi O2_t
+// (DMRG::hamiltonian_vector_multiplication_idmrg, dmrg_inst.cpp:283-371, with
+// every bond term t evaluated by the block loops of
+// BlockTensor::hamiltonian_vector_multiplication, block_tensor_inst.cpp:2170-2318).
+//
+// All sector GEMMs of one matvec run as two grouped launches:
+//   stage 1: one problem per (term, op1 block, psi block):  X = psi_s^T O1_t
+//   stage 2: one problem per output block s', whose K segments are
+//            H_L^T psi, psi H_R and every (term, s -> s') product X^T O2_t,
+//            so the whole right-hand side of an output block accumulates in
+//            MFMA registers of the workgroup that owns the tile.
+// Operator blocks are packed once per plan into one device arena (and one
+// FP32 copy for the mixed-precision path); the block tables are built once
+// and reused by every Lanczos iteration.
+#pragma once
+#include <memory>
+#include <vector>
+#include "btensor.hpp"
+
+namespace sadmrg {
+
+enum class Prec : int { FP64 = 0, FP32 = 1 };
+
+struct HeffStats {
+  double flops = 0;        // algorithmic flops per application
+  double bytes64 = 0;      // algorithmic HBM bytes per application (FP64 operands)
+  int64_t n = 0;           // psi elements
+  int nsec_in = 0;         // psi blocks
+  int stage1_probs = 0, stage2_probs = 0;
+  int64_t stage1_segs = 0, stage2_segs = 0;
+  int tiles1 = 0, tiles2 = 0;
+  int tile_cfg1 = 0, tile_cfg2 = 0;
+};
+
+class HeffPlan {
+ public:
+  // hl / hr may be null tensors.  o1[t] / o2[t] are the bond-term operators
+  // (one of each pair is usually a coupling-weighted sum, built by the caller).
+  HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTensor& hr, const std::vector<const BTensor*>& o1,
+           const std::vector<const BTensor*>& o2, const BTensor& psi_layout);
+
+  // d_in / d_out: device arrays in the layout of psi_layout (n elements).
+  void apply(const double* d_in, double* d_out, Prec prec, int stage = 0);
+  double stage_flops(int stage) const;
+  const HeffStats& stats() const { return stats_; }
+  Context* ctx() const { return ctx_; }
+  const BTensor& layout() const { return layout_; }
+  bool complex() const { return cplx_; }
+  int64_t n() const { return n_; }
+  // output blocks that receive no contribution (null in the reference)
+  const std::vector<uint8_t>& out_null() const { return out_null_; }
+
+ private:
+  Context* ctx_;
+  bool cplx_;
+  int64_t n_ = 0;
+  BTensor layout_;  // structure / offsets only (no data)
+  DevBuf pack64_, pack32_, scratch64_, scratch32_, in32_;
+  int64_t pack_elems_ = 0, scratch_elems_ = 0;
+  GemmBatch g1_, g2_;
+  HeffStats stats_;
+  std::vector<uint8_t> out_null_;
+};
+
+}  // namespace sadmrg

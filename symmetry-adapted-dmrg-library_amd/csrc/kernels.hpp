@@ -1,0 +1,43 @@
+// kernels.hpp -- host-callable launchers for the kernels in kernels.hip.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <cstdint>
+#include "grouped_gemm.hpp"
+
+namespace sadmrg {
+
+enum class GemmPrec : int {
+  F64 = 0,        // double operands, FP64 MFMA, double output
+  F32 = 1,        // float operands, FP32 MFMA, float output
+  F32_OUT64 = 2,  // float operands, FP32 MFMA, double output
+  Z64 = 3,        // complex<double> operands (interleaved), 4 FP64 MFMA products
+};
+
+// tile_cfg 0: 64x64x16 tiles, 1: 128x128x16 tiles.
+hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmTile* tiles, const GemmProb* probs,
+                               const GemmSeg* segs, const ArenaTable& arenas, int ntiles, hipStream_t st);
+
+inline int tile_m(int tile_cfg) { return tile_cfg == 1 ? 128 : 64; }
+inline int tile_n(int tile_cfg) { return tile_cfg == 1 ? 128 : 64; }
+
+// out[c] = sign * sum_i conj(X[i,c]) * y[i] for c < ncol (complex pairs, 2*ncol doubles).
+// `partial` must hold 2 * 1024 * ncol doubles.
+hipError_t launch_gemv_t(const double* X, int64_t ldx, int ncol, const double* y, int64_t n, int cplx,
+                         double* partial, double sign, double* out, hipStream_t st);
+// y += sum_c coef[c] * X[:,c]
+hipError_t launch_gemv_n_update(const double* X, int64_t ldx, int ncol, const double* coef, double* y, int64_t n,
+                                int cplx, hipStream_t st);
+hipError_t launch_axpby(int64_t n, double a, double b, const double* x, double* y, hipStream_t st);
+hipError_t launch_zscal(int64_t n, double sr, double si, const double* x, double* y, hipStream_t st);
+hipError_t launch_f64_to_f32(int64_t n, const double* x, float* y, hipStream_t st);
+hipError_t launch_f32_to_f64(int64_t n, const float* x, double* y, hipStream_t st);
+hipError_t launch_fill(int64_t n, double v, double* y, hipStream_t st);
+// dst[i + j*ldd] (+)= scale * colscale[j] * src[i*s_row + j*s_col]; elements of w doubles.
+hipError_t launch_copy2d(int rows, int cols, const double* src, int64_t s_row, int64_t s_col, double* dst,
+                         int64_t ldd, int w, double scale, const double* colscale, int accumulate, hipStream_t st);
+// p[t*stride] = v for t < n
+hipError_t launch_strided_fill(int64_t n, double v, double* p, int64_t stride, hipStream_t st);
+
+constexpr int kPartialGrid = 1024;
+
+}  // namespace sadmrg

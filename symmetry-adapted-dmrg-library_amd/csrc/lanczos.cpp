@@ -1,0 +1,276 @@
+// lanczos.cpp -- see lanczos.hpp.
+#include "lanczos.hpp"
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include "kernels.hpp"
+
+namespace sadmrg {
+
+// Implicit QL iteration for a symmetric tridiagonal matrix (the EISPACK tql2
+// scheme: Wilkinson shift, chasing with plane rotations accumulated into z).
+void tridiag_eig(int n, std::vector<double>& d, std::vector<double> e, std::vector<double>& z) {
+  z.assign(static_cast<size_t>(n) * n, 0.0);
+  for (int i = 0; i < n; ++i) z[i + i * n] = 1.0;
+  e.resize(n, 0.0);
+  if (n > 0) e[n - 1] = 0.0;
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    int m;
+    do {
+      for (m = l; m < n - 1; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= eps * dd) break;
+      }
+      if (m != l) {
+        if (++iter > 200) fail(kErrNumerics, "tridiagonal eigensolver did not converge");
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        bool underflow = false;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[i];
+          const double b = c * e[i];
+          r = std::hypot(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+            underflow = true;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          p = s * r;
+          d[i + 1] = g + p;
+          g = c * r - b;
+          for (int k = 0; k < n; ++k) {
+            f = z[k + (i + 1) * n];
+            z[k + (i + 1) * n] = s * z[k + i * n] + c * f;
+            z[k + i * n] = c * z[k + i * n] - s * f;
+          }
+        }
+        if (underflow) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  // ascending order, eigenvectors with a non-negative leading component
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+  std::vector<double> d2(n), z2(static_cast<size_t>(n) * n);
+  for (int j = 0; j < n; ++j) {
+    d2[j] = d[idx[j]];
+    const double sg = z[0 + idx[j] * n] < 0 ? -1.0 : 1.0;
+    for (int k = 0; k < n; ++k) z2[k + j * n] = sg * z[k + idx[j] * n];
+  }
+  d = d2;
+  z = z2;
+}
+
+namespace {
+
+struct Dev {
+  Context* ctx;
+  int64_t n;
+  bool cx;
+  int w() const { return cx ? 2 : 1; }
+
+  dcmplex dot(const double* x, const double* y) {
+    SADMRG_HIP(launch_gemv_t(x, 0, 1, y, n, cx ? 1 : 0, ctx->partial(), 1.0, ctx->dscal(), ctx->stream()));
+    double h[2];
+    SADMRG_HIP(hipMemcpyAsync(h, ctx->dscal(), sizeof(h), hipMemcpyDeviceToHost, ctx->stream()));
+    ctx->sync();
+    return dcmplex(h[0], h[1]);
+  }
+  double nrm(const double* x) { return std::sqrt(std::fabs(dot(x, x).real())); }
+  void scal(double a, double* x) { SADMRG_HIP(launch_axpby(n * w(), a, 0.0, nullptr, x, ctx->stream())); }
+  // y = a*y + b*x (real scalars act componentwise on complex data)
+  void axpby(double a, double b, const double* x, double* y) {
+    SADMRG_HIP(launch_axpby(n * w(), a, b, x, y, ctx->stream()));
+  }
+  void copy(const double* x, double* y) {
+    SADMRG_HIP(hipMemcpyAsync(y, x, n * w() * 8, hipMemcpyDeviceToDevice, ctx->stream()));
+  }
+  // w -= V[:, 0..m) (V^H w), twice (classical Gram-Schmidt with re-orthogonalisation)
+  void orth(const double* V, int m, double* x) {
+    if (m <= 0) return;
+    for (int pass = 0; pass < 2; ++pass) {
+      SADMRG_HIP(launch_gemv_t(V, n, m, x, n, cx ? 1 : 0, ctx->partial(), -1.0, ctx->dscal(), ctx->stream()));
+      SADMRG_HIP(launch_gemv_n_update(V, n, m, ctx->dscal(), x, n, cx ? 1 : 0, ctx->stream()));
+    }
+  }
+  // y = sum_j c_j V[:, j]
+  void combine(const double* V, const std::vector<double>& c, double* y) {
+    const int m = static_cast<int>(c.size());
+    std::vector<double> h(2 * m, 0.0);
+    for (int j = 0; j < m; ++j) h[2 * j] = c[j];
+    SADMRG_HIP(hipMemcpyAsync(ctx->dscal(), h.data(), h.size() * 8, hipMemcpyHostToDevice, ctx->stream()));
+    SADMRG_HIP(launch_fill(n * w(), 0.0, y, ctx->stream()));
+    SADMRG_HIP(launch_gemv_n_update(V, n, m, ctx->dscal(), y, n, cx ? 1 : 0, ctx->stream()));
+    ctx->sync();  // h leaves scope
+  }
+};
+
+}  // namespace
+
+LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
+  Context* ctx = H.ctx();
+  Dev D{ctx, H.n(), H.complex()};
+  const int64_t nw = D.n * D.w();
+  const int ml = std::max(2, opt.mlanc);
+  require(ml + 1 <= 2048, "lanczos: basis too large");
+  require(Context::kMaxDscal >= 2 * (ml + 1), "lanczos: device scalar area too small");
+  LanczosOut out;
+  DevBuf Vb(ctx, std::max<int64_t>(nw, 1) * 8 * (ml + 1));
+  DevBuf hvb(ctx, std::max<int64_t>(nw, 1) * 8);
+  double* V = Vb.as<double>();
+  auto col = [&](int j) { return V + static_cast<int64_t>(j) * nw; };
+  double* hv = hvb.as<double>();
+  const Prec bulk = opt.prec;
+  auto matvec = [&](const double* x, double* y, Prec p) {
+    H.apply(x, y, p);
+    if (p == Prec::FP64) out.matvec64++;
+    else out.matvec32++;
+  };
+
+  std::vector<double> aal(ml, 0.0), nnl(ml + 1, 0.0), eig, z;
+  auto diatridiag = [&](int m) {
+    std::vector<double> d(aal.begin(), aal.begin() + m), e(std::max(m - 1, 0));
+    for (int i = 0; i + 1 < m; ++i) e[i] = nnl[i + 1];
+    tridiag_eig(m, d, e, z);
+    eig = d;
+  };
+  auto ritz = [&](int mc) {
+    std::vector<double> c(mc);
+    for (int j = 0; j < mc; ++j) c[j] = z[j];  // eigenvector 0, leading dimension mc
+    D.combine(V, c, d_vec);
+  };
+  // check_eigenvector (dmrg_inst.cpp:1243-1254): returns |<v,Hv>| / |Hv|
+  auto check = [&]() {
+    const double nor = D.nrm(d_vec);
+    D.scal(1.0 / nor, d_vec);
+    matvec(d_vec, hv, bulk);
+    const double prod = D.nrm(hv);
+    const dcmplex ov = D.dot(d_vec, hv);
+    return std::abs(ov) / prod;
+  };
+
+  // initialize_lanczos (dmrg_inst.cpp:1201-1212)
+  D.copy(d_vec, col(0));
+  {
+    const double nor = D.nrm(col(0));
+    if (nor > 1e-12) D.scal(1.0 / nor, col(0));
+  }
+  const double tol = 1e-6;
+  bool done = false;
+  matvec(col(0), col(1), bulk);
+  aal[0] = D.dot(col(1), col(0)).real();
+  D.axpby(1.0, -aal[0], col(0), col(1));
+  nnl[1] = D.nrm(col(1));
+  D.scal(1.0 / nnl[1], col(1));
+  out.steps = 1;
+  if (ml == 2) {
+    if (nnl[1] < tol) {
+      diatridiag(1);
+      ritz(1);
+    } else {
+      matvec(col(1), col(2), bulk);
+      aal[1] = D.dot(col(2), col(1)).real();
+      diatridiag(2);
+      ritz(2);
+    }
+    done = true;
+  }
+  for (int m = 2; !done && m < ml; ++m) {
+    matvec(col(m - 1), col(m), bulk);
+    aal[m - 1] = D.dot(col(m), col(m - 1)).real();
+    D.axpby(1.0, -aal[m - 1], col(m - 1), col(m));
+    D.axpby(1.0, -nnl[m - 1], col(m - 2), col(m));
+    D.orth(V, m, col(m));
+    nnl[m] = D.nrm(col(m));
+    D.scal(1.0 / nnl[m], col(m));
+    out.steps = m;
+    diatridiag(m);
+    if (nnl[m] < tol) {
+      ritz(m);
+      check();
+      done = true;
+      break;
+    }
+    if (m % 4 == 0) {
+      ritz(m);
+      const double ov = check();
+      out.trace.push_back(eig[0]);
+      if (std::fabs(ov - 1.0) < 1e-8) {
+        done = true;
+        break;
+      }
+    }
+  }
+  if (!done) {
+    matvec(col(ml - 1), col(ml), bulk);
+    aal[ml - 1] = D.dot(col(ml), col(ml - 1)).real();
+    diatridiag(ml);
+    ritz(ml);
+    check();
+    out.steps = ml;
+  }
+  out.energy = eig[0];
+
+  if (bulk != Prec::FP64 && opt.refine) {
+    // FP64 residual-correction restarts on the Ritz vector in d_vec.
+    const int k = std::max(1, std::min(opt.refine_krylov, ml - 1));
+    double* x = d_vec;
+    for (int r = 0; r <= opt.max_restarts; ++r) {
+      D.copy(x, col(0));
+      const double nor = D.nrm(col(0));
+      D.scal(1.0 / nor, col(0));
+      matvec(col(0), col(1), Prec::FP64);
+      const double a0 = D.dot(col(1), col(0)).real();
+      D.axpby(1.0, -a0, col(0), col(1));
+      const double b1 = D.nrm(col(1));
+      out.energy = a0;
+      out.residual64 = b1;
+      D.copy(col(0), x);
+      if (b1 <= opt.refine_tol * std::fabs(a0) || r == opt.max_restarts) break;
+      out.restarts++;
+      std::vector<double> ta{a0}, tb{0.0, b1};
+      D.scal(1.0 / b1, col(1));
+      int mk = 2;
+      for (int i = 1; i <= k; ++i) {
+        matvec(col(i), col(i + 1), bulk);
+        const double ai = D.dot(col(i + 1), col(i)).real();
+        ta.push_back(ai);
+        if (i == k) break;
+        D.axpby(1.0, -ai, col(i), col(i + 1));
+        D.axpby(1.0, -tb[i], col(i - 1), col(i + 1));
+        D.orth(V, i + 1, col(i + 1));
+        const double bi = D.nrm(col(i + 1));
+        if (bi < 1e-12 * std::fabs(a0)) break;
+        D.scal(1.0 / bi, col(i + 1));
+        tb.push_back(bi);
+        mk = i + 2;
+      }
+      const int msz = static_cast<int>(ta.size());
+      mk = msz;
+      std::vector<double> d(ta.begin(), ta.end()), e(std::max(mk - 1, 0));
+      for (int i = 0; i + 1 < mk; ++i) e[i] = tb[i + 1];
+      tridiag_eig(mk, d, e, z);
+      std::vector<double> c(mk);
+      for (int j = 0; j < mk; ++j) c[j] = z[j];
+      D.combine(V, c, x);
+    }
+  }
+  return out;
+}
+
+}  // namespace sadmrg

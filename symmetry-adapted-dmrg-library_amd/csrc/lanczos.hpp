@@ -1,0 +1,48 @@
+// lanczos.hpp -- device Lanczos ground-state solver.
+//
+// Mirrors Lanczos<Scalar,Sym>::lanczos1 (dmrg_inst.cpp:1286-1348): Krylov
+// basis of at most mlanc vectors, full re-orthogonalisation, tridiagonal
+// eigensolve after every step, early exit when beta < 1e-6, convergence check
+// every 4th step with one extra matvec (stop when |<v,Hv>|/|Hv| is within 1e-8
+// of 1).  Krylov vectors stay on the device in FP64; only the tridiagonal
+// (<= 40 x 40) goes to the host.
+//
+// Mixed precision: the bulk matvecs run with FP32 operands on FP32 MFMA; the
+// result is then refined by restarts whose first matvec is FP64 (an exact
+// residual r = H x - (x^T H x) x of the current Ritz vector) followed by a few
+// FP32 Krylov steps on that residual.  The energy returned is the FP64
+// Rayleigh quotient of the final vector.
+#pragma once
+#include <vector>
+#include "heff.hpp"
+
+namespace sadmrg {
+
+struct LanczosOpts {
+  int mlanc = 40;
+  Prec prec = Prec::FP64;     // precision of the bulk matvecs
+  bool refine = false;        // FP64 residual-correction restarts (FP32 bulk only)
+  double refine_tol = 1e-7;   // stop when |r64| <= refine_tol * |E|
+  int refine_krylov = 6;      // FP32 Krylov steps per restart
+  int max_restarts = 6;
+};
+
+struct LanczosOut {
+  double energy = 0;
+  int matvec64 = 0, matvec32 = 0;
+  int steps = 0;
+  int restarts = 0;
+  double residual64 = -1;     // |H x - E x| of the returned vector (FP64), if measured
+  std::vector<double> trace;  // Ritz values at each check
+};
+
+// Symmetric tridiagonal eigensolver (implicit QL with Wilkinson shifts).
+// d[n] diagonal, e[n-1] off-diagonal; z (n x n, column-major) receives the
+// eigenvectors; eigenvalues ascend in d.
+void tridiag_eig(int n, std::vector<double>& d, std::vector<double> e, std::vector<double>& z);
+
+// d_vec: start vector in the layout of H (n elements); on return the
+// normalised Ritz vector.
+LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt);
+
+}  // namespace sadmrg

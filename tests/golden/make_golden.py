@@ -1,0 +1,254 @@
+"""Generates the golden fixtures in tests/golden/ from the reference library.
+
+Runs only in a container where oracle/_ref/libsadmrg_ref.so was built from
+the reference sources (make -C oracle).  Every output is data: inputs and the
+reference's outputs on them.  Usage:
+
+    python tests/golden/make_golden.py            # all fixtures
+    python tests/golden/make_golden.py dmrg_su2_ring10   # one case
+
+Each DMRG case runs in a fresh process so the reference's global generator
+(std::mt19937_64 seeded 42, dense_inst.cpp:23) starts from its initial state.
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import sadmrg_testlib as tl  # noqa: E402
+import refbind as rb  # noqa: E402
+
+M = 20  # max_angm used by the reference driver (dmrg_inst.cpp:181)
+
+
+# ---------------------------------------------------------------------------
+def make_cgc():
+    rb.init(physpn=1, max_angm=M, max_dcut=64, psize=1)
+    L = rb.lib()
+    P2, PS = 3, 9
+    hv = np.zeros(M ** 3)
+    L.ref_fac_hamilt_vec(rb._dptr(hv))
+    out = {"hamilt_vec": hv.reshape(M, M, M)}
+    for which, name in enumerate(["onsite_left", "onsite_rght", "pairup_left", "pairup_rght"]):
+        t = np.zeros(M ** 4 * PS)
+        L.ref_fac_table5(which, rb._dptr(t))
+        nz = np.nonzero(t)[0]
+        out[name + "_idx"] = nz.astype(np.int64)
+        out[name + "_val"] = t[nz]
+    for which, name in enumerate(["transformation_left", "transformation_rght"]):
+        t = np.zeros(M ** 5 * P2)
+        L.ref_fac_transformation(which, rb._dptr(t))
+        nz = np.nonzero(t)[0]
+        out[name + "_idx"] = nz.astype(np.int64)
+        out[name + "_val"] = t[nz]
+    cg = []
+    for j1 in range(0, 7):
+        for j2 in range(0, 7):
+            for j3 in range(abs(j1 - j2), j1 + j2 + 1, 2):
+                for m1 in range(-j1, j1 + 1, 2):
+                    for m2 in range(-j2, j2 + 1, 2):
+                        m3 = m1 + m2
+                        if abs(m3) > j3:
+                            continue
+                        cg.append([j1, m1, j2, m2, j3, m3, L.ref_cg(j1, m1, j2, m2, j3, m3)])
+    out["cg"] = np.asarray(cg)
+    out["max_angm"] = np.asarray(M)
+    np.savez_compressed(os.path.join(HERE, "cgc_su2.npz"), **out)
+    print("cgc_su2.npz", {k: v.shape for k, v in out.items()})
+
+
+# ---------------------------------------------------------------------------
+def _su2_bond(rng, d, js, dmax):
+    return (d, list(js), [int(rng.integers(1, dmax + 1)) for _ in js])
+
+
+def random_hvm_case(rng, sym, cplx):
+    """psi (2-leg), op1/op2 (3-leg, spin-1 channel) with consistent bonds."""
+    if sym == 1:
+        js = sorted(rng.choice(np.arange(0, 9), size=int(rng.integers(2, 5)), replace=False).tolist())
+        # psi sectors share parity so that spin-1 operators connect them
+        par = int(rng.integers(0, 2))
+        js = sorted({2 * (j // 2) + par for j in js})
+        b0 = _su2_bond(rng, 1, js, 7)
+        b1 = (-1, b0[1], [int(rng.integers(1, 8)) for _ in js])
+        ch = (-1, [2], [1])
+        psi = tl.random_tensor(rng, 1, cplx, 0, [b0, b1])
+        op1 = tl.random_tensor(rng, 1, cplx, 0, [(-b0[0], b0[1], b0[2]), ch, b0], density=0.85)
+        op2 = tl.random_tensor(rng, 1, cplx, 0, [(-b1[0], b1[1], b1[2]), ch, b1], density=0.85)
+        hl = tl.random_tensor(rng, 1, cplx, 0, [(-b0[0], b0[1], b0[2]), b0])
+        hr = tl.random_tensor(rng, 1, cplx, 0, [(-b1[0], b1[1], b1[2]), b1])
+    else:
+        qs = sorted(rng.choice(np.arange(-6, 7, 2), size=int(rng.integers(3, 6)), replace=False).tolist())
+        b0 = (1, qs, [int(rng.integers(1, 7)) for _ in qs])
+        b1 = (1, [-q for q in qs], [int(rng.integers(1, 7)) for _ in qs])
+        ch1 = (-1, [-2, 0, 2], [1, 1, 1])
+        ch2 = (1, [-2, 0, 2], [1, 1, 1])
+        psi = tl.random_tensor(rng, 0, cplx, 0, [b0, b1])
+        op1 = tl.random_tensor(rng, 0, cplx, 0, [(-b0[0], b0[1], b0[2]), ch1, b0], density=0.85)
+        op2 = tl.random_tensor(rng, 0, cplx, 0, [(-b1[0], b1[1], b1[2]), ch2, b1], density=0.85)
+        hl = tl.random_tensor(rng, 0, cplx, 0, [(-b0[0], b0[1], b0[2]), b0])
+        hr = tl.random_tensor(rng, 0, cplx, 0, [(-b1[0], b1[1], b1[2]), b1])
+    return psi, op1, op2, hl, hr
+
+
+def make_hvm():
+    rb.init(physpn=1, max_angm=M, max_dcut=64, psize=1)
+    for sym, cplx, tag in [(1, 0, "su2_real"), (0, 0, "u1_real"), (0, 1, "u1_complex"), (1, 1, "su2_complex")]:
+        rng = np.random.default_rng(1000 + 10 * sym + cplx)
+        tens = {}
+        for c in range(4):
+            psi, op1, op2, hl, hr = random_hvm_case(rng, sym, cplx)
+            cg2 = 1 if sym == 1 else 0
+            rp, r1, r2 = tl.to_ref(psi, cg2), tl.to_ref(op1), tl.to_ref(op2)
+            out = tl.from_ref(rb.hvm(rp, r1, r2))
+            rh, rr = tl.to_ref(hl), tl.to_ref(hr)
+            lpsi = tl.from_ref(rb.contract(rh, 0, rp, 0))
+            psir = tl.from_ref(rb.contract(rp, 1, rr, 0))
+            for k, v in dict(psi=psi, op1=op1, op2=op2, hl=hl, hr=hr, out=out, lpsi=lpsi, psir=psir).items():
+                tens[f"c{c}_{k}"] = v
+        tl.save_tensors(os.path.join(HERE, f"hvm_{tag}.npz"), tens, ncase=4)
+        print(f"hvm_{tag}.npz", len(tens))
+
+
+# ---------------------------------------------------------------------------
+def snapshot(d, ns):
+    tens = {}
+    for i in range(ns):
+        for which in ("uu", "hh"):
+            t = tl.from_ref(d.tensor(which, i))
+            if int(t["header"][0]) > 0:
+                tens[f"{which}_{i}"] = t
+        for j in range(ns):
+            t = tl.from_ref(d.tensor("opr", i, j))
+            if int(t["header"][0]) > 0:
+                tens[f"opr_{i}_{j}"] = t
+    return tens
+
+
+def ref_sweep(d, ns, nfree, steps, capture=None):
+    """DMRG::sweep (dmrg_inst.cpp:874-943) step by step; returns per-step records."""
+    recs = []
+
+    def step(kind, i):
+        if kind == "left" or kind == "return":
+            il, ir = i - 2, i - 1
+            d.call("wavefunc_transformation", i - 1, 1)
+            d.call("right2", i, 1)
+            vec = d.input_vector(il, ir)
+            d.call("left1", il)
+            d.call("right1", ir)
+            if kind == "left":
+                p1, p2 = (1.0, 0.0) if i > nfree + 2 else (0.0, 1.0)
+            else:
+                p1, p2 = 1.0, 0.0
+        else:
+            il, ir = i + 1, i + 2
+            d.call("wavefunc_transformation", i + 1, 0)
+            d.call("left2", i, 1)
+            vec = d.input_vector(il, ir)
+            d.call("left1", il)
+            d.call("right1", ir)
+            p1, p2 = (0.0, 1.0) if i < ns - nfree - 3 else (1.0, 0.0)
+        cap = capture is not None and capture["when"] == (kind, i) and "done" not in capture
+        if cap:
+            capture["state"] = snapshot(d, ns)
+            capture["vec"] = tl.from_ref(vec)
+            capture["heff"] = tl.from_ref(d.heff(il, ir, vec))
+            capture["il"], capture["ir"], capture["p1"], capture["p2"] = il, ir, p1, p2
+        e = d.lanczos(il, ir, vec)
+        if cap:
+            capture["lanczos_e"] = e
+            capture["lanczos_vec"] = tl.from_ref(vec)
+        w = d.svd(vec, il, ir, p1, p2)
+        if cap:
+            capture["wout"] = w[:int(np.count_nonzero(w))]
+            capture["uu_l"] = tl.from_ref(d.tensor("uu", il))
+            capture["uu_r"] = tl.from_ref(d.tensor("uu", ir))
+            capture["done"] = True
+        recs.append((il, ir, e))
+
+    for i in range(ns // 2, nfree + 1, -1):
+        step("left", i)
+    for i in range(nfree, ns - nfree - 2):
+        step("right", i)
+    for i in range(ns - nfree - 1, ns // 2, -1):
+        step("return", i)
+    return recs
+
+
+def dmrg_case(name, sym, cplx, xx, yy, D, lattice, nsweep, capture_at):
+    rb.init(physpn=1, max_angm=M, max_dcut=D, psize=1)
+    d = rb.RefDMRG(sym, cplx, xx, yy, D, 1, 0)
+    ns = d.ns
+    if lattice is not None:
+        hmap, coup = lattice
+        d.set_couplings(hmap, coup)
+    hmap, coup = d.couplings()
+    d.idmrg()
+    e_idmrg = d.energy()
+    cap = {"when": tuple(capture_at)}
+    sweeps = []
+    for s in range(nsweep):
+        recs = ref_sweep(d, ns, 2, None, cap if s == 1 else None)
+        sweeps.append(recs)
+    tens = {}
+    for k, v in cap["state"].items():
+        tens["S_" + k] = v
+    for k in ("vec", "heff", "lanczos_vec", "uu_l", "uu_r"):
+        tens[k] = cap[k]
+    energies = np.asarray([[r[2] for r in recs] for recs in sweeps])
+    steps = np.asarray([[r[0], r[1]] for r in sweeps[0]], np.int32)
+    tl.save_tensors(os.path.join(HERE, f"{name}.npz"), tens, hmap=hmap, coup=coup,
+                    params=np.asarray([sym, cplx, xx, yy, D, ns], np.int32), e_idmrg=e_idmrg,
+                    energies=energies, steps=steps, cap_step=np.asarray([cap["il"], cap["ir"]], np.int32),
+                    cap_p=np.asarray([cap["p1"], cap["p2"]]), lanczos_e=cap["lanczos_e"], wout=cap["wout"])
+    print(name, "E_idmrg", e_idmrg, "sweep energies", energies[:, -1])
+
+
+CASES = {
+    # reference-native lattice: periodic snake (xx=1 gives a ring)
+    "dmrg_su2_ring10": dict(sym=1, cplx=0, xx=1, yy=10, D=16, lattice=None, nsweep=2, capture_at=("right", 3)),
+    "dmrg_u1_ring10": dict(sym=0, cplx=0, xx=1, yy=10, D=24, lattice=None, nsweep=2, capture_at=("right", 3)),
+    # open chain and J1-J2 cylinder through the coupling map
+    "dmrg_su2_obc12": dict(sym=1, cplx=0, xx=1, yy=12, D=24, lattice="obc", nsweep=2, capture_at=("left", 5)),
+    "dmrg_su2_j1j2_4x3": dict(sym=1, cplx=0, xx=3, yy=4, D=24, lattice="j1j2", nsweep=2, capture_at=("right", 4)),
+}
+
+
+def run_case(name):
+    c = dict(CASES[name])
+    lat = c.pop("lattice")
+    ns = c["xx"] * c["yy"]
+    if lat == "obc":
+        h = tl.open_chain(ns)
+        lattice = (h, np.asarray([0.0, 1.0] + [0.0] * (ns - 1)))
+    elif lat == "j1j2":
+        h = tl.j1j2_cylinder(c["yy"], c["xx"])
+        lattice = (h, np.asarray([0.0, 1.0, 0.5] + [0.0] * (ns - 2)))
+    else:
+        lattice = None
+    dmrg_case(name, lattice=lattice, **c)
+
+
+def main(argv):
+    if len(argv) > 1 and argv[1] in CASES:
+        run_case(argv[1])
+        return
+    if len(argv) > 1 and argv[1] == "cgc":
+        make_cgc()
+        return
+    if len(argv) > 1 and argv[1] == "hvm":
+        make_hvm()
+        return
+    for sub in ["cgc", "hvm"] + list(CASES):
+        subprocess.run([sys.executable, __file__, sub], check=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv)

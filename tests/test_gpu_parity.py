@@ -1,0 +1,189 @@
+"""Device engine vs the reference library's outputs (committed golden fixtures).
+
+Tolerances: FP64 block products agree to 1e-12 relative to the largest output
+element (GEMM summation order differs from the reference's BLAS); Lanczos and
+DMRG energies to 1e-10 relative; FP32-operand products to 1e-5.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import sadmrg_testlib as tl
+
+pytestmark = pytest.mark.gpu
+
+HVM_TAGS = ["su2_real", "u1_real", "u1_complex", "su2_complex"]
+
+
+def _gemm_tensor(rng, m, k, n, cplx):
+    """Two U(1) single-sector matrices for a plain GEMM through contract()."""
+    a = tl.random_tensor(rng, 0, cplx, 0, [(1, [0], [m]), (-1, [0], [k])])
+    b = tl.random_tensor(rng, 0, cplx, 0, [(1, [0], [k]), (-1, [0], [n])])
+    return a, b
+
+
+def _mat(d):
+    return list(tl.blocks(d).values())[0]
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (16, 4, 16), (37, 53, 29), (130, 70, 257), (64, 300, 64)])
+def test_grouped_gemm_plain_shapes(ctx, sad, shape, cplx):
+    """contract(A, 1, B, 0) = A B on one block: checks MFMA fragment layout and edges."""
+    rng = np.random.default_rng(sum(shape) + 7 * cplx)
+    m, k, n = shape
+    a, b = _gemm_tensor(rng, m, k, n, cplx)
+    out = tl.from_dev(sad.contract(tl.to_dev(ctx, a), 1, tl.to_dev(ctx, b), 0))
+    ref = _mat(a) @ _mat(b)
+    got = _mat(out)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("tag", HVM_TAGS)
+def test_hvm_matches_reference(ctx, sad, cgc, tag):
+    """BlockTensor::hamiltonian_vector_multiplication on seeded random blocks."""
+    t, e = tl.load_tensors(os.path.join(tl.GOLDEN, f"hvm_{tag}.npz"))
+    for c in range(int(e["ncase"])):
+        cg2 = 1 if t[f"c{c}_psi"]["sym"] == 1 else 0
+        psi = tl.to_dev(ctx, t[f"c{c}_psi"], cg2)
+        op1 = tl.to_dev(ctx, t[f"c{c}_op1"])
+        op2 = tl.to_dev(ctx, t[f"c{c}_op2"])
+        got = tl.from_dev(sad.hvm(psi, op1, op2, cgc))
+        exp = t[f"c{c}_out"]
+        assert list(got["present"]) == list(exp["present"]), (tag, c)
+        rel, _ = tl.compare(got, exp)
+        assert rel <= 1e-12, (tag, c, rel)
+
+
+@pytest.mark.parametrize("tag", HVM_TAGS)
+def test_contract_matches_reference(ctx, sad, tag):
+    """BlockTensor::contract for H_L^T psi and psi H_R."""
+    t, e = tl.load_tensors(os.path.join(tl.GOLDEN, f"hvm_{tag}.npz"))
+    for c in range(int(e["ncase"])):
+        cg2 = 1 if t[f"c{c}_psi"]["sym"] == 1 else 0
+        psi = tl.to_dev(ctx, t[f"c{c}_psi"], cg2)
+        hl = tl.to_dev(ctx, t[f"c{c}_hl"])
+        hr = tl.to_dev(ctx, t[f"c{c}_hr"])
+        for got, key in [(sad.contract(hl, 0, psi, 0), "lpsi"), (sad.contract(psi, 1, hr, 0), "psir")]:
+            g = tl.from_dev(got)
+            rel, _ = tl.compare(g, t[f"c{c}_{key}"])
+            assert rel <= 1e-12, (tag, c, key, rel)
+
+
+@pytest.mark.parametrize("tag", ["su2_real", "u1_real", "u1_complex", "su2_complex"])
+def test_heff_plan_equals_sum_of_terms(ctx, sad, cgc, tag):
+    """The fused two-launch plan equals H_L^T psi + psi H_R + hvm (reference outputs)."""
+    t, e = tl.load_tensors(os.path.join(tl.GOLDEN, f"hvm_{tag}.npz"))
+    for c in range(int(e["ncase"])):
+        cg2 = 1 if t[f"c{c}_psi"]["sym"] == 1 else 0
+        psi = tl.to_dev(ctx, t[f"c{c}_psi"], cg2)
+        op1, op2 = tl.to_dev(ctx, t[f"c{c}_op1"]), tl.to_dev(ctx, t[f"c{c}_op2"])
+        hl, hr = tl.to_dev(ctx, t[f"c{c}_hl"]), tl.to_dev(ctx, t[f"c{c}_hr"])
+        h = sad.Heff(ctx, cgc, hl, hr, [op1], [op2], psi)
+        got = tl.blocks(tl.from_dev(h.apply(psi)))
+        exp = {}
+        for key in ("lpsi", "psir", "out"):
+            for b, v in tl.blocks(t[f"c{c}_{key}"]).items():
+                exp[b] = exp.get(b, 0) + v
+        scale = max(np.max(np.abs(v)) for v in exp.values())
+        for b, v in exp.items():
+            assert np.max(np.abs(got[b] - v)) <= 1e-12 * scale, (tag, c, b)
+        if not t[f"c{c}_psi"]["cplx"]:
+            g32 = tl.blocks(tl.from_dev(h.apply(psi, sad.FP32)))
+            for b, v in exp.items():
+                assert np.max(np.abs(g32[b] - v)) <= 2e-5 * scale, (tag, c, b)
+
+
+def _load_case(name):
+    return tl.load_tensors(os.path.join(tl.GOLDEN, f"{name}.npz"))
+
+
+def _engine_from_state(ctx, sad, tens, extra):
+    sym, cplx, xx, yy, D, ns = (int(x) for x in extra["params"])
+    d = sad.DMRG(ctx, sym, cplx, xx, yy, D)
+    d.set_couplings(extra["hmap"], extra["coup"])
+    d.set_max_dcut(D)
+    for k, v in tens.items():
+        if not k.startswith("S_"):
+            continue
+        parts = k[2:].split("_")
+        if parts[0] == "opr":
+            d.set_tensor("opr", int(parts[1]), int(parts[2]), tl.to_dev(ctx, v))
+        else:
+            d.set_tensor(parts[0], int(parts[1]), 0, tl.to_dev(ctx, v))
+    return d
+
+
+DMRG_CASES = ["dmrg_su2_ring10", "dmrg_u1_ring10", "dmrg_su2_obc12", "dmrg_su2_j1j2_4x3"]
+
+
+@pytest.mark.parametrize("name", DMRG_CASES)
+def test_heff_on_reference_environments(ctx, sad, name):
+    """DMRG::hamiltonian_vector_multiplication_idmrg on the reference's own
+    environments captured mid-sweep; FP64 and FP32-operand paths."""
+    tens, extra = _load_case(name)
+    d = _engine_from_state(ctx, sad, tens, extra)
+    il, ir = (int(x) for x in extra["cap_step"])
+    cg2 = 1 if tens["vec"]["sym"] == 1 else 0
+    vec = tl.to_dev(ctx, tens["vec"], cg2)
+    got = tl.from_dev(d.heff(il, ir, vec))
+    rel, _ = tl.compare(got, tens["heff"])
+    assert rel <= 1e-12, rel
+    got32 = tl.from_dev(d.heff(il, ir, vec, sad.FP32))
+    rel32, _ = tl.compare(got32, tens["heff"])
+    assert rel32 <= 1e-5, rel32
+
+
+@pytest.mark.parametrize("name", DMRG_CASES)
+def test_lanczos_and_svd_on_reference_state(ctx, sad, name):
+    """Lanczos (dmrg_inst.cpp:1286-1348) energy and BlockTensor::svd singular values."""
+    tens, extra = _load_case(name)
+    d = _engine_from_state(ctx, sad, tens, extra)
+    il, ir = (int(x) for x in extra["cap_step"])
+    cg2 = 1 if tens["vec"]["sym"] == 1 else 0
+    vec = tl.to_dev(ctx, tens["vec"], cg2)
+    e = d.lanczos(il, ir, vec)
+    e_ref = float(extra["lanczos_e"])
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref), (e, e_ref)
+    # Ritz vector equal up to a global sign
+    v = tl.blocks(tl.from_dev(vec))
+    vr = tl.blocks(tens["lanczos_vec"])
+    flat = np.concatenate([v[b].ravel() for b in sorted(vr)])
+    flat_r = np.concatenate([vr[b].ravel() for b in sorted(vr)])
+    s = np.sign(np.vdot(flat_r, flat).real)
+    assert np.max(np.abs(s * flat - flat_r)) <= 1e-6
+    # SVD on the reference's own Ritz vector: singular values identical
+    vref = tl.to_dev(ctx, tens["lanczos_vec"], cg2)
+    p1, p2 = (float(x) for x in extra["cap_p"])
+    w = d.svd(vref, il, ir, p1, p2)
+    wr = extra["wout"]
+    k = min(len(w), len(wr))
+    assert k > 0
+    assert np.max(np.abs(np.sort(w[:k])[::-1] - np.sort(wr[:k])[::-1])) <= 1e-10 * wr.max()
+    ul = tl.from_dev(d.tensor("uu", il))
+    assert list(ul["header"]) == list(tens["uu_l"]["header"])
+
+
+@pytest.mark.parametrize("name", DMRG_CASES)
+def test_full_dmrg_energies_match_reference(ctx, sad, name):
+    """do_idmrg + two sweeps: every per-step Lanczos energy of the second
+    sweep and the iDMRG energy agree with the reference run."""
+    tens, extra = _load_case(name)
+    sym, cplx, xx, yy, D, ns = (int(x) for x in extra["params"])
+    sad.lib().sadmrg_rng_seed(42)
+    d = sad.DMRG(ctx, sym, cplx, xx, yy, D)
+    d.set_couplings(extra["hmap"], extra["coup"])
+    d.set_max_dcut(D)
+    d.idmrg()
+    e0 = d.energy()
+    ref0 = float(extra["e_idmrg"])
+    assert abs(e0 - ref0) <= 1e-9 * abs(ref0), (e0, ref0)
+    d.records()
+    energies = extra["energies"]
+    for s in range(energies.shape[0]):
+        d.sweep()
+        recs = d.records()
+        got = np.asarray([r["energy"] for r in recs])
+        assert got.shape == energies[s].shape
+        assert np.max(np.abs(got - energies[s])) <= 1e-9 * np.max(np.abs(energies[s])), (s, got, energies[s])
