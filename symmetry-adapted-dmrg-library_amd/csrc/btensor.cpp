@@ -60,6 +60,7 @@ void BTensor::layout(Context* c, bool complex_, const Struct& s, const std::vect
   off.assign(n, -1);
   tag.assign(n, tag0);
   nelem = 0;
+  fuse_id = false;
   for (int b = 0; b < n; ++b)
     if (mask[b]) {
       off[b] = nelem;
@@ -142,6 +143,7 @@ void copy_tensor(BTensor& dst, const BTensor& src) {
   for (int b = 0; b < src.nten(); ++b) mask[b] = src.present(b);
   dst.layout(src.ctx, src.cplx, src.st, mask, kCgcIdent);
   dst.tag = src.tag;
+  dst.fuse_id = src.fuse_id;
   if (src.nelem > 0)
     SADMRG_HIP(hipMemcpyAsync(dst.dptr(), src.dptr(), src.nelem * sizeof(double) * src.w(),
                               hipMemcpyDeviceToDevice, src.ctx->stream()));
@@ -374,6 +376,71 @@ int dims3(const BTensor& t, int b, int k) {
   return d[k];
 }
 
+// Row offset of each block of an identity fusion inside its fused sector
+// (the pos[] bookkeeping of BlockTensor::fuse, block_tensor_inst.cpp:722-738).
+std::vector<int64_t> fuse_positions(const BTensor& f) {
+  std::vector<int64_t> pos(f.nten(), -1), acc(f.st.bonds[2].nmom(), 0);
+  int idx[3], d[3];
+  for (int b = 0; b < f.nten(); ++b) {
+    if (!f.st.allowed(b)) continue;
+    f.st.decode(b, idx);
+    f.st.block_dims(b, d);
+    pos[b] = acc[idx[2]];
+    acc[idx[2]] += int64_t(d[0]) * d[1];
+  }
+  return pos;
+}
+
+// Scatter table builder with a host-side bounds check of every task.
+struct ScatterBuilder {
+  std::vector<ScatterTask> tasks;
+  void copy(int64_t src_off, int64_t lds, int64_t dst_off, int64_t ldd, int rows, int cols, double scale) {
+    for (int c0 = 0; c0 < cols; c0 += 64) {
+      ScatterTask t{};
+      t.src_off = src_off + int64_t(c0) * lds;
+      t.dst_off = dst_off + int64_t(c0) * ldd;
+      t.lds = lds;
+      t.ldd = ldd;
+      t.rows = rows;
+      t.cols = std::min(64, cols - c0);
+      t.kind = 0;
+      t.scale = scale;
+      tasks.push_back(t);
+    }
+  }
+  void stripe(int64_t dst_off, int64_t ldd, int n, double scale) {
+    ScatterTask t{};
+    t.dst_off = dst_off;
+    t.ldd = ldd;
+    t.rows = n;
+    t.cols = 1;
+    t.kind = 1;
+    t.scale = scale;
+    tasks.push_back(t);
+  }
+  void run(Context* ctx, const double* src, int64_t src_n, double* dst, int64_t dst_n, int w) {
+    if (tasks.empty()) return;
+    for (const auto& t : tasks) {
+      if (t.rows <= 0 || t.cols <= 0) continue;
+      if (t.kind == 0) {
+        require(t.src_off >= 0 && t.src_off + (t.rows - 1) + int64_t(t.cols - 1) * t.lds < src_n,
+                "scatter table out of bounds (source)");
+        require(t.dst_off >= 0 && t.dst_off + (t.rows - 1) + int64_t(t.cols - 1) * t.ldd < dst_n,
+                "scatter table out of bounds (destination)");
+      } else {
+        require(t.dst_off >= 0 && t.dst_off + int64_t(t.rows - 1) * (t.ldd + 1) < dst_n,
+                "scatter table out of bounds (stripe)");
+      }
+    }
+    DevBuf d(ctx, tasks.size() * sizeof(ScatterTask));
+    SADMRG_HIP(hipMemcpyAsync(d.get(), tasks.data(), tasks.size() * sizeof(ScatterTask), hipMemcpyHostToDevice,
+                              ctx->stream()));
+    SADMRG_HIP(launch_block_scatter(d.as<ScatterTask>(), static_cast<int>(tasks.size()), src, dst, w,
+                                    ctx->stream()));
+    ctx->sync();
+  }
+};
+
 }  // namespace
 
 void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const BTensor* op1, const BTensor* op2,
@@ -447,6 +514,41 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
   struct Con { int s1, l; double scale; int K; };
   std::vector<std::vector<Con>> per;
 
+  // Enlarged basis (uu = vv = an identity fusion, dmrg_inst.cpp:595,651):
+  // every block chain is a scaled copy of one operator block, or a scaled
+  // identity stripe, placed at the embedded position.  The GEMM lowering
+  // gives the same numbers (products with exact 0/1 entries); the scatter
+  // moves only the operator bytes.
+  bool scatter = uu.fuse_id && vv.fuse_id && uu.st == vv.st;
+  if (scatter && op.nbond() == 3)
+    for (int j = 0; j < op.nten(); ++j)
+      if (op.present(j) && dims3(op, j, 1) != 1) scatter = false;
+  std::vector<int64_t> fpos;
+  if (scatter) fpos = fuse_positions(uu);
+  auto scatter_run = [&](bool stripe) {
+    ScatterBuilder sb;
+    for (int m = 0; m < os.nten(); ++m) {
+      if (per[m].empty()) continue;
+      int dm[3];
+      os.block_dims(m, dm);
+      const int64_t ldd = os.nbond() == 3 ? int64_t(dm[0]) * dm[1] : dm[0];
+      for (const auto& c : per[m]) {
+        const int k = stripe ? c.s1 : s1[c.s1].k;
+        const int64_t dst = out.off[m] + fpos[k] + ldd * fpos[c.l];
+        if (stripe) {
+          sb.stripe(dst, ldd, c.K, c.scale);
+          continue;
+        }
+        const int j = s1[c.s1].j;
+        int d[3];
+        op.st.block_dims(j, d);
+        const int rows = d[0], cols = op.nbond() == 3 ? d[2] : d[1];
+        sb.copy(op.off[j], rows, dst, ldd, rows, cols, c.scale);
+      }
+    }
+    sb.run(ctx, stripe ? nullptr : op.dptr(), stripe ? 0 : op.nelem, out.dptr(), out.nelem, uu.w());
+  };
+
   auto stage1_uu_op = [&](int j, int k) -> S1 {
     // step1(c, y, z) = sum_a uu(a@flag, c) op(a, y, z): uu is (l x r) once
     // the unit physical leg is dropped.
@@ -497,6 +599,10 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
     for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
     out.clean();
     out.layout(ctx, cx, os, mask, kCgcIdent);
+    if (scatter) {
+      scatter_run(false);
+      return;
+    }
     DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double) * uu.w());
     ar.set(4, sbuf.get(), scratch);
     ar.set(5, out.dptr(), out.nelem);
@@ -570,6 +676,10 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
     for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
     out.clean();
     out.layout(ctx, cx, os, mask, kCgcCanon);
+    if (scatter) {
+      scatter_run(true);
+      return;
+    }
     ar.set(5, out.dptr(), out.nelem);
     for (int m = 0; m < os.nten(); ++m) {
       if (per[m].empty()) continue;
@@ -622,6 +732,10 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
     for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
     out.clean();
     out.layout(ctx, cx, os, mask, kCgcCanon);
+    if (scatter) {
+      scatter_run(false);
+      return;
+    }
     DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double) * uu.w());
     ar.set(4, sbuf.get(), scratch);
     ar.set(5, out.dptr(), out.nelem);
@@ -711,6 +825,10 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
     for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
     out.clean();
     out.layout(ctx, cx, os, mask, kCgcIdent);
+    if (scatter) {
+      scatter_run(false);
+      return;
+    }
     DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double) * uu.w());
     ar.set(4, sbuf.get(), scratch);
     ar.set(5, out.dptr(), out.nelem);
@@ -763,6 +881,7 @@ void op_fuse(BTensor& out, Context* ctx, bool cplx, SymKind sym, const Bond& b1,
     SADMRG_HIP(launch_strided_fill(n, 1.0, out.block_ptr(b) + pos[idx[2]] * n * w, (1 + n) * w, ctx->stream()));
     pos[idx[2]] += n;
   }
+  out.fuse_id = true;
 }
 
 void op_fuse_to_singlet(BTensor& out, Context* ctx, bool cplx, SymKind sym, const Bond& b1, const Bond& b2) {

@@ -34,6 +34,7 @@ struct BTensor {
   std::vector<uint8_t> tag;    // per block CGC tag (SU(2))
   DevBuf buf;                  // nelem elements of (cplx ? 16 : 8) bytes
   int64_t nelem = 0;
+  bool fuse_id = false;        // built by op_fuse: 0/1 identity-embedding blocks
 
   BTensor() = default;
   BTensor(BTensor&&) = default;

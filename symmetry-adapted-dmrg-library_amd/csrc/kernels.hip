@@ -471,6 +471,37 @@ __global__ void strided_fill_kernel(int64_t n, double v, double* __restrict__ p,
   for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n; t += g) p[t * stride] = v;
 }
 
+// Block scatter (enlarged-basis environment updates): each workgroup copies
+// one column chunk of one operator block into its embedded position.
+__global__ __launch_bounds__(256) void block_scatter_kernel(const ScatterTask* __restrict__ tasks,
+                                                            const double* __restrict__ src,
+                                                            double* __restrict__ dst, int w) {
+  const ScatterTask t = tasks[blockIdx.x];
+  if (t.kind == 1) {
+    for (int i = threadIdx.x; i < t.rows; i += blockDim.x) {
+      double* d = dst + (t.dst_off + int64_t(i) * (t.ldd + 1)) * w;
+      d[0] = t.scale;
+      if (w == 2) d[1] = 0.0;
+    }
+    return;
+  }
+  const int64_t total = int64_t(t.rows) * t.cols;
+  for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int r = static_cast<int>(e % t.rows);
+    const int c = static_cast<int>(e / t.rows);
+    const double* s = src + (t.src_off + r + int64_t(c) * t.lds) * w;
+    double* d = dst + (t.dst_off + r + int64_t(c) * t.ldd) * w;
+    for (int k = 0; k < w; ++k) d[k] = t.scale * s[k];
+  }
+}
+
+hipError_t launch_block_scatter(const ScatterTask* tasks, int ntasks, const double* src, double* dst, int w,
+                                hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(block_scatter_kernel, dim3(ntasks), dim3(256), 0, st, tasks, src, dst, w);
+  return hipGetLastError();
+}
+
 static int grid_for(int64_t n, int block = 256) {
   int64_t g = (n + block - 1) / block;
   if (g > 4096) g = 4096;

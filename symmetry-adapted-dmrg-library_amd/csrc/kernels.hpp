@@ -40,4 +40,17 @@ hipError_t launch_strided_fill(int64_t n, double v, double* p, int64_t stride, h
 
 constexpr int kPartialGrid = 1024;
 
+// One rectangular piece of a block scatter: dst[r + c*ldd] (+)= scale * src[r + c*lds]
+// for r < rows, c < cols (kind 0), or dst[t*(ldd+1)] = scale for t < rows
+// (kind 1: a scaled identity stripe).  Offsets are in elements.
+struct ScatterTask {
+  int64_t src_off, dst_off, lds, ldd;
+  int32_t rows, cols, kind, pad;
+  double scale;
+};
+// Executes a scatter table: one workgroup per task (tasks are pre-split into
+// column chunks); w = doubles per element (2 for complex).
+hipError_t launch_block_scatter(const ScatterTask* tasks, int ntasks, const double* src, double* dst, int w,
+                                hipStream_t st);
+
 }  // namespace sadmrg

@@ -187,3 +187,36 @@ def test_full_dmrg_energies_match_reference(ctx, sad, name):
         got = np.asarray([r["energy"] for r in recs])
         assert got.shape == energies[s].shape
         assert np.max(np.abs(got - energies[s])) <= 1e-9 * np.max(np.abs(energies[s])), (s, got, energies[s])
+
+
+@pytest.mark.parametrize("name", DMRG_CASES)
+def test_environment_updates_replay_reference(ctx, sad, name):
+    """prepare_site_operator_from_{left,right} in both bases on the reference's
+    captured state (dmrg_inst.cpp:440-695): the recomputed hh and opr equal
+    the reference's.  The enlarged basis exercises the scatter lowering, the
+    truncated basis the grouped-GEMM lowering."""
+    tens, extra = _load_case(name)
+    ns = int(extra["params"][5])
+    il, ir = (int(x) for x in extra["cap_step"])
+    state = {k[2:]: v for k, v in tens.items() if k.startswith("S_")}
+    calls = [("left1", il, None), ("right1", ir, None)]
+    if il - 1 >= 1:
+        calls.append(("left2", il - 1, 1))
+    if ir + 1 <= ns - 2:
+        calls.append(("right2", ir + 1, 1))
+    for call, site, flag in calls:
+        d = _engine_from_state(ctx, sad, tens, extra)
+        if flag is None:
+            d.call(call, site)
+        else:
+            d.call(call, site, flag)
+        checked = 0
+        for key in [f"hh_{site}"] + [k for k in state if k.startswith(f"opr_{site}_")]:
+            if key not in state:
+                continue
+            parts = key.split("_")
+            got = tl.from_dev(d.tensor("opr", site, int(parts[2])) if parts[0] == "opr" else d.tensor("hh", site))
+            rel, _ = tl.compare(got, state[key])
+            assert rel <= 1e-12, (call, site, key, rel)
+            checked += 1
+        assert checked >= 2, (call, site)

@@ -1,0 +1,202 @@
+"""Pins the oracle (oracle/sadmrg_oracle.py) before it is trusted:
+reference golden vectors, the reference library itself where it is built,
+closed forms and exact diagonalisation.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+import sadmrg_testlib as tl
+import sadmrg_oracle as orc
+
+HVM_TAGS = ["su2_real", "u1_real", "u1_complex", "su2_complex"]
+DMRG_CASES = ["dmrg_su2_ring10", "dmrg_u1_ring10", "dmrg_su2_obc12", "dmrg_su2_j1j2_4x3"]
+
+
+# ---- closed forms -------------------------------------------------------------------
+def test_cg_orthonormality_and_known_values():
+    for j1 in range(0, 5):
+        for j2 in range(0, 5):
+            # sum_{m1 m2} C(j3 m3) C(j3' m3') = delta
+            for j3 in range(abs(j1 - j2), j1 + j2 + 1, 2):
+                for j3b in range(abs(j1 - j2), j1 + j2 + 1, 2):
+                    for m3 in range(-j3, j3 + 1, 2):
+                        s = sum(orc.cg(j1, m1, j2, m3 - m1, j3, m3) * orc.cg(j1, m1, j2, m3 - m1, j3b, m3)
+                                for m1 in range(-j1, j1 + 1, 2) if abs(m3 - m1) <= j2)
+                        if abs(m3) <= j3b:
+                            assert abs(s - (1.0 if j3 == j3b else 0.0)) < 1e-13
+    assert abs(orc.cg(1, 1, 1, -1, 2, 0) - np.sqrt(0.5)) < 1e-15
+    assert abs(orc.cg(1, 1, 1, -1, 0, 0) - np.sqrt(0.5)) < 1e-15
+    assert abs(orc.cg(1, -1, 1, 1, 0, 0) + np.sqrt(0.5)) < 1e-15
+    assert abs(orc.cg(2, 2, 1, -1, 1, 1) - np.sqrt(2.0 / 3.0)) < 1e-15
+
+
+def test_cg_against_reference_values():
+    z = np.load(os.path.join(tl.GOLDEN, "cgc_su2.npz"))
+    for row in z["cg"]:
+        j1, m1, j2, m2, j3, m3 = (int(x) for x in row[:6])
+        assert abs(orc.cg(j1, m1, j2, m2, j3, m3) - row[6]) <= 1e-14
+
+
+def test_fac_tables_against_reference():
+    z = np.load(os.path.join(tl.GOLDEN, "cgc_su2.npz"))
+    M = int(z["max_angm"])
+    hv = z["hamilt_vec"]
+    for i in range(M):
+        for k in range(M):
+            assert abs(orc.fac_hamilt_vec(i, 2, k) - hv[i, 2, k]) <= 1e-12
+            assert abs(orc.fac_hamilt_vec(i, 0, k) - hv[i, 0, k]) <= 1e-12
+    for side, name in enumerate(["onsite_left", "onsite_rght"]):
+        for idx, val in zip(z[name + "_idx"], z[name + "_val"]):
+            j, m, l, i, p = (int(x) for x in np.unravel_index(idx, (M, M, M, M, 9)))
+            assert abs(orc.fac_onsite(side, j, m, l, i, p) - val) <= 1e-12 * max(1, abs(val))
+    for side, name in enumerate(["pairup_left", "pairup_rght"]):
+        for idx, val in zip(z[name + "_idx"], z[name + "_val"]):
+            j, i, k, m, p = (int(x) for x in np.unravel_index(idx, (M, M, M, M, 9)))
+            assert abs(orc.fac_pairup(side, j, i, k, m, p) - val) <= 1e-12 * max(1, abs(val))
+    for side, name in enumerate(["transformation_left", "transformation_rght"]):
+        vals = z[name + "_val"]
+        for idx, val in list(zip(z[name + "_idx"], vals))[::3]:
+            j, m, l, i, k, p = (int(x) for x in np.unravel_index(idx, (M, M, M, M, M, 3)))
+            assert abs(orc.fac_transformation(side, j, m, l, i, k, p) - val) <= 1e-12 * max(1, abs(val))
+
+
+def test_fac_hamilt_vec_support_and_singlet_energy():
+    """Two spin-1/2 sites in a singlet: the bond term sigma0 x sigma1 with the
+    reduced elements +-sqrt(3)/2 (dmrg_inst.cpp:236-239) and weight
+    fac_hamilt_vec[1][2][1] must give |<S1.S2>| = 3/4; the weight vanishes off
+    the spin-1 triangle."""
+    f = orc.fac_hamilt_vec(1, 2, 1)
+    assert abs(abs((np.sqrt(3) / 2) * (np.sqrt(3) / 2) * f) - 0.75) < 1e-14
+    for i in range(10):
+        for k in range(10):
+            if k < abs(i - 2) or k > i + 2 or (i + k) % 2:
+                assert orc.fac_hamilt_vec(i, 2, k) == 0.0
+
+
+# ---- block operations vs reference golden vectors ----------------------------------
+@pytest.mark.parametrize("tag", HVM_TAGS)
+def test_oracle_hvm_and_contract(tag):
+    t, e = tl.load_tensors(os.path.join(tl.GOLDEN, f"hvm_{tag}.npz"))
+    for c in range(int(e["ncase"])):
+        psi = orc.from_packed(t[f"c{c}_psi"])
+        op1, op2 = orc.from_packed(t[f"c{c}_op1"]), orc.from_packed(t[f"c{c}_op2"])
+        got = orc.to_packed(orc.hvm(psi, op1, op2))
+        rel, _ = tl.compare(got, t[f"c{c}_out"])
+        assert rel <= 1e-13, (tag, c, rel)
+        hl, hr = orc.from_packed(t[f"c{c}_hl"]), orc.from_packed(t[f"c{c}_hr"])
+        rel, _ = tl.compare(orc.to_packed(orc.contract(hl, 0, psi, 0)), t[f"c{c}_lpsi"])
+        assert rel <= 1e-13
+        rel, _ = tl.compare(orc.to_packed(orc.contract(psi, 1, hr, 0)), t[f"c{c}_psir"])
+        assert rel <= 1e-13
+
+
+def test_planted_mistake_is_caught():
+    """A wrong sign of the recoupling weight would not pass the hvm check."""
+    t, e = tl.load_tensors(os.path.join(tl.GOLDEN, "hvm_su2_real.npz"))
+    psi = orc.from_packed(t["c0_psi"])
+    op1, op2 = orc.from_packed(t["c0_op1"]), orc.from_packed(t["c0_op2"])
+    good = orc.hvm(psi, op1, op2)
+    bad = orc.BT(good.sym, good.cplx, 0, good.bonds, {k: -v for k, v in good.blocks.items()})
+    rel, _ = tl.compare(orc.to_packed(bad), t["c0_out"])
+    assert rel > 0.5
+
+
+def _state(tens):
+    return {k[2:]: v for k, v in tens.items() if k.startswith("S_")}
+
+
+@pytest.mark.parametrize("name", DMRG_CASES)
+def test_oracle_heff_lanczos_svd_on_reference_state(name):
+    tens, extra = tl.load_tensors(os.path.join(tl.GOLDEN, f"{name}.npz"))
+    ns = int(extra["params"][5])
+    il, ir = (int(x) for x in extra["cap_step"])
+    got = orc.heff_from_state(_state(tens), extra["hmap"], extra["coup"], ns, il, ir, tens["vec"])
+    rel, _ = tl.compare(got, tens["heff"])
+    assert rel <= 1e-12, rel
+    # Lanczos on the oracle H_eff, flat vectors in block order
+    state = {k: orc.from_packed(v) for k, v in _state(tens).items()}
+    vec = orc.from_packed(tens["vec"])
+    terms = orc.heff_terms(state, np.asarray(extra["hmap"]), extra["coup"], ns, il, ir)
+    keys = sorted(vec.blocks)
+
+    def flat(t):
+        return np.concatenate([t.blocks[k].ravel(order="F") if k in t.blocks else
+                               np.zeros(vec.blocks[k].size, dtype=vec.blocks[k].dtype) for k in keys])
+
+    def unflat(x):
+        t = orc.BT(vec.sym, vec.cplx, 0, vec.bonds)
+        off = 0
+        for k in keys:
+            n = vec.blocks[k].size
+            t.blocks[k] = x[off:off + n].reshape(vec.blocks[k].shape, order="F")
+            off += n
+        return t
+
+    H = lambda x: flat(orc.heff(state.get(f"hh_{il}"), state.get(f"hh_{ir}"), terms, unflat(x)))  # noqa: E731
+    ndata = sum(v.size for v in vec.blocks.values())
+    e, v, nmv = orc.lanczos1(H, flat(vec), min(40, ndata + 1))
+    assert abs(e - float(extra["lanczos_e"])) <= 1e-10 * abs(e)
+    w, _ = orc.svd_values(orc.from_packed(tens["lanczos_vec"]), int(extra["params"][4]))
+    wr = extra["wout"]
+    k = min(len(w), len(wr))
+    assert np.max(np.abs(np.sort(w[:k]) - np.sort(wr[:k]))) <= 1e-10 * wr.max()
+
+
+# ---- physics pins --------------------------------------------------------------------
+@pytest.mark.parametrize("name,tol", [("dmrg_su2_ring10", 1e-9), ("dmrg_u1_ring10", 1e-9),
+                                      ("dmrg_su2_obc12", 1e-9), ("dmrg_su2_j1j2_4x3", 2e-2)])
+def test_reference_energies_against_exact_diagonalisation(name, tol):
+    """The reference's converged sweep energy versus ED of the same lattice map
+    (the J1-J2 cylinder at D=24 is not yet converged: loose bound)."""
+    tens, extra = tl.load_tensors(os.path.join(tl.GOLDEN, f"{name}.npz"))
+    e_ed = tl.heisenberg_ed(np.asarray(extra["hmap"]), extra["coup"])[0]
+    e = float(extra["energies"][-1][-1])
+    assert e >= e_ed - 1e-9
+    assert abs(e - e_ed) <= tol * abs(e_ed), (e, e_ed)
+
+
+# ---- oracle vs the reference library itself (where it is built) -----------------------
+refbind = pytest.importorskip("refbind")
+
+
+@pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built (no reference sources here)")
+@pytest.mark.parametrize("name", ["dmrg_su2_ring10", "dmrg_u1_ring10"])
+def test_oracle_env_updates_against_reference_library(name):
+    """Each environment update of the oracle equals the reference's on real
+    mid-sweep tensors (both flags, truncated and enlarged bases)."""
+    import refbind as rb
+    rb.init(physpn=1, max_angm=20, max_dcut=64, psize=1)
+    tens, extra = tl.load_tensors(os.path.join(tl.GOLDEN, f"{name}.npz"))
+    st = _state(tens)
+    checked = 0
+    for key, v in st.items():
+        if not key.startswith("uu_"):
+            continue
+        i = int(key.split("_")[1])
+        uu = tl.to_ref(v)
+        for kind, opkey, flag in [("overlap_transformation", f"hh_{i - 1}", 0),
+                                  ("overlap_transformation", f"hh_{i + 1}", 1),
+                                  ("operator_transformation", None, 0),
+                                  ("operator_transformation", None, 1)]:
+            ops = []
+            if opkey is not None and opkey in st:
+                ops = [st[opkey]]
+            elif kind == "operator_transformation":
+                nb = i - 1 if flag == 0 else i + 1
+                ops = [st[k] for k in st if k.startswith(f"opr_{nb}_")][:3]
+            for o in ops:
+                b_uu, b_o = orc.from_packed(v), orc.from_packed(o)
+                # the reference exits the process on a bond mismatch: check first
+                if b_uu.bonds[flag] != orc.inv(b_o.bonds[0]):
+                    continue
+                if b_uu.bonds[flag] != (b_o.bonds[1] if kind == "overlap_transformation" else b_o.bonds[2]):
+                    continue
+                ref_out = tl.from_ref(rb.env(kind, uu, uu, tl.to_ref(o), None, flag))
+                if int(ref_out["header"][0]) == 0:
+                    continue
+                got = orc.to_packed(orc.env(kind, b_uu, b_uu, b_o, None, flag))
+                rel, _ = tl.compare(got, ref_out)
+                assert rel <= 1e-12, (key, kind, flag, rel)
+                checked += 1
+    assert checked > 0
