@@ -160,6 +160,62 @@ def cpu_baseline(wl, budget_s=20.0):
             "output": out}
 
 
+def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0):
+    """Mixed vs FP64 sweep on the SU(2) J1-J2 cylinder (width x length, snake
+    ordering).  Warm-up: iDMRG at warm[0] then one mixed sweep per entry of
+    warm and one at D; the state is then cloned on the device and one sweep
+    is timed in each precision from the identical start."""
+    ns = width * length
+    coup = np.zeros(ns + 1)
+    coup[1], coup[2] = 1.0, j2
+    d = sad.DMRG(ctx, sad.SU2, 0, length, width, D)
+    d.set_couplings(sad.j1j2_cylinder(width, length), coup)
+    d.set_precision(sad.FP32, refine=True, refine_tol=refine_tol)
+    t0 = time.perf_counter()
+    d.set_max_dcut(warm[0])
+    d.idmrg()
+    for dw in list(warm[1:]) + [D]:
+        d.set_max_dcut(dw)
+        d.sweep()
+        print(f"[bench] warm-up sweep D={dw}: E={d.energy():.12f} ({time.perf_counter() - t0:.1f} s)",
+              file=sys.stderr, flush=True)
+    d.records()
+    out = {}
+    for name, prec in (("fp64", sad.FP64), ("mixed", sad.FP32)):
+        c = d.clone()
+        c.set_precision(prec, refine=(prec != sad.FP64), refine_tol=refine_tol)
+        ctx.sync()
+        t1 = time.perf_counter()
+        c.sweep()
+        ctx.sync()
+        wall = time.perf_counter() - t1
+        recs = c.records()
+        out[name] = {"seconds": wall, "energy": c.energy(), "recs": recs}
+        print(f"[bench] {name} sweep: {wall:.2f} s, E={c.energy():.12f}, steps={len(recs)}",
+              file=sys.stderr, flush=True)
+        del c
+    e64 = np.asarray([r["energy"] for r in out["fp64"]["recs"]])
+    emx = np.asarray([r["energy"] for r in out["mixed"]["recs"]])
+    rel = np.abs(emx - e64) / np.abs(e64)
+
+    def tsum(name, key):
+        return float(sum(r[key] for r in out[name]["recs"]))
+
+    return {
+        "lattice": f"SU(2) J1-J2 cylinder {width}x{length} (J2/J1={j2}), snake ordering, {ns} sites",
+        "D": D, "warm_up": list(warm) + [D],
+        "fp64_sweep_s": round(out["fp64"]["seconds"], 3), "mixed_sweep_s": round(out["mixed"]["seconds"], 3),
+        "speedup": round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3),
+        "energy_fp64": out["fp64"]["energy"], "energy_mixed": out["mixed"]["energy"],
+        "max_rel_dE_per_step": float(rel.max()), "final_rel_dE": float(rel[-1]),
+        "steps": len(e64), "max_kept": int(max(r["kept"] for r in out["fp64"]["recs"])),
+        "fp64_split_s": {k: round(tsum("fp64", k), 3) for k in ("t_env", "t_lanczos", "t_svd")},
+        "mixed_split_s": {k: round(tsum("mixed", k), 3) for k in ("t_env", "t_lanczos", "t_svd")},
+        "matvecs_fp64": int(tsum("fp64", "matvec64")),
+        "matvecs_mixed": [int(tsum("mixed", "matvec32")), int(tsum("mixed", "matvec64"))],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +224,8 @@ def main():
     ap.add_argument("--D", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--sweep", default="6x4:512:64,128,256",
+                    help="WIDTHxLENGTH:D:warm-up Ds for the mixed vs FP64 sweep comparison ('none' skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the engine has no multi-GPU path); got --gpus {args.gpus}",
@@ -227,12 +285,19 @@ def main():
             print(f"[bench] cpu: {cpu['seconds_per_apply']:.3f} s/apply on {cpu['threads']} threads, "
                   f"parity {relc:.2e}", file=sys.stderr, flush=True)
 
+    sweep = None
+    if args.sweep != "none":
+        lat, dd, warm = args.sweep.split(":")
+        w_, l_ = (int(x) for x in lat.split("x"))
+        print(f"[bench] sweep comparison {lat} D={dd}", file=sys.stderr, flush=True)
+        sweep = sweep_compare(sad, ctx, w_, l_, int(dd), [int(x) for x in warm.split(",")])
+
     dom = "stage2" if res["fp32"]["stage2_ms"] >= res["fp32"]["stage1_ms"] else "stage1"
     dom_flops = f2 if dom == "stage2" else f1
     dom_ms = res["fp32"][dom + "_ms"]
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
     line = {
-        "metric": "H_eff*psi TFLOP/s, SU(2) D=2048 centre bond (mixed FP32-MFMA vs FP64)",
+        "metric": "H_eff\u00b7\u03c8 TFLOP/s + sweep time, SU(2) D=2048: mixed vs FP64 GPU at |\u0394E|\u22641e-9",
         "value": round(res["fp32"]["tflops"], 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -252,6 +317,7 @@ def main():
         "fp32": {k: round(v, 4) for k, v in res["fp32"].items()},
         "mixed_vs_fp64_speedup": round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3),
         "fp32_rel_err_vs_fp64": rel32,
+        "sweep": sweep,
         "roofline": {"bound": "mfma", "kernel": f"grouped_gemm {dom} (fp32)", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
                      "traffic": None,

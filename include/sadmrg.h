@@ -147,6 +147,10 @@ int sadmrg_dmrg_create(sadmrg_ctx* ctx, int sym, int cplx, int xx, int yy, int b
                        sadmrg_dmrg** out);
 int sadmrg_dmrg_destroy(sadmrg_dmrg* d);
 int sadmrg_dmrg_ns(sadmrg_dmrg* d, int* ns);
+/* deep copy of src's couplings, MPS, environments and settings into dst
+   (created with the same lattice, symmetry and scalar type); the reference
+   copies a DMRG state only through its checkpoint files (dmrg_inst.cpp:947-1177) */
+int sadmrg_dmrg_copy_state(sadmrg_dmrg* dst, const sadmrg_dmrg* src);
 /* replaces hmap (ns x ns, row-major) and coup (dmrg_inst.cpp:117-176) */
 int sadmrg_dmrg_set_couplings(sadmrg_dmrg* d, const int* hmap, const double* coup, int ncoup);
 int sadmrg_dmrg_get_couplings(sadmrg_dmrg* d, int* hmap, double* coup, int cap);

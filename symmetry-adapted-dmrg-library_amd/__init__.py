@@ -86,6 +86,7 @@ def lib():
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_destroy": (i, [vp]),
         "sadmrg_dmrg_ns": (i, [vp, ip]),
+        "sadmrg_dmrg_copy_state": (i, [vp, vp]),
         "sadmrg_dmrg_set_couplings": (i, [vp, ip, dp, i]),
         "sadmrg_dmrg_get_couplings": (i, [vp, ip, dp, i]),
         "sadmrg_dmrg_set_max_dcut": (i, [vp, i]),
@@ -402,6 +403,7 @@ class DMRG:
         p = C.c_void_p()
         check(lib().sadmrg_dmrg_create(ctx.ptr, sym, cplx, xx, yy, bondd, physpn, totspin, C.byref(p)))
         self.ptr, self.ctx, self.sym, self.cplx = p, ctx, sym, cplx
+        self._xx, self._yy, self._bondd, self._physpn, self._totspin = xx, yy, bondd, physpn, totspin
         n = C.c_int()
         check(lib().sadmrg_dmrg_ns(p, C.byref(n)))
         self.ns = n.value
@@ -409,6 +411,18 @@ class DMRG:
     def __del__(self):
         if getattr(self, "ptr", None) and _lib is not None:
             _lib.sadmrg_dmrg_destroy(self.ptr)
+
+    def clone(self):
+        """A new driver holding a device-side deep copy of this one's state."""
+        p = C.c_void_p()
+        check(lib().sadmrg_dmrg_create(self.ctx.ptr, self.sym, self.cplx, self._xx, self._yy, self._bondd,
+                                       self._physpn, self._totspin, C.byref(p)))
+        other = DMRG.__new__(DMRG)
+        other.ptr, other.ctx, other.sym, other.cplx, other.ns = p, self.ctx, self.sym, self.cplx, self.ns
+        other._xx, other._yy, other._bondd, other._physpn, other._totspin = (
+            self._xx, self._yy, self._bondd, self._physpn, self._totspin)
+        check(lib().sadmrg_dmrg_copy_state(p, self.ptr))
+        return other
 
     def set_couplings(self, hmap, coup):
         h = np.ascontiguousarray(hmap, dtype=np.int32).reshape(-1)
@@ -487,3 +501,55 @@ class DMRG:
         n = C.c_int()
         check(lib().sadmrg_dmrg_svd(self.ptr, vec.ptr, a, b, p1, p2, _dptr(w), cap, C.byref(n)))
         return w[:n.value].copy()
+
+
+# ---- lattice maps for DMRG.set_couplings -------------------------------------
+def snake_site(x, y, width):
+    """Snake ordering of the reference constructor (dmrg_inst.cpp:121-140):
+    even columns run up, odd columns run down."""
+    return x * width + (y if x % 2 == 0 else width - 1 - y)
+
+
+def reference_lattice(xx, yy):
+    """hmap built by DMRG(xx, yy, ...) (dmrg_inst.cpp:117-144): nearest
+    neighbours, periodic in both directions, coupling index 1."""
+    ns = xx * yy
+    h = np.zeros((ns, ns), dtype=np.int32)
+    for i in range(ns):
+        x1 = i // yy
+        y1 = i % yy if x1 % 2 == 0 else yy - 1 - (i % yy)
+        for x2, y2 in (((x1 + 1) % xx, y1), (x1, (y1 + 1) % yy)):
+            j = snake_site(x2, y2, yy)
+            if i != j:
+                h[i, j] = h[j, i] = 1
+    return h
+
+
+def open_chain(length):
+    """Open chain, coupling index 1 on every bond."""
+    h = np.zeros((length, length), dtype=np.int32)
+    for i in range(length - 1):
+        h[i, i + 1] = h[i + 1, i] = 1
+    return h
+
+
+def j1j2_cylinder(width, length):
+    """Snake-ordered width x length cylinder, periodic around the width and
+    open along the length: index 1 on nearest-neighbour bonds (J1), index 2 on
+    diagonal bonds between neighbouring columns (J2)."""
+    ns = width * length
+    h = np.zeros((ns, ns), dtype=np.int32)
+    for x in range(length):
+        for y in range(width):
+            i = snake_site(x, y, width)
+            nb = [(x, (y + 1) % width)] + ([(x + 1, y)] if x + 1 < length else [])
+            for a, b in nb:
+                j = snake_site(a, b, width)
+                if i != j:
+                    h[i, j] = h[j, i] = 1
+            if x + 1 < length:
+                for dy in (1, -1):
+                    j = snake_site(x + 1, (y + dy) % width, width)
+                    if i != j and h[i, j] == 0:
+                        h[i, j] = h[j, i] = 2
+    return h

@@ -589,6 +589,15 @@ int sadmrg_dmrg_ns(sadmrg_dmrg* d, int* ns) {
   });
 }
 
+int sadmrg_dmrg_copy_state(sadmrg_dmrg* dst, const sadmrg_dmrg* src) {
+  return guard([&] {
+    need(dst, "dst");
+    need(src, "src");
+    if (dst->ctx != src->ctx) fail(kErrInvalid, "copy_state: drivers belong to different contexts");
+    dst->d->copy_state_from(*src->d);
+  });
+}
+
 int sadmrg_dmrg_set_couplings(sadmrg_dmrg* d, const int* hmap, const double* coup, int ncoup) {
   return guard([&] {
     need(d, "dmrg");

@@ -60,6 +60,26 @@ void Dmrg::set_couplings(const std::vector<int>& hmap, const std::vector<double>
   }
 }
 
+void Dmrg::copy_state_from(const Dmrg& o) {
+  require(o.ns_ == ns_ && o.sym_ == sym_ && o.cplx_ == cplx_ && o.physpn_ == physpn_,
+          "copy_state_from: drivers differ in lattice size, symmetry or scalar type");
+  hmap_ = o.hmap_;
+  fll_ = o.fll_;
+  frr_ = o.frr_;
+  coup_ = o.coup_;
+  totspin_ = o.totspin_;
+  max_dcut_ = o.max_dcut_;
+  lopt_ = o.lopt_;
+  gs_enr_ = o.gs_enr_;
+  wout_ = o.wout_;
+  for (int i = 0; i < ns_; ++i) {
+    copy_tensor(uu_[i], o.uu_[i]);
+    copy_tensor(hh_[i], o.hh_[i]);
+    for (int j = 0; j < ns_; ++j) copy_tensor(opr_[i][j], o.opr_[i][j]);
+  }
+  ctx_->sync();
+}
+
 // dmrg_inst.cpp:222-260
 void Dmrg::initialize_local_operators() {
   op_make_spinor_start(sigma_[0], ctx_, cplx_, sym_, physpn_);

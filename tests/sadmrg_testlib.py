@@ -201,55 +201,16 @@ def load_tensors(path):
 # ---- lattices ---------------------------------------------------------------------
 def reference_lattice(xx, yy):
     """hmap of the reference constructor (snake ordering, periodic in x and y)."""
-    ns = xx * yy
-    lx, ly = xx, yy
-    h = np.zeros((ns, ns), dtype=np.int32)
-    for k in range(2):
-        for i in range(ns):
-            x1 = i // ly
-            y1 = i % ly if x1 % 2 == 0 else ly - 1 - (i % ly)
-            if k == 0:
-                x2, y2 = (x1 + 1) % lx, y1
-            else:
-                x2, y2 = x1, (y1 + 1) % ly
-            j = x2 * ly + y2 if x2 % 2 == 0 else x2 * ly + (ly - 1 - y2)
-            if i != j:
-                h[i, j] = h[j, i] = 1
-    return h
+    return load_pkg().reference_lattice(xx, yy)
 
 
 def open_chain(L):
-    h = np.zeros((L, L), dtype=np.int32)
-    for i in range(L - 1):
-        h[i, i + 1] = h[i + 1, i] = 1
-    return h
+    return load_pkg().open_chain(L)
 
 
 def j1j2_cylinder(width, length):
-    """Snake-ordered width x length cylinder (periodic around the width, open
-    along the length); coupling index 1 = J1 (nearest), 2 = J2 (diagonal)."""
-    ns = width * length
-
-    def site(x, y):
-        return x * width + (y if x % 2 == 0 else width - 1 - y)
-
-    h = np.zeros((ns, ns), dtype=np.int32)
-    for x in range(length):
-        for y in range(width):
-            i = site(x, y)
-            nb1 = [(x, (y + 1) % width)]
-            if x + 1 < length:
-                nb1.append((x + 1, y))
-            for (a, b) in nb1:
-                j = site(a, b)
-                if i != j:
-                    h[i, j] = h[j, i] = 1
-            if x + 1 < length:
-                for dy in (1, -1):
-                    j = site(x + 1, (y + dy) % width)
-                    if i != j and h[i, j] == 0:
-                        h[i, j] = h[j, i] = 2
-    return h
+    """Snake-ordered width x length cylinder; index 1 = J1, 2 = J2 (diagonal)."""
+    return load_pkg().j1j2_cylinder(width, length)
 
 
 def heisenberg_ed(hmap, coup, nlow=1):

@@ -200,3 +200,22 @@ def test_oracle_env_updates_against_reference_library(name):
                 assert rel <= 1e-12, (key, kind, flag, rel)
                 checked += 1
     assert checked > 0
+
+
+@pytest.mark.parametrize("name", ["dmrg_su2_ring10", "dmrg_u1_ring10"])
+def test_package_lattice_matches_reference_constructor_fixture(name):
+    """The package's reference_lattice reproduces the map the reference
+    constructor built for the ring fixtures (dmrg_inst.cpp:117-144)."""
+    _, extra = tl.load_tensors(os.path.join(tl.GOLDEN, f"{name}.npz"))
+    xx, yy = int(extra["params"][2]), int(extra["params"][3])
+    got = tl.load_pkg().reference_lattice(xx, yy)
+    assert (got == np.asarray(extra["hmap"]).reshape(got.shape)).all()
+
+
+@pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built (no reference sources here)")
+@pytest.mark.parametrize("xx,yy", [(4, 3), (6, 6), (5, 4), (2, 6)])
+def test_package_lattice_matches_reference_constructor(xx, yy):
+    d = refbind.RefDMRG(1, 0, xx, yy, 8)
+    h, _ = d.couplings()
+    got = tl.load_pkg().reference_lattice(xx, yy)
+    assert (np.asarray(h).reshape(got.shape) == got).all()
