@@ -32,57 +32,6 @@ PEAK_FP32 = 157.3   # TFLOP/s, MI355X dense FP32 matrix
 PEAK_HBM = 8000.0   # GB/s
 
 
-def rotor_dims(D, sigma, two_j_parity=0, jmax2=16):
-    js = list(range(two_j_parity, jmax2 + 1, 2))
-    w = np.array([(j + 1) * np.exp(-(j / 2) * (j / 2 + 1) / (2 * sigma ** 2)) for j in js])
-    d = np.maximum(1, np.round(D * w / w.sum())).astype(int)
-    keep = d >= 2
-    return [j for j, k in zip(js, keep) if k], [int(x) for x, k in zip(d, keep) if k]
-
-
-def fused(js, ds):
-    """fuse(block bond, spin-1/2) sector dimensions (triangle rule)."""
-    out = {}
-    for j, d in zip(js, ds):
-        for jj in (j - 1, j + 1):
-            if jj >= 0:
-                out[jj] = out.get(jj, 0) + d
-    ks = sorted(out)
-    return ks, [out[k] for k in ks]
-
-
-def build_workload(D=2048, n_op=12, sigma=1.8, seed=1):
-    import sadmrg_testlib as tl
-    rng = np.random.default_rng(seed)
-    js, ds = rotor_dims(D, sigma)
-    fj, fd = fused(js, ds)
-    b0 = (1, fj, fd)
-    b1 = (-1, fj, fd)
-    ch = (-1, [2], [1])
-    psi = tl.random_tensor(rng, 1, 0, 0, [b0, b1])
-    psi["data"] /= np.linalg.norm(psi["data"])
-    scale_l = 1.0 / np.sqrt(max(fd))
-    hl = tl.random_tensor(rng, 1, 0, 0, [(-1, fj, fd), b0], scale=scale_l)
-    hr = tl.random_tensor(rng, 1, 0, 0, [(1, fj, fd), b1], scale=scale_l)
-    # symmetrise the block Hamiltonians: (A + A^T) / 2 per diagonal block
-    for h in (hl, hr):
-        off = 0
-        _, _, bonds = tl.parse_header(h["header"])
-        for b in range(tl.nten(bonds)):
-            if not h["present"][b]:
-                continue
-            dims = tl.block_dims(bonds, b)
-            n = dims[0] * dims[1]
-            m = h["data"][off:off + n].reshape(dims, order="F")
-            h["data"][off:off + n] = (0.5 * (m + m.T)).ravel(order="F")
-            off += n
-    o1 = [tl.random_tensor(rng, 1, 0, 0, [(-1, fj, fd), ch, b0], scale=scale_l) for _ in range(n_op)]
-    o2 = [tl.random_tensor(rng, 1, 0, 0, [(1, fj, fd), ch, b1], scale=scale_l) for _ in range(n_op)]
-    info = {"two_j_block": js, "dims_block": ds, "two_j_fused": fj, "dims_fused": fd,
-            "n": int(sum(d * d for d in fd)), "n_op": n_op, "D": D, "sigma": sigma}
-    return dict(psi=psi, hl=hl, hr=hr, o1=o1, o2=o2, info=info)
-
-
 class HipTimer:
     """HIP events recorded on the engine's own stream."""
 
@@ -131,23 +80,8 @@ def cpu_baseline(wl, budget_s=20.0):
         ncpu = len(os.sched_getaffinity(0))
     except AttributeError:
         ncpu = os.cpu_count() or 1
-    n_op = wl["info"]["n_op"]
-    threads = max(1, min(n_op, ncpu))
-    rb.init(physpn=1, max_angm=20, max_dcut=wl["info"]["D"], psize=threads)
-    ns = 2 * n_op
-    d = rb.RefDMRG(1, 0, 4, ns // 4, wl["info"]["D"], 1, 0)
-    hmap = np.zeros((ns, ns), dtype=np.int32)
-    for j in range(n_op):
-        hmap[j, n_op + j] = hmap[n_op + j, j] = 1
-    coup = np.zeros(ns + 1)
-    coup[1] = 1.0
-    d.set_couplings(hmap, coup)
-    il, ir = n_op - 1, n_op
-    d.set_tensor("hh", il, 0, tl.to_ref(wl["hl"]))
-    d.set_tensor("hh", ir, 0, tl.to_ref(wl["hr"]))
-    for j in range(n_op):
-        d.set_tensor("opr", il, j, tl.to_ref(wl["o1"][j]))
-        d.set_tensor("opr", ir, n_op + j, tl.to_ref(wl["o2"][j]))
+    threads = max(1, min(wl["info"]["n_op"], ncpu))
+    d, il, ir = tl.reference_heff_setup(rb, wl, threads)
     vin = tl.to_ref(wl["psi"], 1)
     t0 = time.time()
     sec = d.time_heff(il, ir, vin, 1)
@@ -224,6 +158,8 @@ def main():
     ap.add_argument("--D", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device tensor 5 times (HBM counter calibration) and exit")
     ap.add_argument("--sweep", default="6x4:512:64,128,256",
                     help="WIDTHxLENGTH:D:warm-up Ds for the mixed vs FP64 sweep comparison ('none' skips it)")
     args = ap.parse_args()
@@ -234,8 +170,19 @@ def main():
 
     import sadmrg_testlib as tl
     sad = tl.load_pkg()
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 8
+        ctx = sad.Context(0)
+        t = tl.to_dev(ctx, dict(sym=0, cplx=0, header=tl.make_header(0, [(1, [0], [n]), (-1, [0], [1])]),
+                                present=np.ones(1, np.uint8), data=np.ones(n)))
+        for _ in range(5):
+            t.copy()
+        ctx.sync()
+        print(json.dumps({"calibration": "device-to-device tensor copy", "bytes_read": n * 8,
+                          "bytes_written": n * 8, "copies": 5}))
+        return
     print(f"[bench] building C3 workload D={args.D}", file=sys.stderr, flush=True)
-    wl = build_workload(D=args.D)
+    wl = tl.build_workload(D=args.D)
     ctx = sad.Context(0)
     cgc = sad.Cgc(1)
     psi = tl.to_dev(ctx, wl["psi"], 1)

@@ -133,7 +133,8 @@ int sadmrg_heff_apply_device(sadmrg_heff* h, const void* d_in, void* d_out, int 
 int sadmrg_heff_apply_stage(sadmrg_heff* h, const void* d_in, void* d_out, int precision, int stage);
 int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops);
 /* info[8] = {psi blocks, stage-1 problems, stage-2 problems, stage-1 tiles,
-   stage-2 tiles, stage-1 tile cfg, stage-2 tile cfg, reserved} */
+   stage-2 tiles, stage-1 tile cfg, stage-2 tile cfg, output tiles with a
+   split reduction} */
 int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n, int* info);
 /* Lanczos::lanczos1 (lanczos.hpp:70) on the plan; psi is the start vector on
    entry and the Ritz vector on return.  counters[4] = {FP64 matvecs, FP32
@@ -163,7 +164,8 @@ int sadmrg_dmrg_sweep(sadmrg_dmrg* d);         /* sweep (dmrg.hpp:111) */
 int sadmrg_dmrg_energy(sadmrg_dmrg* d, double* e); /* get_enr (dmrg.hpp:102) */
 /* per-step records since the last call: 12 doubles each {il, ir, energy,
    truncation error, kept, FP64 matvecs, FP32 matvecs, t_env, t_lanczos, t_svd,
-   H_eff flops, 0}; *n = number written; records are consumed */
+   H_eff flops, reduced elements of the wave function}; *n = number written;
+   records are consumed */
 int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n);
 /* which: 0 uu[i], 1 hh[i], 2 opr[i][j] */
 int sadmrg_dmrg_get_tensor(sadmrg_dmrg* d, int which, int i, int j, sadmrg_tensor* out);

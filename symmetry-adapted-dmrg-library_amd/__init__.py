@@ -380,7 +380,8 @@ class Heff:
         check(lib().sadmrg_heff_stats(self.ptr, C.byref(f), C.byref(b), C.byref(n), _iptr(info)))
         return {"flops": f.value, "bytes": b.value, "n": n.value, "blocks": int(info[0]),
                 "stage1_problems": int(info[1]), "stage2_problems": int(info[2]), "tiles1": int(info[3]),
-                "tiles2": int(info[4]), "tile_cfg1": int(info[5]), "tile_cfg2": int(info[6])}
+                "tiles2": int(info[4]), "tile_cfg1": int(info[5]), "tile_cfg2": int(info[6]),
+                "split_tiles": int(info[7])}
 
     def lanczos(self, psi, mlanc=40, precision=FP64, refine=False, refine_tol=0.0):
         e = C.c_double()
@@ -393,7 +394,7 @@ class Heff:
 
 
 REC_FIELDS = ("il", "ir", "energy", "trunc_err", "kept", "matvec64", "matvec32", "t_env", "t_lanczos", "t_svd",
-              "heff_flops", "reserved")
+              "heff_flops", "ndata")
 
 
 class DMRG:

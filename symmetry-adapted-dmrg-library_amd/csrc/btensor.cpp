@@ -45,7 +45,7 @@ void run_batch(GemmBatch& b, Context* ctx, bool cplx, const Arenas& a) {
   if (b.empty()) return;
   for (int i = 0; i < kMaxArenas; ++i)
     if (a.n[i] >= 0) b.limit(i, a.n[i]);
-  b.finalize(ctx, cplx ? 0 : -1);
+  b.finalize(ctx, cplx ? 0 : -1, !cplx);
   b.run(ctx, prec_of(cplx), a.at);
 }
 

@@ -534,7 +534,7 @@ int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n
       info[4] = s.tiles2;
       info[5] = s.tile_cfg1;
       info[6] = s.tile_cfg2;
-      info[7] = 0;
+      info[7] = s.split_tiles;
     }
   });
 }
@@ -687,7 +687,7 @@ int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n) {
       p[8] = s.t_lanczos;
       p[9] = s.t_svd;
       p[10] = s.heff_flops;
-      p[11] = 0;
+      p[11] = static_cast<double>(s.ndata);
     }
     r.erase(r.begin(), r.begin() + m);
     if (n) *n = m;

@@ -310,6 +310,7 @@ double Dmrg::lanczos_solve(int il, int ir, BTensor& vec) {
   cur_.matvec64 = r.matvec64;
   cur_.matvec32 = r.matvec32;
   cur_.heff_flops = plan->stats().flops;
+  cur_.ndata = ndata;
   cur_.t_lanczos += since(t0);
   return r.energy;
 }

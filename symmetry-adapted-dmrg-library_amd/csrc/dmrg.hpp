@@ -23,6 +23,7 @@ struct StepRecord {
   int matvec64, matvec32;
   double t_env, t_lanczos, t_svd;  // seconds
   double heff_flops;  // algorithmic flops per H_eff application at this step
+  int64_t ndata;      // reduced elements of the two-site wave function
 };
 
 class Dmrg {

@@ -44,7 +44,7 @@ double GemmBatch::flops(bool cplx) const {
   return cplx ? 4.0 * f : f;
 }
 
-void GemmBatch::finalize(Context* ctx, int tile_cfg) {
+void GemmBatch::finalize(Context* ctx, int tile_cfg, bool allow_split) {
   require(!finalized_, "GemmBatch: finalize twice");
   finalized_ = true;
   if (tile_cfg < 0) {
@@ -80,26 +80,77 @@ void GemmBatch::finalize(Context* ctx, int tile_cfg) {
   const int tm = tile_m(tile_cfg), tn = tile_n(tile_cfg);
   std::vector<GemmProb> hp;
   std::vector<GemmSeg> hs;
-  std::vector<GemmTile> ht;
   hp.reserve(probs_.size());
+  std::vector<int64_t> klen;
+  int64_t base_tiles = 0;
   for (size_t i = 0; i < probs_.size(); ++i) {
     GemmProb p = probs_[i].p;
     if (p.M <= 0 || p.N <= 0) continue;
     p.seg_begin = static_cast<int32_t>(hs.size());
-    for (const auto& s : probs_[i].segs) hs.push_back(s);
+    int64_t k = 0;
+    for (const auto& s : probs_[i].segs) {
+      hs.push_back(s);
+      k += s.K;
+    }
     p.seg_end = static_cast<int32_t>(hs.size());
-    const int pi = static_cast<int>(hp.size());
     hp.push_back(p);
+    klen.push_back(k);
+    base_tiles += int64_t((p.M + tm - 1) / tm) * ((p.N + tn - 1) / tn);
+  }
+  // Split factor: enough workgroups to fill the GPU, each keeping at least
+  // kMinSplitK of reduction depth, cut at segment boundaries by cumulative K.
+  constexpr int64_t kMinSplitK = 512;
+  int want = 1;
+  if (allow_split && base_tiles > 0 && base_tiles < kTargetWorkgroups)
+    want = static_cast<int>((kTargetWorkgroups + base_tiles - 1) / base_tiles);
+  std::vector<GemmTile> ht;
+  std::vector<GemmReduce> hr;
+  int nparts = 0;
+  for (size_t pi = 0; pi < hp.size(); ++pi) {
+    const GemmProb& p = hp[pi];
+    const int nseg = p.seg_end - p.seg_begin;
+    int split = std::min<int64_t>({int64_t(want), int64_t(nseg), klen[pi] / kMinSplitK});
+    // segment groups [cut[g], cut[g+1]) with near-equal K
+    std::vector<int> cut{p.seg_begin};
+    if (split > 1) {
+      int64_t acc = 0;
+      int g = 1;
+      for (int s = p.seg_begin; s < p.seg_end && g < split; ++s) {
+        acc += hs[s].K;
+        if (acc * split >= klen[pi] * g && s + 1 < p.seg_end) {
+          cut.push_back(s + 1);
+          ++g;
+        }
+      }
+    }
+    cut.push_back(p.seg_end);
+    split = static_cast<int>(cut.size()) - 1;
     for (int n0 = 0; n0 < p.N; n0 += tn)
-      for (int m0 = 0; m0 < p.M; m0 += tm) ht.push_back(GemmTile{pi, m0, n0, 0});
+      for (int m0 = 0; m0 < p.M; m0 += tm) {
+        if (split <= 1) {
+          ht.push_back(GemmTile{static_cast<int32_t>(pi), m0, n0, p.seg_begin, p.seg_end, -1});
+          continue;
+        }
+        hr.push_back(GemmReduce{static_cast<int32_t>(pi), m0, n0, nparts, split, 0});
+        for (int g = 0; g < split; ++g) ht.push_back(GemmTile{static_cast<int32_t>(pi), m0, n0, cut[g], cut[g + 1], nparts + g});
+        nparts += split;
+      }
   }
   // Longest reductions first so the tail of the launch is short tiles.
-  std::vector<int64_t> klen(hp.size(), 0);
-  for (size_t i = 0; i < hp.size(); ++i)
-    for (int s = hp[i].seg_begin; s < hp[i].seg_end; ++s) klen[i] += hs[s].K;
-  std::stable_sort(ht.begin(), ht.end(),
-                   [&](const GemmTile& a, const GemmTile& b) { return klen[a.prob] > klen[b.prob]; });
+  auto tile_k = [&](const GemmTile& t) {
+    int64_t k = 0;
+    for (int s = t.seg_lo; s < t.seg_hi; ++s) k += hs[s].K;
+    return k;
+  };
+  std::vector<std::pair<int64_t, size_t>> order(ht.size());
+  for (size_t i = 0; i < ht.size(); ++i) order[i] = {tile_k(ht[i]), i};
+  std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<GemmTile> sorted(ht.size());
+  for (size_t i = 0; i < ht.size(); ++i) sorted[i] = ht[order[i].second];
+  ht.swap(sorted);
   ntiles_ = static_cast<int>(ht.size());
+  nreduce_ = static_cast<int>(hr.size());
+  nparts_ = nparts;
   if (ntiles_ == 0) return;
   d_probs_ = DevBuf(ctx, hp.size() * sizeof(GemmProb));
   d_segs_ = DevBuf(ctx, std::max<size_t>(hs.size(), 1) * sizeof(GemmSeg));
@@ -111,6 +162,13 @@ void GemmBatch::finalize(Context* ctx, int tile_cfg) {
                               ctx->stream()));
   SADMRG_HIP(hipMemcpyAsync(d_tiles_.get(), ht.data(), ht.size() * sizeof(GemmTile), hipMemcpyHostToDevice,
                             ctx->stream()));
+  if (nreduce_ > 0) {
+    d_reduce_ = DevBuf(ctx, hr.size() * sizeof(GemmReduce));
+    SADMRG_HIP(hipMemcpyAsync(d_reduce_.get(), hr.data(), hr.size() * sizeof(GemmReduce), hipMemcpyHostToDevice,
+                              ctx->stream()));
+    // room for double partials (the widest output type)
+    d_partial_ = DevBuf(ctx, int64_t(nparts_) * tm * tn * sizeof(double));
+  }
   // host vectors go out of scope: make the uploads complete first
   ctx->sync();
 }
@@ -118,8 +176,18 @@ void GemmBatch::finalize(Context* ctx, int tile_cfg) {
 void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas) const {
   require(finalized_, "GemmBatch: run before finalize");
   if (ntiles_ == 0) return;
-  SADMRG_HIP(launch_grouped_gemm(prec, tile_cfg_, d_tiles_.as<GemmTile>(), d_probs_.as<GemmProb>(),
-                                 d_segs_.as<GemmSeg>(), arenas, ntiles_, ctx->stream()));
+  require(!(prec == GemmPrec::Z64 && (nreduce_ > 0 || tile_cfg_ != 0)),
+          "GemmBatch: complex tables must use 64x64 tiles without split reductions");
+  GemmLaunch L;
+  L.tiles = d_tiles_.as<GemmTile>();
+  L.probs = d_probs_.as<GemmProb>();
+  L.segs = d_segs_.as<GemmSeg>();
+  L.reduce = nreduce_ > 0 ? d_reduce_.as<GemmReduce>() : nullptr;
+  L.partial = nreduce_ > 0 ? d_partial_.get() : nullptr;
+  L.arenas = arenas;
+  L.ntiles = ntiles_;
+  L.nreduce = nreduce_;
+  SADMRG_HIP(launch_grouped_gemm(prec, tile_cfg_, L, ctx->stream()));
 }
 
 }  // namespace sadmrg

@@ -35,11 +35,18 @@ class GemmBatch {
   // Algorithmic flops (real multiply-adds x2; complex x4) of the table.
   double flops(bool cplx) const;
 
-  // tile_cfg < 0 chooses from the problem shapes.
-  void finalize(Context* ctx, int tile_cfg = -1);
+  // tile_cfg < 0 chooses from the problem shapes.  allow_split lets the
+  // planner split long reductions over several workgroups when the table
+  // has too few output tiles to fill the GPU (real precisions only).
+  void finalize(Context* ctx, int tile_cfg = -1, bool allow_split = true);
   void run(Context* ctx, GemmPrec prec, const ArenaTable& arenas) const;
   int tile_cfg() const { return tile_cfg_; }
   int ntiles() const { return ntiles_; }
+  int nsplit_tiles() const { return nreduce_; }
+  int nparts() const { return nparts_; }
+
+  // Workgroups the planner aims for before splitting reductions.
+  static constexpr int kTargetWorkgroups = 2048;
 
  private:
   struct P {
@@ -48,8 +55,10 @@ class GemmBatch {
   };
   std::vector<P> probs_;
   int64_t lim_[kMaxArenas] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
-  DevBuf d_tiles_, d_probs_, d_segs_;
+  DevBuf d_tiles_, d_probs_, d_segs_, d_reduce_, d_partial_;
   int ntiles_ = 0;
+  int nreduce_ = 0;
+  int nparts_ = 0;
   int tile_cfg_ = 0;
   bool finalized_ = false;
 };

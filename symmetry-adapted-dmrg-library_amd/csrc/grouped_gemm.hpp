@@ -44,10 +44,29 @@ struct GemmProb {
   int32_t pad;
 };
 
+// One workgroup's work: output tile (m0, n0) of problem `prob`, reduced over
+// the problem's segments [seg_lo, seg_hi).  part < 0: the tile owns the whole
+// reduction and writes C directly.  part >= 0 (split reduction): the tile
+// writes its partial sum, dense TM x TN column-major, into partial slot
+// `part`; a GemmReduce task then folds the slots into C in slot order, so the
+// result does not depend on scheduling.
 struct GemmTile {
   int32_t prob;
   int32_t m0;
   int32_t n0;
+  int32_t seg_lo;
+  int32_t seg_hi;
+  int32_t part;
+};
+
+// C tile (m0, n0) of problem `prob` (=|+=) sum of partial slots
+// [part_begin, part_begin + nparts) in increasing slot order.
+struct GemmReduce {
+  int32_t prob;
+  int32_t m0;
+  int32_t n0;
+  int32_t part_begin;
+  int32_t nparts;
   int32_t pad;
 };
 

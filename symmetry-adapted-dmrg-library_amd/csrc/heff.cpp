@@ -189,8 +189,8 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
     g->limit(kArScratch, scratch_elems_);
     g->limit(kArPack, pack_elems_);
   }
-  g1_.finalize(ctx, cplx_ ? 0 : -1);
-  g2_.finalize(ctx, cplx_ ? 0 : -1);
+  g1_.finalize(ctx, cplx_ ? 0 : -1, !cplx_);
+  g2_.finalize(ctx, cplx_ ? 0 : -1, !cplx_);
   scratch64_ = DevBuf(ctx, std::max<int64_t>(scratch_elems_, 1) * 8 * w);
   stats_.n = n_;
   stats_.flops = g1_.flops(cplx_) + g2_.flops(cplx_);
@@ -203,6 +203,7 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
   stats_.tiles2 = g2_.ntiles();
   stats_.tile_cfg1 = g1_.tile_cfg();
   stats_.tile_cfg2 = g2_.tile_cfg();
+  stats_.split_tiles = g1_.nsplit_tiles() + g2_.nsplit_tiles();
   ctx->sync();
 }
 

@@ -32,6 +32,7 @@ struct HeffStats {
   int64_t stage1_segs = 0, stage2_segs = 0;
   int tiles1 = 0, tiles2 = 0;
   int tile_cfg1 = 0, tile_cfg2 = 0;
+  int split_tiles = 0;  // output tiles whose reduction is split over workgroups (both stages)
 };
 
 class HeffPlan {

@@ -31,6 +31,14 @@ template <class T> struct Acc4;
 template <> struct Acc4<double> { typedef d4_t type; };
 template <> struct Acc4<float> { typedef f4_t type; };
 
+// Output row of accumulator element r held by lane group g = lane / 16 (the
+// column is lane % 16).  The FP64 16x16x4 MFMA interleaves rows across lane
+// groups (row = g + 4r); the FP32 16x16x4 MFMA gives each lane group four
+// consecutive rows (row = 4g + r).
+template <class T> __device__ inline int acc_row(int g, int r);
+template <> __device__ inline int acc_row<double>(int g, int r) { return g + 4 * r; }
+template <> __device__ inline int acc_row<float>(int g, int r) { return 4 * g + r; }
+
 __device__ inline const void* arena_ptr(const ArenaTable& at, int64_t off, int elem_size) {
   const int a = static_cast<int>(static_cast<uint64_t>(off) >> kArenaShift);
   const int64_t o = off & ((int64_t(1) << kArenaShift) - 1);
@@ -47,7 +55,8 @@ template <class T, class TOut, int TM, int TN, int TK, int WM, int WN>
 __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
                                                            const GemmProb* __restrict__ probs,
                                                            const GemmSeg* __restrict__ segs,
-                                                           ArenaTable arenas, int tile_base) {
+                                                           ArenaTable arenas, TOut* __restrict__ partial,
+                                                           int tile_base) {
   static_assert(WM * WN == 4, "four waves per workgroup");
   constexpr int FM = TM / WM / 16;  // 16x16 fragments per wave along M
   constexpr int FN = TN / WN / 16;
@@ -124,7 +133,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __res
     }
   };
 
-  KCursor cur{prob.seg_begin, prob.seg_end, 0};
+  KCursor cur{tile.seg_lo, tile.seg_hi, 0};
   // skip empty segments
   while (cur.seg < cur.seg_end && segs[cur.seg].K <= 0) ++cur.seg;
   if (cur.seg < cur.seg_end) load_step(cur.seg, 0);
@@ -157,6 +166,22 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __res
     }
   }
 
+  if (tile.part >= 0) {
+    // split reduction: dense partial tile, reduced later in slot order
+    TOut* P = partial + int64_t(tile.part) * (TM * TN);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int n = (wn * FN + j) * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = (wm * FM + i) * 16 + acc_row<T>(lane >> 4, r);
+          P[m + n * TM] = static_cast<TOut>(acc[i][j][r]);
+        }
+      }
+    return;
+  }
   TOut* C = static_cast<TOut*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, sizeof(TOut))));
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -166,7 +191,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __res
       if (gn >= prob.N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = tile.m0 + (wm * FM + i) * 16 + 4 * (lane >> 4) + r;
+        const int gm = tile.m0 + (wm * FM + i) * 16 + acc_row<T>(lane >> 4, r);
         if (gm >= prob.M) continue;
         TOut* p = C + gm + int64_t(gn) * prob.ldc;
         const TOut v = static_cast<TOut>(acc[i][j][r]);
@@ -174,6 +199,29 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __res
         else *p = v;
       }
     }
+}
+
+// Folds split-reduction partial tiles into C: one workgroup per output tile,
+// slots summed in increasing order.
+template <class TOut, int TM, int TN>
+__global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmReduce* __restrict__ tasks,
+                                                          const GemmProb* __restrict__ probs,
+                                                          ArenaTable arenas, const TOut* __restrict__ partial) {
+  const GemmReduce t = tasks[blockIdx.x];
+  const GemmProb prob = probs[t.prob];
+  TOut* C = static_cast<TOut*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, sizeof(TOut))));
+  const int rows = min(TM, prob.M - t.m0);
+  const int cols = min(TN, prob.N - t.n0);
+  for (int idx = threadIdx.x; idx < TM * TN; idx += 256) {
+    const int m = idx % TM, n = idx / TM;
+    if (m >= rows || n >= cols) continue;
+    const TOut* P = partial + int64_t(t.part_begin) * (TM * TN) + idx;
+    TOut v = P[0];
+    for (int q = 1; q < t.nparts; ++q) v += P[int64_t(q) * (TM * TN)];
+    TOut* p = C + (t.m0 + m) + int64_t(t.n0 + n) * prob.ldc;
+    if (prob.accumulate) *p += v;
+    else *p = v;
+  }
 }
 
 // Complex grouped GEMM on interleaved (re, im) double data.  Four real
@@ -205,7 +253,7 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 #pragma unroll
     for (int j = 0; j < FN; ++j) { accr[i][j] = d4_t{0, 0, 0, 0}; acci[i][j] = d4_t{0, 0, 0, 0}; }
 
-  for (int si = prob.seg_begin; si < prob.seg_end; ++si) {
+  for (int si = tile.seg_lo; si < tile.seg_hi; ++si) {
     const GemmSeg sg = segs[si];
     const double* A = static_cast<const double*>(arena_ptr(arenas, sg.a_off, 16));
     const double* B = static_cast<const double*>(arena_ptr(arenas, sg.b_off, 16));
@@ -282,7 +330,7 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
       if (gn >= prob.N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = tile.m0 + (wm * FM + i) * 16 + 4 * (lane >> 4) + r;
+        const int gm = tile.m0 + (wm * FM + i) * 16 + acc_row<double>(lane >> 4, r);
         if (gm >= prob.M) continue;
         double* p = C + 2 * (gm + int64_t(gn) * prob.ldc);
         if (prob.accumulate) { p[0] += accr[i][j][r]; p[1] += acci[i][j][r]; }
@@ -295,34 +343,37 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 // Launch helpers.  The grid is split in chunks so that gridDim.x stays within
 // limits for very large tile tables.
 template <class T, class TOut, int TM, int TN, int TK, int WM, int WN>
-static hipError_t launch_real(const GemmTile* tiles, const GemmProb* probs, const GemmSeg* segs,
-                              const ArenaTable& arenas, int ntiles, hipStream_t st) {
+static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
+  TOut* part = static_cast<TOut*>(L.partial);
   const int chunk = 1 << 30;
-  for (int base = 0; base < ntiles; base += chunk) {
-    const int n = ntiles - base < chunk ? ntiles - base : chunk;
+  for (int base = 0; base < L.ntiles; base += chunk) {
+    const int n = L.ntiles - base < chunk ? L.ntiles - base : chunk;
     hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WM, WN>), dim3(n), dim3(256), 0, st,
-                       tiles, probs, segs, arenas, base);
+                       L.tiles, L.probs, L.segs, L.arenas, part, base);
   }
+  if (L.nreduce > 0)
+    hipLaunchKernelGGL((gemm_reduce_kernel<TOut, TM, TN>), dim3(L.nreduce), dim3(256), 0, st, L.reduce, L.probs,
+                       L.arenas, static_cast<const TOut*>(part));
   return hipGetLastError();
 }
 
-hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmTile* tiles, const GemmProb* probs,
-                               const GemmSeg* segs, const ArenaTable& arenas, int ntiles,
-                               hipStream_t st) {
-  if (ntiles <= 0) return hipSuccess;
+hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L, hipStream_t st) {
+  if (L.ntiles <= 0) return hipSuccess;
+  if (L.nreduce > 0 && L.partial == nullptr) return hipErrorInvalidValue;
   switch (prec) {
     case GemmPrec::F64:
-      if (tile_cfg == 1) return launch_real<double, double, 128, 128, 16, 2, 2>(tiles, probs, segs, arenas, ntiles, st);
-      return launch_real<double, double, 64, 64, 16, 2, 2>(tiles, probs, segs, arenas, ntiles, st);
+      if (tile_cfg == 1) return launch_real<double, double, 128, 128, 16, 2, 2>(L, st);
+      return launch_real<double, double, 64, 64, 16, 2, 2>(L, st);
     case GemmPrec::F32:
-      if (tile_cfg == 1) return launch_real<float, float, 128, 128, 16, 2, 2>(tiles, probs, segs, arenas, ntiles, st);
-      return launch_real<float, float, 64, 64, 16, 2, 2>(tiles, probs, segs, arenas, ntiles, st);
+      if (tile_cfg == 1) return launch_real<float, float, 128, 128, 16, 2, 2>(L, st);
+      return launch_real<float, float, 64, 64, 16, 2, 2>(L, st);
     case GemmPrec::F32_OUT64:
-      if (tile_cfg == 1) return launch_real<float, double, 128, 128, 16, 2, 2>(tiles, probs, segs, arenas, ntiles, st);
-      return launch_real<float, double, 64, 64, 16, 2, 2>(tiles, probs, segs, arenas, ntiles, st);
+      if (tile_cfg == 1) return launch_real<float, double, 128, 128, 16, 2, 2>(L, st);
+      return launch_real<float, double, 64, 64, 16, 2, 2>(L, st);
     case GemmPrec::Z64: {
-      hipLaunchKernelGGL((grouped_zgemm_kernel<64, 64, 16, 2, 2>), dim3(ntiles), dim3(256), 0, st, tiles,
-                         probs, segs, arenas, 0);
+      if (L.nreduce > 0 || tile_cfg != 0) return hipErrorInvalidValue;  // no split or large tiles for complex
+      hipLaunchKernelGGL((grouped_zgemm_kernel<64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st, L.tiles,
+                         L.probs, L.segs, L.arenas, 0);
       return hipGetLastError();
     }
   }

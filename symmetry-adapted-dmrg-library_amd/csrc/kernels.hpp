@@ -13,9 +13,22 @@ enum class GemmPrec : int {
   Z64 = 3,        // complex<double> operands (interleaved), 4 FP64 MFMA products
 };
 
-// tile_cfg 0: 64x64x16 tiles, 1: 128x128x16 tiles.
-hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmTile* tiles, const GemmProb* probs,
-                               const GemmSeg* segs, const ArenaTable& arenas, int ntiles, hipStream_t st);
+// Device tables of one grouped launch (see grouped_gemm.hpp).  partial holds
+// nparts dense TM x TN tiles of the output type when nreduce > 0.
+struct GemmLaunch {
+  const GemmTile* tiles = nullptr;
+  const GemmProb* probs = nullptr;
+  const GemmSeg* segs = nullptr;
+  const GemmReduce* reduce = nullptr;
+  void* partial = nullptr;
+  ArenaTable arenas{};
+  int ntiles = 0;
+  int nreduce = 0;
+};
+
+// tile_cfg 0: 64x64x16 tiles, 1: 128x128x16 tiles.  Split-reduction tables
+// run as the grouped launch followed by one reduce launch.
+hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& launch, hipStream_t st);
 
 inline int tile_m(int tile_cfg) { return tile_cfg == 1 ? 128 : 64; }
 inline int tile_n(int tile_cfg) { return tile_cfg == 1 ? 128 : 64; }

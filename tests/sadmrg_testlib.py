@@ -238,3 +238,82 @@ def heisenberg_ed(hmap, coup, nlow=1):
     H = sp.csr_matrix((vals, (rows, cols)), shape=(len(states), len(states))) + sp.diags(diag)
     w = sla.eigsh(H, k=nlow, which="SA", return_eigenvectors=False)
     return np.sort(w)
+
+
+# ---- synthetic centre-bond workload --------------------------------------------
+def rotor_dims(D, sigma, two_j_parity=0, jmax2=16):
+    js = list(range(two_j_parity, jmax2 + 1, 2))
+    w = np.array([(j + 1) * np.exp(-(j / 2) * (j / 2 + 1) / (2 * sigma ** 2)) for j in js])
+    d = np.maximum(1, np.round(D * w / w.sum())).astype(int)
+    keep = d >= 2
+    return [j for j, k in zip(js, keep) if k], [int(x) for x, k in zip(d, keep) if k]
+
+
+def fused(js, ds):
+    """fuse(block bond, spin-1/2) sector dimensions (triangle rule)."""
+    out = {}
+    for j, d in zip(js, ds):
+        for jj in (j - 1, j + 1):
+            if jj >= 0:
+                out[jj] = out.get(jj, 0) + d
+    ks = sorted(out)
+    return ks, [out[k] for k in ks]
+
+
+def build_workload(D=2048, n_op=12, sigma=1.8, seed=1):
+    """Synthetic centre-bond H_eff operands: rotor sector histogram for the
+    block bond, fused with spin-1/2 on each side; Gaussian entries."""
+    import sys as _s
+    tl = _s.modules[__name__]
+    rng = np.random.default_rng(seed)
+    js, ds = rotor_dims(D, sigma)
+    fj, fd = fused(js, ds)
+    b0 = (1, fj, fd)
+    b1 = (-1, fj, fd)
+    ch = (-1, [2], [1])
+    psi = tl.random_tensor(rng, 1, 0, 0, [b0, b1])
+    psi["data"] /= np.linalg.norm(psi["data"])
+    scale_l = 1.0 / np.sqrt(max(fd))
+    hl = tl.random_tensor(rng, 1, 0, 0, [(-1, fj, fd), b0], scale=scale_l)
+    hr = tl.random_tensor(rng, 1, 0, 0, [(1, fj, fd), b1], scale=scale_l)
+    # symmetrise the block Hamiltonians: (A + A^T) / 2 per diagonal block
+    for h in (hl, hr):
+        off = 0
+        _, _, bonds = tl.parse_header(h["header"])
+        for b in range(tl.nten(bonds)):
+            if not h["present"][b]:
+                continue
+            dims = tl.block_dims(bonds, b)
+            n = dims[0] * dims[1]
+            m = h["data"][off:off + n].reshape(dims, order="F")
+            h["data"][off:off + n] = (0.5 * (m + m.T)).ravel(order="F")
+            off += n
+    o1 = [tl.random_tensor(rng, 1, 0, 0, [(-1, fj, fd), ch, b0], scale=scale_l) for _ in range(n_op)]
+    o2 = [tl.random_tensor(rng, 1, 0, 0, [(1, fj, fd), ch, b1], scale=scale_l) for _ in range(n_op)]
+    info = {"two_j_block": js, "dims_block": ds, "two_j_fused": fj, "dims_fused": fd,
+            "n": int(sum(d * d for d in fd)), "n_op": n_op, "D": D, "sigma": sigma}
+    return dict(psi=psi, hl=hl, hr=hr, o1=o1, o2=o2, info=info)
+
+
+def reference_heff_setup(rb, wl, psize):
+    """A reference DMRG object whose step (n_op-1, n_op) applies the workload's
+    H_eff: left site j couples to right site n_op+j with coupling 1, so
+    hamiltonian_vector_multiplication_idmrg (dmrg_inst.cpp:283-371) sums the
+    n_op bond terms O1_j^T psi O2_j plus H_L^T psi + psi H_R.  Returns (d, il, ir)."""
+    n_op = wl["info"]["n_op"]
+    rb.init(physpn=1, max_angm=20, max_dcut=wl["info"]["D"], psize=psize)
+    ns = max(2 * n_op, 4)
+    d = rb.RefDMRG(1, 0, 1, ns, wl["info"]["D"], 1, 0)
+    hmap = np.zeros((ns, ns), dtype=np.int32)
+    for j in range(n_op):
+        hmap[j, n_op + j] = hmap[n_op + j, j] = 1
+    coup = np.zeros(ns + 1)
+    coup[1] = 1.0
+    d.set_couplings(hmap, coup)
+    il, ir = n_op - 1, n_op
+    d.set_tensor("hh", il, 0, to_ref(wl["hl"]))
+    d.set_tensor("hh", ir, 0, to_ref(wl["hr"]))
+    for j in range(n_op):
+        d.set_tensor("opr", il, j, to_ref(wl["o1"][j]))
+        d.set_tensor("opr", ir, n_op + j, to_ref(wl["o2"][j]))
+    return d, il, ir

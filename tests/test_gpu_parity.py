@@ -136,6 +136,21 @@ def test_heff_on_reference_environments(ctx, sad, name):
 
 
 @pytest.mark.parametrize("name", DMRG_CASES)
+def test_input_vector_matches_reference(ctx, sad, name):
+    """DMRG::prepare_input_vector (dmrg_inst.cpp:412-437) on the reference's
+    state: same sectors, same present blocks, same values."""
+    tens, extra = _load_case(name)
+    d = _engine_from_state(ctx, sad, tens, extra)
+    il, ir = (int(x) for x in extra["cap_step"])
+    got = tl.from_dev(d.input_vector(il, ir))
+    exp = tens["vec"]
+    assert list(got["header"]) == list(exp["header"])
+    assert list(got["present"]) == list(exp["present"])
+    rel, _ = tl.compare(got, exp)
+    assert rel <= 1e-12, rel
+
+
+@pytest.mark.parametrize("name", DMRG_CASES)
 def test_lanczos_and_svd_on_reference_state(ctx, sad, name):
     """Lanczos (dmrg_inst.cpp:1286-1348) energy and BlockTensor::svd singular values."""
     tens, extra = _load_case(name)
@@ -178,8 +193,8 @@ def test_full_dmrg_energies_match_reference(ctx, sad, name):
     d.idmrg()
     e0 = d.energy()
     ref0 = float(extra["e_idmrg"])
-    assert abs(e0 - ref0) <= 1e-9 * abs(ref0), (e0, ref0)
-    d.records()
+    steps = [(int(r["il"]), int(r["ir"]), r["energy"], int(r["matvec64"]), int(r["ndata"])) for r in d.records()]
+    assert abs(e0 - ref0) <= 1e-9 * abs(ref0), (e0, ref0, steps)
     energies = extra["energies"]
     for s in range(energies.shape[0]):
         d.sweep()
@@ -220,3 +235,30 @@ def test_environment_updates_replay_reference(ctx, sad, name):
             assert rel <= 1e-12, (call, site, key, rel)
             checked += 1
         assert checked >= 2, (call, site)
+
+
+@pytest.mark.parametrize("D,n_op", [(256, 3), (640, 2)])
+def test_heff_split_reduction_matches_reference(ctx, sad, cgc, D, n_op):
+    """Centre-bond H_eff at a size where the planner splits the long stage-2
+    reductions over several workgroups (partial tiles folded in slot order):
+    FP64 output equals the reference library's hamiltonian_vector_multiplication_idmrg
+    on the same tensors; the FP32-operand path stays within 2e-5."""
+    import refbind as rb
+    if not rb.available():
+        pytest.skip("oracle/_ref not built")
+    wl = tl.build_workload(D=D, n_op=n_op, seed=3)
+    psi = tl.to_dev(ctx, wl["psi"], 1)
+    hl, hr = tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"])
+    o1 = [tl.to_dev(ctx, t) for t in wl["o1"]]
+    o2 = [tl.to_dev(ctx, t) for t in wl["o2"]]
+    plan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
+    st = plan.stats()
+    assert st["split_tiles"] > 0, st
+    got = tl.from_dev(plan.apply(psi))
+    d, il, ir = tl.reference_heff_setup(rb, wl, 4)
+    exp = tl.from_ref(d.heff(il, ir, tl.to_ref(wl["psi"], 1)))
+    rel, _ = tl.compare(got, exp)
+    assert rel <= 1e-12, (rel, st)
+    got32 = tl.from_dev(plan.apply(psi, sad.FP32))
+    rel32, _ = tl.compare(got32, exp)
+    assert rel32 <= 2e-5, (rel32, st)
