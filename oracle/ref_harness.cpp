@@ -24,6 +24,7 @@
 #include <fstream>
 #include <sstream>
 #include <vector>
+#include <random>
 #include <string>
 #include <algorithm>
 #include <random>
@@ -647,6 +648,23 @@ double ref_dmrg_time_heff(void* p, int il, int ir, void* vin, int reps) {
     sec = std::chrono::duration<double>(t1 - t0).count() / reps;
   });
   return sec;
+}
+
+// Number of uniform(-1, 1) draws the reference's global generator
+// (dense_inst.cpp:23-24) has produced so far: draws one more value through
+// Dense::random_init and locates it in a fresh mt19937_64(42) stream.
+// Returns -1 if not found within maxn draws.  Consumes one draw.
+long long ref_rng_position(long long maxn) {
+  int one = 1;
+  Dense<double> t;
+  t.alloc(1, &one);
+  t.random_init();
+  const double v = t.getptr()[0];
+  std::mt19937_64 g(42);
+  std::uniform_real_distribution<double> dist(-1.0, 1.0);
+  for (long long k = 0; k < maxn; ++k)
+    if (dist(g) == v) return k;
+  return -1;
 }
 
 }  // extern "C"

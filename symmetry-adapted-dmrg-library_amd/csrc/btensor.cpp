@@ -1182,7 +1182,7 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
   const int w = vec.w();
   DevBuf du(ctx, std::max<int64_t>(u_tot, 1) * 8 * w), dvt(ctx, std::max<int64_t>(vt_tot, 1) * 8 * w);
   DevBuf ds(ctx, std::max<int64_t>(s_tot, 1) * 8), dwork(ctx, std::max<int64_t>(a_max, 1) * 8 * w);
-  DevBuf de(ctx, std::max<int64_t>(s_tot, 1) * 8 + 64), dinfo(ctx, sizeof(int) * std::max<size_t>(secs.size(), 1));
+  DevBuf dinfo(ctx, sizeof(int) * std::max<size_t>(secs.size(), 1));
   rocblas_handle h = static_cast<rocblas_handle>(ctx->solver());
   for (size_t si = 0; si < secs.size(); ++si) {
     const Sec& s = secs[si];
@@ -1191,16 +1191,15 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
                               ctx->stream()));
     rocblas_status st;
     if (!cx)
-      st = rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, s.m, s.n, dwork.as<double>(), s.m,
+      st = rocsolver_dgesdd(h, rocblas_svect_singular, rocblas_svect_singular, s.m, s.n, dwork.as<double>(), s.m,
                             ds.as<double>() + s.s_off, du.as<double>() + s.u_off, s.m, dvt.as<double>() + s.vt_off,
-                            s.mn, de.as<double>(), rocblas_outofplace, dinfo.as<int>() + si);
+                            s.mn, dinfo.as<int>() + si);
     else
-      st = rocsolver_zgesvd(h, rocblas_svect_singular, rocblas_svect_singular, s.m, s.n,
+      st = rocsolver_zgesdd(h, rocblas_svect_singular, rocblas_svect_singular, s.m, s.n,
                             dwork.as<rocblas_double_complex>(), s.m, ds.as<double>() + s.s_off,
                             du.as<rocblas_double_complex>() + s.u_off, s.m,
-                            dvt.as<rocblas_double_complex>() + s.vt_off, s.mn, de.as<double>(), rocblas_outofplace,
-                            dinfo.as<int>() + si);
-    if (st != rocblas_status_success) fail(kErrNumerics, "rocsolver gesvd failed");
+                            dvt.as<rocblas_double_complex>() + s.vt_off, s.mn, dinfo.as<int>() + si);
+    if (st != rocblas_status_success) fail(kErrNumerics, "rocsolver gesdd failed");
   }
   std::vector<double> hs(std::max<int64_t>(s_tot, 1));
   std::vector<int> hinfo(std::max<size_t>(secs.size(), 1));
@@ -1208,7 +1207,7 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
   SADMRG_HIP(hipMemcpyAsync(hinfo.data(), dinfo.get(), secs.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
   ctx->sync();
   for (size_t si = 0; si < secs.size(); ++si)
-    if (hinfo[si] != 0) fail(kErrNumerics, "Dense::svd: gesvd info=" + std::to_string(hinfo[si]));
+    if (hinfo[si] != 0) fail(kErrNumerics, "Dense::svd: gesdd info=" + std::to_string(hinfo[si]));
   // kept per sector: S/S_max > 1e-10, at least one, at most max_dcut
   std::vector<int> dc(nm1, 0);
   struct SV { double v; int sec; int idx; };

@@ -1,6 +1,8 @@
 // gemm_plan.cpp -- see gemm_plan.hpp.
 #include "gemm_plan.hpp"
 #include <algorithm>
+#include <map>
+#include <utility>
 
 namespace sadmrg {
 
@@ -42,6 +44,64 @@ double GemmBatch::flops(bool cplx) const {
   for (const auto& p : probs_)
     for (const auto& s : p.segs) f += 2.0 * p.p.M * double(p.p.N) * s.K;
   return cplx ? 4.0 * f : f;
+}
+
+// Launch order of the tiles.  Workgroups are dealt to the XCDs round-robin
+// (workgroup w runs on XCD w % kXcds), and each XCD has its own L2.  Tiles
+// that share operand panels -- the tiles of one problem and one reduction
+// split -- form a group; groups are balanced over kXcds queues by
+// longest-processing-time-first on their reduction work, and the queues are
+// interleaved so that queue x occupies workgroup slots x, x + kXcds, ...
+// Within a queue the longest reductions come first, and a group's tiles are
+// consecutive, so the workgroups resident on one XCD read the same panels.
+std::vector<GemmTile> GemmBatch::xcd_order(const std::vector<GemmTile>& tiles, const std::vector<GemmSeg>& segs) {
+  constexpr int kXcds = 8;
+  struct Group {
+    int64_t k = 0;      // reduction length of each tile
+    int64_t work = 0;   // k * tiles
+    std::vector<int> idx;
+  };
+  std::vector<Group> groups;
+  std::map<std::pair<int, int>, int> gid;
+  for (size_t i = 0; i < tiles.size(); ++i) {
+    const auto key = std::make_pair(tiles[i].prob, tiles[i].seg_lo);
+    auto it = gid.find(key);
+    int g;
+    if (it == gid.end()) {
+      g = static_cast<int>(groups.size());
+      gid.emplace(key, g);
+      groups.emplace_back();
+      for (int s = tiles[i].seg_lo; s < tiles[i].seg_hi; ++s) groups[g].k += segs[s].K;
+    } else {
+      g = it->second;
+    }
+    groups[g].idx.push_back(static_cast<int>(i));
+  }
+  std::vector<int> order(groups.size());
+  for (size_t g = 0; g < groups.size(); ++g) {
+    order[g] = static_cast<int>(g);
+    groups[g].work = groups[g].k * static_cast<int64_t>(groups[g].idx.size());
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return groups[a].k > groups[b].k; });
+  std::vector<std::vector<int>> queue(kXcds);
+  std::vector<int64_t> load(kXcds, 0);
+  for (int g : order) {
+    const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += groups[g].work + 1;
+    for (int i : groups[g].idx) queue[x].push_back(i);
+  }
+  std::vector<GemmTile> out;
+  out.reserve(tiles.size());
+  std::vector<size_t> pos(kXcds, 0);
+  for (bool more = true; more;) {
+    more = false;
+    for (int x = 0; x < kXcds; ++x)
+      if (pos[x] < queue[x].size()) {
+        out.push_back(tiles[queue[x][pos[x]++]]);
+        more = true;
+      }
+  }
+  return out;
 }
 
 void GemmBatch::finalize(Context* ctx, int tile_cfg, bool allow_split) {
@@ -128,26 +188,17 @@ void GemmBatch::finalize(Context* ctx, int tile_cfg, bool allow_split) {
     for (int n0 = 0; n0 < p.N; n0 += tn)
       for (int m0 = 0; m0 < p.M; m0 += tm) {
         if (split <= 1) {
-          ht.push_back(GemmTile{static_cast<int32_t>(pi), m0, n0, p.seg_begin, p.seg_end, -1});
+          ht.push_back(GemmTile{static_cast<int32_t>(pi), m0, n0, p.seg_begin, p.seg_end, -1, -1, 0});
           continue;
         }
+        const int red = static_cast<int>(hr.size());
         hr.push_back(GemmReduce{static_cast<int32_t>(pi), m0, n0, nparts, split, 0});
-        for (int g = 0; g < split; ++g) ht.push_back(GemmTile{static_cast<int32_t>(pi), m0, n0, cut[g], cut[g + 1], nparts + g});
+        for (int g = 0; g < split; ++g)
+          ht.push_back(GemmTile{static_cast<int32_t>(pi), m0, n0, cut[g], cut[g + 1], nparts + g, red, 0});
         nparts += split;
       }
   }
-  // Longest reductions first so the tail of the launch is short tiles.
-  auto tile_k = [&](const GemmTile& t) {
-    int64_t k = 0;
-    for (int s = t.seg_lo; s < t.seg_hi; ++s) k += hs[s].K;
-    return k;
-  };
-  std::vector<std::pair<int64_t, size_t>> order(ht.size());
-  for (size_t i = 0; i < ht.size(); ++i) order[i] = {tile_k(ht[i]), i};
-  std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-  std::vector<GemmTile> sorted(ht.size());
-  for (size_t i = 0; i < ht.size(); ++i) sorted[i] = ht[order[i].second];
-  ht.swap(sorted);
+  ht = xcd_order(ht, hs);
   ntiles_ = static_cast<int>(ht.size());
   nreduce_ = static_cast<int>(hr.size());
   nparts_ = nparts;
@@ -168,6 +219,8 @@ void GemmBatch::finalize(Context* ctx, int tile_cfg, bool allow_split) {
                               ctx->stream()));
     // room for double partials (the widest output type)
     d_partial_ = DevBuf(ctx, int64_t(nparts_) * tm * tn * sizeof(double));
+    d_counters_ = DevBuf(ctx, hr.size() * sizeof(int));
+    SADMRG_HIP(hipMemsetAsync(d_counters_.get(), 0, hr.size() * sizeof(int), ctx->stream()));
   }
   // host vectors go out of scope: make the uploads complete first
   ctx->sync();
@@ -184,6 +237,7 @@ void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas) const
   L.segs = d_segs_.as<GemmSeg>();
   L.reduce = nreduce_ > 0 ? d_reduce_.as<GemmReduce>() : nullptr;
   L.partial = nreduce_ > 0 ? d_partial_.get() : nullptr;
+  L.counters = nreduce_ > 0 ? d_counters_.as<int>() : nullptr;
   L.arenas = arenas;
   L.ntiles = ntiles_;
   L.nreduce = nreduce_;

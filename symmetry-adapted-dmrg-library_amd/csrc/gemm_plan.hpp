@@ -53,9 +53,10 @@ class GemmBatch {
     GemmProb p;
     std::vector<GemmSeg> segs;
   };
+  static std::vector<GemmTile> xcd_order(const std::vector<GemmTile>& tiles, const std::vector<GemmSeg>& segs);
   std::vector<P> probs_;
   int64_t lim_[kMaxArenas] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
-  DevBuf d_tiles_, d_probs_, d_segs_, d_reduce_, d_partial_;
+  DevBuf d_tiles_, d_probs_, d_segs_, d_reduce_, d_partial_, d_counters_;
   int ntiles_ = 0;
   int nreduce_ = 0;
   int nparts_ = 0;

@@ -48,8 +48,9 @@ struct GemmProb {
 // the problem's segments [seg_lo, seg_hi).  part < 0: the tile owns the whole
 // reduction and writes C directly.  part >= 0 (split reduction): the tile
 // writes its partial sum, dense TM x TN column-major, into partial slot
-// `part`; a GemmReduce task then folds the slots into C in slot order, so the
-// result does not depend on scheduling.
+// `part` and signs in on the counter of GemmReduce task `red`; the workgroup
+// that signs in last folds all slots of the task into C in slot order, so the
+// result does not depend on which workgroup finishes last.
 struct GemmTile {
   int32_t prob;
   int32_t m0;
@@ -57,6 +58,8 @@ struct GemmTile {
   int32_t seg_lo;
   int32_t seg_hi;
   int32_t part;
+  int32_t red;
+  int32_t pad;
 };
 
 // C tile (m0, n0) of problem `prob` (=|+=) sum of partial slots

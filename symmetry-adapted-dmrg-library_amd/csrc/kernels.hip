@@ -45,36 +45,142 @@ __device__ inline const void* arena_ptr(const ArenaTable& at, int64_t off, int e
   return static_cast<const char*>(at.base[a]) + o * elem_size;
 }
 
-// Cursor over the flattened (segment, k0) sequence of one problem.
-struct KCursor {
-  int seg, seg_end, k0;
+// ---------------------------------------------------------------------------
+// Real grouped GEMM.
+//
+// Workgroup = 256 threads = 2 x 2 waves, output tile TM x TN, K steps of TK.
+// Operand tiles are staged through LDS in the orientation they have in HBM,
+// so every global load is coalesced along the contiguous index:
+//   RK: element (r, k) at r + k*ld  (A plain, B transposed) -> lds[k*LR + r]
+//   KR: element (r, k) at k + r*ld  (A transposed, B plain) -> lds[r*LK + k]
+// ("r" is m for A and n for B).  MFMA k-slot order: in each 16-deep chunk,
+// lane group g = lane/16 feeds k = 4g + j to the j-th of four MFMAs, for A
+// and B alike, so a KR tile is read with one 4-wide LDS load per fragment.
+// Pads: LR = R + 4 puts rows 4g (g = 0..3) on distinct bank groups; LK =
+// TK + 4 (FP32) / TK + 2 (FP64) spreads the 16 rows of a fragment over all
+// banks.  LDS is double buffered with one barrier per K step; global loads
+// run two steps ahead of the MFMAs.
+template <class T, int R, int TK>
+struct OperandTile {
+  static constexpr int E = R * TK / 256;  // elements staged per thread and step
+  static constexpr int LR = R + 4;
+  static constexpr int LK = TK + (sizeof(T) == 8 ? 2 : 4);
+  static constexpr int ELEMS = (TK * LR > R * LK) ? TK * LR : R * LK;
+  static_assert(256 % R == 0 && 256 % TK == 0 && TK % 16 == 0, "tile shape");
 };
 
-// TM x TN tile, TK deep, 256 threads as WM x WN waves.
-template <class T, class TOut, int TM, int TN, int TK, int WM, int WN>
-__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
-                                                           const GemmProb* __restrict__ probs,
-                                                           const GemmSeg* __restrict__ segs,
-                                                           ArenaTable arenas, TOut* __restrict__ partial,
-                                                           int tile_base) {
-  static_assert(WM * WN == 4, "four waves per workgroup");
-  constexpr int FM = TM / WM / 16;  // 16x16 fragments per wave along M
-  constexpr int FN = TN / WN / 16;
-  constexpr int PAD = 2;
-  constexpr int A_ELEMS = TM * TK / 256;  // elements each thread stages per step
-  constexpr int B_ELEMS = TN * TK / 256;
+// Per-thread view of one operand of the current segment.
+template <class T>
+struct OperandCursor {
+  const T* safe;    // an element that always exists (the operand's first)
+  const T* p;       // this thread's element 0 of the current step
+  int64_t estride;  // element e is p[e * estride]
+  int64_t kstep;    // p advance per K step
+  int rr;           // rows left for this thread's first element (<= 0: none)
+  int kk;           // k left for this thread's first element at the current step
+  int epos;         // per-e increment of the varying index (rows for KR, k for RK)
+  bool kr;
+};
+
+template <class T, int R, int TK>
+__device__ inline void cursor_init(OperandCursor<T>& c, const T* base, int64_t ld, bool kr, int r0, int rows, int K,
+                                   int tid) {
+  c.kr = kr;
+  c.safe = base;
+  if (!kr) {  // RK
+    const int rt = tid % R, kt = tid / R;
+    c.p = base + r0 + rt + int64_t(kt) * ld;
+    c.epos = 256 / R;
+    c.estride = int64_t(c.epos) * ld;
+    c.kstep = int64_t(TK) * ld;
+    c.rr = rows - r0 - rt;
+    c.kk = K - kt;
+  } else {    // KR
+    const int kt = tid % TK, rt = tid / TK;
+    c.p = base + kt + int64_t(r0 + rt) * ld;
+    c.epos = 256 / TK;
+    c.estride = int64_t(c.epos) * ld;
+    c.kstep = TK;
+    c.rr = rows - r0 - rt;
+    c.kk = K - kt;
+  }
+}
+
+template <class T, int E>
+__device__ inline void cursor_load(const OperandCursor<T>& c, T (&v)[E]) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    // branch-free: out-of-range lanes read a valid element and drop it
+    const bool ok = c.kr ? (c.kk > 0 && e * c.epos < c.rr) : (c.rr > 0 && e * c.epos < c.kk);
+    const T x = *(ok ? c.p + e * c.estride : c.safe);
+    v[e] = ok ? x : T(0);
+  }
+}
+
+template <class T>
+__device__ inline void cursor_advance(OperandCursor<T>& c, int TK) {
+  c.p += c.kstep;
+  c.kk -= TK;
+}
+
+template <class T, int R, int TK>
+__device__ inline void tile_store(T* lds, bool kr, const T (&v)[OperandTile<T, R, TK>::E], T scale, int tid) {
+  using OT = OperandTile<T, R, TK>;
+  if (!kr) {
+    const int rt = tid % R, kt = tid / R;
+#pragma unroll
+    for (int e = 0; e < OT::E; ++e) lds[(kt + e * (256 / R)) * OT::LR + rt] = v[e] * scale;
+  } else {
+    const int kt = tid % TK, rt = tid / TK;
+#pragma unroll
+    for (int e = 0; e < OT::E; ++e) lds[(rt + e * (256 / TK)) * OT::LK + kt] = v[e] * scale;
+  }
+}
+
+// NQ consecutive k-slot values (j = j0 .. j0+NQ-1) of fragment row `row` in
+// chunk kc: one 16-byte LDS read for a KR tile.
+template <class T, int R, int TK, int NQ>
+__device__ inline void tile_frag(const T* lds, bool kr, int row, int kc, int g, int j0, T (&f)[NQ]) {
+  using OT = OperandTile<T, R, TK>;
+  static_assert(NQ * sizeof(T) == 16, "one 16-byte read per fragment");
+  if (kr) {
+    const T* q = lds + row * OT::LK + kc + 4 * g + j0;
+    if constexpr (sizeof(T) == 4) {
+      const float4 x = *reinterpret_cast<const float4*>(q);
+      f[0] = x.x; f[1] = x.y; f[2] = x.z; f[3] = x.w;
+    } else {
+      const double2 x = *reinterpret_cast<const double2*>(q);
+      f[0] = x.x; f[1] = x.y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) f[j] = lds[(kc + 4 * g + j0 + j) * OT::LR + row];
+  }
+}
+
+template <class T, class TOut, int TM, int TN, int TK>
+__global__ __launch_bounds__(256, 2) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
+                                                              const GemmProb* __restrict__ probs,
+                                                              const GemmSeg* __restrict__ segs,
+                                                              ArenaTable arenas, TOut* __restrict__ partial,
+                                                              const GemmReduce* __restrict__ reduce,
+                                                              int* __restrict__ counters, int tile_base) {
+  using TA = OperandTile<T, TM, TK>;
+  using TB = OperandTile<T, TN, TK>;
+  constexpr int FM = TM / 32;  // 16x16 fragments per wave (2 x 2 waves)
+  constexpr int FN = TN / 32;
   typedef typename Acc4<T>::type acc_t;
 
-  __shared__ T sA[TK][TM + PAD];
-  __shared__ T sB[TK][TN + PAD];
+  __shared__ __align__(16) T lds[2][TA::ELEMS + TB::ELEMS];
 
   const GemmTile tile = tiles[tile_base + blockIdx.x];
   const GemmProb prob = probs[tile.prob];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave % WM;
-  const int wn = wave / WM;
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  const int g = lane >> 4;
 
   acc_t acc[FM][FN];
 #pragma unroll
@@ -82,104 +188,145 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __res
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  T ra[A_ELEMS], rb[B_ELEMS];
-
-  // Stage the (seg, k0) step into registers.
-  auto load_step = [&](int si, int k0) {
+  // loader: segment cursor two steps ahead of the MFMAs
+  int lseg = tile.seg_lo;
+  while (lseg < tile.seg_hi && segs[lseg].K <= 0) ++lseg;
+  int lk0 = 0;
+  OperandCursor<T> ca, cb;
+  T lscale = T(1);
+  bool lkra = false, lkrb = false;
+  auto enter = [&](int si) {
     const GemmSeg sg = segs[si];
     const T* A = static_cast<const T*>(arena_ptr(arenas, sg.a_off, sizeof(T)));
     const T* B = static_cast<const T*>(arena_ptr(arenas, sg.b_off, sizeof(T)));
-    const bool ta = sg.flags & kTransA;
-    const bool tb = sg.flags & kTransB;
-    const T scale = static_cast<T>(sg.scale);
-#pragma unroll
-    for (int e = 0; e < A_ELEMS; ++e) {
-      int m, k;
-      if (!ta) { const int idx = tid + e * 256; m = idx % TM; k = idx / TM; }
-      else     { const int idx = tid + e * 256; k = idx % TK; m = idx / TK; }
-      const int gm = tile.m0 + m, gk = k0 + k;
-      T v = T(0);
-      if (gm < prob.M && gk < sg.K) v = ta ? A[gk + int64_t(gm) * sg.lda] : A[gm + int64_t(gk) * sg.lda];
-      ra[e] = v * scale;
-    }
-#pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e) {
-      int n, k;
-      if (tb) { const int idx = tid + e * 256; n = idx % TN; k = idx / TN; }
-      else    { const int idx = tid + e * 256; k = idx % TK; n = idx / TK; }
-      const int gn = tile.n0 + n, gk = k0 + k;
-      T v = T(0);
-      if (gn < prob.N && gk < sg.K) v = tb ? B[gn + int64_t(gk) * sg.ldb] : B[gk + int64_t(gn) * sg.ldb];
-      rb[e] = v;
-    }
+    lkra = (sg.flags & kTransA) != 0;
+    lkrb = (sg.flags & kTransB) == 0;
+    cursor_init<T, TM, TK>(ca, A, sg.lda, lkra, tile.m0, prob.M, sg.K, tid);
+    cursor_init<T, TN, TK>(cb, B, sg.ldb, lkrb, tile.n0, prob.N, sg.K, tid);
+    lscale = static_cast<T>(sg.scale);
   };
-  auto store_step = [&](int si) {
-    const int fl = segs[si].flags;
-    const bool ta = fl & kTransA;
-    const bool tb = fl & kTransB;
-#pragma unroll
-    for (int e = 0; e < A_ELEMS; ++e) {
-      int m, k;
-      if (!ta) { const int idx = tid + e * 256; m = idx % TM; k = idx / TM; }
-      else     { const int idx = tid + e * 256; k = idx % TK; m = idx / TK; }
-      sA[k][m] = ra[e];
+  // advance the loader past the step it just loaded; false when exhausted
+  auto next = [&]() {
+    lk0 += TK;
+    if (lk0 < segs[lseg].K) {
+      cursor_advance(ca, TK);
+      cursor_advance(cb, TK);
+      return true;
     }
-#pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e) {
-      int n, k;
-      if (tb) { const int idx = tid + e * 256; n = idx % TN; k = idx / TN; }
-      else    { const int idx = tid + e * 256; k = idx % TK; n = idx / TK; }
-      sB[k][n] = rb[e];
-    }
+    lk0 = 0;
+    ++lseg;
+    while (lseg < tile.seg_hi && segs[lseg].K <= 0) ++lseg;
+    if (lseg >= tile.seg_hi) return false;
+    enter(lseg);
+    return true;
   };
 
-  KCursor cur{tile.seg_lo, tile.seg_hi, 0};
-  // skip empty segments
-  while (cur.seg < cur.seg_end && segs[cur.seg].K <= 0) ++cur.seg;
-  if (cur.seg < cur.seg_end) load_step(cur.seg, 0);
-
-  while (cur.seg < cur.seg_end) {
-    const int this_seg = cur.seg;
-    // advance cursor to the next step
-    cur.k0 += TK;
-    if (cur.k0 >= segs[cur.seg].K) {
-      cur.k0 = 0;
-      ++cur.seg;
-      while (cur.seg < cur.seg_end && segs[cur.seg].K <= 0) ++cur.seg;
+  T ra[TA::E], rb[TB::E];
+  T pscale = T(1);
+  bool pkra = false, pkrb = false;  // orientation of the data held in ra/rb
+  unsigned skra = 0, skrb = 0;      // bit b: orientation of LDS buffer b
+  bool have = lseg < tile.seg_hi;
+  if (have) {
+    enter(lseg);
+    cursor_load<T, TA::E>(ca, ra);
+    cursor_load<T, TB::E>(cb, rb);
+    tile_store<T, TM, TK>(&lds[0][0], lkra, ra, lscale, tid);
+    tile_store<T, TN, TK>(&lds[0][TA::ELEMS], lkrb, rb, T(1), tid);
+    skra = lkra ? 1u : 0u;
+    skrb = lkrb ? 1u : 0u;
+    bool more = next();
+    if (more) {
+      cursor_load<T, TA::E>(ca, ra);
+      cursor_load<T, TB::E>(cb, rb);
+      pscale = lscale;
+      pkra = lkra;
+      pkrb = lkrb;
     }
     __syncthreads();
-    store_step(this_seg);
-    __syncthreads();
-    if (cur.seg < cur.seg_end) load_step(cur.seg, cur.k0);  // prefetch overlaps the MFMAs
+    for (int s = 0;; ++s) {
+      const int cb_ = s & 1, nb = cb_ ^ 1;
+      const bool has_next = more;
+      if (has_next) {
+        tile_store<T, TM, TK>(&lds[nb][0], pkra, ra, pscale, tid);
+        tile_store<T, TN, TK>(&lds[nb][TA::ELEMS], pkrb, rb, T(1), tid);
+        skra = (skra & ~(1u << nb)) | (pkra ? 1u << nb : 0u);
+        skrb = (skrb & ~(1u << nb)) | (pkrb ? 1u << nb : 0u);
+        more = next();
+        if (more) {
+          cursor_load<T, TA::E>(ca, ra);
+          cursor_load<T, TB::E>(cb, rb);
+          pscale = lscale;
+          pkra = lkra;
+          pkrb = lkrb;
+        }
+      }
+      const T* sa = &lds[cb_][0];
+      const T* sb = &lds[cb_][TA::ELEMS];
+      const bool kra = (skra >> cb_) & 1u, krb = (skrb >> cb_) & 1u;
+      constexpr int NQ = 16 / sizeof(T);  // k-slots per 16-byte fragment read
 #pragma unroll
-    for (int kk = 0; kk < TK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      T af[FM], bf[FN];
+      for (int kc = 0; kc < TK; kc += 16)
 #pragma unroll
-      for (int i = 0; i < FM; ++i) af[i] = sA[kr][(wm * FM + i) * 16 + (lane & 15)];
+        for (int j0 = 0; j0 < 4; j0 += NQ) {
+          T af[FM][NQ], bf[FN][NQ];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bf[j] = sB[kr][(wn * FN + j) * 16 + (lane & 15)];
+          for (int i = 0; i < FM; ++i)
+            tile_frag<T, TM, TK, NQ>(sa, kra, wm * (TM / 2) + i * 16 + (lane & 15), kc, g, j0, af[i]);
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+          for (int j = 0; j < FN; ++j)
+            tile_frag<T, TN, TK, NQ>(sb, krb, wn * (TN / 2) + j * 16 + (lane & 15), kc, g, j0, bf[j]);
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = mfma_16x16x4(af[i], bf[j], acc[i][j]);
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j) acc[i][j] = mfma_16x16x4(af[i][q], bf[j][q], acc[i][j]);
+        }
+      if (!has_next) break;
+      __syncthreads();
     }
   }
 
   if (tile.part >= 0) {
-    // split reduction: dense partial tile, reduced later in slot order
+    // split reduction: dense partial tile; the last workgroup of the task to
+    // sign in folds every slot into C in slot order
     TOut* P = partial + int64_t(tile.part) * (TM * TN);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        const int n = (wn * FN + j) * 16 + (lane & 15);
+        const int n = wn * (TN / 2) + j * 16 + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = (wm * FM + i) * 16 + acc_row<T>(lane >> 4, r);
+          const int m = wm * (TM / 2) + i * 16 + acc_row<T>(g, r);
           P[m + n * TM] = static_cast<TOut>(acc[i][j][r]);
         }
       }
+    __shared__ int s_last;
+    __threadfence();  // partial visible device-wide before signing in
+    __syncthreads();
+    const GemmReduce rt = reduce[tile.red];
+    if (tid == 0) {
+      const int prev = __hip_atomic_fetch_add(&counters[tile.red], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      s_last = (prev == rt.nparts - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // every other slot's partial is visible from here on
+    TOut* C = static_cast<TOut*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, sizeof(TOut))));
+    const int rows = min(TM, prob.M - rt.m0);
+    const int cols = min(TN, prob.N - rt.n0);
+    for (int idx = tid; idx < TM * TN; idx += 256) {
+      const int m = idx % TM, n = idx / TM;
+      if (m >= rows || n >= cols) continue;
+      const TOut* Q = partial + int64_t(rt.part_begin) * (TM * TN) + idx;
+      TOut v = Q[0];
+      for (int q = 1; q < rt.nparts; ++q) v += Q[int64_t(q) * (TM * TN)];
+      TOut* p = C + (rt.m0 + m) + int64_t(rt.n0 + n) * prob.ldc;
+      if (prob.accumulate) *p += v;
+      else *p = v;
+    }
+    if (tid == 0) __hip_atomic_store(&counters[tile.red], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   TOut* C = static_cast<TOut*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, sizeof(TOut))));
@@ -187,11 +334,11 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __res
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const int gn = tile.n0 + (wn * FN + j) * 16 + (lane & 15);
+      const int gn = tile.n0 + wn * (TN / 2) + j * 16 + (lane & 15);
       if (gn >= prob.N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = tile.m0 + (wm * FM + i) * 16 + acc_row<T>(lane >> 4, r);
+        const int gm = tile.m0 + wm * (TM / 2) + i * 16 + acc_row<T>(g, r);
         if (gm >= prob.M) continue;
         TOut* p = C + gm + int64_t(gn) * prob.ldc;
         const TOut v = static_cast<TOut>(acc[i][j][r]);
@@ -199,29 +346,6 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GemmTile* __res
         else *p = v;
       }
     }
-}
-
-// Folds split-reduction partial tiles into C: one workgroup per output tile,
-// slots summed in increasing order.
-template <class TOut, int TM, int TN>
-__global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmReduce* __restrict__ tasks,
-                                                          const GemmProb* __restrict__ probs,
-                                                          ArenaTable arenas, const TOut* __restrict__ partial) {
-  const GemmReduce t = tasks[blockIdx.x];
-  const GemmProb prob = probs[t.prob];
-  TOut* C = static_cast<TOut*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, sizeof(TOut))));
-  const int rows = min(TM, prob.M - t.m0);
-  const int cols = min(TN, prob.N - t.n0);
-  for (int idx = threadIdx.x; idx < TM * TN; idx += 256) {
-    const int m = idx % TM, n = idx / TM;
-    if (m >= rows || n >= cols) continue;
-    const TOut* P = partial + int64_t(t.part_begin) * (TM * TN) + idx;
-    TOut v = P[0];
-    for (int q = 1; q < t.nparts; ++q) v += P[int64_t(q) * (TM * TN)];
-    TOut* p = C + (t.m0 + m) + int64_t(t.n0 + n) * prob.ldc;
-    if (prob.accumulate) *p += v;
-    else *p = v;
-  }
 }
 
 // Complex grouped GEMM on interleaved (re, im) double data.  Four real
@@ -342,34 +466,32 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 // ---------------------------------------------------------------------------
 // Launch helpers.  The grid is split in chunks so that gridDim.x stays within
 // limits for very large tile tables.
-template <class T, class TOut, int TM, int TN, int TK, int WM, int WN>
+template <class T, class TOut, int TM, int TN, int TK>
 static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   TOut* part = static_cast<TOut*>(L.partial);
   const int chunk = 1 << 30;
   for (int base = 0; base < L.ntiles; base += chunk) {
     const int n = L.ntiles - base < chunk ? L.ntiles - base : chunk;
-    hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WM, WN>), dim3(n), dim3(256), 0, st,
-                       L.tiles, L.probs, L.segs, L.arenas, part, base);
+    hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK>), dim3(n), dim3(256), 0, st,
+                       L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base);
   }
-  if (L.nreduce > 0)
-    hipLaunchKernelGGL((gemm_reduce_kernel<TOut, TM, TN>), dim3(L.nreduce), dim3(256), 0, st, L.reduce, L.probs,
-                       L.arenas, static_cast<const TOut*>(part));
   return hipGetLastError();
 }
 
 hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L, hipStream_t st) {
   if (L.ntiles <= 0) return hipSuccess;
-  if (L.nreduce > 0 && L.partial == nullptr) return hipErrorInvalidValue;
+  if (L.nreduce > 0 && (L.partial == nullptr || L.reduce == nullptr || L.counters == nullptr))
+    return hipErrorInvalidValue;
   switch (prec) {
     case GemmPrec::F64:
-      if (tile_cfg == 1) return launch_real<double, double, 128, 128, 16, 2, 2>(L, st);
-      return launch_real<double, double, 64, 64, 16, 2, 2>(L, st);
+      if (tile_cfg == 1) return launch_real<double, double, 128, 128, 16>(L, st);
+      return launch_real<double, double, 64, 64, 16>(L, st);
     case GemmPrec::F32:
-      if (tile_cfg == 1) return launch_real<float, float, 128, 128, 16, 2, 2>(L, st);
-      return launch_real<float, float, 64, 64, 16, 2, 2>(L, st);
+      if (tile_cfg == 1) return launch_real<float, float, 128, 128, 32>(L, st);
+      return launch_real<float, float, 64, 64, 32>(L, st);
     case GemmPrec::F32_OUT64:
-      if (tile_cfg == 1) return launch_real<float, double, 128, 128, 16, 2, 2>(L, st);
-      return launch_real<float, double, 64, 64, 16, 2, 2>(L, st);
+      if (tile_cfg == 1) return launch_real<float, double, 128, 128, 32>(L, st);
+      return launch_real<float, double, 64, 64, 32>(L, st);
     case GemmPrec::Z64: {
       if (L.nreduce > 0 || tile_cfg != 0) return hipErrorInvalidValue;  // no split or large tiles for complex
       hipLaunchKernelGGL((grouped_zgemm_kernel<64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st, L.tiles,

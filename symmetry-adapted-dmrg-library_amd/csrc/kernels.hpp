@@ -13,21 +13,24 @@ enum class GemmPrec : int {
   Z64 = 3,        // complex<double> operands (interleaved), 4 FP64 MFMA products
 };
 
-// Device tables of one grouped launch (see grouped_gemm.hpp).  partial holds
-// nparts dense TM x TN tiles of the output type when nreduce > 0.
+// Device tables of one grouped launch (see grouped_gemm.hpp).  With split
+// reductions (nreduce > 0): partial holds nparts dense TM x TN tiles of the
+// output type, counters holds nreduce zero-initialised ints that the kernel
+// returns to zero, so one table must not run on two streams at once.
 struct GemmLaunch {
   const GemmTile* tiles = nullptr;
   const GemmProb* probs = nullptr;
   const GemmSeg* segs = nullptr;
   const GemmReduce* reduce = nullptr;
   void* partial = nullptr;
+  int* counters = nullptr;
   ArenaTable arenas{};
   int ntiles = 0;
   int nreduce = 0;
 };
 
-// tile_cfg 0: 64x64x16 tiles, 1: 128x128x16 tiles.  Split-reduction tables
-// run as the grouped launch followed by one reduce launch.
+// tile_cfg 0: 64x64x16 tiles, 1: 128x128x16 tiles.  One launch, split
+// reductions included.
 hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& launch, hipStream_t st);
 
 inline int tile_m(int tile_cfg) { return tile_cfg == 1 ? 128 : 64; }
