@@ -94,7 +94,7 @@ def cpu_baseline(wl, budget_s=20.0):
             "output": out}
 
 
-def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0):
+def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6):
     """Mixed vs FP64 sweep on the SU(2) J1-J2 cylinder (width x length, snake
     ordering).  Warm-up: iDMRG at warm[0] then one mixed sweep per entry of
     warm and one at D; the state is then cloned on the device and one sweep
@@ -118,6 +118,7 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0):
     for name, prec in (("fp64", sad.FP64), ("mixed", sad.FP32)):
         c = d.clone()
         c.set_precision(prec, refine=(prec != sad.FP64), refine_tol=refine_tol)
+        c.set_refinement(refine_krylov, 6)
         ctx.sync()
         t1 = time.perf_counter()
         c.sweep()
@@ -137,7 +138,7 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0):
 
     return {
         "lattice": f"SU(2) J1-J2 cylinder {width}x{length} (J2/J1={j2}), snake ordering, {ns} sites",
-        "D": D, "warm_up": list(warm) + [D],
+        "D": D, "warm_up": list(warm) + [D], "refine_tol": refine_tol or 1e-7, "refine_krylov": refine_krylov,
         "fp64_sweep_s": round(out["fp64"]["seconds"], 3), "mixed_sweep_s": round(out["mixed"]["seconds"], 3),
         "speedup": round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3),
         "energy_fp64": out["fp64"]["energy"], "energy_mixed": out["mixed"]["energy"],
@@ -160,6 +161,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device tensor 5 times (HBM counter calibration) and exit")
+    ap.add_argument("--refine-tol", type=float, default=0.0,
+                    help="FP64 residual tolerance of the mixed path (relative to |E|; 0: library default)")
+    ap.add_argument("--refine-krylov", type=int, default=6, help="FP32 Krylov steps per refinement restart")
     ap.add_argument("--sweep", default="6x4:512:64,128,256",
                     help="WIDTHxLENGTH:D:warm-up Ds for the mixed vs FP64 sweep comparison ('none' skips it)")
     args = ap.parse_args()
@@ -211,9 +215,11 @@ def main():
         wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
         s1 = time_stage(plan, timer, d_in, d_out, prec, 1, max(3, args.steps // 2))
         s2 = time_stage(plan, timer, d_in, d_out, prec, 2, max(3, args.steps // 2))
+        stp = plan.stats()
         res[name] = {"ms_per_step": wall_ms, "tflops": flops / (wall_ms * 1e-3) / 1e12,
                      "stage1_ms": s1, "stage2_ms": s2,
-                     "stage1_tflops": f1 / (s1 * 1e-3) / 1e12, "stage2_tflops": f2 / (s2 * 1e-3) / 1e12}
+                     "stage1_tflops": f1 / (s1 * 1e-3) / 1e12, "stage2_tflops": f2 / (s2 * 1e-3) / 1e12,
+                     "tile_cfg": [stp["tile_cfg1"], stp["tile_cfg2"]]}
         print(f"[bench] {name}: {wall_ms:.3f} ms/apply {res[name]['tflops']:.2f} TFLOP/s "
               f"(stage1 {s1:.3f} ms, stage2 {s2:.3f} ms)", file=sys.stderr, flush=True)
 
@@ -237,7 +243,8 @@ def main():
         lat, dd, warm = args.sweep.split(":")
         w_, l_ = (int(x) for x in lat.split("x"))
         print(f"[bench] sweep comparison {lat} D={dd}", file=sys.stderr, flush=True)
-        sweep = sweep_compare(sad, ctx, w_, l_, int(dd), [int(x) for x in warm.split(",")])
+        sweep = sweep_compare(sad, ctx, w_, l_, int(dd), [int(x) for x in warm.split(",")],
+                              refine_tol=args.refine_tol, refine_krylov=args.refine_krylov)
 
     dom = "stage2" if res["fp32"]["stage2_ms"] >= res["fp32"]["stage1_ms"] else "stage1"
     dom_flops = f2 if dom == "stage2" else f1
@@ -260,8 +267,9 @@ def main():
                                f"N={st['n']} reduced elements, one H_eff*psi per step",
                    "flops_per_step": flops, "sectors": wl["info"]["two_j_fused"],
                    "sector_dims": wl["info"]["dims_fused"]},
-        "fp64": {k: round(v, 4) for k, v in res["fp64"].items()},
-        "fp32": {k: round(v, 4) for k, v in res["fp32"].items()},
+        "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp64"].items()},
+        "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp32"].items()},
+        "tile_configs": "0: 64x64, 1: 128x128, 2: 128x64 output tiles (autotuned per stage)",
         "mixed_vs_fp64_speedup": round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3),
         "fp32_rel_err_vs_fp64": rel32,
         "sweep": sweep,

@@ -92,7 +92,10 @@ int sadmrg_hamiltonian_vector_multiplication(sadmrg_tensor* out, const sadmrg_te
 int sadmrg_contract(sadmrg_tensor* out, const sadmrg_tensor* t1, int i1, const sadmrg_tensor* t2, int i2);
 /* Environment updates (block_tensor.hpp:163-176).  kind: 0 overlap_initial,
    1 overlap_transformation(uu,vv,op1), 2 operator_initial(uu,vv,op1),
-   3 operator_transformation(uu,vv,op1), 4 operator_pairup(uu,vv,op1,op2). */
+   3 operator_transformation(uu,vv,op1), 4 operator_pairup(uu,vv,op1,op2),
+   5 adjoint of overlap_transformation: out = T*(op1) with the structure of
+   op2, where T(X) = overlap_transformation(uu,vv,X) (no reference
+   counterpart; real tensors). */
 int sadmrg_env_update(sadmrg_tensor* out, int kind, const sadmrg_tensor* uu, const sadmrg_tensor* vv,
                       const sadmrg_tensor* op1, const sadmrg_tensor* op2, int flag, sadmrg_cgc* cgc);
 /* BlockTensor::svd (block_tensor.hpp:135) with the global max_dcut as an
@@ -136,6 +139,11 @@ int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops);
    stage-2 tiles, stage-1 tile cfg, stage-2 tile cfg, output tiles with a
    split reduction} */
 int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n, int* info);
+/* tile-configuration autotuner of the H_eff stages (process-wide, on by
+   default): enable 0/1 (-1 leaves the setting), clear forgets the tuned
+   entries, *entries receives the table size.  No reference counterpart (the
+   reference has one BLAS path). */
+int sadmrg_autotune(int enable, int clear, int* entries);
 /* Lanczos::lanczos1 (lanczos.hpp:70) on the plan; psi is the start vector on
    entry and the Ritz vector on return.  counters[4] = {FP64 matvecs, FP32
    matvecs, Lanczos steps, refinement restarts}. */
@@ -158,6 +166,9 @@ int sadmrg_dmrg_get_couplings(sadmrg_dmrg* d, int* hmap, double* coup, int cap);
 /* the global max_dcut (main/main_dmrg_su2_real.cpp:32) */
 int sadmrg_dmrg_set_max_dcut(sadmrg_dmrg* d, int max_dcut);
 int sadmrg_dmrg_set_precision(sadmrg_dmrg* d, int precision, int refine, double refine_tol);
+/* refinement restarts of the mixed path: FP32 Krylov steps per restart and
+   the restart cap (defaults 6 and 6); no reference counterpart */
+int sadmrg_dmrg_set_refinement(sadmrg_dmrg* d, int krylov_steps, int max_restarts);
 int sadmrg_dmrg_idmrg(sadmrg_dmrg* d);         /* do_idmrg (dmrg.hpp:107) */
 int sadmrg_dmrg_prepare_sweep(sadmrg_dmrg* d); /* prepare_sweep (dmrg.hpp:110) */
 int sadmrg_dmrg_sweep(sadmrg_dmrg* d);         /* sweep (dmrg.hpp:111) */
@@ -174,6 +185,13 @@ int sadmrg_dmrg_set_tensor(sadmrg_dmrg* d, int which, int i, int j, const sadmrg
    2 prepare_site_operator_from_right(a,b), 3 prepare_site_operator_from_left(a),
    4 prepare_site_operator_from_right(a), 5 prepare_sweep() (dmrg.hpp:90-121) */
 int sadmrg_dmrg_call(sadmrg_dmrg* d, int code, int a, int b);
+/* dE/dJ_c for every coupling index c (cap entries of grad; ns+1 covers every
+   index of set_couplings' coup array) at the two-site wave function of the
+   last Lanczos solve: bonds across the two sites are measured directly,
+   bonds inside a block through the adjoint of the block-Hamiltonian
+   recursion (env kind 5).  *energy = <psi|H|psi> from the same pieces.  No
+   reference counterpart (the reference computes no gradients). */
+int sadmrg_dmrg_coupling_gradient(sadmrg_dmrg* d, double* grad, int cap, double* energy);
 /* prepare_input_vector (dmrg.hpp:121) */
 int sadmrg_dmrg_input_vector(sadmrg_dmrg* d, int il, int ir, sadmrg_tensor* out);
 /* hamiltonian_vector_multiplication_idmrg (dmrg.hpp:114) */

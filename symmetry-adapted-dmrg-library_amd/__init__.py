@@ -83,6 +83,7 @@ def lib():
         "sadmrg_heff_stage_flops": (i, [vp, i, dp]),
         "sadmrg_heff_stats": (i, [vp, dp, dp, llp, ip]),
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
+        "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_destroy": (i, [vp]),
         "sadmrg_dmrg_ns": (i, [vp, ip]),
@@ -91,6 +92,7 @@ def lib():
         "sadmrg_dmrg_get_couplings": (i, [vp, ip, dp, i]),
         "sadmrg_dmrg_set_max_dcut": (i, [vp, i]),
         "sadmrg_dmrg_set_precision": (i, [vp, i, i, d]),
+        "sadmrg_dmrg_set_refinement": (i, [vp, i, i]),
         "sadmrg_dmrg_idmrg": (i, [vp]),
         "sadmrg_dmrg_prepare_sweep": (i, [vp]),
         "sadmrg_dmrg_sweep": (i, [vp]),
@@ -100,6 +102,7 @@ def lib():
         "sadmrg_dmrg_set_tensor": (i, [vp, i, i, i, vp]),
         "sadmrg_dmrg_call": (i, [vp, i, i, i]),
         "sadmrg_dmrg_input_vector": (i, [vp, i, i, vp]),
+        "sadmrg_dmrg_coupling_gradient": (i, [vp, dp, i, dp]),
         "sadmrg_dmrg_heff": (i, [vp, i, i, vp, vp, i]),
         "sadmrg_dmrg_lanczos": (i, [vp, i, i, vp, dp]),
         "sadmrg_dmrg_svd": (i, [vp, vp, i, i, d, d, dp, i, ip]),
@@ -310,7 +313,7 @@ def contract(t1, i1, t2, i2):
     return out
 
 
-ENV_KIND = {"overlap_initial": 0, "overlap_transformation": 1, "operator_initial": 2,
+ENV_KIND = {"overlap_initial": 0, "overlap_transformation": 1, "operator_initial": 2, "overlap_adjoint": 5,
             "operator_transformation": 3, "operator_pairup": 4}
 
 
@@ -393,6 +396,13 @@ class Heff:
                          "restarts": int(cnt[3]), "residual64": res.value}
 
 
+def autotune(enable=None, clear=False):
+    """Tile-configuration autotuner (process-wide); returns the table size."""
+    n = C.c_int()
+    check(lib().sadmrg_autotune(-1 if enable is None else int(bool(enable)), int(bool(clear)), C.byref(n)))
+    return n.value
+
+
 REC_FIELDS = ("il", "ir", "energy", "trunc_err", "kept", "matvec64", "matvec32", "t_env", "t_lanczos", "t_svd",
               "heff_flops", "ndata")
 
@@ -442,6 +452,9 @@ class DMRG:
     def set_precision(self, precision=FP64, refine=False, refine_tol=0.0):
         check(lib().sadmrg_dmrg_set_precision(self.ptr, precision, 1 if refine else 0, refine_tol))
 
+    def set_refinement(self, krylov_steps=6, max_restarts=6):
+        check(lib().sadmrg_dmrg_set_refinement(self.ptr, krylov_steps, max_restarts))
+
     def idmrg(self):
         check(lib().sadmrg_dmrg_idmrg(self.ptr))
 
@@ -481,6 +494,13 @@ class DMRG:
         code = {"wavefunc_transformation": 0, "left2": 1, "right2": 2, "left1": 3, "right1": 4,
                 "prepare_sweep": 5}[name]
         check(lib().sadmrg_dmrg_call(self.ptr, code, a, b))
+
+    def coupling_gradient(self):
+        """(dE/dJ_c for c = 0..ns, <psi|H|psi>) at the last Lanczos solve."""
+        g = np.zeros(self.ns + 1)
+        e = C.c_double()
+        check(lib().sadmrg_dmrg_coupling_gradient(self.ptr, _dptr(g), len(g), C.byref(e)))
+        return g, e.value
 
     def input_vector(self, il, ir):
         t = Tensor.null(self.ctx, self.sym, self.cplx)

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <rocsolver/rocsolver.h>
+#include <map>
 #include "kernels.hpp"
 
 namespace sadmrg {
@@ -45,7 +46,7 @@ void run_batch(GemmBatch& b, Context* ctx, bool cplx, const Arenas& a) {
   if (b.empty()) return;
   for (int i = 0; i < kMaxArenas; ++i)
     if (a.n[i] >= 0) b.limit(i, a.n[i]);
-  b.finalize(ctx, cplx ? 0 : -1, !cplx);
+  b.finalize(ctx, cplx);
   b.run(ctx, prec_of(cplx), a.at);
 }
 
@@ -1150,6 +1151,99 @@ void op_random_init(BTensor& t) {
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of overlap_transformation.  The forward update is, per output
+// block m and contributing (X block j, uu block k, vv block l),
+//   T(X)_m += uu_k^T X_j vv_l            (uu_k: l x r, X_j: l x y, vv_l: y x r')
+// so <lam, T(X)> = sum_j <X_j, sum_{k,l,m} uu_k lam_m vv_l^T>, with the same
+// block matching as the forward loop nest.  Lowered as two grouped launches:
+// t_{k,m} = uu_k lam_m, then out_j = sum t_{k,m} vv_l^T.
+void op_env_adjoint(BTensor& out, const BTensor& lam, const BTensor& uu, const BTensor& vv, const BTensor& like,
+                    int flag) {
+  Context* ctx = uu.ctx;
+  require(!uu.cplx && !vv.cplx && !lam.cplx, "adjoint environment update: real tensors only");
+  require(flag == 0 || flag == 1, "adjoint environment update: flag must be 0 or 1");
+  require(uu.nbond() == 3 && vv.nbond() == 3 && lam.nbond() == 2 && like.nbond() == 2,
+          "adjoint environment update: ranks");
+  require(lam.st.bonds[0] == uu.st.bonds[2] && lam.st.bonds[1] == vv.st.bonds[2].inverted(),
+          "adjoint environment update: lam does not match the output of the forward update");
+  require(uu.st.bonds[flag] == like.st.bonds[0].inverted() && vv.st.bonds[flag] == like.st.bonds[1],
+          "adjoint environment update: `like` does not match the input of the forward update");
+  int a1[3], a2[3], a3[3];
+  struct T1 { int k, m; int64_t off; int M, N, K; };
+  std::vector<T1> t1;
+  std::map<std::pair<int, int>, int> t1_of;
+  struct Con { int t, l; };
+  std::vector<std::vector<Con>> per(like.nten());
+  int64_t scratch = 0;
+  for (int j = 0; j < like.nten(); ++j) {
+    if (!like.st.allowed(j)) continue;
+    labels_of(like, j, a1);
+    for (int k = 0; k < uu.nten(); ++k) {
+      if (!uu.st.allowed(k) || !uu.present(k)) continue;
+      labels_of(uu, k, a2);
+      if (a1[0] != a2[flag]) continue;
+      for (int l = 0; l < vv.nten(); ++l) {
+        if (!vv.st.allowed(l) || !vv.present(l)) continue;
+        labels_of(vv, l, a3);
+        if (a1[1] != a3[flag] || a2[1 - flag] != a3[1 - flag]) continue;
+        const int lab[2] = {a2[2], a3[2]};
+        if (!lam.st.allowed_labels(lab)) continue;
+        const int m = lam.st.index_of_labels(lab);
+        if (m < 0 || !lam.present(m)) continue;
+        require(dims3(uu, k, 1 - flag) == 1 && dims3(vv, l, 1 - flag) == 1,
+                "adjoint environment update: physical leg dimension > 1 is unsupported");
+        auto it = t1_of.find({k, m});
+        int ti;
+        if (it == t1_of.end()) {
+          const int l0 = dims3(uu, k, flag), r = dims3(uu, k, 2), r2 = dims3(vv, l, 2);
+          ti = static_cast<int>(t1.size());
+          t1.push_back(T1{k, m, scratch, l0, r2, r});
+          t1_of.emplace(std::make_pair(k, m), ti);
+          scratch += int64_t(l0) * r2;
+        } else {
+          ti = it->second;
+        }
+        per[j].push_back(Con{ti, l});
+      }
+    }
+  }
+  std::vector<uint8_t> mask(like.nten());
+  for (int j = 0; j < like.nten(); ++j) mask[j] = !per[j].empty();
+  out.clean();
+  out.layout(ctx, false, like.st, mask, kCgcIdent);
+  DevBuf sbuf(ctx, std::max<int64_t>(scratch, 1) * sizeof(double));
+  Arenas ar;
+  ar.set(0, uu.dptr(), uu.nelem);
+  ar.set(1, vv.dptr(), vv.nelem);
+  ar.set(2, lam.dptr(), lam.nelem);
+  ar.set(4, sbuf.get(), scratch);
+  ar.set(5, out.dptr(), out.nelem);
+  GemmBatch g1, g2;
+  for (const auto& t : t1) {
+    const int p = g1.add_problem(4, t.off, t.M, t.M, t.N, false);
+    g1.add_seg(p, view(0, uu, t.k, t.M, false), view(2, lam, t.m, t.K, false), t.K, 1.0);
+  }
+  for (int j = 0; j < like.nten(); ++j) {
+    if (per[j].empty()) continue;
+    int d[2];
+    like.st.block_dims(j, d);
+    const int p = g2.add_problem(5, out.off[j], d[0], d[0], d[1], false);
+    for (const auto& c : per[j]) {
+      const T1& t = t1[c.t];
+      require(t.M == d[0] && dims3(vv, c.l, flag) == d[1], "adjoint environment update: dimension mismatch");
+      Operand A;
+      A.arena = 4;
+      A.off = t.off;
+      A.ld = t.M;
+      g2.add_seg(p, A, view(1, vv, c.l, d[1], true), t.N, 1.0);
+    }
+  }
+  run_batch(g1, ctx, false, ar);
+  run_batch(g2, ctx, false, ar);
+  ctx->sync();
+}
+
+// ---------------------------------------------------------------------------
 // Block SVD truncation (block_tensor_inst.cpp:501-637, dense_inst.cpp:447-541).
 void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, int max_dcut,
             std::vector<double>& wout, std::vector<int>* kept_out) {
@@ -1184,7 +1278,8 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
   DevBuf ds(ctx, std::max<int64_t>(s_tot, 1) * 8), dwork(ctx, std::max<int64_t>(a_max, 1) * 8 * w);
   DevBuf dinfo(ctx, sizeof(int) * std::max<size_t>(secs.size(), 1));
   rocblas_handle h = static_cast<rocblas_handle>(ctx->solver());
-  for (size_t si = 0; si < secs.size(); ++si) {
+  // One sector, unpadded, straight into its U / V' / S slots.
+  auto single = [&](size_t si) {
     const Sec& s = secs[si];
     const int b = s.i + s.j * nm1;
     SADMRG_HIP(hipMemcpyAsync(dwork.get(), vec.block_ptr(b), int64_t(s.m) * s.n * 8 * w, hipMemcpyDeviceToDevice,
@@ -1200,7 +1295,214 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
                             du.as<rocblas_double_complex>() + s.u_off, s.m,
                             dvt.as<rocblas_double_complex>() + s.vt_off, s.mn, dinfo.as<int>() + si);
     if (st != rocblas_status_success) fail(kErrNumerics, "rocsolver gesdd failed");
+  };
+  // Sectors of similar size share one strided-batched gesdd, each padded with
+  // zeros to the batch's largest (m, n).  Zero padding appends zero singular
+  // values and leaves the leading singular triplets unchanged, their vectors
+  // vanishing on the padding, so a sector's U (m x mn) and V' (mn x n) are the
+  // leading blocks of its padded factors.  A sector whose largest singular
+  // value is round-off (kept as one state by the reference's rule) is redone
+  // unpadded, because its singular vectors are then not determined by the
+  // data and a padded solve may return one that lives on the padding.
+  std::vector<size_t> order(secs.size());
+  for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    return std::max(secs[x].m, secs[x].n) > std::max(secs[y].m, secs[y].n);
+  });
+  std::vector<uint8_t> padded(secs.size(), 0);
+  constexpr int kMaxBatch = 32;
+  // Density-matrix route for large sectors of sweep steps, where one side
+  // absorbs S (p1, p2) = (1, 0) or (0, 1): the isometric side is the
+  // eigenbasis of the Gram matrix of that side (psi^T psi for V, psi psi^T for
+  // U; strided-batched syevd, descending order), the other side is psi times
+  // it, divided by S -- the projector builder multiplies S back in, so the
+  // division is undone exactly and small S cost no accuracy there.  A Gram
+  // eigenvalue resolves S only down to ~sqrt(eps) S_max: eigenvalues below
+  // kGramRes * lambda_max are set to zero, which drops those states from the
+  // kept candidates (the SVD route keeps S/S_max > 1e-10).
+  constexpr int kGramMin = 128;
+  constexpr double kGramRes = 1e-13;
+  const bool right_gram = (p1 == 1.0 && p2 == 0.0), left_gram = (p1 == 0.0 && p2 == 1.0);
+  const bool gram = !cx && (right_gram || left_gram);
+  std::vector<uint8_t> gram_sec(secs.size(), 0);
+  std::vector<double> gram_s(s_tot, 0.0);
+  auto gram_batch = [&](size_t k0, size_t k1, int mp, int np) {
+    const int cnt = static_cast<int>(k1 - k0);
+    const int g = right_gram ? np : mp;  // Gram order
+    const int64_t sa = int64_t(mp) * np, sg = int64_t(g) * g;
+    DevBuf ba(ctx, sa * cnt * 8), bg(ctx, sg * cnt * 8), by(ctx, sa * cnt * 8), bd(ctx, int64_t(g) * cnt * 8),
+        be(ctx, int64_t(g) * cnt * 8), binfo(ctx, sizeof(int) * cnt);
+    SADMRG_HIP(launch_fill(sa * cnt, 0.0, ba.as<double>(), ctx->stream()));
+    for (int q = 0; q < cnt; ++q) {
+      const Sec& sc = secs[order[k0 + q]];
+      SADMRG_HIP(launch_copy2d(sc.m, sc.n, vec.block_ptr(sc.i + sc.j * nm1), 1, sc.m, ba.as<double>() + sa * q, mp,
+                               1, 1.0, nullptr, 0, ctx->stream()));
+    }
+    rocblas_handle rb = h;
+    const double one = 1.0, zero = 0.0;
+    rocblas_status st;
+    if (right_gram)
+      st = rocblas_dgemm_strided_batched(rb, rocblas_operation_transpose, rocblas_operation_none, np, np, mp, &one,
+                                         ba.as<double>(), mp, sa, ba.as<double>(), mp, sa, &zero, bg.as<double>(),
+                                         np, sg, cnt);
+    else
+      st = rocblas_dgemm_strided_batched(rb, rocblas_operation_none, rocblas_operation_transpose, mp, mp, np, &one,
+                                         ba.as<double>(), mp, sa, ba.as<double>(), mp, sa, &zero, bg.as<double>(),
+                                         mp, sg, cnt);
+    if (st != rocblas_status_success) fail(kErrNumerics, "rocblas gemm (Gram) failed");
+    if (rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, g, bg.as<double>(), g, sg,
+                                         bd.as<double>(), g, be.as<double>(), g, binfo.as<int>(), cnt) !=
+        rocblas_status_success)
+      fail(kErrNumerics, "rocsolver syevd (Gram) failed");
+    std::vector<double> hd(int64_t(g) * cnt);
+    std::vector<int> hi(cnt);
+    SADMRG_HIP(hipMemcpyAsync(hd.data(), bd.get(), hd.size() * 8, hipMemcpyDeviceToHost, ctx->stream()));
+    SADMRG_HIP(hipMemcpyAsync(hi.data(), binfo.get(), hi.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
+    ctx->sync();
+    // S (descending) and 1/S per sector; the eigenvectors are G's columns,
+    // ascending, so column g-1-k is the k-th largest
+    std::vector<double> inv(int64_t(g) * cnt, 0.0);
+    for (int q = 0; q < cnt; ++q) {
+      const size_t si = order[k0 + q];
+      const Sec& sc = secs[si];
+      if (hi[q] != 0) fail(kErrNumerics, "Dense::svd: syevd info=" + std::to_string(hi[q]));
+      const double* lam = hd.data() + int64_t(g) * q;
+      const double lmax = std::max(lam[g - 1], 0.0);
+      for (int k = 0; k < sc.mn; ++k) {
+        const double l = lam[g - 1 - k];
+        const double sv = (l > kGramRes * lmax && l > 0.0) ? std::sqrt(l) : 0.0;
+        gram_s[sc.s_off + k] = sv;
+        inv[int64_t(g) * q + k] = sv > 0.0 ? 1.0 / sv : 0.0;
+      }
+      gram_sec[si] = 1;
+    }
+    DevBuf dinv(ctx, inv.size() * 8);
+    SADMRG_HIP(hipMemcpyAsync(dinv.get(), inv.data(), inv.size() * 8, hipMemcpyHostToDevice, ctx->stream()));
+    if (right_gram) {
+      // Y = A V (mp x np); U = Y[:, desc] / S, V' = V[:, desc]^T
+      if (rocblas_dgemm_strided_batched(rb, rocblas_operation_none, rocblas_operation_none, mp, np, np, &one,
+                                        ba.as<double>(), mp, sa, bg.as<double>(), np, sg, &zero, by.as<double>(), mp,
+                                        sa, cnt) != rocblas_status_success)
+        fail(kErrNumerics, "rocblas gemm (Gram) failed");
+      for (int q = 0; q < cnt; ++q) {
+        const Sec& sc = secs[order[k0 + q]];
+        SADMRG_HIP(launch_copy2d(sc.m, sc.mn, by.as<double>() + sa * q + int64_t(np - 1) * mp, 1, -int64_t(mp),
+                                 du.as<double>() + sc.u_off, sc.m, 1, 1.0, dinv.as<double>() + int64_t(g) * q, 0,
+                                 ctx->stream()));
+        SADMRG_HIP(launch_copy2d(sc.mn, sc.n, bg.as<double>() + sg * q + int64_t(np - 1) * np, -int64_t(np), 1,
+                                 dvt.as<double>() + sc.vt_off, sc.mn, 1, 1.0, nullptr, 0, ctx->stream()));
+      }
+    } else {
+      // U = G[:, desc]; V' row k = (U_k^T A) / S_k, from Y = (G diag(1/S'))^T A
+      for (int q = 0; q < cnt; ++q) {
+        const Sec& sc = secs[order[k0 + q]];
+        SADMRG_HIP(launch_copy2d(sc.m, sc.mn, bg.as<double>() + sg * q + int64_t(mp - 1) * mp, 1, -int64_t(mp),
+                                 du.as<double>() + sc.u_off, sc.m, 1, 1.0, nullptr, 0, ctx->stream()));
+      }
+      // scale G's columns in place: column c (ascending) gets 1/S of rank g-1-c
+      std::vector<double> cinv(int64_t(g) * cnt, 0.0);
+      for (int q = 0; q < cnt; ++q)
+        for (int c = 0; c < g; ++c) cinv[int64_t(g) * q + c] = inv[int64_t(g) * q + (g - 1 - c)];
+      DevBuf dcinv(ctx, cinv.size() * 8);
+      SADMRG_HIP(hipMemcpyAsync(dcinv.get(), cinv.data(), cinv.size() * 8, hipMemcpyHostToDevice, ctx->stream()));
+      for (int q = 0; q < cnt; ++q)
+        SADMRG_HIP(launch_copy2d(g, g, bg.as<double>() + sg * q, 1, g, bg.as<double>() + sg * q, g, 1, 1.0,
+                                 dcinv.as<double>() + int64_t(g) * q, 0, ctx->stream()));
+      if (rocblas_dgemm_strided_batched(rb, rocblas_operation_transpose, rocblas_operation_none, mp, np, mp, &one,
+                                        bg.as<double>(), mp, sg, ba.as<double>(), mp, sa, &zero, by.as<double>(), mp,
+                                        sa, cnt) != rocblas_status_success)
+        fail(kErrNumerics, "rocblas gemm (Gram) failed");
+      for (int q = 0; q < cnt; ++q) {
+        const Sec& sc = secs[order[k0 + q]];
+        SADMRG_HIP(launch_copy2d(sc.mn, sc.n, by.as<double>() + sa * q + (mp - 1), -1, mp,
+                                 dvt.as<double>() + sc.vt_off, sc.mn, 1, 1.0, nullptr, 0, ctx->stream()));
+      }
+    }
+    ctx->sync();  // batch buffers are released at the end of this scope
+  };
+  for (size_t k0 = 0; k0 < order.size();) {
+    const int lead = std::max(secs[order[k0]].m, secs[order[k0]].n);
+    size_t k1 = k0 + 1;
+    while (k1 < order.size() && static_cast<int>(k1 - k0) < kMaxBatch &&
+           10 * std::max(secs[order[k1]].m, secs[order[k1]].n) >= 6 * lead)
+      ++k1;
+    const int cnt = static_cast<int>(k1 - k0);
+    int mp = 0, np = 0;
+    for (size_t k = k0; k < k1; ++k) {
+      mp = std::max(mp, secs[order[k]].m);
+      np = std::max(np, secs[order[k]].n);
+    }
+    if (gram && std::max(mp, np) >= kGramMin) {
+      gram_batch(k0, k1, mp, np);
+      k0 = k1;
+      continue;
+    }
+    if (cnt == 1) {
+      single(order[k0]);
+      k0 = k1;
+      continue;
+    }
+    const int kp = std::min(mp, np);
+    const int64_t sa = int64_t(mp) * np, su = int64_t(mp) * kp, sv = int64_t(kp) * np;
+    DevBuf ba(ctx, sa * cnt * 8 * w), bs(ctx, int64_t(kp) * cnt * 8), bu(ctx, su * cnt * 8 * w),
+        bv(ctx, sv * cnt * 8 * w), binfo(ctx, sizeof(int) * cnt);
+    SADMRG_HIP(launch_fill(sa * cnt * w, 0.0, ba.as<double>(), ctx->stream()));
+    for (int q = 0; q < cnt; ++q) {
+      const Sec& s = secs[order[k0 + q]];
+      SADMRG_HIP(launch_copy2d(s.m, s.n, vec.block_ptr(s.i + s.j * nm1), 1, s.m, ba.as<double>() + sa * q * w, mp, w,
+                               1.0, nullptr, 0, ctx->stream()));
+    }
+    rocblas_status st;
+    if (!cx)
+      st = rocsolver_dgesdd_strided_batched(h, rocblas_svect_singular, rocblas_svect_singular, mp, np, ba.as<double>(),
+                                            mp, sa, bs.as<double>(), kp, bu.as<double>(), mp, su, bv.as<double>(), kp,
+                                            sv, binfo.as<int>(), cnt);
+    else
+      st = rocsolver_zgesdd_strided_batched(
+          h, rocblas_svect_singular, rocblas_svect_singular, mp, np, ba.as<rocblas_double_complex>(), mp, sa,
+          bs.as<double>(), kp, bu.as<rocblas_double_complex>(), mp, su, bv.as<rocblas_double_complex>(), kp, sv,
+          binfo.as<int>(), cnt);
+    if (st != rocblas_status_success) fail(kErrNumerics, "rocsolver gesdd (batched) failed");
+    for (int q = 0; q < cnt; ++q) {
+      const size_t si = order[k0 + q];
+      const Sec& s = secs[si];
+      padded[si] = 1;
+      SADMRG_HIP(hipMemcpyAsync(ds.as<double>() + s.s_off, bs.as<double>() + int64_t(kp) * q, int64_t(s.mn) * 8,
+                                hipMemcpyDeviceToDevice, ctx->stream()));
+      SADMRG_HIP(hipMemcpyAsync(dinfo.as<int>() + si, binfo.as<int>() + q, sizeof(int), hipMemcpyDeviceToDevice,
+                                ctx->stream()));
+      SADMRG_HIP(launch_copy2d(s.m, s.mn, bu.as<double>() + su * q * w, 1, mp, du.as<double>() + s.u_off * w, s.m, w,
+                               1.0, nullptr, 0, ctx->stream()));
+      SADMRG_HIP(launch_copy2d(s.mn, s.n, bv.as<double>() + sv * q * w, 1, kp, dvt.as<double>() + s.vt_off * w, s.mn,
+                               w, 1.0, nullptr, 0, ctx->stream()));
+    }
+    ctx->sync();  // batch buffers are released at the end of this scope
+    k0 = k1;
   }
+  {
+    // redo round-off sectors of padded batches unpadded
+    std::vector<double> lead(secs.size(), 0.0);
+    for (size_t si = 0; si < secs.size(); ++si)
+      if (padded[si])
+        SADMRG_HIP(hipMemcpyAsync(&lead[si], ds.as<double>() + secs[si].s_off, 8, hipMemcpyDeviceToHost,
+                                  ctx->stream()));
+    ctx->sync();
+    for (size_t si = 0; si < secs.size(); ++si)
+      if (padded[si] && lead[si] <= 1e-16) single(si);
+  }
+  // a Gram sector without any resolved value (psi block ~ 0) is decomposed
+  // unpadded, like the SVD route's round-off sectors
+  for (size_t si = 0; si < secs.size(); ++si)
+    if (gram_sec[si] && gram_s[secs[si].s_off] == 0.0) {
+      gram_sec[si] = 0;
+      single(si);
+    }
+  for (size_t si = 0; si < secs.size(); ++si)
+    if (gram_sec[si]) {
+      SADMRG_HIP(hipMemcpyAsync(ds.as<double>() + secs[si].s_off, gram_s.data() + secs[si].s_off,
+                                int64_t(secs[si].mn) * 8, hipMemcpyHostToDevice, ctx->stream()));
+      SADMRG_HIP(hipMemsetAsync(dinfo.as<int>() + si, 0, sizeof(int), ctx->stream()));
+    }
   std::vector<double> hs(std::max<int64_t>(s_tot, 1));
   std::vector<int> hinfo(std::max<size_t>(secs.size(), 1));
   SADMRG_HIP(hipMemcpyAsync(hs.data(), ds.get(), s_tot * 8, hipMemcpyDeviceToHost, ctx->stream()));

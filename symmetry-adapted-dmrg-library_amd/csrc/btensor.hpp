@@ -83,6 +83,13 @@ enum EnvKind : int {
 void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const BTensor* op1, const BTensor* op2,
             int flag, CgcTables* cgc);
 
+// Adjoint of the overlap_transformation update (no reference counterpart;
+// used for coupling gradients).  With T(X) = overlap_transformation(uu, vv,
+// X, flag), out = T*(lam) has the structure of `like` and satisfies
+// <lam, T(X)> = <out, X> for every X of that structure.  Real tensors.
+void op_env_adjoint(BTensor& out, const BTensor& lam, const BTensor& uu, const BTensor& vv, const BTensor& like,
+                    int flag);
+
 // BlockTensor::svd (block_tensor_inst.cpp:501-637).  tu/tv are overwritten;
 // wout receives kept values (sector order) followed by discarded values.
 void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, int max_dcut,

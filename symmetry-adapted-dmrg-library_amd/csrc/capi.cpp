@@ -333,13 +333,14 @@ int sadmrg_env_update(sadmrg_tensor* out, int kind, const sadmrg_tensor* uu, con
       need(op1, "op1");
       same_kind(uu, op1);
     }
-    if (kind == 4) {
+    if (kind == 4 || kind == 5) {
       need(op2, "op2");
       same_kind(uu, op2);
     }
-    if (kind < 0 || kind > 4) fail(kErrInvalid, "env_update: kind must be 0..4");
+    if (kind < 0 || kind > 5) fail(kErrInvalid, "env_update: kind must be 0..5");
     BTensor r;
-    op_env(r, kind, uu->t, vv->t, op1 ? &op1->t : nullptr, op2 ? &op2->t : nullptr, flag, cg_of(cgc, uu->sym));
+    if (kind == 5) op_env_adjoint(r, op1->t, uu->t, vv->t, op2->t, flag);
+    else op_env(r, kind, uu->t, vv->t, op1 ? &op1->t : nullptr, op2 ? &op2->t : nullptr, flag, cg_of(cgc, uu->sym));
     out->t = std::move(r);
     fix_kind(out);
   });
@@ -539,6 +540,14 @@ int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n
   });
 }
 
+int sadmrg_autotune(int enable, int clear, int* entries) {
+  return guard([&] {
+    if (enable >= 0) set_autotune(enable != 0);
+    if (clear) autotune_clear();
+    if (entries) *entries = autotune_entries();
+  });
+}
+
 int sadmrg_lanczos(sadmrg_heff* h, sadmrg_tensor* psi, int mlanc, int precision, int refine, double refine_tol,
                    double* energy, int* counters, double* residual) {
   return guard([&] {
@@ -639,6 +648,17 @@ int sadmrg_dmrg_set_precision(sadmrg_dmrg* d, int precision, int refine, double 
   });
 }
 
+int sadmrg_dmrg_set_refinement(sadmrg_dmrg* d, int krylov_steps, int max_restarts) {
+  return guard([&] {
+    need(d, "dmrg");
+    if (krylov_steps < 1 || max_restarts < 0) fail(kErrInvalid, "set_refinement: krylov_steps >= 1, max_restarts >= 0");
+    LanczosOpts o = d->d->lanczos_opts();
+    o.refine_krylov = krylov_steps;
+    o.max_restarts = max_restarts;
+    d->d->set_lanczos(o);
+  });
+}
+
 int sadmrg_dmrg_idmrg(sadmrg_dmrg* d) {
   return guard([&] {
     need(d, "dmrg");
@@ -735,6 +755,15 @@ int sadmrg_dmrg_call(sadmrg_dmrg* d, int code, int a, int b) {
       default: fail(kErrInvalid, "dmrg_call: unknown code");
     }
     d->ctx->sync();
+  });
+}
+
+int sadmrg_dmrg_coupling_gradient(sadmrg_dmrg* d, double* grad, int cap, double* energy) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(grad, "grad");
+    const std::vector<double> g = d->d->coupling_gradient(energy);
+    for (int c = 0; c < cap; ++c) grad[c] = c < static_cast<int>(g.size()) ? g[c] : 0.0;
   });
 }
 

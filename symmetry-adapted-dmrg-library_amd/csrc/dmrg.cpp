@@ -311,6 +311,9 @@ double Dmrg::lanczos_solve(int il, int ir, BTensor& vec) {
   cur_.matvec32 = r.matvec32;
   cur_.heff_flops = plan->stats().flops;
   cur_.ndata = ndata;
+  copy_tensor(last_vec_, vec);
+  last_il_ = il;
+  last_ir_ = ir;
   cur_.t_lanczos += since(t0);
   return r.energy;
 }
@@ -327,6 +330,139 @@ void Dmrg::svd(const BTensor& vec, int a, int b, double p1, double p2) {
   cur_.t_svd += since(t0);
   rec_.push_back(cur_);
   cur_ = StepRecord{};
+}
+
+// ---- coupling gradient -------------------------------------------------------
+namespace {
+
+// dE/dH for E = <psi, H_L^T psi> (left, H[a, a'] -> sum_b psi[a, b] psi[a', b])
+// or E = <psi, psi H_R> (right, H[b, b'] -> sum_a psi[a, b] psi[a, b']), laid
+// out with the block Hamiltonian's structure.
+void block_density(BTensor& lam, const BTensor& psi, const BTensor& h, bool left) {
+  Context* ctx = psi.ctx;
+  std::vector<std::vector<int>> per(h.nten());
+  const int n0 = psi.st.bonds[0].nmom(), n1 = psi.st.bonds[1].nmom();
+  const Bond& hb0 = h.st.bonds[0];
+  const Bond& hb1 = h.st.bonds[1];
+  for (int m = 0; m < h.nten(); ++m) {
+    if (!h.st.allowed(m)) continue;
+    const int i0 = m % hb0.nmom(), i1 = m / hb0.nmom();
+    if (hb0.q[i0] != hb1.q[i1]) continue;
+    for (int b = 0; b < psi.nten(); ++b) {
+      if (!psi.present(b)) continue;
+      const int r = b % n0, c = b / n0;
+      const int q = left ? psi.st.bonds[0].q[r] : psi.st.bonds[1].q[c];
+      if (q == hb0.q[i0]) per[m].push_back(b);
+    }
+  }
+  std::vector<uint8_t> mask(h.nten());
+  for (int m = 0; m < h.nten(); ++m) mask[m] = !per[m].empty();
+  lam.clean();
+  lam.layout(ctx, false, h.st, mask, kCgcIdent);
+  GemmBatch g;
+  for (int m = 0; m < h.nten(); ++m) {
+    if (per[m].empty()) continue;
+    int d[2];
+    h.st.block_dims(m, d);
+    const int p = g.add_problem(1, lam.off[m], d[0], d[0], d[1], false);
+    for (int b : per[m]) {
+      int pd[2];
+      psi.st.block_dims(b, pd);
+      Operand A, B;
+      A.arena = 0;
+      A.off = psi.off[b];
+      A.ld = pd[0];
+      B = A;
+      if (left) {  // psi_b psi_b^T
+        require(pd[0] == d[0], "block density: dimension mismatch");
+        B.trans = true;
+        g.add_seg(p, A, B, pd[1], 1.0);
+      } else {     // psi_b^T psi_b
+        require(pd[1] == d[0], "block density: dimension mismatch");
+        A.trans = true;
+        g.add_seg(p, A, B, pd[0], 1.0);
+      }
+    }
+  }
+  g.limit(0, psi.nelem);
+  g.limit(1, lam.nelem);
+  g.finalize(ctx, false);
+  ArenaTable at{};
+  at.base[0] = psi.dptr();
+  at.base[1] = lam.dptr();
+  g.run(ctx, GemmPrec::F64, at);
+  ctx->sync();
+}
+
+}  // namespace
+
+std::vector<double> Dmrg::coupling_gradient(double* energy) {
+  require(last_il_ >= 0 && !last_vec_.is_null(), "coupling gradient: no Lanczos solve has run");
+  require(!cplx_, "coupling gradient: real tensors only");
+  const int il = last_il_, ir = last_ir_;
+  const BTensor& psi = last_vec_;
+  CgcTables* cg = cgc_.get();
+  std::vector<double> grad(coup_.size(), 0.0);
+  double e = 0;
+  // bonds across the two sites: <psi| O1_j^T psi O2_k>
+  for (int j = 0; j <= il; ++j)
+    for (int k = ir; k < ns_; ++k) {
+      if (hm(j, k) <= 0 || opr_[il][j].is_null() || opr_[ir][k].is_null()) continue;
+      BTensor t;
+      op_hvm(t, psi, opr_[il][j], opr_[ir][k], cg);
+      if (t.is_null()) continue;
+      const double v = op_dot(psi, t).real();
+      grad[hm(j, k)] += v;
+      e += coup_[hm(j, k)] * v;
+    }
+  // left block
+  if (!hh_[il].is_null()) {
+    BTensor lam;
+    block_density(lam, psi, hh_[il], true);
+    e += op_dot(lam, hh_[il]).real();
+    for (int i = il; i >= 1; --i) {
+      BTensor u1;
+      if (i == il) op_fuse(u1, ctx_, cplx_, sym_, uu_[i].st.bonds[0], uu_[i].st.bonds[1]);
+      const BTensor& basis = (i == il) ? u1 : uu_[i];
+      for (int k = 0; k < i; ++k) {
+        if (hm(i, k) <= 0 || opr_[i - 1][k].is_null()) continue;
+        BTensor ht;
+        op_env(ht, kOperatorPairup, basis, basis, &opr_[i - 1][k], &sigma_[1], 0, cg);
+        if (ht.is_null()) continue;
+        require(ht.st == lam.st, "coupling gradient: pair-up term and adjoint differ in structure");
+        grad[hm(i, k)] += op_dot(lam, ht).real();
+      }
+      if (i - 1 < 1 || hh_[i - 1].is_null()) break;
+      BTensor nl;
+      op_env_adjoint(nl, lam, basis, basis, hh_[i - 1], 0);
+      lam = std::move(nl);
+    }
+  }
+  // right block
+  if (!hh_[ir].is_null()) {
+    BTensor lam;
+    block_density(lam, psi, hh_[ir], false);
+    e += op_dot(lam, hh_[ir]).real();
+    for (int i = ir; i <= ns_ - 2; ++i) {
+      BTensor u1;
+      if (i == ir) op_fuse(u1, ctx_, cplx_, sym_, uu_[i].st.bonds[0], uu_[i].st.bonds[1]);
+      const BTensor& basis = (i == ir) ? u1 : uu_[i];
+      for (int k = i + 1; k < ns_; ++k) {
+        if (hm(i, k) <= 0 || opr_[i + 1][k].is_null()) continue;
+        BTensor ht;
+        op_env(ht, kOperatorPairup, basis, basis, &opr_[i + 1][k], &sigma_[0], 1, cg);
+        if (ht.is_null()) continue;
+        require(ht.st == lam.st, "coupling gradient: pair-up term and adjoint differ in structure");
+        grad[hm(i, k)] += op_dot(lam, ht).real();
+      }
+      if (i + 1 > ns_ - 2 || hh_[i + 1].is_null()) break;
+      BTensor nl;
+      op_env_adjoint(nl, lam, basis, basis, hh_[i + 1], 1);
+      lam = std::move(nl);
+    }
+  }
+  if (energy) *energy = e;
+  return grad;
 }
 
 // ---- drivers ------------------------------------------------------------------

@@ -60,6 +60,16 @@ class Dmrg {
   double lanczos_solve(int il, int ir, BTensor& vec);
   void svd(const BTensor& vec, int a, int b, double p1, double p2);
 
+  // dE/dJ_c for every coupling index c (Hellmann-Feynman, <psi| dH/dJ_c |psi>)
+  // at the two-site wave function of the last Lanczos solve, with the
+  // environments of that step: bonds across the two sites are measured
+  // directly; bonds inside a block through the adjoint of the block-
+  // Hamiltonian recursion (lam_{i-1} = T_i^*(lam_i), starting from
+  // lam = dE/dH_block) contracted with each bond's pair-up term.  *energy
+  // receives <psi|H|psi> assembled from the same pieces.  No reference
+  // counterpart.
+  std::vector<double> coupling_gradient(double* energy);
+
   BTensor& uu(int i) { return uu_[i]; }
   BTensor& hh(int i) { return hh_[i]; }
   BTensor& opr(int i, int j) { return opr_[i][j]; }
@@ -86,6 +96,8 @@ class Dmrg {
   std::unique_ptr<CgcTables> cgc_;
   LanczosOpts lopt_;
   double gs_enr_ = 0;
+  BTensor last_vec_;  // Ritz vector of the last Lanczos solve
+  int last_il_ = -1, last_ir_ = -1;
   std::vector<double> wout_;
   std::vector<StepRecord> rec_;
   StepRecord cur_{};

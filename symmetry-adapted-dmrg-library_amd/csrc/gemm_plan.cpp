@@ -34,6 +34,7 @@ void GemmBatch::add_seg(int prob, const Operand& A, const Operand& B, int K, dou
 }
 
 int64_t GemmBatch::nseg() const {
+  if (finalized_) return static_cast<int64_t>(hs_.size());
   int64_t n = 0;
   for (const auto& p : probs_) n += static_cast<int64_t>(p.segs.size());
   return n;
@@ -41,8 +42,13 @@ int64_t GemmBatch::nseg() const {
 
 double GemmBatch::flops(bool cplx) const {
   double f = 0;
-  for (const auto& p : probs_)
-    for (const auto& s : p.segs) f += 2.0 * p.p.M * double(p.p.N) * s.K;
+  if (finalized_) {
+    for (const auto& p : hp_)
+      for (int s = p.seg_begin; s < p.seg_end; ++s) f += 2.0 * p.M * double(p.N) * hs_[s].K;
+  } else {
+    for (const auto& p : probs_)
+      for (const auto& s : p.segs) f += 2.0 * p.p.M * double(p.p.N) * s.K;
+  }
   return cplx ? 4.0 * f : f;
 }
 
@@ -104,22 +110,36 @@ std::vector<GemmTile> GemmBatch::xcd_order(const std::vector<GemmTile>& tiles, c
   return out;
 }
 
-void GemmBatch::finalize(Context* ctx, int tile_cfg, bool allow_split) {
+void GemmBatch::finalize(Context* ctx, bool complex) {
   require(!finalized_, "GemmBatch: finalize twice");
   finalized_ = true;
-  if (tile_cfg < 0) {
-    // large tiles when the table is dominated by big problems
-    double big = 0, all = 0;
+  complex_ = complex;
+  // large tiles when the table is dominated by big problems
+  double big = 0, all = 0;
+  for (const auto& p : probs_) {
+    int64_t k = 0;
+    for (const auto& s : p.segs) k += s.K;
+    const double w = double(p.p.M) * p.p.N * double(k);
+    all += w;
+    if (p.p.M >= 256 && p.p.N >= 256) big += w;
+  }
+  big_ = all > 0 && big / all > 0.6;
+  {
+    // work fractions (eighths) in min(M, N) classes <64, <128, <256, <512, >=512
+    double cls[5] = {0, 0, 0, 0, 0};
     for (const auto& p : probs_) {
       int64_t k = 0;
       for (const auto& s : p.segs) k += s.K;
-      const double w = double(p.p.M) * p.p.N * double(k);
-      all += w;
-      if (p.p.M >= 256 && p.p.N >= 256) big += w;
+      const int mn = std::min(p.p.M, p.p.N);
+      const int c = mn < 64 ? 0 : mn < 128 ? 1 : mn < 256 ? 2 : mn < 512 ? 3 : 4;
+      cls[c] += double(p.p.M) * p.p.N * double(k);
     }
-    tile_cfg = (all > 0 && big / all > 0.6) ? 1 : 0;
+    uint64_t key = 0;
+    for (int c = 0; c < 5; ++c) key = key * 9 + (all > 0 ? static_cast<uint64_t>(8.0 * cls[c] / all + 0.5) : 0);
+    int lg = 0;
+    for (double f = all; f >= 2.0 && lg < 63; f /= 2.0) ++lg;
+    key_ = key * 64 + static_cast<uint64_t>(lg / 2);
   }
-  tile_cfg_ = tile_cfg;
   // bounds check of every access against the declared arena sizes
   auto check = [&](int64_t aoff, int64_t rows, int64_t cols, int64_t ld, bool trans, const char* what) {
     const int a = static_cast<int>(static_cast<uint64_t>(aoff) >> kArenaShift);
@@ -137,47 +157,69 @@ void GemmBatch::finalize(Context* ctx, int tile_cfg, bool allow_split) {
       check(s.b_off, s.K, p.p.N, s.ldb, (s.flags & kTransB) != 0, "B");
     }
   }
-  const int tm = tile_m(tile_cfg), tn = tile_n(tile_cfg);
-  std::vector<GemmProb> hp;
-  std::vector<GemmSeg> hs;
-  hp.reserve(probs_.size());
-  std::vector<int64_t> klen;
-  int64_t base_tiles = 0;
+  hp_.reserve(probs_.size());
   for (size_t i = 0; i < probs_.size(); ++i) {
     GemmProb p = probs_[i].p;
     if (p.M <= 0 || p.N <= 0) continue;
-    p.seg_begin = static_cast<int32_t>(hs.size());
+    p.seg_begin = static_cast<int32_t>(hs_.size());
     int64_t k = 0;
     for (const auto& s : probs_[i].segs) {
-      hs.push_back(s);
+      hs_.push_back(s);
       k += s.K;
     }
-    p.seg_end = static_cast<int32_t>(hs.size());
-    hp.push_back(p);
-    klen.push_back(k);
-    base_tiles += int64_t((p.M + tm - 1) / tm) * ((p.N + tn - 1) / tn);
+    p.seg_end = static_cast<int32_t>(hs_.size());
+    hp_.push_back(p);
+    klen_.push_back(k);
   }
+  probs_.clear();
+  probs_.shrink_to_fit();
+  if (hp_.empty()) return;
+  d_probs_ = DevBuf(ctx, hp_.size() * sizeof(GemmProb));
+  d_segs_ = DevBuf(ctx, std::max<size_t>(hs_.size(), 1) * sizeof(GemmSeg));
+  SADMRG_HIP(hipMemcpyAsync(d_probs_.get(), hp_.data(), hp_.size() * sizeof(GemmProb), hipMemcpyHostToDevice,
+                            ctx->stream()));
+  if (!hs_.empty())
+    SADMRG_HIP(hipMemcpyAsync(d_segs_.get(), hs_.data(), hs_.size() * sizeof(GemmSeg), hipMemcpyHostToDevice,
+                              ctx->stream()));
+  ctx->sync();
+}
+
+int GemmBatch::default_cfg(GemmPrec prec) const {
+  if (complex_ || !big_) return 0;
+  return prec == GemmPrec::F64 ? 2 : 1;
+}
+
+GemmBatch::TileSet& GemmBatch::tileset(Context* ctx, int cfg) {
+  require(finalized_, "GemmBatch: run before finalize");
+  require(cfg >= 0 && cfg < kTileConfigs, "GemmBatch: bad tile configuration");
+  require(!(complex_ && cfg != 0), "GemmBatch: complex tables use tile configuration 0");
+  TileSet& ts = sets_[cfg];
+  if (ts.built) return ts;
+  ts.built = true;
+  const int tm = tile_m(cfg), tn = tile_n(cfg);
+  int64_t base_tiles = 0;
+  for (const auto& p : hp_) base_tiles += int64_t((p.M + tm - 1) / tm) * ((p.N + tn - 1) / tn);
   // Split factor: enough workgroups to fill the GPU, each keeping at least
   // kMinSplitK of reduction depth, cut at segment boundaries by cumulative K.
   constexpr int64_t kMinSplitK = 512;
   int want = 1;
-  if (allow_split && base_tiles > 0 && base_tiles < kTargetWorkgroups)
+  if (!complex_ && base_tiles > 0 && base_tiles < kTargetWorkgroups)
     want = static_cast<int>((kTargetWorkgroups + base_tiles - 1) / base_tiles);
   std::vector<GemmTile> ht;
   std::vector<GemmReduce> hr;
   int nparts = 0;
-  for (size_t pi = 0; pi < hp.size(); ++pi) {
-    const GemmProb& p = hp[pi];
+  for (size_t pi = 0; pi < hp_.size(); ++pi) {
+    const GemmProb& p = hp_[pi];
     const int nseg = p.seg_end - p.seg_begin;
-    int split = std::min<int64_t>({int64_t(want), int64_t(nseg), klen[pi] / kMinSplitK});
+    int split = static_cast<int>(std::min<int64_t>({int64_t(want), int64_t(nseg), klen_[pi] / kMinSplitK}));
     // segment groups [cut[g], cut[g+1]) with near-equal K
     std::vector<int> cut{p.seg_begin};
     if (split > 1) {
       int64_t acc = 0;
       int g = 1;
       for (int s = p.seg_begin; s < p.seg_end && g < split; ++s) {
-        acc += hs[s].K;
-        if (acc * split >= klen[pi] * g && s + 1 < p.seg_end) {
+        acc += hs_[s].K;
+        if (acc * split >= klen_[pi] * g && s + 1 < p.seg_end) {
           cut.push_back(s + 1);
           ++g;
         }
@@ -198,50 +240,49 @@ void GemmBatch::finalize(Context* ctx, int tile_cfg, bool allow_split) {
         nparts += split;
       }
   }
-  ht = xcd_order(ht, hs);
-  ntiles_ = static_cast<int>(ht.size());
-  nreduce_ = static_cast<int>(hr.size());
-  nparts_ = nparts;
-  if (ntiles_ == 0) return;
-  d_probs_ = DevBuf(ctx, hp.size() * sizeof(GemmProb));
-  d_segs_ = DevBuf(ctx, std::max<size_t>(hs.size(), 1) * sizeof(GemmSeg));
-  d_tiles_ = DevBuf(ctx, ht.size() * sizeof(GemmTile));
-  SADMRG_HIP(hipMemcpyAsync(d_probs_.get(), hp.data(), hp.size() * sizeof(GemmProb), hipMemcpyHostToDevice,
+  ht = xcd_order(ht, hs_);
+  ts.ntiles = static_cast<int>(ht.size());
+  ts.nreduce = static_cast<int>(hr.size());
+  ts.nparts = nparts;
+  if (ts.ntiles == 0) return ts;
+  ts.tiles = DevBuf(ctx, ht.size() * sizeof(GemmTile));
+  SADMRG_HIP(hipMemcpyAsync(ts.tiles.get(), ht.data(), ht.size() * sizeof(GemmTile), hipMemcpyHostToDevice,
                             ctx->stream()));
-  if (!hs.empty())
-    SADMRG_HIP(hipMemcpyAsync(d_segs_.get(), hs.data(), hs.size() * sizeof(GemmSeg), hipMemcpyHostToDevice,
-                              ctx->stream()));
-  SADMRG_HIP(hipMemcpyAsync(d_tiles_.get(), ht.data(), ht.size() * sizeof(GemmTile), hipMemcpyHostToDevice,
-                            ctx->stream()));
-  if (nreduce_ > 0) {
-    d_reduce_ = DevBuf(ctx, hr.size() * sizeof(GemmReduce));
-    SADMRG_HIP(hipMemcpyAsync(d_reduce_.get(), hr.data(), hr.size() * sizeof(GemmReduce), hipMemcpyHostToDevice,
+  if (ts.nreduce > 0) {
+    ts.reduce = DevBuf(ctx, hr.size() * sizeof(GemmReduce));
+    SADMRG_HIP(hipMemcpyAsync(ts.reduce.get(), hr.data(), hr.size() * sizeof(GemmReduce), hipMemcpyHostToDevice,
                               ctx->stream()));
     // room for double partials (the widest output type)
-    d_partial_ = DevBuf(ctx, int64_t(nparts_) * tm * tn * sizeof(double));
-    d_counters_ = DevBuf(ctx, hr.size() * sizeof(int));
-    SADMRG_HIP(hipMemsetAsync(d_counters_.get(), 0, hr.size() * sizeof(int), ctx->stream()));
+    ts.partial = DevBuf(ctx, int64_t(nparts) * tm * tn * sizeof(double));
+    ts.counters = DevBuf(ctx, hr.size() * sizeof(int));
+    SADMRG_HIP(hipMemsetAsync(ts.counters.get(), 0, hr.size() * sizeof(int), ctx->stream()));
   }
   // host vectors go out of scope: make the uploads complete first
   ctx->sync();
+  return ts;
 }
 
-void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas) const {
+int GemmBatch::ntiles(Context* ctx, int cfg) { return tileset(ctx, cfg).ntiles; }
+int GemmBatch::nsplit_tiles(Context* ctx, int cfg) { return tileset(ctx, cfg).nreduce; }
+
+void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int cfg) {
   require(finalized_, "GemmBatch: run before finalize");
-  if (ntiles_ == 0) return;
-  require(!(prec == GemmPrec::Z64 && (nreduce_ > 0 || tile_cfg_ != 0)),
-          "GemmBatch: complex tables must use 64x64 tiles without split reductions");
+  if (hp_.empty()) return;
+  if (cfg < 0) cfg = default_cfg(prec);
+  require(!(prec == GemmPrec::Z64) || complex_, "GemmBatch: complex precision on a real table");
+  const TileSet& ts = tileset(ctx, cfg);
+  if (ts.ntiles == 0) return;
   GemmLaunch L;
-  L.tiles = d_tiles_.as<GemmTile>();
+  L.tiles = ts.tiles.as<GemmTile>();
   L.probs = d_probs_.as<GemmProb>();
   L.segs = d_segs_.as<GemmSeg>();
-  L.reduce = nreduce_ > 0 ? d_reduce_.as<GemmReduce>() : nullptr;
-  L.partial = nreduce_ > 0 ? d_partial_.get() : nullptr;
-  L.counters = nreduce_ > 0 ? d_counters_.as<int>() : nullptr;
+  L.reduce = ts.nreduce > 0 ? ts.reduce.as<GemmReduce>() : nullptr;
+  L.partial = ts.nreduce > 0 ? ts.partial.get() : nullptr;
+  L.counters = ts.nreduce > 0 ? ts.counters.as<int>() : nullptr;
   L.arenas = arenas;
-  L.ntiles = ntiles_;
-  L.nreduce = nreduce_;
-  SADMRG_HIP(launch_grouped_gemm(prec, tile_cfg_, L, ctx->stream()));
+  L.ntiles = ts.ntiles;
+  L.nreduce = ts.nreduce;
+  SADMRG_HIP(launch_grouped_gemm(prec, cfg, L, ctx->stream()));
 }
 
 }  // namespace sadmrg

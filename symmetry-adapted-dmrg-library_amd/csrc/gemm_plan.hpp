@@ -1,12 +1,15 @@
 // gemm_plan.hpp -- host builder for grouped GEMM block tables.
 //
 // A GemmBatch collects output problems (one per output block) and, per
-// problem, an ordered list of K segments.  finalize() flattens them into the
-// device tables of grouped_gemm.hpp, picks the tile shape and uploads; run()
-// launches one grouped kernel.  Tables persist, so a batch built once (e.g.
-// the H_eff plan of one Lanczos solve) is re-run with new arena bases.
+// problem, an ordered list of K segments.  finalize() checks every access
+// against the declared arena sizes and uploads the problem and segment
+// tables; the tile list of a tile configuration (grouped_gemm.hpp) is built
+// and uploaded the first time that configuration runs.  Tables persist, so a
+// batch built once (e.g. the H_eff plan of one Lanczos solve) is re-run with
+// new arena bases, in either precision and any tile configuration.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <vector>
 #include "core.hpp"
 #include "grouped_gemm.hpp"
@@ -29,21 +32,30 @@ class GemmBatch {
   // Declares the element count of an arena; finalize() rejects any problem or
   // segment whose extent leaves a declared arena (host-side bounds check).
   void limit(int arena, int64_t elems) { lim_[arena] = elems; }
-  int nprob() const { return static_cast<int>(probs_.size()); }
+  int nprob() const { return finalized_ ? static_cast<int>(hp_.size()) : static_cast<int>(probs_.size()); }
   int64_t nseg() const;
   bool empty() const { return probs_.empty(); }
   // Algorithmic flops (real multiply-adds x2; complex x4) of the table.
   double flops(bool cplx) const;
 
-  // tile_cfg < 0 chooses from the problem shapes.  allow_split lets the
-  // planner split long reductions over several workgroups when the table
-  // has too few output tiles to fill the GPU (real precisions only).
-  void finalize(Context* ctx, int tile_cfg = -1, bool allow_split = true);
-  void run(Context* ctx, GemmPrec prec, const ArenaTable& arenas) const;
-  int tile_cfg() const { return tile_cfg_; }
-  int ntiles() const { return ntiles_; }
-  int nsplit_tiles() const { return nreduce_; }
-  int nparts() const { return nparts_; }
+  // complex: complex tables (configuration 0 only, no split reductions).
+  void finalize(Context* ctx, bool complex);
+  // cfg < 0: the default configuration for prec (default_cfg).
+  void run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int cfg = -1);
+  // Default tile configuration for a precision, from the problem shapes:
+  // large tiles when most of the work is in problems >= 256 x 256 (128 x 128
+  // for FP32 operands, 128 x 64 for FP64, whose wider accumulators would
+  // otherwise cost occupancy).
+  int default_cfg(GemmPrec prec) const;
+  int big_tiles() const { return big_; }
+  // Coarse signature of the problem shapes (work fractions per size class of
+  // min(M, N), plus the table's size class): equal keys mark tables for which
+  // one tuned tile configuration serves.
+  uint64_t shape_key() const { return key_; }
+  bool complex() const { return complex_; }
+  // Tiles / split output tiles of a configuration (builds it if needed).
+  int ntiles(Context* ctx, int cfg);
+  int nsplit_tiles(Context* ctx, int cfg);
 
   // Workgroups the planner aims for before splitting reductions.
   static constexpr int kTargetWorkgroups = 2048;
@@ -53,14 +65,25 @@ class GemmBatch {
     GemmProb p;
     std::vector<GemmSeg> segs;
   };
+  struct TileSet {
+    bool built = false;
+    DevBuf tiles, reduce, partial, counters;
+    int ntiles = 0, nreduce = 0, nparts = 0;
+  };
+  TileSet& tileset(Context* ctx, int cfg);
   static std::vector<GemmTile> xcd_order(const std::vector<GemmTile>& tiles, const std::vector<GemmSeg>& segs);
+
   std::vector<P> probs_;
   int64_t lim_[kMaxArenas] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
-  DevBuf d_tiles_, d_probs_, d_segs_, d_reduce_, d_partial_, d_counters_;
-  int ntiles_ = 0;
-  int nreduce_ = 0;
-  int nparts_ = 0;
-  int tile_cfg_ = 0;
+  // flattened tables (host copies kept for building tile sets)
+  std::vector<GemmProb> hp_;
+  std::vector<GemmSeg> hs_;
+  std::vector<int64_t> klen_;
+  DevBuf d_probs_, d_segs_;
+  TileSet sets_[kTileConfigs];
+  bool complex_ = false;
+  bool big_ = false;
+  uint64_t key_ = 0;
   bool finalized_ = false;
 };
 

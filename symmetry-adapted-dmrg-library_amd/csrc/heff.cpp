@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include "kernels.hpp"
 
 namespace sadmrg {
@@ -189,8 +190,8 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
     g->limit(kArScratch, scratch_elems_);
     g->limit(kArPack, pack_elems_);
   }
-  g1_.finalize(ctx, cplx_ ? 0 : -1, !cplx_);
-  g2_.finalize(ctx, cplx_ ? 0 : -1, !cplx_);
+  g1_.finalize(ctx, cplx_);
+  g2_.finalize(ctx, cplx_);
   scratch64_ = DevBuf(ctx, std::max<int64_t>(scratch_elems_, 1) * 8 * w);
   stats_.n = n_;
   stats_.flops = g1_.flops(cplx_) + g2_.flops(cplx_);
@@ -199,12 +200,77 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
   stats_.stage2_probs = g2_.nprob();
   stats_.stage1_segs = g1_.nseg();
   stats_.stage2_segs = g2_.nseg();
-  stats_.tiles1 = g1_.ntiles();
-  stats_.tiles2 = g2_.ntiles();
-  stats_.tile_cfg1 = g1_.tile_cfg();
-  stats_.tile_cfg2 = g2_.tile_cfg();
-  stats_.split_tiles = g1_.nsplit_tiles() + g2_.nsplit_tiles();
+  const GemmPrec p64 = cplx_ ? GemmPrec::Z64 : GemmPrec::F64;
+  stats_.tile_cfg1 = g1_.default_cfg(p64);
+  stats_.tile_cfg2 = g2_.default_cfg(p64);
+  stats_.tiles1 = g1_.ntiles(ctx, stats_.tile_cfg1);
+  stats_.tiles2 = g2_.ntiles(ctx, stats_.tile_cfg2);
+  stats_.split_tiles = g1_.nsplit_tiles(ctx, stats_.tile_cfg1) + g2_.nsplit_tiles(ctx, stats_.tile_cfg2);
   ctx->sync();
+}
+
+namespace {
+struct TuneKey {
+  int prec, stage;
+  uint64_t shape;
+  bool operator<(const TuneKey& o) const {
+    return prec != o.prec ? prec < o.prec : stage != o.stage ? stage < o.stage : shape < o.shape;
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+bool g_autotune = true;
+}  // namespace
+
+void set_autotune(bool on) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_autotune = on;
+}
+bool autotune() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_autotune;
+}
+int autotune_entries() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return static_cast<int>(g_tune.size());
+}
+void autotune_clear() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+}
+
+// Tile configuration for one stage: tuned (cached per key) or the default.
+int HeffPlan::cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& at) {
+  if (g.complex() || !autotune()) return g.default_cfg(prec);
+  const TuneKey key{static_cast<int>(prec), stage, g.shape_key()};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second;
+  }
+  hipEvent_t e0, e1;
+  SADMRG_HIP(hipEventCreate(&e0));
+  SADMRG_HIP(hipEventCreate(&e1));
+  int best = g.default_cfg(prec);
+  float best_ms = 0;
+  for (int cfg = 0; cfg < kTileConfigs; ++cfg) {
+    g.run(ctx_, prec, at, cfg);  // builds the tile set and warms caches
+    SADMRG_HIP(hipEventRecord(e0, ctx_->stream()));
+    g.run(ctx_, prec, at, cfg);
+    SADMRG_HIP(hipEventRecord(e1, ctx_->stream()));
+    SADMRG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SADMRG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (cfg == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  SADMRG_HIP(hipEventDestroy(e0));
+  SADMRG_HIP(hipEventDestroy(e1));
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = best;
+  return best;
 }
 
 double HeffPlan::stage_flops(int stage) const {
@@ -222,8 +288,15 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage) {
     at.base[kArOut] = d_out;
     at.base[kArScratch] = scratch64_.get();
     at.base[kArPack] = pack64_.get();
-    if (stage != 2) g1_.run(ctx_, cplx_ ? GemmPrec::Z64 : GemmPrec::F64, at);
-    if (stage != 1) g2_.run(ctx_, cplx_ ? GemmPrec::Z64 : GemmPrec::F64, at);
+    const GemmPrec p = cplx_ ? GemmPrec::Z64 : GemmPrec::F64;
+    if (stage != 2) {
+      stats_.tile_cfg1 = cfg_for(g1_, p, 1, at);
+      g1_.run(ctx_, p, at, stats_.tile_cfg1);
+    }
+    if (stage != 1) {
+      stats_.tile_cfg2 = cfg_for(g2_, p, 2, at);
+      g2_.run(ctx_, p, at, stats_.tile_cfg2);
+    }
     return;
   }
   if (!pack32_.get()) {
@@ -237,8 +310,14 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage) {
   at.base[kArOut] = d_out;
   at.base[kArScratch] = scratch32_.get();
   at.base[kArPack] = pack32_.get();
-  if (stage != 2) g1_.run(ctx_, GemmPrec::F32, at);
-  if (stage != 1) g2_.run(ctx_, GemmPrec::F32_OUT64, at);
+  if (stage != 2) {
+    stats_.tile_cfg1 = cfg_for(g1_, GemmPrec::F32, 1, at);
+    g1_.run(ctx_, GemmPrec::F32, at, stats_.tile_cfg1);
+  }
+  if (stage != 1) {
+    stats_.tile_cfg2 = cfg_for(g2_, GemmPrec::F32_OUT64, 2, at);
+    g2_.run(ctx_, GemmPrec::F32_OUT64, at, stats_.tile_cfg2);
+  }
 }
 
 }  // namespace sadmrg

@@ -31,9 +31,20 @@ struct HeffStats {
   int stage1_probs = 0, stage2_probs = 0;
   int64_t stage1_segs = 0, stage2_segs = 0;
   int tiles1 = 0, tiles2 = 0;
-  int tile_cfg1 = 0, tile_cfg2 = 0;
+  int tile_cfg1 = 0, tile_cfg2 = 0;  // configurations of the last application
   int split_tiles = 0;  // output tiles whose reduction is split over workgroups (both stages)
 };
+
+// Tile-configuration autotuner.  On by default: the first application of a
+// plan whose (precision, stage, shape signature) is not in the process-wide
+// table times every legal configuration once (HIP events on the engine's
+// stream) and records the fastest; later plans with the same key reuse it.
+// Off: each table uses its shape-based default.
+void set_autotune(bool on);
+bool autotune();
+// Number of entries in the tuning table / forget all entries.
+int autotune_entries();
+void autotune_clear();
 
 class HeffPlan {
  public:
@@ -61,6 +72,7 @@ class HeffPlan {
   DevBuf pack64_, pack32_, scratch64_, scratch32_, in32_;
   int64_t pack_elems_ = 0, scratch_elems_ = 0;
   GemmBatch g1_, g2_;
+  int cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& at);
   HeffStats stats_;
   std::vector<uint8_t> out_null_;
 };

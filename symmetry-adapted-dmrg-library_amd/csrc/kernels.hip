@@ -52,21 +52,28 @@ __device__ inline const void* arena_ptr(const ArenaTable& at, int64_t off, int e
 // Operand tiles are staged through LDS in the orientation they have in HBM,
 // so every global load is coalesced along the contiguous index:
 //   RK: element (r, k) at r + k*ld  (A plain, B transposed) -> lds[k*LR + r]
-//   KR: element (r, k) at k + r*ld  (A transposed, B plain) -> lds[r*LK + k]
+//   KR: element (r, k) at k + r*ld  (A transposed, B plain) -> lds[kr_off(r, k)]
 // ("r" is m for A and n for B).  MFMA k-slot order: in each 16-deep chunk,
 // lane group g = lane/16 feeds k = 4g + j to the j-th of four MFMAs, for A
-// and B alike, so a KR tile is read with one 4-wide LDS load per fragment.
-// Pads: LR = R + 4 puts rows 4g (g = 0..3) on distinct bank groups; LK =
-// TK + 4 (FP32) / TK + 2 (FP64) spreads the 16 rows of a fragment over all
-// banks.  LDS is double buffered with one barrier per K step; global loads
-// run two steps ahead of the MFMAs.
+// and B alike, so a KR tile is read with one 16-byte LDS load per fragment.
+// RK rows are padded, LR = R + 4, which puts the rows 4g (g = 0..3) read by
+// one instruction on distinct bank groups.  KR rows are TK elements with the
+// 16-byte groups of row r XOR-swizzled by (r/2) mod 8: the two rows a wave
+// stores per instruction land in opposite bank halves, and the 16 rows of a
+// fragment read land in 16 distinct 16-byte bank slots.  LDS is double
+// buffered with one barrier per K step; global loads run two steps ahead of
+// the MFMAs.
 template <class T, int R, int TK>
 struct OperandTile {
   static constexpr int E = R * TK / 256;  // elements staged per thread and step
+  static constexpr int NQ = 16 / sizeof(T);  // elements per 16-byte group
   static constexpr int LR = R + 4;
-  static constexpr int LK = TK + (sizeof(T) == 8 ? 2 : 4);
+  static constexpr int LK = TK;
   static constexpr int ELEMS = (TK * LR > R * LK) ? TK * LR : R * LK;
   static_assert(256 % R == 0 && 256 % TK == 0 && TK % 16 == 0, "tile shape");
+  static_assert(TK / NQ == 8, "KR swizzle assumes 8 groups of 16 bytes per row");
+  // element offset of (r, k) in a KR tile
+  __device__ static inline int kr_off(int r, int k) { return r * LK + (((k / NQ) ^ ((r >> 1) & 7)) * NQ) + k % NQ; }
 };
 
 // Per-thread view of one operand of the current segment.
@@ -133,33 +140,63 @@ __device__ inline void tile_store(T* lds, bool kr, const T (&v)[OperandTile<T, R
   } else {
     const int kt = tid % TK, rt = tid / TK;
 #pragma unroll
-    for (int e = 0; e < OT::E; ++e) lds[(rt + e * (256 / TK)) * OT::LK + kt] = v[e] * scale;
+    for (int e = 0; e < OT::E; ++e) lds[OT::kr_off(rt + e * (256 / TK), kt)] = v[e] * scale;
   }
 }
 
-// NQ consecutive k-slot values (j = j0 .. j0+NQ-1) of fragment row `row` in
-// chunk kc: one 16-byte LDS read for a KR tile.
-template <class T, int R, int TK, int NQ>
-__device__ inline void tile_frag(const T* lds, bool kr, int row, int kc, int g, int j0, T (&f)[NQ]) {
+// 16-byte view of LDS for the KR fragment reads (rows of LK elements, LK a
+// multiple of 16 bytes, so a 4-slot group never straddles a 16-byte word).
+template <class T> struct Vec16;
+template <> struct Vec16<float> { typedef float4 type; };
+template <> struct Vec16<double> { typedef double2 type; };
+
+// NQ = 16 / sizeof(T) consecutive k-slot values (j = j0 .. j0+NQ-1) of
+// fragment row `row` in chunk kc.
+template <class T, int R, int TK, bool KR>
+__device__ inline void tile_frag(const T* lds, int row, int kc, int g, int j0, T (&f)[16 / sizeof(T)]) {
   using OT = OperandTile<T, R, TK>;
-  static_assert(NQ * sizeof(T) == 16, "one 16-byte read per fragment");
-  if (kr) {
-    const T* q = lds + row * OT::LK + kc + 4 * g + j0;
-    if constexpr (sizeof(T) == 4) {
-      const float4 x = *reinterpret_cast<const float4*>(q);
-      f[0] = x.x; f[1] = x.y; f[2] = x.z; f[3] = x.w;
-    } else {
-      const double2 x = *reinterpret_cast<const double2*>(q);
-      f[0] = x.x; f[1] = x.y;
-    }
+  constexpr int NQ = 16 / sizeof(T);
+  if constexpr (KR) {
+    static_assert((OT::LK * sizeof(T)) % 16 == 0, "KR rows must be 16-byte multiples");
+    const auto* v = reinterpret_cast<const typename Vec16<T>::type*>(lds);
+    const auto x = v[OT::kr_off(row, kc + 4 * g + j0) / NQ];
+    if constexpr (NQ == 4) { f[0] = x.x; f[1] = x.y; f[2] = x.z; f[3] = x.w; }
+    else { f[0] = x.x; f[1] = x.y; }
   } else {
 #pragma unroll
     for (int j = 0; j < NQ; ++j) f[j] = lds[(kc + 4 * g + j0 + j) * OT::LR + row];
   }
 }
 
-template <class T, class TOut, int TM, int TN, int TK>
-__global__ __launch_bounds__(256, 2) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
+// All MFMAs of one K step on an LDS buffer, operand orientations fixed.
+template <class T, int TM, int TN, int TK, bool KRA, bool KRB, class Acc, int FM, int FN>
+__device__ inline void mfma_step(const T* sa, const T* sb, Acc (&acc)[FM][FN], int wm, int wn, int lane) {
+  constexpr int NQ = 16 / sizeof(T);
+  constexpr int NGRP = (TK / 16) * (4 / NQ);  // fragment groups per K step
+  const int g = lane >> 4;
+  // one fragment group live at a time keeps the accumulators, the staged
+  // next step and the fragments inside the 256-register budget
+#pragma unroll 1
+  for (int gi = 0; gi < NGRP; ++gi) {
+    const int kc = (gi / (4 / NQ)) * 16, j0 = (gi % (4 / NQ)) * NQ;
+    T af[FM][NQ], bf[FN][NQ];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) tile_frag<T, TM, TK, KRA>(sa, wm * (TM / 2) + i * 16 + (lane & 15), kc, g, j0, af[i]);
+#pragma unroll
+    for (int j = 0; j < FN; ++j) tile_frag<T, TN, TK, KRB>(sb, wn * (TN / 2) + j * 16 + (lane & 15), kc, g, j0, bf[j]);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = mfma_16x16x4(af[i][q], bf[j][q], acc[i][j]);
+  }
+}
+
+// WPE: waves per SIMD the register allocation must allow (2: two workgroups
+// per CU, 256 registers per lane; 1: one workgroup, 512 registers).
+template <class T, class TOut, int TM, int TN, int TK, int WPE>
+__global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
                                                               const GemmProb* __restrict__ probs,
                                                               const GemmSeg* __restrict__ segs,
                                                               ArenaTable arenas, TOut* __restrict__ partial,
@@ -262,26 +299,12 @@ __global__ __launch_bounds__(256, 2) void grouped_gemm_kernel(const GemmTile* __
       }
       const T* sa = &lds[cb_][0];
       const T* sb = &lds[cb_][TA::ELEMS];
-      const bool kra = (skra >> cb_) & 1u, krb = (skrb >> cb_) & 1u;
-      constexpr int NQ = 16 / sizeof(T);  // k-slots per 16-byte fragment read
-#pragma unroll
-      for (int kc = 0; kc < TK; kc += 16)
-#pragma unroll
-        for (int j0 = 0; j0 < 4; j0 += NQ) {
-          T af[FM][NQ], bf[FN][NQ];
-#pragma unroll
-          for (int i = 0; i < FM; ++i)
-            tile_frag<T, TM, TK, NQ>(sa, kra, wm * (TM / 2) + i * 16 + (lane & 15), kc, g, j0, af[i]);
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            tile_frag<T, TN, TK, NQ>(sb, krb, wn * (TN / 2) + j * 16 + (lane & 15), kc, g, j0, bf[j]);
-#pragma unroll
-          for (int q = 0; q < NQ; ++q)
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-              for (int j = 0; j < FN; ++j) acc[i][j] = mfma_16x16x4(af[i][q], bf[j][q], acc[i][j]);
-        }
+      switch (((skra >> cb_) & 1u) | (((skrb >> cb_) & 1u) << 1)) {
+        case 0: mfma_step<T, TM, TN, TK, false, false>(sa, sb, acc, wm, wn, lane); break;
+        case 1: mfma_step<T, TM, TN, TK, true, false>(sa, sb, acc, wm, wn, lane); break;
+        case 2: mfma_step<T, TM, TN, TK, false, true>(sa, sb, acc, wm, wn, lane); break;
+        default: mfma_step<T, TM, TN, TK, true, true>(sa, sb, acc, wm, wn, lane); break;
+      }
       if (!has_next) break;
       __syncthreads();
     }
@@ -466,13 +489,13 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 // ---------------------------------------------------------------------------
 // Launch helpers.  The grid is split in chunks so that gridDim.x stays within
 // limits for very large tile tables.
-template <class T, class TOut, int TM, int TN, int TK>
+template <class T, class TOut, int TM, int TN, int TK, int WPE = 2>
 static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   TOut* part = static_cast<TOut*>(L.partial);
   const int chunk = 1 << 30;
   for (int base = 0; base < L.ntiles; base += chunk) {
     const int n = L.ntiles - base < chunk ? L.ntiles - base : chunk;
-    hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK>), dim3(n), dim3(256), 0, st,
+    hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE>), dim3(n), dim3(256), 0, st,
                        L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base);
   }
   return hipGetLastError();
@@ -484,13 +507,16 @@ hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L,
     return hipErrorInvalidValue;
   switch (prec) {
     case GemmPrec::F64:
-      if (tile_cfg == 1) return launch_real<double, double, 128, 128, 16>(L, st);
+      if (tile_cfg == 1) return launch_real<double, double, 128, 128, 16, 1>(L, st);
+      if (tile_cfg == 2) return launch_real<double, double, 128, 64, 16>(L, st);
       return launch_real<double, double, 64, 64, 16>(L, st);
     case GemmPrec::F32:
       if (tile_cfg == 1) return launch_real<float, float, 128, 128, 32>(L, st);
+      if (tile_cfg == 2) return launch_real<float, float, 128, 64, 32>(L, st);
       return launch_real<float, float, 64, 64, 32>(L, st);
     case GemmPrec::F32_OUT64:
       if (tile_cfg == 1) return launch_real<float, double, 128, 128, 32>(L, st);
+      if (tile_cfg == 2) return launch_real<float, double, 128, 64, 32>(L, st);
       return launch_real<float, double, 64, 64, 32>(L, st);
     case GemmPrec::Z64: {
       if (L.nreduce > 0 || tile_cfg != 0) return hipErrorInvalidValue;  // no split or large tiles for complex

@@ -29,11 +29,13 @@ struct GemmLaunch {
   int nreduce = 0;
 };
 
-// tile_cfg 0: 64x64x16 tiles, 1: 128x128x16 tiles.  One launch, split
-// reductions included.
+// Output tile per configuration: 0: 64x64, 1: 128x128, 2: 128x64 (K steps of
+// 16 for FP64, 32 for FP32 operands).  Complex tables use configuration 0.
+// One launch, split reductions included.
+constexpr int kTileConfigs = 3;
 hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& launch, hipStream_t st);
 
-inline int tile_m(int tile_cfg) { return tile_cfg == 1 ? 128 : 64; }
+inline int tile_m(int tile_cfg) { return tile_cfg == 0 ? 64 : 128; }
 inline int tile_n(int tile_cfg) { return tile_cfg == 1 ? 128 : 64; }
 
 // out[c] = sign * sum_i conj(X[i,c]) * y[i] for c < ncol (complex pairs, 2*ncol doubles).

@@ -240,6 +240,37 @@ def heisenberg_ed(hmap, coup, nlow=1):
     return np.sort(w)
 
 
+def heisenberg_bond_expectations(hmap, coup):
+    """(E0, {(i, j): <S_i.S_j>}) over the bonds of hmap, from the Sz=0 ground
+    state of H = sum_{i<j} coup[hmap[i,j]] S_i.S_j (exact diagonalisation)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    ns = hmap.shape[0]
+    states = [s for s in range(1 << ns) if bin(s).count("1") == ns // 2]
+    index = {s: k for k, s in enumerate(states)}
+    bonds = [(i, j) for i in range(ns) for j in range(i + 1, ns) if hmap[i, j] > 0]
+
+    def bond_op(i, j, J):
+        rows, cols, vals = [], [], []
+        diag = np.zeros(len(states))
+        for k, s in enumerate(states):
+            si, sj = (s >> i) & 1, (s >> j) & 1
+            if si == sj:
+                diag[k] += 0.25 * J
+            else:
+                diag[k] -= 0.25 * J
+                rows.append(index[s ^ (1 << i) ^ (1 << j)])
+                cols.append(k)
+                vals.append(0.5 * J)
+        return sp.csr_matrix((vals, (rows, cols)), shape=(len(states), len(states))) + sp.diags(diag)
+
+    ops = {b: bond_op(b[0], b[1], 1.0) for b in bonds}
+    H = sum(coup[hmap[i, j]] * ops[(i, j)] for i, j in bonds)
+    w, v = sla.eigsh(H, k=1, which="SA")
+    psi = v[:, 0]
+    return float(w[0]), {b: float(psi @ (ops[b] @ psi)) for b in bonds}
+
+
 # ---- synthetic centre-bond workload --------------------------------------------
 def rotor_dims(D, sigma, two_j_parity=0, jmax2=16):
     js = list(range(two_j_parity, jmax2 + 1, 2))

@@ -311,6 +311,7 @@ def test_heff_split_reduction_matches_reference(ctx, sad, cgc, D, n_op):
     plan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
     st = plan.stats()
     assert st["split_tiles"] > 0, st
+    assert st["flops"] > 0 and st["stage1_problems"] > 0 and st["stage2_problems"] > 0, st
     got = tl.from_dev(plan.apply(psi))
     d, il, ir = tl.reference_heff_setup(rb, wl, 4)
     exp = tl.from_ref(d.heff(il, ir, tl.to_ref(wl["psi"], 1)))
@@ -319,3 +320,96 @@ def test_heff_split_reduction_matches_reference(ctx, sad, cgc, D, n_op):
     got32 = tl.from_dev(plan.apply(psi, sad.FP32))
     rel32, _ = tl.compare(got32, exp)
     assert rel32 <= 2e-5, (rel32, st)
+
+
+@pytest.mark.parametrize("name,flag", [("dmrg_su2_obc12", 0), ("dmrg_su2_j1j2_4x3", 0), ("dmrg_su2_obc12", 1),
+                                       ("dmrg_su2_j1j2_4x3", 1)])
+def test_overlap_adjoint_identity(ctx, sad, cgc, name, flag):
+    """<lam, T(X)> = <T*(lam), X> for T = overlap_transformation on the
+    reference's own site tensors (random X and lam with the right structures)."""
+    tens, extra = _load_case(name)
+    ns = int(extra["params"][5])
+    rng = np.random.default_rng(17 + flag)
+    checked = 0
+    for i in range(1, ns - 1):
+        nb = i - 1 if flag == 0 else i + 1
+        if f"S_uu_{i}" not in tens or f"S_hh_{nb}" not in tens:
+            continue
+        uu = tl.to_dev(ctx, tens[f"S_uu_{i}"])
+        like = tens[f"S_hh_{nb}"]
+        x = dict(like)
+        x["data"] = rng.standard_normal(like["data"].shape)
+        X = tl.to_dev(ctx, x)
+        tx = tl.from_dev(sad.env("overlap_transformation", uu, uu, X, flag=flag, cgc=cgc))
+        lam = dict(tx)
+        lam["data"] = rng.standard_normal(tx["data"].shape)
+        L = tl.to_dev(ctx, lam)
+        adj = tl.from_dev(sad.env("overlap_adjoint", uu, uu, L, tl.to_dev(ctx, like), flag=flag, cgc=cgc))
+        lhs = float(np.dot(lam["data"], tx["data"]))
+        bx, ba = tl.blocks(x), tl.blocks(adj)
+        rhs = float(sum(np.sum(bx[b] * ba[b]) for b in ba if b in bx))
+        assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs)), (i, lhs, rhs)
+        checked += 1
+    assert checked >= 2
+
+
+def test_coupling_gradient_matches_exact_bond_energies(ctx, sad):
+    """dE/dJ_i on an open chain of 10 sites with a distinct random coupling per
+    bond (D large enough to be exact): equal to the exact-diagonalisation bond
+    energies <S_i.S_i+1> (Hellmann-Feynman), and sum_i J_i dE/dJ_i = E."""
+    L = 10
+    rng = np.random.default_rng(11)
+    J = rng.uniform(0.5, 1.5, L - 1)
+    hmap = np.zeros((L, L), dtype=np.int32)
+    coup = np.zeros(L + 1)
+    for i in range(L - 1):
+        hmap[i, i + 1] = hmap[i + 1, i] = i + 1
+        coup[i + 1] = J[i]
+    e_ed, bond = tl.heisenberg_bond_expectations(hmap, coup)
+    sad.lib().sadmrg_rng_seed(42)
+    d = sad.DMRG(ctx, 1, 0, 1, L, 64)
+    d.set_couplings(hmap, coup)
+    d.set_max_dcut(64)
+    d.idmrg()
+    for _ in range(3):
+        d.sweep()
+    g, e = d.coupling_gradient()
+    assert abs(e - d.energy()) <= 1e-10 * abs(e_ed), (e, d.energy())
+    assert abs(e - e_ed) <= 1e-9 * abs(e_ed), (e, e_ed)
+    for i in range(L - 1):
+        assert abs(g[i + 1] - bond[(i, i + 1)]) <= 1e-7, (i, g[i + 1], bond[(i, i + 1)])
+    assert abs(float(np.dot(coup, g)) - e) <= 1e-10 * abs(e)
+
+
+@pytest.mark.parametrize("p1,p2", [(1.0, 0.0), (0.0, 1.0)])
+def test_gram_truncation_matches_dense_svd(ctx, sad, p1, p2):
+    """Large-sector truncation through the Gram (density-matrix) route: the
+    singular values equal numpy's SVD of every block of psi (values above the
+    route's resolution), and psi rebuilt from the new site tensors
+    (prepare_input_vector) equals psi when nothing is truncated."""
+    width, length, D = 6, 4, 400
+    ns = width * length
+    coup = np.zeros(ns + 1)
+    coup[1], coup[2] = 1.0, 0.5
+    sad.lib().sadmrg_rng_seed(42)
+    d = sad.DMRG(ctx, 1, 0, length, width, D)
+    d.set_couplings(sad.j1j2_cylinder(width, length), coup)
+    d.set_max_dcut(D)
+    d.idmrg()
+    d.sweep()
+    il, ir = ns // 2 - 1, ns // 2
+    vec = d.input_vector(il, ir)
+    v = tl.from_dev(vec)
+    blocks = tl.blocks(v)
+    assert max(min(b.shape) for b in blocks.values()) >= 128, [b.shape for b in blocks.values()]
+    exact = np.sort(np.concatenate([np.linalg.svd(b, compute_uv=False) for b in blocks.values()]))[::-1]
+    c = d.clone()
+    c.set_max_dcut(100000)
+    w = np.sort(np.asarray(c.svd(vec, il, ir, p1, p2)))[::-1]  # wout: kept (sector order), then discarded
+    smax = exact[0]
+    resolved = exact[exact > 1e-6 * smax]
+    assert len(w) >= len(resolved)
+    assert np.max(np.abs(w[:len(resolved)] - resolved)) <= 1e-10 * smax
+    back = tl.from_dev(c.input_vector(il, ir))
+    rel, _ = tl.compare(back, v)
+    assert rel <= 1e-6, rel
