@@ -94,11 +94,14 @@ def cpu_baseline(wl, budget_s=20.0):
             "output": out}
 
 
-def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6):
+def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6, check_every=(4,)):
     """Mixed vs FP64 sweep on the SU(2) J1-J2 cylinder (width x length, snake
     ordering).  Warm-up: iDMRG at warm[0] then one mixed sweep per entry of
     warm and one at D; the state is then cloned on the device and one sweep
-    is timed in each precision from the identical start."""
+    is timed in each precision from the identical start.  The FP64 sweep
+    keeps the reference's Lanczos schedule (check every 4th step); the mixed
+    sweep checks every check_every[0]-th step, further entries are timed as
+    variants."""
     ns = width * length
     coup = np.zeros(ns + 1)
     coup[1], coup[2] = 1.0, j2
@@ -115,10 +118,13 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refi
               file=sys.stderr, flush=True)
     d.records()
     out = {}
-    for name, prec in (("fp64", sad.FP64), ("mixed", sad.FP32)):
+    legs = [("fp64", sad.FP64, 4), ("mixed", sad.FP32, check_every[0])]
+    legs += [(f"mixed_check{k}", sad.FP32, k) for k in check_every[1:]]
+    for name, prec, every in legs:
         c = d.clone()
         c.set_precision(prec, refine=(prec != sad.FP64), refine_tol=refine_tol)
         c.set_refinement(refine_krylov, 6)
+        c.set_check_interval(every)
         ctx.sync()
         t1 = time.perf_counter()
         c.sweep()
@@ -148,7 +154,69 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refi
         "mixed_split_s": {k: round(tsum("mixed", k), 3) for k in ("t_env", "t_lanczos", "t_svd")},
         "matvecs_fp64": int(tsum("fp64", "matvec64")),
         "matvecs_mixed": [int(tsum("mixed", "matvec32")), int(tsum("mixed", "matvec64"))],
+        "mixed_check_every": check_every[0],
+        "variants": {name: {"sweep_s": round(out[name]["seconds"], 3),
+                            "speedup": round(out["fp64"]["seconds"] / out[name]["seconds"], 3),
+                            "final_rel_dE": float(abs(out[name]["energy"] - out["fp64"]["energy"]) /
+                                                  abs(out["fp64"]["energy"])),
+                            "matvecs": [int(tsum(name, "matvec32")), int(tsum(name, "matvec64"))]}
+                     for name, _, _ in legs[2:]},
     }
+
+
+def gradient_run(sad, ctx, L, D, warm, seed=11, fd_bonds=(0, None, -1), h=1e-5, sweeps=3):
+    """dE/dJ_i for every bond of an open SU(2) chain with random couplings
+    J_i ~ U[0.5, 1.5] (seed), at D kept multiplets: adjoint gradient timed,
+    checked by Euler's identity sum_i J_i dE/dJ_i = E and by central finite
+    differences on a few bonds (each side re-converged from the state)."""
+    rng = np.random.default_rng(seed)
+    J = rng.uniform(0.5, 1.5, L - 1)
+    hmap = np.zeros((L, L), dtype=np.int32)
+    coup = np.zeros(L + 1)
+    for i in range(L - 1):
+        hmap[i, i + 1] = hmap[i + 1, i] = i + 1
+        coup[i + 1] = J[i]
+    d = sad.DMRG(ctx, sad.SU2, 0, 1, L, D)
+    d.set_couplings(hmap, coup)
+    d.set_precision(sad.FP64)
+    t0 = time.perf_counter()
+    d.set_max_dcut(warm[0])
+    d.idmrg()
+    for dw in list(warm[1:]) + [D] * sweeps:
+        d.set_max_dcut(dw)
+        d.sweep()
+        print(f"[bench] gradient chain L={L}: sweep D={dw} E={d.energy():.12f} ({time.perf_counter() - t0:.1f} s)",
+              file=sys.stderr, flush=True)
+    d.records()
+    ctx.sync()
+    t1 = time.perf_counter()
+    g, e = d.coupling_gradient()
+    ctx.sync()
+    t_grad = time.perf_counter() - t1
+    euler = float(np.dot(coup, g))
+    picks = sorted({(L - 1) // 2 if b is None else (b % (L - 1)) for b in fd_bonds})
+    fd = []
+    for b in picks:
+        vals = []
+        for sgn in (1.0, -1.0):
+            c = d.clone()
+            cp = coup.copy()
+            cp[b + 1] += sgn * h
+            c.set_couplings(hmap, cp)
+            c.prepare_sweep()
+            for _ in range(2):
+                c.sweep()
+            vals.append(c.energy())
+            c.records()
+            del c
+        fdv = (vals[0] - vals[1]) / (2 * h)
+        fd.append({"bond": b, "adjoint": float(g[b + 1]), "finite_difference": fdv,
+                   "abs_diff": abs(fdv - float(g[b + 1]))})
+        print(f"[bench] bond {b}: adjoint {g[b + 1]:.10f} fd {fdv:.10f}", file=sys.stderr, flush=True)
+    return {"chain": f"SU(2) open chain L={L}, J_i ~ U[0.5,1.5] seed {seed}", "D": D, "bonds": L - 1,
+            "energy": e, "sweep_energy": d.energy(), "gradient_seconds": round(t_grad, 4),
+            "euler_rel_err": abs(euler - e) / abs(e), "finite_difference": fd,
+            "dE_dJ": [float(x) for x in g[1:L]]}
 
 
 def main():
@@ -161,10 +229,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device tensor 5 times (HBM counter calibration) and exit")
-    ap.add_argument("--refine-tol", type=float, default=0.0,
+    ap.add_argument("--refine-tol", type=float, default=1e-5,
                     help="FP64 residual tolerance of the mixed path (relative to |E|; 0: library default)")
-    ap.add_argument("--refine-krylov", type=int, default=6, help="FP32 Krylov steps per refinement restart")
-    ap.add_argument("--sweep", default="6x4:512:64,128,256",
+    ap.add_argument("--refine-krylov", type=int, default=3, help="FP32 Krylov steps per refinement restart")
+    ap.add_argument("--check-every", default="2",
+                    help="mixed sweep: Lanczos convergence check interval (comma list: first is reported, "
+                         "the others are timed as variants)")
+    ap.add_argument("--gradient", default="none", metavar="L:D:WARM",
+                    help="only run the coupling-gradient workload (e.g. 48:1024:128,256,512) and exit")
+    ap.add_argument("--sweep", default="6x12:2048:128,256,512,1024",
                     help="WIDTHxLENGTH:D:warm-up Ds for the mixed vs FP64 sweep comparison ('none' skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -184,6 +257,12 @@ def main():
         ctx.sync()
         print(json.dumps({"calibration": "device-to-device tensor copy", "bytes_read": n * 8,
                           "bytes_written": n * 8, "copies": 5}))
+        return
+    if args.gradient != "none":
+        Ls, Ds, Ws = args.gradient.split(":")
+        ctx = sad.Context(0)
+        out = gradient_run(sad, ctx, int(Ls), int(Ds), [int(x) for x in Ws.split(",")])
+        print(json.dumps({"workload": "coupling gradient", **out}))
         return
     print(f"[bench] building C3 workload D={args.D}", file=sys.stderr, flush=True)
     wl = tl.build_workload(D=args.D)
@@ -244,7 +323,8 @@ def main():
         w_, l_ = (int(x) for x in lat.split("x"))
         print(f"[bench] sweep comparison {lat} D={dd}", file=sys.stderr, flush=True)
         sweep = sweep_compare(sad, ctx, w_, l_, int(dd), [int(x) for x in warm.split(",")],
-                              refine_tol=args.refine_tol, refine_krylov=args.refine_krylov)
+                              refine_tol=args.refine_tol, refine_krylov=args.refine_krylov,
+                              check_every=tuple(int(x) for x in args.check_every.split(",")))
 
     dom = "stage2" if res["fp32"]["stage2_ms"] >= res["fp32"]["stage1_ms"] else "stage1"
     dom_flops = f2 if dom == "stage2" else f1

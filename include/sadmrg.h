@@ -144,6 +144,14 @@ int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n
    entries, *entries receives the table size.  No reference counterpart (the
    reference has one BLAS path). */
 int sadmrg_autotune(int enable, int clear, int* entries);
+/* Truncation route of sweep-step SVDs (BlockTensor::svd with (p1, p2) =
+   (1, 0) or (0, 1), block_tensor_inst.cpp:501-637, real data): sectors whose
+   larger side has at least gram_min rows are decomposed through the Gram
+   matrix when that provably leaves the kept set unchanged, by gesdd
+   otherwise.  value > 0 sets gram_min, 0 turns the Gram route off, -1 leaves
+   the setting; *previous receives the old value.  Process-wide, default 256.
+   No reference counterpart (the reference always calls gesvd). */
+int sadmrg_svd_gram_min(int value, int* previous);
 /* Lanczos::lanczos1 (lanczos.hpp:70) on the plan; psi is the start vector on
    entry and the Ritz vector on return.  counters[4] = {FP64 matvecs, FP32
    matvecs, Lanczos steps, refinement restarts}. */
@@ -169,6 +177,35 @@ int sadmrg_dmrg_set_precision(sadmrg_dmrg* d, int precision, int refine, double 
 /* refinement restarts of the mixed path: FP32 Krylov steps per restart and
    the restart cap (defaults 6 and 6); no reference counterpart */
 int sadmrg_dmrg_set_refinement(sadmrg_dmrg* d, int krylov_steps, int max_restarts);
+/* Lanczos convergence check after every `every`-th Krylov step; the
+   reference checks every 4th (dmrg_inst.cpp:1334), the default.  A shorter
+   interval lets the mixed path stop with fewer FP32 matvecs. */
+int sadmrg_dmrg_set_check_interval(sadmrg_dmrg* d, int every);
+/* Checkpoint files in the reference's format (dmrg_inst.cpp:947-1177), in
+   directory dir (NULL or "" = the current directory, where the reference
+   writes): uu-{ns}-{bondd}-{totspin}-{exci}-{i}.dat / .bin per site
+   (save_mps2, dmrg.hpp:129), ww-{ns}-{bondd}-{totspin}-{exci}.bin (save_ww,
+   dmrg.hpp:130), enr-{ns}-{bondd}-{totspin}-{exci}.dat (save_enr, appended,
+   dmrg.hpp:125).  The engine targets exci = 0; save_mps names the files
+   with the given exci as save_mps2 does. */
+int sadmrg_dmrg_save_mps(sadmrg_dmrg* d, const char* dir, int exci);
+int sadmrg_dmrg_save_ww(sadmrg_dmrg* d, const char* dir);
+int sadmrg_dmrg_save_enr(sadmrg_dmrg* d, const char* dir);
+/* read_mps(read1) (dmrg.hpp:127) when read2 < 0, read_mps(read1, read2)
+   (dmrg.hpp:128: also read_ww and read_enr) otherwise; *pass = 1 when every
+   site's files were read, as the reference's return value.  A malformed or
+   short file, or a CGC block that is not the canonical one of its labels,
+   is an error (the reference reads such files unchecked). */
+int sadmrg_dmrg_read_mps(sadmrg_dmrg* d, const char* dir, int read1, int read2, int* pass);
+int sadmrg_dmrg_read_ww(sadmrg_dmrg* d, const char* dir, int read1);   /* read_ww (dmrg.hpp:131) */
+int sadmrg_dmrg_read_enr(sadmrg_dmrg* d, const char* dir, int read1);  /* read_enr (dmrg.hpp:126) */
+/* sweep() ends with save_mps2(exci), save_ww(), save_enr() into dir, as the
+   reference's sweep does (dmrg_inst.cpp:940-942); NULL (the default) turns
+   the files off. */
+int sadmrg_dmrg_set_checkpoint_dir(sadmrg_dmrg* d, const char* dir);
+/* ww[exci]: singular values of the last truncation (kept, then discarded) or
+   what read_ww loaded; *n receives the count, up to cap are copied. */
+int sadmrg_dmrg_ww(sadmrg_dmrg* d, double* w, int cap, int* n);
 int sadmrg_dmrg_idmrg(sadmrg_dmrg* d);         /* do_idmrg (dmrg.hpp:107) */
 int sadmrg_dmrg_prepare_sweep(sadmrg_dmrg* d); /* prepare_sweep (dmrg.hpp:110) */
 int sadmrg_dmrg_sweep(sadmrg_dmrg* d);         /* sweep (dmrg.hpp:111) */

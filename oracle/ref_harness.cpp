@@ -30,6 +30,7 @@
 #include <random>
 #include <chrono>
 #include <omp.h>
+#include <unistd.h>
 
 #define private public
 #include "dmrg.hpp"
@@ -665,6 +666,44 @@ long long ref_rng_position(long long maxn) {
   for (long long k = 0; k < maxn; ++k)
     if (dist(g) == v) return k;
   return -1;
+}
+
+// The reference's checkpoint I/O writes and reads relative paths in the
+// current directory; these run it with the working directory set to dir.
+// DMRG::save_mps2(exci), save_ww(), save_enr() (dmrg_inst.cpp:940-942, 947-1003,
+// 1146-1153).
+int ref_dmrg_save(void* p, const char* dir, int exci) {
+  RefDMRG* h = static_cast<RefDMRG*>(p);
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof(cwd)) || chdir(dir) != 0) return -1;
+  DISPATCH(h, {
+    auto* d = asd<S, Y>(h);
+    d->save_mps2(exci);
+    d->save_ww();
+    d->save_enr();
+  });
+  return chdir(cwd) == 0 ? 0 : -1;
+}
+
+// DMRG::read_mps(read1, read2) (dmrg_inst.cpp:1078-1141); returns its pass flag.
+int ref_dmrg_read(void* p, const char* dir, int read1, int read2) {
+  RefDMRG* h = static_cast<RefDMRG*>(p);
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof(cwd)) || chdir(dir) != 0) return -1;
+  bool pass = false;
+  DISPATCH(h, pass = asd<S, Y>(h)->read_mps(read1, read2));
+  if (chdir(cwd) != 0) return -1;
+  return pass ? 1 : 0;
+}
+
+// First cap entries of ww[exci] (the singular values of the last truncation).
+int ref_dmrg_ww(void* p, double* out, int cap) {
+  RefDMRG* h = static_cast<RefDMRG*>(p);
+  DISPATCH(h, {
+    auto* d = asd<S, Y>(h);
+    for (int k = 0; k < cap; ++k) out[k] = d->ww[d->exci][k];
+  });
+  return 0;
 }
 
 }  // extern "C"

@@ -77,6 +77,9 @@ def lib():
         "ref_dmrg_lanczos": (C.c_double, [vp, C.c_int, C.c_int, vp]),
         "ref_dmrg_svd": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, dp, C.c_int]),
         "ref_dmrg_time_heff": (C.c_double, [vp, C.c_int, C.c_int, vp, C.c_int]),
+        "ref_dmrg_save": (C.c_int, [vp, C.c_char_p, C.c_int]),
+        "ref_dmrg_read": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int]),
+        "ref_dmrg_ww": (C.c_int, [vp, dp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -282,3 +285,20 @@ class RefDMRG:
 
     def time_heff(self, il, ir, vin, reps):
         return lib().ref_dmrg_time_heff(self.ptr, il, ir, vin.ptr, reps)
+
+    def save(self, directory, exci=0):
+        """save_mps2(exci); save_ww(); save_enr() into `directory`."""
+        if lib().ref_dmrg_save(self.ptr, str(directory).encode(), exci) != 0:
+            raise OSError(f"cannot enter {directory}")
+
+    def read(self, directory, read1, read2=0):
+        """read_mps(read1, read2) from `directory`; returns the pass flag."""
+        rc = lib().ref_dmrg_read(self.ptr, str(directory).encode(), read1, read2)
+        if rc < 0:
+            raise OSError(f"cannot enter {directory}")
+        return rc == 1
+
+    def ww(self, cap):
+        w = np.zeros(cap)
+        lib().ref_dmrg_ww(self.ptr, _dptr(w), cap)
+        return w

@@ -657,3 +657,80 @@ def svd_values(vec, max_dcut):
         kept += list(per[i][:dim[i]])
         disc += list(per[i][dim[i]:])
     return np.asarray(kept + disc), dim
+
+
+# ---- checkpoint files (dmrg_inst.cpp:947-1177) ---------------------------------------
+def _nelement(t):
+    """SU2Struct::get_nelement (su2.cpp:535-548) / U1Struct (u1.cpp:406-417):
+    reduced and CGC sizes summed over every allowed block."""
+    allowed = [b for b in range(t.nten()) if t.allowed(b)]
+    ndata = sum(math.prod(t.dims(b)) for b in allowed)
+    ncgc = sum(math.prod(j + 1 for j in t.labels(b)) for b in allowed) if t.sym == 1 else 0
+    return allowed, ndata, ncgc
+
+
+def mps_site_files(t):
+    """save_mps2's two files for one site tensor (dmrg_inst.cpp:961-988):
+    the .dat text and the .bin bytes.  The .bin holds ndata Scalars then ncgc
+    doubles (SU(2)); present blocks come first in block order
+    (BlockTensor::get_elems, block_tensor_inst.cpp:643-660), the rest of each
+    section is zero (the buffers are value-initialised, dmrg_inst.cpp:982-983).
+    CGC blocks of 3-leg tensors are the canonical fill_block tensors.  For
+    U(1) the reference leaves ncgc uninitialised (U1Struct::get_nelement
+    never sets it), so its CGC section has no defined length; here it is
+    empty."""
+    lines = [f"{t.nbond}\t{t.locspin}\n"]
+    for d, q, n in t.bonds:
+        lines.append(f"{d}\t{len(q)}\n")
+        lines.append("".join(f"{x}\t" for x in q) + "\n")
+        lines.append("".join(f"{x}\t" for x in n) + "\n")
+    allowed, ndata, ncgc = _nelement(t)
+    dt = np.complex128 if t.cplx else np.float64
+    buf = np.zeros(ndata, dt)
+    off = 0
+    for b in allowed:
+        if b in t.blocks:
+            x = np.asarray(t.blocks[b], dt).ravel(order="F")
+            buf[off:off + x.size] = x
+            off += x.size
+    out = buf.tobytes()
+    if t.sym == 1:
+        assert t.nbond == 3, "MPS site tensors have three bonds"
+        cg = np.zeros(ncgc)
+        off = 0
+        for b in allowed:
+            if b in t.blocks:
+                x = cgc_tensor(*t.labels(b)).ravel(order="F")
+                cg[off:off + x.size] = x
+                off += x.size
+        out += cg.tobytes()
+    return "".join(lines), out
+
+
+def read_mps_site(dat, binary, sym, cplx):
+    """read_mps for one site (dmrg_inst.cpp:1046-1072) with
+    BlockTensor::set(struct, data, cgc) (block_tensor_inst.cpp:120-159):
+    every allowed block of the structure is filled, in block order, from
+    consecutive elements of the file.  Returns the tensor and the CGC section
+    (one flat array, SU(2); empty for U(1))."""
+    tok = [int(x) for x in dat.split()]
+    nb, ls = tok[0], tok[1]
+    p, bonds = 2, []
+    for _ in range(nb):
+        dr, nm = tok[p], tok[p + 1]
+        q, n = tok[p + 2:p + 2 + nm], tok[p + 2 + nm:p + 2 + 2 * nm]
+        order = sorted(range(nm), key=lambda k: q[k])  # SU2Bond::set sorts by label (su2.cpp:125)
+        bonds.append((dr, [q[k] for k in order], [n[k] for k in order]))
+        p += 2 + 2 * nm
+    t = BT(sym, cplx, ls, bonds)
+    allowed, ndata, ncgc = _nelement(t)
+    dt = np.complex128 if cplx else np.float64
+    data = np.frombuffer(binary, dt, count=ndata)
+    off = 0
+    for b in allowed:
+        dims = t.dims(b)
+        n = math.prod(dims)
+        t.blocks[b] = data[off:off + n].reshape(dims, order="F").copy()
+        off += n
+    cgc = np.frombuffer(binary, np.float64, count=ncgc, offset=ndata * data.itemsize) if sym == 1 else np.zeros(0)
+    return t, cgc

@@ -84,6 +84,7 @@ def lib():
         "sadmrg_heff_stats": (i, [vp, dp, dp, llp, ip]),
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
         "sadmrg_autotune": (i, [i, i, ip]),
+        "sadmrg_svd_gram_min": (i, [i, ip]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_destroy": (i, [vp]),
         "sadmrg_dmrg_ns": (i, [vp, ip]),
@@ -93,6 +94,15 @@ def lib():
         "sadmrg_dmrg_set_max_dcut": (i, [vp, i]),
         "sadmrg_dmrg_set_precision": (i, [vp, i, i, d]),
         "sadmrg_dmrg_set_refinement": (i, [vp, i, i]),
+        "sadmrg_dmrg_set_check_interval": (i, [vp, i]),
+        "sadmrg_dmrg_save_mps": (i, [vp, C.c_char_p, i]),
+        "sadmrg_dmrg_save_ww": (i, [vp, C.c_char_p]),
+        "sadmrg_dmrg_save_enr": (i, [vp, C.c_char_p]),
+        "sadmrg_dmrg_read_mps": (i, [vp, C.c_char_p, i, i, ip]),
+        "sadmrg_dmrg_read_ww": (i, [vp, C.c_char_p, i]),
+        "sadmrg_dmrg_read_enr": (i, [vp, C.c_char_p, i]),
+        "sadmrg_dmrg_set_checkpoint_dir": (i, [vp, C.c_char_p]),
+        "sadmrg_dmrg_ww": (i, [vp, dp, i, ip]),
         "sadmrg_dmrg_idmrg": (i, [vp]),
         "sadmrg_dmrg_prepare_sweep": (i, [vp]),
         "sadmrg_dmrg_sweep": (i, [vp]),
@@ -403,6 +413,19 @@ def autotune(enable=None, clear=False):
     return n.value
 
 
+def _path(directory):
+    return b"" if directory is None else os.fsencode(str(directory))
+
+
+def svd_gram_min(value=-1):
+    """Smallest sector (rows of its larger side) that sweep-step truncations
+    decompose through the Gram matrix; 0 turns the route off, -1 only reads.
+    Process-wide.  Returns the previous value."""
+    prev = C.c_int()
+    check(lib().sadmrg_svd_gram_min(int(value), C.byref(prev)))
+    return prev.value
+
+
 REC_FIELDS = ("il", "ir", "energy", "trunc_err", "kept", "matvec64", "matvec32", "t_env", "t_lanczos", "t_svd",
               "heff_flops", "ndata")
 
@@ -454,6 +477,49 @@ class DMRG:
 
     def set_refinement(self, krylov_steps=6, max_restarts=6):
         check(lib().sadmrg_dmrg_set_refinement(self.ptr, krylov_steps, max_restarts))
+
+    def set_check_interval(self, every=4):
+        """Lanczos convergence check after every `every`-th step (reference: 4)."""
+        check(lib().sadmrg_dmrg_set_check_interval(self.ptr, every))
+
+    # checkpoint files in the reference's format (dmrg_inst.cpp:947-1177)
+    def save_mps(self, directory=None, exci=0):
+        """save_mps2(exci): uu-{ns}-{bondd}-{totspin}-{exci}-{i}.dat/.bin per site."""
+        check(lib().sadmrg_dmrg_save_mps(self.ptr, _path(directory), exci))
+
+    def save_ww(self, directory=None):
+        check(lib().sadmrg_dmrg_save_ww(self.ptr, _path(directory)))
+
+    def save_enr(self, directory=None):
+        check(lib().sadmrg_dmrg_save_enr(self.ptr, _path(directory)))
+
+    def read_mps(self, directory, read1, read2=None):
+        """read_mps(read1) or, with read2, read_mps(read1, read2) (also ww and
+        the energy); returns the reference's pass flag."""
+        ok = C.c_int()
+        check(lib().sadmrg_dmrg_read_mps(self.ptr, _path(directory), read1, -1 if read2 is None else read2,
+                                         C.byref(ok)))
+        return bool(ok.value)
+
+    def read_ww(self, directory, read1):
+        check(lib().sadmrg_dmrg_read_ww(self.ptr, _path(directory), read1))
+
+    def read_enr(self, directory, read1):
+        check(lib().sadmrg_dmrg_read_enr(self.ptr, _path(directory), read1))
+
+    def set_checkpoint_dir(self, directory):
+        """sweep() writes the reference's checkpoint files into directory
+        (None: no files, the default)."""
+        check(lib().sadmrg_dmrg_set_checkpoint_dir(self.ptr, None if directory is None else _path(directory)))
+
+    def ww(self):
+        """Singular values of the last truncation (or what read_ww loaded)."""
+        n = C.c_int()
+        check(lib().sadmrg_dmrg_ww(self.ptr, None, 0, C.byref(n)))
+        w = np.zeros(n.value)
+        if n.value:
+            check(lib().sadmrg_dmrg_ww(self.ptr, _dptr(w), n.value, C.byref(n)))
+        return w
 
     def idmrg(self):
         check(lib().sadmrg_dmrg_idmrg(self.ptr))

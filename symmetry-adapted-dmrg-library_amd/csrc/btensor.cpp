@@ -1,5 +1,7 @@
 // btensor.cpp -- see btensor.hpp.
 #include "btensor.hpp"
+#include <atomic>
+#include <functional>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -1243,6 +1245,12 @@ void op_env_adjoint(BTensor& out, const BTensor& lam, const BTensor& uu, const B
   ctx->sync();
 }
 
+namespace {
+std::atomic<int> g_gram_min{256};
+}
+void set_gram_min(int n) { g_gram_min.store(n); }
+int gram_min() { return g_gram_min.load(); }
+
 // ---------------------------------------------------------------------------
 // Block SVD truncation (block_tensor_inst.cpp:501-637, dense_inst.cpp:447-541).
 void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, int max_dcut,
@@ -1304,37 +1312,86 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
   // value is round-off (kept as one state by the reference's rule) is redone
   // unpadded, because its singular vectors are then not determined by the
   // data and a padded solve may return one that lives on the padding.
-  std::vector<size_t> order(secs.size());
-  for (size_t k = 0; k < order.size(); ++k) order[k] = k;
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-    return std::max(secs[x].m, secs[x].n) > std::max(secs[y].m, secs[y].n);
-  });
+  // Sectors of similar size share one strided-batched gesdd, each padded with
+  // zeros to the batch's largest (m, n).  Zero padding appends zero singular
+  // values and leaves the leading singular triplets unchanged, their vectors
+  // vanishing on the padding, so a sector's U (m x mn) and V' (mn x n) are the
+  // leading blocks of its padded factors.  A sector whose largest singular
+  // value is round-off (kept as one state by the reference's rule) is redone
+  // unpadded, because its singular vectors are then not determined by the
+  // data and a padded solve may return one that lives on the padding.
   std::vector<uint8_t> padded(secs.size(), 0);
   constexpr int kMaxBatch = 32;
+  auto svd_batch = [&](const std::vector<size_t>& list, int mp, int np) {
+    const int cnt = static_cast<int>(list.size());
+    const int kp = std::min(mp, np);
+    const int64_t sa = int64_t(mp) * np, su = int64_t(mp) * kp, sv = int64_t(kp) * np;
+    DevBuf ba(ctx, sa * cnt * 8 * w), bs(ctx, int64_t(kp) * cnt * 8), bu(ctx, su * cnt * 8 * w),
+        bv(ctx, sv * cnt * 8 * w), binfo(ctx, sizeof(int) * cnt);
+    SADMRG_HIP(launch_fill(sa * cnt * w, 0.0, ba.as<double>(), ctx->stream()));
+    for (int q = 0; q < cnt; ++q) {
+      const Sec& s = secs[list[q]];
+      SADMRG_HIP(launch_copy2d(s.m, s.n, vec.block_ptr(s.i + s.j * nm1), 1, s.m, ba.as<double>() + sa * q * w, mp, w,
+                               1.0, nullptr, 0, ctx->stream()));
+    }
+    rocblas_status st;
+    if (!cx)
+      st = rocsolver_dgesdd_strided_batched(h, rocblas_svect_singular, rocblas_svect_singular, mp, np, ba.as<double>(),
+                                            mp, sa, bs.as<double>(), kp, bu.as<double>(), mp, su, bv.as<double>(), kp,
+                                            sv, binfo.as<int>(), cnt);
+    else
+      st = rocsolver_zgesdd_strided_batched(
+          h, rocblas_svect_singular, rocblas_svect_singular, mp, np, ba.as<rocblas_double_complex>(), mp, sa,
+          bs.as<double>(), kp, bu.as<rocblas_double_complex>(), mp, su, bv.as<rocblas_double_complex>(), kp, sv,
+          binfo.as<int>(), cnt);
+    if (st != rocblas_status_success) fail(kErrNumerics, "rocsolver gesdd (batched) failed");
+    for (int q = 0; q < cnt; ++q) {
+      const size_t si = list[q];
+      const Sec& s = secs[si];
+      padded[si] = 1;
+      SADMRG_HIP(hipMemcpyAsync(ds.as<double>() + s.s_off, bs.as<double>() + int64_t(kp) * q, int64_t(s.mn) * 8,
+                                hipMemcpyDeviceToDevice, ctx->stream()));
+      SADMRG_HIP(hipMemcpyAsync(dinfo.as<int>() + si, binfo.as<int>() + q, sizeof(int), hipMemcpyDeviceToDevice,
+                                ctx->stream()));
+      SADMRG_HIP(launch_copy2d(s.m, s.mn, bu.as<double>() + su * q * w, 1, mp, du.as<double>() + s.u_off * w, s.m, w,
+                               1.0, nullptr, 0, ctx->stream()));
+      SADMRG_HIP(launch_copy2d(s.mn, s.n, bv.as<double>() + sv * q * w, 1, kp, dvt.as<double>() + s.vt_off * w, s.mn,
+                               w, 1.0, nullptr, 0, ctx->stream()));
+    }
+    ctx->sync();  // batch buffers are released at the end of this scope
+  };
   // Density-matrix route for large sectors of sweep steps, where one side
-  // absorbs S (p1, p2) = (1, 0) or (0, 1): the isometric side is the
+  // absorbs S, (p1, p2) = (1, 0) or (0, 1): the isometric side is the
   // eigenbasis of the Gram matrix of that side (psi^T psi for V, psi psi^T for
-  // U; strided-batched syevd, descending order), the other side is psi times
-  // it, divided by S -- the projector builder multiplies S back in, so the
-  // division is undone exactly and small S cost no accuracy there.  A Gram
-  // eigenvalue resolves S only down to ~sqrt(eps) S_max: eigenvalues below
-  // kGramRes * lambda_max are set to zero, which drops those states from the
-  // kept candidates (the SVD route keeps S/S_max > 1e-10).
-  constexpr int kGramMin = 128;
+  // U; strided-batched syevd), the other side is psi times it divided by S --
+  // the projector builder multiplies S back in, so the division is undone
+  // exactly and small S cost no accuracy there.  A Gram eigenvalue lambda_k
+  // determines S_k = sqrt(lambda_k) only while lambda_k > kGramRes lambda_max;
+  // below that ("unresolved") S_k is reported as the norm of psi times the
+  // eigenvector, which is exact to round-off of S_max but, inside a cluster of
+  // unresolved eigenvalues, belongs to an arbitrary mixture of those singular
+  // vectors.  The route is therefore only accepted when no unresolved value
+  // can take part in the kept set (see the check after the decompositions);
+  // otherwise the sector is decomposed again by gesdd.
+  const int gmin = gram_min();
   constexpr double kGramRes = 1e-13;
   const bool right_gram = (p1 == 1.0 && p2 == 0.0), left_gram = (p1 == 0.0 && p2 == 1.0);
-  const bool gram = !cx && (right_gram || left_gram);
+  // the route can only be accepted when the cap can bind: more singular
+  // values in total than max_dcut
+  const bool gram = gmin > 0 && !cx && (right_gram || left_gram) && max_dcut > 0 && s_tot > max_dcut;
   std::vector<uint8_t> gram_sec(secs.size(), 0);
   std::vector<double> gram_s(s_tot, 0.0);
-  auto gram_batch = [&](size_t k0, size_t k1, int mp, int np) {
-    const int cnt = static_cast<int>(k1 - k0);
+  std::vector<double> gram_res(secs.size(), 0.0);  // > 0: the sector has unresolved values below this
+  std::vector<int> gram_nres(secs.size(), 0);        // resolved values of a Gram sector
+  auto gram_batch = [&](const std::vector<size_t>& list, int mp, int np) {
+    const int cnt = static_cast<int>(list.size());
     const int g = right_gram ? np : mp;  // Gram order
     const int64_t sa = int64_t(mp) * np, sg = int64_t(g) * g;
     DevBuf ba(ctx, sa * cnt * 8), bg(ctx, sg * cnt * 8), by(ctx, sa * cnt * 8), bd(ctx, int64_t(g) * cnt * 8),
-        be(ctx, int64_t(g) * cnt * 8), binfo(ctx, sizeof(int) * cnt);
+        be(ctx, int64_t(g) * cnt * 8), bn(ctx, int64_t(g) * cnt * 8), binfo(ctx, sizeof(int) * cnt);
     SADMRG_HIP(launch_fill(sa * cnt, 0.0, ba.as<double>(), ctx->stream()));
     for (int q = 0; q < cnt; ++q) {
-      const Sec& sc = secs[order[k0 + q]];
+      const Sec& sc = secs[list[q]];
       SADMRG_HIP(launch_copy2d(sc.m, sc.n, vec.block_ptr(sc.i + sc.j * nm1), 1, sc.m, ba.as<double>() + sa * q, mp,
                                1, 1.0, nullptr, 0, ctx->stream()));
     }
@@ -1354,133 +1411,110 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
                                          bd.as<double>(), g, be.as<double>(), g, binfo.as<int>(), cnt) !=
         rocblas_status_success)
       fail(kErrNumerics, "rocsolver syevd (Gram) failed");
-    std::vector<double> hd(int64_t(g) * cnt);
-    std::vector<int> hi(cnt);
-    SADMRG_HIP(hipMemcpyAsync(hd.data(), bd.get(), hd.size() * 8, hipMemcpyDeviceToHost, ctx->stream()));
-    SADMRG_HIP(hipMemcpyAsync(hi.data(), binfo.get(), hi.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
-    ctx->sync();
-    // S (descending) and 1/S per sector; the eigenvectors are G's columns,
-    // ascending, so column g-1-k is the k-th largest
-    std::vector<double> inv(int64_t(g) * cnt, 0.0);
-    for (int q = 0; q < cnt; ++q) {
-      const size_t si = order[k0 + q];
-      const Sec& sc = secs[si];
-      if (hi[q] != 0) fail(kErrNumerics, "Dense::svd: syevd info=" + std::to_string(hi[q]));
-      const double* lam = hd.data() + int64_t(g) * q;
-      const double lmax = std::max(lam[g - 1], 0.0);
-      for (int k = 0; k < sc.mn; ++k) {
-        const double l = lam[g - 1 - k];
-        const double sv = (l > kGramRes * lmax && l > 0.0) ? std::sqrt(l) : 0.0;
-        gram_s[sc.s_off + k] = sv;
-        inv[int64_t(g) * q + k] = sv > 0.0 ? 1.0 / sv : 0.0;
-      }
-      gram_sec[si] = 1;
-    }
-    DevBuf dinv(ctx, inv.size() * 8);
-    SADMRG_HIP(hipMemcpyAsync(dinv.get(), inv.data(), inv.size() * 8, hipMemcpyHostToDevice, ctx->stream()));
+    // Y = psi times the eigenvectors: right, Y = A V (mp x np, column c of Y
+    // belongs to eigenvector c); left, Y = G^T A (row c of Y to eigenvector c).
+    // bn receives the norms of those columns / rows.
     if (right_gram) {
-      // Y = A V (mp x np); U = Y[:, desc] / S, V' = V[:, desc]^T
       if (rocblas_dgemm_strided_batched(rb, rocblas_operation_none, rocblas_operation_none, mp, np, np, &one,
                                         ba.as<double>(), mp, sa, bg.as<double>(), np, sg, &zero, by.as<double>(), mp,
                                         sa, cnt) != rocblas_status_success)
         fail(kErrNumerics, "rocblas gemm (Gram) failed");
-      for (int q = 0; q < cnt; ++q) {
-        const Sec& sc = secs[order[k0 + q]];
-        SADMRG_HIP(launch_copy2d(sc.m, sc.mn, by.as<double>() + sa * q + int64_t(np - 1) * mp, 1, -int64_t(mp),
-                                 du.as<double>() + sc.u_off, sc.m, 1, 1.0, dinv.as<double>() + int64_t(g) * q, 0,
-                                 ctx->stream()));
-        SADMRG_HIP(launch_copy2d(sc.mn, sc.n, bg.as<double>() + sg * q + int64_t(np - 1) * np, -int64_t(np), 1,
-                                 dvt.as<double>() + sc.vt_off, sc.mn, 1, 1.0, nullptr, 0, ctx->stream()));
-      }
+      SADMRG_HIP(launch_strided_norms(np * cnt, mp, by.as<double>(), 1, mp, bn.as<double>(), ctx->stream()));
     } else {
-      // U = G[:, desc]; V' row k = (U_k^T A) / S_k, from Y = (G diag(1/S'))^T A
-      for (int q = 0; q < cnt; ++q) {
-        const Sec& sc = secs[order[k0 + q]];
-        SADMRG_HIP(launch_copy2d(sc.m, sc.mn, bg.as<double>() + sg * q + int64_t(mp - 1) * mp, 1, -int64_t(mp),
-                                 du.as<double>() + sc.u_off, sc.m, 1, 1.0, nullptr, 0, ctx->stream()));
-      }
-      // scale G's columns in place: column c (ascending) gets 1/S of rank g-1-c
-      std::vector<double> cinv(int64_t(g) * cnt, 0.0);
-      for (int q = 0; q < cnt; ++q)
-        for (int c = 0; c < g; ++c) cinv[int64_t(g) * q + c] = inv[int64_t(g) * q + (g - 1 - c)];
-      DevBuf dcinv(ctx, cinv.size() * 8);
-      SADMRG_HIP(hipMemcpyAsync(dcinv.get(), cinv.data(), cinv.size() * 8, hipMemcpyHostToDevice, ctx->stream()));
-      for (int q = 0; q < cnt; ++q)
-        SADMRG_HIP(launch_copy2d(g, g, bg.as<double>() + sg * q, 1, g, bg.as<double>() + sg * q, g, 1, 1.0,
-                                 dcinv.as<double>() + int64_t(g) * q, 0, ctx->stream()));
       if (rocblas_dgemm_strided_batched(rb, rocblas_operation_transpose, rocblas_operation_none, mp, np, mp, &one,
                                         bg.as<double>(), mp, sg, ba.as<double>(), mp, sa, &zero, by.as<double>(), mp,
                                         sa, cnt) != rocblas_status_success)
         fail(kErrNumerics, "rocblas gemm (Gram) failed");
-      for (int q = 0; q < cnt; ++q) {
-        const Sec& sc = secs[order[k0 + q]];
+      for (int q = 0; q < cnt; ++q)
+        SADMRG_HIP(launch_strided_norms(mp, np, by.as<double>() + sa * q, mp, 1, bn.as<double>() + int64_t(g) * q,
+                                        ctx->stream()));
+    }
+    std::vector<double> hd(int64_t(g) * cnt), hn(int64_t(g) * cnt);
+    std::vector<int> hi(cnt);
+    SADMRG_HIP(hipMemcpyAsync(hd.data(), bd.get(), hd.size() * 8, hipMemcpyDeviceToHost, ctx->stream()));
+    SADMRG_HIP(hipMemcpyAsync(hn.data(), bn.get(), hn.size() * 8, hipMemcpyDeviceToHost, ctx->stream()));
+    SADMRG_HIP(hipMemcpyAsync(hi.data(), binfo.get(), hi.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
+    ctx->sync();
+    // S (descending rank k <-> eigenvector column g-1-k) and the scale 1/S
+    // of the non-isometric factor, by rank (kinv) and by eigenvector column (cinv)
+    std::vector<double> kinv(int64_t(g) * cnt, 0.0), cinv(int64_t(g) * cnt, 0.0);
+    for (int q = 0; q < cnt; ++q) {
+      const size_t si = list[q];
+      const Sec& sc = secs[si];
+      if (hi[q] != 0) fail(kErrNumerics, "Dense::svd: syevd info=" + std::to_string(hi[q]));
+      const double* lam = hd.data() + int64_t(g) * q;
+      const double* nrm = hn.data() + int64_t(g) * q;
+      const double lmax = std::max(lam[g - 1], 0.0);
+      double* S = gram_s.data() + sc.s_off;
+      int nres = 0;
+      for (int k = 0; k < sc.mn; ++k) {
+        const int c = g - 1 - k;
+        const double l = lam[c];
+        const bool resolved = l > kGramRes * lmax && l > 0.0;
+        S[k] = resolved ? std::sqrt(l) : nrm[c];
+        if (resolved) nres = k + 1;
+        cinv[int64_t(g) * q + c] = S[k] > 0.0 ? 1.0 / S[k] : 0.0;
+        kinv[int64_t(g) * q + k] = cinv[int64_t(g) * q + c];
+      }
+      gram_nres[si] = nres;
+      if (nres < sc.mn) {
+        // unresolved tail: values only, in descending order
+        std::sort(S + nres, S + sc.mn, std::greater<double>());
+        gram_res[si] = std::sqrt(kGramRes * lmax) + 1e-15 * std::sqrt(lmax);
+      }
+      gram_sec[si] = 1;
+    }
+    DevBuf dcinv(ctx, cinv.size() * 8), dkinv(ctx, kinv.size() * 8);
+    SADMRG_HIP(hipMemcpyAsync(dcinv.get(), cinv.data(), cinv.size() * 8, hipMemcpyHostToDevice, ctx->stream()));
+    SADMRG_HIP(hipMemcpyAsync(dkinv.get(), kinv.data(), kinv.size() * 8, hipMemcpyHostToDevice, ctx->stream()));
+    for (int q = 0; q < cnt; ++q) {
+      const Sec& sc = secs[list[q]];
+      const double* ci = dcinv.as<double>() + int64_t(g) * q;
+      const double* ki = dkinv.as<double>() + int64_t(g) * q;
+      if (right_gram) {
+        // U = Y[:, desc] / S, V' = V[:, desc]^T
+        SADMRG_HIP(launch_copy2d(sc.m, sc.mn, by.as<double>() + sa * q + int64_t(np - 1) * mp, 1, -int64_t(mp),
+                                 du.as<double>() + sc.u_off, sc.m, 1, 1.0, ki, 0, ctx->stream()));
+        SADMRG_HIP(launch_copy2d(sc.mn, sc.n, bg.as<double>() + sg * q + int64_t(np - 1) * np, -int64_t(np), 1,
+                                 dvt.as<double>() + sc.vt_off, sc.mn, 1, 1.0, nullptr, 0, ctx->stream()));
+      } else {
+        // U = G[:, desc]; V' row k = (U_k^T A) / S_k
+        SADMRG_HIP(launch_copy2d(sc.m, sc.mn, bg.as<double>() + sg * q + int64_t(mp - 1) * mp, 1, -int64_t(mp),
+                                 du.as<double>() + sc.u_off, sc.m, 1, 1.0, nullptr, 0, ctx->stream()));
+        SADMRG_HIP(launch_strided_scale(mp, np, by.as<double>() + sa * q, mp, 1, ci, ctx->stream()));
         SADMRG_HIP(launch_copy2d(sc.mn, sc.n, by.as<double>() + sa * q + (mp - 1), -1, mp,
                                  dvt.as<double>() + sc.vt_off, sc.mn, 1, 1.0, nullptr, 0, ctx->stream()));
       }
     }
     ctx->sync();  // batch buffers are released at the end of this scope
   };
-  for (size_t k0 = 0; k0 < order.size();) {
-    const int lead = std::max(secs[order[k0]].m, secs[order[k0]].n);
-    size_t k1 = k0 + 1;
-    while (k1 < order.size() && static_cast<int>(k1 - k0) < kMaxBatch &&
-           10 * std::max(secs[order[k1]].m, secs[order[k1]].n) >= 6 * lead)
-      ++k1;
-    const int cnt = static_cast<int>(k1 - k0);
-    int mp = 0, np = 0;
-    for (size_t k = k0; k < k1; ++k) {
-      mp = std::max(mp, secs[order[k]].m);
-      np = std::max(np, secs[order[k]].n);
-    }
-    if (gram && std::max(mp, np) >= kGramMin) {
-      gram_batch(k0, k1, mp, np);
+  // Decomposes the listed sectors: groups of similar size, largest first.
+  auto decompose = [&](std::vector<size_t> list, bool allow_gram) {
+    std::stable_sort(list.begin(), list.end(), [&](size_t x, size_t y) {
+      return std::max(secs[x].m, secs[x].n) > std::max(secs[y].m, secs[y].n);
+    });
+    for (size_t k0 = 0; k0 < list.size();) {
+      const int lead = std::max(secs[list[k0]].m, secs[list[k0]].n);
+      size_t k1 = k0 + 1;
+      while (k1 < list.size() && static_cast<int>(k1 - k0) < kMaxBatch &&
+             10 * std::max(secs[list[k1]].m, secs[list[k1]].n) >= 6 * lead)
+        ++k1;
+      std::vector<size_t> part(list.begin() + k0, list.begin() + k1);
+      int mp = 0, np = 0;
+      for (size_t si : part) {
+        mp = std::max(mp, secs[si].m);
+        np = std::max(np, secs[si].n);
+      }
+      if (allow_gram && gram && std::max(mp, np) >= gmin) gram_batch(part, mp, np);
+      else if (part.size() == 1) single(part[0]);
+      else svd_batch(part, mp, np);
       k0 = k1;
-      continue;
     }
-    if (cnt == 1) {
-      single(order[k0]);
-      k0 = k1;
-      continue;
-    }
-    const int kp = std::min(mp, np);
-    const int64_t sa = int64_t(mp) * np, su = int64_t(mp) * kp, sv = int64_t(kp) * np;
-    DevBuf ba(ctx, sa * cnt * 8 * w), bs(ctx, int64_t(kp) * cnt * 8), bu(ctx, su * cnt * 8 * w),
-        bv(ctx, sv * cnt * 8 * w), binfo(ctx, sizeof(int) * cnt);
-    SADMRG_HIP(launch_fill(sa * cnt * w, 0.0, ba.as<double>(), ctx->stream()));
-    for (int q = 0; q < cnt; ++q) {
-      const Sec& s = secs[order[k0 + q]];
-      SADMRG_HIP(launch_copy2d(s.m, s.n, vec.block_ptr(s.i + s.j * nm1), 1, s.m, ba.as<double>() + sa * q * w, mp, w,
-                               1.0, nullptr, 0, ctx->stream()));
-    }
-    rocblas_status st;
-    if (!cx)
-      st = rocsolver_dgesdd_strided_batched(h, rocblas_svect_singular, rocblas_svect_singular, mp, np, ba.as<double>(),
-                                            mp, sa, bs.as<double>(), kp, bu.as<double>(), mp, su, bv.as<double>(), kp,
-                                            sv, binfo.as<int>(), cnt);
-    else
-      st = rocsolver_zgesdd_strided_batched(
-          h, rocblas_svect_singular, rocblas_svect_singular, mp, np, ba.as<rocblas_double_complex>(), mp, sa,
-          bs.as<double>(), kp, bu.as<rocblas_double_complex>(), mp, su, bv.as<rocblas_double_complex>(), kp, sv,
-          binfo.as<int>(), cnt);
-    if (st != rocblas_status_success) fail(kErrNumerics, "rocsolver gesdd (batched) failed");
-    for (int q = 0; q < cnt; ++q) {
-      const size_t si = order[k0 + q];
-      const Sec& s = secs[si];
-      padded[si] = 1;
-      SADMRG_HIP(hipMemcpyAsync(ds.as<double>() + s.s_off, bs.as<double>() + int64_t(kp) * q, int64_t(s.mn) * 8,
-                                hipMemcpyDeviceToDevice, ctx->stream()));
-      SADMRG_HIP(hipMemcpyAsync(dinfo.as<int>() + si, binfo.as<int>() + q, sizeof(int), hipMemcpyDeviceToDevice,
-                                ctx->stream()));
-      SADMRG_HIP(launch_copy2d(s.m, s.mn, bu.as<double>() + su * q * w, 1, mp, du.as<double>() + s.u_off * w, s.m, w,
-                               1.0, nullptr, 0, ctx->stream()));
-      SADMRG_HIP(launch_copy2d(s.mn, s.n, bv.as<double>() + sv * q * w, 1, kp, dvt.as<double>() + s.vt_off * w, s.mn,
-                               w, 1.0, nullptr, 0, ctx->stream()));
-    }
-    ctx->sync();  // batch buffers are released at the end of this scope
-    k0 = k1;
-  }
-  {
-    // redo round-off sectors of padded batches unpadded
+  };
+  // S of every sector on the host; padded sectors whose largest value is
+  // round-off and Gram sectors without a resolved value are redone unpadded.
+  std::vector<double> hs(std::max<int64_t>(s_tot, 1));
+  auto finish = [&]() {
     std::vector<double> lead(secs.size(), 0.0);
     for (size_t si = 0; si < secs.size(); ++si)
       if (padded[si])
@@ -1488,46 +1522,76 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
                                   ctx->stream()));
     ctx->sync();
     for (size_t si = 0; si < secs.size(); ++si)
-      if (padded[si] && lead[si] <= 1e-16) single(si);
-  }
-  // a Gram sector without any resolved value (psi block ~ 0) is decomposed
-  // unpadded, like the SVD route's round-off sectors
-  for (size_t si = 0; si < secs.size(); ++si)
-    if (gram_sec[si] && gram_s[secs[si].s_off] == 0.0) {
-      gram_sec[si] = 0;
-      single(si);
-    }
-  for (size_t si = 0; si < secs.size(); ++si)
-    if (gram_sec[si]) {
-      SADMRG_HIP(hipMemcpyAsync(ds.as<double>() + secs[si].s_off, gram_s.data() + secs[si].s_off,
-                                int64_t(secs[si].mn) * 8, hipMemcpyHostToDevice, ctx->stream()));
-      SADMRG_HIP(hipMemsetAsync(dinfo.as<int>() + si, 0, sizeof(int), ctx->stream()));
-    }
-  std::vector<double> hs(std::max<int64_t>(s_tot, 1));
-  std::vector<int> hinfo(std::max<size_t>(secs.size(), 1));
-  SADMRG_HIP(hipMemcpyAsync(hs.data(), ds.get(), s_tot * 8, hipMemcpyDeviceToHost, ctx->stream()));
-  SADMRG_HIP(hipMemcpyAsync(hinfo.data(), dinfo.get(), secs.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
-  ctx->sync();
-  for (size_t si = 0; si < secs.size(); ++si)
-    if (hinfo[si] != 0) fail(kErrNumerics, "Dense::svd: gesdd info=" + std::to_string(hinfo[si]));
+      if (padded[si] && lead[si] <= 1e-16) {
+        padded[si] = 0;
+        single(si);
+      }
+    for (size_t si = 0; si < secs.size(); ++si)
+      if (gram_sec[si] && gram_nres[si] == 0) {
+        gram_sec[si] = 0;
+        gram_res[si] = 0.0;
+        single(si);
+      }
+    for (size_t si = 0; si < secs.size(); ++si)
+      if (gram_sec[si]) {
+        SADMRG_HIP(hipMemcpyAsync(ds.as<double>() + secs[si].s_off, gram_s.data() + secs[si].s_off,
+                                  int64_t(secs[si].mn) * 8, hipMemcpyHostToDevice, ctx->stream()));
+        SADMRG_HIP(hipMemsetAsync(dinfo.as<int>() + si, 0, sizeof(int), ctx->stream()));
+      }
+    std::vector<int> hinfo(std::max<size_t>(secs.size(), 1));
+    SADMRG_HIP(hipMemcpyAsync(hs.data(), ds.get(), s_tot * 8, hipMemcpyDeviceToHost, ctx->stream()));
+    SADMRG_HIP(hipMemcpyAsync(hinfo.data(), dinfo.get(), secs.size() * sizeof(int), hipMemcpyDeviceToHost,
+                              ctx->stream()));
+    ctx->sync();
+    for (size_t si = 0; si < secs.size(); ++si)
+      if (hinfo[si] != 0) fail(kErrNumerics, "Dense::svd: gesdd info=" + std::to_string(hinfo[si]));
+  };
   // kept per sector: S/S_max > 1e-10, at least one, at most max_dcut
   std::vector<int> dc(nm1, 0);
   struct SV { double v; int sec; int idx; };
   std::vector<SV> all;
-  for (size_t si = 0; si < secs.size(); ++si) {
-    const Sec& s = secs[si];
-    const double* S = hs.data() + s.s_off;
-    const double smax = S[0] > 1e-16 ? S[0] : 1.0;
-    int kept = 0;
-    for (int k = 0; k < s.mn; ++k)
-      if (S[k] / smax > 1e-10) ++kept;
-    if (kept == 0) kept = 1;
-    if (max_dcut > 0 && kept > max_dcut) kept = max_dcut;
-    dc[s.i] = kept;
-    for (int k = 0; k < kept; ++k) all.push_back(SV{S[k], s.i, k});
+  auto select = [&]() {
+    all.clear();
+    for (size_t si = 0; si < secs.size(); ++si) {
+      const Sec& s = secs[si];
+      const double* S = hs.data() + s.s_off;
+      const double smax = S[0] > 1e-16 ? S[0] : 1.0;
+      int kept = 0;
+      for (int k = 0; k < s.mn; ++k)
+        if (S[k] / smax > 1e-10) ++kept;
+      if (kept == 0) kept = 1;
+      if (max_dcut > 0 && kept > max_dcut) kept = max_dcut;
+      dc[s.i] = kept;
+      for (int k = 0; k < kept; ++k) all.push_back(SV{S[k], s.i, k});
+    }
+    // global descending order; ties broken by sector then index (deterministic)
+    std::stable_sort(all.begin(), all.end(), [](const SV& a, const SV& b) { return a.v > b.v; });
+  };
+  std::vector<size_t> every(secs.size());
+  for (size_t k = 0; k < every.size(); ++k) every[k] = k;
+  decompose(every, true);
+  finish();
+  select();
+  {
+    // Gram sectors are accepted when the kept set cannot depend on their
+    // unresolved values: the cut (the smallest kept value) lies a decade
+    // above the sector's resolution limit and the cap max_dcut is binding.
+    const int n0 = static_cast<int>(all.size());
+    const bool binding = max_dcut > 0 && n0 > max_dcut;
+    const double vcut = binding ? all[max_dcut - 1].v : 0.0;
+    std::vector<size_t> redo;
+    for (size_t si = 0; si < secs.size(); ++si)
+      if (gram_sec[si] && gram_res[si] > 0.0 && !(binding && vcut > 10.0 * gram_res[si])) redo.push_back(si);
+    if (!redo.empty()) {
+      for (size_t si : redo) {
+        gram_sec[si] = 0;
+        gram_res[si] = 0.0;
+      }
+      decompose(redo, false);
+      finish();
+      select();
+    }
   }
-  // global descending order; ties broken by sector then index (deterministic)
-  std::stable_sort(all.begin(), all.end(), [](const SV& a, const SV& b) { return a.v > b.v; });
   const int n = static_cast<int>(all.size());
   const int dcut = n > max_dcut ? max_dcut : n;
   std::vector<int> dim(nm1, 0);

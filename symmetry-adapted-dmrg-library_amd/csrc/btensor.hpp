@@ -92,8 +92,15 @@ void op_env_adjoint(BTensor& out, const BTensor& lam, const BTensor& uu, const B
 
 // BlockTensor::svd (block_tensor_inst.cpp:501-637).  tu/tv are overwritten;
 // wout receives kept values (sector order) followed by discarded values.
+// Sweep-step truncations ((p1, p2) = (1, 0) or (0, 1), real data) decompose
+// sectors whose larger side has at least gram_min() rows through the Gram
+// matrix (eigenvectors of psi^T psi or psi psi^T) when that provably leaves
+// the kept set unchanged, and by gesdd otherwise; gram_min() <= 0 turns the
+// Gram route off.  Process-wide, default 256.
 void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, int max_dcut,
             std::vector<double>& wout, std::vector<int>* kept_per_sector = nullptr);
+void set_gram_min(int n);
+int gram_min();
 
 // BlockTensor::fuse(BondType, BondType) (block_tensor_inst.cpp:708-743)
 void op_fuse(BTensor& out, Context* ctx, bool cplx, SymKind sym, const Bond& b1, const Bond& b2);

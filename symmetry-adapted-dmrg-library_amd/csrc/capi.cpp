@@ -548,6 +548,14 @@ int sadmrg_autotune(int enable, int clear, int* entries) {
   });
 }
 
+int sadmrg_svd_gram_min(int value, int* previous) {
+  return guard([&] {
+    if (value < -1) fail(kErrInvalid, "svd_gram_min: value must be >= -1");
+    if (previous) *previous = gram_min();
+    if (value >= 0) set_gram_min(value);
+  });
+}
+
 int sadmrg_lanczos(sadmrg_heff* h, sadmrg_tensor* psi, int mlanc, int precision, int refine, double refine_tol,
                    double* energy, int* counters, double* residual) {
   return guard([&] {
@@ -656,6 +664,82 @@ int sadmrg_dmrg_set_refinement(sadmrg_dmrg* d, int krylov_steps, int max_restart
     o.refine_krylov = krylov_steps;
     o.max_restarts = max_restarts;
     d->d->set_lanczos(o);
+  });
+}
+
+int sadmrg_dmrg_set_check_interval(sadmrg_dmrg* d, int every) {
+  return guard([&] {
+    need(d, "dmrg");
+    if (every < 1) fail(kErrInvalid, "set_check_interval: every >= 1");
+    LanczosOpts o = d->d->lanczos_opts();
+    o.check_every = every;
+    d->d->set_lanczos(o);
+  });
+}
+
+namespace {
+std::string dir_arg(const char* dir) { return dir ? std::string(dir) : std::string(); }
+}  // namespace
+
+int sadmrg_dmrg_save_mps(sadmrg_dmrg* d, const char* dir, int exci) {
+  return guard([&] {
+    need(d, "dmrg");
+    if (exci < 0) fail(kErrInvalid, "save_mps: exci >= 0");
+    d->d->save_mps(dir_arg(dir), exci);
+  });
+}
+
+int sadmrg_dmrg_save_ww(sadmrg_dmrg* d, const char* dir) {
+  return guard([&] {
+    need(d, "dmrg");
+    d->d->save_ww(dir_arg(dir));
+  });
+}
+
+int sadmrg_dmrg_save_enr(sadmrg_dmrg* d, const char* dir) {
+  return guard([&] {
+    need(d, "dmrg");
+    d->d->save_enr(dir_arg(dir));
+  });
+}
+
+int sadmrg_dmrg_read_mps(sadmrg_dmrg* d, const char* dir, int read1, int read2, int* pass) {
+  return guard([&] {
+    need(d, "dmrg");
+    if (read1 < 1) fail(kErrInvalid, "read_mps: read1 >= 1");
+    const bool ok = read2 >= 0 ? d->d->read_mps(dir_arg(dir), read1, read2) : d->d->read_mps(dir_arg(dir), read1);
+    if (pass) *pass = ok ? 1 : 0;
+  });
+}
+
+int sadmrg_dmrg_read_ww(sadmrg_dmrg* d, const char* dir, int read1) {
+  return guard([&] {
+    need(d, "dmrg");
+    d->d->read_ww(dir_arg(dir), read1);
+  });
+}
+
+int sadmrg_dmrg_read_enr(sadmrg_dmrg* d, const char* dir, int read1) {
+  return guard([&] {
+    need(d, "dmrg");
+    d->d->read_enr(dir_arg(dir), read1);
+  });
+}
+
+int sadmrg_dmrg_set_checkpoint_dir(sadmrg_dmrg* d, const char* dir) {
+  return guard([&] {
+    need(d, "dmrg");
+    d->d->set_checkpoint(dir_arg(dir), dir != nullptr);
+  });
+}
+
+int sadmrg_dmrg_ww(sadmrg_dmrg* d, double* w, int cap, int* n) {
+  return guard([&] {
+    need(d, "dmrg");
+    const auto& v = d->d->ww();
+    if (n) *n = static_cast<int>(v.size());
+    if (w)
+      for (int k = 0; k < cap && k < static_cast<int>(v.size()); ++k) w[k] = v[k];
   });
 }
 

@@ -12,7 +12,7 @@ double since(Clock::time_point t0) { return std::chrono::duration<double>(Clock:
 
 Dmrg::Dmrg(Context* ctx, SymKind sym, bool cplx, int xx, int yy, int bondd, int physpn, int totspin)
     : ctx_(ctx), sym_(sym), cplx_(cplx), lx_(xx), ly_(yy), ns_(xx * yy), physpn_(physpn),
-      totspin_(totspin), max_dcut_(bondd) {
+      totspin_(totspin), max_dcut_(bondd), bondd_(bondd) {
   require(ns_ >= 4, "DMRG: at least 4 sites are required");
   require(physpn == 1, "DMRG: only spin-1/2 sites (physpn = 1) are supported");
   hmap_.assign(ns_ * ns_, 0);
@@ -341,7 +341,7 @@ namespace {
 void block_density(BTensor& lam, const BTensor& psi, const BTensor& h, bool left) {
   Context* ctx = psi.ctx;
   std::vector<std::vector<int>> per(h.nten());
-  const int n0 = psi.st.bonds[0].nmom(), n1 = psi.st.bonds[1].nmom();
+  const int n0 = psi.st.bonds[0].nmom();
   const Bond& hb0 = h.st.bonds[0];
   const Bond& hb1 = h.st.bonds[1];
   for (int m = 0; m < h.nten(); ++m) {
@@ -502,7 +502,7 @@ void Dmrg::prepare_sweep() {
   }
 }
 
-// dmrg_inst.cpp:874-943 (checkpoint files are not written)
+// dmrg_inst.cpp:874-943 (checkpoint files only when a directory is set)
 void Dmrg::sweep() {
   BTensor vec;
   for (int i = ns_ / 2; i >= nfree_ + 2; --i) {
@@ -535,6 +535,11 @@ void Dmrg::sweep() {
     svd(vec, i - 2, i - 1, 1.0, 0.0);
   }
   ctx_->sync();
+  if (ckpt_on_) {
+    save_mps(ckpt_dir_, 0);
+    save_ww(ckpt_dir_);
+    save_enr(ckpt_dir_);
+  }
 }
 
 }  // namespace sadmrg

@@ -8,6 +8,7 @@
 #pragma once
 #include <chrono>
 #include <memory>
+#include <string>
 #include <vector>
 #include "btensor.hpp"
 #include "heff.hpp"
@@ -70,6 +71,21 @@ class Dmrg {
   // counterpart.
   std::vector<double> coupling_gradient(double* energy);
 
+  // Checkpoint files in the reference's format (dmrg_inst.cpp:947-1177):
+  // uu-{ns}-{bondd}-{totspin}-{exci}-{i}.dat/.bin per site, ww-...bin,
+  // enr-...dat, in directory dir ("" = the current directory).
+  void save_mps(const std::string& dir, int exci) const;  // save_mps2
+  void save_ww(const std::string& dir) const;
+  void save_enr(const std::string& dir) const;
+  bool read_mps(const std::string& dir, int read1);             // read_mps(int)
+  bool read_mps(const std::string& dir, int read1, int read2);  // + read_ww, read_enr
+  void read_ww(const std::string& dir, int read1);
+  void read_enr(const std::string& dir, int read1);
+  // sweep() ends with save_mps / save_ww / save_enr into dir, as the
+  // reference's does (dmrg_inst.cpp:940-942); off by default.
+  void set_checkpoint(const std::string& dir, bool on) { ckpt_dir_ = dir; ckpt_on_ = on; }
+  const std::vector<double>& ww() const { return wout_; }
+
   BTensor& uu(int i) { return uu_[i]; }
   BTensor& hh(int i) { return hh_[i]; }
   BTensor& opr(int i, int j) { return opr_[i][j]; }
@@ -83,11 +99,14 @@ class Dmrg {
   void initialize_local_operators();
   void boundary_tensors();
   int hm(int i, int j) const { return hmap_[i * ns_ + j]; }
+  std::string file_tag(int dim, int exci) const;
 
   Context* ctx_;
   SymKind sym_;
   bool cplx_;
-  int lx_, ly_, ns_, nfree_ = 2, physpn_, totspin_, max_dcut_;
+  int lx_, ly_, ns_, nfree_ = 2, physpn_, totspin_, max_dcut_, bondd_;
+  std::string ckpt_dir_;
+  bool ckpt_on_ = false;
   std::vector<int> hmap_, fll_, frr_;
   std::vector<double> coup_;
   std::vector<BTensor> uu_, hh_;

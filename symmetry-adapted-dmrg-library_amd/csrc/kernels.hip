@@ -664,6 +664,40 @@ __global__ void copy2d_kernel(int rows, int cols, const double* __restrict__ src
   }
 }
 
+// out[v] = || A[v*s_vec + e*s_elem], e < len ||_2 for v < count: one workgroup
+// per vector (column norms: s_elem = 1, s_vec = ld; row norms: s_elem = ld,
+// s_vec = 1).
+__global__ void __launch_bounds__(256) strided_norms_kernel(int count, int len, const double* __restrict__ a,
+                                                            int64_t s_elem, int64_t s_vec, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int v = blockIdx.x;
+  if (v >= count) return;
+  const double* p = a + v * s_vec;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < len; e += blockDim.x) {
+    const double x = p[e * s_elem];
+    acc += x * x;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[v] = sqrt(red[0]);
+}
+
+// A[v*s_vec + e*s_elem] *= f[v] for v < count, e < len.
+__global__ void strided_scale_kernel(int count, int len, double* __restrict__ a, int64_t s_elem, int64_t s_vec,
+                                     const double* __restrict__ f) {
+  const int64_t total = int64_t(count) * len;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < total; t += stride) {
+    const int64_t v = t / len, e = t % len;
+    a[v * s_vec + e * s_elem] *= f[v];
+  }
+}
+
 // p[t * stride] = v  for t < n  (diagonal / strided fill)
 __global__ void strided_fill_kernel(int64_t n, double v, double* __restrict__ p, int64_t stride) {
   const int64_t g = int64_t(gridDim.x) * blockDim.x;
@@ -758,6 +792,21 @@ hipError_t launch_copy2d(int rows, int cols, const double* src, int64_t s_row, i
   if (total <= 0) return hipSuccess;
   hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for(total)), dim3(256), 0, st, rows, cols, src, s_row, s_col, dst,
                      ldd, w, scale, colscale, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_strided_norms(int count, int len, const double* a, int64_t s_elem, int64_t s_vec, double* out,
+                                hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(strided_norms_kernel, dim3(count), dim3(256), 0, st, count, len, a, s_elem, s_vec, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_strided_scale(int count, int len, double* a, int64_t s_elem, int64_t s_vec, const double* f,
+                                hipStream_t st) {
+  const int64_t total = int64_t(count) * len;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(strided_scale_kernel, dim3(grid_for(total)), dim3(256), 0, st, count, len, a, s_elem, s_vec, f);
   return hipGetLastError();
 }
 

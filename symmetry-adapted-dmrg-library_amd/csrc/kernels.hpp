@@ -55,6 +55,12 @@ hipError_t launch_copy2d(int rows, int cols, const double* src, int64_t s_row, i
                          int64_t ldd, int w, double scale, const double* colscale, int accumulate, hipStream_t st);
 // p[t*stride] = v for t < n
 hipError_t launch_strided_fill(int64_t n, double v, double* p, int64_t stride, hipStream_t st);
+// out[v] = 2-norm of (A[v*s_vec + e*s_elem], e < len) for v < count
+hipError_t launch_strided_norms(int count, int len, const double* a, int64_t s_elem, int64_t s_vec, double* out,
+                                hipStream_t st);
+// A[v*s_vec + e*s_elem] *= f[v] for v < count, e < len
+hipError_t launch_strided_scale(int count, int len, double* a, int64_t s_elem, int64_t s_vec, const double* f,
+                                hipStream_t st);
 
 constexpr int kPartialGrid = 1024;
 

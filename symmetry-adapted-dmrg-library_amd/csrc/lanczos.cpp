@@ -149,16 +149,22 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
     tridiag_eig(m, d, e, z);
     eig = d;
   };
+  bool hv_valid = false;  // hv = H d_vec, from the last convergence check
   auto ritz = [&](int mc) {
     std::vector<double> c(mc);
     for (int j = 0; j < mc; ++j) c[j] = z[j];  // eigenvector 0, leading dimension mc
     D.combine(V, c, d_vec);
+    hv_valid = false;
   };
-  // check_eigenvector (dmrg_inst.cpp:1243-1254): returns |<v,Hv>| / |Hv|
+  // check_eigenvector (dmrg_inst.cpp:1243-1254): returns |<v,Hv>| / |Hv|.
+  // With refinement the check matvec is FP64: the stopping test then sees the
+  // exact operator, and hv = H v is reused as the first residual below.
+  const Prec check_prec = (bulk != Prec::FP64 && opt.refine) ? Prec::FP64 : bulk;
   auto check = [&]() {
     const double nor = D.nrm(d_vec);
     D.scal(1.0 / nor, d_vec);
-    matvec(d_vec, hv, bulk);
+    matvec(d_vec, hv, check_prec);
+    hv_valid = true;
     const double prod = D.nrm(hv);
     const dcmplex ov = D.dot(d_vec, hv);
     return std::abs(ov) / prod;
@@ -206,7 +212,7 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
       done = true;
       break;
     }
-    if (m % 4 == 0) {
+    if (m % opt.check_every == 0) {
       ritz(m);
       const double ov = check();
       out.trace.push_back(eig[0]);
@@ -234,7 +240,8 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
       D.copy(x, col(0));
       const double nor = D.nrm(col(0));
       D.scal(1.0 / nor, col(0));
-      matvec(col(0), col(1), Prec::FP64);
+      if (r == 0 && hv_valid) D.copy(hv, col(1));  // the last check's FP64 H v (x already normalised)
+      else matvec(col(0), col(1), Prec::FP64);
       const double a0 = D.dot(col(1), col(0)).real();
       D.axpby(1.0, -a0, col(0), col(1));
       const double b1 = D.nrm(col(1));

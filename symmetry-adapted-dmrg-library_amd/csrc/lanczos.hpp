@@ -7,10 +7,12 @@
 // of 1).  Krylov vectors stay on the device in FP64; only the tridiagonal
 // (<= 40 x 40) goes to the host.
 //
-// Mixed precision: the bulk matvecs run with FP32 operands on FP32 MFMA; the
-// result is then refined by restarts whose first matvec is FP64 (an exact
-// residual r = H x - (x^T H x) x of the current Ritz vector) followed by a few
-// FP32 Krylov steps on that residual.  The energy returned is the FP64
+// Mixed precision: the Krylov matvecs run with FP32 operands on FP32 MFMA and
+// the convergence-check matvecs in FP64, so the stopping test sees the exact
+// operator and the last check's H x gives the exact residual
+// r = H x - (x^T H x) x of the Ritz vector.  If |r| exceeds refine_tol |E|
+// the vector is refined by restarts: a few FP32 Krylov steps on r, then an
+// FP64 matvec for the new residual.  The energy returned is the FP64
 // Rayleigh quotient of the final vector.
 #pragma once
 #include <vector>
@@ -25,6 +27,7 @@ struct LanczosOpts {
   double refine_tol = 1e-7;   // stop when |r64| <= refine_tol * |E|
   int refine_krylov = 6;      // FP32 Krylov steps per restart
   int max_restarts = 6;
+  int check_every = 4;        // convergence check after every check_every-th step (dmrg_inst.cpp:1334)
 };
 
 struct LanczosOut {

@@ -131,7 +131,7 @@ def snapshot(d, ns):
     return tens
 
 
-def ref_sweep(d, ns, nfree, steps, capture=None):
+def ref_sweep(d, ns, nfree, steps, capture=None, d_max=1 << 30):
     """DMRG::sweep (dmrg_inst.cpp:874-943) step by step; returns per-step records."""
     recs = []
 
@@ -171,7 +171,9 @@ def ref_sweep(d, ns, nfree, steps, capture=None):
             capture["uu_l"] = tl.from_ref(d.tensor("uu", il))
             capture["uu_r"] = tl.from_ref(d.tensor("uu", ir))
             capture["done"] = True
-        recs.append((il, ir, e))
+        nw = int(np.count_nonzero(w))
+        dcut = min(nw, d_max)
+        recs.append((il, ir, e, float(np.sum(w[dcut:nw] ** 2))))
 
     for i in range(ns // 2, nfree + 1, -1):
         step("left", i)
@@ -195,7 +197,7 @@ def dmrg_case(name, sym, cplx, xx, yy, D, lattice, nsweep, capture_at):
     cap = {"when": tuple(capture_at)}
     sweeps = []
     for s in range(nsweep):
-        recs = ref_sweep(d, ns, 2, None, cap if s == 1 else None)
+        recs = ref_sweep(d, ns, 2, None, cap if s == 1 else None, d_max=D)
         sweeps.append(recs)
     tens = {}
     for k, v in cap["state"].items():
@@ -203,10 +205,11 @@ def dmrg_case(name, sym, cplx, xx, yy, D, lattice, nsweep, capture_at):
     for k in ("vec", "heff", "lanczos_vec", "uu_l", "uu_r"):
         tens[k] = cap[k]
     energies = np.asarray([[r[2] for r in recs] for recs in sweeps])
+    trunc = np.asarray([[r[3] for r in recs] for recs in sweeps])
     steps = np.asarray([[r[0], r[1]] for r in sweeps[0]], np.int32)
     tl.save_tensors(os.path.join(HERE, f"{name}.npz"), tens, hmap=hmap, coup=coup,
                     params=np.asarray([sym, cplx, xx, yy, D, ns], np.int32), e_idmrg=e_idmrg,
-                    energies=energies, steps=steps, cap_step=np.asarray([cap["il"], cap["ir"]], np.int32),
+                    energies=energies, trunc=trunc, steps=steps, cap_step=np.asarray([cap["il"], cap["ir"]], np.int32),
                     cap_p=np.asarray([cap["p1"], cap["p2"]]), lanczos_e=cap["lanczos_e"], wout=cap["wout"])
     print(name, "E_idmrg", e_idmrg, "sweep energies", energies[:, -1])
 
@@ -218,7 +221,77 @@ CASES = {
     # open chain and J1-J2 cylinder through the coupling map
     "dmrg_su2_obc12": dict(sym=1, cplx=0, xx=1, yy=12, D=24, lattice="obc", nsweep=2, capture_at=("left", 5)),
     "dmrg_su2_j1j2_4x3": dict(sym=1, cplx=0, xx=3, yy=4, D=24, lattice="j1j2", nsweep=2, capture_at=("right", 4)),
+    # U(1) open chain L=32, m=64 kept states, four sweeps
+    "dmrg_u1_obc32": dict(sym=0, cplx=0, xx=1, yy=32, D=64, lattice="obc", nsweep=4, capture_at=("right", 12)),
 }
+
+
+# ---------------------------------------------------------------------------
+# Checkpoint files written by the reference's own sweep() (dmrg_inst.cpp:940-942:
+# save_mps2, save_ww, save_enr into the working directory), and the resume the
+# reference's driver performs from them (main/main_dmrg_su2_real.cpp:73-80:
+# read_mps(read, read2), prepare_sweep, sweep).
+CKPT_CASES = {
+    "ckpt_su2_ring10": dict(xx=1, yy=10, D=16, lattice=None),
+    "ckpt_su2_obc12": dict(xx=1, yy=12, D=24, lattice="obc"),
+}
+
+
+def ckpt_case(name, xx, yy, D, lattice):
+    import shutil
+    import tempfile
+    rb.init(physpn=1, max_angm=M, max_dcut=D, psize=1)
+    ns = xx * yy
+    out_dir = os.path.join(HERE, name)
+    if os.path.isdir(out_dir):
+        shutil.rmtree(out_dir)
+    os.makedirs(out_dir)
+    cwd = os.getcwd()
+    try:
+        os.chdir(out_dir)
+        d = rb.RefDMRG(1, 0, xx, yy, D, 1, 0)
+        if lattice is not None:
+            d.set_couplings(*lattice)
+        hmap, coup = d.couplings()
+        d.idmrg()
+        d.sweep()  # writes uu-*, ww-*, enr-* into out_dir
+        e_saved = d.energy()
+        tens = {f"uu_{i}": tl.from_ref(d.tensor("uu", i)) for i in range(ns)}
+        ww = d.ww(2 * D)
+        with tempfile.TemporaryDirectory() as tmp:
+            os.chdir(tmp)  # the resumed sweep writes its own files here
+            r = rb.RefDMRG(1, 0, xx, yy, D, 1, 0)
+            if lattice is not None:
+                r.set_couplings(*lattice)
+            ok = r.read(out_dir, D, 0)
+            read_uu = {f"read_uu_{i}": tl.from_ref(r.tensor("uu", i)) for i in range(ns)}
+            e_read = r.energy()
+            r.call("prepare_sweep")
+            r.sweep()
+            e_resume = r.energy()
+            os.chdir(out_dir)
+    finally:
+        os.chdir(cwd)
+    tens.update(read_uu)
+    tl.save_tensors(os.path.join(HERE, f"{name}.npz"), tens, hmap=hmap, coup=coup,
+                    params=np.asarray([1, 0, xx, yy, D, ns], np.int32), e_saved=e_saved, e_read=e_read,
+                    e_resume=e_resume, read_pass=int(ok), ww=ww)
+    print(name, "E saved", e_saved, "read pass", ok, "E read", e_read, "E after resumed sweep", e_resume)
+
+
+def lattice_of(lat, xx, yy):
+    ns = xx * yy
+    if lat == "obc":
+        return tl.open_chain(ns), np.asarray([0.0, 1.0] + [0.0] * (ns - 1))
+    if lat == "j1j2":
+        return tl.j1j2_cylinder(yy, xx), np.asarray([0.0, 1.0, 0.5] + [0.0] * (ns - 2))
+    return None
+
+
+def run_ckpt(name):
+    c = dict(CKPT_CASES[name])
+    lat = c.pop("lattice")
+    ckpt_case(name, lattice=lattice_of(lat, c["xx"], c["yy"]), **c)
 
 
 def run_case(name):
@@ -240,13 +313,16 @@ def main(argv):
     if len(argv) > 1 and argv[1] in CASES:
         run_case(argv[1])
         return
+    if len(argv) > 1 and argv[1] in CKPT_CASES:
+        run_ckpt(argv[1])
+        return
     if len(argv) > 1 and argv[1] == "cgc":
         make_cgc()
         return
     if len(argv) > 1 and argv[1] == "hvm":
         make_hvm()
         return
-    for sub in ["cgc", "hvm"] + list(CASES):
+    for sub in ["cgc", "hvm"] + list(CASES) + list(CKPT_CASES):
         subprocess.run([sys.executable, __file__, sub], check=True)
 
 

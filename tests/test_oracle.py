@@ -10,7 +10,7 @@ import sadmrg_testlib as tl
 import sadmrg_oracle as orc
 
 HVM_TAGS = ["su2_real", "u1_real", "u1_complex", "su2_complex"]
-DMRG_CASES = ["dmrg_su2_ring10", "dmrg_u1_ring10", "dmrg_su2_obc12", "dmrg_su2_j1j2_4x3"]
+DMRG_CASES = ["dmrg_su2_ring10", "dmrg_u1_ring10", "dmrg_su2_obc12", "dmrg_su2_j1j2_4x3", "dmrg_u1_obc32"]
 
 
 # ---- closed forms -------------------------------------------------------------------
