@@ -94,6 +94,34 @@ def cpu_baseline(wl, budget_s=20.0):
             "output": out}
 
 
+# Template arguments of grouped_gemm_kernel per tile configuration (kernels.hip):
+# <T, TOut, TM, TN, TK, waves per SIMD, prefetch depth>.
+_CFG_ARGS = {"fp32": {0: "64, 64, 32, 2, 2", 1: "128, 128, 32, 2, 1", 2: "128, 64, 32, 2, 2"},
+             "fp64": {0: "64, 64, 16, 2, 2", 1: "128, 128, 16, 1, 2", 2: "128, 64, 16, 2, 2"}}
+
+
+def gemm_kernel_name(prec, stage, cfg):
+    """Kernel symbol of an H_eff stage (stage 1 writes FP32 intermediates on
+    the FP32 path, stage 2 writes the FP64 output)."""
+    if prec == "fp64":
+        return f"sadmrg::grouped_gemm_kernel<double, double, {_CFG_ARGS['fp64'][cfg]}>"
+    tout = "float" if stage == 1 else "double"
+    return f"sadmrg::grouped_gemm_kernel<float, {tout}, {_CFG_ARGS['fp32'][cfg]}>"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a copy of known size) from the committed rocprofv3 counter
+    summary of this workload; None if the summary does not cover it."""
+    path = os.path.join(ROOT, "profiles", "grouped_gemm_heff_su2_D2048_hbm_counters.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if k is None else int(k["read_bytes"] + k["write_bytes"])
+
+
 def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6, check_every=(4,)):
     """Mixed vs FP64 sweep on the SU(2) J1-J2 cylinder (width x length, snake
     ordering).  Warm-up: iDMRG at warm[0] then one mixed sweep per entry of
@@ -126,12 +154,13 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refi
         c.set_refinement(refine_krylov, 6)
         c.set_check_interval(every)
         ctx.sync()
+        sad.svd_stats(reset=True)
         t1 = time.perf_counter()
         c.sweep()
         ctx.sync()
         wall = time.perf_counter() - t1
         recs = c.records()
-        out[name] = {"seconds": wall, "energy": c.energy(), "recs": recs}
+        out[name] = {"seconds": wall, "energy": c.energy(), "recs": recs, "svd": sad.svd_stats(reset=True)}
         print(f"[bench] {name} sweep: {wall:.2f} s, E={c.energy():.12f}, steps={len(recs)}",
               file=sys.stderr, flush=True)
         del c
@@ -155,6 +184,7 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refi
         "matvecs_fp64": int(tsum("fp64", "matvec64")),
         "matvecs_mixed": [int(tsum("mixed", "matvec32")), int(tsum("mixed", "matvec64"))],
         "mixed_check_every": check_every[0],
+        "truncation": {"fp64": out["fp64"]["svd"], "mixed": out["mixed"]["svd"]},
         "variants": {name: {"sweep_s": round(out[name]["seconds"], 3),
                             "speedup": round(out["fp64"]["seconds"] / out[name]["seconds"], 3),
                             "final_rel_dE": float(abs(out[name]["energy"] - out["fp64"]["energy"]) /
@@ -327,6 +357,8 @@ def main():
                               check_every=tuple(int(x) for x in args.check_every.split(",")))
 
     dom = "stage2" if res["fp32"]["stage2_ms"] >= res["fp32"]["stage1_ms"] else "stage1"
+    dom_kernel = gemm_kernel_name("fp32", 2 if dom == "stage2" else 1, res["fp32"]["tile_cfg"][1 if dom == "stage2" else 0])
+    traffic = measured_traffic(dom_kernel)
     dom_flops = f2 if dom == "stage2" else f1
     dom_ms = res["fp32"][dom + "_ms"]
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
@@ -353,9 +385,10 @@ def main():
         "mixed_vs_fp64_speedup": round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3),
         "fp32_rel_err_vs_fp64": rel32,
         "sweep": sweep,
-        "roofline": {"bound": "mfma", "kernel": f"grouped_gemm {dom} (fp32)", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "kernel": f"grouped_gemm {dom} (fp32): {dom_kernel}",
+                     "achieved": round(achieved, 3),
                      "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4),
-                     "traffic": None,
+                     "traffic": traffic,
                      "fp64_path": {"achieved": round(res["fp64"]["tflops"], 3), "peak": PEAK_FP64,
                                    "frac": round(res["fp64"]["tflops"] / PEAK_FP64, 4)}},
         "cpu_baseline": None if cpu is None else {

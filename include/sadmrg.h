@@ -152,6 +152,11 @@ int sadmrg_autotune(int enable, int clear, int* entries);
    the setting; *previous receives the old value.  Process-wide, default 256.
    No reference counterpart (the reference always calls gesvd). */
 int sadmrg_svd_gram_min(int value, int* previous);
+/* Truncation counters (process-wide): counts[4] = {svd calls, sectors
+   decomposed through the Gram route, sectors decomposed by gesdd, Gram
+   sectors redone by gesdd}; *seconds = wall time inside svd; reset != 0
+   clears them after reading.  No reference counterpart. */
+int sadmrg_svd_stats(long long* counts, double* seconds, int reset);
 /* Lanczos::lanczos1 (lanczos.hpp:70) on the plan; psi is the start vector on
    entry and the Ritz vector on return.  counters[4] = {FP64 matvecs, FP32
    matvecs, Lanczos steps, refinement restarts}. */

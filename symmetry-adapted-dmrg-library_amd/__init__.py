@@ -85,6 +85,7 @@ def lib():
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
         "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_svd_gram_min": (i, [i, ip]),
+        "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_destroy": (i, [vp]),
         "sadmrg_dmrg_ns": (i, [vp, ip]),
@@ -411,6 +412,15 @@ def autotune(enable=None, clear=False):
     n = C.c_int()
     check(lib().sadmrg_autotune(-1 if enable is None else int(bool(enable)), int(bool(clear)), C.byref(n)))
     return n.value
+
+
+def svd_stats(reset=False):
+    """Truncation counters since the last reset (process-wide)."""
+    c = (C.c_longlong * 4)()
+    sec = C.c_double()
+    check(lib().sadmrg_svd_stats(c, C.byref(sec), int(bool(reset))))
+    return {"calls": c[0], "gram_sectors": c[1], "gesdd_sectors": c[2], "gram_redone": c[3],
+            "seconds": sec.value}
 
 
 def _path(directory):

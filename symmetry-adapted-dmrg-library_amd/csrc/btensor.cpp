@@ -1,6 +1,7 @@
 // btensor.cpp -- see btensor.hpp.
 #include "btensor.hpp"
 #include <atomic>
+#include <chrono>
 #include <functional>
 #include <algorithm>
 #include <cmath>
@@ -1247,14 +1248,18 @@ void op_env_adjoint(BTensor& out, const BTensor& lam, const BTensor& uu, const B
 
 namespace {
 std::atomic<int> g_gram_min{256};
+SvdStats g_svd_stats;
 }
 void set_gram_min(int n) { g_gram_min.store(n); }
 int gram_min() { return g_gram_min.load(); }
+SvdStats& svd_stats() { return g_svd_stats; }
 
 // ---------------------------------------------------------------------------
 // Block SVD truncation (block_tensor_inst.cpp:501-637, dense_inst.cpp:447-541).
 void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, int max_dcut,
             std::vector<double>& wout, std::vector<int>* kept_out) {
+  const auto t_start = std::chrono::steady_clock::now();
+  g_svd_stats.calls++;
   Context* ctx = vec.ctx;
   const SymKind sym = vec.sym();
   const bool cx = vec.cplx;
@@ -1489,10 +1494,27 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
     ctx->sync();  // batch buffers are released at the end of this scope
   };
   // Decomposes the listed sectors: groups of similar size, largest first.
+  // The Gram route's eigensolver is launch-latency bound (one chain of small
+  // kernels per matrix column, batched over the matrices), so a step whose
+  // largest sector qualifies sends every sector through one padded batch.
+  constexpr int kMaxGramBatch = 64;
   auto decompose = [&](std::vector<size_t> list, bool allow_gram) {
     std::stable_sort(list.begin(), list.end(), [&](size_t x, size_t y) {
       return std::max(secs[x].m, secs[x].n) > std::max(secs[y].m, secs[y].n);
     });
+    if (allow_gram && gram && !list.empty() && std::max(secs[list[0]].m, secs[list[0]].n) >= gmin) {
+      for (size_t k0 = 0; k0 < list.size(); k0 += kMaxGramBatch) {
+        std::vector<size_t> part(list.begin() + k0, list.begin() + std::min(list.size(), k0 + kMaxGramBatch));
+        int mp = 0, np = 0;
+        for (size_t si : part) {
+          mp = std::max(mp, secs[si].m);
+          np = std::max(np, secs[si].n);
+        }
+        gram_batch(part, mp, np);
+        g_svd_stats.gram_sectors += static_cast<int64_t>(part.size());
+      }
+      return;
+    }
     for (size_t k0 = 0; k0 < list.size();) {
       const int lead = std::max(secs[list[k0]].m, secs[list[k0]].n);
       size_t k1 = k0 + 1;
@@ -1505,9 +1527,9 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
         mp = std::max(mp, secs[si].m);
         np = std::max(np, secs[si].n);
       }
-      if (allow_gram && gram && std::max(mp, np) >= gmin) gram_batch(part, mp, np);
-      else if (part.size() == 1) single(part[0]);
+      if (part.size() == 1) single(part[0]);
       else svd_batch(part, mp, np);
+      g_svd_stats.svd_sectors += static_cast<int64_t>(part.size());
       k0 = k1;
     }
   };
@@ -1574,15 +1596,17 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
   select();
   {
     // Gram sectors are accepted when the kept set cannot depend on their
-    // unresolved values: the cut (the smallest kept value) lies a decade
-    // above the sector's resolution limit and the cap max_dcut is binding.
+    // unresolved values: the cap max_dcut is binding and the cut (the
+    // smallest kept value) lies above twice the sector's resolution limit,
+    // which bounds every unresolved value.
     const int n0 = static_cast<int>(all.size());
     const bool binding = max_dcut > 0 && n0 > max_dcut;
     const double vcut = binding ? all[max_dcut - 1].v : 0.0;
     std::vector<size_t> redo;
     for (size_t si = 0; si < secs.size(); ++si)
-      if (gram_sec[si] && gram_res[si] > 0.0 && !(binding && vcut > 10.0 * gram_res[si])) redo.push_back(si);
+      if (gram_sec[si] && gram_res[si] > 0.0 && !(binding && vcut > 2.0 * gram_res[si])) redo.push_back(si);
     if (!redo.empty()) {
+      g_svd_stats.gram_redo += static_cast<int64_t>(redo.size());
       for (size_t si : redo) {
         gram_sec[si] = 0;
         gram_res[si] = 0.0;
@@ -1662,6 +1686,7 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
   op_contract(tu, left, 2, pu, 0);
   op_contract(tv, rght, 2, pv, 0);
   ctx->sync();
+  g_svd_stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
 
 }  // namespace sadmrg

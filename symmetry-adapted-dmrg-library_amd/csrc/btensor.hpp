@@ -101,6 +101,13 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
             std::vector<double>& wout, std::vector<int>* kept_per_sector = nullptr);
 void set_gram_min(int n);
 int gram_min();
+// Process-wide truncation counters: op_svd calls, sectors decomposed through
+// the Gram route / by gesdd, Gram sectors redone by gesdd, wall seconds.
+struct SvdStats {
+  int64_t calls = 0, gram_sectors = 0, svd_sectors = 0, gram_redo = 0;
+  double seconds = 0;
+};
+SvdStats& svd_stats();
 
 // BlockTensor::fuse(BondType, BondType) (block_tensor_inst.cpp:708-743)
 void op_fuse(BTensor& out, Context* ctx, bool cplx, SymKind sym, const Bond& b1, const Bond& b2);

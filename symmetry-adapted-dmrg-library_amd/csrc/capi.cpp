@@ -548,6 +548,20 @@ int sadmrg_autotune(int enable, int clear, int* entries) {
   });
 }
 
+int sadmrg_svd_stats(long long* counts, double* seconds, int reset) {
+  return guard([&] {
+    SvdStats& st = svd_stats();
+    if (counts) {
+      counts[0] = st.calls;
+      counts[1] = st.gram_sectors;
+      counts[2] = st.svd_sectors;
+      counts[3] = st.gram_redo;
+    }
+    if (seconds) *seconds = st.seconds;
+    if (reset) st = SvdStats{};
+  });
+}
+
 int sadmrg_svd_gram_min(int value, int* previous) {
   return guard([&] {
     if (value < -1) fail(kErrInvalid, "svd_gram_min: value must be >= -1");

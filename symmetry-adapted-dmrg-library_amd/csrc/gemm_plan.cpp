@@ -148,6 +148,10 @@ void GemmBatch::finalize(Context* ctx, bool complex) {
     const int64_t last = trans ? off + (rows - 1) * ld + (cols - 1) : off + (rows - 1) + (cols - 1) * ld;
     if (off < 0 || last >= lim_[a] || (trans ? ld < cols : ld < rows))
       fail(kErrInvalid, std::string("block table out of bounds (") + what + ", arena " + std::to_string(a) + ")");
+    // the kernels address an operand with 32-bit byte offsets from its first
+    // element (16-byte complex elements at most)
+    if ((last - off + 1) > (int64_t(1) << 27))
+      fail(kErrUnsupported, std::string("operand block above 2^27 elements (") + what + ")");
   };
   for (const auto& p : probs_) {
     if (p.p.M <= 0 || p.p.N <= 0) continue;

@@ -39,10 +39,15 @@ template <class T> __device__ inline int acc_row(int g, int r);
 template <> __device__ inline int acc_row<double>(int g, int r) { return g + 4 * r; }
 template <> __device__ inline int acc_row<float>(int g, int r) { return 4 * g + r; }
 
+// The arena bases are device allocations: the pointer is formed in the
+// global address space, so accesses through it compile to global (not FLAT)
+// memory instructions.
 __device__ inline const void* arena_ptr(const ArenaTable& at, int64_t off, int elem_size) {
+  typedef __attribute__((address_space(1))) const char gchar;
   const int a = static_cast<int>(static_cast<uint64_t>(off) >> kArenaShift);
   const int64_t o = off & ((int64_t(1) << kArenaShift) - 1);
-  return static_cast<const char*>(at.base[a]) + o * elem_size;
+  gchar* g = (gchar*)(at.base[a]);
+  return (const void*)(g + o * elem_size);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,7 +200,9 @@ __device__ inline void mfma_step(const T* sa, const T* sb, Acc (&acc)[FM][FN], i
 
 // WPE: waves per SIMD the register allocation must allow (2: two workgroups
 // per CU, 256 registers per lane; 1: one workgroup, 512 registers).
-template <class T, class TOut, int TM, int TN, int TK, int WPE>
+// PF: K steps that global loads run ahead of the LDS stage (1 or 2; 2 holds
+// two register sets of staged operands).
+template <class T, class TOut, int TM, int TN, int TK, int WPE, int PF>
 __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
                                                               const GemmProb* __restrict__ probs,
                                                               const GemmSeg* __restrict__ segs,
@@ -228,7 +235,7 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
   // loader: segment cursor two steps ahead of the MFMAs
   int lseg = tile.seg_lo;
   while (lseg < tile.seg_hi && segs[lseg].K <= 0) ++lseg;
-  int lk0 = 0;
+  int lk0 = 0, lK = 0;
   OperandCursor<T> ca, cb;
   T lscale = T(1);
   bool lkra = false, lkrb = false;
@@ -241,11 +248,12 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
     cursor_init<T, TM, TK>(ca, A, sg.lda, lkra, tile.m0, prob.M, sg.K, tid);
     cursor_init<T, TN, TK>(cb, B, sg.ldb, lkrb, tile.n0, prob.N, sg.K, tid);
     lscale = static_cast<T>(sg.scale);
+    lK = sg.K;
   };
   // advance the loader past the step it just loaded; false when exhausted
   auto next = [&]() {
     lk0 += TK;
-    if (lk0 < segs[lseg].K) {
+    if (lk0 < lK) {
       cursor_advance(ca, TK);
       cursor_advance(cb, TK);
       return true;
@@ -258,55 +266,91 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
     return true;
   };
 
-  T ra[TA::E], rb[TB::E];
-  T pscale = T(1);
-  bool pkra = false, pkrb = false;  // orientation of the data held in ra/rb
-  unsigned skra = 0, skrb = 0;      // bit b: orientation of LDS buffer b
-  bool have = lseg < tile.seg_hi;
-  if (have) {
-    enter(lseg);
-    cursor_load<T, TA::E>(ca, ra);
-    cursor_load<T, TB::E>(cb, rb);
-    tile_store<T, TM, TK>(&lds[0][0], lkra, ra, lscale, tid);
-    tile_store<T, TN, TK>(&lds[0][TA::ELEMS], lkrb, rb, T(1), tid);
-    skra = lkra ? 1u : 0u;
-    skrb = lkrb ? 1u : 0u;
-    bool more = next();
-    if (more) {
-      cursor_load<T, TA::E>(ca, ra);
-      cursor_load<T, TB::E>(cb, rb);
-      pscale = lscale;
-      pkra = lkra;
-      pkrb = lkrb;
+  unsigned skra = 0, skrb = 0;  // bit b: orientation of LDS buffer b
+  auto compute = [&](int buf) {
+    const T* sa = &lds[buf][0];
+    const T* sb = &lds[buf][TA::ELEMS];
+    switch (((skra >> buf) & 1u) | (((skrb >> buf) & 1u) << 1)) {
+      case 0: mfma_step<T, TM, TN, TK, false, false>(sa, sb, acc, wm, wn, lane); break;
+      case 1: mfma_step<T, TM, TN, TK, true, false>(sa, sb, acc, wm, wn, lane); break;
+      case 2: mfma_step<T, TM, TN, TK, false, true>(sa, sb, acc, wm, wn, lane); break;
+      default: mfma_step<T, TM, TN, TK, true, true>(sa, sb, acc, wm, wn, lane); break;
     }
-    __syncthreads();
-    for (int s = 0;; ++s) {
-      const int cb_ = s & 1, nb = cb_ ^ 1;
-      const bool has_next = more;
-      if (has_next) {
-        tile_store<T, TM, TK>(&lds[nb][0], pkra, ra, pscale, tid);
-        tile_store<T, TN, TK>(&lds[nb][TA::ELEMS], pkrb, rb, T(1), tid);
-        skra = (skra & ~(1u << nb)) | (pkra ? 1u << nb : 0u);
-        skrb = (skrb & ~(1u << nb)) | (pkrb ? 1u << nb : 0u);
-        more = next();
-        if (more) {
-          cursor_load<T, TA::E>(ca, ra);
-          cursor_load<T, TB::E>(cb, rb);
-          pscale = lscale;
-          pkra = lkra;
-          pkrb = lkrb;
-        }
-      }
-      const T* sa = &lds[cb_][0];
-      const T* sb = &lds[cb_][TA::ELEMS];
-      switch (((skra >> cb_) & 1u) | (((skrb >> cb_) & 1u) << 1)) {
-        case 0: mfma_step<T, TM, TN, TK, false, false>(sa, sb, acc, wm, wn, lane); break;
-        case 1: mfma_step<T, TM, TN, TK, true, false>(sa, sb, acc, wm, wn, lane); break;
-        case 2: mfma_step<T, TM, TN, TK, false, true>(sa, sb, acc, wm, wn, lane); break;
-        default: mfma_step<T, TM, TN, TK, true, true>(sa, sb, acc, wm, wn, lane); break;
-      }
-      if (!has_next) break;
+  };
+  // one staged K step in registers, with the orientation and scale it was
+  // loaded with
+  struct Staged {
+    T a[TA::E], b[TB::E];
+    T scale;
+    bool kra, krb, valid;
+  };
+  bool more = lseg < tile.seg_hi;
+  if (more) enter(lseg);
+  auto fetch = [&](Staged& r) {
+    r.valid = more;
+    if (!more) return;
+    cursor_load<T, TA::E>(ca, r.a);
+    cursor_load<T, TB::E>(cb, r.b);
+    r.scale = lscale;
+    r.kra = lkra;
+    r.krb = lkrb;
+    more = next();
+  };
+  auto put = [&](const Staged& r, int buf) {
+    tile_store<T, TM, TK>(&lds[buf][0], r.kra, r.a, r.scale, tid);
+    tile_store<T, TN, TK>(&lds[buf][TA::ELEMS], r.krb, r.b, T(1), tid);
+    skra = (skra & ~(1u << buf)) | (r.kra ? 1u << buf : 0u);
+    skrb = (skrb & ~(1u << buf)) | (r.krb ? 1u << buf : 0u);
+  };
+  if constexpr (PF == 1) {
+    // global loads one K step ahead of the LDS stage
+    Staged r;
+    fetch(r);
+    if (r.valid) {
+      put(r, 0);
+      fetch(r);
       __syncthreads();
+      for (int s = 0;; ++s) {
+        const int cur = s & 1, nb = cur ^ 1;
+        const bool has_next = r.valid;
+        if (has_next) {
+          put(r, nb);
+          fetch(r);
+        }
+        compute(cur);
+        if (!has_next) break;
+        __syncthreads();
+      }
+    }
+  } else {
+    // global loads two K steps ahead of the LDS stage, alternating between
+    // two register sets (the loop is unrolled by two so that each set is a
+    // fixed set of registers)
+    Staged r0, r1;
+    fetch(r0);
+    if (r0.valid) {
+      put(r0, 0);
+      fetch(r0);
+      fetch(r1);
+      __syncthreads();
+      for (;;) {
+        bool nxt = r0.valid;
+        if (nxt) {
+          put(r0, 1);
+          fetch(r0);
+        }
+        compute(0);
+        if (!nxt) break;
+        __syncthreads();
+        nxt = r1.valid;
+        if (nxt) {
+          put(r1, 0);
+          fetch(r1);
+        }
+        compute(1);
+        if (!nxt) break;
+        __syncthreads();
+      }
     }
   }
 
@@ -489,13 +533,13 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 // ---------------------------------------------------------------------------
 // Launch helpers.  The grid is split in chunks so that gridDim.x stays within
 // limits for very large tile tables.
-template <class T, class TOut, int TM, int TN, int TK, int WPE = 2>
+template <class T, class TOut, int TM, int TN, int TK, int WPE = 2, int PF = 2>
 static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   TOut* part = static_cast<TOut*>(L.partial);
   const int chunk = 1 << 30;
   for (int base = 0; base < L.ntiles; base += chunk) {
     const int n = L.ntiles - base < chunk ? L.ntiles - base : chunk;
-    hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE>), dim3(n), dim3(256), 0, st,
+    hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF>), dim3(n), dim3(256), 0, st,
                        L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base);
   }
   return hipGetLastError();
@@ -511,11 +555,11 @@ hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L,
       if (tile_cfg == 2) return launch_real<double, double, 128, 64, 16>(L, st);
       return launch_real<double, double, 64, 64, 16>(L, st);
     case GemmPrec::F32:
-      if (tile_cfg == 1) return launch_real<float, float, 128, 128, 32>(L, st);
+      if (tile_cfg == 1) return launch_real<float, float, 128, 128, 32, 2, 1>(L, st);
       if (tile_cfg == 2) return launch_real<float, float, 128, 64, 32>(L, st);
       return launch_real<float, float, 64, 64, 32>(L, st);
     case GemmPrec::F32_OUT64:
-      if (tile_cfg == 1) return launch_real<float, double, 128, 128, 32>(L, st);
+      if (tile_cfg == 1) return launch_real<float, double, 128, 128, 32, 2, 1>(L, st);
       if (tile_cfg == 2) return launch_real<float, double, 128, 64, 32>(L, st);
       return launch_real<float, double, 64, 64, 32>(L, st);
     case GemmPrec::Z64: {
