@@ -77,6 +77,10 @@ void Dmrg::copy_state_from(const Dmrg& o) {
     copy_tensor(hh_[i], o.hh_[i]);
     for (int j = 0; j < ns_; ++j) copy_tensor(opr_[i][j], o.opr_[i][j]);
   }
+  // the last Lanczos solve's wave function and bond (coupling_gradient)
+  copy_tensor(last_vec_, o.last_vec_);
+  last_il_ = o.last_il_;
+  last_ir_ = o.last_ir_;
   ctx_->sync();
 }
 

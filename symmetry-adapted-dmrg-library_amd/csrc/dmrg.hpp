@@ -39,8 +39,9 @@ class Dmrg {
   void set_lanczos(const LanczosOpts& o) { lopt_ = o; }
   const LanczosOpts& lanczos_opts() const { return lopt_; }
 
-  // Deep copy of another driver's lattice maps, MPS, environments and
-  // settings (same symmetry, scalar and lattice size); device-to-device.
+  // Deep copy of another driver's lattice maps, MPS, environments, settings
+  // and last Lanczos wave function (same symmetry, scalar and lattice size);
+  // device-to-device.
   void copy_state_from(const Dmrg& o);
 
   void do_idmrg();
