@@ -122,6 +122,105 @@ def measured_traffic(kernel):
     return None if k is None else int(k["read_bytes"] + k["write_bytes"])
 
 
+def c2_run(sad, args):
+    """C2 (SU(2) open chain L=100, D=512 multiplets, FP64 throughout): the
+    centre-bond H_eff*psi (n_op = 1) as one grouped launch per stage vs one
+    launch per output block, beside the reference CPU path on the same
+    tensors; the sector histogram of the bond and of psi."""
+    import sadmrg_testlib as tl
+    wl = tl.build_workload(D=512, n_op=1, sigma=1.2)
+    ctx = sad.Context(0)
+    cgc = sad.Cgc(1)
+    psi = tl.to_dev(ctx, wl["psi"], 1)
+    hl, hr = tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"])
+    o1 = [tl.to_dev(ctx, t) for t in wl["o1"]]
+    o2 = [tl.to_dev(ctx, t) for t in wl["o2"]]
+    plan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
+    st = plan.stats()
+    out, out_u = psi.copy(), psi.copy()
+    d_in = psi.device_ptr()
+    timer = HipTimer(ctx.stream())
+    res = {}
+    for name, unb, d_out in (("grouped", False, out.device_ptr()), ("per_block", True, out_u.device_ptr())):
+        for _ in range(args.warmup):
+            plan.apply_device(d_in, d_out, sad.FP64, unbatched=unb)
+        ctx.sync()
+        e0, e1 = timer.event(), timer.event()
+        timer.record(e0)
+        for _ in range(args.steps):
+            plan.apply_device(d_in, d_out, sad.FP64, unbatched=unb)
+        timer.record(e1)
+        ms = timer.elapsed_ms(e0, e1) / args.steps
+        res[name] = {"ms": round(ms, 4), "tflops": round(st["flops"] / (ms * 1e-3) / 1e12, 3)}
+        print(f"[bench] C2 {name}: {ms:.3f} ms {res[name]['tflops']:.2f} TFLOP/s", file=sys.stderr, flush=True)
+    rel_u, _ = tl.compare(tl.from_dev(out_u), tl.from_dev(out))
+    cpu = None if args.no_cpu else cpu_baseline(wl, args.cpu_budget)
+    line = {"workload": "C2 centre bond: SU(2) open chain L=100, D=512 multiplets, n_op=1, FP64",
+            "n": st["n"], "flops_per_apply": st["flops"], "stage_problems": [st["stage1_problems"], st["stage2_problems"]],
+            "bond_histogram": {"two_j": wl["info"]["two_j_block"], "multiplets": wl["info"]["dims_block"]},
+            "psi_sectors": {"two_j": wl["info"]["two_j_fused"], "dims": wl["info"]["dims_fused"]},
+            "grouped": res["grouped"], "per_block": res["per_block"],
+            "grouped_over_per_block": round(res["per_block"]["ms"] / res["grouped"]["ms"], 3),
+            "per_block_rel_diff": rel_u}
+    if cpu is not None:
+        relc, _ = tl.compare(tl.from_dev(out), cpu["output"])
+        line["cpu_reference"] = {"ms": round(cpu["seconds_per_apply"] * 1e3, 3), "threads": cpu["threads"],
+                                 "rel_diff": relc,
+                                 "grouped_over_cpu": round(cpu["seconds_per_apply"] * 1e3 / res["grouped"]["ms"], 2)}
+    return line
+
+
+def c1_run(sad, args):
+    """C1 (U(1) open chain L=32, m=64 kept states, 4 sweeps, FP64): the
+    engine's iDMRG + 4 sweeps against the reference library's own step
+    sequence on the host (same lattice, same seed); per-sweep energies and
+    truncation errors side by side, wall times of both."""
+    L, m, nsweep = 32, 64, 4
+    hmap = np.asarray(sad.open_chain(L), dtype=np.int32)
+    coup = np.zeros(L + 1)
+    coup[1] = 1.0
+    ctx = sad.Context(0)
+    sad.lib().sadmrg_rng_seed(42)
+    d = sad.DMRG(ctx, sad.U1, 0, 1, L, m)
+    d.set_couplings(hmap, coup)
+    d.set_max_dcut(m)
+    ctx.sync()
+    t0 = time.perf_counter()
+    d.idmrg()
+    gpu_e, gpu_tr = [], []
+    for _ in range(nsweep):
+        d.records()
+        d.sweep()
+        recs = d.records()
+        gpu_e.append(d.energy())
+        gpu_tr.append(recs[-1]["trunc_err"])
+    ctx.sync()
+    gpu_s = time.perf_counter() - t0
+    line = {"workload": "C1: U(1) open chain L=32, m=64 kept states, iDMRG + 4 sweeps, FP64",
+            "gpu_seconds": round(gpu_s, 3), "gpu_sweep_energies": gpu_e, "gpu_last_step_trunc": gpu_tr}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        import refbind as rb
+        if rb.available():
+            from make_golden import ref_sweep
+            rb.init(physpn=1, max_angm=20, max_dcut=m, psize=1)
+            r = rb.RefDMRG(0, 0, 1, L, m, 1, 0)
+            r.set_couplings(hmap, coup)
+            t1 = time.perf_counter()
+            r.idmrg()
+            cpu_e, cpu_tr = [], []
+            for _ in range(nsweep):
+                recs = ref_sweep(r, L, 2, None, None, d_max=m)
+                cpu_e.append(recs[-1][2])
+                cpu_tr.append(recs[-1][3])
+            cpu_s = time.perf_counter() - t1
+            line.update({"cpu_reference_seconds": round(cpu_s, 3), "cpu_threads": 1,
+                         "cpu_sweep_energies": cpu_e, "cpu_last_step_trunc": cpu_tr,
+                         "max_rel_dE_last_sweep": abs(gpu_e[-1] - cpu_e[-1]) / abs(cpu_e[-1])})
+    return line
+
+
 def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6, check_every=(4,)):
     """Mixed vs FP64 sweep on the SU(2) J1-J2 cylinder (width x length, snake
     ordering).  Warm-up: iDMRG at warm[0] then one mixed sweep per entry of
@@ -267,6 +366,9 @@ def main():
                          "the others are timed as variants)")
     ap.add_argument("--gradient", default="none", metavar="L:D:WARM",
                     help="only run the coupling-gradient workload (e.g. 48:1024:128,256,512) and exit")
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c1"],
+                    help="c3: the metric (default); c2: L=100 D=512 grouped vs per-block launch; "
+                         "c1: U(1) L=32 m=64 four sweeps vs the reference")
     ap.add_argument("--sweep", default="6x12:2048:128,256,512,1024",
                     help="WIDTHxLENGTH:D:warm-up Ds for the mixed vs FP64 sweep comparison ('none' skips it)")
     args = ap.parse_args()
@@ -287,6 +389,12 @@ def main():
         ctx.sync()
         print(json.dumps({"calibration": "device-to-device tensor copy", "bytes_read": n * 8,
                           "bytes_written": n * 8, "copies": 5}))
+        return
+    if args.workload == "c2":
+        print(json.dumps(c2_run(sad, args)))
+        return
+    if args.workload == "c1":
+        print(json.dumps(c1_run(sad, args)))
         return
     if args.gradient != "none":
         Ls, Ds, Ws = args.gradient.split(":")

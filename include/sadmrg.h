@@ -134,6 +134,11 @@ int sadmrg_heff_apply_device(sadmrg_heff* h, const void* d_in, void* d_out, int 
 /* one grouped launch of the pair (stage 1: X = psi^T O1, stage 2: output
    blocks); stage 0 runs both.  For per-kernel timing. */
 int sadmrg_heff_apply_stage(sadmrg_heff* h, const void* d_in, void* d_out, int precision, int stage);
+/* The same product with one launch per output block of each stage (no
+   grouping across blocks, no split reductions): the per-block baseline of
+   the grouped launch, as the reference issues one GEMM per block.  No
+   reference counterpart. */
+int sadmrg_heff_apply_unbatched(sadmrg_heff* h, const void* d_in, void* d_out, int precision);
 int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops);
 /* info[8] = {psi blocks, stage-1 problems, stage-2 problems, stage-1 tiles,
    stage-2 tiles, stage-1 tile cfg, stage-2 tile cfg, output tiles with a

@@ -80,6 +80,7 @@ def lib():
         "sadmrg_heff_apply": (i, [vp, vp, vp, i]),
         "sadmrg_heff_apply_device": (i, [vp, vp, vp, i]),
         "sadmrg_heff_apply_stage": (i, [vp, vp, vp, i, i]),
+        "sadmrg_heff_apply_unbatched": (i, [vp, vp, vp, i]),
         "sadmrg_heff_stage_flops": (i, [vp, i, dp]),
         "sadmrg_heff_stats": (i, [vp, dp, dp, llp, ip]),
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
@@ -376,8 +377,11 @@ class Heff:
         check(lib().sadmrg_heff_apply(self.ptr, psi.ptr, out.ptr, precision))
         return out
 
-    def apply_device(self, d_in, d_out, precision=FP64):
-        check(lib().sadmrg_heff_apply_device(self.ptr, C.c_void_p(d_in), C.c_void_p(d_out), precision))
+    def apply_device(self, d_in, d_out, precision=FP64, unbatched=False):
+        """H_eff psi on raw device arrays; unbatched: one launch per output
+        block (the per-block baseline) instead of the grouped launch."""
+        f = lib().sadmrg_heff_apply_unbatched if unbatched else lib().sadmrg_heff_apply_device
+        check(f(self.ptr, C.c_void_p(d_in), C.c_void_p(d_out), precision))
 
     def apply_stage(self, d_in, d_out, precision=FP64, stage=0):
         check(lib().sadmrg_heff_apply_stage(self.ptr, C.c_void_p(d_in), C.c_void_p(d_out), precision, stage))

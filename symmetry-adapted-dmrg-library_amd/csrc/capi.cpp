@@ -502,6 +502,15 @@ int sadmrg_heff_apply_device(sadmrg_heff* h, const void* d_in, void* d_out, int 
   });
 }
 
+int sadmrg_heff_apply_unbatched(sadmrg_heff* h, const void* d_in, void* d_out, int precision) {
+  return guard([&] {
+    need(h, "heff");
+    need(d_in, "d_in");
+    need(d_out, "d_out");
+    h->p->apply(static_cast<const double*>(d_in), static_cast<double*>(d_out), prec_of(precision), 0, true);
+  });
+}
+
 int sadmrg_heff_apply_stage(sadmrg_heff* h, const void* d_in, void* d_out, int precision, int stage) {
   return guard([&] {
     need(h, "heff");

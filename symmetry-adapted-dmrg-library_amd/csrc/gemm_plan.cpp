@@ -289,4 +289,49 @@ void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int c
   SADMRG_HIP(launch_grouped_gemm(prec, cfg, L, ctx->stream()));
 }
 
+void GemmBatch::run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int cfg) {
+  require(finalized_, "GemmBatch: run before finalize");
+  if (hp_.empty()) return;
+  if (cfg < 0) cfg = default_cfg(prec);
+  require(cfg >= 0 && cfg < kTileConfigs, "GemmBatch: bad tile configuration");
+  require(!(complex_ && cfg != 0), "GemmBatch: complex tables use tile configuration 0");
+  require(!(prec == GemmPrec::Z64) || complex_, "GemmBatch: complex precision on a real table");
+  ProbTiles& pt = pp_[cfg];
+  if (!pt.built) {
+    pt.built = true;
+    const int tm = tile_m(cfg), tn = tile_n(cfg);
+    std::vector<GemmTile> ht;
+    pt.begin.assign(hp_.size() + 1, 0);
+    for (size_t pi = 0; pi < hp_.size(); ++pi) {
+      pt.begin[pi] = static_cast<int>(ht.size());
+      const GemmProb& p = hp_[pi];
+      for (int n0 = 0; n0 < p.N; n0 += tn)
+        for (int m0 = 0; m0 < p.M; m0 += tm)
+          ht.push_back(GemmTile{static_cast<int32_t>(pi), m0, n0, p.seg_begin, p.seg_end, -1, -1, 0});
+    }
+    pt.begin[hp_.size()] = static_cast<int>(ht.size());
+    if (!ht.empty()) {
+      pt.tiles = DevBuf(ctx, ht.size() * sizeof(GemmTile));
+      SADMRG_HIP(hipMemcpyAsync(pt.tiles.get(), ht.data(), ht.size() * sizeof(GemmTile), hipMemcpyHostToDevice,
+                                ctx->stream()));
+      ctx->sync();
+    }
+  }
+  for (size_t pi = 0; pi < hp_.size(); ++pi) {
+    const int n = pt.begin[pi + 1] - pt.begin[pi];
+    if (n <= 0) continue;
+    GemmLaunch L;
+    L.tiles = pt.tiles.as<GemmTile>() + pt.begin[pi];
+    L.probs = d_probs_.as<GemmProb>();
+    L.segs = d_segs_.as<GemmSeg>();
+    L.reduce = nullptr;
+    L.partial = nullptr;
+    L.counters = nullptr;
+    L.arenas = arenas;
+    L.ntiles = n;
+    L.nreduce = 0;
+    SADMRG_HIP(launch_grouped_gemm(prec, cfg, L, ctx->stream()));
+  }
+}
+
 }  // namespace sadmrg

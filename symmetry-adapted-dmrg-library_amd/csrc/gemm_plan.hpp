@@ -42,6 +42,9 @@ class GemmBatch {
   void finalize(Context* ctx, bool complex);
   // cfg < 0: the default configuration for prec (default_cfg).
   void run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int cfg = -1);
+  // One launch per problem, over that problem's tiles only and without split
+  // reductions: the per-block baseline the grouped launch is measured against.
+  void run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int cfg = -1);
   // Default tile configuration for a precision, from the problem shapes:
   // large tiles when most of the work is in problems >= 256 x 256 (128 x 128
   // for FP32 operands, 128 x 64 for FP64, whose wider accumulators would
@@ -71,6 +74,12 @@ class GemmBatch {
     int ntiles = 0, nreduce = 0, nparts = 0;
   };
   TileSet& tileset(Context* ctx, int cfg);
+  struct ProbTiles {
+    bool built = false;
+    DevBuf tiles;
+    std::vector<int> begin;  // tiles of problem p: [begin[p], begin[p + 1])
+  };
+  ProbTiles pp_[kTileConfigs];
   static std::vector<GemmTile> xcd_order(const std::vector<GemmTile>& tiles, const std::vector<GemmSeg>& segs);
 
   std::vector<P> probs_;

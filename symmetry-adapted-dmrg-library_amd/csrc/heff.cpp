@@ -279,7 +279,11 @@ double HeffPlan::stage_flops(int stage) const {
   return g1_.flops(cplx_) + g2_.flops(cplx_);
 }
 
-void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage) {
+void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bool unbatched) {
+  auto exec = [&](GemmBatch& g, GemmPrec p, const ArenaTable& at, int cfg) {
+    if (unbatched) g.run_per_problem(ctx_, p, at, cfg);
+    else g.run(ctx_, p, at, cfg);
+  };
   ArenaTable at;
   std::memset(&at, 0, sizeof(at));
   if (prec == Prec::FP64 || cplx_) {
@@ -291,11 +295,11 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage) {
     const GemmPrec p = cplx_ ? GemmPrec::Z64 : GemmPrec::F64;
     if (stage != 2) {
       stats_.tile_cfg1 = cfg_for(g1_, p, 1, at);
-      g1_.run(ctx_, p, at, stats_.tile_cfg1);
+      exec(g1_, p, at, stats_.tile_cfg1);
     }
     if (stage != 1) {
       stats_.tile_cfg2 = cfg_for(g2_, p, 2, at);
-      g2_.run(ctx_, p, at, stats_.tile_cfg2);
+      exec(g2_, p, at, stats_.tile_cfg2);
     }
     return;
   }
@@ -312,11 +316,11 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage) {
   at.base[kArPack] = pack32_.get();
   if (stage != 2) {
     stats_.tile_cfg1 = cfg_for(g1_, GemmPrec::F32, 1, at);
-    g1_.run(ctx_, GemmPrec::F32, at, stats_.tile_cfg1);
+    exec(g1_, GemmPrec::F32, at, stats_.tile_cfg1);
   }
   if (stage != 1) {
     stats_.tile_cfg2 = cfg_for(g2_, GemmPrec::F32_OUT64, 2, at);
-    g2_.run(ctx_, GemmPrec::F32_OUT64, at, stats_.tile_cfg2);
+    exec(g2_, GemmPrec::F32_OUT64, at, stats_.tile_cfg2);
   }
 }
 

@@ -54,7 +54,9 @@ class HeffPlan {
            const std::vector<const BTensor*>& o2, const BTensor& psi_layout);
 
   // d_in / d_out: device arrays in the layout of psi_layout (n elements).
-  void apply(const double* d_in, double* d_out, Prec prec, int stage = 0);
+  // unbatched: one launch per output block of each stage instead of one
+  // grouped launch (the per-block baseline).
+  void apply(const double* d_in, double* d_out, Prec prec, int stage = 0, bool unbatched = false);
   double stage_flops(int stage) const;
   const HeffStats& stats() const { return stats_; }
   Context* ctx() const { return ctx_; }

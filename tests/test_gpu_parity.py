@@ -344,6 +344,27 @@ def test_heff_split_reduction_matches_reference(ctx, sad, cgc, D, n_op):
     assert rel32 <= 2e-5, (rel32, st)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_heff_per_block_launch_matches_grouped(ctx, sad, cgc, prec):
+    """The per-block baseline (one launch per output block, no split
+    reductions) computes the same product as the grouped launch: FP64 to
+    1e-13 (split-reduction summation order differs), FP32 operands to 1e-5
+    (FP32 partial sums in a different order; the FP32 path is ~2e-6 from
+    FP64 either way)."""
+    wl = tl.build_workload(D=256, n_op=3, seed=5)
+    psi = tl.to_dev(ctx, wl["psi"], 1)
+    hl, hr = tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"])
+    o1 = [tl.to_dev(ctx, t) for t in wl["o1"]]
+    o2 = [tl.to_dev(ctx, t) for t in wl["o2"]]
+    plan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
+    a, b = psi.copy(), psi.copy()
+    plan.apply_device(psi.device_ptr(), a.device_ptr(), prec)
+    plan.apply_device(psi.device_ptr(), b.device_ptr(), prec, unbatched=True)
+    ctx.sync()
+    rel, _ = tl.compare(tl.from_dev(b), tl.from_dev(a))
+    assert rel <= (1e-13 if prec == 0 else 1e-5), rel
+
+
 @pytest.mark.parametrize("name,flag", [("dmrg_su2_obc12", 0), ("dmrg_su2_j1j2_4x3", 0), ("dmrg_su2_obc12", 1),
                                        ("dmrg_su2_j1j2_4x3", 1)])
 def test_overlap_adjoint_identity(ctx, sad, cgc, name, flag):
