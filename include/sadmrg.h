@@ -172,8 +172,24 @@ int sadmrg_lanczos(sadmrg_heff* h, sadmrg_tensor* psi, int mlanc, int precision,
 /* DMRG(xx, yy, bondd, physpn, totspin, 0) (dmrg.hpp:99) */
 int sadmrg_dmrg_create(sadmrg_ctx* ctx, int sym, int cplx, int xx, int yy, int bondd, int physpn, int totspin,
                        sadmrg_dmrg** out);
+/* DMRG(xx, yy, bondd, physpn, totspin, max_exci) (dmrg.hpp:99,
+   dmrg_inst.cpp:63-110): the driver targets state max_exci and keeps
+   stored states 0..max_exci-1 (orth) with their energies.  Every H_eff
+   on two adjacent sites then carries the penalty -E_j |phi_j><phi_j|,
+   phi_j = stored state j projected on the current basis
+   (dmrg_inst.cpp:352-363).  max_exci <= 64; excited states need ns >= 6. */
+int sadmrg_dmrg_create_exci(sadmrg_ctx* ctx, int sym, int cplx, int xx, int yy, int bondd, int physpn, int totspin,
+                            int max_exci, sadmrg_dmrg** out);
 int sadmrg_dmrg_destroy(sadmrg_dmrg* d);
 int sadmrg_dmrg_ns(sadmrg_dmrg* d, int* ns);
+/* max_exci of the driver */
+int sadmrg_dmrg_exci(sadmrg_dmrg* d, int* exci);
+/* stored state j < max_exci := src's MPS and energy (device copy).  The
+   reference loads stored states only from files (read_mps with read2,
+   dmrg_inst.cpp:1082-1141); no reference counterpart. */
+int sadmrg_dmrg_set_lower_state(sadmrg_dmrg* d, int j, const sadmrg_dmrg* src);
+/* gs_enr[j] of stored state j < max_exci (the penalty weight is -gs_enr[j]) */
+int sadmrg_dmrg_lower_energy(sadmrg_dmrg* d, int j, double* e);
 /* deep copy of src's couplings, MPS, environments and settings into dst
    (created with the same lattice, symmetry and scalar type); the reference
    copies a DMRG state only through its checkpoint files (dmrg_inst.cpp:947-1177) */
@@ -196,16 +212,19 @@ int sadmrg_dmrg_set_check_interval(sadmrg_dmrg* d, int every);
    writes): uu-{ns}-{bondd}-{totspin}-{exci}-{i}.dat / .bin per site
    (save_mps2, dmrg.hpp:129), ww-{ns}-{bondd}-{totspin}-{exci}.bin (save_ww,
    dmrg.hpp:130), enr-{ns}-{bondd}-{totspin}-{exci}.dat (save_enr, appended,
-   dmrg.hpp:125).  The engine targets exci = 0; save_mps names the files
-   with the given exci as save_mps2 does. */
+   dmrg.hpp:125).  The own files carry the driver's max_exci as exci;
+   save_mps names the files with the given exci as save_mps2 does. */
 int sadmrg_dmrg_save_mps(sadmrg_dmrg* d, const char* dir, int exci);
 int sadmrg_dmrg_save_ww(sadmrg_dmrg* d, const char* dir);
 int sadmrg_dmrg_save_enr(sadmrg_dmrg* d, const char* dir);
 /* read_mps(read1) (dmrg.hpp:127) when read2 < 0, read_mps(read1, read2)
    (dmrg.hpp:128: also read_ww and read_enr) otherwise; *pass = 1 when every
-   site's files were read, as the reference's return value.  A malformed or
-   short file, or a CGC block that is not the canonical one of its labels,
-   is an error (the reference reads such files unchecked). */
+   site's files were read, as the reference's return value.  With
+   max_exci > 0, read_mps(read1, read2) also loads stored states
+   0..max_exci-1 and their energies from the files of bond dimension read2;
+   missing files of a stored state are an error.  A malformed or short
+   file, or a CGC block that is not the canonical one of its labels, is an
+   error (the reference reads such files unchecked). */
 int sadmrg_dmrg_read_mps(sadmrg_dmrg* d, const char* dir, int read1, int read2, int* pass);
 int sadmrg_dmrg_read_ww(sadmrg_dmrg* d, const char* dir, int read1);   /* read_ww (dmrg.hpp:131) */
 int sadmrg_dmrg_read_enr(sadmrg_dmrg* d, const char* dir, int read1);  /* read_enr (dmrg.hpp:126) */
@@ -225,7 +244,8 @@ int sadmrg_dmrg_energy(sadmrg_dmrg* d, double* e); /* get_enr (dmrg.hpp:102) */
    H_eff flops, reduced elements of the wave function}; *n = number written;
    records are consumed */
 int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n);
-/* which: 0 uu[i], 1 hh[i], 2 opr[i][j] */
+/* which: 0 uu[i], 1 hh[i], 2 opr[i][j]; for stored state j < max_exci:
+   3 orth[j][i], 4 ovlp[j][i], 5 overlapvec[j] (i ignored) */
 int sadmrg_dmrg_get_tensor(sadmrg_dmrg* d, int which, int i, int j, sadmrg_tensor* out);
 int sadmrg_dmrg_set_tensor(sadmrg_dmrg* d, int which, int i, int j, const sadmrg_tensor* t);
 /* step methods: 0 wavefunc_transformation(a,b), 1 prepare_site_operator_from_left(a,b),

@@ -459,10 +459,10 @@ int ref_inner_prod_c(void* x, void* y, double* out2) {
 }
 
 // ---- DMRG driver -----------------------------------------------------------
-void* ref_dmrg_new(int sym, int cplx, int ly, int lx, int bondd, int physpn, int totspin) {
+void* ref_dmrg_new_exci(int sym, int cplx, int ly, int lx, int bondd, int physpn, int totspin, int exci) {
   RefDMRG* h = new RefDMRG{sym, cplx, nullptr};
   DISPATCH(h, {
-    auto* d = new DMRG<S, Y>(ly, lx, bondd, physpn, totspin, 0);
+    auto* d = new DMRG<S, Y>(ly, lx, bondd, physpn, totspin, exci);
     // BlockTensor::svd writes kept + discarded values (block_tensor_inst.cpp:589-600)
     // into ww[exci], which the constructor sizes to bondd (dmrg_inst.cpp:82-85),
     // and save_ww reads bondd*phdim of it; give it room for both.
@@ -473,6 +473,11 @@ void* ref_dmrg_new(int sym, int cplx, int ly, int lx, int bondd, int physpn, int
     h->d = d;
   });
   return h;
+}
+
+// DMRG(ly, lx, bondd, physpn, totspin, max_exci = 0)
+void* ref_dmrg_new(int sym, int cplx, int ly, int lx, int bondd, int physpn, int totspin) {
+  return ref_dmrg_new_exci(sym, cplx, ly, lx, bondd, physpn, totspin, 0);
 }
 
 void ref_dmrg_free(void* p) {
@@ -543,13 +548,29 @@ double ref_dmrg_energy(void* p) {
   return e;
 }
 
-// which: 0 uu[i], 1 hh[i], 2 opr[i][j].  Returns a deep copy.
+// which: 0 uu[i], 1 hh[i], 2 opr[i][j]; for stored state j: 3 orth[j][i],
+// 4 ovlp[j][i], 5 overlapvec[j] (dmrg.hpp:64-67).
+extern "C++" {
+template <class S, class Y>
+static BlockTensor<S, Y>* dmrg_member(DMRG<S, Y>* d, int which, int i, int j) {
+  switch (which) {
+    case 0: return &d->uu[i];
+    case 1: return &d->hh[i];
+    case 2: return &d->opr[i][j];
+    case 3: return &d->orth[j][i];
+    case 4: return &d->ovlp[j][i];
+    default: return &d->overlapvec[j];
+  }
+}
+}
+
+// Returns a deep copy.
 void* ref_dmrg_tensor(void* p, int which, int i, int j) {
   RefDMRG* h = static_cast<RefDMRG*>(p);
   RefTensor* out = nullptr;
   DISPATCH(h, {
     auto* d = asd<S, Y>(h);
-    BlockTensor<S, Y>* src = which == 0 ? &d->uu[i] : (which == 1 ? &d->hh[i] : &d->opr[i][j]);
+    BlockTensor<S, Y>* src = dmrg_member(d, which, i, j);
     // copying a null tensor exits inside SU2Struct::set_su2struct (nbond < 1)
     out = wrap(src->is_null() ? new BlockTensor<S, Y>() : new BlockTensor<S, Y>(*src));
   });
@@ -560,10 +581,18 @@ int ref_dmrg_set_tensor(void* p, int which, int i, int j, void* t) {
   RefDMRG* h = static_cast<RefDMRG*>(p);
   DISPATCH(h, {
     auto* d = asd<S, Y>(h);
-    BlockTensor<S, Y>* dst = which == 0 ? &d->uu[i] : (which == 1 ? &d->hh[i] : &d->opr[i][j]);
+    BlockTensor<S, Y>* dst = dmrg_member(d, which, i, j);
     *dst = *as<S, Y>((RefTensor*)t);
   });
   return 0;
+}
+
+// gs_enr[j] (dmrg.hpp:77): the energy of stored state j, the penalty weight
+double ref_dmrg_state_energy(void* p, int j) {
+  RefDMRG* h = static_cast<RefDMRG*>(p);
+  double e = 0;
+  DISPATCH(h, e = asd<S, Y>(h)->gs_enr[j]);
+  return e;
 }
 
 // Individual sweep-step methods (dmrg_inst.cpp:874-943 calls them in order).

@@ -62,6 +62,8 @@ def lib():
         "ref_tensor_binop": (C.c_int, [vp, C.c_int, vp]),
         "ref_inner_prod_c": (C.c_int, [vp, vp, dp]),
         "ref_dmrg_new": (vp, [C.c_int] * 7),
+        "ref_dmrg_new_exci": (vp, [C.c_int] * 8),
+        "ref_dmrg_state_energy": (C.c_double, [vp, C.c_int]),
         "ref_dmrg_free": (None, [vp]),
         "ref_dmrg_ns": (C.c_int, [vp]),
         "ref_dmrg_set_couplings": (C.c_int, [vp, ip, dp, C.c_int]),
@@ -224,9 +226,9 @@ def env(kind, uu, vv, op1=None, op2=None, flag=0):
 class RefDMRG:
     """Handle to the reference DMRG<Scalar,Sym> (dmrg.hpp:29-133)."""
 
-    def __init__(self, sym, cplx, xx, yy, bondd, physpn=1, totspin=0):
+    def __init__(self, sym, cplx, xx, yy, bondd, physpn=1, totspin=0, exci=0):
         self.sym, self.cplx = sym, cplx
-        self.ptr = C.c_void_p(lib().ref_dmrg_new(sym, cplx, xx, yy, bondd, physpn, totspin))
+        self.ptr = C.c_void_p(lib().ref_dmrg_new_exci(sym, cplx, xx, yy, bondd, physpn, totspin, exci))
         self.ns = lib().ref_dmrg_ns(self.ptr)
 
     def __del__(self):
@@ -256,12 +258,18 @@ class RefDMRG:
     def energy(self):
         return lib().ref_dmrg_energy(self.ptr)
 
+    SLOTS = {"uu": 0, "hh": 1, "opr": 2, "orth": 3, "ovlp": 4, "overlapvec": 5}
+
     def tensor(self, which, i, j=0):
-        code = {"uu": 0, "hh": 1, "opr": 2}[which]
+        """uu[i], hh[i], opr[i][j]; orth[j][i], ovlp[j][i], overlapvec[j]."""
+        code = self.SLOTS[which]
         return RefTensor(lib().ref_dmrg_tensor(self.ptr, code, i, j), self.sym, self.cplx)
 
+    def state_energy(self, j):
+        return lib().ref_dmrg_state_energy(self.ptr, j)
+
     def set_tensor(self, which, i, j, t):
-        code = {"uu": 0, "hh": 1, "opr": 2}[which]
+        code = self.SLOTS[which]
         lib().ref_dmrg_set_tensor(self.ptr, code, i, j, t.ptr)
 
     def call(self, name, a=0, b=0):

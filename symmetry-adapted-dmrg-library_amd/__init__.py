@@ -88,6 +88,10 @@ def lib():
         "sadmrg_svd_gram_min": (i, [i, ip]),
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
+        "sadmrg_dmrg_create_exci": (i, [vp, i, i, i, i, i, i, i, i, pp]),
+        "sadmrg_dmrg_exci": (i, [vp, ip]),
+        "sadmrg_dmrg_set_lower_state": (i, [vp, i, vp]),
+        "sadmrg_dmrg_lower_energy": (i, [vp, i, dp]),
         "sadmrg_dmrg_destroy": (i, [vp]),
         "sadmrg_dmrg_ns": (i, [vp, ip]),
         "sadmrg_dmrg_copy_state": (i, [vp, vp]),
@@ -447,11 +451,12 @@ REC_FIELDS = ("il", "ir", "energy", "trunc_err", "kept", "matvec64", "matvec32",
 class DMRG:
     """DMRG<Scalar,Sym> on the device engine (sadmrg_dmrg)."""
 
-    def __init__(self, ctx, sym, cplx, xx, yy, bondd, physpn=1, totspin=0):
+    def __init__(self, ctx, sym, cplx, xx, yy, bondd, physpn=1, totspin=0, exci=0):
         p = C.c_void_p()
-        check(lib().sadmrg_dmrg_create(ctx.ptr, sym, cplx, xx, yy, bondd, physpn, totspin, C.byref(p)))
+        check(lib().sadmrg_dmrg_create_exci(ctx.ptr, sym, cplx, xx, yy, bondd, physpn, totspin, exci, C.byref(p)))
         self.ptr, self.ctx, self.sym, self.cplx = p, ctx, sym, cplx
         self._xx, self._yy, self._bondd, self._physpn, self._totspin = xx, yy, bondd, physpn, totspin
+        self.exci = exci
         n = C.c_int()
         check(lib().sadmrg_dmrg_ns(p, C.byref(n)))
         self.ns = n.value
@@ -463,14 +468,24 @@ class DMRG:
     def clone(self):
         """A new driver holding a device-side deep copy of this one's state."""
         p = C.c_void_p()
-        check(lib().sadmrg_dmrg_create(self.ctx.ptr, self.sym, self.cplx, self._xx, self._yy, self._bondd,
-                                       self._physpn, self._totspin, C.byref(p)))
+        check(lib().sadmrg_dmrg_create_exci(self.ctx.ptr, self.sym, self.cplx, self._xx, self._yy, self._bondd,
+                                            self._physpn, self._totspin, self.exci, C.byref(p)))
         other = DMRG.__new__(DMRG)
         other.ptr, other.ctx, other.sym, other.cplx, other.ns = p, self.ctx, self.sym, self.cplx, self.ns
+        other.exci = self.exci
         other._xx, other._yy, other._bondd, other._physpn, other._totspin = (
             self._xx, self._yy, self._bondd, self._physpn, self._totspin)
         check(lib().sadmrg_dmrg_copy_state(p, self.ptr))
         return other
+
+    def set_lower_state(self, j, other):
+        """Stored state j (< exci) := other's MPS and energy."""
+        check(lib().sadmrg_dmrg_set_lower_state(self.ptr, j, other.ptr))
+
+    def lower_energy(self, j):
+        e = C.c_double()
+        check(lib().sadmrg_dmrg_lower_energy(self.ptr, j, C.byref(e)))
+        return e.value
 
     def set_couplings(self, hmap, coup):
         h = np.ascontiguousarray(hmap, dtype=np.int32).reshape(-1)
@@ -560,14 +575,18 @@ class DMRG:
             if n.value < 256:
                 return out
 
+    _SLOTS = {"uu": 0, "hh": 1, "opr": 2, "orth": 3, "ovlp": 4, "overlapvec": 5}
+
     def tensor(self, which, i, j=0):
-        code = {"uu": 0, "hh": 1, "opr": 2}[which]
+        """uu[i], hh[i], opr[i][j]; orth[j][i], ovlp[j][i], overlapvec[j] of
+        stored state j."""
+        code = self._SLOTS[which]
         t = Tensor.null(self.ctx, self.sym, self.cplx)
         check(lib().sadmrg_dmrg_get_tensor(self.ptr, code, i, j, t.ptr))
         return t
 
     def set_tensor(self, which, i, j, t):
-        code = {"uu": 0, "hh": 1, "opr": 2}[which]
+        code = self._SLOTS[which]
         check(lib().sadmrg_dmrg_set_tensor(self.ptr, code, i, j, t.ptr))
 
     def call(self, name, a=0, b=0):

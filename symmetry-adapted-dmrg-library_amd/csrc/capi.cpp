@@ -617,6 +617,43 @@ int sadmrg_dmrg_create(sadmrg_ctx* ctx, int sym, int cplx, int xx, int yy, int b
   });
 }
 
+int sadmrg_dmrg_create_exci(sadmrg_ctx* ctx, int sym, int cplx, int xx, int yy, int bondd, int physpn, int totspin,
+                            int max_exci, sadmrg_dmrg** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    if (xx < 1 || yy < 1 || bondd < 1) fail(kErrInvalid, "dmrg: lattice sizes and bondd must be positive");
+    auto d = std::make_unique<sadmrg_dmrg>();
+    d->ctx = ctx->c.get();
+    d->d = std::make_unique<Dmrg>(ctx->c.get(), sym_of(sym), cplx != 0, xx, yy, bondd, physpn, totspin, max_exci);
+    *out = d.release();
+  });
+}
+
+int sadmrg_dmrg_exci(sadmrg_dmrg* d, int* exci) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(exci, "exci");
+    *exci = d->d->exci();
+  });
+}
+
+int sadmrg_dmrg_set_lower_state(sadmrg_dmrg* d, int j, const sadmrg_dmrg* src) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(src, "src");
+    d->d->set_lower_state(j, *src->d);
+  });
+}
+
+int sadmrg_dmrg_lower_energy(sadmrg_dmrg* d, int j, double* e) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(e, "e");
+    *e = d->d->lower_energy(j);
+  });
+}
+
 int sadmrg_dmrg_destroy(sadmrg_dmrg* d) {
   return guard([&] { delete d; });
 }
@@ -823,11 +860,16 @@ int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n) {
 
 static BTensor& dmrg_slot(sadmrg_dmrg* d, int which, int i, int j) {
   const int ns = d->d->ns();
+  if (which == 5) i = 0;
   if (i < 0 || i >= ns || j < 0 || j >= ns) fail(kErrInvalid, "dmrg tensor index out of range");
+  if (which >= 3 && which <= 5 && j >= d->d->exci()) fail(kErrInvalid, "dmrg tensor: stored state index out of range");
   if (which == 0) return d->d->uu(i);
   if (which == 1) return d->d->hh(i);
   if (which == 2) return d->d->opr(i, j);
-  fail(kErrInvalid, "dmrg tensor: which must be 0 (uu), 1 (hh) or 2 (opr)");
+  if (which == 3) return d->d->orth(j, i);
+  if (which == 4) return d->d->ovlp(j, i);
+  if (which == 5) return d->d->overlapvec(j);
+  fail(kErrInvalid, "dmrg tensor: which must be 0 (uu), 1 (hh), 2 (opr), 3 (orth), 4 (ovlp) or 5 (overlapvec)");
 }
 
 int sadmrg_dmrg_get_tensor(sadmrg_dmrg* d, int which, int i, int j, sadmrg_tensor* out) {

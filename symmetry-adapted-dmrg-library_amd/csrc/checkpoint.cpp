@@ -216,7 +216,7 @@ void Dmrg::save_mps(const std::string& dir, int exci) const {
 
 // dmrg_inst.cpp:1146-1153: bondd * phdim doubles of ww[exci]
 void Dmrg::save_ww(const std::string& dir) const {
-  const std::string name = path_in(dir, "ww-" + file_tag(bondd_, 0) + ".bin");
+  const std::string name = path_in(dir, "ww-" + file_tag(bondd_, exci_) + ".bin");
   std::ofstream f(name, std::ios::binary);
   if (!f) fail(kErrInvalid, "checkpoint: cannot write " + name);
   std::vector<double> w(size_t(bondd_) * (physpn_ + 1), 0.0);
@@ -226,7 +226,7 @@ void Dmrg::save_ww(const std::string& dir) const {
 
 // dmrg_inst.cpp:997-1003
 void Dmrg::save_enr(const std::string& dir) const {
-  const std::string name = path_in(dir, "enr-" + file_tag(bondd_, 0) + ".dat");
+  const std::string name = path_in(dir, "enr-" + file_tag(bondd_, exci_) + ".dat");
   std::ofstream f(name, std::ios::app);
   if (!f) fail(kErrInvalid, "checkpoint: cannot write " + name);
   f << std::setprecision(16) << gs_enr_ << "\n";
@@ -234,7 +234,7 @@ void Dmrg::save_enr(const std::string& dir) const {
 
 // dmrg_inst.cpp:1031-1076
 bool Dmrg::read_mps(const std::string& dir, int read1) {
-  const std::string base = "uu-" + file_tag(read1, 0) + "-";
+  const std::string base = "uu-" + file_tag(read1, exci_) + "-";
   bool pass = true;
   for (int i = 0; i < ns_; ++i) {
     BTensor t;
@@ -248,17 +248,32 @@ bool Dmrg::read_mps(const std::string& dir, int read1) {
   return pass;
 }
 
-// dmrg_inst.cpp:1078-1141 at exci = 0 (no stored excited states to read)
-bool Dmrg::read_mps(const std::string& dir, int read1, int /*read2*/) {
+// dmrg_inst.cpp:1078-1141: the driver's own files (tag exci), then the
+// stored states 0..exci-1 from the files of bond dimension read2.  The
+// reference skips a stored state whose files are missing and later works on
+// empty tensors; here that is an error.
+bool Dmrg::read_mps(const std::string& dir, int read1, int read2) {
   const bool pass = read_mps(dir, read1);
+  for (int idx = 0; idx < exci_; ++idx) {
+    const std::string base = "uu-" + file_tag(read2, idx) + "-";
+    for (int i = 0; i < ns_; ++i) {
+      BTensor t;
+      if (!read_site(t, ctx_, sym_, cplx_, path_in(dir, base + std::to_string(i) + ".dat"),
+                     path_in(dir, base + std::to_string(i) + ".bin")))
+        fail(kErrInvalid, "read_mps: stored state " + std::to_string(idx) + " is missing " + base +
+                              std::to_string(i) + ".dat/.bin");
+      orth_[idx][i] = std::move(t);
+    }
+  }
+  ctx_->sync();
   read_ww(dir, read1);
-  read_enr(dir, read1);
+  read_enr(dir, read1, read2);
   return pass;
 }
 
 // dmrg_inst.cpp:1156-1176: read1 doubles, or ones when the file is missing
 void Dmrg::read_ww(const std::string& dir, int read1) {
-  std::ifstream f(path_in(dir, "ww-" + file_tag(read1, 0) + ".bin"), std::ios::binary);
+  std::ifstream f(path_in(dir, "ww-" + file_tag(read1, exci_) + ".bin"), std::ios::binary);
   wout_.assign(std::max(read1, 0), 1.0);
   if (f) {
     std::fill(wout_.begin(), wout_.end(), 0.0);
@@ -268,9 +283,27 @@ void Dmrg::read_ww(const std::string& dir, int read1) {
 
 // dmrg_inst.cpp:1007-1027: the last value of the file
 void Dmrg::read_enr(const std::string& dir, int read1) {
-  std::ifstream f(path_in(dir, "enr-" + file_tag(read1, 0) + ".dat"));
+  std::ifstream f(path_in(dir, "enr-" + file_tag(read1, exci_) + ".dat"));
   double e;
   while (f >> e) gs_enr_ = e;
+}
+
+// the stored states' energies from the files of bond dimension read2 (the
+// penalty weights; a missing file is an error)
+void Dmrg::read_enr(const std::string& dir, int read1, int read2) {
+  read_enr(dir, read1);
+  for (int idx = 0; idx < exci_; ++idx) {
+    const std::string name = path_in(dir, "enr-" + file_tag(read2, idx) + ".dat");
+    std::ifstream f(name);
+    if (!f) fail(kErrInvalid, "read_enr: missing " + name);
+    double e;
+    bool any = false;
+    while (f >> e) {
+      lower_enr_[idx] = e;
+      any = true;
+    }
+    if (!any) fail(kErrInvalid, "read_enr: no energy in " + name);
+  }
 }
 
 }  // namespace sadmrg

@@ -10,10 +10,13 @@ using Clock = std::chrono::steady_clock;
 double since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 }  // namespace
 
-Dmrg::Dmrg(Context* ctx, SymKind sym, bool cplx, int xx, int yy, int bondd, int physpn, int totspin)
+Dmrg::Dmrg(Context* ctx, SymKind sym, bool cplx, int xx, int yy, int bondd, int physpn, int totspin, int max_exci)
     : ctx_(ctx), sym_(sym), cplx_(cplx), lx_(xx), ly_(yy), ns_(xx * yy), physpn_(physpn),
-      totspin_(totspin), max_dcut_(bondd), bondd_(bondd) {
+      totspin_(totspin), max_dcut_(bondd), bondd_(bondd), exci_(max_exci) {
   require(ns_ >= 4, "DMRG: at least 4 sites are required");
+  require(max_exci >= 0 && max_exci <= 64, "DMRG: max_exci must lie in [0, 64]");
+  // the overlap environments of the centre pair reach three sites past it
+  require(max_exci == 0 || ns_ >= 6, "DMRG: excited states need at least 6 sites");
   require(physpn == 1, "DMRG: only spin-1/2 sites (physpn = 1) are supported");
   hmap_.assign(ns_ * ns_, 0);
   fll_.assign(ns_ * ns_, 0);
@@ -41,7 +44,29 @@ Dmrg::Dmrg(Context* ctx, SymKind sym, bool cplx, int xx, int yy, int bondd, int 
   opr_.resize(ns_);
   for (auto& r : opr_) r.resize(ns_);
   if (sym_ == SymKind::SU2) cgc_ = std::make_unique<CgcTables>(physpn_);
+  orth_.resize(exci_);
+  ovlp_.resize(exci_);
+  for (int j = 0; j < exci_; ++j) {
+    orth_[j].resize(ns_);
+    ovlp_[j].resize(ns_);
+  }
+  overlapvec_.resize(exci_);
+  lower_enr_.assign(exci_, 0.0);
   initialize_local_operators();
+}
+
+void Dmrg::set_lower_state(int j, const Dmrg& o) {
+  require(j >= 0 && j < exci_, "set_lower_state: state index out of range");
+  require(o.ns_ == ns_ && o.sym_ == sym_ && o.cplx_ == cplx_ && o.totspin_ == totspin_,
+          "set_lower_state: drivers differ in lattice size, symmetry, scalar type or sector");
+  for (int i = 0; i < ns_; ++i) copy_tensor(orth_[j][i], o.uu_[i]);
+  lower_enr_[j] = o.gs_enr_;
+  ctx_->sync();
+}
+
+double Dmrg::lower_energy(int j) const {
+  require(j >= 0 && j < exci_, "lower_energy: state index out of range");
+  return lower_enr_[j];
 }
 
 void Dmrg::set_couplings(const std::vector<int>& hmap, const std::vector<double>& coup) {
@@ -77,6 +102,15 @@ void Dmrg::copy_state_from(const Dmrg& o) {
     copy_tensor(hh_[i], o.hh_[i]);
     for (int j = 0; j < ns_; ++j) copy_tensor(opr_[i][j], o.opr_[i][j]);
   }
+  require(o.exci_ == exci_, "copy_state_from: drivers target different excited states");
+  for (int j = 0; j < exci_; ++j) {
+    for (int i = 0; i < ns_; ++i) {
+      copy_tensor(orth_[j][i], o.orth_[j][i]);
+      copy_tensor(ovlp_[j][i], o.ovlp_[j][i]);
+    }
+    copy_tensor(overlapvec_[j], o.overlapvec_[j]);
+  }
+  lower_enr_ = o.lower_enr_;
   // the last Lanczos solve's wave function and bond (coupling_gradient)
   copy_tensor(last_vec_, o.last_vec_);
   last_il_ = o.last_il_;
@@ -117,12 +151,57 @@ void Dmrg::boundary_tensors() {
 }
 
 // ---- environment updates ---------------------------------------------------
-// dmrg_inst.cpp:440-511 (truncated basis, exci = 0)
-void Dmrg::prepare_site_operator_from_left(int i, int /*flag*/) {
+// Overlap environments of the stored states (dmrg_inst.cpp:448-449, 493-500):
+// ovlp[j][i] = <orth_j| uu> over sites 0..i; with flag 1 the two sites right
+// of i get the 3-leg pieces prepare_input_vector combines into overlapvec:
+// ovlp[j][i+1] = ovlp[j][i] . orth_j[i+1] and ovlp[j][i+2] = orth_j[i+2] .
+// ovlp[j][i+3] (legs shifted).
+void Dmrg::overlap_env_left(int i, int flag) {
+  CgcTables* cg = cgc_.get();
+  for (int j = 0; j < exci_; ++j) {
+    auto& ov = ovlp_[j];
+    if (i == 0) {
+      op_env(ov[0], kOverlapInitial, orth_[j][0], uu_[0], nullptr, nullptr, 0, cg);
+      continue;
+    }
+    require(!ov[i - 1].is_null(), "overlap environment: the left environment of the stored state is missing");
+    op_env(ov[i], kOverlapTransformation, orth_[j][i], uu_[i], &ov[i - 1], nullptr, 0, cg);
+    if (flag == 1) {
+      require(i + 3 < ns_, "overlap environment: site out of range");
+      op_contract(ov[i + 1], ov[i], 0, orth_[j][i + 1], 0);
+      op_contract(ov[i + 2], orth_[j][i + 2], 1, ov[i + 3], 0);
+      op_shift(ov[i + 2], 1, 0);
+    }
+  }
+}
+
+// mirror image of overlap_env_left (dmrg_inst.cpp:522-523, 567-574)
+void Dmrg::overlap_env_right(int i, int flag) {
+  CgcTables* cg = cgc_.get();
+  for (int j = 0; j < exci_; ++j) {
+    auto& ov = ovlp_[j];
+    if (i == ns_ - 1) {
+      op_env(ov[i], kOverlapInitial, orth_[j][i], uu_[i], nullptr, nullptr, 1, cg);
+      continue;
+    }
+    require(!ov[i + 1].is_null(), "overlap environment: the right environment of the stored state is missing");
+    op_env(ov[i], kOverlapTransformation, orth_[j][i], uu_[i], &ov[i + 1], nullptr, 1, cg);
+    if (flag == 1) {
+      require(i - 3 >= 0, "overlap environment: site out of range");
+      op_contract(ov[i - 2], ov[i - 3], 0, orth_[j][i - 2], 0);
+      op_contract(ov[i - 1], orth_[j][i - 1], 1, ov[i], 0);
+      op_shift(ov[i - 1], 1, 0);
+    }
+  }
+}
+
+// dmrg_inst.cpp:440-511
+void Dmrg::prepare_site_operator_from_left(int i, int flag) {
   const auto t0 = Clock::now();
   CgcTables* cg = cgc_.get();
   if (i == 0) {
     op_env(opr_[0][0], kOperatorInitial, uu_[0], uu_[0], &sigma_[0], nullptr, 0, cg);
+    overlap_env_left(0, 0);
     hh_[0].clean();
     cur_.t_env += since(t0);
     return;
@@ -141,15 +220,17 @@ void Dmrg::prepare_site_operator_from_left(int i, int /*flag*/) {
     op_axpy(hh_[i], 1.0, 0.0, ht);
   }
   op_env(opr_[i][i], kOperatorInitial, uu_[i], uu_[i], &sigma_[0], nullptr, 0, cg);
+  overlap_env_left(i, flag);
   cur_.t_env += since(t0);
 }
 
 // dmrg_inst.cpp:515-583
-void Dmrg::prepare_site_operator_from_right(int i, int /*flag*/) {
+void Dmrg::prepare_site_operator_from_right(int i, int flag) {
   const auto t0 = Clock::now();
   CgcTables* cg = cgc_.get();
   if (i == ns_ - 1) {
     op_env(opr_[i][i], kOperatorInitial, uu_[i], uu_[i], &sigma_[1], nullptr, 1, cg);
+    overlap_env_right(i, 0);
     hh_[i].clean();
     cur_.t_env += since(t0);
     return;
@@ -172,6 +253,7 @@ void Dmrg::prepare_site_operator_from_right(int i, int /*flag*/) {
     op_axpy(hh_[i], 1.0, 0.0, ht);
   }
   op_env(opr_[i][i], kOperatorInitial, uu_[i], uu_[i], &sigma_[1], nullptr, 1, cg);
+  overlap_env_right(i, flag);
   cur_.t_env += since(t0);
 }
 
@@ -235,6 +317,22 @@ void Dmrg::prepare_input_vector_initial(int il, int ir, BTensor& vec) {
   op_contract(tmp3, left, 0, tmp, 0);
   op_contract(vec, tmp3, 1, rght, 0);
   op_makeup_input_vector(vec);
+  if (il + 1 == ir) build_overlapvec(il, ir, t1, t2);
+}
+
+// overlapvec[j]: stored state j projected on the two-site basis of (il, ir)
+// (dmrg_inst.cpp:399-407, 429-436)
+void Dmrg::build_overlapvec(int il, int ir, const BTensor& t1, const BTensor& t2) {
+  for (int j = 0; j < exci_; ++j) {
+    require(!ovlp_[j][il].is_null() && !ovlp_[j][ir].is_null(),
+            "overlap vector: the environments of the stored state are missing");
+    BTensor left, rght;
+    op_env(left, kOverlapInitial, ovlp_[j][il], t1, nullptr, nullptr, 0, cgc_.get());
+    op_env(rght, kOverlapInitial, ovlp_[j][ir], t2, nullptr, nullptr, 1, cgc_.get());
+    if (sym_ == SymKind::SU2) op_take_conjugate(left, 0);
+    op_contract(overlapvec_[j], left, 0, rght, 0);
+    op_makeup_input_vector(overlapvec_[j]);
+  }
 }
 
 // dmrg_inst.cpp:412-437
@@ -248,10 +346,15 @@ void Dmrg::prepare_input_vector(int il, int ir, BTensor& vec) {
   if (sym_ == SymKind::SU2) op_take_conjugate(left, 0);
   op_contract(vec, left, 0, rght, 0);
   op_makeup_input_vector(vec);
+  if (il + 1 == ir) build_overlapvec(il, ir, t1, t2);
   cur_.t_env += since(t0);
 }
 
-void Dmrg::wavefunc_transformation(int i, int flag) { op_vectran(uu_[i], flag != 0); }
+// dmrg_inst.cpp:858-870: the stored states' site tensors follow uu's layout
+void Dmrg::wavefunc_transformation(int i, int flag) {
+  op_vectran(uu_[i], flag != 0);
+  for (int j = 0; j < exci_; ++j) op_vectran(orth_[j][i], flag != 0);
+}
 
 // ---- H_eff and Lanczos ----------------------------------------------------------
 // Bond terms of DMRG::hamiltonian_vector_multiplication_idmrg (dmrg_inst.cpp:310-351):
@@ -288,7 +391,14 @@ std::unique_ptr<HeffPlan> Dmrg::make_heff(int il, int ir, const BTensor& psi,
     }
     hold.push_back(std::move(op));
   }
-  return std::make_unique<HeffPlan>(ctx_, cgc_.get(), hh_[il], hh_[ir], o1, o2, psi);
+  auto plan = std::make_unique<HeffPlan>(ctx_, cgc_.get(), hh_[il], hh_[ir], o1, o2, psi);
+  // orthogonality penalty on adjacent sites (dmrg_inst.cpp:352-363)
+  if (il == ir - 1)
+    for (int j = 0; j < exci_; ++j) {
+      require(!overlapvec_[j].is_null(), "H_eff: the overlap vector of a stored state is missing");
+      plan->add_projector(overlapvec_[j], -lower_enr_[j]);
+    }
+  return plan;
 }
 
 // dmrg_inst.cpp:264-279
@@ -402,6 +512,7 @@ void block_density(BTensor& lam, const BTensor& psi, const BTensor& h, bool left
 
 std::vector<double> Dmrg::coupling_gradient(double* energy) {
   require(last_il_ >= 0 && !last_vec_.is_null(), "coupling gradient: no Lanczos solve has run");
+  require(exci_ == 0, "coupling gradient: ground-state drivers only (max_exci = 0)");
   require(!cplx_, "coupling gradient: real tensors only");
   const int il = last_il_, ir = last_ir_;
   const BTensor& psi = last_vec_;
@@ -540,7 +651,7 @@ void Dmrg::sweep() {
   }
   ctx_->sync();
   if (ckpt_on_) {
-    save_mps(ckpt_dir_, 0);
+    save_mps(ckpt_dir_, exci_);
     save_ww(ckpt_dir_);
     save_enr(ckpt_dir_);
   }

@@ -29,8 +29,9 @@ struct StepRecord {
 
 class Dmrg {
  public:
-  // xx, yy, bondd, physpn, totspin as DMRG(int,int,int,int,int,int) (dmrg_inst.cpp:63-187)
-  Dmrg(Context* ctx, SymKind sym, bool cplx, int xx, int yy, int bondd, int physpn, int totspin);
+  // xx, yy, bondd, physpn, totspin, max_exci as DMRG(int,int,int,int,int,int)
+  // (dmrg_inst.cpp:63-187)
+  Dmrg(Context* ctx, SymKind sym, bool cplx, int xx, int yy, int bondd, int physpn, int totspin, int max_exci = 0);
 
   int ns() const { return ns_; }
   // Replace the lattice map and recompute fll/frr (dmrg_inst.cpp:163-176).
@@ -82,10 +83,26 @@ class Dmrg {
   bool read_mps(const std::string& dir, int read1, int read2);  // + read_ww, read_enr
   void read_ww(const std::string& dir, int read1);
   void read_enr(const std::string& dir, int read1);
+  void read_enr(const std::string& dir, int read1, int read2);  // + stored states' energies
   // sweep() ends with save_mps / save_ww / save_enr into dir, as the
   // reference's does (dmrg_inst.cpp:940-942); off by default.
   void set_checkpoint(const std::string& dir, bool on) { ckpt_dir_ = dir; ckpt_on_ = on; }
   const std::vector<double>& ww() const { return wout_; }
+
+  // Excited states (dmrg_inst.cpp:283-583).  The driver targets state
+  // exci() and keeps the stored states 0..exci()-1 (orth), their overlap
+  // environments with the current MPS (ovlp) and their energies; every
+  // H_eff on two adjacent sites carries the penalty -E_j |phi_j><phi_j|,
+  // phi_j = stored state j projected on the current two-site basis
+  // (overlapvec).  The reference loads the stored states from files
+  // (read_mps with read2); set_lower_state takes them from another driver
+  // (no reference counterpart).
+  int exci() const { return exci_; }
+  void set_lower_state(int j, const Dmrg& o);
+  double lower_energy(int j) const;
+  BTensor& orth(int j, int i) { return orth_.at(j).at(i); }
+  BTensor& ovlp(int j, int i) { return ovlp_.at(j).at(i); }
+  BTensor& overlapvec(int j) { return overlapvec_.at(j); }
 
   BTensor& uu(int i) { return uu_[i]; }
   BTensor& hh(int i) { return hh_[i]; }
@@ -101,6 +118,9 @@ class Dmrg {
   void boundary_tensors();
   int hm(int i, int j) const { return hmap_[i * ns_ + j]; }
   std::string file_tag(int dim, int exci) const;
+  void overlap_env_left(int i, int flag);
+  void overlap_env_right(int i, int flag);
+  void build_overlapvec(int il, int ir, const BTensor& t1, const BTensor& t2);
 
   Context* ctx_;
   SymKind sym_;
@@ -121,6 +141,10 @@ class Dmrg {
   std::vector<double> wout_;
   std::vector<StepRecord> rec_;
   StepRecord cur_{};
+  int exci_ = 0;
+  std::vector<std::vector<BTensor>> orth_, ovlp_;  // [state][site]
+  std::vector<BTensor> overlapvec_;
+  std::vector<double> lower_enr_;
 };
 
 }  // namespace sadmrg

@@ -301,6 +301,7 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
       stats_.tile_cfg2 = cfg_for(g2_, p, 2, at);
       exec(g2_, p, at, stats_.tile_cfg2);
     }
+    if (stage == 0) apply_projectors(d_in, d_out);
     return;
   }
   if (!pack32_.get()) {
@@ -322,6 +323,46 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
     stats_.tile_cfg2 = cfg_for(g2_, GemmPrec::F32_OUT64, 2, at);
     exec(g2_, GemmPrec::F32_OUT64, at, stats_.tile_cfg2);
   }
+  if (stage == 0) apply_projectors(d_in, d_out);
+}
+
+void HeffPlan::add_projector(const BTensor& phi, double weight) {
+  require(phi.st == layout_.st && phi.off == layout_.off && phi.nelem == n_ && phi.cplx == cplx_,
+          "H_eff projector: phi must have the layout of the wave function");
+  const int P = static_cast<int>(proj_w_.size()) + 1;
+  require(P <= 64, "H_eff projector: at most 64 projectors");
+  const int w = cplx_ ? 2 : 1;
+  const size_t col = static_cast<size_t>(std::max<int64_t>(n_, 1)) * w * 8;
+  DevBuf nb(ctx_, 2 * P * col);
+  char* base = nb.as<char>();
+  const char* old = proj_.as<char>();
+  for (int j = 0; j + 1 < P; ++j) {  // keep the earlier columns
+    SADMRG_HIP(hipMemcpyAsync(base + j * col, old + j * col, col, hipMemcpyDeviceToDevice, ctx_->stream()));
+    SADMRG_HIP(hipMemcpyAsync(base + (P + j) * col, old + (P - 1 + j) * col, col, hipMemcpyDeviceToDevice,
+                              ctx_->stream()));
+  }
+  if (n_ > 0) {
+    double* wcol = reinterpret_cast<double*>(base + (2 * P - 1) * col);
+    SADMRG_HIP(hipMemcpyAsync(base + (P - 1) * col, phi.dptr(), n_ * w * 8, hipMemcpyDeviceToDevice, ctx_->stream()));
+    SADMRG_HIP(hipMemcpyAsync(wcol, phi.dptr(), n_ * w * 8, hipMemcpyDeviceToDevice, ctx_->stream()));
+    SADMRG_HIP(launch_axpby(n_ * w, weight, 0.0, nullptr, wcol, ctx_->stream()));
+  }
+  ctx_->sync();
+  proj_ = std::move(nb);
+  proj_dot_ = DevBuf(ctx_, 2 * P * 8);
+  proj_w_.push_back(weight);
+}
+
+// y += sum_j (weight_j phi_j) <phi_j|x>: one reduction over all phi_j, one update.
+void HeffPlan::apply_projectors(const double* d_in, double* d_out) {
+  const int P = projectors();
+  if (P == 0 || n_ == 0) return;
+  const int64_t ld = n_;
+  const double* phi = proj_.as<double>();
+  const double* wphi = phi + static_cast<int64_t>(P) * ld * (cplx_ ? 2 : 1);
+  SADMRG_HIP(launch_gemv_t(phi, ld, P, d_in, n_, cplx_ ? 1 : 0, ctx_->partial(), 1.0, proj_dot_.as<double>(),
+                           ctx_->stream()));
+  SADMRG_HIP(launch_gemv_n_update(wphi, ld, P, proj_dot_.as<double>(), d_out, n_, cplx_ ? 1 : 0, ctx_->stream()));
 }
 
 }  // namespace sadmrg

@@ -56,7 +56,16 @@ class HeffPlan {
   // d_in / d_out: device arrays in the layout of psi_layout (n elements).
   // unbatched: one launch per output block of each stage instead of one
   // grouped launch (the per-block baseline).
+  // The full application (stage 0) ends with the rank-one terms added by
+  // add_projector.
   void apply(const double* d_in, double* d_out, Prec prec, int stage = 0, bool unbatched = false);
+  // Adds weight * phi <phi|x> to every full application: the orthogonality
+  // penalty of the excited-state solve (dmrg_inst.cpp:352-363, weight =
+  // -E_j, phi = the stored state j projected on the current basis).  phi is
+  // copied; it must have the structure and block offsets of psi_layout.
+  // Evaluated in FP64 at both precisions.
+  void add_projector(const BTensor& phi, double weight);
+  int projectors() const { return static_cast<int>(proj_w_.size()); }
   double stage_flops(int stage) const;
   const HeffStats& stats() const { return stats_; }
   Context* ctx() const { return ctx_; }
@@ -77,6 +86,10 @@ class HeffPlan {
   int cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& at);
   HeffStats stats_;
   std::vector<uint8_t> out_null_;
+  // projector columns: phi_0..phi_{P-1}, then weight_j * phi_j (n elements each)
+  DevBuf proj_, proj_dot_;
+  std::vector<double> proj_w_;
+  void apply_projectors(const double* d_in, double* d_out);
 };
 
 }  // namespace sadmrg

@@ -117,7 +117,7 @@ def make_hvm():
 
 
 # ---------------------------------------------------------------------------
-def snapshot(d, ns):
+def snapshot(d, ns, exci=0):
     tens = {}
     for i in range(ns):
         for which in ("uu", "hh"):
@@ -128,10 +128,17 @@ def snapshot(d, ns):
             t = tl.from_ref(d.tensor("opr", i, j))
             if int(t["header"][0]) > 0:
                 tens[f"opr_{i}_{j}"] = t
+        for j in range(exci):  # stored states: site tensors and overlap environments
+            for which in ("orth", "ovlp"):
+                t = tl.from_ref(d.tensor(which, i, j))
+                if int(t["header"][0]) > 0:
+                    tens[f"{which}_{i}_{j}"] = t
+    for j in range(exci):
+        tens[f"overlapvec_0_{j}"] = tl.from_ref(d.tensor("overlapvec", 0, j))
     return tens
 
 
-def ref_sweep(d, ns, nfree, steps, capture=None, d_max=1 << 30):
+def ref_sweep(d, ns, nfree, steps, capture=None, d_max=1 << 30, exci=0):
     """DMRG::sweep (dmrg_inst.cpp:874-943) step by step; returns per-step records."""
     recs = []
 
@@ -157,7 +164,7 @@ def ref_sweep(d, ns, nfree, steps, capture=None, d_max=1 << 30):
             p1, p2 = (0.0, 1.0) if i < ns - nfree - 3 else (1.0, 0.0)
         cap = capture is not None and capture["when"] == (kind, i) and "done" not in capture
         if cap:
-            capture["state"] = snapshot(d, ns)
+            capture["state"] = snapshot(d, ns, exci)
             capture["vec"] = tl.from_ref(vec)
             capture["heff"] = tl.from_ref(d.heff(il, ir, vec))
             capture["il"], capture["ir"], capture["p1"], capture["p2"] = il, ir, p1, p2
@@ -279,6 +286,62 @@ def ckpt_case(name, xx, yy, D, lattice):
     print(name, "E saved", e_saved, "read pass", ok, "E read", e_read, "E after resumed sweep", e_resume)
 
 
+# Excited state exci = 1 (dmrg_inst.cpp:283-583) as the reference's driver
+# runs it (main/main_dmrg_su2_real.cpp:72-96): read_mps(D, D) finds no files
+# of state 1, loads state 0 (orth[0], gs_enr[0]) from the checkpoint fixture
+# of the same lattice, then do_idmrg and sweeps.  Runs in a fresh process, so
+# the generator starts from its initial state; the mid-sweep capture of the
+# second sweep holds the stored state's tensors, its overlap environments and
+# the overlap vector next to the usual state.
+EXCI_CASES = {
+    "exci_su2_ring10": dict(ckpt="ckpt_su2_ring10", xx=1, yy=10, D=16, lattice=None, nsweep=2,
+                            capture_at=("right", 3)),
+    "exci_su2_obc12": dict(ckpt="ckpt_su2_obc12", xx=1, yy=12, D=24, lattice="obc", nsweep=2,
+                           capture_at=("left", 5)),
+}
+
+
+def exci_case(name, ckpt, xx, yy, D, lattice, nsweep, capture_at):
+    import tempfile
+    rb.init(physpn=1, max_angm=M, max_dcut=D, psize=1)
+    ns = xx * yy
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as tmp:
+        os.chdir(tmp)  # nothing is written here by the step-by-step sweeps; kept as a guard
+        try:
+            d = rb.RefDMRG(1, 0, xx, yy, D, 1, 0, exci=1)
+            if lattice is not None:
+                d.set_couplings(*lattice)
+            hmap, coup = d.couplings()
+            ok = d.read(os.path.join(HERE, ckpt), D, D)
+            e_lower = d.state_energy(0)
+            d.idmrg()
+            e_idmrg = d.energy()
+            cap = {"when": tuple(capture_at)}
+            sweeps = []
+            for s_ in range(nsweep):
+                sweeps.append(ref_sweep(d, ns, 2, None, cap if s_ == 1 else None, d_max=D, exci=1))
+        finally:
+            os.chdir(cwd)
+    tens = {"S_" + k: v for k, v in cap["state"].items()}
+    for k in ("vec", "heff", "lanczos_vec", "uu_l", "uu_r"):
+        tens[k] = cap[k]
+    energies = np.asarray([[r[2] for r in recs] for recs in sweeps])
+    steps = np.asarray([[r[0], r[1]] for r in sweeps[0]], np.int32)
+    tl.save_tensors(os.path.join(HERE, f"{name}.npz"), tens, hmap=hmap, coup=coup,
+                    params=np.asarray([1, 0, xx, yy, D, ns], np.int32), e_idmrg=e_idmrg, e_lower=e_lower,
+                    read_pass=int(ok), energies=energies, steps=steps,
+                    cap_step=np.asarray([cap["il"], cap["ir"]], np.int32),
+                    cap_p=np.asarray([cap["p1"], cap["p2"]]), lanczos_e=cap["lanczos_e"])
+    print(name, "read pass", ok, "E0", e_lower, "E_idmrg", e_idmrg, "sweep energies", energies[:, -1])
+
+
+def run_exci(name):
+    c = dict(EXCI_CASES[name])
+    lat = c.pop("lattice")
+    exci_case(name, lattice=lattice_of(lat, c["xx"], c["yy"]), **c)
+
+
 def lattice_of(lat, xx, yy):
     ns = xx * yy
     if lat == "obc":
@@ -316,13 +379,16 @@ def main(argv):
     if len(argv) > 1 and argv[1] in CKPT_CASES:
         run_ckpt(argv[1])
         return
+    if len(argv) > 1 and argv[1] in EXCI_CASES:
+        run_exci(argv[1])
+        return
     if len(argv) > 1 and argv[1] == "cgc":
         make_cgc()
         return
     if len(argv) > 1 and argv[1] == "hvm":
         make_hvm()
         return
-    for sub in ["cgc", "hvm"] + list(CASES) + list(CKPT_CASES):
+    for sub in ["cgc", "hvm"] + list(CASES) + list(CKPT_CASES) + list(EXCI_CASES):
         subprocess.run([sys.executable, __file__, sub], check=True)
 
 

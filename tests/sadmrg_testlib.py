@@ -240,6 +240,22 @@ def heisenberg_ed(hmap, coup, nlow=1):
     return np.sort(w)
 
 
+def heisenberg_ed_singlets(hmap, coup, nlow=2):
+    """Lowest nlow energies of H = sum_{i<j} coup[hmap[i,j]] S_i.S_j among
+    total-spin-0 states.  H + c S^2 with S^2 = 3N/4 + 2 sum_{i<j} S_i.S_j
+    is a pair Hamiltonian (coupling + 2c on every pair); c = 10 lifts every
+    S > 0 state by at least 2c and leaves singlets at E."""
+    ns = hmap.shape[0]
+    c = 10.0
+    coup = np.asarray(coup, dtype=float)
+    h2 = np.ones((ns, ns), dtype=np.int32) - np.eye(ns, dtype=np.int32)  # index 1: pairs outside H
+    mask = np.asarray(hmap) > 0
+    h2[mask] = np.asarray(hmap)[mask] + 1                                # index k+1: coupling k
+    coup2 = np.concatenate([[0.0, 2 * c], coup[1:] + 2 * c])
+    w = heisenberg_ed(h2, coup2, nlow=nlow + 4) + c * 0.75 * ns
+    return np.sort(w)[:nlow]
+
+
 def heisenberg_bond_expectations(hmap, coup):
     """(E0, {(i, j): <S_i.S_j>}) over the bonds of hmap, from the Sz=0 ground
     state of H = sum_{i<j} coup[hmap[i,j]] S_i.S_j (exact diagonalisation)."""
