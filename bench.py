@@ -180,24 +180,40 @@ def c1_run(sad, args):
     coup = np.zeros(L + 1)
     coup[1] = 1.0
     ctx = sad.Context(0)
-    sad.lib().sadmrg_rng_seed(42)
-    d = sad.DMRG(ctx, sad.U1, 0, 1, L, m)
-    d.set_couplings(hmap, coup)
-    d.set_max_dcut(m)
-    ctx.sync()
-    t0 = time.perf_counter()
-    d.idmrg()
-    gpu_e, gpu_tr = [], []
-    for _ in range(nsweep):
-        d.records()
-        d.sweep()
-        recs = d.records()
-        gpu_e.append(d.energy())
-        gpu_tr.append(recs[-1]["trunc_err"])
-    ctx.sync()
-    gpu_s = time.perf_counter() - t0
+
+    def run(device_steps, graph):
+        prev = sad.lanczos_mode(device_steps, graph, reset=True)
+        sad.lib().sadmrg_rng_seed(42)
+        d = sad.DMRG(ctx, sad.U1, 0, 1, L, m)
+        d.set_couplings(hmap, coup)
+        d.set_max_dcut(m)
+        ctx.sync()
+        t0 = time.perf_counter()
+        d.idmrg()
+        es, trs, t_lan = [], [], 0.0
+        for _ in range(nsweep):
+            d.records()
+            d.sweep()
+            recs = d.records()
+            es.append(d.energy())
+            trs.append(recs[-1]["trunc_err"])
+            t_lan += sum(r["t_lanczos"] for r in recs)
+        ctx.sync()
+        secs = time.perf_counter() - t0
+        replays = sad.lanczos_mode(prev[0], prev[1])[2]
+        return secs, es, trs, t_lan, replays
+
+    run(True, True)  # first use: autotuner entries, rocSOLVER and allocator warm-up
+    modes = {}
+    for name, dev, graph in (("host_steps", False, False), ("device_steps", True, False),
+                             ("graph_steps", True, True)):
+        secs, es, trs, t_lan, replays = run(dev, graph)
+        modes[name] = {"seconds": round(secs, 3), "sweep_lanczos_seconds": round(t_lan, 3),
+                       "graph_replays": replays, "last_energy": es[-1]}
+    gpu_s, gpu_e, gpu_tr, _, _ = run(True, True)
     line = {"workload": "C1: U(1) open chain L=32, m=64 kept states, iDMRG + 4 sweeps, FP64",
-            "gpu_seconds": round(gpu_s, 3), "gpu_sweep_energies": gpu_e, "gpu_last_step_trunc": gpu_tr}
+            "gpu_seconds": round(gpu_s, 3), "gpu_sweep_energies": gpu_e, "gpu_last_step_trunc": gpu_tr,
+            "lanczos_step_modes": modes}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
@@ -254,12 +270,14 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refi
         c.set_check_interval(every)
         ctx.sync()
         sad.svd_stats(reset=True)
+        sad.lanczos_mode(reset=True)
         t1 = time.perf_counter()
         c.sweep()
         ctx.sync()
         wall = time.perf_counter() - t1
         recs = c.records()
-        out[name] = {"seconds": wall, "energy": c.energy(), "recs": recs, "svd": sad.svd_stats(reset=True)}
+        out[name] = {"seconds": wall, "energy": c.energy(), "recs": recs, "svd": sad.svd_stats(reset=True),
+                     "graph_replays": sad.lanczos_mode()[2]}
         print(f"[bench] {name} sweep: {wall:.2f} s, E={c.energy():.12f}, steps={len(recs)}",
               file=sys.stderr, flush=True)
         del c
@@ -283,6 +301,7 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refi
         "matvecs_fp64": int(tsum("fp64", "matvec64")),
         "matvecs_mixed": [int(tsum("mixed", "matvec32")), int(tsum("mixed", "matvec64"))],
         "mixed_check_every": check_every[0],
+        "lanczos_graph_replays": {"fp64": out["fp64"]["graph_replays"], "mixed": out["mixed"]["graph_replays"]},
         "truncation": {"fp64": out["fp64"]["svd"], "mixed": out["mixed"]["svd"]},
         "variants": {name: {"sweep_s": round(out[name]["seconds"], 3),
                             "speedup": round(out["fp64"]["seconds"] / out[name]["seconds"], 3),

@@ -162,6 +162,15 @@ int sadmrg_svd_gram_min(int value, int* previous);
    sectors redone by gesdd}; *seconds = wall time inside svd; reset != 0
    clears them after reading.  No reference counterpart. */
 int sadmrg_svd_stats(long long* counts, double* seconds, int reset);
+/* Lanczos step mode (process-wide, both on by default): device_steps runs
+   the Krylov steps between convergence checks without host
+   synchronisation (alpha, beta and the step index stay on the device);
+   graph records one such step as a HIP graph once per solve and replays
+   it.  The arithmetic is that of the host-driven step.  -1 leaves a
+   setting; prev[2] (optional) receives the old values; counts[2]
+   (optional) = {steps replayed from graphs, graphs captured} since the
+   last reset.  No reference counterpart. */
+int sadmrg_lanczos_mode(int device_steps, int graph, int* prev, long long* counts, int reset);
 /* Lanczos::lanczos1 (lanczos.hpp:70) on the plan; psi is the start vector on
    entry and the Ritz vector on return.  counters[4] = {FP64 matvecs, FP32
    matvecs, Lanczos steps, refinement restarts}. */

@@ -87,6 +87,7 @@ def lib():
         "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_svd_gram_min": (i, [i, ip]),
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
+        "sadmrg_lanczos_mode": (i, [i, i, ip, C.POINTER(C.c_longlong), i]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_create_exci": (i, [vp, i, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_exci": (i, [vp, ip]),
@@ -420,6 +421,18 @@ def autotune(enable=None, clear=False):
     n = C.c_int()
     check(lib().sadmrg_autotune(-1 if enable is None else int(bool(enable)), int(bool(clear)), C.byref(n)))
     return n.value
+
+
+def lanczos_mode(device_steps=None, graph=None, reset=False):
+    """Process-wide Lanczos step mode.  Sets the given switches and returns
+    (device_steps, graph, graph replays, graphs captured) as they were before
+    the call."""
+    prev = (C.c_int * 2)()
+    counts = (C.c_longlong * 2)()
+    check(lib().sadmrg_lanczos_mode(-1 if device_steps is None else int(bool(device_steps)),
+                                    -1 if graph is None else int(bool(graph)),
+                                    prev, counts, 1 if reset else 0))
+    return bool(prev[0]), bool(prev[1]), int(counts[0]), int(counts[1])
 
 
 def svd_stats(reset=False):

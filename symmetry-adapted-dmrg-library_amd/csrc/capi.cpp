@@ -557,6 +557,24 @@ int sadmrg_autotune(int enable, int clear, int* entries) {
   });
 }
 
+int sadmrg_lanczos_mode(int device_steps, int graph, int* prev, long long* counts, int reset) {
+  return guard([&] {
+    bool ds, gr;
+    lanczos_mode(&ds, &gr);
+    if (prev) {
+      prev[0] = ds ? 1 : 0;
+      prev[1] = gr ? 1 : 0;
+    }
+    set_lanczos_mode(device_steps < 0 ? ds : device_steps != 0, graph < 0 ? gr : graph != 0);
+    long long r = 0, c = 0;
+    lanczos_graph_counts(&r, &c, reset != 0);
+    if (counts) {
+      counts[0] = r;
+      counts[1] = c;
+    }
+  });
+}
+
 int sadmrg_svd_stats(long long* counts, double* seconds, int reset) {
   return guard([&] {
     SvdStats& st = svd_stats();
@@ -737,9 +755,9 @@ int sadmrg_dmrg_set_check_interval(sadmrg_dmrg* d, int every) {
   });
 }
 
-namespace {
-std::string dir_arg(const char* dir) { return dir ? std::string(dir) : std::string(); }
-}  // namespace
+extern "C++" {
+static std::string dir_arg(const char* dir) { return dir ? std::string(dir) : std::string(); }
+}
 
 int sadmrg_dmrg_save_mps(sadmrg_dmrg* d, const char* dir, int exci) {
   return guard([&] {

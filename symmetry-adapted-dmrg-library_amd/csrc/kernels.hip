@@ -580,11 +580,15 @@ constexpr int kRedBlock = 256;
 constexpr int kRedGrid = 1024;
 
 // partial[g*ncol + c] = sum over this block's slice of conj(X[:,c]) . y
+// dncol (optional, device): the column count is min(*dncol, ncol), read by
+// every thread, so one launch serves steps of different width.
 __global__ __launch_bounds__(kRedBlock) void gemv_t_partial(const double* __restrict__ X, int64_t ldx,
-                                                            int ncol, const double* __restrict__ y,
+                                                            int ncol, const int* __restrict__ dncol,
+                                                            const double* __restrict__ y,
                                                             int64_t n, int cplx,
                                                             double* __restrict__ partial) {
   __shared__ double red[kRedBlock / 64][2];
+  if (dncol) ncol = min(*dncol, ncol);
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int c = 0; c < ncol; ++c) {
     const double* x = X + c * ldx * (cplx ? 2 : 1);
@@ -615,9 +619,11 @@ __global__ __launch_bounds__(kRedBlock) void gemv_t_partial(const double* __rest
 }
 
 // out[c] (complex pair) = sum_g partial[g, c]; out is scaled by `sign`.
-__global__ void gemv_t_finish(const double* __restrict__ partial, int ngrid, int ncol, double sign,
-                              double* __restrict__ out) {
+__global__ void gemv_t_finish(const double* __restrict__ partial, int ngrid, int ncol,
+                              const int* __restrict__ dncol, double sign, double* __restrict__ out) {
   const int c = blockIdx.x;
+  if (dncol) ncol = min(*dncol, ncol);
+  if (c >= ncol) return;
   __shared__ double red[64][2];
   double sr = 0, si = 0;
   for (int g = threadIdx.x; g < ngrid; g += blockDim.x) {
@@ -636,8 +642,9 @@ __global__ void gemv_t_finish(const double* __restrict__ partial, int ngrid, int
 }
 
 // y += sum_c coef[c] * X[:,c]   (coef complex pairs, device resident)
-__global__ void gemv_n_update(const double* __restrict__ X, int64_t ldx, int ncol,
+__global__ void gemv_n_update(const double* __restrict__ X, int64_t ldx, int ncol, const int* __restrict__ dncol,
                               const double* __restrict__ coef, double* __restrict__ y, int64_t n, int cplx) {
+  if (dncol) ncol = min(*dncol, ncol);
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += stride) {
     if (!cplx) {
@@ -658,11 +665,63 @@ __global__ void gemv_n_update(const double* __restrict__ X, int64_t ldx, int nco
   }
 }
 
-// y = a*y + b*x with host scalars (real)
+// ---- Lanczos steps without host synchronisation ----------------------------
+// One step produces column m = s->m of the Krylov basis V (columns of ldv
+// doubles, nd doubles used) from w = H x, x = column m-1:
+//   alpha[m-1] = <x, w>;  w -= alpha x;  w -= beta[m-1] V[:, m-2]   (update)
+//   w -= V[:, 0..m) (V^H w), twice                                  (gemv kernels)
+//   beta[m] = |w|;  V[:, m] = x = w / beta[m];  m += 1              (finish, advance)
+// The same expressions and rounding order as the host-driven step.  A step
+// whose beta falls below tol sets s->stop and leaves m on that step; every
+// later step is then a no-op.
+__global__ void lanczos_update_kernel(LanczosDev* __restrict__ s, double* __restrict__ w,
+                                      const double* __restrict__ x, const double* __restrict__ V, int64_t ldv,
+                                      int64_t nd) {
+  if (s->stop) return;
+  const int m = s->m;
+  const double a = s->dot[0];
+  const double b = m >= 2 ? s->beta[m - 1] : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) s->alpha[m - 1] = a;
+  const double* v2 = m >= 2 ? V + int64_t(m - 2) * ldv : nullptr;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nd; i += stride) {
+    double t = fma(-a, x[i], w[i]);
+    if (v2) t = fma(-b, v2[i], t);
+    w[i] = t;
+  }
+}
+
+__global__ void lanczos_finish_kernel(LanczosDev* __restrict__ s, const double* __restrict__ w,
+                                      double* __restrict__ x, double* __restrict__ V, int64_t ldv, int64_t nd,
+                                      double tol) {
+  if (s->stop) return;
+  const int m = s->m;
+  const double b = sqrt(fabs(s->nrm[0]));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s->beta[m] = b;
+    if (b < tol) s->stop = 1;
+  }
+  if (b < tol) return;
+  const double inv = 1.0 / b;
+  double* col = V + int64_t(m) * ldv;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nd; i += stride) {
+    const double t = inv * w[i];
+    col[i] = t;
+    x[i] = t;
+  }
+}
+
+__global__ void lanczos_advance_kernel(LanczosDev* __restrict__ s) {
+  if (threadIdx.x == 0 && !s->stop) s->m += 1;
+}
+
+// y = a*y + b*x with host scalars (real), one rounding of b*x + a*y (a*y is
+// exact for a = 1, so the Lanczos update matches the device-step kernels)
 __global__ void axpby_kernel(int64_t n, double a, double b, const double* __restrict__ x, double* __restrict__ y) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += stride)
-    y[i] = a * y[i] + (x ? b * x[i] : 0.0);
+    y[i] = x ? fma(b, x[i], a * y[i]) : a * y[i];
 }
 
 // y *= complex scalar (or y = x * s), interleaved
@@ -792,15 +851,51 @@ hipError_t launch_gemv_t(const double* X, int64_t ldx, int ncol, const double* y
   if (ncol > 64) return hipErrorInvalidValue;  // partial holds 2 * kPartialGrid * 64 doubles
   int g = grid_for(n, kRedBlock);
   if (g > kRedGrid) g = kRedGrid;
-  hipLaunchKernelGGL(gemv_t_partial, dim3(g), dim3(kRedBlock), 0, st, X, ldx, ncol, y, n, cplx, partial);
-  hipLaunchKernelGGL(gemv_t_finish, dim3(ncol), dim3(64), 0, st, partial, g, ncol, sign, out);
+  hipLaunchKernelGGL(gemv_t_partial, dim3(g), dim3(kRedBlock), 0, st, X, ldx, ncol, nullptr, y, n, cplx, partial);
+  hipLaunchKernelGGL(gemv_t_finish, dim3(ncol), dim3(64), 0, st, partial, g, ncol, nullptr, sign, out);
   return hipGetLastError();
 }
 
 hipError_t launch_gemv_n_update(const double* X, int64_t ldx, int ncol, const double* coef, double* y, int64_t n,
                                 int cplx, hipStream_t st) {
   if (ncol <= 0) return hipSuccess;
-  hipLaunchKernelGGL(gemv_n_update, dim3(grid_for(n)), dim3(256), 0, st, X, ldx, ncol, coef, y, n, cplx);
+  hipLaunchKernelGGL(gemv_n_update, dim3(grid_for(n)), dim3(256), 0, st, X, ldx, ncol, nullptr, coef, y, n, cplx);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemv_t_dev(const double* X, int64_t ldx, int maxcol, const int* dncol, const double* y, int64_t n,
+                             int cplx, double* partial, double sign, double* out, hipStream_t st) {
+  if (maxcol <= 0) return hipSuccess;
+  if (maxcol > 64 || dncol == nullptr) return hipErrorInvalidValue;
+  int g = grid_for(n, kRedBlock);
+  if (g > kRedGrid) g = kRedGrid;
+  hipLaunchKernelGGL(gemv_t_partial, dim3(g), dim3(kRedBlock), 0, st, X, ldx, maxcol, dncol, y, n, cplx, partial);
+  hipLaunchKernelGGL(gemv_t_finish, dim3(maxcol), dim3(64), 0, st, partial, g, maxcol, dncol, sign, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemv_n_update_dev(const double* X, int64_t ldx, int maxcol, const int* dncol, const double* coef,
+                                    double* y, int64_t n, int cplx, hipStream_t st) {
+  if (maxcol <= 0) return hipSuccess;
+  if (dncol == nullptr) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(gemv_n_update, dim3(grid_for(n)), dim3(256), 0, st, X, ldx, maxcol, dncol, coef, y, n, cplx);
+  return hipGetLastError();
+}
+
+hipError_t launch_lanczos_update(LanczosDev* s, double* w, const double* x, const double* V, int64_t ldv, int64_t nd,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(lanczos_update_kernel, dim3(grid_for(nd)), dim3(256), 0, st, s, w, x, V, ldv, nd);
+  return hipGetLastError();
+}
+
+hipError_t launch_lanczos_finish(LanczosDev* s, const double* w, double* x, double* V, int64_t ldv, int64_t nd,
+                                 double tol, hipStream_t st) {
+  hipLaunchKernelGGL(lanczos_finish_kernel, dim3(grid_for(nd)), dim3(256), 0, st, s, w, x, V, ldv, nd, tol);
+  return hipGetLastError();
+}
+
+hipError_t launch_lanczos_advance(LanczosDev* s, hipStream_t st) {
+  hipLaunchKernelGGL(lanczos_advance_kernel, dim3(1), dim3(64), 0, st, s);
   return hipGetLastError();
 }
 

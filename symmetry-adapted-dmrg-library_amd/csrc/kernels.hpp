@@ -45,6 +45,31 @@ hipError_t launch_gemv_t(const double* X, int64_t ldx, int ncol, const double* y
 // y += sum_c coef[c] * X[:,c]
 hipError_t launch_gemv_n_update(const double* X, int64_t ldx, int ncol, const double* coef, double* y, int64_t n,
                                 int cplx, hipStream_t st);
+// The same with the column count min(*dncol, maxcol) read on the device
+// (maxcol <= 64), so a captured launch serves every step of a solve.
+hipError_t launch_gemv_t_dev(const double* X, int64_t ldx, int maxcol, const int* dncol, const double* y, int64_t n,
+                             int cplx, double* partial, double sign, double* out, hipStream_t st);
+hipError_t launch_gemv_n_update_dev(const double* X, int64_t ldx, int maxcol, const int* dncol, const double* coef,
+                                    double* y, int64_t n, int cplx, hipStream_t st);
+
+// Device state of a Lanczos solve whose Krylov steps run without host
+// synchronisation (and can be captured once in a graph and replayed).
+constexpr int kLanczosDevMax = 64;
+struct LanczosDev {
+  int m;     // column the next step produces
+  int stop;  // 1: the step at m had beta below the breakdown threshold
+  int pad[2];
+  double dot[2];  // <x, w> of the current step (complex pair)
+  double nrm[2];  // <w, w>
+  double alpha[kLanczosDevMax];
+  double beta[kLanczosDevMax + 1];
+};
+// nd = doubles per vector, ldv = doubles per column of V
+hipError_t launch_lanczos_update(LanczosDev* s, double* w, const double* x, const double* V, int64_t ldv, int64_t nd,
+                                 hipStream_t st);
+hipError_t launch_lanczos_finish(LanczosDev* s, const double* w, double* x, double* V, int64_t ldv, int64_t nd,
+                                 double tol, hipStream_t st);
+hipError_t launch_lanczos_advance(LanczosDev* s, hipStream_t st);
 hipError_t launch_axpby(int64_t n, double a, double b, const double* x, double* y, hipStream_t st);
 hipError_t launch_zscal(int64_t n, double sr, double si, const double* x, double* y, hipStream_t st);
 hipError_t launch_f64_to_f32(int64_t n, const double* x, float* y, hipStream_t st);

@@ -1,6 +1,7 @@
 // lanczos.cpp -- see lanczos.hpp.
 #include "lanczos.hpp"
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <limits>
 #include "kernels.hpp"
@@ -77,6 +78,40 @@ void tridiag_eig(int n, std::vector<double>& d, std::vector<double> e, std::vect
 }
 
 namespace {
+std::atomic<bool> g_device_steps{true}, g_graph{true};
+std::atomic<long long> g_replays{0}, g_captures{0};
+}  // namespace
+
+void lanczos_graph_counts(long long* replays, long long* captures, bool reset) {
+  if (replays) *replays = g_replays;
+  if (captures) *captures = g_captures;
+  if (reset) {
+    g_replays = 0;
+    g_captures = 0;
+  }
+}
+
+void set_lanczos_mode(bool device_steps, bool graph) {
+  g_device_steps = device_steps;
+  g_graph = graph;
+}
+
+void lanczos_mode(bool* device_steps, bool* graph) {
+  if (device_steps) *device_steps = g_device_steps;
+  if (graph) *graph = g_graph;
+}
+
+namespace {
+
+// One Krylov step captured in a HIP graph, destroyed with the solve.
+struct StepGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~StepGraph() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+  }
+};
 
 struct Dev {
   Context* ctx;
@@ -196,7 +231,97 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
     }
     done = true;
   }
-  for (int m = 2; !done && m < ml; ++m) {
+  const bool dev_steps = !done && opt.device_steps && g_device_steps && ml <= kLanczosDevMax;
+  if (dev_steps) {
+    // Steps m = 2 .. ml-1 on the device: x = column m-1 (kept in xin), w = H x.
+    DevBuf stb(ctx, sizeof(LanczosDev));
+    DevBuf xb(ctx, std::max<int64_t>(nw, 1) * 8), wb(ctx, std::max<int64_t>(nw, 1) * 8);
+    LanczosDev* st = stb.as<LanczosDev>();
+    double* xin = xb.as<double>();
+    double* w = wb.as<double>();
+    LanczosDev hs{};
+    hs.m = 2;
+    hs.alpha[0] = aal[0];
+    hs.beta[1] = nnl[1];
+    SADMRG_HIP(hipMemcpyAsync(st, &hs, sizeof(hs), hipMemcpyHostToDevice, ctx->stream()));
+    D.copy(col(1), xin);
+    ctx->sync();  // hs is reused as the read-back buffer
+    const int cx = D.cx ? 1 : 0;
+    auto step = [&]() {
+      H.apply(xin, w, bulk);
+      SADMRG_HIP(launch_gemv_t(xin, 0, 1, w, D.n, cx, ctx->partial(), 1.0, st->dot, ctx->stream()));
+      SADMRG_HIP(launch_lanczos_update(st, w, xin, V, nw, nw, ctx->stream()));
+      for (int pass = 0; pass < 2; ++pass) {
+        SADMRG_HIP(launch_gemv_t_dev(V, D.n, ml, &st->m, w, D.n, cx, ctx->partial(), -1.0, ctx->dscal(),
+                                     ctx->stream()));
+        SADMRG_HIP(launch_gemv_n_update_dev(V, D.n, ml, &st->m, ctx->dscal(), w, D.n, cx, ctx->stream()));
+      }
+      SADMRG_HIP(launch_gemv_t(w, 0, 1, w, D.n, cx, ctx->partial(), 1.0, st->nrm, ctx->stream()));
+      SADMRG_HIP(launch_lanczos_finish(st, w, xin, V, nw, nw, tol, ctx->stream()));
+      SADMRG_HIP(launch_lanczos_advance(st, ctx->stream()));
+    };
+    StepGraph sg;
+    if (opt.graph && g_graph && ml > 3) {
+      SADMRG_HIP(hipStreamBeginCapture(ctx->stream(), hipStreamCaptureModeThreadLocal));
+      try {
+        step();
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(ctx->stream(), &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+      }
+      SADMRG_HIP(hipStreamEndCapture(ctx->stream(), &sg.graph));
+      SADMRG_HIP(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
+      g_captures++;
+    }
+    // host copies of alpha / beta and the stop state
+    auto fetch = [&]() {
+      SADMRG_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream()));
+      ctx->sync();
+      const int top = hs.stop ? hs.m : hs.m - 1;  // last step that completed
+      for (int i = 1; i <= top && i < ml; ++i) aal[i] = hs.alpha[i];
+      for (int i = 2; i <= top && i <= ml; ++i) nnl[i] = hs.beta[i];
+    };
+    for (int m = 2; !done && m < ml; ++m) {
+      if (sg.exec) {
+        SADMRG_HIP(hipGraphLaunch(sg.exec, ctx->stream()));
+        out.graph_launches++;
+        g_replays++;
+      } else {
+        step();
+      }
+      if (bulk == Prec::FP64) out.matvec64++;
+      else out.matvec32++;
+      out.steps = m;
+      if (m % opt.check_every != 0 && m != ml - 1) continue;
+      fetch();
+      if (hs.stop) {
+        // breakdown at step k: the host-driven loop stops there
+        const int k = hs.m;
+        const int wasted = m - k;
+        if (bulk == Prec::FP64) out.matvec64 -= wasted;
+        else out.matvec32 -= wasted;
+        out.steps = k;
+        diatridiag(k);
+        ritz(k);
+        check();
+        done = true;
+        break;
+      }
+      if (m % opt.check_every != 0) continue;
+      diatridiag(m);
+      ritz(m);
+      const double ov = check();
+      out.trace.push_back(eig[0]);
+      if (std::fabs(ov - 1.0) < 1e-8) {
+        done = true;
+        break;
+      }
+    }
+    ctx->sync();  // the step buffers leave scope
+  }
+  for (int m = 2; !done && !dev_steps && m < ml; ++m) {
     matvec(col(m - 1), col(m), bulk);
     aal[m - 1] = D.dot(col(m), col(m - 1)).real();
     D.axpby(1.0, -aal[m - 1], col(m - 1), col(m));

@@ -28,12 +28,27 @@ struct LanczosOpts {
   int refine_krylov = 6;      // FP32 Krylov steps per restart
   int max_restarts = 6;
   int check_every = 4;        // convergence check after every check_every-th step (dmrg_inst.cpp:1334)
+  // Krylov steps between checks run without host synchronisation (alpha,
+  // beta and the step index stay on the device); with graph, one step is
+  // captured in a HIP graph once per solve and replayed.  Same arithmetic
+  // as the host-driven step.  Bases of more than 64 vectors use the
+  // host-driven step.
+  bool device_steps = true;
+  bool graph = true;
 };
+
+// Process-wide switches (default on): a solve uses device steps / graphs
+// when both its options and these allow them.
+void set_lanczos_mode(bool device_steps, bool graph);
+void lanczos_mode(bool* device_steps, bool* graph);
+// graph replays and captures since the last reset (process-wide)
+void lanczos_graph_counts(long long* replays, long long* captures, bool reset);
 
 struct LanczosOut {
   double energy = 0;
   int matvec64 = 0, matvec32 = 0;
   int steps = 0;
+  int graph_launches = 0;   // Krylov steps replayed from a captured graph
   int restarts = 0;
   double residual64 = -1;     // |H x - E x| of the returned vector (FP64), if measured
   std::vector<double> trace;  // Ritz values at each check

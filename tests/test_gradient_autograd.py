@@ -63,7 +63,7 @@ def test_autograd_gradient_matches_finite_differences(ctx, sad):
     (g,) = torch.autograd.grad(E, J)
     hmap, coup = _chain(L, J.detach().numpy())
     e_ed, bond = tl.heisenberg_bond_expectations(hmap, coup)
-    assert abs(float(E) - e_ed) <= 1e-9 * abs(e_ed)
+    assert abs(float(E.detach()) - e_ed) <= 1e-9 * abs(e_ed)
     for i in range(L - 1):
         assert abs(float(g[i]) - bond[(i, i + 1)]) <= 1e-7, (i, float(g[i]), bond[(i, i + 1)])
     h = 1e-5
