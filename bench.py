@@ -29,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 PEAK_FP64 = 78.6    # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP32 = 157.3   # TFLOP/s, MI355X dense FP32 matrix
+PEAK_BF16 = 2500.0  # TFLOP/s, MI355X dense BF16 matrix
 PEAK_HBM = 8000.0   # GB/s
 
 
@@ -237,7 +238,8 @@ def c1_run(sad, args):
     return line
 
 
-def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6, check_every=(4,)):
+def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6, check_every=(4,),
+                  bf16=False):
     """Mixed vs FP64 sweep on the SU(2) J1-J2 cylinder (width x length, snake
     ordering).  Warm-up: iDMRG at warm[0] then one mixed sweep per entry of
     warm and one at D; the state is then cloned on the device and one sweep
@@ -263,6 +265,8 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refi
     out = {}
     legs = [("fp64", sad.FP64, 4), ("mixed", sad.FP32, check_every[0])]
     legs += [(f"mixed_check{k}", sad.FP32, k) for k in check_every[1:]]
+    if bf16:  # ablation: BF16 Krylov matvecs, same FP64 checks and refinement
+        legs.append(("bf16", sad.BF16, check_every[0]))
     for name, prec, every in legs:
         c = d.clone()
         c.set_precision(prec, refine=(prec != sad.FP64), refine_tol=refine_tol)
@@ -307,6 +311,9 @@ def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refi
                             "speedup": round(out["fp64"]["seconds"] / out[name]["seconds"], 3),
                             "final_rel_dE": float(abs(out[name]["energy"] - out["fp64"]["energy"]) /
                                                   abs(out["fp64"]["energy"])),
+                            "max_rel_dE_per_step": float(np.max(np.abs(
+                                np.asarray([r["energy"] for r in out[name]["recs"]]) - e64) / np.abs(e64))),
+                            "split_s": {k: round(tsum(name, k), 3) for k in ("t_env", "t_lanczos", "t_svd")},
                             "matvecs": [int(tsum(name, "matvec32")), int(tsum(name, "matvec64"))]}
                      for name, _, _ in legs[2:]},
     }
@@ -380,6 +387,7 @@ def main():
     ap.add_argument("--refine-tol", type=float, default=1e-5,
                     help="FP64 residual tolerance of the mixed path (relative to |E|; 0: library default)")
     ap.add_argument("--refine-krylov", type=int, default=3, help="FP32 Krylov steps per refinement restart")
+    ap.add_argument("--no-bf16-sweep", action="store_true", help="skip the BF16 ablation sweep")
     ap.add_argument("--check-every", default="2",
                     help="mixed sweep: Lanczos convergence check interval (comma list: first is reported, "
                          "the others are timed as variants)")
@@ -440,7 +448,7 @@ def main():
           flush=True)
 
     res = {}
-    for name, prec in (("fp64", sad.FP64), ("fp32", sad.FP32)):
+    for name, prec in (("fp64", sad.FP64), ("fp32", sad.FP32), ("bf16", sad.BF16)):
         for _ in range(args.warmup):
             plan.apply_device(d_in, d_out, prec)
         ctx.sync()
@@ -463,6 +471,8 @@ def main():
     o64 = tl.from_dev(plan.apply(psi, sad.FP64))
     o32 = tl.from_dev(plan.apply(psi, sad.FP32))
     rel32, _ = tl.compare(o32, o64)
+    o16 = tl.from_dev(plan.apply(psi, sad.BF16))
+    rel16, _ = tl.compare(o16, o64)
 
     cpu = None
     if not args.no_cpu:
@@ -481,7 +491,8 @@ def main():
         print(f"[bench] sweep comparison {lat} D={dd}", file=sys.stderr, flush=True)
         sweep = sweep_compare(sad, ctx, w_, l_, int(dd), [int(x) for x in warm.split(",")],
                               refine_tol=args.refine_tol, refine_krylov=args.refine_krylov,
-                              check_every=tuple(int(x) for x in args.check_every.split(",")))
+                              check_every=tuple(int(x) for x in args.check_every.split(",")),
+                              bf16=not args.no_bf16_sweep)
 
     dom = "stage2" if res["fp32"]["stage2_ms"] >= res["fp32"]["stage1_ms"] else "stage1"
     dom_kernel = gemm_kernel_name("fp32", 2 if dom == "stage2" else 1, res["fp32"]["tile_cfg"][1 if dom == "stage2" else 0])
@@ -511,6 +522,13 @@ def main():
         "tile_configs": "0: 64x64, 1: 128x128, 2: 128x64 output tiles (autotuned per stage)",
         "mixed_vs_fp64_speedup": round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3),
         "fp32_rel_err_vs_fp64": rel32,
+        "bf16_ablation": {"ms_per_step": round(res["bf16"]["ms_per_step"], 4),
+                          "tflops": round(res["bf16"]["tflops"], 3),
+                          "stage_ms": [round(res["bf16"]["stage1_ms"], 4), round(res["bf16"]["stage2_ms"], 4)],
+                          "tile_cfg": res["bf16"]["tile_cfg"],
+                          "frac_of_bf16_peak": round(res["bf16"]["tflops"] / PEAK_BF16, 4),
+                          "rel_err_vs_fp64": rel16,
+                          "speedup_vs_fp32": round(res["fp32"]["ms_per_step"] / res["bf16"]["ms_per_step"], 3)},
         "sweep": sweep,
         "roofline": {"bound": "mfma", "kernel": f"grouped_gemm {dom} (fp32): {dom_kernel}",
                      "achieved": round(achieved, 3),

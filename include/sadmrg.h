@@ -42,7 +42,9 @@ enum {
 };
 enum { SADMRG_U1 = 0, SADMRG_SU2 = 1 };
 enum { SADMRG_REAL = 0, SADMRG_COMPLEX = 1 };
-enum { SADMRG_FP64 = 0, SADMRG_FP32 = 1 };
+/* operand precision of the bulk H_eff products (outputs are FP64):
+   SADMRG_BF16 is an ablation (bfloat16 operands, FP32 accumulation) */
+enum { SADMRG_FP64 = 0, SADMRG_FP32 = 1, SADMRG_BF16 = 2 };
 
 /* ---- runtime ------------------------------------------------------------ */
 const char* sadmrg_last_error(void);

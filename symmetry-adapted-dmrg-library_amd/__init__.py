@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libsadmrg.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include", "sadmrg.h")
 
 U1, SU2 = 0, 1
-FP64, FP32 = 0, 1
+FP64, FP32, BF16 = 0, 1, 2
 
 _lib = None
 

@@ -59,8 +59,9 @@ SymKind sym_of(int s) {
 }
 
 Prec prec_of(int p) {
-  if (p != SADMRG_FP64 && p != SADMRG_FP32) fail(kErrInvalid, "precision must be SADMRG_FP64 or SADMRG_FP32");
-  return p == SADMRG_FP32 ? Prec::FP32 : Prec::FP64;
+  if (p != SADMRG_FP64 && p != SADMRG_FP32 && p != SADMRG_BF16)
+    fail(kErrInvalid, "precision must be SADMRG_FP64, SADMRG_FP32 or SADMRG_BF16");
+  return p == SADMRG_FP32 ? Prec::FP32 : (p == SADMRG_BF16 ? Prec::BF16 : Prec::FP64);
 }
 
 Bond bond_of(const int* b) {

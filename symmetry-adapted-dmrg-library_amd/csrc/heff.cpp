@@ -287,7 +287,7 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
   ArenaTable at;
   std::memset(&at, 0, sizeof(at));
   if (prec == Prec::FP64 || cplx_) {
-    require(prec == Prec::FP64, "H_eff: FP32 path is real-only");
+    require(prec == Prec::FP64, "H_eff: the FP32 and BF16 paths are real-only");
     at.base[kArIn] = d_in;
     at.base[kArOut] = d_out;
     at.base[kArScratch] = scratch64_.get();
@@ -300,6 +300,29 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
     if (stage != 1) {
       stats_.tile_cfg2 = cfg_for(g2_, p, 2, at);
       exec(g2_, p, at, stats_.tile_cfg2);
+    }
+    if (stage == 0) apply_projectors(d_in, d_out);
+    return;
+  }
+  if (prec == Prec::BF16) {
+    if (!pack16_.get()) {
+      pack16_ = DevBuf(ctx_, std::max<int64_t>(pack_elems_, 1) * 2);
+      scratch16_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 2);
+      in16_ = DevBuf(ctx_, std::max<int64_t>(n_, 1) * 2);
+      SADMRG_HIP(launch_f64_to_bf16(pack_elems_, pack64_.as<double>(), pack16_.as<uint16_t>(), ctx_->stream()));
+    }
+    if (stage != 2) SADMRG_HIP(launch_f64_to_bf16(n_, d_in, in16_.as<uint16_t>(), ctx_->stream()));
+    at.base[kArIn] = in16_.get();
+    at.base[kArOut] = d_out;
+    at.base[kArScratch] = scratch16_.get();
+    at.base[kArPack] = pack16_.get();
+    if (stage != 2) {
+      stats_.tile_cfg1 = cfg_for(g1_, GemmPrec::BF16, 1, at);
+      exec(g1_, GemmPrec::BF16, at, stats_.tile_cfg1);
+    }
+    if (stage != 1) {
+      stats_.tile_cfg2 = cfg_for(g2_, GemmPrec::BF16_OUT64, 2, at);
+      exec(g2_, GemmPrec::BF16_OUT64, at, stats_.tile_cfg2);
     }
     if (stage == 0) apply_projectors(d_in, d_out);
     return;

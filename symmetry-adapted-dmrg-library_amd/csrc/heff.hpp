@@ -21,7 +21,10 @@
 
 namespace sadmrg {
 
-enum class Prec : int { FP64 = 0, FP32 = 1 };
+// Operand precision of the bulk products: FP64; FP32 operands with FP32
+// accumulation; BF16 operands with FP32 accumulation (an ablation: 8
+// significant bits).  Outputs are FP64 in every case.
+enum class Prec : int { FP64 = 0, FP32 = 1, BF16 = 2 };
 
 struct HeffStats {
   double flops = 0;        // algorithmic flops per application
@@ -81,6 +84,7 @@ class HeffPlan {
   int64_t n_ = 0;
   BTensor layout_;  // structure / offsets only (no data)
   DevBuf pack64_, pack32_, scratch64_, scratch32_, in32_;
+  DevBuf pack16_, scratch16_, in16_;  // BF16 operand copies (built on first BF16 use)
   int64_t pack_elems_ = 0, scratch_elems_ = 0;
   GemmBatch g1_, g2_;
   int cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& at);

@@ -6,7 +6,8 @@
 //    terms that feed the tile), accumulating in MFMA registers.  No atomics:
 //    every output element is produced by exactly one workgroup in a fixed
 //    order, so results are bitwise reproducible run to run.
-//      FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32.
+//      FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32,
+//      BF16 (FP32 accumulation): v_mfma_f32_16x16x16_bf16.
 //  * vector kernels used by Lanczos: deterministic two-stage reductions for
 //    dot products / norms, axpy with a device-resident coefficient, block
 //    Gram-Schmidt (V^T w and w -= V c), precision conversion.
@@ -19,6 +20,8 @@ namespace sadmrg {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16_t;
+typedef short s4_t __attribute__((ext_vector_type(4)));
 
 __device__ inline d4_t mfma_16x16x4(double a, double b, d4_t c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -30,6 +33,22 @@ __device__ inline f4_t mfma_16x16x4(float a, float b, f4_t c) {
 template <class T> struct Acc4;
 template <> struct Acc4<double> { typedef d4_t type; };
 template <> struct Acc4<float> { typedef f4_t type; };
+template <> struct Acc4<bf16_t> { typedef f4_t type; };
+
+// Segment scale factors are applied in this type when a K step is staged
+// (BF16 operands are scaled in FP32 and rounded once); split-reduction
+// partial tiles are held in PartT.
+template <class T> struct ScaleT { typedef T type; };
+template <> struct ScaleT<bf16_t> { typedef float type; };
+template <class T> struct PartT { typedef T type; };
+template <> struct PartT<bf16_t> { typedef float type; };
+template <class T> __device__ inline T scaled(T v, typename ScaleT<T>::type s) { return v * s; }
+template <> __device__ inline bf16_t scaled<bf16_t>(bf16_t v, float s) {
+  return static_cast<bf16_t>(static_cast<float>(v) * s);
+}
+// Values of one k-slot group a lane reads per fragment: four FP32 or BF16,
+// two FP64 (16 bytes, or 8 for BF16).
+template <class T> struct FragW { static constexpr int NF = sizeof(T) == 8 ? 2 : 4; };
 
 // Output row of accumulator element r held by lane group g = lane / 16 (the
 // column is lane % 16).  The FP64 16x16x4 MFMA interleaves rows across lane
@@ -38,6 +57,7 @@ template <> struct Acc4<float> { typedef f4_t type; };
 template <class T> __device__ inline int acc_row(int g, int r);
 template <> __device__ inline int acc_row<double>(int g, int r) { return g + 4 * r; }
 template <> __device__ inline int acc_row<float>(int g, int r) { return 4 * g + r; }
+template <> __device__ inline int acc_row<bf16_t>(int g, int r) { return 4 * g + r; }
 
 // The arena bases are device allocations: the pointer is formed in the
 // global address space, so accesses through it compile to global (not FLAT)
@@ -136,16 +156,17 @@ __device__ inline void cursor_advance(OperandCursor<T>& c, int TK) {
 }
 
 template <class T, int R, int TK>
-__device__ inline void tile_store(T* lds, bool kr, const T (&v)[OperandTile<T, R, TK>::E], T scale, int tid) {
+__device__ inline void tile_store(T* lds, bool kr, const T (&v)[OperandTile<T, R, TK>::E],
+                                  typename ScaleT<T>::type scale, int tid) {
   using OT = OperandTile<T, R, TK>;
   if (!kr) {
     const int rt = tid % R, kt = tid / R;
 #pragma unroll
-    for (int e = 0; e < OT::E; ++e) lds[(kt + e * (256 / R)) * OT::LR + rt] = v[e] * scale;
+    for (int e = 0; e < OT::E; ++e) lds[(kt + e * (256 / R)) * OT::LR + rt] = scaled(v[e], scale);
   } else {
     const int kt = tid % TK, rt = tid / TK;
 #pragma unroll
-    for (int e = 0; e < OT::E; ++e) lds[OT::kr_off(rt + e * (256 / TK), kt)] = v[e] * scale;
+    for (int e = 0; e < OT::E; ++e) lds[OT::kr_off(rt + e * (256 / TK), kt)] = scaled(v[e], scale);
   }
 }
 
@@ -155,46 +176,76 @@ template <class T> struct Vec16;
 template <> struct Vec16<float> { typedef float4 type; };
 template <> struct Vec16<double> { typedef double2 type; };
 
-// NQ = 16 / sizeof(T) consecutive k-slot values (j = j0 .. j0+NQ-1) of
+// NF = FragW<T>::NF consecutive k-slot values (j = j0 .. j0+NF-1) of
 // fragment row `row` in chunk kc.
 template <class T, int R, int TK, bool KR>
-__device__ inline void tile_frag(const T* lds, int row, int kc, int g, int j0, T (&f)[16 / sizeof(T)]) {
+__device__ inline void tile_frag(const T* lds, int row, int kc, int g, int j0, T (&f)[FragW<T>::NF]) {
   using OT = OperandTile<T, R, TK>;
   constexpr int NQ = 16 / sizeof(T);
+  constexpr int NF = FragW<T>::NF;
   if constexpr (KR) {
     static_assert((OT::LK * sizeof(T)) % 16 == 0, "KR rows must be 16-byte multiples");
-    const auto* v = reinterpret_cast<const typename Vec16<T>::type*>(lds);
-    const auto x = v[OT::kr_off(row, kc + 4 * g + j0) / NQ];
-    if constexpr (NQ == 4) { f[0] = x.x; f[1] = x.y; f[2] = x.z; f[3] = x.w; }
-    else { f[0] = x.x; f[1] = x.y; }
+    if constexpr (NF == NQ) {
+      const auto* v = reinterpret_cast<const typename Vec16<T>::type*>(lds);
+      const auto x = v[OT::kr_off(row, kc + 4 * g + j0) / NQ];
+      if constexpr (NQ == 4) { f[0] = x.x; f[1] = x.y; f[2] = x.z; f[3] = x.w; }
+      else { f[0] = x.x; f[1] = x.y; }
+    } else {
+      // BF16: four values, 8 bytes, inside one 16-byte group (k = kc + 4g is a
+      // multiple of 4, so kr_off is too)
+      static_assert(NF * sizeof(T) == 8, "8-byte fragment");
+      const uint2 x = reinterpret_cast<const uint2*>(lds)[OT::kr_off(row, kc + 4 * g + j0) / 4];
+      f[0] = __builtin_bit_cast(T, static_cast<unsigned short>(x.x & 0xffffu));
+      f[1] = __builtin_bit_cast(T, static_cast<unsigned short>(x.x >> 16));
+      f[2] = __builtin_bit_cast(T, static_cast<unsigned short>(x.y & 0xffffu));
+      f[3] = __builtin_bit_cast(T, static_cast<unsigned short>(x.y >> 16));
+    }
   } else {
 #pragma unroll
-    for (int j = 0; j < NQ; ++j) f[j] = lds[(kc + 4 * g + j0 + j) * OT::LR + row];
+    for (int j = 0; j < NF; ++j) f[j] = lds[(kc + 4 * g + j0 + j) * OT::LR + row];
   }
 }
 
 // All MFMAs of one K step on an LDS buffer, operand orientations fixed.
 template <class T, int TM, int TN, int TK, bool KRA, bool KRB, class Acc, int FM, int FN>
 __device__ inline void mfma_step(const T* sa, const T* sb, Acc (&acc)[FM][FN], int wm, int wn, int lane) {
-  constexpr int NQ = 16 / sizeof(T);
-  constexpr int NGRP = (TK / 16) * (4 / NQ);  // fragment groups per K step
+  constexpr int NF = FragW<T>::NF;
+  constexpr int NGRP = (TK / 16) * (4 / NF);  // fragment groups per K step
   const int g = lane >> 4;
   // one fragment group live at a time keeps the accumulators, the staged
   // next step and the fragments inside the 256-register budget
 #pragma unroll 1
   for (int gi = 0; gi < NGRP; ++gi) {
-    const int kc = (gi / (4 / NQ)) * 16, j0 = (gi % (4 / NQ)) * NQ;
-    T af[FM][NQ], bf[FN][NQ];
+    const int kc = (gi / (4 / NF)) * 16, j0 = (gi % (4 / NF)) * NF;
+    T af[FM][NF], bf[FN][NF];
 #pragma unroll
     for (int i = 0; i < FM; ++i) tile_frag<T, TM, TK, KRA>(sa, wm * (TM / 2) + i * 16 + (lane & 15), kc, g, j0, af[i]);
 #pragma unroll
     for (int j = 0; j < FN; ++j) tile_frag<T, TN, TK, KRB>(sb, wn * (TN / 2) + j * 16 + (lane & 15), kc, g, j0, bf[j]);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
+    if constexpr (sizeof(T) == 2) {
+      // BF16: the four k-slot values of lane group g feed one 16-deep MFMA
+      s4_t pa[FM], pb[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = mfma_16x16x4(af[i][q], bf[j][q], acc[i][j]);
+        for (int q = 0; q < 4; ++q) pa[i][q] = __builtin_bit_cast(short, af[i][q]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pb[j][q] = __builtin_bit_cast(short, bf[j][q]);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pa[i], pb[j], acc[i][j], 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NF; ++q)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_16x16x4(af[i][q], bf[j][q], acc[i][j]);
+    }
   }
 }
 
@@ -206,7 +257,8 @@ template <class T, class TOut, int TM, int TN, int TK, int WPE, int PF>
 __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
                                                               const GemmProb* __restrict__ probs,
                                                               const GemmSeg* __restrict__ segs,
-                                                              ArenaTable arenas, TOut* __restrict__ partial,
+                                                              ArenaTable arenas,
+                                                              typename PartT<TOut>::type* __restrict__ partial,
                                                               const GemmReduce* __restrict__ reduce,
                                                               int* __restrict__ counters, int tile_base) {
   using TA = OperandTile<T, TM, TK>;
@@ -214,6 +266,8 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
   constexpr int FM = TM / 32;  // 16x16 fragments per wave (2 x 2 waves)
   constexpr int FN = TN / 32;
   typedef typename Acc4<T>::type acc_t;
+  typedef typename ScaleT<T>::type scale_t;
+  typedef typename PartT<TOut>::type part_t;
 
   __shared__ __align__(16) T lds[2][TA::ELEMS + TB::ELEMS];
 
@@ -237,7 +291,7 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
   while (lseg < tile.seg_hi && segs[lseg].K <= 0) ++lseg;
   int lk0 = 0, lK = 0;
   OperandCursor<T> ca, cb;
-  T lscale = T(1);
+  scale_t lscale = 1;
   bool lkra = false, lkrb = false;
   auto enter = [&](int si) {
     const GemmSeg sg = segs[si];
@@ -247,7 +301,7 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
     lkrb = (sg.flags & kTransB) == 0;
     cursor_init<T, TM, TK>(ca, A, sg.lda, lkra, tile.m0, prob.M, sg.K, tid);
     cursor_init<T, TN, TK>(cb, B, sg.ldb, lkrb, tile.n0, prob.N, sg.K, tid);
-    lscale = static_cast<T>(sg.scale);
+    lscale = static_cast<scale_t>(sg.scale);
     lK = sg.K;
   };
   // advance the loader past the step it just loaded; false when exhausted
@@ -281,7 +335,7 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
   // loaded with
   struct Staged {
     T a[TA::E], b[TB::E];
-    T scale;
+    scale_t scale;
     bool kra, krb, valid;
   };
   bool more = lseg < tile.seg_hi;
@@ -298,7 +352,7 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
   };
   auto put = [&](const Staged& r, int buf) {
     tile_store<T, TM, TK>(&lds[buf][0], r.kra, r.a, r.scale, tid);
-    tile_store<T, TN, TK>(&lds[buf][TA::ELEMS], r.krb, r.b, T(1), tid);
+    tile_store<T, TN, TK>(&lds[buf][TA::ELEMS], r.krb, r.b, scale_t(1), tid);
     skra = (skra & ~(1u << buf)) | (r.kra ? 1u << buf : 0u);
     skrb = (skrb & ~(1u << buf)) | (r.krb ? 1u << buf : 0u);
   };
@@ -357,7 +411,7 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
   if (tile.part >= 0) {
     // split reduction: dense partial tile; the last workgroup of the task to
     // sign in folds every slot into C in slot order
-    TOut* P = partial + int64_t(tile.part) * (TM * TN);
+    part_t* P = partial + int64_t(tile.part) * (TM * TN);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -366,7 +420,7 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int m = wm * (TM / 2) + i * 16 + acc_row<T>(g, r);
-          P[m + n * TM] = static_cast<TOut>(acc[i][j][r]);
+          P[m + n * TM] = static_cast<part_t>(acc[i][j][r]);
         }
       }
     __shared__ int s_last;
@@ -386,12 +440,12 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
     for (int idx = tid; idx < TM * TN; idx += 256) {
       const int m = idx % TM, n = idx / TM;
       if (m >= rows || n >= cols) continue;
-      const TOut* Q = partial + int64_t(rt.part_begin) * (TM * TN) + idx;
-      TOut v = Q[0];
+      const part_t* Q = partial + int64_t(rt.part_begin) * (TM * TN) + idx;
+      part_t v = Q[0];
       for (int q = 1; q < rt.nparts; ++q) v += Q[int64_t(q) * (TM * TN)];
       TOut* p = C + (rt.m0 + m) + int64_t(rt.n0 + n) * prob.ldc;
-      if (prob.accumulate) *p += v;
-      else *p = v;
+      if (prob.accumulate) *p = static_cast<TOut>(static_cast<part_t>(*p) + v);
+      else *p = static_cast<TOut>(v);
     }
     if (tid == 0) __hip_atomic_store(&counters[tile.red], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
@@ -408,9 +462,15 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
         const int gm = tile.m0 + wm * (TM / 2) + i * 16 + acc_row<T>(g, r);
         if (gm >= prob.M) continue;
         TOut* p = C + gm + int64_t(gn) * prob.ldc;
-        const TOut v = static_cast<TOut>(acc[i][j][r]);
-        if (prob.accumulate) *p += v;
-        else *p = v;
+        if constexpr (sizeof(TOut) == 2) {
+          // BF16 output: one rounding of the FP32 sum
+          const float v = acc[i][j][r];
+          *p = static_cast<TOut>(prob.accumulate ? static_cast<float>(*p) + v : v);
+        } else {
+          const TOut v = static_cast<TOut>(acc[i][j][r]);
+          if (prob.accumulate) *p += v;
+          else *p = v;
+        }
       }
     }
 }
@@ -535,7 +595,7 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 // limits for very large tile tables.
 template <class T, class TOut, int TM, int TN, int TK, int WPE = 2, int PF = 2>
 static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
-  TOut* part = static_cast<TOut*>(L.partial);
+  typename PartT<TOut>::type* part = static_cast<typename PartT<TOut>::type*>(L.partial);
   const int chunk = 1 << 30;
   for (int base = 0; base < L.ntiles; base += chunk) {
     const int n = L.ntiles - base < chunk ? L.ntiles - base : chunk;
@@ -562,6 +622,14 @@ hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L,
       if (tile_cfg == 1) return launch_real<float, double, 128, 128, 32, 2, 1>(L, st);
       if (tile_cfg == 2) return launch_real<float, double, 128, 64, 32>(L, st);
       return launch_real<float, double, 64, 64, 32>(L, st);
+    case GemmPrec::BF16:
+      if (tile_cfg == 1) return launch_real<bf16_t, bf16_t, 128, 128, 64, 1, 1>(L, st);
+      if (tile_cfg == 2) return launch_real<bf16_t, bf16_t, 128, 64, 64, 2, 1>(L, st);
+      return launch_real<bf16_t, bf16_t, 64, 64, 64, 2, 1>(L, st);
+    case GemmPrec::BF16_OUT64:
+      if (tile_cfg == 1) return launch_real<bf16_t, double, 128, 128, 64, 1, 1>(L, st);
+      if (tile_cfg == 2) return launch_real<bf16_t, double, 128, 64, 64, 2, 1>(L, st);
+      return launch_real<bf16_t, double, 64, 64, 64, 2, 1>(L, st);
     case GemmPrec::Z64: {
       if (L.nreduce > 0 || tile_cfg != 0) return hipErrorInvalidValue;  // no split or large tiles for complex
       hipLaunchKernelGGL((grouped_zgemm_kernel<64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st, L.tiles,
@@ -740,6 +808,12 @@ __global__ void f64_to_f32_kernel(int64_t n, const double* __restrict__ x, float
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += stride) y[i] = static_cast<float>(x[i]);
 }
 
+__global__ void f64_to_bf16_kernel(int64_t n, const double* __restrict__ x, bf16_t* __restrict__ y) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += stride)
+    y[i] = static_cast<bf16_t>(x[i]);
+}
+
 __global__ void f32_to_f64_kernel(int64_t n, const float* __restrict__ x, double* __restrict__ y) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += stride) y[i] = static_cast<double>(x[i]);
@@ -911,6 +985,11 @@ hipError_t launch_zscal(int64_t n, double sr, double si, const double* x, double
 
 hipError_t launch_f64_to_f32(int64_t n, const double* x, float* y, hipStream_t st) {
   hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_f64_to_bf16(int64_t n, const double* x, uint16_t* y, hipStream_t st) {
+  hipLaunchKernelGGL(f64_to_bf16_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, x, reinterpret_cast<bf16_t*>(y));
   return hipGetLastError();
 }
 

@@ -11,6 +11,8 @@ enum class GemmPrec : int {
   F32 = 1,        // float operands, FP32 MFMA, float output
   F32_OUT64 = 2,  // float operands, FP32 MFMA, double output
   Z64 = 3,        // complex<double> operands (interleaved), 4 FP64 MFMA products
+  BF16 = 4,       // bfloat16 operands, BF16 MFMA with FP32 accumulation, bfloat16 output
+  BF16_OUT64 = 5, // bfloat16 operands, BF16 MFMA with FP32 accumulation, double output
 };
 
 // Device tables of one grouped launch (see grouped_gemm.hpp).  With split
@@ -30,7 +32,8 @@ struct GemmLaunch {
 };
 
 // Output tile per configuration: 0: 64x64, 1: 128x128, 2: 128x64 (K steps of
-// 16 for FP64, 32 for FP32 operands).  Complex tables use configuration 0.
+// 16 for FP64, 32 for FP32, 64 for BF16 operands).  Complex tables use
+// configuration 0.
 // One launch, split reductions included.
 constexpr int kTileConfigs = 3;
 hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& launch, hipStream_t st);
@@ -74,6 +77,8 @@ hipError_t launch_axpby(int64_t n, double a, double b, const double* x, double* 
 hipError_t launch_zscal(int64_t n, double sr, double si, const double* x, double* y, hipStream_t st);
 hipError_t launch_f64_to_f32(int64_t n, const double* x, float* y, hipStream_t st);
 hipError_t launch_f32_to_f64(int64_t n, const float* x, double* y, hipStream_t st);
+// round to nearest even; y holds bfloat16 bit patterns
+hipError_t launch_f64_to_bf16(int64_t n, const double* x, uint16_t* y, hipStream_t st);
 hipError_t launch_fill(int64_t n, double v, double* y, hipStream_t st);
 // dst[i + j*ldd] (+)= scale * colscale[j] * src[i*s_row + j*s_col]; elements of w doubles.
 hipError_t launch_copy2d(int rows, int cols, const double* src, int64_t s_row, int64_t s_col, double* dst,

@@ -153,6 +153,30 @@ def test_heff_on_reference_environments(ctx, sad, name):
     got32 = tl.from_dev(d.heff(il, ir, vec, sad.FP32))
     rel32, _ = tl.compare(got32, tens["heff"])
     assert rel32 <= 1e-5, rel32
+    # BF16 operands (8 significant bits, FP32 accumulation): every product
+    # carries up to 2 * 2^-9 relative rounding of its factors, so the output
+    # error stays near 1e-3 of the largest element, and far above the FP32
+    # path's (the BF16 kernels ran)
+    got16 = tl.from_dev(d.heff(il, ir, vec, sad.BF16))
+    rel16, _ = tl.compare(got16, tens["heff"])
+    assert 1e-5 < rel16 <= 2e-2, rel16
+
+
+@pytest.mark.parametrize("name", ["dmrg_su2_ring10", "dmrg_su2_j1j2_4x3", "dmrg_u1_obc32"])
+def test_bf16_lanczos_with_fp64_refinement(ctx, sad, name):
+    """BF16 Krylov matvecs with FP64 convergence checks and residual
+    refinement (the ablation of the mixed path): the FP64 Rayleigh quotient
+    of the refined vector reaches the reference's Lanczos energy to 1e-9
+    relative."""
+    tens, extra = _load_case(name)
+    d = _engine_from_state(ctx, sad, tens, extra)
+    il, ir = (int(x) for x in extra["cap_step"])
+    cg2 = 1 if tens["vec"]["sym"] == 1 else 0
+    d.set_precision(sad.BF16, refine=True, refine_tol=1e-6)
+    d.set_refinement(6, 12)
+    e = d.lanczos(il, ir, tl.to_dev(ctx, tens["vec"], cg2))
+    e_ref = float(extra["lanczos_e"])
+    assert abs(e - e_ref) <= 1e-9 * abs(e_ref), (e, e_ref)
 
 
 @pytest.mark.parametrize("name", DMRG_CASES)
