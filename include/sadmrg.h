@@ -208,6 +208,18 @@ int sadmrg_dmrg_copy_state(sadmrg_dmrg* dst, const sadmrg_dmrg* src);
 /* replaces hmap (ns x ns, row-major) and coup (dmrg_inst.cpp:117-176) */
 int sadmrg_dmrg_set_couplings(sadmrg_dmrg* d, const int* hmap, const double* coup, int ncoup);
 int sadmrg_dmrg_get_couplings(sadmrg_dmrg* d, int* hmap, double* coup, int cap);
+/* Channel weights, U(1) drivers only (no reference counterpart: the
+   reference's couplings are real, one per bond, dmrg.hpp:79).  A bond
+   (j < k) with coupling index c contributes coup[c] * sum_q w[c][q]
+   O_j(q) O_k(q) over the spin operators' channel labels q = -2, 0, +2,
+   instead of coup[c] * S_j.S_k (all weights 1).  w holds 6 doubles per
+   coupling index: (re, im) for q = -2, 0, +2; n = 6 * ncoup, or 0 to
+   restore the isotropic exchange.  w(-2) must be the conjugate of w(+2)
+   and w(0) real (Hermitian bond terms); complex weights need a complex
+   driver.  XXZ with a z-axis Dzyaloshinskii-Moriya term D (S^x S^y -
+   S^y S^x): w = (Jxy - iD, Jz, Jxy + iD); the sign convention of D does not
+   change energies. */
+int sadmrg_dmrg_set_channel_weights(sadmrg_dmrg* d, const double* w, int n);
 /* the global max_dcut (main/main_dmrg_su2_real.cpp:32) */
 int sadmrg_dmrg_set_max_dcut(sadmrg_dmrg* d, int max_dcut);
 int sadmrg_dmrg_set_precision(sadmrg_dmrg* d, int precision, int refine, double refine_tol);

@@ -91,6 +91,7 @@ def lib():
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_create_exci": (i, [vp, i, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_exci": (i, [vp, ip]),
+        "sadmrg_dmrg_set_channel_weights": (i, [vp, dp, i]),
         "sadmrg_dmrg_set_lower_state": (i, [vp, i, vp]),
         "sadmrg_dmrg_lower_energy": (i, [vp, i, dp]),
         "sadmrg_dmrg_destroy": (i, [vp]),
@@ -505,6 +506,16 @@ class DMRG:
         c = np.ascontiguousarray(coup, dtype=np.float64)
         check(lib().sadmrg_dmrg_set_couplings(self.ptr, _iptr(h), _dptr(c), len(c)))
 
+    def set_channel_weights(self, w):
+        """w: (ncoup, 3) complex weights for the channel labels q = -2, 0, +2
+        (None restores the isotropic exchange).  See xxz_dm_weights."""
+        if w is None:
+            check(lib().sadmrg_dmrg_set_channel_weights(self.ptr, None, 0))
+            return
+        w = np.asarray(w, dtype=np.complex128).reshape(-1, 3)
+        flat = np.ascontiguousarray(np.stack([w.real, w.imag], axis=-1).reshape(-1))
+        check(lib().sadmrg_dmrg_set_channel_weights(self.ptr, _dptr(flat), len(flat)))
+
     def couplings(self):
         h = np.zeros(self.ns * self.ns, dtype=np.int32)
         c = np.zeros(self.ns + 1)
@@ -637,6 +648,18 @@ class DMRG:
 
 
 # ---- lattice maps for DMRG.set_couplings -------------------------------------
+def xxz_dm_weights(ncoup, params):
+    """Channel weights of XXZ exchange with a z-axis Dzyaloshinskii-Moriya
+    term, coup[c] * [Jxy (S^x S^x + S^y S^y) + Jz S^z S^z + D (S^x S^y - S^y S^x)]
+    for the coupling indices in params = {c: (Jxy, Jz, D)}; other indices
+    keep the isotropic exchange.  Returns an (ncoup, 3) complex array for
+    DMRG.set_channel_weights."""
+    w = np.ones((ncoup, 3), dtype=np.complex128)
+    for c, (jxy, jz, dm) in params.items():
+        w[c] = (jxy - 1j * dm, jz, jxy + 1j * dm)
+    return w
+
+
 def snake_site(x, y, width):
     """Snake ordering of the reference constructor (dmrg_inst.cpp:121-140):
     even columns run up, odd columns run down."""

@@ -1030,6 +1030,30 @@ void op_scale(BTensor& t, double re, double im) {
   }
 }
 
+void op_axpy_channels(BTensor& y, double s, const dcmplex (&w)[3], const BTensor& x, int bond) {
+  if (x.is_null()) return;
+  require(x.sym() == SymKind::U1, "channel weights: U(1) tensors only");
+  require(bond >= 0 && bond < x.nbond(), "channel weights: bond out of range");
+  for (const dcmplex& c : w)
+    require(x.cplx || c.imag() == 0.0, "channel weights: complex weights need complex tensors");
+  BTensor t;
+  copy_tensor(t, x);
+  std::vector<int> lab(t.nbond());
+  for (int b = 0; b < t.nten(); ++b) {
+    if (!t.present(b) || t.block_size(b) == 0) continue;
+    t.st.labels(b, lab.data());
+    const int q = lab[bond];
+    require(q == -2 || q == 0 || q == 2, "channel weights: channel label outside {-2, 0, 2}");
+    const dcmplex c = s * w[q / 2 + 1];
+    if (!t.cplx)
+      SADMRG_HIP(launch_axpby(t.block_size(b), c.real(), 0.0, nullptr, t.block_ptr(b), t.ctx->stream()));
+    else
+      SADMRG_HIP(launch_zscal(t.block_size(b), c.real(), c.imag(), t.block_ptr(b), t.block_ptr(b), t.ctx->stream()));
+  }
+  op_axpy(y, 1.0, 0.0, t);
+  t.ctx->sync();  // t leaves scope
+}
+
 void op_fill(BTensor& t, double v) {
   if (t.is_null() || t.nelem == 0) return;
   if (!t.cplx) SADMRG_HIP(launch_fill(t.nelem, v, t.dptr(), t.ctx->stream()));

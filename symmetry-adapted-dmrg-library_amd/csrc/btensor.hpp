@@ -129,6 +129,11 @@ void op_scale(BTensor& t, double re, double im = 0.0);            // *=, /=
 void op_axpy(BTensor& y, double re, double im, const BTensor& x); // += (1,0), -= (-1,0)
 dcmplex op_dot(const BTensor& x, const BTensor& y);               // inner_prod_c
 double op_normalize(BTensor& t);                                  // normalize_vector
+// y += s * w[c] * x with w[c] chosen per block by the label c of bond `bond`
+// (U(1) channel labels -2, 0, +2 -> w[0], w[1], w[2]).  Complex weights need
+// complex tensors.  No reference counterpart (the reference's couplings are
+// one real number per bond).
+void op_axpy_channels(BTensor& y, double s, const dcmplex (&w)[3], const BTensor& x, int bond);
 void op_fill(BTensor& t, double v);                               // operator=(double)
 
 // Global generator of the reference (dense_inst.cpp:23-24): mt19937_64(42),

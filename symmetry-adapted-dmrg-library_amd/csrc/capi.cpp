@@ -673,6 +673,15 @@ int sadmrg_dmrg_lower_energy(sadmrg_dmrg* d, int j, double* e) {
   });
 }
 
+int sadmrg_dmrg_set_channel_weights(sadmrg_dmrg* d, const double* w, int n) {
+  return guard([&] {
+    need(d, "dmrg");
+    if (n < 0) fail(kErrInvalid, "channel weights: n >= 0");
+    if (n > 0) need(w, "w");
+    d->d->set_channel_weights(std::vector<double>(w, w + n));
+  });
+}
+
 int sadmrg_dmrg_destroy(sadmrg_dmrg* d) {
   return guard([&] { delete d; });
 }

@@ -71,6 +71,8 @@ double Dmrg::lower_energy(int j) const {
 
 void Dmrg::set_couplings(const std::vector<int>& hmap, const std::vector<double>& coup) {
   require(static_cast<int>(hmap.size()) == ns_ * ns_, "set_couplings: hmap must be ns x ns");
+  require(chw_.empty() || chw_.size() == 3 * coup.size(),
+          "set_couplings: the channel weights cover a different number of coupling indices; set them again");
   for (int v : hmap) require(v >= 0 && v < static_cast<int>(coup.size()), "set_couplings: coupling index out of range");
   hmap_ = hmap;
   coup_ = coup;
@@ -83,6 +85,34 @@ void Dmrg::set_couplings(const std::vector<int>& hmap, const std::vector<double>
       for (int k = 0; k < i; ++k)
         if (hm(j, k) != 0) frr_[i * ns_ + j] = 1;
   }
+}
+
+void Dmrg::set_channel_weights(const std::vector<double>& w) {
+  if (w.empty()) {
+    chw_.clear();
+    return;
+  }
+  require(sym_ == SymKind::U1, "channel weights: U(1) drivers only (SU(2) fixes the isotropic exchange)");
+  require(w.size() == 6 * coup_.size(), "channel weights: 6 doubles per coupling index are required");
+  std::vector<dcmplex> c(3 * coup_.size());
+  for (size_t i = 0; i < c.size(); ++i) {
+    c[i] = dcmplex(w[2 * i], w[2 * i + 1]);
+    require(cplx_ || c[i].imag() == 0.0, "channel weights: complex weights need a complex driver");
+  }
+  for (size_t k = 0; k < coup_.size(); ++k)  // Hermitian bond terms: w(-q) = conj(w(q))
+    require(std::abs(c[3 * k] - std::conj(c[3 * k + 2])) <= 1e-14 * (1.0 + std::abs(c[3 * k])) &&
+                c[3 * k + 1].imag() == 0.0,
+            "channel weights: w(-2) must be the conjugate of w(+2) and w(0) real");
+  chw_ = c;
+}
+
+void Dmrg::add_coupled(BTensor& op, int c, const BTensor& x) const {
+  if (chw_.empty()) {
+    op_axpy(op, coup_[c], 0.0, x);
+    return;
+  }
+  const dcmplex w[3] = {chw_[3 * c], chw_[3 * c + 1], chw_[3 * c + 2]};
+  op_axpy_channels(op, coup_[c], w, x, 1);
 }
 
 void Dmrg::copy_state_from(const Dmrg& o) {
@@ -111,6 +141,7 @@ void Dmrg::copy_state_from(const Dmrg& o) {
     copy_tensor(overlapvec_[j], o.overlapvec_[j]);
   }
   lower_enr_ = o.lower_enr_;
+  chw_ = o.chw_;
   // the last Lanczos solve's wave function and bond (coupling_gradient)
   copy_tensor(last_vec_, o.last_vec_);
   last_il_ = o.last_il_;
@@ -213,7 +244,7 @@ void Dmrg::prepare_site_operator_from_left(int i, int flag) {
       op_env(opr_[i][k], kOperatorTransformation, uu_[i], uu_[i], &opr_[i - 1][k], nullptr, 0, cg);
   BTensor op;
   for (int k = 0; k < i; ++k)
-    if (hm(i, k) > 0 && !opr_[i - 1][k].is_null()) op_axpy(op, coup_[hm(i, k)], 0.0, opr_[i - 1][k]);
+    if (hm(i, k) > 0 && !opr_[i - 1][k].is_null()) add_coupled(op, hm(i, k), opr_[i - 1][k]);
   if (!op.is_null()) {
     BTensor ht;
     op_env(ht, kOperatorPairup, uu_[i], uu_[i], &op, &sigma_[1], 0, cg);
@@ -245,7 +276,7 @@ void Dmrg::prepare_site_operator_from_right(int i, int flag) {
   BTensor op;
   for (int m = 0; m < ns_ - 1 - i; ++m) {
     const int k = i + 1 + m;
-    if (hm(i, k) > 0 && !opr_[i + 1][k].is_null()) op_axpy(op, coup_[hm(i, k)], 0.0, opr_[i + 1][k]);
+    if (hm(i, k) > 0 && !opr_[i + 1][k].is_null()) add_coupled(op, hm(i, k), opr_[i + 1][k]);
   }
   if (!op.is_null()) {
     BTensor ht;
@@ -269,7 +300,7 @@ void Dmrg::prepare_site_operator_from_left(int i) {
     if (fll_[i * ns_ + k] == 1) op_env(opr_[i][k], kOperatorTransformation, u1, u1, &opr_[i - 1][k], nullptr, 0, cg);
   BTensor op;
   for (int k = 0; k < i; ++k)
-    if (hm(i, k) > 0 && !opr_[i - 1][k].is_null()) op_axpy(op, coup_[hm(i, k)], 0.0, opr_[i - 1][k]);
+    if (hm(i, k) > 0 && !opr_[i - 1][k].is_null()) add_coupled(op, hm(i, k), opr_[i - 1][k]);
   if (!op.is_null()) {
     BTensor ht;
     op_env(ht, kOperatorPairup, u1, u1, &op, &sigma_[1], 0, cg);
@@ -294,7 +325,7 @@ void Dmrg::prepare_site_operator_from_right(int i) {
   BTensor op;
   for (int m = 0; m < ns_ - 1 - i; ++m) {
     const int k = i + 1 + m;
-    if (hm(i, k) > 0 && !opr_[i + 1][k].is_null()) op_axpy(op, coup_[hm(i, k)], 0.0, opr_[i + 1][k]);
+    if (hm(i, k) > 0 && !opr_[i + 1][k].is_null()) add_coupled(op, hm(i, k), opr_[i + 1][k]);
   }
   if (!op.is_null()) {
     BTensor ht;
@@ -374,7 +405,7 @@ std::unique_ptr<HeffPlan> Dmrg::make_heff(int il, int ir, const BTensor& psi,
       const int j = m0;
       for (int m = 0; m < m4; ++m) {
         const int k = ir + m;
-        if (hm(j, k) > 0 && !opr_[ir][k].is_null()) op_axpy(*op, coup_[hm(j, k)], 0.0, opr_[ir][k]);
+        if (hm(j, k) > 0 && !opr_[ir][k].is_null()) add_coupled(*op, hm(j, k), opr_[ir][k]);
       }
       if (op->is_null()) continue;
       o1.push_back(&opr_[il][j]);
@@ -383,7 +414,7 @@ std::unique_ptr<HeffPlan> Dmrg::make_heff(int il, int ir, const BTensor& psi,
       const int k = ir + m0;
       for (int m = 0; m < m4; ++m) {
         const int j = m;
-        if (hm(j, k) > 0 && !opr_[il][j].is_null()) op_axpy(*op, coup_[hm(j, k)], 0.0, opr_[il][j]);
+        if (hm(j, k) > 0 && !opr_[il][j].is_null()) add_coupled(*op, hm(j, k), opr_[il][j]);
       }
       if (op->is_null()) continue;
       o1.push_back(op.get());
@@ -513,6 +544,7 @@ void block_density(BTensor& lam, const BTensor& psi, const BTensor& h, bool left
 std::vector<double> Dmrg::coupling_gradient(double* energy) {
   require(last_il_ >= 0 && !last_vec_.is_null(), "coupling gradient: no Lanczos solve has run");
   require(exci_ == 0, "coupling gradient: ground-state drivers only (max_exci = 0)");
+  require(chw_.empty(), "coupling gradient: isotropic couplings only (no channel weights)");
   require(!cplx_, "coupling gradient: real tensors only");
   const int il = last_il_, ir = last_ir_;
   const BTensor& psi = last_vec_;

@@ -36,6 +36,14 @@ class Dmrg {
   int ns() const { return ns_; }
   // Replace the lattice map and recompute fll/frr (dmrg_inst.cpp:163-176).
   void set_couplings(const std::vector<int>& hmap, const std::vector<double>& coup);
+  // Channel weights (U(1) only; no reference counterpart): bond (j < k) with
+  // coupling index c contributes coup[c] * sum_q w[c][q] O_j(q) O_k(q) over
+  // the channel labels q = -2, 0, +2 of the spin operators, instead of
+  // coup[c] * S_j.S_k.  w holds 6 doubles (re, im for q = -2, 0, +2) per
+  // coupling index; an empty vector restores the isotropic exchange.  XXZ
+  // with a z-axis Dzyaloshinskii-Moriya term: w = (Jxy - i D, Jz, Jxy + i D).
+  void set_channel_weights(const std::vector<double>& w);
+  bool has_channel_weights() const { return !chw_.empty(); }
   void set_max_dcut(int d) { max_dcut_ = d; }
   void set_lanczos(const LanczosOpts& o) { lopt_ = o; }
   const LanczosOpts& lanczos_opts() const { return lopt_; }
@@ -117,6 +125,8 @@ class Dmrg {
   void initialize_local_operators();
   void boundary_tensors();
   int hm(int i, int j) const { return hmap_[i * ns_ + j]; }
+  // op += coup[c] * x, channel weighted when weights are set
+  void add_coupled(BTensor& op, int c, const BTensor& x) const;
   std::string file_tag(int dim, int exci) const;
   void overlap_env_left(int i, int flag);
   void overlap_env_right(int i, int flag);
@@ -145,6 +155,7 @@ class Dmrg {
   std::vector<std::vector<BTensor>> orth_, ovlp_;  // [state][site]
   std::vector<BTensor> overlapvec_;
   std::vector<double> lower_enr_;
+  std::vector<dcmplex> chw_;  // 3 per coupling index (q = -2, 0, +2), empty: isotropic
 };
 
 }  // namespace sadmrg

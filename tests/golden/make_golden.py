@@ -225,6 +225,8 @@ CASES = {
     # reference-native lattice: periodic snake (xx=1 gives a ring)
     "dmrg_su2_ring10": dict(sym=1, cplx=0, xx=1, yy=10, D=16, lattice=None, nsweep=2, capture_at=("right", 3)),
     "dmrg_u1_ring10": dict(sym=0, cplx=0, xx=1, yy=10, D=24, lattice=None, nsweep=2, capture_at=("right", 3)),
+    # complex scalars (DMRG<complex<double>, U1Symmetry>), same lattice
+    "dmrg_u1c_ring10": dict(sym=0, cplx=1, xx=1, yy=10, D=24, lattice=None, nsweep=2, capture_at=("right", 3)),
     # open chain and J1-J2 cylinder through the coupling map
     "dmrg_su2_obc12": dict(sym=1, cplx=0, xx=1, yy=12, D=24, lattice="obc", nsweep=2, capture_at=("left", 5)),
     "dmrg_su2_j1j2_4x3": dict(sym=1, cplx=0, xx=3, yy=4, D=24, lattice="j1j2", nsweep=2, capture_at=("right", 4)),

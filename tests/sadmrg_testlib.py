@@ -240,6 +240,48 @@ def heisenberg_ed(hmap, coup, nlow=1):
     return np.sort(w)
 
 
+def xxz_dm_hamiltonian(hmap, coup, params, sz=0):
+    """Sparse H and its basis (bit k of a state = site k, set = spin up) in
+    the S^z = sz sector of
+    H = sum_{i<j} coup[c] [Jxy (S^x S^x + S^y S^y) + Jz S^z S^z + D (S^x_i S^y_j - S^y_i S^x_j)],
+    c = hmap[i, j] > 0, (Jxy, Jz, D) = params.get(c, (1, 1, 0)).  In the S^z
+    basis the bond term is Jz S^z S^z + (Jxy + iD)/2 S^+_i S^-_j + (Jxy - iD)/2 S^-_i S^+_j."""
+    import scipy.sparse as sp
+    ns = hmap.shape[0]
+    nup = ns // 2 + sz
+    states = [s for s in range(1 << ns) if bin(s).count("1") == nup]
+    index = {s: k for k, s in enumerate(states)}
+    rows, cols, vals = [], [], []
+    diag = np.zeros(len(states))
+    bonds = [(i, j, hmap[i, j]) for i in range(ns) for j in range(i + 1, ns) if hmap[i, j] > 0]
+    for k, s in enumerate(states):
+        for i, j, c in bonds:
+            jxy, jz, dm = params.get(int(c), (1.0, 1.0, 0.0))
+            f = coup[c]
+            si, sj = (s >> i) & 1, (s >> j) & 1
+            diag[k] += f * jz * (0.25 if si == sj else -0.25)
+            if si != sj:
+                t = s ^ (1 << i) ^ (1 << j)
+                # S^+_i S^-_j raises site i (si = 0 -> 1): amplitude (Jxy + iD)/2
+                amp = (jxy + 1j * dm) / 2 if si == 0 else (jxy - 1j * dm) / 2
+                rows.append(index[t])
+                cols.append(k)
+                vals.append(f * amp)
+    H = sp.csr_matrix((vals, (rows, cols)), shape=(len(states), len(states)), dtype=np.complex128)
+    return H + sp.diags(diag.astype(np.complex128)), states
+
+
+def xxz_dm_ed(hmap, coup, params, nlow=1, sz=0):
+    """Lowest nlow energies of xxz_dm_hamiltonian."""
+    import scipy.sparse.linalg as sla
+    H, states = xxz_dm_hamiltonian(hmap, coup, params, sz)
+    if len(states) <= 400:
+        w = np.linalg.eigvalsh(H.toarray())[:nlow]
+    else:
+        w = sla.eigsh(H, k=nlow, which="SA", return_eigenvectors=False)
+    return np.sort(np.real(w))
+
+
 def heisenberg_ed_singlets(hmap, coup, nlow=2):
     """Lowest nlow energies of H = sum_{i<j} coup[hmap[i,j]] S_i.S_j among
     total-spin-0 states.  H + c S^2 with S^2 = 3N/4 + 2 sum_{i<j} S_i.S_j

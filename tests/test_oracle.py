@@ -10,7 +10,8 @@ import sadmrg_testlib as tl
 import sadmrg_oracle as orc
 
 HVM_TAGS = ["su2_real", "u1_real", "u1_complex", "su2_complex"]
-DMRG_CASES = ["dmrg_su2_ring10", "dmrg_u1_ring10", "dmrg_su2_obc12", "dmrg_su2_j1j2_4x3", "dmrg_u1_obc32"]
+DMRG_CASES = ["dmrg_su2_ring10", "dmrg_u1_ring10", "dmrg_u1c_ring10", "dmrg_su2_obc12", "dmrg_su2_j1j2_4x3",
+              "dmrg_u1_obc32"]
 
 
 # ---- closed forms -------------------------------------------------------------------
@@ -144,7 +145,7 @@ def test_oracle_heff_lanczos_svd_on_reference_state(name):
 
 
 # ---- physics pins --------------------------------------------------------------------
-@pytest.mark.parametrize("name,tol", [("dmrg_su2_ring10", 1e-9), ("dmrg_u1_ring10", 1e-9),
+@pytest.mark.parametrize("name,tol", [("dmrg_su2_ring10", 1e-9), ("dmrg_u1_ring10", 1e-9), ("dmrg_u1c_ring10", 1e-9),
                                       ("dmrg_su2_obc12", 1e-9), ("dmrg_su2_j1j2_4x3", 2e-2)])
 def test_reference_energies_against_exact_diagonalisation(name, tol):
     """The reference's converged sweep energy versus ED of the same lattice map
@@ -161,7 +162,7 @@ refbind = pytest.importorskip("refbind")
 
 
 @pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built (no reference sources here)")
-@pytest.mark.parametrize("name", ["dmrg_su2_ring10", "dmrg_u1_ring10"])
+@pytest.mark.parametrize("name", ["dmrg_su2_ring10", "dmrg_u1_ring10", "dmrg_u1c_ring10"])
 def test_oracle_env_updates_against_reference_library(name):
     """Each environment update of the oracle equals the reference's on real
     mid-sweep tensors (both flags, truncated and enlarged bases)."""
