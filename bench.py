@@ -238,6 +238,105 @@ def c1_run(sad, args):
     return line
 
 
+def c4_run(sad, args):
+    """C4 (64 independent U(1) complex XXZ + Dzyaloshinskii-Moriya chains,
+    L in {32, 48, 64, 96}, Delta ~ U[0.5, 1.5], D_z ~ U[0, 0.5], from seed 7,
+    m = 256): the centre-bond H_eff psi of every instance in one launch per
+    stage (merged block tables) against one plan per instance, complex128
+    against complex64 operands, and the reference's CPU path on a sample of
+    instances.  Sector structure per instance from its length; entries
+    synthetic, with the instance's channel weights (1 - i D_z, Delta,
+    1 + i D_z) on the bond operator."""
+    import sadmrg_testlib as tl
+    n_inst, m = 64, 256
+    rng = np.random.default_rng(7)
+    lengths = [int(x) for x in rng.choice([32, 48, 64, 96], size=n_inst)]
+    delta = rng.uniform(0.5, 1.5, n_inst)
+    dz = rng.uniform(0.0, 0.5, n_inst)
+    wls = []
+    for i, L in enumerate(lengths):
+        wl = tl.build_workload_u1(m=m, length=L, seed=8 + i, cplx=1)
+        o2 = wl["o2"][0]
+        _, _, bonds = tl.parse_header(o2["header"])
+        w = {-2: 1 - 1j * dz[i], 0: delta[i] + 0j, 2: 1 + 1j * dz[i]}
+        off = 0
+        for b in range(tl.nten(bonds)):
+            if not o2["present"][b]:
+                continue
+            lab, _ = tl.block_labels(bonds, b)
+            k = int(np.prod(tl.block_dims(bonds, b)))
+            z = o2["data"][2 * off:2 * (off + k):2] + 1j * o2["data"][2 * off + 1:2 * (off + k):2]
+            z = z * w[lab[1]]
+            o2["data"][2 * off:2 * (off + k):2], o2["data"][2 * off + 1:2 * (off + k):2] = z.real, z.imag
+            off += k
+        wls.append(wl)
+    ctx = sad.Context(0)
+    plans, ins, outs = [], [], []
+    for wl in wls:
+        psi = tl.to_dev(ctx, wl["psi"])
+        plans.append(sad.Heff(ctx, None, tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"]),
+                              [tl.to_dev(ctx, wl["o1"][0])], [tl.to_dev(ctx, wl["o2"][0])], psi))
+        ins.append(psi)
+        outs.append(psi.copy())
+    merged = sad.Heff.merge(plans)
+    st = merged.stats()
+    data = np.concatenate([wl["psi"]["data"] for wl in wls])
+    n = data.size // 2
+    vin = tl.to_dev(ctx, dict(sym=0, cplx=1, header=tl.make_header(0, [(1, [0], [n]), (-1, [0], [1])]),
+                              present=np.ones(1, np.uint8), data=data))
+    vout = vin.copy()
+    flops = st["flops"]
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    res = {}
+    for name, prec in (("complex128", sad.FP64), ("complex64", sad.FP32)):
+        ms_m = timed(lambda: merged.apply_device(vin.device_ptr(), vout.device_ptr(), prec))
+        ms_p = timed(lambda: [p.apply_device(a.device_ptr(), b.device_ptr(), prec)
+                              for p, a, b in zip(plans, ins, outs)])
+        res[name] = {"merged_ms": round(ms_m, 4), "per_instance_ms": round(ms_p, 4),
+                     "merged_tflops": round(flops / (ms_m * 1e-3) / 1e12, 3),
+                     "batching_speedup": round(ms_p / ms_m, 3)}
+        print(f"[bench] c4 {name}: merged {ms_m:.3f} ms, per-instance {ms_p:.3f} ms", file=sys.stderr, flush=True)
+    merged.apply_device(vin.device_ptr(), vout.device_ptr(), sad.FP64)
+    o64 = tl.from_dev(vout)["data"]
+    per = np.concatenate([tl.from_dev(p.apply(a, sad.FP64))["data"] for p, a in zip(plans, ins)])
+    merged.apply_device(vin.device_ptr(), vout.device_ptr(), sad.FP32)
+    o32 = tl.from_dev(vout)["data"]
+    line = {"workload": "C4: 64 U(1) complex XXZ+DM chains (L in {32,48,64,96}, seed 7), m=256, centre-bond "
+                        "H_eff*psi of all instances per step",
+            "instances": n_inst, "m": m, "lengths": {str(L): lengths.count(L) for L in (32, 48, 64, 96)},
+            "n_total_complex": int(n), "flops_per_step": flops, "stage_problems": [st["stage1_problems"], st["stage2_problems"]],
+            "launches_per_step": {"merged": 2, "per_instance": 2 * n_inst},
+            **res,
+            "merged_equals_per_instance_fp64": bool(np.array_equal(o64, per)),
+            "complex64_rel_err_vs_complex128": float(np.max(np.abs(o32 - o64)) / np.max(np.abs(o64)))}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import refbind as rb
+        if rb.available():
+            sample = list(range(0, n_inst, 16))
+            secs = 0.0
+            for i in sample:
+                d, il, ir = tl.reference_heff_setup(rb, wls[i], 1)
+                secs += d.time_heff(il, ir, tl.to_ref(wls[i]["psi"]), 3)
+            per_inst = secs / len(sample)
+            line["cpu_reference"] = {"seconds_per_instance_apply": per_inst, "threads": 1,
+                                     "sample": f"instances {sample}, 3 applications each (complex128, MKL sequential)",
+                                     "seconds_all_instances": per_inst * n_inst,
+                                     "gpu_merged_complex64_speedup": round(per_inst * n_inst /
+                                                                           (res["complex64"]["merged_ms"] * 1e-3), 1)}
+    return line
+
+
 def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6, check_every=(4,),
                   bf16=False):
     """Mixed vs FP64 sweep on the SU(2) J1-J2 cylinder (width x length, snake
@@ -393,9 +492,10 @@ def main():
                          "the others are timed as variants)")
     ap.add_argument("--gradient", default="none", metavar="L:D:WARM",
                     help="only run the coupling-gradient workload (e.g. 48:1024:128,256,512) and exit")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c1"],
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c1", "c4"],
                     help="c3: the metric (default); c2: L=100 D=512 grouped vs per-block launch; "
-                         "c1: U(1) L=32 m=64 four sweeps vs the reference")
+                         "c1: U(1) L=32 m=64 four sweeps vs the reference; "
+                         "c4: 64 U(1) complex chains at m=256, all centre bonds in one launch")
     ap.add_argument("--sweep", default="6x12:2048:128,256,512,1024",
                     help="WIDTHxLENGTH:D:warm-up Ds for the mixed vs FP64 sweep comparison ('none' skips it)")
     args = ap.parse_args()
@@ -422,6 +522,9 @@ def main():
         return
     if args.workload == "c1":
         print(json.dumps(c1_run(sad, args)))
+        return
+    if args.workload == "c4":
+        print(json.dumps(c4_run(sad, args)))
         return
     if args.gradient != "none":
         Ls, Ds, Ws = args.gradient.split(":")

@@ -141,6 +141,15 @@ int sadmrg_heff_apply_stage(sadmrg_heff* h, const void* d_in, void* d_out, int p
    the grouped launch, as the reference issues one GEMM per block.  No
    reference counterpart. */
 int sadmrg_heff_apply_unbatched(sadmrg_heff* h, const void* d_in, void* d_out, int precision);
+/* One plan over n independent plans (instances, e.g. many chains of
+   different lengths): their block tables merged, so each H_eff stage of
+   every instance runs in a single launch.  Apply it with
+   sadmrg_heff_apply_device on the instances' vectors stored back to back in
+   the order of parts (sadmrg_heff_stats reports the total length).  The
+   parts must share the context and scalar type and carry no penalty
+   projectors; they may be destroyed afterwards.  Tensor-level apply and
+   Lanczos reject a merged plan.  No reference counterpart. */
+int sadmrg_heff_merge(const sadmrg_heff* const* parts, int n, sadmrg_heff** out);
 int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops);
 /* info[8] = {psi blocks, stage-1 problems, stage-2 problems, stage-1 tiles,
    stage-2 tiles, stage-1 tile cfg, stage-2 tile cfg, output tiles with a

@@ -83,6 +83,7 @@ def lib():
         "sadmrg_heff_apply_unbatched": (i, [vp, vp, vp, i]),
         "sadmrg_heff_stage_flops": (i, [vp, i, dp]),
         "sadmrg_heff_stats": (i, [vp, dp, dp, llp, ip]),
+        "sadmrg_heff_merge": (i, [C.POINTER(vp), i, pp]),
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
         "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_svd_gram_min": (i, [i, ip]),
@@ -377,6 +378,18 @@ class Heff:
     def __del__(self):
         if getattr(self, "ptr", None) and _lib is not None:
             _lib.sadmrg_heff_destroy(self.ptr)
+
+    @classmethod
+    def merge(cls, parts):
+        """One plan over independent plans: every stage of every part in one
+        launch; apply_device on the parts' vectors back to back."""
+        arr = (C.c_void_p * len(parts))(*[p.ptr.value for p in parts])
+        q = C.c_void_p()
+        check(lib().sadmrg_heff_merge(arr, len(parts), C.byref(q)))
+        h = cls.__new__(cls)
+        h.ptr, h.ctx = q, parts[0].ctx
+        h._keep = ()
+        return h
 
     def apply(self, psi, precision=FP64):
         out = Tensor.null(psi.ctx, psi.sym, psi.cplx)

@@ -475,9 +475,26 @@ int sadmrg_heff_destroy(sadmrg_heff* h) {
   return guard([&] { delete h; });
 }
 
+int sadmrg_heff_merge(const sadmrg_heff* const* parts, int n, sadmrg_heff** out) {
+  return guard([&] {
+    need(parts, "parts");
+    need(out, "out");
+    if (n < 1) fail(kErrInvalid, "heff_merge: at least one plan");
+    std::vector<const HeffPlan*> ps;
+    for (int i = 0; i < n; ++i) {
+      need(parts[i], "parts[i]");
+      ps.push_back(parts[i]->p.get());
+    }
+    auto h = std::make_unique<sadmrg_heff>();
+    h->p = HeffPlan::merge(ps);
+    *out = h.release();
+  });
+}
+
 int sadmrg_heff_apply(sadmrg_heff* h, const sadmrg_tensor* in, sadmrg_tensor* out, int precision) {
   return guard([&] {
     need(h, "heff");
+    if (h->p->merged()) fail(kErrInvalid, "heff_apply: a merged plan takes device vectors (heff_apply_device)");
     need(in, "psi_in");
     need(out, "psi_out");
     const BTensor& L = h->p->layout();
@@ -603,6 +620,7 @@ int sadmrg_lanczos(sadmrg_heff* h, sadmrg_tensor* psi, int mlanc, int precision,
   return guard([&] {
     need(h, "heff");
     need(psi, "psi");
+    if (h->p->merged()) fail(kErrInvalid, "lanczos: a merged plan has no single wave function");
     const BTensor& L = h->p->layout();
     if (psi->t.st != L.st || psi->t.off != L.off) fail(kErrInvalid, "lanczos: psi layout differs from the plan");
     LanczosOpts o;

@@ -33,6 +33,28 @@ void GemmBatch::add_seg(int prob, const Operand& A, const Operand& B, int K, dou
   probs_[prob].segs.push_back(s);
 }
 
+void GemmBatch::absorb(const GemmBatch& o, const int64_t (&shift)[kMaxArenas]) {
+  require(!finalized_, "GemmBatch: absorb after finalize");
+  require(o.finalized_, "GemmBatch: absorb needs a finalized source");
+  constexpr int64_t mask = (int64_t(1) << kArenaShift) - 1;
+  auto move = [&](int64_t off) {
+    const int a = static_cast<int>(static_cast<uint64_t>(off) >> kArenaShift);
+    return arena_off(a, (off & mask) + shift[a]);
+  };
+  for (const GemmProb& q : o.hp_) {
+    P p{};
+    p.p = q;
+    p.p.c_off = move(q.c_off);
+    for (int s = q.seg_begin; s < q.seg_end; ++s) {
+      GemmSeg g = o.hs_[s];
+      g.a_off = move(g.a_off);
+      g.b_off = move(g.b_off);
+      p.segs.push_back(g);
+    }
+    probs_.push_back(std::move(p));
+  }
+}
+
 int64_t GemmBatch::nseg() const {
   if (finalized_) return static_cast<int64_t>(hs_.size());
   int64_t n = 0;
@@ -273,7 +295,8 @@ void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int c
   require(finalized_, "GemmBatch: run before finalize");
   if (hp_.empty()) return;
   if (cfg < 0) cfg = default_cfg(prec);
-  require(!(prec == GemmPrec::Z64) || complex_, "GemmBatch: complex precision on a real table");
+  const bool zprec = prec == GemmPrec::Z64 || prec == GemmPrec::Z32 || prec == GemmPrec::Z32_OUT64;
+  require(zprec == complex_, "GemmBatch: precision and table disagree on complex data");
   const TileSet& ts = tileset(ctx, cfg);
   if (ts.ntiles == 0) return;
   GemmLaunch L;
@@ -295,7 +318,8 @@ void GemmBatch::run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& a
   if (cfg < 0) cfg = default_cfg(prec);
   require(cfg >= 0 && cfg < kTileConfigs, "GemmBatch: bad tile configuration");
   require(!(complex_ && cfg != 0), "GemmBatch: complex tables use tile configuration 0");
-  require(!(prec == GemmPrec::Z64) || complex_, "GemmBatch: complex precision on a real table");
+  const bool zprec = prec == GemmPrec::Z64 || prec == GemmPrec::Z32 || prec == GemmPrec::Z32_OUT64;
+  require(zprec == complex_, "GemmBatch: precision and table disagree on complex data");
   ProbTiles& pt = pp_[cfg];
   if (!pt.built) {
     pt.built = true;

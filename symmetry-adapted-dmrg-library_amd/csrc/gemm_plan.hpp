@@ -38,6 +38,9 @@ class GemmBatch {
   // Algorithmic flops (real multiply-adds x2; complex x4) of the table.
   double flops(bool cplx) const;
 
+  // Appends every problem and segment of a finalized batch, each element
+  // offset moved by shift[arena] (independent tables merged into one launch).
+  void absorb(const GemmBatch& o, const int64_t (&shift)[kMaxArenas]);
   // complex: complex tables (configuration 0 only, no split reductions).
   void finalize(Context* ctx, bool complex);
   // cfg < 0: the default configuration for prec (default_cfg).

@@ -209,6 +209,66 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
   ctx->sync();
 }
 
+std::unique_ptr<HeffPlan> HeffPlan::merge(const std::vector<const HeffPlan*>& parts) {
+  require(!parts.empty(), "H_eff merge: no plans");
+  Context* ctx = parts[0]->ctx_;
+  const bool cx = parts[0]->cplx_;
+  const int w = cx ? 2 : 1;
+  std::unique_ptr<HeffPlan> m(new HeffPlan(ctx, cx));
+  m->merged_ = true;
+  int64_t pack = 0, scratch = 0, n = 0;
+  for (const HeffPlan* p : parts) {
+    require(p && p->ctx_ == ctx && p->cplx_ == cx, "H_eff merge: plans differ in context or scalar type");
+    require(p->projectors() == 0, "H_eff merge: plans with projectors are not merged");
+    int64_t shift[kMaxArenas] = {};
+    shift[kArIn] = shift[kArOut] = n;
+    shift[kArScratch] = scratch;
+    shift[kArPack] = pack;
+    m->g1_.absorb(p->g1_, shift);
+    m->g2_.absorb(p->g2_, shift);
+    n += p->n_;
+    pack += p->pack_elems_;
+    scratch += p->scratch_elems_;
+  }
+  m->n_ = n;
+  m->pack_elems_ = pack;
+  m->scratch_elems_ = scratch;
+  m->pack64_ = DevBuf(ctx, std::max<int64_t>(pack, 1) * 8 * w);
+  int64_t at = 0;
+  for (const HeffPlan* p : parts) {
+    if (p->pack_elems_)
+      SADMRG_HIP(hipMemcpyAsync(m->pack64_.as<double>() + at * w, p->pack64_.get(), p->pack_elems_ * 8 * w,
+                                hipMemcpyDeviceToDevice, ctx->stream()));
+    at += p->pack_elems_;
+  }
+  m->scratch64_ = DevBuf(ctx, std::max<int64_t>(scratch, 1) * 8 * w);
+  for (GemmBatch* g : {&m->g1_, &m->g2_}) {
+    g->limit(kArIn, n);
+    g->limit(kArOut, n);
+    g->limit(kArScratch, scratch);
+    g->limit(kArPack, pack);
+  }
+  m->g1_.finalize(ctx, cx);
+  m->g2_.finalize(ctx, cx);
+  HeffStats& st = m->stats_;
+  st.n = n;
+  st.flops = m->g1_.flops(cx) + m->g2_.flops(cx);
+  st.bytes64 = 8.0 * w * (2.0 * n + pack);
+  st.stage1_probs = m->g1_.nprob();
+  st.stage2_probs = m->g2_.nprob();
+  st.stage1_segs = m->g1_.nseg();
+  st.stage2_segs = m->g2_.nseg();
+  const GemmPrec p64 = cx ? GemmPrec::Z64 : GemmPrec::F64;
+  st.tile_cfg1 = m->g1_.default_cfg(p64);
+  st.tile_cfg2 = m->g2_.default_cfg(p64);
+  st.tiles1 = m->g1_.ntiles(ctx, st.tile_cfg1);
+  st.tiles2 = m->g2_.ntiles(ctx, st.tile_cfg2);
+  st.split_tiles = m->g1_.nsplit_tiles(ctx, st.tile_cfg1) + m->g2_.nsplit_tiles(ctx, st.tile_cfg2);
+  for (const HeffPlan* p : parts) st.nsec_in += p->stats_.nsec_in;
+  ctx->sync();
+  return m;
+}
+
 namespace {
 struct TuneKey {
   int prec, stage;
@@ -286,8 +346,32 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
   };
   ArenaTable at;
   std::memset(&at, 0, sizeof(at));
+  if (cplx_ && prec == Prec::FP32) {
+    // complex<float> operands, FP32 MFMA, complex<double> output
+    if (!pack32_.get()) {
+      pack32_ = DevBuf(ctx_, std::max<int64_t>(pack_elems_, 1) * 8);
+      scratch32_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 8);
+      in32_ = DevBuf(ctx_, std::max<int64_t>(n_, 1) * 8);
+      SADMRG_HIP(launch_f64_to_f32(2 * pack_elems_, pack64_.as<double>(), pack32_.as<float>(), ctx_->stream()));
+    }
+    if (stage != 2) SADMRG_HIP(launch_f64_to_f32(2 * n_, d_in, in32_.as<float>(), ctx_->stream()));
+    at.base[kArIn] = in32_.get();
+    at.base[kArOut] = d_out;
+    at.base[kArScratch] = scratch32_.get();
+    at.base[kArPack] = pack32_.get();
+    if (stage != 2) {
+      stats_.tile_cfg1 = 0;
+      exec(g1_, GemmPrec::Z32, at, 0);
+    }
+    if (stage != 1) {
+      stats_.tile_cfg2 = 0;
+      exec(g2_, GemmPrec::Z32_OUT64, at, 0);
+    }
+    if (stage == 0) apply_projectors(d_in, d_out);
+    return;
+  }
   if (prec == Prec::FP64 || cplx_) {
-    require(prec == Prec::FP64, "H_eff: the FP32 and BF16 paths are real-only");
+    require(prec == Prec::FP64, "H_eff: the BF16 path is real-only");
     at.base[kArIn] = d_in;
     at.base[kArOut] = d_out;
     at.base[kArScratch] = scratch64_.get();
@@ -350,6 +434,7 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
 }
 
 void HeffPlan::add_projector(const BTensor& phi, double weight) {
+  require(!merged_, "H_eff projector: merged plans have no single wave function");
   require(phi.st == layout_.st && phi.off == layout_.off && phi.nelem == n_ && phi.cplx == cplx_,
           "H_eff projector: phi must have the layout of the wave function");
   const int P = static_cast<int>(proj_w_.size()) + 1;

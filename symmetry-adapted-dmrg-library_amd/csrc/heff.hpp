@@ -56,6 +56,14 @@ class HeffPlan {
   HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTensor& hr, const std::vector<const BTensor*>& o1,
            const std::vector<const BTensor*>& o2, const BTensor& psi_layout);
 
+  // One plan over independent plans (instances of different sizes): their
+  // block tables merged into one stage-1 and one stage-2 table, so each
+  // stage of every instance runs in a single launch.  Vectors are the
+  // instances' vectors back to back, in the order of parts.  The parts must
+  // share the context and scalar type and carry no projectors; they may be
+  // destroyed afterwards (operators are copied).  No reference counterpart.
+  static std::unique_ptr<HeffPlan> merge(const std::vector<const HeffPlan*>& parts);
+
   // d_in / d_out: device arrays in the layout of psi_layout (n elements).
   // unbatched: one launch per output block of each stage instead of one
   // grouped launch (the per-block baseline).
@@ -69,6 +77,7 @@ class HeffPlan {
   // Evaluated in FP64 at both precisions.
   void add_projector(const BTensor& phi, double weight);
   int projectors() const { return static_cast<int>(proj_w_.size()); }
+  bool merged() const { return merged_; }
   double stage_flops(int stage) const;
   const HeffStats& stats() const { return stats_; }
   Context* ctx() const { return ctx_; }
@@ -79,6 +88,7 @@ class HeffPlan {
   const std::vector<uint8_t>& out_null() const { return out_null_; }
 
  private:
+  explicit HeffPlan(Context* ctx, bool cplx) : ctx_(ctx), cplx_(cplx) {}
   Context* ctx_;
   bool cplx_;
   int64_t n_ = 0;
@@ -93,6 +103,7 @@ class HeffPlan {
   // projector columns: phi_0..phi_{P-1}, then weight_j * phi_j (n elements each)
   DevBuf proj_, proj_dot_;
   std::vector<double> proj_w_;
+  bool merged_ = false;
   void apply_projectors(const double* d_in, double* d_out);
 };
 

@@ -475,9 +475,10 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
     }
 }
 
-// Complex grouped GEMM on interleaved (re, im) double data.  Four real
-// MFMA products per step; conjugation flags select the signs.
-template <int TM, int TN, int TK, int WM, int WN>
+// Complex grouped GEMM on interleaved (re, im) data, FP64 or FP32 operands
+// (FP32 accumulation, FP32 or FP64 output).  Four real MFMA products per
+// step; conjugation flags select the signs.
+template <class T, class TOut, int TM, int TN, int TK, int WM, int WN>
 __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __restrict__ tiles,
                                                             const GemmProb* __restrict__ probs,
                                                             const GemmSeg* __restrict__ segs,
@@ -487,8 +488,9 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
   constexpr int PAD = 2;
   constexpr int A_ELEMS = TM * TK / 256;
   constexpr int B_ELEMS = TN * TK / 256;
-  __shared__ double sAr[TK][TM + PAD], sAi[TK][TM + PAD];
-  __shared__ double sBr[TK][TN + PAD], sBi[TK][TN + PAD];
+  typedef typename Acc4<T>::type acc_t;
+  __shared__ T sAr[TK][TM + PAD], sAi[TK][TM + PAD];
+  __shared__ T sBr[TK][TN + PAD], sBi[TK][TN + PAD];
 
   const GemmTile tile = tiles[tile_base + blockIdx.x];
   const GemmProb prob = probs[tile.prob];
@@ -498,19 +500,20 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
   const int wm = wave % WM;
   const int wn = wave / WM;
 
-  d4_t accr[FM][FN], acci[FM][FN];
+  acc_t accr[FM][FN], acci[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) { accr[i][j] = d4_t{0, 0, 0, 0}; acci[i][j] = d4_t{0, 0, 0, 0}; }
+    for (int j = 0; j < FN; ++j) { accr[i][j] = acc_t{0, 0, 0, 0}; acci[i][j] = acc_t{0, 0, 0, 0}; }
 
   for (int si = tile.seg_lo; si < tile.seg_hi; ++si) {
     const GemmSeg sg = segs[si];
-    const double* A = static_cast<const double*>(arena_ptr(arenas, sg.a_off, 16));
-    const double* B = static_cast<const double*>(arena_ptr(arenas, sg.b_off, 16));
+    const T* A = static_cast<const T*>(arena_ptr(arenas, sg.a_off, 2 * sizeof(T)));
+    const T* B = static_cast<const T*>(arena_ptr(arenas, sg.b_off, 2 * sizeof(T)));
     const bool ta = sg.flags & kTransA, tb = sg.flags & kTransB;
-    const double ca = (sg.flags & kConjA) ? -1.0 : 1.0;
-    const double cb = (sg.flags & kConjB) ? -1.0 : 1.0;
+    const T ca = (sg.flags & kConjA) ? T(-1) : T(1);
+    const T cb = (sg.flags & kConjB) ? T(-1) : T(1);
+    const T sr = static_cast<T>(sg.scale), sim = static_cast<T>(sg.scale_im);
     for (int k0 = 0; k0 < sg.K; k0 += TK) {
       __syncthreads();
 #pragma unroll
@@ -519,14 +522,14 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
         const int idx = tid + e * 256;
         if (!ta) { m = idx % TM; k = idx / TM; } else { k = idx % TK; m = idx / TK; }
         const int gm = tile.m0 + m, gk = k0 + k;
-        double xr = 0, xi = 0;
+        T xr = 0, xi = 0;
         if (gm < prob.M && gk < sg.K) {
           const int64_t o = ta ? (gk + int64_t(gm) * sg.lda) : (gm + int64_t(gk) * sg.lda);
           xr = A[2 * o];
           xi = ca * A[2 * o + 1];
         }
-        sAr[k][m] = xr * sg.scale - xi * sg.scale_im;
-        sAi[k][m] = xr * sg.scale_im + xi * sg.scale;
+        sAr[k][m] = xr * sr - xi * sim;
+        sAi[k][m] = xr * sim + xi * sr;
       }
 #pragma unroll
       for (int e = 0; e < B_ELEMS; ++e) {
@@ -534,7 +537,7 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
         const int idx = tid + e * 256;
         if (tb) { n = idx % TN; k = idx / TN; } else { k = idx % TK; n = idx / TK; }
         const int gn = tile.n0 + n, gk = k0 + k;
-        double xr = 0, xi = 0;
+        T xr = 0, xi = 0;
         if (gn < prob.N && gk < sg.K) {
           const int64_t o = tb ? (gn + int64_t(gk) * sg.ldb) : (gk + int64_t(gn) * sg.ldb);
           xr = B[2 * o];
@@ -547,7 +550,7 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 #pragma unroll
       for (int kk = 0; kk < TK; kk += 4) {
         const int kr = kk + (lane >> 4);
-        double ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
+        T ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
           ar[i] = sAr[kr][(wm * FM + i) * 16 + (lane & 15)];
@@ -572,7 +575,7 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
     }
   }
 
-  double* C = static_cast<double*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, 16)));
+  TOut* C = static_cast<TOut*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, 2 * sizeof(TOut))));
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -581,11 +584,12 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
       if (gn >= prob.N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = tile.m0 + (wm * FM + i) * 16 + acc_row<double>(lane >> 4, r);
+        const int gm = tile.m0 + (wm * FM + i) * 16 + acc_row<T>(lane >> 4, r);
         if (gm >= prob.M) continue;
-        double* p = C + 2 * (gm + int64_t(gn) * prob.ldc);
-        if (prob.accumulate) { p[0] += accr[i][j][r]; p[1] += acci[i][j][r]; }
-        else { p[0] = accr[i][j][r]; p[1] = acci[i][j][r]; }
+        TOut* p = C + 2 * (gm + int64_t(gn) * prob.ldc);
+        const TOut vr = static_cast<TOut>(accr[i][j][r]), vi = static_cast<TOut>(acci[i][j][r]);
+        if (prob.accumulate) { p[0] += vr; p[1] += vi; }
+        else { p[0] = vr; p[1] = vi; }
       }
     }
 }
@@ -630,10 +634,19 @@ hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L,
       if (tile_cfg == 1) return launch_real<bf16_t, double, 128, 128, 64, 1, 1>(L, st);
       if (tile_cfg == 2) return launch_real<bf16_t, double, 128, 64, 64, 2, 1>(L, st);
       return launch_real<bf16_t, double, 64, 64, 64, 2, 1>(L, st);
-    case GemmPrec::Z64: {
+    case GemmPrec::Z64:
+    case GemmPrec::Z32:
+    case GemmPrec::Z32_OUT64: {
       if (L.nreduce > 0 || tile_cfg != 0) return hipErrorInvalidValue;  // no split or large tiles for complex
-      hipLaunchKernelGGL((grouped_zgemm_kernel<64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st, L.tiles,
-                         L.probs, L.segs, L.arenas, 0);
+      if (prec == GemmPrec::Z64)
+        hipLaunchKernelGGL((grouped_zgemm_kernel<double, double, 64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0,
+                           st, L.tiles, L.probs, L.segs, L.arenas, 0);
+      else if (prec == GemmPrec::Z32)
+        hipLaunchKernelGGL((grouped_zgemm_kernel<float, float, 64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0,
+                           st, L.tiles, L.probs, L.segs, L.arenas, 0);
+      else
+        hipLaunchKernelGGL((grouped_zgemm_kernel<float, double, 64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0,
+                           st, L.tiles, L.probs, L.segs, L.arenas, 0);
       return hipGetLastError();
     }
   }

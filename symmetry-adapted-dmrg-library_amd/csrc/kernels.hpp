@@ -13,6 +13,8 @@ enum class GemmPrec : int {
   Z64 = 3,        // complex<double> operands (interleaved), 4 FP64 MFMA products
   BF16 = 4,       // bfloat16 operands, BF16 MFMA with FP32 accumulation, bfloat16 output
   BF16_OUT64 = 5, // bfloat16 operands, BF16 MFMA with FP32 accumulation, double output
+  Z32 = 6,        // complex<float> operands, 4 FP32 MFMA products, complex<float> output
+  Z32_OUT64 = 7,  // complex<float> operands, 4 FP32 MFMA products, complex<double> output
 };
 
 // Device tables of one grouped launch (see grouped_gemm.hpp).  With split

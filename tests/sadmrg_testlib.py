@@ -384,6 +384,76 @@ def build_workload(D=2048, n_op=12, sigma=1.8, seed=1):
     return dict(psi=psi, hl=hl, hr=hr, o1=o1, o2=o2, info=info)
 
 
+def u1_block_dims(m, sigma):
+    """Sector dimensions of a U(1) block bond with m states: labels 2S^z in
+    steps of 2 around 0, dims proportional to exp(-q^2 / (2 sigma^2))."""
+    qs = list(range(-16, 17, 2))
+    w = np.array([np.exp(-q * q / (2.0 * sigma ** 2)) for q in qs])
+    d = np.round(m * w / w.sum()).astype(int)
+    keep = d >= 1
+    return [q for q, k in zip(qs, keep) if k], [int(x) for x, k in zip(d, keep) if k]
+
+
+def u1_fused(qs, ds):
+    """fuse(block bond, spin-1/2 site with 2S^z = -1, +1) sector dimensions."""
+    out = {}
+    for q, d in zip(qs, ds):
+        for s in (-1, 1):
+            out[q + s] = out.get(q + s, 0) + d
+    ks = sorted(out)
+    return ks, [out[k] for k in ks]
+
+
+def build_workload_u1(m=256, length=32, seed=7, cplx=1, sigma=None):
+    """Synthetic centre-bond H_eff operands of a U(1) spin-1/2 chain of
+    `length` sites with m kept states: Gaussian 2S^z histogram on the block
+    bond (width growing with log length), fused with the site on each side;
+    one bond term across the centre (nearest neighbours, channels 2S^z
+    transfer -2, 0, +2); Hermitian block Hamiltonians; Gaussian entries
+    (complex for cplx=1)."""
+    import sys as _s
+    tl = _s.modules[__name__]
+    rng = np.random.default_rng(seed)
+    if sigma is None:
+        sigma = 2.0 * (1.0 + 0.5 * np.log2(length / 16.0))
+    qs, ds = u1_block_dims(m, sigma)
+    fq, fd = u1_fused(qs, ds)
+    b0 = (1, fq, fd)
+    b1 = (-1, fq, fd)
+    # the channel legs of the two bond-term operators point opposite ways
+    # (U(1) has no CGC to carry the coupling; block_tensor_inst.cpp:2180-2194)
+    ch1 = (-1, [-2, 0, 2], [1, 1, 1])
+    ch2 = (1, [-2, 0, 2], [1, 1, 1])
+    psi = tl.random_tensor(rng, 0, cplx, 0, [b0, b1])
+    psi["data"] /= np.linalg.norm(psi["data"])
+    scale = 1.0 / np.sqrt(max(fd))
+    hl = tl.random_tensor(rng, 0, cplx, 0, [(-1, fq, fd), b0], scale=scale)
+    hr = tl.random_tensor(rng, 0, cplx, 0, [(1, fq, fd), b1], scale=scale)
+    w = 2 if cplx else 1
+    for h in (hl, hr):  # Hermitian diagonal blocks: (A + A^H) / 2
+        off = 0
+        _, _, bonds = tl.parse_header(h["header"])
+        for b in range(tl.nten(bonds)):
+            if not h["present"][b]:
+                continue
+            dims = tl.block_dims(bonds, b)
+            n = dims[0] * dims[1]
+            raw = h["data"][w * off:w * (off + n)]
+            mat = (raw[0::2] + 1j * raw[1::2] if cplx else raw).reshape(dims, order="F")
+            mat = 0.5 * (mat + mat.conj().T)
+            flat = mat.ravel(order="F")
+            if cplx:
+                raw[0::2], raw[1::2] = flat.real, flat.imag
+            else:
+                raw[:] = flat
+            off += n
+    o1 = [tl.random_tensor(rng, 0, cplx, 0, [(-1, fq, fd), ch1, b0], scale=scale)]
+    o2 = [tl.random_tensor(rng, 0, cplx, 0, [(1, fq, fd), ch2, b1], scale=scale)]
+    info = {"sym": 0, "cplx": cplx, "two_sz_block": qs, "dims_block": ds, "two_sz_fused": fq, "dims_fused": fd,
+            "n": int(sum(d * d for d in fd)), "n_op": 1, "D": m, "length": length, "sigma": float(sigma)}
+    return dict(psi=psi, hl=hl, hr=hr, o1=o1, o2=o2, info=info)
+
+
 def reference_heff_setup(rb, wl, psize):
     """A reference DMRG object whose step (n_op-1, n_op) applies the workload's
     H_eff: left site j couples to right site n_op+j with coupling 1, so
@@ -392,7 +462,7 @@ def reference_heff_setup(rb, wl, psize):
     n_op = wl["info"]["n_op"]
     rb.init(physpn=1, max_angm=20, max_dcut=wl["info"]["D"], psize=psize)
     ns = max(2 * n_op, 4)
-    d = rb.RefDMRG(1, 0, 1, ns, wl["info"]["D"], 1, 0)
+    d = rb.RefDMRG(wl["info"].get("sym", 1), wl["info"].get("cplx", 0), 1, ns, wl["info"]["D"], 1, 0)
     hmap = np.zeros((ns, ns), dtype=np.int32)
     for j in range(n_op):
         hmap[j, n_op + j] = hmap[n_op + j, j] = 1

@@ -1,0 +1,144 @@
+"""Instances batched into one launch (HeffPlan::merge): the block tables of
+independent H_eff plans -- chains of different lengths, here with U(1)
+complex operands as in the XXZ + Dzyaloshinskii-Moriya batch -- merged so
+that each stage of every instance runs in a single grouped launch.  No
+reference counterpart: the reference applies one H_eff at a time
+(dmrg_inst.cpp:283-371), which is what each instance is checked against.
+"""
+import numpy as np
+import pytest
+
+import sadmrg_testlib as tl
+
+
+def _instances(n, m, seed=7, cplx=1):
+    rng = np.random.default_rng(seed)
+    lengths = [int(x) for x in rng.choice([32, 48, 64, 96], size=n)]
+    return [tl.build_workload_u1(m=m, length=L, seed=seed + 1 + i, cplx=cplx) for i, L in enumerate(lengths)]
+
+
+def test_u1_workload_structure():
+    """The synthetic U(1) centre bond: psi holds exactly the S^z = 0 blocks
+    (equal fused labels on both legs), its length is the sum of squared
+    fused dimensions, the block Hamiltonians are Hermitian, and wider
+    chains give wider 2S^z histograms."""
+    wl = tl.build_workload_u1(m=128, length=48, seed=3)
+    _, _, bonds = tl.parse_header(wl["psi"]["header"])
+    fq, fd = wl["info"]["two_sz_fused"], wl["info"]["dims_fused"]
+    present = [tl.block_labels(bonds, b)[0] for b in range(tl.nten(bonds)) if wl["psi"]["present"][b]]
+    assert sorted(p[0] for p in present) == sorted(fq) and all(p[0] == p[1] for p in present)
+    assert wl["psi"]["data"].size == 2 * sum(d * d for d in fd)
+    assert sum(wl["info"]["dims_block"]) in range(120, 137)
+    blocks = tl.blocks(wl["hl"])
+    for v in blocks.values():
+        assert np.allclose(v, v.conj().T)
+    s32 = tl.build_workload_u1(m=128, length=32, seed=3)["info"]["sigma"]
+    s96 = tl.build_workload_u1(m=128, length=96, seed=3)["info"]["sigma"]
+    assert s96 > s32
+
+
+def _plans(ctx, sad, wls):
+    plans, psis = [], []
+    for wl in wls:
+        psi = tl.to_dev(ctx, wl["psi"])
+        o1 = [tl.to_dev(ctx, t) for t in wl["o1"]]
+        o2 = [tl.to_dev(ctx, t) for t in wl["o2"]]
+        plans.append(sad.Heff(ctx, None, tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"]), o1, o2, psi))
+        psis.append(psi)
+    return plans, psis
+
+
+def _concat(ctx, sad, wls):
+    """All instances' psi back to back as one device tensor (a flat U(1)
+    block of the total length) and the offsets of each instance."""
+    data = np.concatenate([wl["psi"]["data"] for wl in wls])
+    n = data.size // 2
+    t = tl.to_dev(ctx, dict(sym=0, cplx=1, header=tl.make_header(0, [(1, [0], [n]), (-1, [0], [1])]),
+                            present=np.ones(1, np.uint8), data=data))
+    return t, n
+
+
+def _flat(t):
+    return tl.from_dev(t)["data"]
+
+
+@pytest.mark.gpu
+def test_merged_plan_equals_individual_plans(ctx, sad):
+    """One merged launch per stage gives, instance by instance, the output of
+    that instance's own plan bit for bit (FP64, complex: every output tile is
+    reduced by one workgroup in segment order either way); the complex64
+    operand path stays within 1e-5 of FP64 and differs from it."""
+    wls = _instances(6, 64)
+    plans, psis = _plans(ctx, sad, wls)
+    merged = sad.Heff.merge(plans)
+    assert merged.stats()["n"] == sum(wl["psi"]["data"].size // 2 for wl in wls)
+    vin, n = _concat(ctx, sad, wls)
+    out = vin.copy()
+    merged.apply_device(vin.device_ptr(), out.device_ptr(), sad.FP64)
+    got = _flat(out)
+    off = 0
+    for plan, psi, wl in zip(plans, psis, wls):
+        ref = tl.from_dev(plan.apply(psi, sad.FP64))["data"]
+        k = ref.size
+        assert np.array_equal(got[off:off + k], ref)
+        off += k
+    out32 = vin.copy()
+    merged.apply_device(vin.device_ptr(), out32.device_ptr(), sad.FP32)
+    g32 = _flat(out32)
+    rel = np.max(np.abs(g32 - got)) / np.max(np.abs(got))
+    assert 1e-9 < rel <= 1e-5, rel
+    with pytest.raises(sad.SadmrgError, match="merged"):
+        merged.apply(psis[0])
+
+
+@pytest.mark.gpu
+def test_merged_plan_matches_reference_per_instance(ctx, sad):
+    """Each instance's slice of the merged output equals the reference's
+    hamiltonian_vector_multiplication_idmrg (DMRG<complex<double>, U1>) on
+    the same tensors (1e-12)."""
+    import refbind as rb
+    if not rb.available():
+        pytest.skip("oracle/_ref not built")
+    wls = _instances(3, 48, seed=11)
+    plans, _ = _plans(ctx, sad, wls)
+    merged = sad.Heff.merge(plans)
+    vin, n = _concat(ctx, sad, wls)
+    out = vin.copy()
+    merged.apply_device(vin.device_ptr(), out.device_ptr(), sad.FP64)
+    got = _flat(out)
+    off = 0
+    for wl in wls:
+        d, il, ir = tl.reference_heff_setup(rb, wl, 1)
+        ref = tl.from_ref(d.heff(il, ir, tl.to_ref(wl["psi"])))
+        mine = dict(wl["psi"])
+        mine["data"] = got[off:off + wl["psi"]["data"].size]
+        rel, _ = tl.compare(mine, ref)
+        assert rel <= 1e-12, rel
+        off += wl["psi"]["data"].size
+
+
+@pytest.mark.gpu
+def test_merged_real_plans(ctx, sad):
+    """Real SU(2) centre bonds of different sizes merged: each slice equals
+    its own plan's output to 1e-13 (the merged table may split long
+    reductions differently)."""
+    wls = [tl.build_workload(D=d, n_op=2, seed=s) for d, s in ((96, 1), (160, 2), (64, 3))]
+    plans, psis = [], []
+    for wl in wls:
+        psi = tl.to_dev(ctx, wl["psi"], 1)
+        o1 = [tl.to_dev(ctx, t) for t in wl["o1"]]
+        o2 = [tl.to_dev(ctx, t) for t in wl["o2"]]
+        plans.append(sad.Heff(ctx, sad.Cgc(1), tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"]), o1, o2, psi))
+        psis.append(psi)
+    merged = sad.Heff.merge(plans)
+    data = np.concatenate([wl["psi"]["data"] for wl in wls])
+    vin = tl.to_dev(ctx, dict(sym=0, cplx=0, header=tl.make_header(0, [(1, [0], [data.size]), (-1, [0], [1])]),
+                              present=np.ones(1, np.uint8), data=data))
+    out = vin.copy()
+    merged.apply_device(vin.device_ptr(), out.device_ptr(), sad.FP64)
+    got = _flat(out)
+    off = 0
+    for plan, psi, wl in zip(plans, psis, wls):
+        ref = tl.from_dev(plan.apply(psi, sad.FP64))["data"]
+        assert np.max(np.abs(got[off:off + ref.size] - ref)) <= 1e-13 * np.max(np.abs(ref))
+        off += ref.size
