@@ -337,6 +337,46 @@ def c4_run(sad, args):
     return line
 
 
+def env_run(sad, args):
+    """Environment updates at the C3 centre bond (SU(2), block bond of D
+    multiplets from the rotor histogram, spin-1/2 site), FP64: the
+    enlarged-basis operator_transformation (identity fusion of block and
+    site: a block scatter, memory bound) and the truncated-basis one (a
+    random site tensor keeping half of each fused sector: grouped GEMMs).
+    Milliseconds per update from the host (table building and
+    synchronisation included) and algorithmic bytes of the scatter."""
+    import sadmrg_testlib as tl
+    ctx = sad.Context(0)
+    rng = np.random.default_rng(5)
+    js, ds = tl.rotor_dims(args.D, 1.8)
+    b = (1, js, ds)
+    site = (1, [1], [1])
+    fj, fd = tl.fused(js, ds)
+    op = tl.to_dev(ctx, tl.random_tensor(rng, 1, 0, 0, [(-1, js, ds), (-1, [2], [1]), b]))
+    u1 = sad.fuse(ctx, sad.SU2, 0, b, site)
+    uu = tl.to_dev(ctx, tl.random_tensor(rng, 1, 0, 0, [b, site, (-1, fj, [max(1, d // 2) for d in fd])]))
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = fn()
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, out
+
+    cgc = sad.Cgc(1)
+    ms_s, out_s = timed(lambda: sad.env("operator_transformation", u1, u1, op, flag=0, cgc=cgc))
+    ms_t, out_t = timed(lambda: sad.env("operator_transformation", uu, uu, op, flag=0, cgc=cgc))
+    op_b, out_b = op.nelem() * 8, out_s.nelem() * 8
+    return {"workload": f"environment updates at the C3 centre bond, SU(2) block bond D={args.D} multiplets, FP64",
+            "enlarged_basis_scatter": {"ms": round(ms_s, 4), "operator_bytes": op_b, "output_bytes": out_b,
+                                       "algorithmic_GBps": round((op_b + out_b) / (ms_s * 1e-3) / 1e9, 1),
+                                       "note": "host time per update: table build, scatter launch, synchronisation"},
+            "truncated_basis_gemm": {"ms": round(ms_t, 4), "output_bytes": out_t.nelem() * 8}}
+
+
 def sweep_compare(sad, ctx, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6, check_every=(4,),
                   bf16=False):
     """Mixed vs FP64 sweep on the SU(2) J1-J2 cylinder (width x length, snake
@@ -487,15 +527,19 @@ def main():
                     help="FP64 residual tolerance of the mixed path (relative to |E|; 0: library default)")
     ap.add_argument("--refine-krylov", type=int, default=3, help="FP32 Krylov steps per refinement restart")
     ap.add_argument("--no-bf16-sweep", action="store_true", help="skip the BF16 ablation sweep")
+    ap.add_argument("--lanczos-graph", default="on", choices=["on", "off"],
+                    help="replay Lanczos steps from HIP graphs (off: the same kernels launched from the host, "
+                         "for tools that do not follow graph launches)")
     ap.add_argument("--check-every", default="2",
                     help="mixed sweep: Lanczos convergence check interval (comma list: first is reported, "
                          "the others are timed as variants)")
     ap.add_argument("--gradient", default="none", metavar="L:D:WARM",
                     help="only run the coupling-gradient workload (e.g. 48:1024:128,256,512) and exit")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c1", "c4"],
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c1", "c4", "env"],
                     help="c3: the metric (default); c2: L=100 D=512 grouped vs per-block launch; "
                          "c1: U(1) L=32 m=64 four sweeps vs the reference; "
-                         "c4: 64 U(1) complex chains at m=256, all centre bonds in one launch")
+                         "c4: 64 U(1) complex chains at m=256, all centre bonds in one launch; "
+                         "env: environment updates at the C3 centre bond")
     ap.add_argument("--sweep", default="6x12:2048:128,256,512,1024",
                     help="WIDTHxLENGTH:D:warm-up Ds for the mixed vs FP64 sweep comparison ('none' skips it)")
     args = ap.parse_args()
@@ -506,6 +550,8 @@ def main():
 
     import sadmrg_testlib as tl
     sad = tl.load_pkg()
+    if args.lanczos_graph == "off":
+        sad.lanczos_mode(graph=False)
     if args.calibrate_copy > 0:
         n = args.calibrate_copy * (1 << 20) // 8
         ctx = sad.Context(0)
@@ -525,6 +571,9 @@ def main():
         return
     if args.workload == "c4":
         print(json.dumps(c4_run(sad, args)))
+        return
+    if args.workload == "env":
+        print(json.dumps(env_run(sad, args)))
         return
     if args.gradient != "none":
         Ls, Ds, Ws = args.gradient.split(":")
