@@ -101,13 +101,15 @@ _CFG_ARGS = {"fp32": {0: "64, 64, 32, 2, 2", 1: "128, 128, 32, 2, 1", 2: "128, 6
              "fp64": {0: "64, 64, 16, 2, 2", 1: "128, 128, 16, 1, 2", 2: "128, 64, 16, 2, 2"}}
 
 
-def gemm_kernel_name(prec, stage, cfg):
+def gemm_kernel_name(prec, stage, cfg, vec=True):
     """Kernel symbol of an H_eff stage (stage 1 writes FP32 intermediates on
-    the FP32 path, stage 2 writes the FP64 output)."""
+    the FP32 path, stage 2 writes the FP64 output); vec: the 16-byte staging
+    instantiation the H_eff plans run."""
+    tail = ", true" if vec else ", false"
     if prec == "fp64":
-        return f"sadmrg::grouped_gemm_kernel<double, double, {_CFG_ARGS['fp64'][cfg]}>"
+        return f"sadmrg::grouped_gemm_kernel<double, double, {_CFG_ARGS['fp64'][cfg]}{tail}>"
     tout = "float" if stage == 1 else "double"
-    return f"sadmrg::grouped_gemm_kernel<float, {tout}, {_CFG_ARGS['fp32'][cfg]}>"
+    return f"sadmrg::grouped_gemm_kernel<float, {tout}, {_CFG_ARGS['fp32'][cfg]}{tail}>"
 
 
 def measured_traffic(kernel):
@@ -600,7 +602,10 @@ def main():
           flush=True)
 
     res = {}
-    for name, prec in (("fp64", sad.FP64), ("fp32", sad.FP32), ("bf16", sad.BF16)):
+    legs = [("fp64", sad.FP64, True), ("fp32", sad.FP32, True), ("bf16", sad.BF16, True),
+            ("fp64_elem", sad.FP64, False), ("fp32_elem", sad.FP32, False)]
+    for name, prec, vec in legs:
+        sad.vec_staging(vec)
         for _ in range(args.warmup):
             plan.apply_device(d_in, d_out, prec)
         ctx.sync()
@@ -618,6 +623,7 @@ def main():
                      "tile_cfg": [stp["tile_cfg1"], stp["tile_cfg2"]]}
         print(f"[bench] {name}: {wall_ms:.3f} ms/apply {res[name]['tflops']:.2f} TFLOP/s "
               f"(stage1 {s1:.3f} ms, stage2 {s2:.3f} ms)", file=sys.stderr, flush=True)
+    sad.vec_staging(True)
 
     # accuracy of the mixed path against FP64 on the same input
     o64 = tl.from_dev(plan.apply(psi, sad.FP64))
@@ -673,6 +679,11 @@ def main():
         "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp32"].items()},
         "tile_configs": "0: 64x64, 1: 128x128, 2: 128x64 output tiles (autotuned per stage)",
         "mixed_vs_fp64_speedup": round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3),
+        "element_staging": {"fp32_ms": round(res["fp32_elem"]["ms_per_step"], 4),
+                            "fp64_ms": round(res["fp64_elem"]["ms_per_step"], 4),
+                            "fp32_stage_ms": [round(res["fp32_elem"]["stage1_ms"], 4),
+                                              round(res["fp32_elem"]["stage2_ms"], 4)],
+                            "note": "the same kernels staging one element per lane instead of 16-byte vectors"},
         "fp32_rel_err_vs_fp64": rel32,
         "bf16_ablation": {"ms_per_step": round(res["bf16"]["ms_per_step"], 4),
                           "tflops": round(res["bf16"]["tflops"], 3),

@@ -160,6 +160,11 @@ int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n
    entries, *entries receives the table size.  No reference counterpart (the
    reference has one BLAS path). */
 int sadmrg_autotune(int enable, int clear, int* entries);
+/* 16-byte operand staging in the H_eff grouped GEMMs (process-wide, on by
+   default; the H_eff plans lay their operands out for it): enable 0/1, -1
+   leaves the setting; *previous receives the old value.  Off stages single
+   elements (the comparison baseline).  No reference counterpart. */
+int sadmrg_gemm_vec_staging(int enable, int* previous);
 /* Truncation route of sweep-step SVDs (BlockTensor::svd with (p1, p2) =
    (1, 0) or (0, 1), block_tensor_inst.cpp:501-637, real data): sectors whose
    larger side has at least gram_min rows are decomposed through the Gram

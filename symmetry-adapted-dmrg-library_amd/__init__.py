@@ -89,6 +89,7 @@ def lib():
         "sadmrg_svd_gram_min": (i, [i, ip]),
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
         "sadmrg_lanczos_mode": (i, [i, i, ip, C.POINTER(C.c_longlong), i]),
+        "sadmrg_gemm_vec_staging": (i, [i, ip]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_create_exci": (i, [vp, i, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_exci": (i, [vp, ip]),
@@ -435,6 +436,14 @@ def autotune(enable=None, clear=False):
     n = C.c_int()
     check(lib().sadmrg_autotune(-1 if enable is None else int(bool(enable)), int(bool(clear)), C.byref(n)))
     return n.value
+
+
+def vec_staging(enable=None):
+    """16-byte operand staging of the H_eff GEMMs (process-wide); returns the
+    previous setting."""
+    prev = C.c_int()
+    check(lib().sadmrg_gemm_vec_staging(-1 if enable is None else int(bool(enable)), C.byref(prev)))
+    return bool(prev.value)
 
 
 def lanczos_mode(device_steps=None, graph=None, reset=False):

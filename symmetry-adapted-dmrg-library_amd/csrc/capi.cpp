@@ -593,6 +593,13 @@ int sadmrg_lanczos_mode(int device_steps, int graph, int* prev, long long* count
   });
 }
 
+int sadmrg_gemm_vec_staging(int enable, int* previous) {
+  return guard([&] {
+    if (previous) *previous = GemmBatch::vec_staging() ? 1 : 0;
+    if (enable >= 0) GemmBatch::set_vec_staging(enable != 0);
+  });
+}
+
 int sadmrg_svd_stats(long long* counts, double* seconds, int reset) {
   return guard([&] {
     SvdStats& st = svd_stats();

@@ -1,10 +1,18 @@
 // gemm_plan.cpp -- see gemm_plan.hpp.
 #include "gemm_plan.hpp"
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <utility>
 
 namespace sadmrg {
+
+namespace {
+std::atomic<bool> g_vec_staging{true};
+}
+
+void GemmBatch::set_vec_staging(bool on) { g_vec_staging = on; }
+bool GemmBatch::vec_staging() { return g_vec_staging; }
 
 int GemmBatch::add_problem(int c_arena, int64_t c_off, int64_t ldc, int M, int N, bool accumulate) {
   require(!finalized_, "GemmBatch: add after finalize");
@@ -183,6 +191,14 @@ void GemmBatch::finalize(Context* ctx, bool complex) {
       check(s.b_off, s.K, p.p.N, s.ldb, (s.flags & kTransB) != 0, "B");
     }
   }
+  if (vec_req_) {
+    constexpr int64_t mask = (int64_t(1) << kArenaShift) - 1;
+    for (const auto& p : probs_)
+      for (const auto& g : p.segs)
+        require((g.a_off & mask) % 4 == 0 && (g.b_off & mask) % 4 == 0 && g.lda % 4 == 0 && g.ldb % 4 == 0,
+                "GemmBatch: 16-byte staging requested for an operand that is not 4-element aligned");
+    vec_ = true;
+  }
   hp_.reserve(probs_.size());
   for (size_t i = 0; i < probs_.size(); ++i) {
     GemmProb p = probs_[i].p;
@@ -309,6 +325,8 @@ void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int c
   L.arenas = arenas;
   L.ntiles = ts.ntiles;
   L.nreduce = ts.nreduce;
+  L.vec = vec_ && g_vec_staging && !complex_ &&
+          (prec == GemmPrec::F64 || prec == GemmPrec::F32 || prec == GemmPrec::F32_OUT64);
   SADMRG_HIP(launch_grouped_gemm(prec, cfg, L, ctx->stream()));
 }
 
@@ -354,6 +372,8 @@ void GemmBatch::run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& a
     L.arenas = arenas;
     L.ntiles = n;
     L.nreduce = 0;
+    L.vec = vec_ && g_vec_staging && !complex_ &&
+            (prec == GemmPrec::F64 || prec == GemmPrec::F32 || prec == GemmPrec::F32_OUT64);
     SADMRG_HIP(launch_grouped_gemm(prec, cfg, L, ctx->stream()));
   }
 }

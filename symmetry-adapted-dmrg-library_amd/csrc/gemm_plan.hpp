@@ -41,6 +41,13 @@ class GemmBatch {
   // Appends every problem and segment of a finalized batch, each element
   // offset moved by shift[arena] (independent tables merged into one launch).
   void absorb(const GemmBatch& o, const int64_t (&shift)[kMaxArenas]);
+  // The caller guarantees the layout of 16-byte staging (every operand block
+  // starts on a multiple of 4 elements, leading dimensions are multiples of
+  // 4, and elements between an operand's contiguous extent and its leading
+  // dimension are zero); finalize() checks the alignment part and fails if
+  // it does not hold.  Real FP64 / FP32 runs then stage 16-byte vectors.
+  void request_vec() { vec_req_ = true; }
+  bool vec() const { return vec_; }
   // complex: complex tables (configuration 0 only, no split reductions).
   void finalize(Context* ctx, bool complex);
   // cfg < 0: the default configuration for prec (default_cfg).
@@ -65,6 +72,10 @@ class GemmBatch {
 
   // Workgroups the planner aims for before splitting reductions.
   static constexpr int kTargetWorkgroups = 2048;
+  // Process-wide switch for 16-byte staging of tables that allow it
+  // (default on; off: element staging, for comparison).
+  static void set_vec_staging(bool on);
+  static bool vec_staging();
 
  private:
   struct P {
@@ -95,6 +106,7 @@ class GemmBatch {
   TileSet sets_[kTileConfigs];
   bool complex_ = false;
   bool big_ = false;
+  bool vec_req_ = false, vec_ = false;
   uint64_t key_ = 0;
   bool finalized_ = false;
 };

@@ -1,5 +1,6 @@
 // heff.cpp -- see heff.hpp.
 #include "heff.hpp"
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -10,7 +11,28 @@ namespace sadmrg {
 
 namespace {
 enum { kArIn = 0, kArOut = 1, kArScratch = 2, kArPack = 3 };
+inline int64_t pad4(int64_t x) { return (x + 3) / 4 * 4; }
+// zero a device buffer of `bytes` (allocations are rounded to 256 bytes)
+void zero_bytes(Context* ctx, void* p, size_t bytes) {
+  if (bytes) SADMRG_HIP(launch_fill(static_cast<int64_t>((bytes + 7) / 8), 0.0, static_cast<double*>(p), ctx->stream()));
 }
+// column chunks of one block copy (one workgroup each)
+void add_copy(std::vector<ScatterTask>& v, int64_t src_off, int64_t lds, int64_t dst_off, int64_t ldd, int rows,
+              int cols) {
+  for (int c0 = 0; c0 < cols; c0 += 64) {
+    ScatterTask t{};
+    t.src_off = src_off + int64_t(c0) * lds;
+    t.dst_off = dst_off + int64_t(c0) * ldd;
+    t.lds = lds;
+    t.ldd = ldd;
+    t.rows = rows;
+    t.cols = std::min(64, cols - c0);
+    t.kind = 0;
+    t.scale = 1.0;
+    v.push_back(t);
+  }
+}
+}  // namespace
 
 HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTensor& hr,
                    const std::vector<const BTensor*>& o1, const std::vector<const BTensor*>& o2,
@@ -45,27 +67,74 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
     if (o1[t] && !o1[t]->is_null()) tid_of(o1[t]);
     if (o2[t] && !o2[t]->is_null()) tid_of(o2[t]);
   }
-  std::vector<int64_t> pbase(tens.size(), 0);
+  // padded pack: block b of tensor i at poff[i][b], leading dimension pld[i][b]
+  // (a block's rows are its elements over its last dimension)
+  std::vector<std::vector<int64_t>> poff(tens.size()), pld(tens.size());
   for (size_t i = 0; i < tens.size(); ++i) {
     require(tens[i]->cplx == cplx_ && tens[i]->sym() == sym, "H_eff: operator scalar/symmetry mismatch");
-    pbase[i] = pack_elems_;
-    pack_elems_ += tens[i]->nelem;
+    const BTensor& t = *tens[i];
+    poff[i].assign(t.nten(), -1);
+    pld[i].assign(t.nten(), 0);
+    int dd[4];
+    for (int b = 0; b < t.nten(); ++b) {
+      if (!t.present(b) || t.block_size(b) == 0) continue;
+      t.st.block_dims(b, dd);
+      const int64_t cols = dd[t.nbond() - 1];
+      const int64_t rows = t.block_size(b) / cols;
+      poff[i][b] = pack_elems_;
+      pld[i][b] = pad4(rows);
+      pack_elems_ += pld[i][b] * cols;
+    }
   }
   pack64_ = DevBuf(ctx, std::max<int64_t>(pack_elems_, 1) * 8 * w);
-  for (size_t i = 0; i < tens.size(); ++i)
-    if (tens[i]->nelem)
-      SADMRG_HIP(hipMemcpyAsync(pack64_.as<double>() + pbase[i] * w, tens[i]->dptr(), tens[i]->nelem * 8 * w,
-                                hipMemcpyDeviceToDevice, ctx->stream()));
+  zero_bytes(ctx, pack64_.get(), pack64_.bytes());
+  {
+    // one scatter launch per source tensor (each has its own base)
+    for (size_t i = 0; i < tens.size(); ++i) {
+      std::vector<ScatterTask> ti;
+      for (int b = 0; b < tens[i]->nten(); ++b) {
+        if (poff[i][b] < 0) continue;
+        int dd[4];
+        tens[i]->st.block_dims(b, dd);
+        const int64_t cols = dd[tens[i]->nbond() - 1];
+        add_copy(ti, tens[i]->off[b], tens[i]->block_size(b) / cols, poff[i][b], pld[i][b],
+                 static_cast<int>(tens[i]->block_size(b) / cols), static_cast<int>(cols));
+      }
+      if (ti.empty()) continue;
+      DevBuf tb(ctx, ti.size() * sizeof(ScatterTask));
+      SADMRG_HIP(hipMemcpyAsync(tb.get(), ti.data(), ti.size() * sizeof(ScatterTask), hipMemcpyHostToDevice,
+                                ctx->stream()));
+      SADMRG_HIP(launch_block_scatter(tb.as<ScatterTask>(), static_cast<int>(ti.size()), tens[i]->dptr(),
+                                      pack64_.as<double>(), w, ctx->stream()));
+      ctx->sync();  // tb and ti leave scope
+    }
+  }
   auto pk = [&](const BTensor* t, int b) -> Operand {
+    const int i = tid_of(t);
+    require(poff[i][b] >= 0, "H_eff: operator block missing from the pack");
     Operand o;
     o.arena = kArPack;
-    o.off = pbase[tid_of(t)] + t->off[b];
+    o.off = poff[i][b];
+    o.ld = pld[i][b];
     return o;
   };
+  // padded input copy: psi block b at in_off[b], leading dimension pad4(rows)
+  std::vector<int64_t> in_off(ntp, -1);
+  for (int b = 0; b < ntp; ++b) {
+    if (!psi.present(b)) continue;
+    int dd[2];
+    ps.block_dims(b, dd);
+    in_off[b] = in_elems_;
+    add_copy(in_tasks_, psi.off[b], dd[0], in_elems_, pad4(dd[0]), dd[0], dd[1]);
+    in_elems_ += pad4(dd[0]) * dd[1];
+  }
   auto pin = [&](int b) -> Operand {
+    int dd[2];
+    ps.block_dims(b, dd);
     Operand o;
     o.arena = kArIn;
-    o.off = psi.off[b];
+    o.off = in_off[b];
+    o.ld = pad4(dd[0]);
     return o;
   };
 
@@ -93,9 +162,7 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
       if (hb < 0 || !hl.present(hb)) continue;
       ps.block_dims(b, d);
       Operand A = pk(&hl, hb), B = pin(b);
-      A.ld = d[0];
       A.trans = true;
-      B.ld = d[0];
       g2_.add_seg(prob_of[b], A, B, d[0], 1.0);
       contributions[b]++;
     }
@@ -112,8 +179,6 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
       if (hb < 0 || !hr.present(hb)) continue;
       ps.block_dims(b, d);
       Operand A = pin(b), B = pk(&hr, hb);
-      A.ld = d[0];
-      B.ld = d[1];
       g2_.add_seg(prob_of[b], A, B, d[1], 1.0);
       contributions[b]++;
     }
@@ -156,15 +221,14 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
         }
         require(psi.present(m), "H_eff: output block missing from the psi layout");
         if (xoff < 0) {
-          // X = psi_k^T O1_j : (b) x (c * a')
+          // X = psi_k^T O1_j : (b) x (c * a'), leading dimension padded
+          require(d1[1] == 1, "H_eff: operator channel dimension must be 1");
           xoff = scratch_elems_;
           xm = d[1];
-          scratch_elems_ += int64_t(d[1]) * d1[1] * d1[2];
-          const int p = g1_.add_problem(kArScratch, xoff, d[1], d[1], d1[1] * d1[2], false);
+          scratch_elems_ += pad4(d[1]) * d1[1] * d1[2];
+          const int p = g1_.add_problem(kArScratch, xoff, pad4(d[1]), d[1], d1[1] * d1[2], false);
           Operand A = pin(k), B = pk(op1, j);
-          A.ld = d[0];
           A.trans = true;
-          B.ld = d[0];
           g1_.add_seg(p, A, B, d[0], 1.0);
         }
         op2->st.block_dims(l, d3);
@@ -173,9 +237,8 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
         Operand A, B = pk(op2, l);
         A.arena = kArScratch;
         A.off = xoff;
-        A.ld = kk;
+        A.ld = pad4(kk);
         A.trans = true;
-        B.ld = kk;
         g2_.add_seg(prob_of[m], A, B, kk, fac);
         contributions[m]++;
       }
@@ -185,14 +248,23 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
   for (int b = 0; b < ntp; ++b) out_null_[b] = psi.present(b) && contributions[b] == 0;
 
   for (GemmBatch* g : {&g1_, &g2_}) {
-    g->limit(kArIn, n_);
+    g->limit(kArIn, in_elems_);
     g->limit(kArOut, n_);
     g->limit(kArScratch, scratch_elems_);
     g->limit(kArPack, pack_elems_);
+    g->request_vec();  // the padded layout above
   }
   g1_.finalize(ctx, cplx_);
   g2_.finalize(ctx, cplx_);
   scratch64_ = DevBuf(ctx, std::max<int64_t>(scratch_elems_, 1) * 8 * w);
+  zero_bytes(ctx, scratch64_.get(), scratch64_.bytes());  // padding rows stay zero
+  in64_ = DevBuf(ctx, std::max<int64_t>(in_elems_, 1) * 8 * w);
+  zero_bytes(ctx, in64_.get(), in64_.bytes());
+  if (!in_tasks_.empty()) {
+    in_tasks_dev_ = DevBuf(ctx, in_tasks_.size() * sizeof(ScatterTask));
+    SADMRG_HIP(hipMemcpyAsync(in_tasks_dev_.get(), in_tasks_.data(), in_tasks_.size() * sizeof(ScatterTask),
+                              hipMemcpyHostToDevice, ctx->stream()));
+  }
   stats_.n = n_;
   stats_.flops = g1_.flops(cplx_) + g2_.flops(cplx_);
   stats_.bytes64 = 8.0 * w * (2.0 * n_ + pack_elems_);
@@ -221,12 +293,19 @@ std::unique_ptr<HeffPlan> HeffPlan::merge(const std::vector<const HeffPlan*>& pa
     require(p && p->ctx_ == ctx && p->cplx_ == cx, "H_eff merge: plans differ in context or scalar type");
     require(p->projectors() == 0, "H_eff merge: plans with projectors are not merged");
     int64_t shift[kMaxArenas] = {};
-    shift[kArIn] = shift[kArOut] = n;
+    shift[kArIn] = m->in_elems_;
+    shift[kArOut] = n;
     shift[kArScratch] = scratch;
     shift[kArPack] = pack;
     m->g1_.absorb(p->g1_, shift);
     m->g2_.absorb(p->g2_, shift);
+    for (ScatterTask t : p->in_tasks_) {
+      t.src_off += n;
+      t.dst_off += m->in_elems_;
+      m->in_tasks_.push_back(t);
+    }
     n += p->n_;
+    m->in_elems_ += p->in_elems_;
     pack += p->pack_elems_;
     scratch += p->scratch_elems_;
   }
@@ -242,8 +321,17 @@ std::unique_ptr<HeffPlan> HeffPlan::merge(const std::vector<const HeffPlan*>& pa
     at += p->pack_elems_;
   }
   m->scratch64_ = DevBuf(ctx, std::max<int64_t>(scratch, 1) * 8 * w);
+  zero_bytes(ctx, m->scratch64_.get(), m->scratch64_.bytes());
+  m->in64_ = DevBuf(ctx, std::max<int64_t>(m->in_elems_, 1) * 8 * w);
+  zero_bytes(ctx, m->in64_.get(), m->in64_.bytes());
+  if (!m->in_tasks_.empty()) {
+    m->in_tasks_dev_ = DevBuf(ctx, m->in_tasks_.size() * sizeof(ScatterTask));
+    SADMRG_HIP(hipMemcpyAsync(m->in_tasks_dev_.get(), m->in_tasks_.data(), m->in_tasks_.size() * sizeof(ScatterTask),
+                              hipMemcpyHostToDevice, ctx->stream()));
+  }
   for (GemmBatch* g : {&m->g1_, &m->g2_}) {
-    g->limit(kArIn, n);
+    g->request_vec();
+    g->limit(kArIn, m->in_elems_);
     g->limit(kArOut, n);
     g->limit(kArScratch, scratch);
     g->limit(kArPack, pack);
@@ -351,10 +439,12 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
     if (!pack32_.get()) {
       pack32_ = DevBuf(ctx_, std::max<int64_t>(pack_elems_, 1) * 8);
       scratch32_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 8);
-      in32_ = DevBuf(ctx_, std::max<int64_t>(n_, 1) * 8);
+      in32_ = DevBuf(ctx_, std::max<int64_t>(in_elems_, 1) * 8);
+      zero_bytes(ctx_, scratch32_.get(), scratch32_.bytes());
+      zero_bytes(ctx_, in32_.get(), in32_.bytes());
       SADMRG_HIP(launch_f64_to_f32(2 * pack_elems_, pack64_.as<double>(), pack32_.as<float>(), ctx_->stream()));
     }
-    if (stage != 2) SADMRG_HIP(launch_f64_to_f32(2 * n_, d_in, in32_.as<float>(), ctx_->stream()));
+    if (stage != 2) repack_input(d_in, in32_.get(), 1);
     at.base[kArIn] = in32_.get();
     at.base[kArOut] = d_out;
     at.base[kArScratch] = scratch32_.get();
@@ -372,7 +462,8 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
   }
   if (prec == Prec::FP64 || cplx_) {
     require(prec == Prec::FP64, "H_eff: the BF16 path is real-only");
-    at.base[kArIn] = d_in;
+    if (stage != 2) repack_input(d_in, in64_.get(), 0);
+    at.base[kArIn] = in64_.get();
     at.base[kArOut] = d_out;
     at.base[kArScratch] = scratch64_.get();
     at.base[kArPack] = pack64_.get();
@@ -392,10 +483,12 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
     if (!pack16_.get()) {
       pack16_ = DevBuf(ctx_, std::max<int64_t>(pack_elems_, 1) * 2);
       scratch16_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 2);
-      in16_ = DevBuf(ctx_, std::max<int64_t>(n_, 1) * 2);
+      in16_ = DevBuf(ctx_, std::max<int64_t>(in_elems_, 1) * 2);
+      zero_bytes(ctx_, scratch16_.get(), scratch16_.bytes());
+      zero_bytes(ctx_, in16_.get(), in16_.bytes());
       SADMRG_HIP(launch_f64_to_bf16(pack_elems_, pack64_.as<double>(), pack16_.as<uint16_t>(), ctx_->stream()));
     }
-    if (stage != 2) SADMRG_HIP(launch_f64_to_bf16(n_, d_in, in16_.as<uint16_t>(), ctx_->stream()));
+    if (stage != 2) repack_input(d_in, in16_.get(), 2);
     at.base[kArIn] = in16_.get();
     at.base[kArOut] = d_out;
     at.base[kArScratch] = scratch16_.get();
@@ -414,10 +507,12 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
   if (!pack32_.get()) {
     pack32_ = DevBuf(ctx_, std::max<int64_t>(pack_elems_, 1) * 4);
     scratch32_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 4);
-    in32_ = DevBuf(ctx_, std::max<int64_t>(n_, 1) * 4);
+    in32_ = DevBuf(ctx_, std::max<int64_t>(in_elems_, 1) * 4);
+    zero_bytes(ctx_, scratch32_.get(), scratch32_.bytes());
+    zero_bytes(ctx_, in32_.get(), in32_.bytes());
     SADMRG_HIP(launch_f64_to_f32(pack_elems_, pack64_.as<double>(), pack32_.as<float>(), ctx_->stream()));
   }
-  if (stage != 2) SADMRG_HIP(launch_f64_to_f32(n_, d_in, in32_.as<float>(), ctx_->stream()));
+  if (stage != 2) repack_input(d_in, in32_.get(), 1);
   at.base[kArIn] = in32_.get();
   at.base[kArOut] = d_out;
   at.base[kArScratch] = scratch32_.get();
@@ -431,6 +526,12 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
     exec(g2_, GemmPrec::F32_OUT64, at, stats_.tile_cfg2);
   }
   if (stage == 0) apply_projectors(d_in, d_out);
+}
+
+void HeffPlan::repack_input(const double* d_in, void* dst, int out_type) {
+  if (in_tasks_.empty()) return;
+  SADMRG_HIP(launch_repack(in_tasks_dev_.as<ScatterTask>(), static_cast<int>(in_tasks_.size()), d_in, dst,
+                           cplx_ ? 2 : 1, out_type, ctx_->stream()));
 }
 
 void HeffPlan::add_projector(const BTensor& phi, double weight) {

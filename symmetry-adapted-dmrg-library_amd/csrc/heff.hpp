@@ -93,9 +93,18 @@ class HeffPlan {
   bool cplx_;
   int64_t n_ = 0;
   BTensor layout_;  // structure / offsets only (no data)
-  DevBuf pack64_, pack32_, scratch64_, scratch32_, in32_;
+  // Operand layout: every block of the operator pack, of the stage-1
+  // intermediate and of the input copy is column-major with its leading
+  // dimension padded to a multiple of 4 elements and starts on a multiple of
+  // 4 elements (16 bytes in FP32); the padding is zero.  The input vector is
+  // repacked into that layout (and converted) once per application; the
+  // output is written in the caller's layout.
+  DevBuf pack64_, pack32_, scratch64_, scratch32_, in32_, in64_;
   DevBuf pack16_, scratch16_, in16_;  // BF16 operand copies (built on first BF16 use)
-  int64_t pack_elems_ = 0, scratch_elems_ = 0;
+  int64_t pack_elems_ = 0, scratch_elems_ = 0, in_elems_ = 0;
+  std::vector<ScatterTask> in_tasks_;  // input repack table (host copy)
+  DevBuf in_tasks_dev_;
+  void repack_input(const double* d_in, void* dst, int out_type);
   GemmBatch g1_, g2_;
   int cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& at);
   HeffStats stats_;

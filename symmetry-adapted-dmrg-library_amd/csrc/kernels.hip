@@ -13,6 +13,7 @@
 //    Gram-Schmidt (V^T w and w -= V c), precision conversion.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 #include "grouped_gemm.hpp"
 #include "kernels.hpp"
 
@@ -155,6 +156,82 @@ __device__ inline void cursor_advance(OperandCursor<T>& c, int TK) {
   c.kk -= TK;
 }
 
+// ---- 16-byte staging (tables whose operands are 16-byte aligned and
+// zero-padded to a 16-byte leading dimension) --------------------------------
+// A thread stages VW = 16 / sizeof(T) consecutive elements along the
+// operand's contiguous index per vector, NV vectors per K step.
+template <class T> struct Vec16;
+template <> struct Vec16<float> { typedef float4 type; };
+template <> struct Vec16<double> { typedef double2 type; };
+template <> struct Vec16<bf16_t> { typedef uint4 type; };  // BF16 tables stage elements (no VEC variant)
+
+template <class T, int R, int TK>
+struct VecTile {
+  static constexpr int VW = 16 / sizeof(T);
+  static constexpr int NV = R * TK / (256 * VW);  // vectors per thread and step
+  static_assert(NV >= 1 && (R / VW) <= 256 && 256 % (R / VW) == 0 && 256 % (TK / VW) == 0, "vector tile shape");
+};
+
+template <class T, int R, int TK>
+__device__ inline void vcursor_init(OperandCursor<T>& c, const T* base, int64_t ld, bool kr, int r0, int rows, int K,
+                                    int tid) {
+  constexpr int VW = VecTile<T, R, TK>::VW;
+  c.kr = kr;
+  c.safe = base;
+  if (!kr) {  // RK: VW consecutive rows at one k
+    constexpr int NR = R / VW;
+    const int rv = (tid % NR) * VW, kt = tid / NR;
+    c.p = base + r0 + rv + int64_t(kt) * ld;
+    c.epos = 256 / NR;
+    c.estride = int64_t(c.epos) * ld;
+    c.kstep = int64_t(TK) * ld;
+    c.rr = rows - r0 - rv;  // the vector is used if it starts inside the rows
+    c.kk = K - kt;
+  } else {    // KR: VW consecutive k of one row
+    constexpr int NK = TK / VW;
+    const int kv = (tid % NK) * VW, rt = tid / NK;
+    c.p = base + kv + int64_t(r0 + rt) * ld;
+    c.epos = 256 / NK;
+    c.estride = int64_t(c.epos) * ld;
+    c.kstep = TK;
+    c.rr = rows - r0 - rt;
+    c.kk = K - kv;          // elements past K in the vector are the zero padding
+  }
+}
+
+template <class T, int NV>
+__device__ inline void vcursor_load(const OperandCursor<T>& c, typename Vec16<T>::type (&v)[NV]) {
+  typedef typename Vec16<T>::type V;
+#pragma unroll
+  for (int e = 0; e < NV; ++e) {
+    const bool ok = c.kr ? (c.kk > 0 && e * c.epos < c.rr) : (c.rr > 0 && e * c.epos < c.kk);
+    const V x = *reinterpret_cast<const V*>(ok ? c.p + e * c.estride : c.safe);
+    v[e] = ok ? x : V{};
+  }
+}
+
+template <class T, int R, int TK>
+__device__ inline void vtile_store(T* lds, bool kr, const typename Vec16<T>::type (&v)[VecTile<T, R, TK>::NV],
+                                   typename ScaleT<T>::type scale, int tid) {
+  using OT = OperandTile<T, R, TK>;
+  typedef typename Vec16<T>::type V;
+  constexpr int VW = VecTile<T, R, TK>::VW;
+  constexpr int NV = VecTile<T, R, TK>::NV;
+  if (!kr) {
+    constexpr int NR = R / VW;
+    const int rv = (tid % NR) * VW, kt = tid / NR;
+#pragma unroll
+    for (int e = 0; e < NV; ++e)
+      *reinterpret_cast<V*>(&lds[(kt + e * (256 / NR)) * OT::LR + rv]) = v[e] * scale;
+  } else {
+    constexpr int NK = TK / VW;
+    const int kv = (tid % NK) * VW, rt = tid / NK;
+#pragma unroll
+    for (int e = 0; e < NV; ++e)
+      *reinterpret_cast<V*>(&lds[OT::kr_off(rt + e * (256 / NK), kv)]) = v[e] * scale;
+  }
+}
+
 template <class T, int R, int TK>
 __device__ inline void tile_store(T* lds, bool kr, const T (&v)[OperandTile<T, R, TK>::E],
                                   typename ScaleT<T>::type scale, int tid) {
@@ -170,11 +247,8 @@ __device__ inline void tile_store(T* lds, bool kr, const T (&v)[OperandTile<T, R
   }
 }
 
-// 16-byte view of LDS for the KR fragment reads (rows of LK elements, LK a
-// multiple of 16 bytes, so a 4-slot group never straddles a 16-byte word).
-template <class T> struct Vec16;
-template <> struct Vec16<float> { typedef float4 type; };
-template <> struct Vec16<double> { typedef double2 type; };
+// KR fragment reads use the 16-byte view Vec16 of LDS (rows of LK elements,
+// LK a multiple of 16 bytes, so a 4-slot group never straddles a 16-byte word).
 
 // NF = FragW<T>::NF consecutive k-slot values (j = j0 .. j0+NF-1) of
 // fragment row `row` in chunk kc.
@@ -253,7 +327,7 @@ __device__ inline void mfma_step(const T* sa, const T* sb, Acc (&acc)[FM][FN], i
 // per CU, 256 registers per lane; 1: one workgroup, 512 registers).
 // PF: K steps that global loads run ahead of the LDS stage (1 or 2; 2 holds
 // two register sets of staged operands).
-template <class T, class TOut, int TM, int TN, int TK, int WPE, int PF>
+template <class T, class TOut, int TM, int TN, int TK, int WPE, int PF, bool VEC = false>
 __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
                                                               const GemmProb* __restrict__ probs,
                                                               const GemmSeg* __restrict__ segs,
@@ -299,8 +373,13 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
     const T* B = static_cast<const T*>(arena_ptr(arenas, sg.b_off, sizeof(T)));
     lkra = (sg.flags & kTransA) != 0;
     lkrb = (sg.flags & kTransB) == 0;
-    cursor_init<T, TM, TK>(ca, A, sg.lda, lkra, tile.m0, prob.M, sg.K, tid);
-    cursor_init<T, TN, TK>(cb, B, sg.ldb, lkrb, tile.n0, prob.N, sg.K, tid);
+    if constexpr (VEC) {
+      vcursor_init<T, TM, TK>(ca, A, sg.lda, lkra, tile.m0, prob.M, sg.K, tid);
+      vcursor_init<T, TN, TK>(cb, B, sg.ldb, lkrb, tile.n0, prob.N, sg.K, tid);
+    } else {
+      cursor_init<T, TM, TK>(ca, A, sg.lda, lkra, tile.m0, prob.M, sg.K, tid);
+      cursor_init<T, TN, TK>(cb, B, sg.ldb, lkrb, tile.n0, prob.N, sg.K, tid);
+    }
     lscale = static_cast<scale_t>(sg.scale);
     lK = sg.K;
   };
@@ -332,27 +411,44 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
     }
   };
   // one staged K step in registers, with the orientation and scale it was
-  // loaded with
-  struct Staged {
+  // loaded with (elements, or 16-byte vectors when VEC)
+  typedef typename Vec16<T>::type vec_t;
+  struct StagedS {
     T a[TA::E], b[TB::E];
     scale_t scale;
     bool kra, krb, valid;
   };
+  struct StagedV {
+    vec_t a[VecTile<T, TM, TK>::NV], b[VecTile<T, TN, TK>::NV];
+    scale_t scale;
+    bool kra, krb, valid;
+  };
+  typedef typename std::conditional<VEC, StagedV, StagedS>::type Staged;
   bool more = lseg < tile.seg_hi;
   if (more) enter(lseg);
   auto fetch = [&](Staged& r) {
     r.valid = more;
     if (!more) return;
-    cursor_load<T, TA::E>(ca, r.a);
-    cursor_load<T, TB::E>(cb, r.b);
+    if constexpr (VEC) {
+      vcursor_load<T, VecTile<T, TM, TK>::NV>(ca, r.a);
+      vcursor_load<T, VecTile<T, TN, TK>::NV>(cb, r.b);
+    } else {
+      cursor_load<T, TA::E>(ca, r.a);
+      cursor_load<T, TB::E>(cb, r.b);
+    }
     r.scale = lscale;
     r.kra = lkra;
     r.krb = lkrb;
     more = next();
   };
   auto put = [&](const Staged& r, int buf) {
-    tile_store<T, TM, TK>(&lds[buf][0], r.kra, r.a, r.scale, tid);
-    tile_store<T, TN, TK>(&lds[buf][TA::ELEMS], r.krb, r.b, scale_t(1), tid);
+    if constexpr (VEC) {
+      vtile_store<T, TM, TK>(&lds[buf][0], r.kra, r.a, r.scale, tid);
+      vtile_store<T, TN, TK>(&lds[buf][TA::ELEMS], r.krb, r.b, scale_t(1), tid);
+    } else {
+      tile_store<T, TM, TK>(&lds[buf][0], r.kra, r.a, r.scale, tid);
+      tile_store<T, TN, TK>(&lds[buf][TA::ELEMS], r.krb, r.b, scale_t(1), tid);
+    }
     skra = (skra & ~(1u << buf)) | (r.kra ? 1u << buf : 0u);
     skrb = (skrb & ~(1u << buf)) | (r.krb ? 1u << buf : 0u);
   };
@@ -603,6 +699,13 @@ static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   const int chunk = 1 << 30;
   for (int base = 0; base < L.ntiles; base += chunk) {
     const int n = L.ntiles - base < chunk ? L.ntiles - base : chunk;
+    if constexpr (sizeof(T) >= 4) {
+      if (L.vec) {
+        hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF, true>), dim3(n), dim3(256), 0, st,
+                           L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base);
+        continue;
+      }
+    }
     hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF>), dim3(n), dim3(256), 0, st,
                        L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base);
   }
@@ -916,6 +1019,34 @@ __global__ __launch_bounds__(256) void block_scatter_kernel(const ScatterTask* _
     double* d = dst + (t.dst_off + r + int64_t(c) * t.ldd) * w;
     for (int k = 0; k < w; ++k) d[k] = t.scale * s[k];
   }
+}
+
+// Block copies with conversion: dst(T)[t.dst_off + r + c*t.ldd] = src[t.src_off + r + c*t.lds]
+// (w doubles per element), one workgroup per task.
+template <class TOut>
+__global__ __launch_bounds__(256) void repack_kernel(const ScatterTask* __restrict__ tasks,
+                                                     const double* __restrict__ src, TOut* __restrict__ dst, int w) {
+  const ScatterTask t = tasks[blockIdx.x];
+  const int64_t total = int64_t(t.rows) * t.cols;
+  for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int r = static_cast<int>(e % t.rows);
+    const int c = static_cast<int>(e / t.rows);
+    const double* s = src + (t.src_off + r + int64_t(c) * t.lds) * w;
+    TOut* d = dst + (t.dst_off + r + int64_t(c) * t.ldd) * w;
+    for (int k = 0; k < w; ++k) d[k] = static_cast<TOut>(s[k]);
+  }
+}
+
+hipError_t launch_repack(const ScatterTask* tasks, int ntasks, const double* src, void* dst, int w, int out_type,
+                         hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  if (out_type == 0)
+    hipLaunchKernelGGL(repack_kernel<double>, dim3(ntasks), dim3(256), 0, st, tasks, src, static_cast<double*>(dst), w);
+  else if (out_type == 1)
+    hipLaunchKernelGGL(repack_kernel<float>, dim3(ntasks), dim3(256), 0, st, tasks, src, static_cast<float*>(dst), w);
+  else
+    hipLaunchKernelGGL(repack_kernel<bf16_t>, dim3(ntasks), dim3(256), 0, st, tasks, src, static_cast<bf16_t*>(dst), w);
+  return hipGetLastError();
 }
 
 hipError_t launch_block_scatter(const ScatterTask* tasks, int ntasks, const double* src, double* dst, int w,

@@ -31,6 +31,11 @@ struct GemmLaunch {
   ArenaTable arenas{};
   int ntiles = 0;
   int nreduce = 0;
+  // Every operand of the table starts on a 16-byte boundary, has a leading
+  // dimension that is a multiple of 16 bytes, and is zero between its
+  // contiguous extent and that rounded up to 16 bytes: operands are then
+  // staged in 16-byte vectors (FP64 and FP32 tables).
+  bool vec = false;
 };
 
 // Output tile per configuration: 0: 64x64, 1: 128x128, 2: 128x64 (K steps of
@@ -108,5 +113,9 @@ struct ScatterTask {
 // column chunks); w = doubles per element (2 for complex).
 hipError_t launch_block_scatter(const ScatterTask* tasks, int ntasks, const double* src, double* dst, int w,
                                 hipStream_t st);
+// Kind-0 tasks as block copies from FP64 into out_type 0: double, 1: float,
+// 2: bfloat16 (round to nearest even), w values per element.
+hipError_t launch_repack(const ScatterTask* tasks, int ntasks, const double* src, void* dst, int w, int out_type,
+                         hipStream_t st);
 
 }  // namespace sadmrg
