@@ -41,11 +41,12 @@ class GemmBatch {
   // Appends every problem and segment of a finalized batch, each element
   // offset moved by shift[arena] (independent tables merged into one launch).
   void absorb(const GemmBatch& o, const int64_t (&shift)[kMaxArenas]);
-  // The caller guarantees the layout of 16-byte staging (every operand block
-  // starts on a multiple of 4 elements, leading dimensions are multiples of
-  // 4, and elements between an operand's contiguous extent and its leading
-  // dimension are zero); finalize() checks the alignment part and fails if
-  // it does not hold.  Real FP64 / FP32 runs then stage 16-byte vectors.
+  // The caller guarantees the layout of 16-byte staging: every operand
+  // block starts on a multiple of 4 elements, leading dimensions are
+  // multiples of 4, and the elements between an operand's contiguous extent
+  // and that rounded up to 4 are zero.  finalize() checks the alignment and
+  // fails if it does not hold.  Real FP64 / FP32 runs then stage 16-byte
+  // vectors.
   void request_vec() { vec_req_ = true; }
   bool vec() const { return vec_; }
   // complex: complex tables (configuration 0 only, no split reductions).

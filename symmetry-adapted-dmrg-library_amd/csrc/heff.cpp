@@ -18,15 +18,15 @@ void zero_bytes(Context* ctx, void* p, size_t bytes) {
 }
 // column chunks of one block copy (one workgroup each)
 void add_copy(std::vector<ScatterTask>& v, int64_t src_off, int64_t lds, int64_t dst_off, int64_t ldd, int rows,
-              int cols) {
-  for (int c0 = 0; c0 < cols; c0 += 64) {
+              int cols, int chunk = 64) {
+  for (int c0 = 0; c0 < cols; c0 += chunk) {
     ScatterTask t{};
     t.src_off = src_off + int64_t(c0) * lds;
     t.dst_off = dst_off + int64_t(c0) * ldd;
     t.lds = lds;
     t.ldd = ldd;
     t.rows = rows;
-    t.cols = std::min(64, cols - c0);
+    t.cols = std::min(chunk, cols - c0);
     t.kind = 0;
     t.scale = 1.0;
     v.push_back(t);
@@ -87,26 +87,34 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
     }
   }
   pack64_ = DevBuf(ctx, std::max<int64_t>(pack_elems_, 1) * 8 * w);
-  zero_bytes(ctx, pack64_.get(), pack64_.bytes());
   {
-    // one scatter launch per source tensor (each has its own base)
+    // one launch for every block of every tensor (task.pad indexes the
+    // source bases); the copies write the zero padding
+    std::vector<ScatterTask> ti;
+    std::vector<const double*> bases(tens.size());
     for (size_t i = 0; i < tens.size(); ++i) {
-      std::vector<ScatterTask> ti;
+      bases[i] = tens[i]->dptr();
       for (int b = 0; b < tens[i]->nten(); ++b) {
         if (poff[i][b] < 0) continue;
         int dd[4];
         tens[i]->st.block_dims(b, dd);
         const int64_t cols = dd[tens[i]->nbond() - 1];
+        const size_t first = ti.size();
         add_copy(ti, tens[i]->off[b], tens[i]->block_size(b) / cols, poff[i][b], pld[i][b],
                  static_cast<int>(tens[i]->block_size(b) / cols), static_cast<int>(cols));
+        for (size_t k = first; k < ti.size(); ++k) ti[k].pad = static_cast<int32_t>(i);
       }
-      if (ti.empty()) continue;
-      DevBuf tb(ctx, ti.size() * sizeof(ScatterTask));
-      SADMRG_HIP(hipMemcpyAsync(tb.get(), ti.data(), ti.size() * sizeof(ScatterTask), hipMemcpyHostToDevice,
-                                ctx->stream()));
-      SADMRG_HIP(launch_block_scatter(tb.as<ScatterTask>(), static_cast<int>(ti.size()), tens[i]->dptr(),
-                                      pack64_.as<double>(), w, ctx->stream()));
-      ctx->sync();  // tb and ti leave scope
+    }
+    if (!ti.empty()) {
+      const size_t tb_bytes = ti.size() * sizeof(ScatterTask);
+      DevBuf tb(ctx, tb_bytes + bases.size() * sizeof(double*));
+      SADMRG_HIP(hipMemcpyAsync(tb.get(), ti.data(), tb_bytes, hipMemcpyHostToDevice, ctx->stream()));
+      const double* const* dbases = reinterpret_cast<const double* const*>(static_cast<char*>(tb.get()) + tb_bytes);
+      SADMRG_HIP(hipMemcpyAsync(const_cast<const double**>(dbases), bases.data(), bases.size() * sizeof(double*),
+                                hipMemcpyHostToDevice, ctx->stream()));
+      SADMRG_HIP(launch_repack(tb.as<ScatterTask>(), static_cast<int>(ti.size()), nullptr, dbases, pack64_.get(), w, 0,
+                               ctx->stream()));
+      ctx->sync();  // tb, ti and bases leave scope
     }
   }
   auto pk = [&](const BTensor* t, int b) -> Operand {
@@ -125,7 +133,7 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
     int dd[2];
     ps.block_dims(b, dd);
     in_off[b] = in_elems_;
-    add_copy(in_tasks_, psi.off[b], dd[0], in_elems_, pad4(dd[0]), dd[0], dd[1]);
+    add_copy(in_tasks_, psi.off[b], dd[0], in_elems_, pad4(dd[0]), dd[0], dd[1], 8);  // many workgroups
     in_elems_ += pad4(dd[0]) * dd[1];
   }
   auto pin = [&](int b) -> Operand {
@@ -259,7 +267,6 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
   scratch64_ = DevBuf(ctx, std::max<int64_t>(scratch_elems_, 1) * 8 * w);
   zero_bytes(ctx, scratch64_.get(), scratch64_.bytes());  // padding rows stay zero
   in64_ = DevBuf(ctx, std::max<int64_t>(in_elems_, 1) * 8 * w);
-  zero_bytes(ctx, in64_.get(), in64_.bytes());
   if (!in_tasks_.empty()) {
     in_tasks_dev_ = DevBuf(ctx, in_tasks_.size() * sizeof(ScatterTask));
     SADMRG_HIP(hipMemcpyAsync(in_tasks_dev_.get(), in_tasks_.data(), in_tasks_.size() * sizeof(ScatterTask),
@@ -323,7 +330,6 @@ std::unique_ptr<HeffPlan> HeffPlan::merge(const std::vector<const HeffPlan*>& pa
   m->scratch64_ = DevBuf(ctx, std::max<int64_t>(scratch, 1) * 8 * w);
   zero_bytes(ctx, m->scratch64_.get(), m->scratch64_.bytes());
   m->in64_ = DevBuf(ctx, std::max<int64_t>(m->in_elems_, 1) * 8 * w);
-  zero_bytes(ctx, m->in64_.get(), m->in64_.bytes());
   if (!m->in_tasks_.empty()) {
     m->in_tasks_dev_ = DevBuf(ctx, m->in_tasks_.size() * sizeof(ScatterTask));
     SADMRG_HIP(hipMemcpyAsync(m->in_tasks_dev_.get(), m->in_tasks_.data(), m->in_tasks_.size() * sizeof(ScatterTask),
@@ -441,7 +447,6 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
       scratch32_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 8);
       in32_ = DevBuf(ctx_, std::max<int64_t>(in_elems_, 1) * 8);
       zero_bytes(ctx_, scratch32_.get(), scratch32_.bytes());
-      zero_bytes(ctx_, in32_.get(), in32_.bytes());
       SADMRG_HIP(launch_f64_to_f32(2 * pack_elems_, pack64_.as<double>(), pack32_.as<float>(), ctx_->stream()));
     }
     if (stage != 2) repack_input(d_in, in32_.get(), 1);
@@ -485,7 +490,6 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
       scratch16_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 2);
       in16_ = DevBuf(ctx_, std::max<int64_t>(in_elems_, 1) * 2);
       zero_bytes(ctx_, scratch16_.get(), scratch16_.bytes());
-      zero_bytes(ctx_, in16_.get(), in16_.bytes());
       SADMRG_HIP(launch_f64_to_bf16(pack_elems_, pack64_.as<double>(), pack16_.as<uint16_t>(), ctx_->stream()));
     }
     if (stage != 2) repack_input(d_in, in16_.get(), 2);
@@ -509,7 +513,6 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
     scratch32_ = DevBuf(ctx_, std::max<int64_t>(scratch_elems_, 1) * 4);
     in32_ = DevBuf(ctx_, std::max<int64_t>(in_elems_, 1) * 4);
     zero_bytes(ctx_, scratch32_.get(), scratch32_.bytes());
-    zero_bytes(ctx_, in32_.get(), in32_.bytes());
     SADMRG_HIP(launch_f64_to_f32(pack_elems_, pack64_.as<double>(), pack32_.as<float>(), ctx_->stream()));
   }
   if (stage != 2) repack_input(d_in, in32_.get(), 1);
@@ -530,7 +533,7 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
 
 void HeffPlan::repack_input(const double* d_in, void* dst, int out_type) {
   if (in_tasks_.empty()) return;
-  SADMRG_HIP(launch_repack(in_tasks_dev_.as<ScatterTask>(), static_cast<int>(in_tasks_.size()), d_in, dst,
+  SADMRG_HIP(launch_repack(in_tasks_dev_.as<ScatterTask>(), static_cast<int>(in_tasks_.size()), d_in, nullptr, dst,
                            cplx_ ? 2 : 1, out_type, ctx_->stream()));
 }
 

@@ -96,9 +96,11 @@ class HeffPlan {
   // Operand layout: every block of the operator pack, of the stage-1
   // intermediate and of the input copy is column-major with its leading
   // dimension padded to a multiple of 4 elements and starts on a multiple of
-  // 4 elements (16 bytes in FP32); the padding is zero.  The input vector is
-  // repacked into that layout (and converted) once per application; the
-  // output is written in the caller's layout.
+  // 4 elements (16 bytes in FP32), so the GEMMs stage 16-byte vectors; the
+  // padding is zero (written by the pack and input copies, filled once for
+  // the intermediate, whose padding rows the GEMMs never write).  The input
+  // vector is repacked into that layout (and converted) once per
+  // application; the output is written in the caller's layout.
   DevBuf pack64_, pack32_, scratch64_, scratch32_, in32_, in64_;
   DevBuf pack16_, scratch16_, in16_;  // BF16 operand copies (built on first BF16 use)
   int64_t pack_elems_ = 0, scratch_elems_ = 0, in_elems_ = 0;

@@ -1021,31 +1021,44 @@ __global__ __launch_bounds__(256) void block_scatter_kernel(const ScatterTask* _
   }
 }
 
-// Block copies with conversion: dst(T)[t.dst_off + r + c*t.ldd] = src[t.src_off + r + c*t.lds]
-// (w doubles per element), one workgroup per task.
+// Block copies into a padded layout, with conversion: for every column c of
+// a task, dst(T)[t.dst_off + r + c*t.ldd] = src[t.src_off + r + c*t.lds] for
+// r < t.rows and 0 for t.rows <= r < t.ldd (w doubles per element); one
+// workgroup per task.  srcs (optional): per-task source bases, t.pad indexing
+// them; otherwise src.
 template <class TOut>
 __global__ __launch_bounds__(256) void repack_kernel(const ScatterTask* __restrict__ tasks,
-                                                     const double* __restrict__ src, TOut* __restrict__ dst, int w) {
+                                                     const double* __restrict__ src,
+                                                     const double* const* __restrict__ srcs, TOut* __restrict__ dst,
+                                                     int w) {
   const ScatterTask t = tasks[blockIdx.x];
-  const int64_t total = int64_t(t.rows) * t.cols;
+  const double* base = srcs ? srcs[t.pad] : src;
+  const int64_t total = t.ldd * t.cols;
   for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
-    const int r = static_cast<int>(e % t.rows);
-    const int c = static_cast<int>(e / t.rows);
-    const double* s = src + (t.src_off + r + int64_t(c) * t.lds) * w;
+    const int r = static_cast<int>(e % t.ldd);
+    const int c = static_cast<int>(e / t.ldd);
     TOut* d = dst + (t.dst_off + r + int64_t(c) * t.ldd) * w;
-    for (int k = 0; k < w; ++k) d[k] = static_cast<TOut>(s[k]);
+    if (r < t.rows) {
+      const double* s = base + (t.src_off + r + int64_t(c) * t.lds) * w;
+      for (int k = 0; k < w; ++k) d[k] = static_cast<TOut>(s[k]);
+    } else {
+      for (int k = 0; k < w; ++k) d[k] = TOut(0);
+    }
   }
 }
 
-hipError_t launch_repack(const ScatterTask* tasks, int ntasks, const double* src, void* dst, int w, int out_type,
-                         hipStream_t st) {
+hipError_t launch_repack(const ScatterTask* tasks, int ntasks, const double* src, const double* const* srcs,
+                         void* dst, int w, int out_type, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   if (out_type == 0)
-    hipLaunchKernelGGL(repack_kernel<double>, dim3(ntasks), dim3(256), 0, st, tasks, src, static_cast<double*>(dst), w);
+    hipLaunchKernelGGL(repack_kernel<double>, dim3(ntasks), dim3(256), 0, st, tasks, src, srcs,
+                       static_cast<double*>(dst), w);
   else if (out_type == 1)
-    hipLaunchKernelGGL(repack_kernel<float>, dim3(ntasks), dim3(256), 0, st, tasks, src, static_cast<float*>(dst), w);
+    hipLaunchKernelGGL(repack_kernel<float>, dim3(ntasks), dim3(256), 0, st, tasks, src, srcs,
+                       static_cast<float*>(dst), w);
   else
-    hipLaunchKernelGGL(repack_kernel<bf16_t>, dim3(ntasks), dim3(256), 0, st, tasks, src, static_cast<bf16_t*>(dst), w);
+    hipLaunchKernelGGL(repack_kernel<bf16_t>, dim3(ntasks), dim3(256), 0, st, tasks, src, srcs,
+                       static_cast<bf16_t*>(dst), w);
   return hipGetLastError();
 }
 

@@ -113,9 +113,11 @@ struct ScatterTask {
 // column chunks); w = doubles per element (2 for complex).
 hipError_t launch_block_scatter(const ScatterTask* tasks, int ntasks, const double* src, double* dst, int w,
                                 hipStream_t st);
-// Kind-0 tasks as block copies from FP64 into out_type 0: double, 1: float,
-// 2: bfloat16 (round to nearest even), w values per element.
-hipError_t launch_repack(const ScatterTask* tasks, int ntasks, const double* src, void* dst, int w, int out_type,
-                         hipStream_t st);
+// Block copies from FP64 into a padded layout of out_type 0: double, 1: float,
+// 2: bfloat16 (round to nearest even), w values per element: rows [0, rows)
+// copied and [rows, ldd) zeroed in each column.  srcs (optional): per-task
+// source bases indexed by the task's pad field.
+hipError_t launch_repack(const ScatterTask* tasks, int ntasks, const double* src, const double* const* srcs,
+                         void* dst, int w, int out_type, hipStream_t st);
 
 }  // namespace sadmrg
