@@ -106,6 +106,7 @@ def gemm_kernel_name(prec, stage, cfg, vec=True):
     the FP32 path, stage 2 writes the FP64 output); vec: the 16-byte staging
     instantiation the H_eff plans run."""
     tail = ", true" if vec else ", false"
+    cfg %= 3  # launch plan -> tile configuration
     if prec == "fp64":
         return f"sadmrg::grouped_gemm_kernel<double, double, {_CFG_ARGS['fp64'][cfg]}{tail}>"
     tout = "float" if stage == 1 else "double"
@@ -625,6 +626,29 @@ def main():
               f"(stage1 {s1:.3f} ms, stage2 {s2:.3f} ms)", file=sys.stderr, flush=True)
     sad.vec_staging(True)
 
+    # ablation: the uniform split policy (every stage-2 problem cut into the
+    # same number of reduction parts), on a plan built and tuned under it
+    sad.balanced_split(False)
+    sad.autotune(clear=True)
+    uplan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
+    uniform = {}
+    for name, prec in (("fp32", sad.FP32), ("fp64", sad.FP64)):
+        for _ in range(args.warmup):
+            uplan.apply_device(d_in, d_out, prec)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            uplan.apply_device(d_in, d_out, prec)
+        ctx.sync()
+        uniform[name + "_ms"] = round((time.perf_counter() - t0) * 1e3 / args.steps, 4)
+        uniform[name + "_stage2_ms"] = round(time_stage(uplan, timer, d_in, d_out, prec, 2, max(3, args.steps // 2)), 4)
+    del uplan
+    sad.balanced_split(True)
+    sad.autotune(clear=True)
+    uniform["note"] = ("stage-2 reductions cut into the same number of parts per problem instead of parts of equal "
+                       "depth sized to the resident workgroups")
+    print(f"[bench] uniform split: {uniform}", file=sys.stderr, flush=True)
+
     # accuracy of the mixed path against FP64 on the same input
     o64 = tl.from_dev(plan.apply(psi, sad.FP64))
     o32 = tl.from_dev(plan.apply(psi, sad.FP32))
@@ -677,13 +701,16 @@ def main():
                    "sector_dims": wl["info"]["dims_fused"]},
         "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp64"].items()},
         "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp32"].items()},
-        "tile_configs": "0: 64x64, 1: 128x128, 2: 128x64 output tiles (autotuned per stage)",
+        "tile_configs": "launch plan = tile configuration (0: 64x64, 1: 128x128, 2: 128x64 output tiles) + 3 x "
+                        "split variant (0: about 3, 1: about 8 reduction parts per resident workgroup), "
+                        "autotuned per stage",
         "mixed_vs_fp64_speedup": round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3),
         "element_staging": {"fp32_ms": round(res["fp32_elem"]["ms_per_step"], 4),
                             "fp64_ms": round(res["fp64_elem"]["ms_per_step"], 4),
                             "fp32_stage_ms": [round(res["fp32_elem"]["stage1_ms"], 4),
                                               round(res["fp32_elem"]["stage2_ms"], 4)],
                             "note": "the same kernels staging one element per lane instead of 16-byte vectors"},
+        "uniform_split": uniform,
         "fp32_rel_err_vs_fp64": rel32,
         "bf16_ablation": {"ms_per_step": round(res["bf16"]["ms_per_step"], 4),
                           "tflops": round(res["bf16"]["tflops"], 3),

@@ -152,8 +152,10 @@ int sadmrg_heff_apply_unbatched(sadmrg_heff* h, const void* d_in, void* d_out, i
 int sadmrg_heff_merge(const sadmrg_heff* const* parts, int n, sadmrg_heff** out);
 int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops);
 /* info[8] = {psi blocks, stage-1 problems, stage-2 problems, stage-1 tiles,
-   stage-2 tiles, stage-1 tile cfg, stage-2 tile cfg, output tiles with a
-   split reduction} */
+   stage-2 tiles, stage-1 launch plan, stage-2 launch plan, output tiles with
+   a split reduction}.  Launch plan = tile configuration (0: 64x64, 1:
+   128x128, 2: 128x64 output tiles) + 3 x split variant (0: about 3, 1: about
+   8 reduction parts per resident workgroup). */
 int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n, int* info);
 /* tile-configuration autotuner of the H_eff stages (process-wide, on by
    default): enable 0/1 (-1 leaves the setting), clear forgets the tuned
@@ -165,6 +167,15 @@ int sadmrg_autotune(int enable, int clear, int* entries);
    leaves the setting; *previous receives the old value.  Off stages single
    elements (the comparison baseline).  No reference counterpart. */
 int sadmrg_gemm_vec_staging(int enable, int* previous);
+/* Split-reduction policy of the grouped GEMMs (process-wide, for plans built
+   afterwards).  A table with fewer output tiles than the device has room for
+   splits its reductions.  Balanced (1, the default): each problem's
+   reduction is cut into parts of about equal depth, sized to a number of
+   parts per resident workgroup that the autotuner picks per table.  Uniform
+   (0, the comparison baseline): every problem is cut into the same number of
+   parts.  -1 leaves the setting; *previous receives the old value.  No
+   reference counterpart. */
+int sadmrg_gemm_balanced_split(int enable, int* previous);
 /* Truncation route of sweep-step SVDs (BlockTensor::svd with (p1, p2) =
    (1, 0) or (0, 1), block_tensor_inst.cpp:501-637, real data): sectors whose
    larger side has at least gram_min rows are decomposed through the Gram

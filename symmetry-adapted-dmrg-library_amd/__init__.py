@@ -90,6 +90,7 @@ def lib():
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
         "sadmrg_lanczos_mode": (i, [i, i, ip, C.POINTER(C.c_longlong), i]),
         "sadmrg_gemm_vec_staging": (i, [i, ip]),
+        "sadmrg_gemm_balanced_split": (i, [i, ip]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_create_exci": (i, [vp, i, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_exci": (i, [vp, ip]),
@@ -443,6 +444,15 @@ def vec_staging(enable=None):
     previous setting."""
     prev = C.c_int()
     check(lib().sadmrg_gemm_vec_staging(-1 if enable is None else int(bool(enable)), C.byref(prev)))
+    return bool(prev.value)
+
+
+def balanced_split(enable=None):
+    """Split-reduction policy of the grouped GEMMs for plans built afterwards
+    (process-wide): balanced (parts of equal depth, the default) or uniform
+    (the same number of parts per problem).  Returns the previous setting."""
+    prev = C.c_int()
+    check(lib().sadmrg_gemm_balanced_split(-1 if enable is None else int(bool(enable)), C.byref(prev)))
     return bool(prev.value)
 
 

@@ -600,6 +600,13 @@ int sadmrg_gemm_vec_staging(int enable, int* previous) {
   });
 }
 
+int sadmrg_gemm_balanced_split(int enable, int* previous) {
+  return guard([&] {
+    if (previous) *previous = GemmBatch::balanced_split() ? 1 : 0;
+    if (enable >= 0) GemmBatch::set_balanced_split(enable != 0);
+  });
+}
+
 int sadmrg_svd_stats(long long* counts, double* seconds, int reset) {
   return guard([&] {
     SvdStats& st = svd_stats();

@@ -9,10 +9,13 @@ namespace sadmrg {
 
 namespace {
 std::atomic<bool> g_vec_staging{true};
+std::atomic<bool> g_balanced_split{true};
 }
 
 void GemmBatch::set_vec_staging(bool on) { g_vec_staging = on; }
 bool GemmBatch::vec_staging() { return g_vec_staging; }
+void GemmBatch::set_balanced_split(bool on) { g_balanced_split = on; }
+bool GemmBatch::balanced_split() { return g_balanced_split; }
 
 int GemmBatch::add_problem(int c_arena, int64_t c_off, int64_t ldc, int M, int N, bool accumulate) {
   require(!finalized_, "GemmBatch: add after finalize");
@@ -227,33 +230,53 @@ void GemmBatch::finalize(Context* ctx, bool complex) {
 }
 
 int GemmBatch::default_cfg(GemmPrec prec) const {
-  if (complex_ || !big_) return 0;
-  return prec == GemmPrec::F64 ? 2 : 1;
+  if (complex_) return 0;
+  // FP64 parts take about twice as long per K step as FP32 ones, so FP64
+  // tables balance best with the finer split variant
+  const int variant = prec == GemmPrec::F64 ? kTileConfigs : 0;
+  if (!big_) return variant;
+  return (prec == GemmPrec::F64 ? 2 : 1) + variant;
 }
 
 GemmBatch::TileSet& GemmBatch::tileset(Context* ctx, int cfg) {
   require(finalized_, "GemmBatch: run before finalize");
-  require(cfg >= 0 && cfg < kTileConfigs, "GemmBatch: bad tile configuration");
+  require(cfg >= 0 && cfg < kPlans, "GemmBatch: bad launch plan");
   require(!(complex_ && cfg != 0), "GemmBatch: complex tables use tile configuration 0");
   TileSet& ts = sets_[cfg];
   if (ts.built) return ts;
   ts.built = true;
-  const int tm = tile_m(cfg), tn = tile_n(cfg);
+  const int tm = tile_m(cfg % kTileConfigs), tn = tile_n(cfg % kTileConfigs);
   int64_t base_tiles = 0;
   for (const auto& p : hp_) base_tiles += int64_t((p.M + tm - 1) / tm) * ((p.N + tn - 1) / tn);
-  // Split factor: enough workgroups to fill the GPU, each keeping at least
-  // kMinSplitK of reduction depth, cut at segment boundaries by cumulative K.
+  // Split reductions, for tables with fewer output tiles than
+  // kTargetWorkgroups.  Each problem's reduction is cut at segment boundaries,
+  // by cumulative K, into parts of about equal length L (at least kMinSplitK),
+  // L chosen so that the table yields about `rounds` parts per workgroup slot
+  // of the device: every slot then runs about the same reduction work and the
+  // launch has no long tail of the deepest problems.  Uniform policy
+  // (rounds = 0): every problem is cut into the same number of parts (enough
+  // workgroups to fill the GPU), whatever its depth.
   constexpr int64_t kMinSplitK = 512;
+  const bool splitting = !complex_ && base_tiles > 0 && base_tiles < kTargetWorkgroups;
+  const int rounds = balanced_split() ? kSplitRounds[cfg / kTileConfigs] : 0;
   int want = 1;
-  if (!complex_ && base_tiles > 0 && base_tiles < kTargetWorkgroups)
+  int64_t part_len = 0;
+  if (splitting && rounds == 0) {
     want = static_cast<int>((kTargetWorkgroups + base_tiles - 1) / base_tiles);
+  } else if (splitting) {
+    int64_t total = 0;
+    for (size_t pi = 0; pi < hp_.size(); ++pi)
+      total += int64_t((hp_[pi].M + tm - 1) / tm) * ((hp_[pi].N + tn - 1) / tn) * klen_[pi];
+    part_len = std::max<int64_t>(kMinSplitK, total / (int64_t(kWorkgroupSlots) * rounds));
+  }
   std::vector<GemmTile> ht;
   std::vector<GemmReduce> hr;
   int nparts = 0;
   for (size_t pi = 0; pi < hp_.size(); ++pi) {
     const GemmProb& p = hp_[pi];
     const int nseg = p.seg_end - p.seg_begin;
-    int split = static_cast<int>(std::min<int64_t>({int64_t(want), int64_t(nseg), klen_[pi] / kMinSplitK}));
+    const int64_t parts = part_len > 0 ? (klen_[pi] + part_len / 2) / part_len : int64_t(want);
+    int split = static_cast<int>(std::min<int64_t>({parts, int64_t(nseg), klen_[pi] / kMinSplitK}));
     // segment groups [cut[g], cut[g+1]) with near-equal K
     std::vector<int> cut{p.seg_begin};
     if (split > 1) {
@@ -327,14 +350,15 @@ void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int c
   L.nreduce = ts.nreduce;
   L.vec = vec_ && g_vec_staging && !complex_ &&
           (prec == GemmPrec::F64 || prec == GemmPrec::F32 || prec == GemmPrec::F32_OUT64);
-  SADMRG_HIP(launch_grouped_gemm(prec, cfg, L, ctx->stream()));
+  SADMRG_HIP(launch_grouped_gemm(prec, cfg % kTileConfigs, L, ctx->stream()));
 }
 
 void GemmBatch::run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int cfg) {
   require(finalized_, "GemmBatch: run before finalize");
   if (hp_.empty()) return;
   if (cfg < 0) cfg = default_cfg(prec);
-  require(cfg >= 0 && cfg < kTileConfigs, "GemmBatch: bad tile configuration");
+  require(cfg >= 0 && cfg < kPlans, "GemmBatch: bad launch plan");
+  cfg %= kTileConfigs;  // no split reductions: the tile configuration only
   require(!(complex_ && cfg != 0), "GemmBatch: complex tables use tile configuration 0");
   const bool zprec = prec == GemmPrec::Z64 || prec == GemmPrec::Z32 || prec == GemmPrec::Z32_OUT64;
   require(zprec == complex_, "GemmBatch: precision and table disagree on complex data");

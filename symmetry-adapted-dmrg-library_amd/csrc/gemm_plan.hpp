@@ -51,15 +51,17 @@ class GemmBatch {
   bool vec() const { return vec_; }
   // complex: complex tables (configuration 0 only, no split reductions).
   void finalize(Context* ctx, bool complex);
+  // cfg: a launch plan id (tile configuration + split variant, see kPlans);
   // cfg < 0: the default configuration for prec (default_cfg).
   void run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int cfg = -1);
   // One launch per problem, over that problem's tiles only and without split
   // reductions: the per-block baseline the grouped launch is measured against.
   void run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int cfg = -1);
-  // Default tile configuration for a precision, from the problem shapes:
-  // large tiles when most of the work is in problems >= 256 x 256 (128 x 128
-  // for FP32 operands, 128 x 64 for FP64, whose wider accumulators would
-  // otherwise cost occupancy).
+  // Default launch plan for a precision.  Tile configuration from the
+  // problem shapes: large tiles when most of the work is in problems
+  // >= 256 x 256 (128 x 128 for FP32 operands, 128 x 64 for FP64, whose wider
+  // accumulators would otherwise cost occupancy).  Split variant: FP64 the
+  // finer one (about 8 parts per workgroup slot), otherwise about 3.
   int default_cfg(GemmPrec prec) const;
   int big_tiles() const { return big_; }
   // Coarse signature of the problem shapes (work fractions per size class of
@@ -73,6 +75,20 @@ class GemmBatch {
 
   // Workgroups the planner aims for before splitting reductions.
   static constexpr int kTargetWorkgroups = 2048;
+  // Workgroups resident at once on the device: 256 CUs, two 256-thread
+  // workgroups each (the large-tile configurations' register and LDS budget).
+  static constexpr int kWorkgroupSlots = 512;
+  // Launch plans: tile configuration (kernels.hpp) x split variant.  Split
+  // variant v cuts reductions into parts of equal depth, about
+  // kSplitRounds[v] parts per workgroup slot; plan id = cfg + kTileConfigs * v.
+  static constexpr int kSplitVariants = 2;
+  static constexpr int kSplitRounds[kSplitVariants] = {3, 8};
+  static constexpr int kPlans = kTileConfigs * kSplitVariants;
+  // Process-wide split policy for tile sets built afterwards: balanced (the
+  // default, parts of equal depth) or uniform (every problem cut into the
+  // same number of parts whatever its depth; the comparison baseline).
+  static void set_balanced_split(bool on);
+  static bool balanced_split();
   // Process-wide switch for 16-byte staging of tables that allow it
   // (default on; off: element staging, for comparison).
   static void set_vec_staging(bool on);
@@ -104,7 +120,7 @@ class GemmBatch {
   std::vector<GemmSeg> hs_;
   std::vector<int64_t> klen_;
   DevBuf d_probs_, d_segs_;
-  TileSet sets_[kTileConfigs];
+  TileSet sets_[kPlans];
   bool complex_ = false;
   bool big_ = false;
   bool vec_req_ = false, vec_ = false;

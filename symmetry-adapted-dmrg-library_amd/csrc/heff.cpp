@@ -407,7 +407,9 @@ int HeffPlan::cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& 
   SADMRG_HIP(hipEventCreate(&e1));
   int best = g.default_cfg(prec);
   float best_ms = 0;
-  for (int cfg = 0; cfg < kTileConfigs; ++cfg) {
+  for (int cfg = 0; cfg < GemmBatch::kPlans; ++cfg) {
+    // split variants only differ for tables that split their reductions
+    if (cfg >= kTileConfigs && g.nsplit_tiles(ctx_, cfg % kTileConfigs) == 0) continue;
     g.run(ctx_, prec, at, cfg);  // builds the tile set and warms caches
     SADMRG_HIP(hipEventRecord(e0, ctx_->stream()));
     g.run(ctx_, prec, at, cfg);

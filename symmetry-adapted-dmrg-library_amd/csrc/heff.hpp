@@ -34,7 +34,7 @@ struct HeffStats {
   int stage1_probs = 0, stage2_probs = 0;
   int64_t stage1_segs = 0, stage2_segs = 0;
   int tiles1 = 0, tiles2 = 0;
-  int tile_cfg1 = 0, tile_cfg2 = 0;  // configurations of the last application
+  int tile_cfg1 = 0, tile_cfg2 = 0;  // launch plans of the last application (GemmBatch::kPlans)
   int split_tiles = 0;  // output tiles whose reduction is split over workgroups (both stages)
 };
 

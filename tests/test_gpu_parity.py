@@ -528,3 +528,43 @@ def test_gram_truncation_matches_dense_svd(ctx, sad, p1, p2):
         assert rel <= 1e-10, rel
     finally:
         sad.svd_gram_min(prev)
+
+
+@pytest.mark.parametrize("balanced", [True, False])
+def test_heff_split_policies_match_reference(ctx, sad, cgc, balanced):
+    """Both split-reduction policies on the default launch plans, with the
+    autotuner off: balanced (FP64 about 8 reduction parts per resident
+    workgroup, FP32 about 3) and uniform (the same number of parts for every
+    problem).  Each gives the reference library's product: FP64 to 1e-12, FP32
+    operands to 2e-5; the two policies agree to 1e-13 in FP64."""
+    import refbind as rb
+    if not rb.available():
+        pytest.skip("oracle/_ref not built")
+    wl = tl.build_workload(D=640, n_op=2, seed=3)
+    psi = tl.to_dev(ctx, wl["psi"], 1)
+    hl, hr = tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"])
+    o1 = [tl.to_dev(ctx, t) for t in wl["o1"]]
+    o2 = [tl.to_dev(ctx, t) for t in wl["o2"]]
+    d, il, ir = tl.reference_heff_setup(rb, wl, 4)
+    exp = tl.from_ref(d.heff(il, ir, tl.to_ref(wl["psi"], 1)))
+    sad.autotune(enable=False)  # on by default for the process
+    prev_bal = sad.balanced_split(balanced)
+    try:
+        outs = {}
+        for pol in (balanced, not balanced):
+            sad.balanced_split(pol)
+            plan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
+            got = tl.from_dev(plan.apply(psi, sad.FP64))
+            st = plan.stats()
+            assert st["split_tiles"] > 0, st
+            rel, _ = tl.compare(got, exp)
+            assert rel <= 1e-12, (pol, rel, st)
+            got32 = tl.from_dev(plan.apply(psi, sad.FP32))
+            rel32, _ = tl.compare(got32, exp)
+            assert rel32 <= 2e-5, (pol, rel32, st)
+            outs[pol] = got
+        rel, _ = tl.compare(outs[True], outs[False])
+        assert rel <= 1e-13, rel
+    finally:
+        sad.balanced_split(prev_bal)
+        sad.autotune(enable=True)
