@@ -530,6 +530,8 @@ def main():
                     help="FP64 residual tolerance of the mixed path (relative to |E|; 0: library default)")
     ap.add_argument("--refine-krylov", type=int, default=3, help="FP32 Krylov steps per refinement restart")
     ap.add_argument("--no-bf16-sweep", action="store_true", help="skip the BF16 ablation sweep")
+    ap.add_argument("--no-split-ablation", action="store_true",
+                    help="skip the uniform-split ablation (its launches share kernel names with the main legs)")
     ap.add_argument("--lanczos-graph", default="on", choices=["on", "off"],
                     help="replay Lanczos steps from HIP graphs (off: the same kernels launched from the host, "
                          "for tools that do not follow graph launches)")
@@ -628,26 +630,28 @@ def main():
 
     # ablation: the uniform split policy (every stage-2 problem cut into the
     # same number of reduction parts), on a plan built and tuned under it
-    sad.balanced_split(False)
-    sad.autotune(clear=True)
-    uplan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
-    uniform = {}
-    for name, prec in (("fp32", sad.FP32), ("fp64", sad.FP64)):
-        for _ in range(args.warmup):
-            uplan.apply_device(d_in, d_out, prec)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            uplan.apply_device(d_in, d_out, prec)
-        ctx.sync()
-        uniform[name + "_ms"] = round((time.perf_counter() - t0) * 1e3 / args.steps, 4)
-        uniform[name + "_stage2_ms"] = round(time_stage(uplan, timer, d_in, d_out, prec, 2, max(3, args.steps // 2)), 4)
-    del uplan
-    sad.balanced_split(True)
-    sad.autotune(clear=True)
-    uniform["note"] = ("stage-2 reductions cut into the same number of parts per problem instead of parts of equal "
-                       "depth sized to the resident workgroups")
-    print(f"[bench] uniform split: {uniform}", file=sys.stderr, flush=True)
+    uniform = None
+    if not args.no_split_ablation:
+        sad.balanced_split(False)
+        sad.autotune(clear=True)
+        uplan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
+        uniform = {}
+        for name, prec in (("fp32", sad.FP32), ("fp64", sad.FP64)):
+            for _ in range(args.warmup):
+                uplan.apply_device(d_in, d_out, prec)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                uplan.apply_device(d_in, d_out, prec)
+            ctx.sync()
+            uniform[name + "_ms"] = round((time.perf_counter() - t0) * 1e3 / args.steps, 4)
+            uniform[name + "_stage2_ms"] = round(time_stage(uplan, timer, d_in, d_out, prec, 2, max(3, args.steps // 2)), 4)
+        del uplan
+        sad.balanced_split(True)
+        sad.autotune(clear=True)
+        uniform["note"] = ("stage-2 reductions cut into the same number of parts per problem instead of parts of equal "
+                           "depth sized to the resident workgroups")
+        print(f"[bench] uniform split: {uniform}", file=sys.stderr, flush=True)
 
     # accuracy of the mixed path against FP64 on the same input
     o64 = tl.from_dev(plan.apply(psi, sad.FP64))
