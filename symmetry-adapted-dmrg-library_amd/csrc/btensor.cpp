@@ -423,7 +423,12 @@ struct ScatterBuilder {
     tasks.push_back(t);
   }
   void run(Context* ctx, const double* src, int64_t src_n, double* dst, int64_t dst_n, int w) {
-    if (tasks.empty()) return;
+    run_many(ctx, {{src, dst}}, src_n, dst_n, w);
+  }
+  // The same table on several (source, destination) pairs of equal sizes.
+  void run_many(Context* ctx, const std::vector<std::pair<const double*, double*>>& pairs, int64_t src_n,
+                int64_t dst_n, int w) {
+    if (tasks.empty() || pairs.empty()) return;
     for (const auto& t : tasks) {
       if (t.rows <= 0 || t.cols <= 0) continue;
       if (t.kind == 0) {
@@ -439,16 +444,18 @@ struct ScatterBuilder {
     DevBuf d(ctx, tasks.size() * sizeof(ScatterTask));
     SADMRG_HIP(hipMemcpyAsync(d.get(), tasks.data(), tasks.size() * sizeof(ScatterTask), hipMemcpyHostToDevice,
                               ctx->stream()));
-    SADMRG_HIP(launch_block_scatter(d.as<ScatterTask>(), static_cast<int>(tasks.size()), src, dst, w,
-                                    ctx->stream()));
+    for (const auto& pr : pairs)
+      SADMRG_HIP(launch_block_scatter(d.as<ScatterTask>(), static_cast<int>(tasks.size()), pr.first, pr.second, w,
+                                      ctx->stream()));
     ctx->sync();
   }
 };
 
-}  // namespace
+// Further (output, operator) pairs executed with the table of the first.
+typedef std::vector<std::pair<BTensor*, const BTensor*>> EnvExtra;
 
-void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const BTensor* op1, const BTensor* op2,
-            int flag, CgcTables* cgc) {
+void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const BTensor* op1,
+                 const BTensor* op2, int flag, CgcTables* cgc, const EnvExtra* extra) {
   Context* ctx = uu.ctx;
   const SymKind sym = uu.sym();
   const bool su2 = sym == SymKind::SU2;
@@ -550,7 +557,10 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
         sb.copy(op.off[j], rows, dst, ldd, rows, cols, c.scale);
       }
     }
-    sb.run(ctx, stripe ? nullptr : op.dptr(), stripe ? 0 : op.nelem, out.dptr(), out.nelem, uu.w());
+    std::vector<std::pair<const double*, double*>> pairs{{stripe ? nullptr : op.dptr(), out.dptr()}};
+    if (extra && !stripe)
+      for (const auto& e : *extra) pairs.emplace_back(e.second->dptr(), e.first->dptr());
+    sb.run_many(ctx, pairs, stripe ? 0 : op.nelem, out.nelem, uu.w());
   };
 
   auto stage1_uu_op = [&](int j, int k) -> S1 {
@@ -736,6 +746,11 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
     for (int m = 0; m < os.nten(); ++m) mask[m] = !per[m].empty();
     out.clean();
     out.layout(ctx, cx, os, mask, kCgcCanon);
+    if (extra)
+      for (const auto& e : *extra) {
+        e.first->clean();
+        e.first->layout(ctx, cx, os, mask, kCgcCanon);
+      }
     if (scatter) {
       scatter_run(false);
       return;
@@ -766,6 +781,14 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
     }
     run_batch(g1, ctx, cx, ar);
     run_batch(g2, ctx, cx, ar);
+    if (extra)
+      for (const auto& e : *extra) {  // same tables, other operator and output
+        Arenas ae = ar;
+        ae.set(2, e.second->dptr(), e.second->nelem);
+        ae.set(5, e.first->dptr(), e.first->nelem);
+        if (!g1.empty()) g1.run(ctx, prec_of(cx), ae.at);
+        if (!g2.empty()) g2.run(ctx, prec_of(cx), ae.at);
+      }
     ctx->sync();
     return;
   }
@@ -861,6 +884,34 @@ void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const 
     return;
   }
   fail(kErrInvalid, "environment update: unknown kind");
+}
+
+}  // namespace
+
+void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const BTensor* op1, const BTensor* op2,
+            int flag, CgcTables* cgc) {
+  op_env_impl(out, kind, uu, vv, op1, op2, flag, cgc, nullptr);
+}
+
+void op_env_transform_many(const std::vector<BTensor*>& outs, const BTensor& uu, const BTensor& vv,
+                           const std::vector<const BTensor*>& ops, int flag, CgcTables* cgc) {
+  require(outs.size() == ops.size(), "operator transformations: one output per operator");
+  auto same_layout = [](const BTensor& a, const BTensor& b) {
+    return a.cplx == b.cplx && a.nelem == b.nelem && a.st == b.st && a.off == b.off;
+  };
+  std::vector<uint8_t> done(ops.size(), 0);
+  for (size_t q = 0; q < ops.size(); ++q) {
+    if (done[q]) continue;
+    require(ops[q] != nullptr && outs[q] != nullptr && outs[q] != ops[q], "operator transformations: bad operand");
+    EnvExtra extra;
+    for (size_t r = q + 1; r < ops.size(); ++r)
+      if (!done[r] && ops[r] && outs[r] && outs[r] != ops[r] && same_layout(*ops[r], *ops[q])) {
+        extra.emplace_back(outs[r], ops[r]);
+        done[r] = 1;
+      }
+    op_env_impl(*outs[q], kOperatorTransformation, uu, vv, ops[q], nullptr, flag, cgc, &extra);
+    done[q] = 1;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -82,6 +82,12 @@ enum EnvKind : int {
 };
 void op_env(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, const BTensor* op1, const BTensor* op2,
             int flag, CgcTables* cgc);
+// operator_transformation of several operators in one basis: outs[q] =
+// T(ops[q]).  Operators with the same block layout (structure and block
+// offsets) share one block table, built once and executed per operator with
+// one host synchronisation per group; the results equal op_env's.
+void op_env_transform_many(const std::vector<BTensor*>& outs, const BTensor& uu, const BTensor& vv,
+                           const std::vector<const BTensor*>& ops, int flag, CgcTables* cgc);
 
 // Adjoint of the overlap_transformation update (no reference counterpart;
 // used for coupling gradients).  With T(X) = overlap_transformation(uu, vv,

@@ -239,9 +239,16 @@ void Dmrg::prepare_site_operator_from_left(int i, int flag) {
   }
   if (!hh_[i - 1].is_null()) op_env(hh_[i], kOverlapTransformation, uu_[i], uu_[i], &hh_[i - 1], nullptr, 0, cg);
   else hh_[i].clean();
-  for (int k = 0; k < i; ++k)
-    if (fll_[i * ns_ + k] == 1)
-      op_env(opr_[i][k], kOperatorTransformation, uu_[i], uu_[i], &opr_[i - 1][k], nullptr, 0, cg);
+  {
+    std::vector<BTensor*> outs;
+    std::vector<const BTensor*> ops;
+    for (int k = 0; k < i; ++k)
+      if (fll_[i * ns_ + k] == 1) {
+        outs.push_back(&opr_[i][k]);
+        ops.push_back(&opr_[i - 1][k]);
+      }
+    op_env_transform_many(outs, uu_[i], uu_[i], ops, 0, cg);
+  }
   BTensor op;
   for (int k = 0; k < i; ++k)
     if (hm(i, k) > 0 && !opr_[i - 1][k].is_null()) add_coupled(op, hm(i, k), opr_[i - 1][k]);
@@ -268,10 +275,17 @@ void Dmrg::prepare_site_operator_from_right(int i, int flag) {
   }
   if (!hh_[i + 1].is_null()) op_env(hh_[i], kOverlapTransformation, uu_[i], uu_[i], &hh_[i + 1], nullptr, 1, cg);
   else hh_[i].clean();
-  for (int m0 = 0; m0 < ns_ - 1 - i; ++m0) {
-    const int k = i + 1 + m0;
-    if (frr_[i * ns_ + k] == 1)
-      op_env(opr_[i][k], kOperatorTransformation, uu_[i], uu_[i], &opr_[i + 1][k], nullptr, 1, cg);
+  {
+    std::vector<BTensor*> outs;
+    std::vector<const BTensor*> ops;
+    for (int m0 = 0; m0 < ns_ - 1 - i; ++m0) {
+      const int k = i + 1 + m0;
+      if (frr_[i * ns_ + k] == 1) {
+        outs.push_back(&opr_[i][k]);
+        ops.push_back(&opr_[i + 1][k]);
+      }
+    }
+    op_env_transform_many(outs, uu_[i], uu_[i], ops, 1, cg);
   }
   BTensor op;
   for (int m = 0; m < ns_ - 1 - i; ++m) {
@@ -296,8 +310,16 @@ void Dmrg::prepare_site_operator_from_left(int i) {
   op_fuse(u1, ctx_, cplx_, sym_, uu_[i].st.bonds[0], uu_[i].st.bonds[1]);
   if (!hh_[i - 1].is_null()) op_env(hh_[i], kOverlapTransformation, u1, u1, &hh_[i - 1], nullptr, 0, cg);
   else hh_[i].clean();
-  for (int k = 0; k < i; ++k)
-    if (fll_[i * ns_ + k] == 1) op_env(opr_[i][k], kOperatorTransformation, u1, u1, &opr_[i - 1][k], nullptr, 0, cg);
+  {
+    std::vector<BTensor*> outs;
+    std::vector<const BTensor*> ops;
+    for (int k = 0; k < i; ++k)
+      if (fll_[i * ns_ + k] == 1) {
+        outs.push_back(&opr_[i][k]);
+        ops.push_back(&opr_[i - 1][k]);
+      }
+    op_env_transform_many(outs, u1, u1, ops, 0, cg);
+  }
   BTensor op;
   for (int k = 0; k < i; ++k)
     if (hm(i, k) > 0 && !opr_[i - 1][k].is_null()) add_coupled(op, hm(i, k), opr_[i - 1][k]);
@@ -318,9 +340,17 @@ void Dmrg::prepare_site_operator_from_right(int i) {
   op_fuse(u1, ctx_, cplx_, sym_, uu_[i].st.bonds[0], uu_[i].st.bonds[1]);
   if (!hh_[i + 1].is_null()) op_env(hh_[i], kOverlapTransformation, u1, u1, &hh_[i + 1], nullptr, 1, cg);
   else hh_[i].clean();
-  for (int m0 = 0; m0 < ns_ - 1 - i; ++m0) {
-    const int k = i + 1 + m0;
-    if (frr_[i * ns_ + k] == 1) op_env(opr_[i][k], kOperatorTransformation, u1, u1, &opr_[i + 1][k], nullptr, 1, cg);
+  {
+    std::vector<BTensor*> outs;
+    std::vector<const BTensor*> ops;
+    for (int m0 = 0; m0 < ns_ - 1 - i; ++m0) {
+      const int k = i + 1 + m0;
+      if (frr_[i * ns_ + k] == 1) {
+        outs.push_back(&opr_[i][k]);
+        ops.push_back(&opr_[i + 1][k]);
+      }
+    }
+    op_env_transform_many(outs, u1, u1, ops, 1, cg);
   }
   BTensor op;
   for (int m = 0; m < ns_ - 1 - i; ++m) {

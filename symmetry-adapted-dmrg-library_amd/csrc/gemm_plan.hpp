@@ -34,7 +34,8 @@ class GemmBatch {
   void limit(int arena, int64_t elems) { lim_[arena] = elems; }
   int nprob() const { return finalized_ ? static_cast<int>(hp_.size()) : static_cast<int>(probs_.size()); }
   int64_t nseg() const;
-  bool empty() const { return probs_.empty(); }
+  // no problem to run (before finalize: none added; after: none with work)
+  bool empty() const { return finalized_ ? hp_.empty() : probs_.empty(); }
   // Algorithmic flops (real multiply-adds x2; complex x4) of the table.
   double flops(bool cplx) const;
 
