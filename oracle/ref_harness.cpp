@@ -559,6 +559,7 @@ static BlockTensor<S, Y>* dmrg_member(DMRG<S, Y>* d, int which, int i, int j) {
     case 2: return &d->opr[i][j];
     case 3: return &d->orth[j][i];
     case 4: return &d->ovlp[j][i];
+    case 6: return &d->sigma[i];  // local spin operators (dmrg_inst.cpp:222-260)
     default: return &d->overlapvec[j];
   }
 }

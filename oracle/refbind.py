@@ -258,10 +258,10 @@ class RefDMRG:
     def energy(self):
         return lib().ref_dmrg_energy(self.ptr)
 
-    SLOTS = {"uu": 0, "hh": 1, "opr": 2, "orth": 3, "ovlp": 4, "overlapvec": 5}
+    SLOTS = {"uu": 0, "hh": 1, "opr": 2, "orth": 3, "ovlp": 4, "overlapvec": 5, "sigma": 6}
 
     def tensor(self, which, i, j=0):
-        """uu[i], hh[i], opr[i][j]; orth[j][i], ovlp[j][i], overlapvec[j]."""
+        """uu[i], hh[i], opr[i][j]; orth[j][i], ovlp[j][i], overlapvec[j]; sigma[i]."""
         code = self.SLOTS[which]
         return RefTensor(lib().ref_dmrg_tensor(self.ptr, code, i, j), self.sym, self.cplx)
 

@@ -220,3 +220,154 @@ def test_package_lattice_matches_reference_constructor(xx, yy):
     h, _ = d.couplings()
     got = tl.load_pkg().reference_lattice(xx, yy)
     assert (np.asarray(h).reshape(got.shape) == got).all()
+
+
+def _ref_sigmas(extra):
+    """The reference's local spin operators sigma[0], sigma[1] (dmrg_inst.cpp:222-260),
+    read from a reference DMRG of the fixture's symmetry and scalar type."""
+    import refbind as rb
+    sym, cplx, xx, yy, D = (int(x) for x in extra["params"][:5])
+    d = rb.RefDMRG(sym, cplx, xx, yy, D)
+    return [tl.from_ref(d.tensor("sigma", 0)), tl.from_ref(d.tensor("sigma", 1))]
+
+
+def _same_bond(a, b):
+    return a[0] == b[0] and list(a[1]) == list(b[1]) and list(a[2]) == list(b[2])
+
+
+@pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built (no reference sources here)")
+@pytest.mark.parametrize("name", DMRG_CASES)
+def test_oracle_initial_and_pairup_updates_against_reference_library(name):
+    """overlap_initial, operator_initial and operator_pairup of the oracle equal
+    the reference's (block_tensor_inst.cpp:1661-1788, 1282-1469, 1953-2167) on the
+    calls prepare_site_operator_from_{left,right}(i, flag) makes
+    (dmrg_inst.cpp:440-583): overlap_initial(uu[i], uu[i], flag) with both flags,
+    operator_initial(uu[i], uu[i], sigma[flag], flag), and the pair-up of the
+    coupling-weighted operator sum sum_k coup[hmap[i][k]] opr[i -+ 1][k] with
+    sigma[1 - flag], on every site of the captured mid-sweep state whose
+    neighbouring environment is present.  The block Hamiltonian rebuilt from the
+    oracle's pieces (overlap_transformation of hh[i -+ 1] plus the pair-up) also
+    equals the reference's captured hh[i]."""
+    import refbind as rb
+    tens, extra = tl.load_tensors(os.path.join(tl.GOLDEN, f"{name}.npz"))
+    rb.init(physpn=1, max_angm=20, max_dcut=int(extra["params"][4]), psize=1)
+    st = _state(tens)
+    ns = int(extra["params"][5])
+    hmap = np.asarray(extra["hmap"]).reshape(ns, ns)
+    coup = np.asarray(extra["coup"])
+    sig_packed = _ref_sigmas(extra)
+    sig = [orc.from_packed(s) for s in sig_packed]
+    su2 = int(extra["params"][0]) == 1
+    il, ir = (int(x) for x in extra["cap_step"])
+    counts = {"overlap_initial": 0, "operator_initial": 0, "operator_pairup": 0, "hh": 0}
+
+    def check(kind, uu_p, op_b=None, sg_p=None, flag=0, op_p=None):
+        uu_b = orc.from_packed(uu_p)
+        got = orc.to_packed(orc.env(kind, uu_b, uu_b, op_b, orc.from_packed(sg_p) if sg_p is not None else None,
+                                    flag))
+        ref = tl.from_ref(rb.env(kind, tl.to_ref(uu_p), tl.to_ref(uu_p),
+                                 tl.to_ref(op_p) if op_p is not None else None,
+                                 tl.to_ref(sg_p) if sg_p is not None else None, flag))
+        if int(ref["header"][0]) == 0:
+            assert int(got["header"][0]) == 0 or not any(got["present"]), (kind, flag)
+            return got
+        rel, _ = tl.compare(got, ref)
+        assert rel <= 1e-12, (name, kind, flag, rel)
+        counts[kind] += 1
+        return got
+
+    for key, v in st.items():
+        if not key.startswith("uu_"):
+            continue
+        i = int(key.split("_")[1])
+        uu_b = orc.from_packed(v)
+        for flag in (0, 1):
+            check("overlap_initial", v, flag=flag)
+            s = sig[flag]
+            # operator_initial bond checks (block_tensor_inst.cpp:1296-1311)
+            if _same_bond(uu_b.bonds[1 - flag], orc.inv(s.bonds[0])) and _same_bond(uu_b.bonds[1 - flag],
+                                                                                     s.bonds[2]):
+                check("operator_initial", v, s, None, flag, op_p=sig_packed[flag])
+        for flag in (0, 1):
+            nb = i - 1 if flag == 0 else i + 1
+            # opr[i -+ 1][k] hold block operators of the left (right) environment
+            # only on sites up to il (from ir on) in the captured state
+            if not (1 <= i <= il if flag == 0 else ir <= i <= ns - 2):
+                continue
+            ks = range(0, i) if flag == 0 else range(i + 1, ns)
+            op = None
+            for k in ks:
+                o = st.get(f"opr_{nb}_{k}")
+                if hmap[i][k] > 0 and o is not None:
+                    op = orc.axpy(op, coup[hmap[i][k]], orc.from_packed(o))
+            if op is None or op.nbond == 0:
+                continue
+            s = sig[1 - flag]
+            # operator_pairup bond checks (block_tensor_inst.cpp:1963-1988)
+            b1 = s.bonds[1] if su2 else orc.inv(s.bonds[1])
+            if not (_same_bond(uu_b.bonds[flag], orc.inv(op.bonds[0])) and _same_bond(uu_b.bonds[flag], op.bonds[2])
+                    and _same_bond(op.bonds[1], b1)):
+                continue
+            op_p = orc.to_packed(op)
+            pair = check("operator_pairup", v, orc.from_packed(op_p), sig_packed[1 - flag], flag, op_p=op_p)
+            # hh[i] = overlap_transformation(uu, uu, hh[i -+ 1], flag) + pair-up (dmrg_inst.cpp:455-458, 471-486)
+            hprev = st.get(f"hh_{nb}")
+            want = st.get(f"hh_{i}")
+            # the sweep built hh[i] in the truncated basis on sites left of the
+            # pair (flag 0) and right of it (flag 1); hh[il], hh[ir] are enlarged
+            truncated = 1 <= i < il if flag == 0 else ir < i <= ns - 2
+            if hprev is None or want is None or not truncated:
+                continue
+            hp = orc.from_packed(hprev)
+            if not (_same_bond(uu_b.bonds[flag], orc.inv(hp.bonds[0])) and _same_bond(uu_b.bonds[flag], hp.bonds[1])):
+                continue
+            h = orc.env("overlap_transformation", uu_b, uu_b, hp, None, flag)
+            h = orc.axpy(h, 1.0, orc.from_packed(pair)) if h.nbond else orc.from_packed(pair)
+            rel, _ = tl.compare(orc.to_packed(h), want)
+            assert rel <= 1e-12, (name, i, flag, rel)
+            counts["hh"] += 1
+    print(name, counts)
+    assert counts["overlap_initial"] > 0 and counts["operator_initial"] > 0 and counts["operator_pairup"] > 0, counts
+    assert counts["hh"] > 0, counts
+
+
+@pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built (no reference sources here)")
+def test_oracle_pairup_check_rejects_planted_mistakes():
+    """The comparison above is sharp: the pair-up with the transposed spin
+    operator (the other flag's sigma) or a sign flip of the recoupling weight
+    differs from the reference's output."""
+    import refbind as rb
+    tens, extra = tl.load_tensors(os.path.join(tl.GOLDEN, "dmrg_su2_obc12.npz"))
+    rb.init(physpn=1, max_angm=20, max_dcut=int(extra["params"][4]), psize=1)
+    st = _state(tens)
+    ns = int(extra["params"][5])
+    hmap = np.asarray(extra["hmap"]).reshape(ns, ns)
+    coup = np.asarray(extra["coup"])
+    sig_packed = _ref_sigmas(extra)
+    il = int(extra["cap_step"][0])
+    i = il - 1
+    op = None
+    for k in range(0, i):
+        o = st.get(f"opr_{i - 1}_{k}")
+        if hmap[i][k] > 0 and o is not None:
+            op = orc.axpy(op, coup[hmap[i][k]], orc.from_packed(o))
+    assert op is not None
+    uu_p = st[f"uu_{i}"]
+    uu_b = orc.from_packed(uu_p)
+    op_p = orc.to_packed(op)
+    ref = tl.from_ref(rb.env("operator_pairup", tl.to_ref(uu_p), tl.to_ref(uu_p), tl.to_ref(op_p),
+                             tl.to_ref(sig_packed[1]), 0))
+    good = orc.to_packed(orc.env("operator_pairup", uu_b, uu_b, orc.from_packed(op_p),
+                                 orc.from_packed(sig_packed[1]), 0))
+    assert tl.compare(good, ref)[0] <= 1e-12
+    swapped = orc.to_packed(orc.env("operator_pairup", uu_b, uu_b, orc.from_packed(op_p),
+                                    orc.from_packed(sig_packed[0]), 0))
+    assert tl.compare(swapped, ref)[0] > 1e-3
+    orig = orc.fac_pairup
+    try:
+        orc.fac_pairup = lambda *a, **k: -orig(*a, **k)
+        flipped = orc.to_packed(orc.env("operator_pairup", uu_b, uu_b, orc.from_packed(op_p),
+                                        orc.from_packed(sig_packed[1]), 0))
+    finally:
+        orc.fac_pairup = orig
+    assert tl.compare(flipped, ref)[0] > 1e-3
