@@ -203,6 +203,33 @@ int sadmrg_lanczos_mode(int device_steps, int graph, int* prev, long long* count
    matvecs, Lanczos steps, refinement restarts}. */
 int sadmrg_lanczos(sadmrg_heff* h, sadmrg_tensor* psi, int mlanc, int precision, int refine, double refine_tol,
                    double* energy, int* counters, double* residual);
+/* Every option of the solve behind sadmrg_lanczos, for callers that set
+   more than the basis size and precision.  mlanc: Krylov vectors (1..2047;
+   the reference uses min(40, ndata + 1), dmrg_inst.cpp:267-270);
+   check_every: convergence check after every check_every-th step (the
+   reference's 4, dmrg_inst.cpp:1334; a value >= mlanc runs the whole
+   basis); refine_krylov / max_restarts: FP32 Krylov steps per FP64
+   residual-correction restart and the restart cap; device_steps / graph:
+   the step mode of this solve (both also need the process-wide switches of
+   sadmrg_lanczos_mode).  Bases of more than 64 vectors run host-driven
+   steps.  Arguments out of range return SADMRG_ERR_INVALID.  No reference
+   counterpart beyond Lanczos::lanczos1 (lanczos.hpp:70). */
+typedef struct sadmrg_lanczos_options {
+  int mlanc;
+  int precision;
+  int refine;
+  double refine_tol;
+  int refine_krylov;
+  int max_restarts;
+  int check_every;
+  int device_steps;
+  int graph;
+} sadmrg_lanczos_options;
+/* Fills *opt with the defaults of sadmrg_lanczos (mlanc 40, FP64, no
+   refinement, check every 4th step, device steps and graphs on). */
+int sadmrg_lanczos_default_options(sadmrg_lanczos_options* opt);
+int sadmrg_lanczos_ex(sadmrg_heff* h, sadmrg_tensor* psi, const sadmrg_lanczos_options* opt, double* energy,
+                      int* counters, double* residual);
 
 /* ---- DMRG driver (DMRG<Scalar,Sym>, dmrg.hpp:29-133) --------------------- */
 /* DMRG(xx, yy, bondd, physpn, totspin, 0) (dmrg.hpp:99) */

@@ -430,6 +430,9 @@ int ref_tensor_op(void* h, int op, double a, double b) {
       case 4: T *= a; break;
       case 5: T.normalize_vector(); break;
       case 6: T.initialize_input_vector(); break;
+      case 7: T.shift((int)a, (int)b); break;
+      case 8: T = a; break;
+      case 9: T *= dcmplex(a, b); break;
     }
   });
   return 0;

@@ -23,6 +23,13 @@ FP64, FP32, BF16 = 0, 1, 2
 _lib = None
 
 
+class LanczosOptions(C.Structure):
+    """sadmrg_lanczos_options (include/sadmrg.h)."""
+    _fields_ = [("mlanc", C.c_int), ("precision", C.c_int), ("refine", C.c_int), ("refine_tol", C.c_double),
+                ("refine_krylov", C.c_int), ("max_restarts", C.c_int), ("check_every", C.c_int),
+                ("device_steps", C.c_int), ("graph", C.c_int)]
+
+
 class SadmrgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"sadmrg error {code}: {msg}")
@@ -85,6 +92,8 @@ def lib():
         "sadmrg_heff_stats": (i, [vp, dp, dp, llp, ip]),
         "sadmrg_heff_merge": (i, [C.POINTER(vp), i, pp]),
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
+        "sadmrg_lanczos_default_options": (i, [C.POINTER(LanczosOptions)]),
+        "sadmrg_lanczos_ex": (i, [vp, vp, C.POINTER(LanczosOptions), dp, ip, dp]),
         "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_svd_gram_min": (i, [i, ip]),
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
@@ -421,6 +430,23 @@ class Heff:
                 "stage1_problems": int(info[1]), "stage2_problems": int(info[2]), "tiles1": int(info[3]),
                 "tiles2": int(info[4]), "tile_cfg1": int(info[5]), "tile_cfg2": int(info[6]),
                 "split_tiles": int(info[7])}
+
+    def lanczos_ex(self, psi, **options):
+        """sadmrg_lanczos_ex: every LanczosOptions field may be given by name
+        (mlanc, precision, refine, refine_tol, refine_krylov, max_restarts,
+        check_every, device_steps, graph); the rest keep their defaults."""
+        o = LanczosOptions()
+        check(lib().sadmrg_lanczos_default_options(C.byref(o)))
+        for k, v in options.items():
+            if k not in dict(LanczosOptions._fields_):
+                raise TypeError(f"unknown Lanczos option {k}")
+            setattr(o, k, type(getattr(o, k))(v))
+        e = C.c_double()
+        cnt = np.zeros(4, dtype=np.int32)
+        res = C.c_double()
+        check(lib().sadmrg_lanczos_ex(self.ptr, psi.ptr, C.byref(o), C.byref(e), _iptr(cnt), C.byref(res)))
+        return e.value, {"matvec64": int(cnt[0]), "matvec32": int(cnt[1]), "steps": int(cnt[2]),
+                         "restarts": int(cnt[3]), "residual64": res.value}
 
     def lanczos(self, psi, mlanc=40, precision=FP64, refine=False, refine_tol=0.0):
         e = C.c_double()

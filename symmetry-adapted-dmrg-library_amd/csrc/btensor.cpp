@@ -1013,6 +1013,25 @@ void op_take_conjugate(BTensor& t, int bond) {
   t.st.bonds[bond].dir = -t.st.bonds[bond].dir;
   if (t.sym() == SymKind::SU2 && t.nbond() == 2)
     for (auto& g : t.tag) g = (g == kCgcIdent) ? kCgcSinglet : kCgcOther;
+  if (t.sym() != SymKind::U1 || t.is_null()) return;
+  // U(1): the charge rule sum dir*q = locspin changes with the direction, so
+  // blocks it no longer allows drop out.  The reference keeps their data in
+  // memory but every block loop skips them (U1Struct::take_conjugate only
+  // flips the direction, u1.cpp:295-298); the visible tensor is the same.
+  std::vector<uint8_t> mask(t.nten(), 0);
+  bool drop = false;
+  for (int b = 0; b < t.nten(); ++b) {
+    mask[b] = t.present(b) && t.st.allowed(b);
+    if (t.present(b) && !mask[b]) drop = true;
+  }
+  if (!drop) return;
+  BTensor r;
+  r.layout(t.ctx, t.cplx, t.st, mask, t.tag.empty() ? kCgcIdent : t.tag[0]);
+  for (int b = 0; b < t.nten(); ++b)
+    if (mask[b] && t.block_size(b) > 0)
+      SADMRG_HIP(hipMemcpyAsync(r.block_ptr(b), t.block_ptr(b), t.block_size(b) * sizeof(double) * t.w(),
+                                hipMemcpyDeviceToDevice, t.ctx->stream()));
+  t = std::move(r);
 }
 
 void op_makeup_input_vector(BTensor& t) {
@@ -1170,9 +1189,18 @@ void op_axpy(BTensor& y, double re, double im, const BTensor& x) {
 dcmplex op_dot(const BTensor& x, const BTensor& y) {
   if (x.is_null() || y.is_null()) return dcmplex(0, 0);
   Context* ctx = x.ctx;
+  require(x.cplx == y.cplx && x.ctx == y.ctx, "inner_prod_c: scalar types or contexts differ");
+  // The reference walks x's blocks and indexes y's by the same block number,
+  // exiting when two present blocks differ in size (dense_inst.cpp:637); a
+  // present block of x past y's block count would be read out of range there.
   std::vector<int> blocks;
-  for (int b = 0; b < x.nten() && b < y.nten(); ++b)
-    if (x.st.allowed(b) && x.present(b) && y.present(b)) blocks.push_back(b);
+  for (int b = 0; b < x.nten(); ++b) {
+    if (!x.st.allowed(b) || !x.present(b)) continue;
+    require(b < y.nten(), "inner_prod_c: structures differ (block past the other tensor's block count)");
+    if (!y.present(b)) continue;
+    require(x.block_size(b) == y.block_size(b), "inner_prod_c: size mismatch of block " + std::to_string(b));
+    blocks.push_back(b);
+  }
   const int nb = static_cast<int>(blocks.size());
   require(nb <= Context::kMaxDscal / 2, "inner_prod_c: too many blocks");
   for (int i = 0; i < nb; ++i) {

@@ -637,11 +637,64 @@ int sadmrg_lanczos(sadmrg_heff* h, sadmrg_tensor* psi, int mlanc, int precision,
     if (h->p->merged()) fail(kErrInvalid, "lanczos: a merged plan has no single wave function");
     const BTensor& L = h->p->layout();
     if (psi->t.st != L.st || psi->t.off != L.off) fail(kErrInvalid, "lanczos: psi layout differs from the plan");
+    if (mlanc < 1 || mlanc > 2047) fail(kErrInvalid, "lanczos: mlanc must be in 1..2047");
     LanczosOpts o;
     o.mlanc = mlanc;
     o.prec = prec_of(precision);
     o.refine = refine != 0;
     if (refine_tol > 0) o.refine_tol = refine_tol;
+    const LanczosOut r = lanczos_solve(*h->p, psi->t.dptr(), o);
+    h->p->ctx()->sync();
+    if (energy) *energy = r.energy;
+    if (counters) {
+      counters[0] = r.matvec64;
+      counters[1] = r.matvec32;
+      counters[2] = r.steps;
+      counters[3] = r.restarts;
+    }
+    if (residual) *residual = r.residual64;
+  });
+}
+
+int sadmrg_lanczos_default_options(sadmrg_lanczos_options* opt) {
+  return guard([&] {
+    need(opt, "opt");
+    const LanczosOpts d;
+    opt->mlanc = d.mlanc;
+    opt->precision = SADMRG_FP64;
+    opt->refine = d.refine ? 1 : 0;
+    opt->refine_tol = d.refine_tol;
+    opt->refine_krylov = d.refine_krylov;
+    opt->max_restarts = d.max_restarts;
+    opt->check_every = d.check_every;
+    opt->device_steps = d.device_steps ? 1 : 0;
+    opt->graph = d.graph ? 1 : 0;
+  });
+}
+
+int sadmrg_lanczos_ex(sadmrg_heff* h, sadmrg_tensor* psi, const sadmrg_lanczos_options* opt, double* energy,
+                      int* counters, double* residual) {
+  return guard([&] {
+    need(h, "heff");
+    need(psi, "psi");
+    need(opt, "opt");
+    if (h->p->merged()) fail(kErrInvalid, "lanczos: a merged plan has no single wave function");
+    const BTensor& L = h->p->layout();
+    if (psi->t.st != L.st || psi->t.off != L.off) fail(kErrInvalid, "lanczos: psi layout differs from the plan");
+    if (opt->mlanc < 1 || opt->mlanc > 2047) fail(kErrInvalid, "lanczos: mlanc must be in 1..2047");
+    if (opt->check_every < 1) fail(kErrInvalid, "lanczos: check_every must be >= 1");
+    if (opt->refine_krylov < 1 || opt->max_restarts < 0) fail(kErrInvalid, "lanczos: bad refinement options");
+    if (!(opt->refine_tol > 0)) fail(kErrInvalid, "lanczos: refine_tol must be > 0");
+    LanczosOpts o;
+    o.mlanc = opt->mlanc;
+    o.prec = prec_of(opt->precision);
+    o.refine = opt->refine != 0;
+    o.refine_tol = opt->refine_tol;
+    o.refine_krylov = opt->refine_krylov;
+    o.max_restarts = opt->max_restarts;
+    o.check_every = opt->check_every;
+    o.device_steps = opt->device_steps != 0;
+    o.graph = opt->graph != 0;
     const LanczosOut r = lanczos_solve(*h->p, psi->t.dptr(), o);
     h->p->ctx()->sync();
     if (energy) *energy = r.energy;

@@ -79,8 +79,10 @@ void tridiag_eig(int n, std::vector<double>& d, std::vector<double> e, std::vect
 
 namespace {
 std::atomic<bool> g_device_steps{true}, g_graph{true};
-std::atomic<long long> g_replays{0}, g_captures{0};
+std::atomic<long long> g_replays{0}, g_captures{0}, g_graph_nodes{0};
 }  // namespace
+
+long long lanczos_graph_nodes() { return g_graph_nodes; }
 
 void lanczos_graph_counts(long long* replays, long long* captures, bool reset) {
   if (replays) *replays = g_replays;
@@ -113,6 +115,20 @@ struct StepGraph {
   }
 };
 
+void check_kernel_only(hipGraph_t g) {
+  size_t nn = 0;
+  SADMRG_HIP(hipGraphGetNodes(g, nullptr, &nn));
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (nn) SADMRG_HIP(hipGraphGetNodes(g, nodes.data(), &nn));
+  for (hipGraphNode_t nd : nodes) {
+    hipGraphNodeType t;
+    SADMRG_HIP(hipGraphNodeGetType(nd, &t));
+    if (t != hipGraphNodeTypeKernel)
+      fail(kErrHip, "lanczos: the captured Krylov step holds a non-kernel node (type " + std::to_string(int(t)) + ")");
+  }
+  g_graph_nodes = static_cast<long long>(nn);
+}
+
 struct Dev {
   Context* ctx;
   int64_t n;
@@ -135,12 +151,19 @@ struct Dev {
   void copy(const double* x, double* y) {
     SADMRG_HIP(hipMemcpyAsync(y, x, n * w() * 8, hipMemcpyDeviceToDevice, ctx->stream()));
   }
-  // w -= V[:, 0..m) (V^H w), twice (classical Gram-Schmidt with re-orthogonalisation)
+  // w -= V[:, 0..m) (V^H w), twice (classical Gram-Schmidt with re-orthogonalisation).
+  // The reduction kernel takes at most kOrthPanel columns, so longer bases are
+  // processed panel by panel (block Gram-Schmidt over the panels).
+  static constexpr int kOrthPanel = 64;
   void orth(const double* V, int m, double* x) {
     if (m <= 0) return;
     for (int pass = 0; pass < 2; ++pass) {
-      SADMRG_HIP(launch_gemv_t(V, n, m, x, n, cx ? 1 : 0, ctx->partial(), -1.0, ctx->dscal(), ctx->stream()));
-      SADMRG_HIP(launch_gemv_n_update(V, n, m, ctx->dscal(), x, n, cx ? 1 : 0, ctx->stream()));
+      for (int c0 = 0; c0 < m; c0 += kOrthPanel) {
+        const int mc = std::min(kOrthPanel, m - c0);
+        const double* Vp = V + static_cast<int64_t>(c0) * n * w();
+        SADMRG_HIP(launch_gemv_t(Vp, n, mc, x, n, cx ? 1 : 0, ctx->partial(), -1.0, ctx->dscal(), ctx->stream()));
+        SADMRG_HIP(launch_gemv_n_update(Vp, n, mc, ctx->dscal(), x, n, cx ? 1 : 0, ctx->stream()));
+      }
     }
   }
   // y = sum_j c_j V[:, j]
@@ -272,6 +295,13 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
         throw;
       }
       SADMRG_HIP(hipStreamEndCapture(ctx->stream(), &sg.graph));
+      // The step must hold kernel nodes only: a captured copy or fill would
+      // keep the source pointer or value of capture time, and a host-side
+      // source could be gone by the replay.  Every kernel argument points
+      // into the plan (operator packs, block and tile tables, scratch), the
+      // context (reduction partials, device scalars) or this solve's buffers
+      // (V, the step state, x, w), all of which outlive sg.
+      check_kernel_only(sg.graph);
       SADMRG_HIP(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
       g_captures++;
     }

@@ -43,6 +43,8 @@ void set_lanczos_mode(bool device_steps, bool graph);
 void lanczos_mode(bool* device_steps, bool* graph);
 // graph replays and captures since the last reset (process-wide)
 void lanczos_graph_counts(long long* replays, long long* captures, bool reset);
+// Nodes of the last captured Krylov step (every one a kernel node: checked at capture).
+long long lanczos_graph_nodes();
 
 struct LanczosOut {
   double energy = 0;

@@ -235,7 +235,7 @@ def test_lanczos_and_svd_on_reference_state(ctx, sad, name):
     assert list(ul["header"]) == list(tens["uu_l"]["header"])
 
 
-# How far a full run (do_idmrg + two sweeps from the seeded random start) can
+# How far a full run (do_idmrg + the fixture's sweeps from the seeded random start) can
 # follow the reference's trajectory.  The reference's algorithm is itself
 # ill-conditioned in two places, so bit-level trajectories only coincide
 # where neither is hit:
@@ -248,28 +248,42 @@ def test_lanczos_and_svd_on_reference_state(ctx, sad, name):
 #  * the per-sector kept rule S/S_max > 1e-10 with at least one value per
 #    sector (dense_inst.cpp:497-502) keeps a library-dependent number of
 #    states in sectors whose singular values are all round-off.
-# "exact": every energy to 1e-9 relative.  "converged": iDMRG energy to 1e-6
-# (Lanczos stopping rule), every second-sweep energy to 1e-9 (the reference's
-# second sweep has converged to machine precision).  "variational": the best
-# second-sweep energy agrees with the reference's to 1e-6 relative.
+# Tolerances per fixture: "idmrg" bounds the relative difference of the
+# iDMRG energy; "sweeps"[s] bounds every step energy of sweep s (relative),
+# None leaving that sweep to the variational bound only; "trunc" lists the
+# sweeps whose per-step truncation errors (sum of discarded squared singular
+# values) must agree to 1e-6 relative plus 1e-13 absolute (the resolution of
+# squared singular values near round-off).  Measured on MI355X, engine vs the
+# reference fixtures, maximum per-step relative energy difference:
+#   su2_obc12   iDMRG 3.2e-11, sweeps 1.6e-10 / 3.0e-13
+#   su2_ring10  iDMRG 2.2e-8 (Lanczos stop, see above), sweeps 3.1e-11 / 1.2e-15
+#   u1_ring10   iDMRG 1.4e-8, sweeps 4.1e-10 / 1.1e-14; truncation 7.5e-10 rel
+#   u1c_ring10  iDMRG 1.0e-8, sweeps 1.2e-9 / 2.9e-14; truncation 2.4e-9 rel
+#   u1_obc32    (L=32 m=64, four sweeps) iDMRG 1.7e-3, sweeps 1.1e-3 / 2.3e-8 /
+#               1.4e-12 / 4.7e-15; truncation |diff| 1.3e-14 / 2.1e-15 in
+#               sweeps 3 and 4 (values 6e-15 .. 7e-12)
+#   su2_j1j2_4x3 iDMRG 7e-3, sweeps 6.7e-3 / 1.3e-4: the iDMRG draws 50
+#               numbers against the reference's 40 (round-off sectors, above),
+#               so the runs leave from different states; only the best
+#               energy of the last sweep is compared (1e-6).
 # All cases: every energy is an upper bound of the exact ground state.
 DMRG_PARITY = {
-    "dmrg_su2_obc12": "exact",
-    "dmrg_su2_ring10": "converged",
-    "dmrg_u1_ring10": "variational",
-    "dmrg_u1c_ring10": "variational",
-    "dmrg_su2_j1j2_4x3": "variational",
-    "dmrg_u1_obc32": "variational",
+    "dmrg_su2_obc12": {"idmrg": 1e-9, "sweeps": [1e-9, 1e-9], "trunc": []},
+    "dmrg_su2_ring10": {"idmrg": 1e-6, "sweeps": [1e-9, 1e-9], "trunc": []},
+    "dmrg_u1_ring10": {"idmrg": 1e-6, "sweeps": [1e-9, 1e-9], "trunc": [1]},
+    "dmrg_u1c_ring10": {"idmrg": 1e-6, "sweeps": [3e-9, 1e-9], "trunc": [1]},
+    "dmrg_su2_j1j2_4x3": {"idmrg": None, "sweeps": [None, None], "trunc": [], "best_last": 1e-6},
+    "dmrg_u1_obc32": {"idmrg": 5e-3, "sweeps": [5e-3, 1e-7, 1e-9, 1e-9], "trunc": [2, 3]},
 }
 
 
 @pytest.mark.parametrize("name", DMRG_CASES)
 def test_full_dmrg_energies_match_reference(ctx, sad, name):
-    """do_idmrg + two sweeps (dmrg_inst.cpp:699-943) against the reference run
-    and against exact diagonalisation."""
+    """do_idmrg + the fixture's sweeps (dmrg_inst.cpp:699-943) against the
+    reference run, step by step, and against exact diagonalisation."""
     tens, extra = _load_case(name)
     sym, cplx, xx, yy, D, ns = (int(x) for x in extra["params"])
-    mode = DMRG_PARITY[name]
+    tol = DMRG_PARITY[name]
     hmap = np.asarray(extra["hmap"]).reshape(ns, ns)
     # exact diagonalisation is affordable up to 20 sites; beyond, the bound is
     # the reference's own best energy less its convergence scale
@@ -287,29 +301,28 @@ def test_full_dmrg_energies_match_reference(ctx, sad, name):
     ref0 = float(extra["e_idmrg"])
     steps = [(int(r["il"]), int(r["ir"]), r["energy"], int(r["matvec64"]), int(r["ndata"])) for r in d.records()]
     assert e0 >= floor, (e0, e_ed)
-    if mode == "exact":
-        assert abs(e0 - ref0) <= 1e-9 * abs(ref0), (e0, ref0, steps)
-    elif mode == "converged":
-        assert abs(e0 - ref0) <= 1e-6 * abs(ref0), (e0, ref0, steps)
+    if tol["idmrg"] is not None:
+        assert abs(e0 - ref0) <= tol["idmrg"] * abs(ref0), (e0, ref0, steps)
     energies = extra["energies"]
+    assert len(tol["sweeps"]) == energies.shape[0]
     for s in range(energies.shape[0]):
         d.sweep()
         recs = d.records()
         got = np.asarray([r["energy"] for r in recs])
         assert got.shape == energies[s].shape
         assert got.min() >= floor, (s, got.min(), e_ed)
-        if mode == "exact" or (mode == "converged" and s == energies.shape[0] - 1):
-            assert np.max(np.abs(got - energies[s])) <= 1e-9 * np.max(np.abs(energies[s])), (s, got, energies[s])
-        if mode == "variational" and s == energies.shape[0] - 1:
+        if tol["sweeps"][s] is not None:
+            rel = np.abs(got - energies[s]) / np.abs(energies[s])
+            assert rel.max() <= tol["sweeps"][s], (s, float(rel.max()), int(rel.argmax()))
+        if s in tol["trunc"]:
+            tr = np.asarray([r["trunc_err"] for r in recs])
+            tref = np.asarray(extra["trunc"][s])
+            assert tr.shape == tref.shape
+            dev = np.abs(tr - tref)
+            assert np.all(dev <= 1e-6 * tref + 1e-13), (s, float(dev.max()), tr, tref)
+        if "best_last" in tol and s == energies.shape[0] - 1:
             ref_best = energies[s].min()
-            assert abs(got.min() - ref_best) <= 1e-6 * abs(ref_best), (got.min(), ref_best, e_ed)
-            if "trunc" in extra:
-                # truncation error (sum of discarded squared singular values)
-                # of the last sweep: same scale as the reference's
-                tr = np.asarray([r["trunc_err"] for r in recs])
-                tref = extra["trunc"][s]
-                big = tref > 1e-13
-                assert np.all(np.abs(np.log10(tr[big] + 1e-300) - np.log10(tref[big])) <= 1.0), (tr, tref)
+            assert abs(got.min() - ref_best) <= tol["best_last"] * abs(ref_best), (got.min(), ref_best, e_ed)
 
 
 @pytest.mark.parametrize("name", DMRG_CASES)
@@ -371,6 +384,34 @@ def test_heff_split_reduction_matches_reference(ctx, sad, cgc, D, n_op):
     got32 = tl.from_dev(plan.apply(psi, sad.FP32))
     rel32, _ = tl.compare(got32, exp)
     assert rel32 <= 2e-5, (rel32, st)
+
+
+def test_heff_full_centre_bond_shape_matches_reference(ctx, sad, cgc):
+    """The full metric shape: SU(2) centre bond at D=2048 multiplets (fused
+    sectors up to 1255), n_op=12 bond terms, N = 3.58 M reduced elements.
+    FP64 output equals the reference library's
+    hamiltonian_vector_multiplication_idmrg (dmrg_inst.cpp:283-371) on the
+    same tensors to 1e-12 of the largest element; the FP32-operand path
+    (FP32 MFMA accumulation) stays within 1e-5."""
+    import refbind as rb
+    if not rb.available():
+        pytest.skip("oracle/_ref not built")
+    wl = tl.build_workload(D=2048, n_op=12, seed=1)
+    psi = tl.to_dev(ctx, wl["psi"], 1)
+    hl, hr = tl.to_dev(ctx, wl["hl"]), tl.to_dev(ctx, wl["hr"])
+    o1 = [tl.to_dev(ctx, t) for t in wl["o1"]]
+    o2 = [tl.to_dev(ctx, t) for t in wl["o2"]]
+    plan = sad.Heff(ctx, cgc, hl, hr, o1, o2, psi)
+    got = tl.from_dev(plan.apply(psi))
+    got32 = tl.from_dev(plan.apply(psi, sad.FP32))
+    del plan
+    d, il, ir = tl.reference_heff_setup(rb, wl, min(16, os.cpu_count() or 1))
+    exp = tl.from_ref(d.heff(il, ir, tl.to_ref(wl["psi"], 1)))
+    rel, _ = tl.compare(got, exp)
+    assert rel <= 1e-12, rel
+    rel32, _ = tl.compare(got32, exp)
+    assert rel32 <= 1e-5, rel32
+    print(f"D=2048 n_op=12: FP64 rel err {rel:.2e}, FP32 rel err {rel32:.2e}")
 
 
 @pytest.mark.parametrize("prec", [0, 1])
