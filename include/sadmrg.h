@@ -184,6 +184,25 @@ int sadmrg_gemm_balanced_split(int enable, int* previous);
    the setting; *previous receives the old value.  Process-wide, default 256.
    No reference counterpart (the reference always calls gesvd). */
 int sadmrg_svd_gram_min(int value, int* previous);
+/* Batched symmetric eigensolver of the truncation's Gram route: for each of
+   `count` real symmetric matrices a[q] (n[q] x n[q], column-major, leading
+   dimension lda[q], lower triangle read, contents destroyed; device
+   pointers), all eigenvalues in ascending order into w[q] (device, n[q]
+   doubles) and the eigenvectors of the k[q] largest into z[q] (device,
+   n[q] x k[q], leading dimension ldz[q]; column c belongs to
+   w[q][n[q] - 1 - c]; k[q] = 0 asks for none).  Tridiagonalisation of every
+   matrix in one persistent launch, bisection, inverse iteration with
+   Cholesky-QR re-orthonormalisation, back-transformation; n[q] <= 2048.
+   The batched counterpart of ScalarTraits<double>::heev (scalar.hpp:109-114),
+   which the reference calls one matrix at a time. */
+int sadmrg_eigh_batched(sadmrg_ctx* ctx, int count, const int* n, double* const* a, const int* lda,
+                        double* const* w, const int* k, double* const* z, const int* ldz);
+/* Eigensolver of the Gram route inside BlockTensor::svd (process-wide):
+   1 (default) the hand-written batched solver above; 0 rocSOLVER's
+   strided-batched syevd on matrices padded to the batch's largest order
+   (the comparison).  previous (optional) receives the old setting.  No
+   reference counterpart. */
+int sadmrg_svd_eigensolver(int value, int* previous);
 /* Truncation counters (process-wide): counts[4] = {svd calls, sectors
    decomposed through the Gram route, sectors decomposed by gesdd, Gram
    sectors redone by gesdd}; *seconds = wall time inside svd; reset != 0

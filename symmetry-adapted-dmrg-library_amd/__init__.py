@@ -93,6 +93,8 @@ def lib():
         "sadmrg_heff_merge": (i, [C.POINTER(vp), i, pp]),
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
         "sadmrg_lanczos_default_options": (i, [C.POINTER(LanczosOptions)]),
+        "sadmrg_eigh_batched": (i, [vp, i, ip, pp, ip, pp, ip, pp, ip]),
+        "sadmrg_svd_eigensolver": (i, [i, ip]),
         "sadmrg_lanczos_ex": (i, [vp, vp, C.POINTER(LanczosOptions), dp, ip, dp]),
         "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_svd_gram_min": (i, [i, ip]),
@@ -480,6 +482,28 @@ def balanced_split(enable=None):
     prev = C.c_int()
     check(lib().sadmrg_gemm_balanced_split(-1 if enable is None else int(bool(enable)), C.byref(prev)))
     return bool(prev.value)
+
+
+def eigh_batched(ctx, n, a, lda, w, k=None, z=None, ldz=None):
+    """sadmrg_eigh_batched on device buffers: n, lda, k, ldz are lists of ints,
+    a, w, z lists of device addresses (ints).  Eigenvalues ascending into w;
+    eigenvectors of the k largest into z."""
+    cnt = len(n)
+    arr = lambda xs: (C.c_int * max(cnt, 1))(*xs)  # noqa: E731
+    ptrs = lambda xs: (C.c_void_p * max(cnt, 1))(*[C.c_void_p(x) for x in xs])  # noqa: E731
+    k = k or [0] * cnt
+    z = z or [0] * cnt
+    ldz = ldz or [1] * cnt
+    check(lib().sadmrg_eigh_batched(ctx.ptr, cnt, arr(n), ptrs(a), arr(lda), ptrs(w), arr(k), ptrs(z), arr(ldz)))
+
+
+def svd_eigensolver(value=None):
+    """Eigensolver of the truncation's Gram route (process-wide): 1 the
+    hand-written batched solver, 0 rocSOLVER syevd on padded batches.
+    Returns the previous setting."""
+    prev = C.c_int()
+    check(lib().sadmrg_svd_eigensolver(-1 if value is None else int(value), C.byref(prev)))
+    return prev.value
 
 
 def lanczos_mode(device_steps=None, graph=None, reset=False):

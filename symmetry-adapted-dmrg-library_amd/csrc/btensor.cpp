@@ -1,5 +1,6 @@
 // btensor.cpp -- see btensor.hpp.
 #include "btensor.hpp"
+#include "eigen.hpp"
 #include <atomic>
 #include <chrono>
 #include <functional>
@@ -1351,10 +1352,13 @@ void op_env_adjoint(BTensor& out, const BTensor& lam, const BTensor& uu, const B
 
 namespace {
 std::atomic<int> g_gram_min{256};
+std::atomic<int> g_svd_eig{1};
 SvdStats g_svd_stats;
 }
 void set_gram_min(int n) { g_gram_min.store(n); }
 int gram_min() { return g_gram_min.load(); }
+void set_svd_eigensolver(int v) { g_svd_eig.store(v); }
+int svd_eigensolver() { return g_svd_eig.load(); }
 SvdStats& svd_stats() { return g_svd_stats; }
 
 // ---------------------------------------------------------------------------
@@ -1596,6 +1600,76 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
     }
     ctx->sync();  // batch buffers are released at the end of this scope
   };
+  // Unpadded Gram route on the hand-written batched eigensolver: every
+  // sector's Gram matrix (lower triangle by syrk, straight from the block)
+  // is tridiagonalised and all its eigenvalues found here; the eigenvectors
+  // of the kept values only are computed once the global selection is final
+  // (eig_vectors below).  S_k = sqrt(lambda_k) for resolved eigenvalues; an
+  // unresolved one is reported as sqrt(max(lambda, 0)), within
+  // sqrt(eps lambda_max) of the singular value.
+  std::vector<uint8_t> eig_sec(secs.size(), 0);
+  std::vector<DevBuf> eig_g, eig_w;
+  std::vector<EigJob> eig_jobs;
+  std::vector<size_t> eig_of_job;
+  std::unique_ptr<SymEigBatch> eig;
+  auto eig_batch = [&](const std::vector<size_t>& list) {
+    eig.reset(new SymEigBatch(ctx));
+    eig_g.clear();
+    eig_w.clear();
+    eig_jobs.clear();
+    eig_of_job.clear();
+    const double one = 1.0, zero = 0.0;
+    for (size_t si : list) {
+      const Sec& sc = secs[si];
+      const int g = right_gram ? sc.n : sc.m;
+      eig_g.emplace_back(ctx, int64_t(g) * g * 8);
+      eig_w.emplace_back(ctx, int64_t(g) * 8);
+      const double* A = vec.block_ptr(sc.i + sc.j * nm1);
+      rocblas_status st = right_gram
+          ? rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, g, sc.m, &one, A, sc.m, &zero,
+                          eig_g.back().as<double>(), g)
+          : rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, g, sc.n, &one, A, sc.m, &zero,
+                          eig_g.back().as<double>(), g);
+      if (st != rocblas_status_success) fail(kErrNumerics, "rocblas syrk (Gram) failed");
+      EigJob J;
+      J.a = eig_g.back().as<double>();
+      J.n = g;
+      J.lda = g;
+      J.w = eig_w.back().as<double>();
+      eig_jobs.push_back(J);
+      eig_of_job.push_back(si);
+    }
+    eig->eigenvalues(eig_jobs);
+    std::vector<std::vector<double>> lamv(list.size());
+    for (size_t q = 0; q < list.size(); ++q) {
+      lamv[q].resize(eig_jobs[q].n);
+      SADMRG_HIP(hipMemcpyAsync(lamv[q].data(), eig_jobs[q].w, eig_jobs[q].n * 8, hipMemcpyDeviceToHost,
+                                ctx->stream()));
+    }
+    ctx->sync();
+    for (size_t q = 0; q < list.size(); ++q) {
+      const size_t si = list[q];
+      const Sec& sc = secs[si];
+      std::vector<double>& lam = lamv[q];
+      const int g = eig_jobs[q].n;
+      std::sort(lam.begin(), lam.end());  // bisection values are monotone up to its tolerance
+      const double lmax = std::max(lam[g - 1], 0.0);
+      double* S = gram_s.data() + sc.s_off;
+      int nres = 0;
+      for (int k = 0; k < sc.mn; ++k) {
+        const double l = lam[g - 1 - k];
+        const bool resolved = l > kGramRes * lmax && l > 0.0;
+        S[k] = std::sqrt(std::max(l, 0.0));
+        if (resolved) nres = k + 1;
+      }
+      gram_nres[si] = nres;
+      gram_res[si] = nres < sc.mn ? std::sqrt(kGramRes * lmax) + 1e-15 * std::sqrt(lmax) : 0.0;
+      gram_sec[si] = 1;
+      eig_sec[si] = 1;
+      SADMRG_HIP(hipMemcpyAsync(eig_jobs[q].w, lam.data(), g * 8, hipMemcpyHostToDevice, ctx->stream()));
+    }
+    ctx->sync();
+  };
   // Decomposes the listed sectors: groups of similar size, largest first.
   // The Gram route's eigensolver is launch-latency bound (one chain of small
   // kernels per matrix column, batched over the matrices), so a step whose
@@ -1605,6 +1679,16 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
     std::stable_sort(list.begin(), list.end(), [&](size_t x, size_t y) {
       return std::max(secs[x].m, secs[x].n) > std::max(secs[y].m, secs[y].n);
     });
+    if (allow_gram && gram && !list.empty() && std::max(secs[list[0]].m, secs[list[0]].n) >= gmin &&
+        svd_eigensolver() == 1) {
+      bool fits = true;
+      for (size_t si : list) fits = fits && (right_gram ? secs[si].n : secs[si].m) <= SymEigBatch::max_order();
+      if (fits) {
+        eig_batch(list);
+        g_svd_stats.gram_sectors += static_cast<int64_t>(list.size());
+        return;
+      }
+    }
     if (allow_gram && gram && !list.empty() && std::max(secs[list[0]].m, secs[list[0]].n) >= gmin) {
       for (size_t k0 = 0; k0 < list.size(); k0 += kMaxGramBatch) {
         std::vector<size_t> part(list.begin() + k0, list.begin() + std::min(list.size(), k0 + kMaxGramBatch));
@@ -1655,6 +1739,7 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
       if (gram_sec[si] && gram_nres[si] == 0) {
         gram_sec[si] = 0;
         gram_res[si] = 0.0;
+        eig_sec[si] = 0;
         single(si);
       }
     for (size_t si = 0; si < secs.size(); ++si)
@@ -1713,6 +1798,7 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
       for (size_t si : redo) {
         gram_sec[si] = 0;
         gram_res[si] = 0.0;
+        eig_sec[si] = 0;
       }
       decompose(redo, false);
       finish();
@@ -1736,6 +1822,69 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
     if (dim[i] > 0) { kept_q.push_back(bb0.q[i]); kept_d.push_back(dim[i]); }
   }
   if (kept_out) *kept_out = dim;
+  // Eigen route: eigenvectors of the kept values of each accepted sector, then
+  // the other factor.  Right Gram: V = eigenvectors of psi^T psi (n x k),
+  // U = psi V / S.  Left Gram: U = eigenvectors of psi psi^T (m x k), written
+  // straight into U, V' rows = U^T psi / S.  Only the first dim columns of U /
+  // rows of V' are filled: the projectors take no more.
+  if (eig) {
+    std::vector<EigJob> jobs = eig_jobs;
+    std::vector<DevBuf> zb(jobs.size());
+    bool any = false;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const size_t si = eig_of_job[q];
+      const Sec& sc = secs[si];
+      const int k = (eig_sec[si] && gram_sec[si]) ? dim[sc.i] : 0;
+      jobs[q].k = k;
+      if (k == 0) continue;
+      any = true;
+      if (right_gram) {
+        zb[q] = DevBuf(ctx, int64_t(sc.n) * k * 8);
+        jobs[q].z = zb[q].as<double>();
+        jobs[q].ldz = sc.n;
+      } else {
+        jobs[q].z = du.as<double>() + sc.u_off;
+        jobs[q].ldz = sc.m;
+      }
+    }
+    if (any) {
+      eig->top_vectors(jobs);
+      std::vector<double> inv(s_tot, 0.0);
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const Sec& sc = secs[eig_of_job[q]];
+        for (int c = 0; c < jobs[q].k; ++c) {
+          const double sv = hs[sc.s_off + c];
+          inv[sc.s_off + c] = sv > 0.0 ? 1.0 / sv : 0.0;
+        }
+      }
+      DevBuf dinv(ctx, std::max<int64_t>(s_tot, 1) * 8);
+      SADMRG_HIP(hipMemcpyAsync(dinv.get(), inv.data(), s_tot * 8, hipMemcpyHostToDevice, ctx->stream()));
+      const double one = 1.0, zero = 0.0;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const int k = jobs[q].k;
+        if (k == 0) continue;
+        const Sec& sc = secs[eig_of_job[q]];
+        const double* A = vec.block_ptr(sc.i + sc.j * nm1);
+        const double* iv = dinv.as<double>() + sc.s_off;
+        if (right_gram) {
+          // V' rows c = V column c; U = A V / S
+          SADMRG_HIP(launch_copy2d(k, sc.n, jobs[q].z, sc.n, 1, dvt.as<double>() + sc.vt_off, sc.mn, 1, 1.0, nullptr,
+                                   0, ctx->stream()));
+          if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, sc.m, k, sc.n, &one, A, sc.m, jobs[q].z,
+                            sc.n, &zero, du.as<double>() + sc.u_off, sc.m) != rocblas_status_success)
+            fail(kErrNumerics, "rocblas gemm (Gram) failed");
+          SADMRG_HIP(launch_strided_scale(k, sc.m, du.as<double>() + sc.u_off, 1, sc.m, iv, ctx->stream()));
+        } else {
+          if (rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, k, sc.n, sc.m, &one,
+                            du.as<double>() + sc.u_off, sc.m, A, sc.m, &zero, dvt.as<double>() + sc.vt_off,
+                            sc.mn) != rocblas_status_success)
+            fail(kErrNumerics, "rocblas gemm (Gram) failed");
+          SADMRG_HIP(launch_strided_scale(k, sc.n, dvt.as<double>() + sc.vt_off, sc.mn, 1, iv, ctx->stream()));
+        }
+      }
+      ctx->sync();  // z buffers and the scale table leave scope
+    }
+  }
   // projectors: proju = (bb0^-1, new), projv = (bb1^-1, new')
   const Bond nb = Bond::make(bb0.dir, kept_q, kept_d);
   Bond nbv = nb;

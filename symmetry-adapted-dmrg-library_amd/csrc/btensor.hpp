@@ -107,6 +107,11 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
             std::vector<double>& wout, std::vector<int>* kept_per_sector = nullptr);
 void set_gram_min(int n);
 int gram_min();
+// Eigensolver of the Gram route: 1 (default) the hand-written batched solver
+// (eigen.hpp; unpadded, eigenvectors of the kept values only), 0 rocSOLVER's
+// strided-batched syevd on matrices padded to the batch's largest order.
+void set_svd_eigensolver(int v);
+int svd_eigensolver();
 // Process-wide truncation counters: op_svd calls, sectors decomposed through
 // the Gram route / by gesdd, Gram sectors redone by gesdd, wall seconds.
 struct SvdStats {

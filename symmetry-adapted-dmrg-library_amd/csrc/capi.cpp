@@ -5,6 +5,7 @@
 #include <memory>
 #include "btensor.hpp"
 #include "dmrg.hpp"
+#include "eigen.hpp"
 #include "heff.hpp"
 #include "lanczos.hpp"
 
@@ -618,6 +619,49 @@ int sadmrg_svd_stats(long long* counts, double* seconds, int reset) {
     }
     if (seconds) *seconds = st.seconds;
     if (reset) st = SvdStats{};
+  });
+}
+
+int sadmrg_eigh_batched(sadmrg_ctx* ctx, int count, const int* n, double* const* a, const int* lda,
+                        double* const* w, const int* k, double* const* z, const int* ldz) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (count < 0) fail(kErrInvalid, "eigh: count must be >= 0");
+    if (count == 0) return;
+    need(n, "n");
+    need(a, "a");
+    need(lda, "lda");
+    need(w, "w");
+    std::vector<EigJob> jobs(count);
+    for (int q = 0; q < count; ++q) {
+      EigJob& J = jobs[q];
+      if (n[q] < 0 || n[q] > SymEigBatch::max_order()) fail(kErrInvalid, "eigh: matrix order out of range");
+      if (n[q] > 0 && (a[q] == nullptr || w[q] == nullptr || lda[q] < n[q])) fail(kErrInvalid, "eigh: bad matrix");
+      J.a = a[q];
+      J.n = n[q];
+      J.lda = lda[q];
+      J.w = w[q];
+      J.k = k ? k[q] : 0;
+      if (J.k < 0 || J.k > J.n) fail(kErrInvalid, "eigh: k out of range");
+      if (J.k > 0) {
+        need(z, "z");
+        need(ldz, "ldz");
+        if (z[q] == nullptr || ldz[q] < J.n) fail(kErrInvalid, "eigh: bad eigenvector buffer");
+        J.z = z[q];
+        J.ldz = ldz[q];
+      }
+    }
+    SymEigBatch b(ctx->c.get());
+    b.eigenvalues(jobs);
+    b.top_vectors(jobs);
+  });
+}
+
+int sadmrg_svd_eigensolver(int value, int* previous) {
+  return guard([&] {
+    if (previous) *previous = svd_eigensolver();
+    if (value == 0 || value == 1) set_svd_eigensolver(value);
+    else if (value != -1) fail(kErrInvalid, "svd eigensolver: 0, 1 or -1");
   });
 }
 

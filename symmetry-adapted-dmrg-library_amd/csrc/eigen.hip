@@ -1,0 +1,975 @@
+// eigen.hip -- see eigen.hpp.
+#include "eigen.hpp"
+#include "kernels.hpp"
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace sadmrg {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// 1. Tridiagonalisation, one persistent cooperative launch for every matrix.
+//
+// Matrix q is served by a group of nwg workgroups; workgroup w holds rows
+// r = w + nwg * i (i < rows) of the full symmetric matrix in LDS.  Step k
+// (k = 0 .. n-2) of the reduction:
+//   row pass   every workgroup applies the rank-2 update of step k-1 to its
+//              rows (A -= v w^T + w v^T on the trailing block), computes its
+//              rows of p = tau_k A v_k, and reads its entries of column k+1;
+//   publish    those entries go to the group's buffer as tagged words: each
+//              double as two 64-bit words (one 32-bit half and the step tag
+//              k+1), each stored with one write-through store, so a reader
+//              sees either the old word or the whole new one;
+//   gather     every workgroup polls the buffer until all entries of step k
+//              carry tag k+1 (no atomics, no fence: the data is the signal);
+//   redundant  every workgroup computes, identically (same data, same
+//              operation order), alpha = v^T p, w = p - tau alpha v / 2,
+//              column k+1 of the updated matrix, d[k+1] and the next
+//              reflector (dlarfg: beta, tau, v).
+// Workgroup 0 of the group writes d, e, tau and the reflector vectors
+// (LAPACK layout: reflector i acts on rows i+1..n-1, v[i+1] = 1,
+// v[i+2..n-1] stored in column i below the subdiagonal).  The buffers of
+// even and odd steps are separate, so a workgroup that runs ahead cannot
+// overwrite entries a slower one still reads.  A poll that does not succeed
+// within kSpinLimit rounds sets the error flag and every workgroup ends.
+constexpr int kTrdThreads = 512;
+constexpr int kTrdWaves = kTrdThreads / 64;
+constexpr int kLdsBytes = 160 * 1024;
+constexpr int kTrdLdsDoubles = (kLdsBytes - 512) / 8;  // dynamic part; the rest is static shared
+constexpr long long kSpinLimit = 1LL << 22;
+
+struct TrdTask {
+  double* a;
+  double* d;
+  double* e;
+  double* tau;
+  uint64_t* pub;  // 2 parities x n rows x 4 tagged words {p lo, p hi, col lo, col hi}
+  int n, lda, wg0, nwg, rows;
+  int ldr;        // LDS stride of rows and vectors: n rounded up to a multiple of 64
+};
+
+// DPP move of a double (both halves), lanes without a source reading zero.
+template <int Ctrl, int RowMask = 0xf>
+__device__ inline double dpp_move(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), Ctrl, RowMask, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), Ctrl, RowMask, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave, in a fixed order, with DPP row shifts and
+// row broadcasts (no LDS crossbar); the result is returned to every lane.
+// Every lane of the wave must take part.
+__device__ inline double wave_sum(double x) {
+  x += dpp_move<0x111>(x);        // row_shr:1
+  x += dpp_move<0x112>(x);        // row_shr:2
+  x += dpp_move<0x114>(x);        // row_shr:4
+  x += dpp_move<0x118>(x);        // row_shr:8   (lane 15 of a row: the row's sum)
+  x += dpp_move<0x142, 0xa>(x);   // row_bcast:15 (lane 31: rows 0-1, lane 63: rows 2-3)
+  x += dpp_move<0x143, 0xc>(x);   // row_bcast:31 (lane 63: all rows)
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the workgroup; every thread returns the same value (the wave sums
+// are added in wave order).
+__device__ inline double block_sum(double x, double* red) {
+  x = wave_sum(x);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  double s = 0;
+  for (int i = 0; i < kTrdWaves; ++i) s += red[i];
+  return s;
+}
+
+// dlarfg on c[lo .. n): H (c[lo], c[lo+1..]) = (beta, 0).  c becomes v (v[lo] = 1);
+// returns beta, *tau = tau.
+__device__ double householder(double* c, int lo, int n, double* tau, double* red) {
+  double s = 0;
+  for (int j = lo + 1 + static_cast<int>(threadIdx.x); j < n; j += kTrdThreads) s += c[j] * c[j];
+  const double sig = block_sum(s, red);
+  const double x0 = c[lo];
+  double beta, t, scale;
+  if (sig == 0.0) {
+    beta = x0;
+    t = 0.0;
+    scale = 0.0;
+  } else {
+    beta = -copysign(sqrt(fma(x0, x0, sig)), x0);
+    t = (beta - x0) / beta;
+    scale = 1.0 / (x0 - beta);
+  }
+  __syncthreads();  // c[lo] read by every thread
+  for (int j = lo + 1 + static_cast<int>(threadIdx.x); j < n; j += kTrdThreads) c[j] *= scale;
+  if (threadIdx.x == 0) c[lo] = 1.0;
+  __syncthreads();
+  *tau = t;
+  return beta;
+}
+
+__device__ inline void put_tagged(uint64_t* p, double x, uint32_t tag) {
+  const uint64_t b = __double_as_longlong(x);
+  __hip_atomic_store(p, (b & 0xffffffffull) | (uint64_t(tag) << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, (b >> 32) | (uint64_t(tag) << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NT>
+__global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restrict__ tasks, int ntask,
+                                                          int* __restrict__ err) {
+  extern __shared__ double lds[];
+  __shared__ double red[kTrdWaves];
+  __shared__ int s_abort;
+  int t = 0;
+  while (t < ntask && static_cast<int>(blockIdx.x) >= tasks[t].wg0 + tasks[t].nwg) ++t;
+  if (t >= ntask) return;
+  const TrdTask T = tasks[t];
+  const int n = T.n, W = T.nwg, w = static_cast<int>(blockIdx.x) - T.wg0, lda = T.lda, R = T.rows;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double* A = T.a;
+  if (n == 1) {
+    if (w == 0 && tid == 0) T.d[0] = A[0];
+    return;
+  }
+  if (tid == 0) s_abort = 0;
+  // rows and vectors padded to ldr (a multiple of 64) entries, zero past n;
+  // the row pass loads every slice unconditionally (indices clamped to the
+  // padded row) and masks by value
+  const int ldr = T.ldr;
+  double* rows = lds;
+  double* v = rows + static_cast<size_t>(R) * ldr;
+  double* vp = v + ldr;
+  double* wp = vp + ldr;
+  // own rows, from the lower triangle: row r = column r below the diagonal,
+  // and A(r, j) for j < r
+  for (int i = 0; i < R; ++i) {
+    const int r = w + W * i;
+    for (int j = tid; j < ldr; j += kTrdThreads)
+      rows[static_cast<size_t>(i) * ldr + j] =
+          (r >= n || j >= n) ? 0.0 : (j >= r ? A[j + static_cast<size_t>(r) * lda] : A[r + static_cast<size_t>(j) * lda]);
+  }
+  for (int j = tid; j < 3 * ldr; j += kTrdThreads) v[j] = 0.0;
+  __syncthreads();
+  for (int j = 1 + tid; j < n; j += kTrdThreads) v[j] = A[j];
+  __syncthreads();
+  double tau;
+  const double beta0 = householder(v, 1, n, &tau, red);
+  if (w == 0 && tid == 0) {
+    T.d[0] = A[0];
+    T.e[0] = beta0;
+    T.tau[0] = tau;
+  }
+  for (int k = 0; k <= n - 2; ++k) {
+    uint64_t* pub = T.pub + static_cast<size_t>(k & 1) * 4 * n;
+    const uint32_t tag = static_cast<uint32_t>(k + 1);
+    // row pass: update k-1, p = tau A v, column k+1; this lane's slices of
+    // the three vectors are read once per step
+    {
+      double vs[NT], vps[NT], wps[NT];
+#pragma unroll
+      for (int m = 0; m < NT; ++m) {
+        const int j = lane + 64 * m;
+        const int jc = min(j, ldr - 1);
+        const bool on = j > k && j < n;
+        const double a = v[jc], b = vp[jc], c = wp[jc];
+        vs[m] = on ? a : 0.0;
+        vps[m] = (on && k > 0) ? b : 0.0;
+        wps[m] = (on && k > 0) ? c : 0.0;
+      }
+      for (int i = wv; i < R; i += kTrdWaves) {
+        const int r = w + W * i;
+        if (r >= n || r <= k) continue;
+        double* row = rows + static_cast<size_t>(i) * ldr;
+        const double vr = k > 0 ? vp[r] : 0.0, wr = k > 0 ? wp[r] : 0.0;
+        double a[NT];
+#pragma unroll
+        for (int m = 0; m < NT; ++m) a[m] = row[min(lane + 64 * m, ldr - 1)];
+        double acc = 0.0, cn = 0.0;
+#pragma unroll
+        for (int m = 0; m < NT; ++m) {
+          const int j = lane + 64 * m;
+          // vps, wps, vs are zero for j <= k and j >= n
+          const double x = fma(-wr, vps[m], fma(-vr, wps[m], a[m]));
+          acc = fma(x, vs[m], acc);
+          if (j == k + 1) cn = x;
+          a[m] = x;
+        }
+        if (k > 0) {
+#pragma unroll
+          for (int m = 0; m < NT; ++m) {
+            const int j = lane + 64 * m;
+            if (j > k && j < n) row[j] = a[m];
+          }
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) put_tagged(pub + 4 * static_cast<size_t>(r), tau * acc, tag);
+        if (lane == ((k + 1) & 63)) put_tagged(pub + 4 * static_cast<size_t>(r) + 2, cn, tag);
+      }
+    }
+    // reflector k to column k (every workgroup loaded its rows before it
+    // published step 0, and workgroup 0 is past its own gather of step 0 only
+    // after every workgroup published)
+    if (w == 0 && k > 0)
+      for (int j = k + 1 + tid; j < n; j += kTrdThreads) A[j + static_cast<size_t>(k - 1) * lda] = vp[j];
+    // gather p and column k+1 into the slots of step k-1's vectors
+    double* P = vp;
+    double* C = wp;
+    double s_alpha = 0.0;
+    __syncthreads();  // the row pass read vp, wp
+    {
+      // every entry this thread gathers is requested at once; entries whose
+      // tags are not yet current are requested again
+      constexpr int kMaxRows = (2048 + kTrdThreads - 1) / kTrdThreads;
+      uint64_t x[kMaxRows][4];
+      unsigned pending = 0;
+#pragma unroll
+      for (int c = 0; c < kMaxRows; ++c) {
+        const int j = k + 1 + tid + c * kTrdThreads;
+        if (j < n) pending |= 1u << c;
+      }
+      long long spins = 0;
+      while (pending) {
+#pragma unroll
+        for (int c = 0; c < kMaxRows; ++c)
+          if (pending & (1u << c)) {
+            const uint64_t* q = pub + 4 * static_cast<size_t>(k + 1 + tid + c * kTrdThreads);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[c][u] = __hip_atomic_load(q + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+#pragma unroll
+        for (int c = 0; c < kMaxRows; ++c)
+          if ((pending & (1u << c)) && (x[c][0] >> 32) == tag && (x[c][1] >> 32) == tag && (x[c][2] >> 32) == tag &&
+              (x[c][3] >> 32) == tag) {
+            const int j = k + 1 + tid + c * kTrdThreads;
+            const double pj = __longlong_as_double(static_cast<long long>((x[c][0] & 0xffffffffull) | (x[c][1] << 32)));
+            P[j] = pj;
+            C[j] = __longlong_as_double(static_cast<long long>((x[c][2] & 0xffffffffull) | (x[c][3] << 32)));
+            pending &= ~(1u << c);
+          }
+        if (!pending) break;
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          atomicExch(err, 1);
+          s_abort = 1;
+          break;
+        }
+      }
+    }
+    // alpha = v^T p, this thread's entries added in row order (not in the
+    // order they arrived: every workgroup must form bitwise the same sums);
+    // the block sum's barriers also publish the gathered entries
+    for (int j = k + 1 + tid; j < n; j += kTrdThreads) s_alpha = fma(v[j], P[j], s_alpha);
+    const double alpha = block_sum(s_alpha, red);
+    if (s_abort) return;
+    const double h = 0.5 * tau * alpha;
+    const double vk1 = v[k + 1], wk1 = fma(-h, vk1, P[k + 1]);
+    // w = p - h v, column k+1 of the updated matrix, and the squares the next
+    // reflector needs, in one pass over this thread's entries
+    // (entry k+1 of w is not stored: other threads read P[k+1] for wk1 in
+    // this pass, and the next update uses w only past k+1)
+    double sig = 0.0;
+    for (int j = k + 1 + tid; j < n; j += kTrdThreads) {
+      const double wj = fma(-h, v[j], P[j]);
+      if (j > k + 1) P[j] = wj;
+      const double cj = C[j] - v[j] * wk1 - wj * vk1;
+      C[j] = cj;
+      if (j > k + 2) sig = fma(cj, cj, sig);
+    }
+    sig = block_sum(sig, red);
+    if (w == 0 && tid == 0) T.d[k + 1] = C[k + 1];
+    if (k + 1 <= n - 2) {
+      // dlarfg on C[k+2 ..): beta, tau, v (v[k+2] = 1)
+      const int lo = k + 2;
+      const double x0 = C[lo];
+      double bn, tn, scale;
+      if (sig == 0.0) {
+        bn = x0;
+        tn = 0.0;
+        scale = 0.0;
+      } else {
+        bn = -copysign(sqrt(fma(x0, x0, sig)), x0);
+        tn = (bn - x0) / bn;
+        scale = 1.0 / (x0 - bn);
+      }
+      __syncthreads();  // every thread has read x0 = C[lo]
+      for (int j = k + 1 + tid; j < n; j += kTrdThreads) {
+        if (j > lo) C[j] *= scale;
+        else if (j == lo) C[j] = 1.0;
+      }
+      if (w == 0 && tid == 0) {
+        T.e[k + 1] = bn;
+        T.tau[k + 1] = tn;
+      }
+      tau = tn;
+    }
+    double* vn = C;
+    vp = v;
+    wp = P;
+    v = vn;
+    __syncthreads();
+  }
+  // reflectors 0 .. n-3 were stored at steps 1 .. n-2; reflector n-2 has
+  // length one and nothing to store
+}
+
+template <int NT>
+hipError_t launch_trd(const TrdTask* tasks, int ntask, int* err, int nwg, int lds, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(trd_kernel<NT>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kTrdLdsDoubles * 8);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  void* args[] = {&tasks, &ntask, &err};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(trd_kernel<NT>), dim3(nwg), dim3(kTrdThreads), args,
+                                    static_cast<unsigned>(lds), st);
+}
+
+// ---------------------------------------------------------------------------
+// 2. Eigenvalues of T by multisection on the Sturm count (the bisection of
+// LAPACK dstebz, 65-way): one wave per eigenvalue, lane l evaluates the
+// count at the l-th of 64 interior points of the current interval, and the
+// interval shrinks to the sub-interval where the count passes the index, a
+// factor 65 per round (nine rounds to working precision).
+struct BisTask {
+  const double* d;
+  const double* e;
+  double* w;
+  int n, off;
+};
+
+__device__ inline int sturm_count(const double* __restrict__ d, const double* __restrict__ e, int n, double x,
+                                  double pivmin) {
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q < 0.0;
+  double dn = n > 1 ? d[1] : 0.0, en = n > 1 ? e[0] : 0.0;
+  for (int j = 1; j < n; ++j) {
+    const double dj = dn, ej = en;
+    if (j + 1 < n) {  // operands of the next step, loaded ahead of the division chain
+      dn = d[j + 1];
+      en = e[j];
+    }
+    q = dj - x - ej * ej / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__ tasks, int ntask, int total) {
+  const int gid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // one wave per eigenvalue
+  const int lane = threadIdx.x & 63;
+  if (gid >= total) return;
+  int t = 0;
+  while (t + 1 < ntask && gid >= tasks[t + 1].off) ++t;
+  const BisTask T = tasks[t];
+  const int i = gid - T.off, n = T.n;
+  const double* d = T.d;
+  const double* e = T.e;
+  // Gershgorin interval and pivot threshold, lanes over the rows
+  double lo = INFINITY, hi = -INFINITY, emax2 = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const double r = (j > 0 ? fabs(e[j - 1]) : 0.0) + (j + 1 < n ? fabs(e[j]) : 0.0);
+    lo = fmin(lo, d[j] - r);
+    hi = fmax(hi, d[j] + r);
+    if (j + 1 < n) emax2 = fmax(emax2, e[j] * e[j]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+    emax2 = fmax(emax2, __shfl_xor(emax2, o, 64));
+  }
+  const double eps = DBL_EPSILON;
+  const double tnorm = fmax(fabs(lo), fabs(hi));
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  const double fudge = 2.1 * eps * tnorm * n + 4.2 * pivmin;
+  lo -= fudge;
+  hi += fudge;
+  const double abstol = eps * tnorm;
+  for (int it = 0; it < 64; ++it) {
+    if (hi - lo <= fmax(abstol, 2.0 * eps * fmax(fabs(lo), fabs(hi)))) break;
+    const double step = (hi - lo) / 65.0;
+    const double x = lo + step * (lane + 1);
+    const bool above = sturm_count(d, e, n, x, pivmin) > i;
+    const unsigned long long m = __ballot(above);
+    if (m == 0) {
+      lo = __shfl(x, 63, 64);
+    } else {
+      const int l = __ffsll(static_cast<long long>(m)) - 1;
+      const double nhi = __shfl(x, l, 64);
+      const double nlo = l > 0 ? __shfl(x, l - 1, 64) : lo;
+      lo = nlo;
+      hi = nhi;
+    }
+  }
+  if (lane == 0) T.w[i] = 0.5 * (lo + hi);
+}
+
+// ---------------------------------------------------------------------------
+// 3. Eigenvectors of T by inverse iteration (LAPACK dstein scheme): one
+// lane per cluster of eigenvalues that inverse iteration cannot separate
+// (neighbours closer than kClusterGap * |T|).  For each member, from the
+// largest: three solves of (T - lambda I) x = y from a seeded random start,
+// each followed by Gram-Schmidt against the cluster's earlier members and
+// normalisation.  Each solve runs the LU factorisation with partial pivoting
+// (dlagtf) fused with the forward elimination, then the back substitution.
+// Vectors of different clusters are orthogonal to about eps |T| / gap; the
+// Newton-Schulz steps after this kernel make the set orthonormal.  The
+// recurrences are sequential, so a lane's factors and iterate (4n doubles)
+// live in LDS, next to those of the wave's other lanes, and every step costs
+// LDS latency instead of an HBM round trip; a wave has as many active lanes
+// as its LDS holds (one wave per workgroup).
+constexpr double kClusterGap = 1e3 * DBL_EPSILON;
+constexpr int kInvLdsDoubles = kTrdLdsDoubles;
+
+struct InvTask {
+  const double* d;
+  const double* e;
+  const double* w;
+  double* z;
+  int n, ldz, c0, m;  // members c0 .. c0+m-1 (column c <-> w[n-1-c])
+};
+
+struct InvWave {
+  int task0, ntask, lanes, n;  // tasks [task0, task0 + ntask), lanes = LDS slots of the wave
+};
+
+__device__ inline double seeded_uniform(uint64_t x) {
+  // splitmix64 -> [-1, 1)
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return static_cast<double>(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+__global__ __launch_bounds__(64) void invit_kernel(const InvTask* __restrict__ tasks,
+                                                   const InvWave* __restrict__ waves) {
+  extern __shared__ double lds[];
+  const InvWave W = waves[blockIdx.x];
+  const int lane = threadIdx.x;
+  if (lane >= W.ntask) return;
+  const InvTask T = tasks[W.task0 + lane];
+  const int n = T.n, L = W.lanes;
+  const double* d = T.d;
+  const double* e = T.e;
+  auto at = [&](int arr, int j) -> double& { return lds[(static_cast<size_t>(arr) * n + j) * L + lane]; };
+  constexpr int U0 = 0, U1 = 1, U2 = 2, Y = 3;
+  const double tnorm = fmax(fabs(T.w[0]), fabs(T.w[n - 1]));
+  const double tolp = fmax(DBL_EPSILON * tnorm, DBL_MIN);
+  for (int a = 0; a < T.m; ++a) {
+    const int c = T.c0 + a;
+    const double lam = T.w[n - 1 - c];
+    const uint64_t seed = (static_cast<uint64_t>(n) << 44) ^ (static_cast<uint64_t>(c) << 24);
+    for (int j = 0; j < n; ++j) at(Y, j) = seeded_uniform(seed + j);
+    for (int it = 0; it < 3; ++it) {
+      // factorise T - lambda I = P L U and apply L^-1 P to y in one sweep
+      double aj = d[0] - lam, bj = n > 1 ? e[0] : 0.0, cur = at(Y, 0);
+      double en = n > 1 ? e[0] : 0.0, dn = n > 1 ? d[1] : 0.0, en1 = n > 2 ? e[1] : 0.0;
+      for (int j = 0; j + 1 < n; ++j) {
+        const double cj = en, an = dn - lam, bn = en1;
+        if (j + 2 < n) {  // next step's operands, ahead of the division chain
+          en = en1;
+          dn = d[j + 2];
+          en1 = j + 3 < n ? e[j + 2] : 0.0;
+        }
+        const double y1 = at(Y, j + 1);
+        double mm, top, bot;
+        if (fabs(aj) >= fabs(cj)) {
+          mm = aj != 0.0 ? cj / aj : 0.0;
+          at(U0, j) = aj;
+          at(U1, j) = bj;
+          at(U2, j) = 0.0;
+          top = cur;
+          bot = y1;
+          aj = an - mm * bj;
+          bj = bn;
+        } else {
+          mm = aj / cj;
+          at(U0, j) = cj;
+          at(U1, j) = an;
+          at(U2, j) = bn;
+          top = y1;
+          bot = cur;
+          aj = bj - mm * an;
+          bj = -mm * bn;
+        }
+        at(Y, j) = top;
+        cur = bot - mm * top;
+      }
+      at(U0, n - 1) = aj;
+      at(Y, n - 1) = cur;
+      // back substitution (tiny pivots replaced by +-eps |T|), rescaling when
+      // the iterate grows large
+      double x1 = 0.0, x2 = 0.0;
+      for (int j = n - 1; j >= 0; --j) {
+        double r = at(Y, j);
+        if (j + 1 < n) r -= at(U1, j) * x1;
+        if (j + 2 < n) r -= at(U2, j) * x2;
+        double u = at(U0, j);
+        if (fabs(u) < tolp) u = u < 0.0 ? -tolp : tolp;
+        double x = r / u;
+        if (fabs(x) > 1e150) {
+          for (int i = j + 1; i < n; ++i) at(Y, i) *= 1e-150;
+          x *= 1e-150;
+          x1 *= 1e-150;
+          x2 *= 1e-150;
+        }
+        at(Y, j) = x;
+        x2 = x1;
+        x1 = x;
+      }
+      // Gram-Schmidt against the cluster's earlier members, then normalise
+      for (int b = 0; b < a; ++b) {
+        const double* zb = T.z + static_cast<size_t>(T.c0 + b) * T.ldz;
+        double dt = 0.0;
+        for (int j = 0; j < n; ++j) dt = fma(zb[j], at(Y, j), dt);
+        for (int j = 0; j < n; ++j) at(Y, j) = fma(-dt, zb[j], at(Y, j));
+      }
+      double nrm = 0.0;
+      for (int j = 0; j < n; ++j) nrm = fma(at(Y, j), at(Y, j), nrm);
+      const double s = 1.0 / sqrt(nrm);
+      for (int j = 0; j < n; ++j) at(Y, j) *= s;
+    }
+    double* zc = T.z + static_cast<size_t>(c) * T.ldz;
+    for (int j = 0; j < n; ++j) zc[j] = at(Y, j);
+  }
+}
+
+// B = 1.5 I - 0.5 M on every k x k slot (M = Z^T Z): one Newton-Schulz step
+// Z <- Z B towards the orthonormal polar factor of Z.
+__global__ __launch_bounds__(256) void ns_coef_kernel(const double* __restrict__ M, double* __restrict__ B, int k,
+                                                      int64_t total) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int64_t r = i % (int64_t(k) * k);
+  const double diag = (r % k) == (r / k) ? 1.5 : 0.0;
+  B[i] = diag - 0.5 * M[i];
+}
+
+// max over every slot's leading k_s x k_s block of |M - I| (one workgroup per slot)
+__global__ __launch_bounds__(256) void ortho_err_kernel(const double* __restrict__ M, const int* __restrict__ ks,
+                                                        int k, double* __restrict__ out) {
+  const int s = blockIdx.x;
+  const int kk = ks[s];
+  const double* Ms = M + static_cast<size_t>(s) * k * k;
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < int64_t(kk) * kk; i += 256) {
+    const int r = static_cast<int>(i % kk), c = static_cast<int>(i / kk);
+    m = fmax(m, fabs(Ms[r + static_cast<size_t>(c) * k] - (r == c ? 1.0 : 0.0)));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) out[s] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// ---------------------------------------------------------------------------
+// 4. Back-transformation Z <- H_0 H_1 ... H_{n-2} Z.  One wave per pair of
+// columns, held in registers (lane l keeps rows l + 64 m, m < NT); the
+// reflectors are applied from the last to the first.
+struct BtTask {
+  const double* a;
+  const double* tau;
+  double* z;
+  int n, lda, ldz, k, blk0, nblk;
+};
+
+template <int NT>
+__global__ __launch_bounds__(256) void bt_kernel(const BtTask* __restrict__ tasks, int ntask) {
+  int t = 0;
+  while (t < ntask && static_cast<int>(blockIdx.x) >= tasks[t].blk0 + tasks[t].nblk) ++t;
+  if (t >= ntask) return;
+  const BtTask T = tasks[t];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int c0 = ((static_cast<int>(blockIdx.x) - T.blk0) * 4 + wv) * 2;
+  if (c0 >= T.k) return;
+  const bool two = c0 + 1 < T.k;
+  const int n = T.n;
+  double z0[NT], z1[NT];
+  const double* Z0 = T.z + static_cast<size_t>(c0) * T.ldz;
+  const double* Z1 = Z0 + T.ldz;
+#pragma unroll
+  for (int m = 0; m < NT; ++m) {
+    const int j = lane + 64 * m;
+    z0[m] = j < n ? Z0[j] : 0.0;
+    z1[m] = (two && j < n) ? Z1[j] : 0.0;
+  }
+  // reflector i's vector, loaded one reflector ahead
+  auto load_v = [&](int i, double (&vv)[NT]) {
+    const double* col = T.a + static_cast<size_t>(i) * T.lda;
+#pragma unroll
+    for (int m = 0; m < NT; ++m) {
+      const int j = lane + 64 * m;
+      vv[m] = j == i + 1 ? 1.0 : ((j > i + 1 && j < n) ? col[j] : 0.0);
+    }
+  };
+  double vn[NT];
+  if (n >= 2) load_v(n - 2, vn);
+  for (int i = n - 2; i >= 0; --i) {
+    double vv[NT];
+#pragma unroll
+    for (int m = 0; m < NT; ++m) vv[m] = vn[m];
+    if (i > 0) load_v(i - 1, vn);
+    const double ti = T.tau[i];
+    if (ti == 0.0) continue;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < NT; ++m) {
+      s0 = fma(vv[m], z0[m], s0);
+      s1 = fma(vv[m], z1[m], s1);
+    }
+    s0 = __shfl(wave_sum(s0), 0, 64) * ti;
+    s1 = __shfl(wave_sum(s1), 0, 64) * ti;
+#pragma unroll
+    for (int m = 0; m < NT; ++m) {
+      z0[m] = fma(-s0, vv[m], z0[m]);
+      z1[m] = fma(-s1, vv[m], z1[m]);
+    }
+  }
+  double* W0 = T.z + static_cast<size_t>(c0) * T.ldz;
+  double* W1 = W0 + T.ldz;
+#pragma unroll
+  for (int m = 0; m < NT; ++m) {
+    const int j = lane + 64 * m;
+    if (j < n) {
+      W0[j] = z0[m];
+      if (two) W1[j] = z1[m];
+    }
+  }
+}
+
+template <int NT>
+hipError_t launch_bt(const BtTask* tasks, int ntask, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(bt_kernel<NT>, dim3(nblk), dim3(256), 0, st, tasks, ntask);
+  return hipGetLastError();
+}
+
+std::atomic<long long> g_cholqr_fallbacks{0};
+
+int device_cus() {
+  static int cus = [] {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return 0;
+    return p.multiProcessorCount;
+  }();
+  return cus;
+}
+
+}  // namespace
+
+long long SymEigBatch::cholqr_fallbacks() { return g_cholqr_fallbacks; }
+
+int SymEigBatch::max_order() {
+  // rows of at least one matrix row plus three vectors in LDS, and at most
+  // 64 * 32 rows in the back-transformation's registers
+  return 2048;
+}
+
+void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
+  Context* ctx = ctx_;
+  const int nj = static_cast<int>(jobs.size());
+  doff_.assign(nj + 1, 0);
+  int64_t pub_tot = 0;
+  for (int q = 0; q < nj; ++q) {
+    require(jobs[q].n >= 0 && jobs[q].n <= max_order(), "eigensolver: matrix order out of range");
+    require(jobs[q].n == 0 || (jobs[q].a && jobs[q].w && jobs[q].lda >= jobs[q].n), "eigensolver: bad job");
+    doff_[q + 1] = doff_[q] + 3 * int64_t(jobs[q].n);
+    pub_tot += 4 * int64_t(jobs[q].n);
+  }
+  trd_ = DevBuf(ctx, std::max<int64_t>(doff_[nj], 1) * 8);
+  DevBuf pubs(ctx, std::max<int64_t>(2 * pub_tot, 1) * 8), errb(ctx, sizeof(int));
+  SADMRG_HIP(hipMemsetAsync(pubs.get(), 0, std::max<int64_t>(2 * pub_tot, 1) * 8, ctx->stream()));
+  SADMRG_HIP(hipMemsetAsync(errb.get(), 0, sizeof(int), ctx->stream()));
+  int* err = errb.as<int>();
+  double* base = trd_.as<double>();
+  // workgroups per matrix: as many rows per workgroup as LDS holds next to
+  // three vectors, rows spread evenly
+  struct Plan {
+    int q, W, R;
+  };
+  std::vector<Plan> plans;
+  for (int q = 0; q < nj; ++q) {
+    const int n = jobs[q].n;
+    if (n == 0) continue;
+    const int ldr = (n + 63) / 64 * 64;
+    const int rmax = (kTrdLdsDoubles - 3 * ldr) / ldr;
+    require(rmax >= 1, "eigensolver: matrix too large for the LDS-resident reduction");
+    const int W = (n + rmax - 1) / rmax;
+    plans.push_back(Plan{q, W, (n + W - 1) / W});
+  }
+  std::stable_sort(plans.begin(), plans.end(), [](const Plan& x, const Plan& y) { return x.W > y.W; });
+  const int cus = device_cus();
+  require(cus > 0, "eigensolver: no device properties");
+  // launches of at most `cus` co-resident workgroups
+  int64_t pub_off = 0;
+  std::vector<int64_t> pub_of(nj, 0);
+  for (int q = 0; q < nj; ++q) {
+    pub_of[q] = pub_off;
+    pub_off += 8 * int64_t(jobs[q].n);
+  }
+  for (size_t p0 = 0; p0 < plans.size();) {
+    std::vector<TrdTask> tasks;
+    int wg = 0, lds = 0;
+    size_t p1 = p0;
+    while (p1 < plans.size() && (wg + plans[p1].W <= cus || tasks.empty())) {
+      const Plan& P = plans[p1];
+      require(P.W <= cus, "eigensolver: matrix needs more workgroups than the device holds");
+      const EigJob& J = jobs[P.q];
+      TrdTask t;
+      t.a = J.a;
+      t.d = base + doff_[P.q];
+      t.e = t.d + J.n;
+      t.tau = t.e + J.n;
+      t.pub = pubs.as<uint64_t>() + pub_of[P.q];
+      t.n = J.n;
+      t.lda = J.lda;
+      t.wg0 = wg;
+      t.nwg = P.W;
+      t.rows = P.R;
+      t.ldr = (J.n + 63) / 64 * 64;
+      tasks.push_back(t);
+      wg += P.W;
+      lds = std::max(lds, (P.R + 3) * t.ldr * 8);
+      ++p1;
+    }
+    DevBuf dt(ctx, tasks.size() * sizeof(TrdTask));
+    SADMRG_HIP(hipMemcpyAsync(dt.get(), tasks.data(), tasks.size() * sizeof(TrdTask), hipMemcpyHostToDevice,
+                              ctx->stream()));
+    const TrdTask* tp = dt.as<TrdTask>();
+    const int ntask = static_cast<int>(tasks.size());
+    int nmx = 0;
+    for (const TrdTask& x : tasks) nmx = std::max(nmx, x.n);
+    const int nt = (nmx + 63) / 64;
+    hipError_t le;
+    if (nt <= 4) le = launch_trd<4>(tp, ntask, err, wg, lds, ctx->stream());
+    else if (nt <= 8) le = launch_trd<8>(tp, ntask, err, wg, lds, ctx->stream());
+    else if (nt <= 12) le = launch_trd<12>(tp, ntask, err, wg, lds, ctx->stream());
+    else if (nt <= 16) le = launch_trd<16>(tp, ntask, err, wg, lds, ctx->stream());
+    else if (nt <= 20) le = launch_trd<20>(tp, ntask, err, wg, lds, ctx->stream());
+    else if (nt <= 24) le = launch_trd<24>(tp, ntask, err, wg, lds, ctx->stream());
+    else le = launch_trd<32>(tp, ntask, err, wg, lds, ctx->stream());
+    SADMRG_HIP(le);
+    ctx->sync();  // the task table leaves scope
+    p0 = p1;
+  }
+  int herr = 0;
+  SADMRG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
+  ctx->sync();
+  if (herr) fail(kErrNumerics, "eigensolver: a tridiagonalisation step did not receive its group's data");
+  // eigenvalues
+  std::vector<BisTask> bt;
+  int total = 0;
+  for (int q = 0; q < nj; ++q) {
+    if (jobs[q].n == 0) continue;
+    BisTask b;
+    b.d = base + doff_[q];
+    b.e = b.d + jobs[q].n;
+    b.w = jobs[q].w;
+    b.n = jobs[q].n;
+    b.off = total;
+    bt.push_back(b);
+    total += jobs[q].n;
+  }
+  if (total == 0) return;
+  DevBuf db(ctx, bt.size() * sizeof(BisTask));
+  SADMRG_HIP(hipMemcpyAsync(db.get(), bt.data(), bt.size() * sizeof(BisTask), hipMemcpyHostToDevice, ctx->stream()));
+  hipLaunchKernelGGL(bisect_kernel, dim3((total + 3) / 4), dim3(256), 0, ctx->stream(), db.as<BisTask>(),
+                     static_cast<int>(bt.size()), total);
+  SADMRG_HIP(hipGetLastError());
+  ctx->sync();
+}
+
+void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
+  Context* ctx = ctx_;
+  const int nj = static_cast<int>(jobs.size());
+  require(static_cast<int>(doff_.size()) == nj + 1, "eigensolver: top_vectors before eigenvalues");
+  double* base = trd_.as<double>();
+  // the jobs asking for vectors, in one padded batch: slot s holds an
+  // nmax x kmax block, rows >= n and columns >= k zero
+  std::vector<int> sel;
+  int nmax = 0, kmax = 0;
+  for (int q = 0; q < nj; ++q) {
+    const EigJob& J = jobs[q];
+    if (J.n == 0 || J.k <= 0) continue;
+    require(J.k <= J.n && J.z && J.ldz >= J.n, "eigensolver: bad eigenvector request");
+    sel.push_back(q);
+    nmax = std::max(nmax, J.n);
+    kmax = std::max(kmax, J.k);
+  }
+  const int ns = static_cast<int>(sel.size());
+  if (ns == 0) return;
+  const int64_t slot = int64_t(nmax) * kmax;
+  DevBuf zp(ctx, slot * ns * 8), gm(ctx, int64_t(kmax) * kmax * ns * 8), dinfo(ctx, sizeof(int) * 2 * ns);
+  SADMRG_HIP(launch_fill(slot * ns, 0.0, zp.as<double>(), ctx->stream()));
+  SADMRG_HIP(hipMemsetAsync(dinfo.get(), 0, sizeof(int) * 2 * ns, ctx->stream()));
+  // clusters of eigenvalues closer than kClusterGap |T| (host copy of w)
+  std::vector<InvTask> it;
+  for (int s = 0; s < ns; ++s) {
+    const EigJob& J = jobs[sel[s]];
+    std::vector<double> w(J.n);
+    SADMRG_HIP(hipMemcpyAsync(w.data(), J.w, J.n * 8, hipMemcpyDeviceToHost, ctx->stream()));
+    ctx->sync();
+    const double tn = std::max(std::fabs(w[0]), std::fabs(w[J.n - 1]));
+    InvTask v;
+    v.d = base + doff_[sel[s]];
+    v.e = v.d + J.n;
+    v.w = J.w;
+    v.z = zp.as<double>() + slot * s;
+    v.n = J.n;
+    v.ldz = nmax;
+    for (int c = 0; c < J.k;) {
+      int m = 1;
+      while (c + m < J.k && w[J.n - 1 - (c + m - 1)] - w[J.n - 1 - (c + m)] <= kClusterGap * tn) ++m;
+      v.c0 = c;
+      v.m = m;
+      it.push_back(v);
+      c += m;
+    }
+  }
+  {
+    // waves of lanes with the same order: as many lanes as LDS holds 4n doubles
+    std::vector<InvWave> wv;
+    int lds = 0;
+    for (size_t t0 = 0; t0 < it.size();) {
+      const int n = it[t0].n;
+      const int lanes = std::max(1, std::min(64, kInvLdsDoubles / (4 * n)));
+      int cnt = 0;
+      while (t0 + cnt < it.size() && cnt < lanes && it[t0 + cnt].n == n && it[t0 + cnt].z == it[t0].z) ++cnt;
+      wv.push_back(InvWave{static_cast<int>(t0), cnt, lanes, n});
+      lds = std::max(lds, lanes * 4 * n * 8);
+      t0 += cnt;
+    }
+    static bool attr = false;
+    if (!attr) {
+      SADMRG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(invit_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, kInvLdsDoubles * 8));
+      attr = true;
+    }
+    DevBuf dt(ctx, it.size() * sizeof(InvTask)), dw(ctx, wv.size() * sizeof(InvWave));
+    SADMRG_HIP(
+        hipMemcpyAsync(dt.get(), it.data(), it.size() * sizeof(InvTask), hipMemcpyHostToDevice, ctx->stream()));
+    SADMRG_HIP(
+        hipMemcpyAsync(dw.get(), wv.data(), wv.size() * sizeof(InvWave), hipMemcpyHostToDevice, ctx->stream()));
+    hipLaunchKernelGGL(invit_kernel, dim3(wv.size()), dim3(64), static_cast<unsigned>(lds), ctx->stream(),
+                       dt.as<InvTask>(), dw.as<InvWave>());
+    SADMRG_HIP(hipGetLastError());
+    ctx->sync();
+  }
+  // Newton-Schulz steps Z <- Z (3 I - Z^T Z) / 2 over the whole batch (the
+  // zero padding columns stay zero) until every slot is orthonormal to
+  // kOrthoTol; Cholesky QR twice if that does not happen within kNsMax steps
+  rocblas_handle h = static_cast<rocblas_handle>(ctx->solver());
+  const double one = 1.0, zero = 0.0;
+  const int64_t sg = int64_t(kmax) * kmax;
+  constexpr double kOrthoTol = 1e-13;
+  constexpr int kNsMax = 6;
+  DevBuf zq(ctx, slot * ns * 8), bm(ctx, sg * ns * 8), derr(ctx, ns * 8), dks(ctx, sizeof(int) * ns);
+  {
+    std::vector<int> hk(ns);
+    for (int s = 0; s < ns; ++s) hk[s] = jobs[sel[s]].k;
+    SADMRG_HIP(hipMemcpyAsync(dks.get(), hk.data(), sizeof(int) * ns, hipMemcpyHostToDevice, ctx->stream()));
+  }
+  double* zcur = zp.as<double>();
+  double* znext = zq.as<double>();
+  bool ortho = false;
+  for (int step = 0; step <= kNsMax && !ortho; ++step) {
+    if (rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, kmax, kmax, nmax, &one,
+                                      zcur, nmax, slot, zcur, nmax, slot, &zero, gm.as<double>(), kmax, sg, ns) !=
+        rocblas_status_success)
+      fail(kErrNumerics, "eigensolver: rocblas gemm failed");
+    hipLaunchKernelGGL(ortho_err_kernel, dim3(ns), dim3(256), 0, ctx->stream(), gm.as<double>(), dks.as<int>(), kmax,
+                       derr.as<double>());
+    SADMRG_HIP(hipGetLastError());
+    std::vector<double> herr(ns);
+    SADMRG_HIP(hipMemcpyAsync(herr.data(), derr.get(), ns * 8, hipMemcpyDeviceToHost, ctx->stream()));
+    ctx->sync();
+    ortho = *std::max_element(herr.begin(), herr.end()) <= kOrthoTol;
+    if (ortho || step == kNsMax) break;
+    hipLaunchKernelGGL(ns_coef_kernel, dim3((sg * ns + 255) / 256), dim3(256), 0, ctx->stream(), gm.as<double>(),
+                       bm.as<double>(), kmax, sg * ns);
+    SADMRG_HIP(hipGetLastError());
+    if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nmax, kmax, kmax, &one, zcur,
+                                      nmax, slot, bm.as<double>(), kmax, sg, &zero, znext, nmax, slot, ns) !=
+        rocblas_status_success)
+      fail(kErrNumerics, "eigensolver: rocblas gemm failed");
+    std::swap(zcur, znext);
+  }
+  if (!ortho) {
+    for (int pass = 0; pass < 2; ++pass) {
+      if (rocblas_dsyrk_strided_batched(h, rocblas_fill_lower, rocblas_operation_transpose, kmax, nmax, &one, zcur,
+                                        nmax, slot, &zero, gm.as<double>(), kmax, sg, ns) != rocblas_status_success)
+        fail(kErrNumerics, "eigensolver: rocblas syrk failed");
+      for (int s = 0; s < ns; ++s) {
+        const int k = jobs[sel[s]].k;
+        if (k < kmax)
+          SADMRG_HIP(launch_strided_fill(kmax - k, 1.0, gm.as<double>() + sg * s + int64_t(k) * (kmax + 1),
+                                         kmax + 1, ctx->stream()));
+      }
+      if (rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, kmax, gm.as<double>(), kmax, sg,
+                                           dinfo.as<int>() + pass * ns, ns) != rocblas_status_success)
+        fail(kErrNumerics, "eigensolver: rocsolver potrf failed");
+      if (rocblas_dtrsm_strided_batched(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                        rocblas_diagonal_non_unit, nmax, kmax, &one, gm.as<double>(), kmax, sg, zcur,
+                                        nmax, slot, ns) != rocblas_status_success)
+        fail(kErrNumerics, "eigensolver: rocblas trsm failed");
+    }
+  }
+  g_cholqr_fallbacks += ortho ? 0 : 1;
+  std::vector<int> hinfo(2 * ns);
+  SADMRG_HIP(hipMemcpyAsync(hinfo.data(), dinfo.get(), sizeof(int) * hinfo.size(), hipMemcpyDeviceToHost,
+                            ctx->stream()));
+  // back-transformation on the padded slots, then the k columns to each job
+  std::vector<BtTask> bts;
+  int blk = 0;
+  for (int s = 0; s < ns; ++s) {
+    const EigJob& J = jobs[sel[s]];
+    BtTask b;
+    b.a = J.a;
+    b.tau = base + doff_[sel[s]] + 2 * int64_t(J.n);
+    b.z = zcur + slot * s;
+    b.n = J.n;
+    b.lda = J.lda;
+    b.ldz = nmax;
+    b.k = J.k;
+    b.blk0 = blk;
+    b.nblk = (J.k + 7) / 8;
+    bts.push_back(b);
+    blk += b.nblk;
+  }
+  DevBuf dbt(ctx, bts.size() * sizeof(BtTask));
+  SADMRG_HIP(hipMemcpyAsync(dbt.get(), bts.data(), bts.size() * sizeof(BtTask), hipMemcpyHostToDevice, ctx->stream()));
+  ctx->sync();
+  for (int x : hinfo)
+    if (x != 0) fail(kErrNumerics, "eigensolver: eigenvectors not independent (Cholesky QR failed)");
+  const int nt = (nmax + 63) / 64;
+  const BtTask* tp = dbt.as<BtTask>();
+  const int ntask = static_cast<int>(bts.size());
+  hipError_t e;
+  if (nt <= 4) e = launch_bt<4>(tp, ntask, blk, ctx->stream());
+  else if (nt <= 8) e = launch_bt<8>(tp, ntask, blk, ctx->stream());
+  else if (nt <= 12) e = launch_bt<12>(tp, ntask, blk, ctx->stream());
+  else if (nt <= 16) e = launch_bt<16>(tp, ntask, blk, ctx->stream());
+  else if (nt <= 20) e = launch_bt<20>(tp, ntask, blk, ctx->stream());
+  else if (nt <= 24) e = launch_bt<24>(tp, ntask, blk, ctx->stream());
+  else e = launch_bt<32>(tp, ntask, blk, ctx->stream());
+  SADMRG_HIP(e);
+  for (int s = 0; s < ns; ++s) {
+    const EigJob& J = jobs[sel[s]];
+    SADMRG_HIP(launch_copy2d(J.n, J.k, zcur + slot * s, 1, nmax, J.z, J.ldz, 1, 1.0, nullptr, 0, ctx->stream()));
+  }
+  ctx->sync();
+}
+
+}  // namespace sadmrg

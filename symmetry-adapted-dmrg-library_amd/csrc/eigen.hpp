@@ -1,0 +1,58 @@
+// eigen.hpp -- batched symmetric eigensolver of the truncation's Gram route.
+//
+// The Gram route of BlockTensor::svd (block_tensor_inst.cpp:501-637,
+// dense_inst.cpp:447-541) needs, per sector of one truncation step, every
+// eigenvalue of a symmetric Gram matrix (for the global selection) and the
+// eigenvectors of the kept (largest) ones only.  The reference obtains them
+// from gesvd (dense_inst.cpp:486).  Here the matrices of one step are solved
+// together:
+//
+//   1. tridiagonalisation Q^T G Q = T by Householder reflectors (the LAPACK
+//      dsytd2 'L' scheme), all matrices in one persistent cooperative launch:
+//      the rows of each matrix stay in the LDS of a group of workgroups, one
+//      group barrier per column;
+//   2. every eigenvalue of T by Sturm-sequence multisection (one wave each);
+//   3. eigenvectors of T for the k largest eigenvalues by inverse iteration
+//      with partial-pivoting LU of T - lambda I (one thread per cluster of
+//      eigenvalues closer than inverse iteration separates, Gram-Schmidt
+//      inside a cluster), made orthonormal by Newton-Schulz steps (GEMMs);
+//   4. back-transformation V = Q Z (the reflectors applied in reverse order).
+//
+// Phase 1 (steps 1-2) and phase 2 (steps 3-4) are separate calls, so the
+// caller can choose k from the eigenvalues of every matrix in between.
+#pragma once
+#include <cstdint>
+#include <vector>
+#include "core.hpp"
+
+namespace sadmrg {
+
+struct EigJob {
+  double* a = nullptr;  // n x n, column-major, leading dimension lda; lower triangle read, then overwritten
+  int n = 0, lda = 0;
+  double* w = nullptr;  // n eigenvalues, ascending (device)
+  int k = 0;            // phase 2: eigenvectors of the k largest eigenvalues
+  double* z = nullptr;  // phase 2: n x k, leading dimension ldz; column c belongs to w[n - 1 - c]
+  int ldz = 0;
+};
+
+class SymEigBatch {
+ public:
+  explicit SymEigBatch(Context* ctx) : ctx_(ctx) {}
+  // Phase 1: tridiagonalises every job's matrix and writes all eigenvalues.
+  void eigenvalues(const std::vector<EigJob>& jobs);
+  // Phase 2 (after phase 1, same jobs, k and z set): eigenvectors.
+  void top_vectors(const std::vector<EigJob>& jobs);
+  // Largest matrix order the tridiagonalisation holds in LDS.
+  static int max_order();
+  // Phase-2 batches whose Newton-Schulz orthonormalisation did not converge
+  // and fell back to Cholesky QR (process-wide count).
+  static long long cholqr_fallbacks();
+
+ private:
+  Context* ctx_;
+  std::vector<int64_t> doff_;  // per job: offset of d, e, tau (3n doubles) in trd_
+  DevBuf trd_;
+};
+
+}  // namespace sadmrg
