@@ -301,6 +301,10 @@ int sadmrg_dmrg_set_refinement(sadmrg_dmrg* d, int krylov_steps, int max_restart
    reference checks every 4th (dmrg_inst.cpp:1334), the default.  A shorter
    interval lets the mixed path stop with fewer FP32 matvecs. */
 int sadmrg_dmrg_set_check_interval(sadmrg_dmrg* d, int every);
+/* 1: one line per sweep step on stderr (bond, energy, kept states, stage
+   times), as the reference prints every step of DMRG::sweep to cout
+   (dmrg_inst.cpp:881-897); 0 (default): silent. */
+int sadmrg_dmrg_set_verbose(sadmrg_dmrg* d, int level);
 /* Checkpoint files in the reference's format (dmrg_inst.cpp:947-1177), in
    directory dir (NULL or "" = the current directory, where the reference
    writes): uu-{ns}-{bondd}-{totspin}-{exci}-{i}.dat / .bin per site

@@ -117,6 +117,7 @@ def lib():
         "sadmrg_dmrg_set_precision": (i, [vp, i, i, d]),
         "sadmrg_dmrg_set_refinement": (i, [vp, i, i]),
         "sadmrg_dmrg_set_check_interval": (i, [vp, i]),
+        "sadmrg_dmrg_set_verbose": (i, [vp, i]),
         "sadmrg_dmrg_save_mps": (i, [vp, C.c_char_p, i]),
         "sadmrg_dmrg_save_ww": (i, [vp, C.c_char_p]),
         "sadmrg_dmrg_save_enr": (i, [vp, C.c_char_p]),
@@ -612,6 +613,10 @@ class DMRG:
 
     def set_refinement(self, krylov_steps=6, max_restarts=6):
         check(lib().sadmrg_dmrg_set_refinement(self.ptr, krylov_steps, max_restarts))
+
+    def set_verbose(self, level=1):
+        """One line per sweep step on stderr (sadmrg_dmrg_set_verbose)."""
+        check(lib().sadmrg_dmrg_set_verbose(self.ptr, level))
 
     def set_check_interval(self, every=4):
         """Lanczos convergence check after every `every`-th step (reference: 4)."""

@@ -1679,6 +1679,9 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
     std::stable_sort(list.begin(), list.end(), [&](size_t x, size_t y) {
       return std::max(secs[x].m, secs[x].n) > std::max(secs[y].m, secs[y].n);
     });
+    const auto td = std::chrono::steady_clock::now();
+    if (!list.empty())
+      g_svd_stats.max_sector = std::max<int64_t>(g_svd_stats.max_sector, std::max(secs[list[0]].m, secs[list[0]].n));
     if (allow_gram && gram && !list.empty() && std::max(secs[list[0]].m, secs[list[0]].n) >= gmin &&
         svd_eigensolver() == 1) {
       bool fits = true;
@@ -1686,10 +1689,15 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
       if (fits) {
         eig_batch(list);
         g_svd_stats.gram_sectors += static_cast<int64_t>(list.size());
+        g_svd_stats.t_gram += std::chrono::duration<double>(std::chrono::steady_clock::now() - td).count();
         return;
       }
     }
     if (allow_gram && gram && !list.empty() && std::max(secs[list[0]].m, secs[list[0]].n) >= gmin) {
+      struct Tm {
+        std::chrono::steady_clock::time_point t0;
+        ~Tm() { g_svd_stats.t_gram += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+      } tm{td};
       for (size_t k0 = 0; k0 < list.size(); k0 += kMaxGramBatch) {
         std::vector<size_t> part(list.begin() + k0, list.begin() + std::min(list.size(), k0 + kMaxGramBatch));
         int mp = 0, np = 0;
@@ -1719,6 +1727,8 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
       g_svd_stats.svd_sectors += static_cast<int64_t>(part.size());
       k0 = k1;
     }
+    ctx->sync();
+    g_svd_stats.t_gesdd += std::chrono::duration<double>(std::chrono::steady_clock::now() - td).count();
   };
   // S of every sector on the host; padded sectors whose largest value is
   // round-off and Gram sectors without a resolved value are redone unpadded.
@@ -1828,6 +1838,11 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
   // straight into U, V' rows = U^T psi / S.  Only the first dim columns of U /
   // rows of V' are filled: the projectors take no more.
   if (eig) {
+    const auto tv = std::chrono::steady_clock::now();
+    struct Tm {
+      std::chrono::steady_clock::time_point t0;
+      ~Tm() { g_svd_stats.t_vectors += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+    } tm{tv};
     std::vector<EigJob> jobs = eig_jobs;
     std::vector<DevBuf> zb(jobs.size());
     bool any = false;

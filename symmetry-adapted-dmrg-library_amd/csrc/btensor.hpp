@@ -113,10 +113,12 @@ int gram_min();
 void set_svd_eigensolver(int v);
 int svd_eigensolver();
 // Process-wide truncation counters: op_svd calls, sectors decomposed through
-// the Gram route / by gesdd, Gram sectors redone by gesdd, wall seconds.
+// the Gram route / by gesdd, Gram sectors redone by gesdd, wall seconds; the
+// seconds spent in the Gram decompositions, in gesdd and in the eigen
+// route's vectors, and the largest sector (max of m, n).
 struct SvdStats {
-  int64_t calls = 0, gram_sectors = 0, svd_sectors = 0, gram_redo = 0;
-  double seconds = 0;
+  int64_t calls = 0, gram_sectors = 0, svd_sectors = 0, gram_redo = 0, max_sector = 0;
+  double seconds = 0, t_gram = 0, t_gesdd = 0, t_vectors = 0;
 };
 SvdStats& svd_stats();
 

@@ -884,6 +884,14 @@ int sadmrg_dmrg_set_refinement(sadmrg_dmrg* d, int krylov_steps, int max_restart
   });
 }
 
+int sadmrg_dmrg_set_verbose(sadmrg_dmrg* d, int level) {
+  return guard([&] {
+    need(d, "dmrg");
+    if (level < 0) fail(kErrInvalid, "verbose level must be >= 0");
+    d->d->set_verbose(level);
+  });
+}
+
 int sadmrg_dmrg_set_check_interval(sadmrg_dmrg* d, int every) {
   return guard([&] {
     need(d, "dmrg");

@@ -1,5 +1,7 @@
 // dmrg.cpp -- see dmrg.hpp.
 #include "dmrg.hpp"
+#include "eigen.hpp"
+#include <cstdio>
 #include <cmath>
 #include <iostream>
 
@@ -495,6 +497,8 @@ double Dmrg::lanczos_solve(int il, int ir, BTensor& vec) {
 
 void Dmrg::svd(const BTensor& vec, int a, int b, double p1, double p2) {
   const auto t0 = Clock::now();
+  const SvdStats s0 = svd_stats();
+  const EigProfile e0 = SymEigBatch::profile();
   op_svd(vec, uu_[a], p1, uu_[b], p2, max_dcut_, wout_);
   const int n = static_cast<int>(wout_.size());
   const int dcut = n > max_dcut_ ? max_dcut_ : n;
@@ -504,6 +508,26 @@ void Dmrg::svd(const BTensor& vec, int a, int b, double p1, double p2) {
   cur_.kept = dcut;
   cur_.t_svd += since(t0);
   rec_.push_back(cur_);
+  if (verbose_ > 0) {
+    std::fprintf(stderr, "[sadmrg] step %d-%d E=%.12f kept=%d trunc=%.3e env=%.4fs lanczos=%.4fs svd=%.4fs\n", a, b,
+                 cur_.energy, cur_.kept, te, cur_.t_env, cur_.t_lanczos, cur_.t_svd);
+    if (verbose_ > 1) {
+      const SvdStats& s1 = svd_stats();
+      const EigProfile& e1 = SymEigBatch::profile();
+      std::fprintf(stderr,
+                   "[sadmrg]   svd: gram=%lld gesdd=%lld redo=%lld max_sector=%lld t_gram=%.4f t_gesdd=%.4f "
+                   "t_vectors=%.4f | eig: matrices=%lld launches=%lld max_order=%lld trd=%.4f bisect=%.4f "
+                   "invit=%.4f ortho=%.4f backt=%.4f vectors=%lld clusters=%lld max_cluster=%lld ns_steps=%lld\n",
+                   (long long)(s1.gram_sectors - s0.gram_sectors), (long long)(s1.svd_sectors - s0.svd_sectors),
+                   (long long)(s1.gram_redo - s0.gram_redo), (long long)s1.max_sector, s1.t_gram - s0.t_gram,
+                   s1.t_gesdd - s0.t_gesdd, s1.t_vectors - s0.t_vectors, (long long)(e1.matrices - e0.matrices),
+                   (long long)(e1.launches - e0.launches), (long long)e1.max_order, e1.trd - e0.trd,
+                   e1.bisect - e0.bisect, e1.invit - e0.invit, e1.ortho - e0.ortho, e1.backt - e0.backt,
+                   (long long)(e1.vectors - e0.vectors), (long long)(e1.clusters - e0.clusters),
+                   (long long)e1.max_cluster, (long long)(e1.ns_steps - e0.ns_steps));
+    }
+    std::fflush(stderr);
+  }
   cur_ = StepRecord{};
 }
 

@@ -46,6 +46,9 @@ class Dmrg {
   bool has_channel_weights() const { return !chw_.empty(); }
   void set_max_dcut(int d) { max_dcut_ = d; }
   void set_lanczos(const LanczosOpts& o) { lopt_ = o; }
+  // 1: one line per sweep step on stderr (bond, energy, kept states, stage
+  // times), as the reference prints each step of DMRG::sweep to cout.
+  void set_verbose(int v) { verbose_ = v; }
   const LanczosOpts& lanczos_opts() const { return lopt_; }
 
   // Deep copy of another driver's lattice maps, MPS, environments, settings
@@ -145,6 +148,7 @@ class Dmrg {
   BTensor sigma_[2];
   std::unique_ptr<CgcTables> cgc_;
   LanczosOpts lopt_;
+  int verbose_ = 0;
   double gs_enr_ = 0;
   BTensor last_vec_;  // Ritz vector of the last Lanczos solve
   int last_il_ = -1, last_ir_ = -1;

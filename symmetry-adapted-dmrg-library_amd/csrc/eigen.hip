@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -37,13 +38,16 @@ namespace {
 // (LAPACK layout: reflector i acts on rows i+1..n-1, v[i+1] = 1,
 // v[i+2..n-1] stored in column i below the subdiagonal).  The buffers of
 // even and odd steps are separate, so a workgroup that runs ahead cannot
-// overwrite entries a slower one still reads.  A poll that does not succeed
-// within kSpinLimit rounds sets the error flag and every workgroup ends.
+// overwrite entries a slower one still reads.  A workgroup that waits longer
+// than the wall-clock limit passed by the host (a quarter second; a step
+// normally waits microseconds), or past kSpinLimit polls, sets the error flag,
+// and every workgroup then ends.
 constexpr int kTrdThreads = 512;
 constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kLdsBytes = 160 * 1024;
 constexpr int kTrdLdsDoubles = (kLdsBytes - 512) / 8;  // dynamic part; the rest is static shared
 constexpr long long kSpinLimit = 1LL << 22;
+constexpr int kCuMargin = 16;
 
 struct TrdTask {
   double* a;
@@ -123,7 +127,7 @@ __device__ inline void put_tagged(uint64_t* p, double x, uint32_t tag) {
 
 template <int NT>
 __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restrict__ tasks, int ntask,
-                                                          int* __restrict__ err) {
+                                                          int* __restrict__ err, long long wait_ticks) {
   extern __shared__ double lds[];
   __shared__ double red[kTrdWaves];
   __shared__ int s_abort;
@@ -235,6 +239,7 @@ __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restr
         if (j < n) pending |= 1u << c;
       }
       long long spins = 0;
+      const long long t0 = wall_clock64();
       while (pending) {
 #pragma unroll
         for (int c = 0; c < kMaxRows; ++c)
@@ -255,7 +260,8 @@ __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restr
           }
         if (!pending) break;
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        if (++spins > kSpinLimit || wall_clock64() - t0 > wait_ticks ||
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           atomicExch(err, 1);
           s_abort = 1;
           break;
@@ -320,7 +326,8 @@ __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restr
 }
 
 template <int NT>
-hipError_t launch_trd(const TrdTask* tasks, int ntask, int* err, int nwg, int lds, hipStream_t st) {
+hipError_t launch_trd(const TrdTask* tasks, int ntask, int* err, long long wait_ticks, int nwg, int lds,
+                      hipStream_t st) {
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(trd_kernel<NT>),
@@ -328,7 +335,7 @@ hipError_t launch_trd(const TrdTask* tasks, int ntask, int* err, int nwg, int ld
     if (e != hipSuccess) return e;
     attr = true;
   }
-  void* args[] = {&tasks, &ntask, &err};
+  void* args[] = {&tasks, &ntask, &err, &wait_ticks};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(trd_kernel<NT>), dim3(nwg), dim3(kTrdThreads), args,
                                     static_cast<unsigned>(lds), st);
 }
@@ -656,6 +663,11 @@ hipError_t launch_bt(const BtTask* tasks, int ntask, int nblk, hipStream_t st) {
 }
 
 std::atomic<long long> g_cholqr_fallbacks{0};
+EigProfile g_prof;
+
+double secs_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
 
 int device_cus() {
   static int cus = [] {
@@ -667,9 +679,22 @@ int device_cus() {
   return cus;
 }
 
+// Wall-clock ticks of the tridiagonalisation's longest wait (a quarter second).
+long long wait_ticks() {
+  static long long t = [] {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+      khz = 100000;
+    return static_cast<long long>(khz) * 250;
+  }();
+  return t;
+}
+
 }  // namespace
 
 long long SymEigBatch::cholqr_fallbacks() { return g_cholqr_fallbacks; }
+EigProfile& SymEigBatch::profile() { return g_prof; }
 
 int SymEigBatch::max_order() {
   // rows of at least one matrix row plus three vectors in LDS, and at most
@@ -680,6 +705,7 @@ int SymEigBatch::max_order() {
 void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
   Context* ctx = ctx_;
   const int nj = static_cast<int>(jobs.size());
+  auto t0 = std::chrono::steady_clock::now();
   doff_.assign(nj + 1, 0);
   int64_t pub_tot = 0;
   for (int q = 0; q < nj; ++q) {
@@ -710,8 +736,11 @@ void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
     plans.push_back(Plan{q, W, (n + W - 1) / W});
   }
   std::stable_sort(plans.begin(), plans.end(), [](const Plan& x, const Plan& y) { return x.W > y.W; });
-  const int cus = device_cus();
+  // cooperative launches stay a margin below the CU count, so a grid never
+  // depends on every CU being free at once
+  const int cus = device_cus() - kCuMargin;
   require(cus > 0, "eigensolver: no device properties");
+  const long long wait = wait_ticks();
   // launches of at most `cus` co-resident workgroups
   int64_t pub_off = 0;
   std::vector<int64_t> pub_of(nj, 0);
@@ -753,21 +782,26 @@ void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
     for (const TrdTask& x : tasks) nmx = std::max(nmx, x.n);
     const int nt = (nmx + 63) / 64;
     hipError_t le;
-    if (nt <= 4) le = launch_trd<4>(tp, ntask, err, wg, lds, ctx->stream());
-    else if (nt <= 8) le = launch_trd<8>(tp, ntask, err, wg, lds, ctx->stream());
-    else if (nt <= 12) le = launch_trd<12>(tp, ntask, err, wg, lds, ctx->stream());
-    else if (nt <= 16) le = launch_trd<16>(tp, ntask, err, wg, lds, ctx->stream());
-    else if (nt <= 20) le = launch_trd<20>(tp, ntask, err, wg, lds, ctx->stream());
-    else if (nt <= 24) le = launch_trd<24>(tp, ntask, err, wg, lds, ctx->stream());
-    else le = launch_trd<32>(tp, ntask, err, wg, lds, ctx->stream());
+    if (nt <= 4) le = launch_trd<4>(tp, ntask, err, wait, wg, lds, ctx->stream());
+    else if (nt <= 8) le = launch_trd<8>(tp, ntask, err, wait, wg, lds, ctx->stream());
+    else if (nt <= 12) le = launch_trd<12>(tp, ntask, err, wait, wg, lds, ctx->stream());
+    else if (nt <= 16) le = launch_trd<16>(tp, ntask, err, wait, wg, lds, ctx->stream());
+    else if (nt <= 20) le = launch_trd<20>(tp, ntask, err, wait, wg, lds, ctx->stream());
+    else if (nt <= 24) le = launch_trd<24>(tp, ntask, err, wait, wg, lds, ctx->stream());
+    else le = launch_trd<32>(tp, ntask, err, wait, wg, lds, ctx->stream());
     SADMRG_HIP(le);
     ctx->sync();  // the task table leaves scope
+    g_prof.launches++;
     p0 = p1;
   }
   int herr = 0;
   SADMRG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
   ctx->sync();
   if (herr) fail(kErrNumerics, "eigensolver: a tridiagonalisation step did not receive its group's data");
+  g_prof.trd += secs_since(t0);
+  t0 = std::chrono::steady_clock::now();
+  g_prof.matrices += nj;
+  for (const EigJob& J : jobs) g_prof.max_order = std::max<int64_t>(g_prof.max_order, J.n);
   // eigenvalues
   std::vector<BisTask> bt;
   int total = 0;
@@ -789,6 +823,7 @@ void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
                      static_cast<int>(bt.size()), total);
   SADMRG_HIP(hipGetLastError());
   ctx->sync();
+  g_prof.bisect += secs_since(t0);
 }
 
 void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
@@ -810,6 +845,7 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
   }
   const int ns = static_cast<int>(sel.size());
   if (ns == 0) return;
+  auto t0 = std::chrono::steady_clock::now();
   const int64_t slot = int64_t(nmax) * kmax;
   DevBuf zp(ctx, slot * ns * 8), gm(ctx, int64_t(kmax) * kmax * ns * 8), dinfo(ctx, sizeof(int) * 2 * ns);
   SADMRG_HIP(launch_fill(slot * ns, 0.0, zp.as<double>(), ctx->stream()));
@@ -835,6 +871,9 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
       v.c0 = c;
       v.m = m;
       it.push_back(v);
+      g_prof.max_cluster = std::max<int64_t>(g_prof.max_cluster, m);
+      g_prof.vectors += m;
+      g_prof.clusters++;
       c += m;
     }
   }
@@ -867,6 +906,8 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
     SADMRG_HIP(hipGetLastError());
     ctx->sync();
   }
+  g_prof.invit += secs_since(t0);
+  t0 = std::chrono::steady_clock::now();
   // Newton-Schulz steps Z <- Z (3 I - Z^T Z) / 2 over the whole batch (the
   // zero padding columns stay zero) until every slot is orthonormal to
   // kOrthoTol; Cholesky QR twice if that does not happen within kNsMax steps
@@ -905,6 +946,7 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
         rocblas_status_success)
       fail(kErrNumerics, "eigensolver: rocblas gemm failed");
     std::swap(zcur, znext);
+    g_prof.ns_steps++;
   }
   if (!ortho) {
     for (int pass = 0; pass < 2; ++pass) {
@@ -953,6 +995,8 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
   ctx->sync();
   for (int x : hinfo)
     if (x != 0) fail(kErrNumerics, "eigensolver: eigenvectors not independent (Cholesky QR failed)");
+  g_prof.ortho += secs_since(t0);
+  t0 = std::chrono::steady_clock::now();
   const int nt = (nmax + 63) / 64;
   const BtTask* tp = dbt.as<BtTask>();
   const int ntask = static_cast<int>(bts.size());
@@ -970,6 +1014,7 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
     SADMRG_HIP(launch_copy2d(J.n, J.k, zcur + slot * s, 1, nmax, J.z, J.ldz, 1, 1.0, nullptr, 0, ctx->stream()));
   }
   ctx->sync();
+  g_prof.backt += secs_since(t0);
 }
 
 }  // namespace sadmrg

@@ -27,6 +27,13 @@
 
 namespace sadmrg {
 
+// Process-wide phase timers and counters of the solver (wall seconds; every
+// phase ends with a stream synchronisation, so the host clock times it).
+struct EigProfile {
+  double trd = 0, bisect = 0, invit = 0, ortho = 0, backt = 0;
+  int64_t matrices = 0, launches = 0, vectors = 0, clusters = 0, max_cluster = 0, ns_steps = 0, max_order = 0;
+};
+
 struct EigJob {
   double* a = nullptr;  // n x n, column-major, leading dimension lda; lower triangle read, then overwritten
   int n = 0, lda = 0;
@@ -48,6 +55,7 @@ class SymEigBatch {
   // Phase-2 batches whose Newton-Schulz orthonormalisation did not converge
   // and fell back to Cholesky QR (process-wide count).
   static long long cholqr_fallbacks();
+  static EigProfile& profile();
 
  private:
   Context* ctx_;
