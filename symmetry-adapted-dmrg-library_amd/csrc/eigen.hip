@@ -424,10 +424,17 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
 // ---------------------------------------------------------------------------
 // 3. Eigenvectors of T by inverse iteration (LAPACK dstein scheme): one
 // lane per cluster of eigenvalues that inverse iteration cannot separate
-// (neighbours closer than kClusterGap * |T|).  For each member, from the
-// largest: three solves of (T - lambda I) x = y from a seeded random start,
-// each followed by Gram-Schmidt against the cluster's earlier members and
-// normalisation.  Each solve runs the LU factorisation with partial pivoting
+// (neighbours closer than kClusterGap * |T|, a few times the bisection's
+// absolute accuracy), at most kMaxCluster members each.  For each member,
+// from the largest: three solves of (T - lambda I) x = y from a seeded
+// random start, each followed by Gram-Schmidt against the cluster's earlier
+// members and normalisation.  Eigenvalues further apart converge to vectors
+// whose overlaps are about eps |T| / gap (at most ~0.03), which the
+// orthonormalisation after this kernel removes; a vector's error is then a
+// rotation among eigenvectors of close eigenvalues, which leaves the span of
+// the kept set -- all the truncation uses -- unchanged to that order.  The
+// split of a longer run of exactly degenerate eigenvalues into chunks gives
+// chunks of random, hence well-conditioned, vectors of the same eigenspace.  Each solve runs the LU factorisation with partial pivoting
 // (dlagtf) fused with the forward elimination, then the back substitution.
 // Vectors of different clusters are orthogonal to about eps |T| / gap; the
 // Newton-Schulz steps after this kernel make the set orthonormal.  The
@@ -435,7 +442,8 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
 // live in LDS, next to those of the wave's other lanes, and every step costs
 // LDS latency instead of an HBM round trip; a wave has as many active lanes
 // as its LDS holds (one wave per workgroup).
-constexpr double kClusterGap = 1e3 * DBL_EPSILON;
+constexpr double kClusterGap = 32 * DBL_EPSILON;
+constexpr int kMaxCluster = 8;
 constexpr int kInvLdsDoubles = kTrdLdsDoubles;
 
 struct InvTask {
@@ -702,24 +710,52 @@ int SymEigBatch::max_order() {
   return 2048;
 }
 
-void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
-  Context* ctx = ctx_;
+namespace {
+
+// Device memory of phase 1, one region: d, e, tau (3 n doubles per job, at
+// offset doff[q] from its start), the publication words of the groups (8 n
+// per job) and the error flag (both cleared by a fill kernel before the
+// launches), then the task tables (one TrdTask and one BisTask per
+// non-empty job).  Parts start on 256-byte boundaries.
+struct Phase1Layout {
+  std::vector<int64_t> doff;  // per job, in doubles; doff[nj] = total
+  int64_t pubs = 0, err = 0, tasks = 0, bis = 0, total = 0;  // byte offsets / size
+};
+
+int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+Phase1Layout phase1_layout(const std::vector<EigJob>& jobs) {
+  Phase1Layout L;
+  const int nj = static_cast<int>(jobs.size());
+  L.doff.assign(nj + 1, 0);
+  int64_t nonempty = 0;
+  for (int q = 0; q < nj; ++q) {
+    require(jobs[q].n >= 0 && jobs[q].n <= SymEigBatch::max_order(), "eigensolver: matrix order out of range");
+    require(jobs[q].n == 0 || (jobs[q].a && jobs[q].w && jobs[q].lda >= jobs[q].n), "eigensolver: bad job");
+    L.doff[q + 1] = L.doff[q] + 3 * int64_t(jobs[q].n);
+    nonempty += jobs[q].n > 0;
+  }
+  L.pubs = align256(L.doff[nj] * 8);
+  L.err = L.pubs + (L.doff[nj] / 3) * 8 * 8;  // 8 words per row
+  L.tasks = align256(L.err + 8);
+  L.bis = align256(L.tasks + nonempty * int64_t(sizeof(TrdTask)));
+  L.total = align256(L.bis + nonempty * int64_t(sizeof(BisTask)));
+  return L;
+}
+
+// Phase 1 on the region `mem` laid out by L: tridiagonalisations in
+// cooperative launches of at most (CUs - kCuMargin) workgroups, then
+// bisection.  Synchronises the stream before returning.
+void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L, char* mem) {
   const int nj = static_cast<int>(jobs.size());
   auto t0 = std::chrono::steady_clock::now();
-  doff_.assign(nj + 1, 0);
-  int64_t pub_tot = 0;
-  for (int q = 0; q < nj; ++q) {
-    require(jobs[q].n >= 0 && jobs[q].n <= max_order(), "eigensolver: matrix order out of range");
-    require(jobs[q].n == 0 || (jobs[q].a && jobs[q].w && jobs[q].lda >= jobs[q].n), "eigensolver: bad job");
-    doff_[q + 1] = doff_[q] + 3 * int64_t(jobs[q].n);
-    pub_tot += 4 * int64_t(jobs[q].n);
-  }
-  trd_ = DevBuf(ctx, std::max<int64_t>(doff_[nj], 1) * 8);
-  DevBuf pubs(ctx, std::max<int64_t>(2 * pub_tot, 1) * 8), errb(ctx, sizeof(int));
-  SADMRG_HIP(hipMemsetAsync(pubs.get(), 0, std::max<int64_t>(2 * pub_tot, 1) * 8, ctx->stream()));
-  SADMRG_HIP(hipMemsetAsync(errb.get(), 0, sizeof(int), ctx->stream()));
-  int* err = errb.as<int>();
-  double* base = trd_.as<double>();
+  double* base = reinterpret_cast<double*>(mem);
+  uint64_t* pubs = reinterpret_cast<uint64_t*>(mem + L.pubs);
+  int* err = reinterpret_cast<int*>(mem + L.err);
+  TrdTask* dtasks = reinterpret_cast<TrdTask*>(mem + L.tasks);
+  BisTask* dbis = reinterpret_cast<BisTask*>(mem + L.bis);
+  // publication words and the error flag: zero bits
+  SADMRG_HIP(launch_fill((L.err + 8 - L.pubs) / 8, 0.0, reinterpret_cast<double*>(pubs), ctx->stream()));
   // workgroups per matrix: as many rows per workgroup as LDS holds next to
   // three vectors, rows spread evenly
   struct Plan {
@@ -741,89 +777,94 @@ void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
   const int cus = device_cus() - kCuMargin;
   require(cus > 0, "eigensolver: no device properties");
   const long long wait = wait_ticks();
-  // launches of at most `cus` co-resident workgroups
-  int64_t pub_off = 0;
-  std::vector<int64_t> pub_of(nj, 0);
-  for (int q = 0; q < nj; ++q) {
-    pub_of[q] = pub_off;
-    pub_off += 8 * int64_t(jobs[q].n);
-  }
+  // every launch's tasks and the bisection tasks in one host table, one copy
+  struct Launch {
+    int t0, ntask, wg, lds, nt;
+  };
+  std::vector<Launch> launches;
+  std::vector<char> host(L.total - L.tasks, 0);
+  TrdTask* htasks = reinterpret_cast<TrdTask*>(host.data());
+  BisTask* hbis = reinterpret_cast<BisTask*>(host.data() + (L.bis - L.tasks));
+  int nt_all = 0;
   for (size_t p0 = 0; p0 < plans.size();) {
-    std::vector<TrdTask> tasks;
-    int wg = 0, lds = 0;
+    Launch l{nt_all, 0, 0, 0, 0};
     size_t p1 = p0;
-    while (p1 < plans.size() && (wg + plans[p1].W <= cus || tasks.empty())) {
+    while (p1 < plans.size() && (l.wg + plans[p1].W <= cus || l.ntask == 0)) {
       const Plan& P = plans[p1];
       require(P.W <= cus, "eigensolver: matrix needs more workgroups than the device holds");
       const EigJob& J = jobs[P.q];
       TrdTask t;
       t.a = J.a;
-      t.d = base + doff_[P.q];
+      t.d = base + L.doff[P.q];
       t.e = t.d + J.n;
       t.tau = t.e + J.n;
-      t.pub = pubs.as<uint64_t>() + pub_of[P.q];
+      t.pub = pubs + (L.doff[P.q] / 3) * 8;
       t.n = J.n;
       t.lda = J.lda;
-      t.wg0 = wg;
+      t.wg0 = l.wg;
       t.nwg = P.W;
       t.rows = P.R;
       t.ldr = (J.n + 63) / 64 * 64;
-      tasks.push_back(t);
-      wg += P.W;
-      lds = std::max(lds, (P.R + 3) * t.ldr * 8);
+      std::memcpy(htasks + nt_all, &t, sizeof(TrdTask));
+      ++nt_all;
+      ++l.ntask;
+      l.wg += P.W;
+      l.lds = std::max(l.lds, (P.R + 3) * t.ldr * 8);
+      l.nt = std::max(l.nt, (J.n + 63) / 64);
       ++p1;
     }
-    DevBuf dt(ctx, tasks.size() * sizeof(TrdTask));
-    SADMRG_HIP(hipMemcpyAsync(dt.get(), tasks.data(), tasks.size() * sizeof(TrdTask), hipMemcpyHostToDevice,
-                              ctx->stream()));
-    const TrdTask* tp = dt.as<TrdTask>();
-    const int ntask = static_cast<int>(tasks.size());
-    int nmx = 0;
-    for (const TrdTask& x : tasks) nmx = std::max(nmx, x.n);
-    const int nt = (nmx + 63) / 64;
-    hipError_t le;
-    if (nt <= 4) le = launch_trd<4>(tp, ntask, err, wait, wg, lds, ctx->stream());
-    else if (nt <= 8) le = launch_trd<8>(tp, ntask, err, wait, wg, lds, ctx->stream());
-    else if (nt <= 12) le = launch_trd<12>(tp, ntask, err, wait, wg, lds, ctx->stream());
-    else if (nt <= 16) le = launch_trd<16>(tp, ntask, err, wait, wg, lds, ctx->stream());
-    else if (nt <= 20) le = launch_trd<20>(tp, ntask, err, wait, wg, lds, ctx->stream());
-    else if (nt <= 24) le = launch_trd<24>(tp, ntask, err, wait, wg, lds, ctx->stream());
-    else le = launch_trd<32>(tp, ntask, err, wait, wg, lds, ctx->stream());
-    SADMRG_HIP(le);
-    ctx->sync();  // the task table leaves scope
-    g_prof.launches++;
+    launches.push_back(l);
     p0 = p1;
+  }
+  int total = 0, nb = 0;
+  for (int q = 0; q < nj; ++q) {
+    if (jobs[q].n == 0) continue;
+    BisTask b;
+    b.d = base + L.doff[q];
+    b.e = b.d + jobs[q].n;
+    b.w = jobs[q].w;
+    b.n = jobs[q].n;
+    b.off = total;
+    std::memcpy(hbis + nb, &b, sizeof(BisTask));
+    ++nb;
+    total += jobs[q].n;
+  }
+  if (total == 0) return;
+  SADMRG_HIP(hipMemcpyAsync(mem + L.tasks, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream()));
+  for (const Launch& l : launches) {
+    const TrdTask* tp = dtasks + l.t0;
+    hipError_t le;
+    if (l.nt <= 4) le = launch_trd<4>(tp, l.ntask, err, wait, l.wg, l.lds, ctx->stream());
+    else if (l.nt <= 8) le = launch_trd<8>(tp, l.ntask, err, wait, l.wg, l.lds, ctx->stream());
+    else if (l.nt <= 12) le = launch_trd<12>(tp, l.ntask, err, wait, l.wg, l.lds, ctx->stream());
+    else if (l.nt <= 16) le = launch_trd<16>(tp, l.ntask, err, wait, l.wg, l.lds, ctx->stream());
+    else if (l.nt <= 20) le = launch_trd<20>(tp, l.ntask, err, wait, l.wg, l.lds, ctx->stream());
+    else if (l.nt <= 24) le = launch_trd<24>(tp, l.ntask, err, wait, l.wg, l.lds, ctx->stream());
+    else le = launch_trd<32>(tp, l.ntask, err, wait, l.wg, l.lds, ctx->stream());
+    SADMRG_HIP(le);
+    g_prof.launches++;
   }
   int herr = 0;
   SADMRG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
-  ctx->sync();
+  ctx->sync();  // also: the host task table may leave scope
   if (herr) fail(kErrNumerics, "eigensolver: a tridiagonalisation step did not receive its group's data");
   g_prof.trd += secs_since(t0);
   t0 = std::chrono::steady_clock::now();
   g_prof.matrices += nj;
   for (const EigJob& J : jobs) g_prof.max_order = std::max<int64_t>(g_prof.max_order, J.n);
-  // eigenvalues
-  std::vector<BisTask> bt;
-  int total = 0;
-  for (int q = 0; q < nj; ++q) {
-    if (jobs[q].n == 0) continue;
-    BisTask b;
-    b.d = base + doff_[q];
-    b.e = b.d + jobs[q].n;
-    b.w = jobs[q].w;
-    b.n = jobs[q].n;
-    b.off = total;
-    bt.push_back(b);
-    total += jobs[q].n;
-  }
-  if (total == 0) return;
-  DevBuf db(ctx, bt.size() * sizeof(BisTask));
-  SADMRG_HIP(hipMemcpyAsync(db.get(), bt.data(), bt.size() * sizeof(BisTask), hipMemcpyHostToDevice, ctx->stream()));
-  hipLaunchKernelGGL(bisect_kernel, dim3((total + 3) / 4), dim3(256), 0, ctx->stream(), db.as<BisTask>(),
-                     static_cast<int>(bt.size()), total);
+  hipLaunchKernelGGL(bisect_kernel, dim3((total + 3) / 4), dim3(256), 0, ctx->stream(), dbis, nb, total);
   SADMRG_HIP(hipGetLastError());
   ctx->sync();
   g_prof.bisect += secs_since(t0);
+}
+
+}  // namespace
+
+void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
+  const Phase1Layout L = phase1_layout(jobs);
+  doff_ = L.doff;
+  trd_ = DevBuf(ctx_, std::max<int64_t>(L.total, 256));
+  phase1(ctx_, jobs, L, static_cast<char*>(trd_.get()));
 }
 
 void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
@@ -867,7 +908,8 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
     v.ldz = nmax;
     for (int c = 0; c < J.k;) {
       int m = 1;
-      while (c + m < J.k && w[J.n - 1 - (c + m - 1)] - w[J.n - 1 - (c + m)] <= kClusterGap * tn) ++m;
+      while (c + m < J.k && m < kMaxCluster && w[J.n - 1 - (c + m - 1)] - w[J.n - 1 - (c + m)] <= kClusterGap * tn)
+        ++m;
       v.c0 = c;
       v.m = m;
       it.push_back(v);
