@@ -197,6 +197,18 @@ int sadmrg_svd_gram_min(int value, int* previous);
    which the reference calls one matrix at a time. */
 int sadmrg_eigh_batched(sadmrg_ctx* ctx, int count, const int* n, double* const* a, const int* lda,
                         double* const* w, const int* k, double* const* z, const int* ldz);
+/* sadmrg_eigh_batched with every k[q] = 0 (eigenvalues only), allocating
+   nothing: no hipMalloc, hipFree or allocating library call.  Its working
+   memory (tridiagonal factors, publication words, task tables) is a static
+   device array of 2 MiB per device, looked up when the device's first
+   context is created and held until the process ends; a batch needs 88
+   bytes per matrix row plus 96 per matrix (sum of n[q] up to about 23000)
+   and a larger one is rejected with SADMRG_ERR_INVALID before any work.
+   Calls that overlap on one device (other contexts or threads) share the
+   array and race.  No counterpart in the reference: for callers that cannot
+   allocate (graph-capturing or tight serving loops). */
+int sadmrg_eigvalsh_batched_noalloc(sadmrg_ctx* ctx, int count, const int* n, double* const* a, const int* lda,
+                                    double* const* w);
 /* Eigensolver of the Gram route inside BlockTensor::svd (process-wide):
    1 (default) the hand-written batched solver above; 0 rocSOLVER's
    strided-batched syevd on matrices padded to the batch's largest order

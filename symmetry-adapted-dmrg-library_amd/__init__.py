@@ -94,6 +94,7 @@ def lib():
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
         "sadmrg_lanczos_default_options": (i, [C.POINTER(LanczosOptions)]),
         "sadmrg_eigh_batched": (i, [vp, i, ip, pp, ip, pp, ip, pp, ip]),
+        "sadmrg_eigvalsh_batched_noalloc": (i, [vp, i, ip, pp, ip, pp]),
         "sadmrg_svd_eigensolver": (i, [i, ip]),
         "sadmrg_lanczos_ex": (i, [vp, vp, C.POINTER(LanczosOptions), dp, ip, dp]),
         "sadmrg_autotune": (i, [i, i, ip]),
@@ -496,6 +497,15 @@ def eigh_batched(ctx, n, a, lda, w, k=None, z=None, ldz=None):
     z = z or [0] * cnt
     ldz = ldz or [1] * cnt
     check(lib().sadmrg_eigh_batched(ctx.ptr, cnt, arr(n), ptrs(a), arr(lda), ptrs(w), arr(k), ptrs(z), arr(ldz)))
+
+
+def eigvalsh_batched_noalloc(ctx, n, a, lda, w):
+    """sadmrg_eigvalsh_batched_noalloc: eigenvalues only, ascending into w,
+    with the library's static per-device scratch (no allocation)."""
+    cnt = len(n)
+    arr = lambda xs: (C.c_int * max(cnt, 1))(*xs)  # noqa: E731
+    ptrs = lambda xs: (C.c_void_p * max(cnt, 1))(*[C.c_void_p(x) for x in xs])  # noqa: E731
+    check(lib().sadmrg_eigvalsh_batched_noalloc(ctx.ptr, cnt, arr(n), ptrs(a), arr(lda), ptrs(w)))
 
 
 def svd_eigensolver(value=None):

@@ -657,6 +657,30 @@ int sadmrg_eigh_batched(sadmrg_ctx* ctx, int count, const int* n, double* const*
   });
 }
 
+int sadmrg_eigvalsh_batched_noalloc(sadmrg_ctx* ctx, int count, const int* n, double* const* a, const int* lda,
+                                    double* const* w) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (count < 0) fail(kErrInvalid, "eigvalsh: count must be >= 0");
+    if (count == 0) return;
+    need(n, "n");
+    need(a, "a");
+    need(lda, "lda");
+    need(w, "w");
+    std::vector<EigJob> jobs(count);
+    for (int q = 0; q < count; ++q) {
+      EigJob& J = jobs[q];
+      if (n[q] < 0 || n[q] > SymEigBatch::max_order()) fail(kErrInvalid, "eigvalsh: matrix order out of range");
+      if (n[q] > 0 && (a[q] == nullptr || w[q] == nullptr || lda[q] < n[q])) fail(kErrInvalid, "eigvalsh: bad matrix");
+      J.a = a[q];
+      J.n = n[q];
+      J.lda = lda[q];
+      J.w = w[q];
+    }
+    SymEigBatch::eigenvalues_noalloc(ctx->c.get(), jobs);
+  });
+}
+
 int sadmrg_svd_eigensolver(int value, int* previous) {
   return guard([&] {
     if (previous) *previous = svd_eigensolver();

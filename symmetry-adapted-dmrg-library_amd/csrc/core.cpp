@@ -1,5 +1,6 @@
 // core.cpp -- device context and caching allocator.
 #include "core.hpp"
+#include "eigen.hpp"
 #include <rocsolver/rocsolver.h>
 #include "kernels.hpp"
 
@@ -35,6 +36,7 @@ Context::Context(int device) : device_(device) {
   SADMRG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   partial_ = DevBuf(this, sizeof(double) * 2 * kPartialGrid * 64);
   dscal_ = DevBuf(this, sizeof(double) * kMaxDscal);
+  noalloc_scratch_setup(device);
 }
 
 Context::~Context() {

@@ -10,6 +10,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+
+// Scratch of the allocation-free phase 1 (SymEigBatch::eigenvalues_noalloc):
+// one copy per device, held until the process ends.  Host code uses the
+// device address looked up by noalloc_scratch_setup, never this name.
+static __device__ __align__(256) double scratch[sadmrg::SymEigBatch::kNoAllocBytes / 8];
 
 namespace sadmrg {
 
@@ -859,6 +865,36 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
 }
 
 }  // namespace
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<char*> g_scratch_ptr;  // per device: address of `scratch`, null until looked up
+}  // namespace
+
+void noalloc_scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (static_cast<int>(g_scratch_ptr.size()) <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  SADMRG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  require(p != nullptr, "eigensolver: no device address of the static scratch");
+  g_scratch_ptr[device] = static_cast<char*>(p);
+}
+
+void SymEigBatch::eigenvalues_noalloc(Context* ctx, const std::vector<EigJob>& jobs) {
+  const Phase1Layout L = phase1_layout(jobs);
+  if (L.total > kNoAllocBytes)
+    fail(kErrInvalid, "eigvalsh (no allocation): the batch needs " + std::to_string(L.total) +
+                          " bytes of scratch, more than the " + std::to_string(kNoAllocBytes) +
+                          " held per device (88 bytes per matrix row, 96 per matrix)");
+  char* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (ctx->device() < static_cast<int>(g_scratch_ptr.size())) p = g_scratch_ptr[ctx->device()];
+  }
+  require(p != nullptr, "eigvalsh (no allocation): scratch of this device not set up");
+  phase1(ctx, jobs, L, p);
+}
 
 void SymEigBatch::eigenvalues(const std::vector<EigJob>& jobs) {
   const Phase1Layout L = phase1_layout(jobs);

@@ -50,6 +50,12 @@ class SymEigBatch {
   void eigenvalues(const std::vector<EigJob>& jobs);
   // Phase 2 (after phase 1, same jobs, k and z set): eigenvectors.
   void top_vectors(const std::vector<EigJob>& jobs);
+  // Phase 1 without allocating: the working memory is a static device array
+  // of kNoAllocBytes per device (its address looked up when the device's
+  // first context is created); batches needing more are rejected.  Calls
+  // that overlap on one device share that array and race.
+  static constexpr int64_t kNoAllocBytes = int64_t(2) << 20;
+  static void eigenvalues_noalloc(Context* ctx, const std::vector<EigJob>& jobs);
   // Largest matrix order the tridiagonalisation holds in LDS.
   static int max_order();
   // Phase-2 batches whose Newton-Schulz orthonormalisation did not converge
@@ -62,5 +68,9 @@ class SymEigBatch {
   std::vector<int64_t> doff_;  // per job: offset of d, e, tau (3n doubles) in trd_
   DevBuf trd_;
 };
+
+// Looks up the device address of the allocation-free scratch of `device`
+// (the current device), once per device.
+void noalloc_scratch_setup(int device);
 
 }  // namespace sadmrg
