@@ -373,6 +373,12 @@ int sadmrg_dmrg_coupling_gradient(sadmrg_dmrg* d, double* grad, int cap, double*
 int sadmrg_dmrg_input_vector(sadmrg_dmrg* d, int il, int ir, sadmrg_tensor* out);
 /* hamiltonian_vector_multiplication_idmrg (dmrg.hpp:114) */
 int sadmrg_dmrg_heff(sadmrg_dmrg* d, int il, int ir, const sadmrg_tensor* vin, sadmrg_tensor* vout, int precision);
+/* The H_eff plan of step (il, ir) as the sweep's Lanczos solve builds it:
+   the operand set of hamiltonian_vector_multiplication_idmrg
+   (dmrg_inst.cpp:283-371) on the driver's current environments, for wave
+   functions in the layout of `layout`.  Apply it with sadmrg_heff_apply*,
+   release it with sadmrg_heff_destroy; it holds copies of the operands. */
+int sadmrg_dmrg_heff_create(sadmrg_dmrg* d, int il, int ir, const sadmrg_tensor* layout, sadmrg_heff** out);
 /* lanczos_solve_eigenvector_idmrg (dmrg.hpp:90); vec updated in place */
 int sadmrg_dmrg_lanczos(sadmrg_dmrg* d, int il, int ir, sadmrg_tensor* vec, double* energy);
 /* vec.svd(uu[a], p1, uu[b], p2, ww) as in DMRG::sweep (dmrg_inst.cpp:894) */

@@ -43,6 +43,7 @@ int        myrank       = 0;
 CGCTables* g_cgc_tables = nullptr;
 
 extern "C" int MKL_Set_Threading_Layer(int);
+extern "C" void MKL_Set_Num_Threads(int);
 
 namespace {
 
@@ -240,8 +241,20 @@ void ref_quiet(int on) {
 }
 
 // Globals normally set by main/main_dmrg_*.cpp:32-35 and :188-197.
+// MKL threading before the first MKL call of the process: layer 1
+// sequential (the default: the reference's own OpenMP over bond terms is the
+// parallelism), 3 GNU OpenMP with `threads` BLAS/LAPACK threads.
+namespace {
+bool g_blas_chosen = false;
+}
+void ref_set_blas_threading(int layer, int threads) {
+  MKL_Set_Threading_Layer(layer);
+  if (threads > 0) MKL_Set_Num_Threads(threads);
+  g_blas_chosen = true;
+}
+
 int ref_init(int physpn, int max_angm, int max_dcut_, int psize_) {
-  MKL_Set_Threading_Layer(1);  // sequential BLAS: the reference's own OpenMP is the parallelism
+  if (!g_blas_chosen) MKL_Set_Threading_Layer(1);
   max_dcut = max_dcut_;
   psize = psize_ > 0 ? psize_ : 1;
   omp_set_num_threads(psize);

@@ -43,6 +43,7 @@ def lib():
         "ref_init": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
         "ref_set_max_dcut": (None, [C.c_int]),
         "ref_set_psize": (None, [C.c_int]),
+        "ref_set_blas_threading": (None, [C.c_int, C.c_int]),
         "ref_cg": (C.c_double, [C.c_int] * 6),
         "ref_fac_hamilt_vec": (C.c_int, [dp]),
         "ref_fac_table5": (C.c_int, [C.c_int, dp]),

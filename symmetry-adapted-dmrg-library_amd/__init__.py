@@ -138,6 +138,7 @@ def lib():
         "sadmrg_dmrg_input_vector": (i, [vp, i, i, vp]),
         "sadmrg_dmrg_coupling_gradient": (i, [vp, dp, i, dp]),
         "sadmrg_dmrg_heff": (i, [vp, i, i, vp, vp, i]),
+        "sadmrg_dmrg_heff_create": (i, [vp, i, i, vp, C.POINTER(vp)]),
         "sadmrg_dmrg_lanczos": (i, [vp, i, i, vp, dp]),
         "sadmrg_dmrg_svd": (i, [vp, vp, i, i, d, d, dp, i, ip]),
     }
@@ -731,6 +732,15 @@ class DMRG:
         t = Tensor.null(self.ctx, self.sym, self.cplx)
         check(lib().sadmrg_dmrg_heff(self.ptr, il, ir, vin.ptr, t.ptr, precision))
         return t
+
+    def heff_plan(self, il, ir, layout):
+        """The H_eff plan of step (il, ir) as the sweep builds it (a Heff)."""
+        q = C.c_void_p()
+        check(lib().sadmrg_dmrg_heff_create(self.ptr, il, ir, layout.ptr, C.byref(q)))
+        h = Heff.__new__(Heff)
+        h.ptr, h.ctx = q, self.ctx
+        h._keep = (self, layout)
+        return h
 
     def lanczos(self, il, ir, vec):
         e = C.c_double()

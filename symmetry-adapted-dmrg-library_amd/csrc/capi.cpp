@@ -1133,6 +1133,17 @@ int sadmrg_dmrg_heff(sadmrg_dmrg* d, int il, int ir, const sadmrg_tensor* vin, s
   });
 }
 
+int sadmrg_dmrg_heff_create(sadmrg_dmrg* d, int il, int ir, const sadmrg_tensor* layout, sadmrg_heff** out) {
+  return guard([&] {
+    need(d, "dmrg");
+    need(layout, "layout");
+    need(out, "out");
+    auto h = std::make_unique<sadmrg_heff>();
+    h->p = d->d->make_heff(il, ir, layout->t, nullptr);
+    *out = h.release();
+  });
+}
+
 int sadmrg_dmrg_lanczos(sadmrg_dmrg* d, int il, int ir, sadmrg_tensor* vec, double* energy) {
   return guard([&] {
     need(d, "dmrg");

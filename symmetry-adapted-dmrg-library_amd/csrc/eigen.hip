@@ -179,12 +179,18 @@ __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restr
   for (int k = 0; k <= n - 2; ++k) {
     uint64_t* pub = T.pub + static_cast<size_t>(k & 1) * 4 * n;
     const uint32_t tag = static_cast<uint32_t>(k + 1);
-    // row pass: update k-1, p = tau A v, column k+1; this lane's slices of
-    // the three vectors are read once per step
+    // row pass: update k-1, p = tau A v, column k+1.  This lane's first NR
+    // slices of the three vectors are read once per step and held in
+    // registers; further slices (rows longer than 64 NR) are read from LDS
+    // per row.  Each row is processed in chunks of kChunk slices: loaded,
+    // updated, stored back.
     {
-      double vs[NT], vps[NT], wps[NT];
+      constexpr int NR = NT <= 16 ? NT : 12;
+      constexpr int kChunk = 4;
+      static_assert(NR % kChunk == 0 && NT % kChunk == 0, "chunks must tile the register-held and LDS slices");
+      double vs[NR], vps[NR], wps[NR];
 #pragma unroll
-      for (int m = 0; m < NT; ++m) {
+      for (int m = 0; m < NR; ++m) {
         const int j = lane + 64 * m;
         const int jc = min(j, ldr - 1);
         const bool on = j > k && j < n;
@@ -198,26 +204,55 @@ __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restr
         if (r >= n || r <= k) continue;
         double* row = rows + static_cast<size_t>(i) * ldr;
         const double vr = k > 0 ? vp[r] : 0.0, wr = k > 0 ? wp[r] : 0.0;
-        double a[NT];
-#pragma unroll
-        for (int m = 0; m < NT; ++m) a[m] = row[min(lane + 64 * m, ldr - 1)];
         double acc = 0.0, cn = 0.0;
+        // one chunk of slices m0 .. m0+kChunk-1 (those below NT): load,
+        // update, store; `vec` gives slice m's (v, vp, w) entries
+        auto chunk = [&](int m0, auto vec) {
+          double a[kChunk];
 #pragma unroll
-        for (int m = 0; m < NT; ++m) {
-          const int j = lane + 64 * m;
-          // vps, wps, vs are zero for j <= k and j >= n
-          const double x = fma(-wr, vps[m], fma(-vr, wps[m], a[m]));
-          acc = fma(x, vs[m], acc);
-          if (j == k + 1) cn = x;
-          a[m] = x;
-        }
-        if (k > 0) {
+          for (int q = 0; q < kChunk; ++q)
+            if (m0 + q < NT) a[q] = row[min(lane + 64 * (m0 + q), ldr - 1)];
 #pragma unroll
-          for (int m = 0; m < NT; ++m) {
-            const int j = lane + 64 * m;
-            if (j > k && j < n) row[j] = a[m];
+          for (int q = 0; q < kChunk; ++q) {
+            const int m = m0 + q;
+            if (m < NT) {
+              const int j = lane + 64 * m;
+              double vsm, vpsm, wpsm;
+              vec(q, j, vsm, vpsm, wpsm);
+              // vpsm, wpsm, vsm are zero for j <= k and j >= n
+              const double x = fma(-wr, vpsm, fma(-vr, wpsm, a[q]));
+              acc = fma(x, vsm, acc);
+              if (j == k + 1) cn = x;
+              a[q] = x;
+            }
           }
-        }
+          if (k > 0) {
+#pragma unroll
+            for (int q = 0; q < kChunk; ++q) {
+              const int j = lane + 64 * (m0 + q);
+              if (m0 + q < NT && j > k && j < n) row[j] = a[q];
+            }
+          }
+        };
+        // register-held slices (fully unrolled: the array indices are constants)
+#pragma unroll
+        for (int m0 = 0; m0 < NR; m0 += kChunk)
+          chunk(m0, [&](int q, int, double& x, double& y, double& z) {
+            x = vs[m0 + q];
+            y = vps[m0 + q];
+            z = wps[m0 + q];
+          });
+        // further slices from LDS, one chunk at a time
+#pragma unroll 1
+        for (int m0 = NR; m0 < NT; m0 += kChunk)
+          chunk(m0, [&](int, int j, double& x, double& y, double& z) {
+            const int jc = min(j, ldr - 1);
+            const bool on = j > k && j < n;
+            const double a0 = v[jc], b0 = vp[jc], c0 = wp[jc];
+            x = on ? a0 : 0.0;
+            y = (on && k > 0) ? b0 : 0.0;
+            z = (on && k > 0) ? c0 : 0.0;
+          });
         acc = wave_sum(acc);
         if (lane == 0) put_tagged(pub + 4 * static_cast<size_t>(r), tau * acc, tag);
         if (lane == ((k + 1) & 63)) put_tagged(pub + 4 * static_cast<size_t>(r) + 2, cn, tag);
@@ -711,9 +746,19 @@ long long SymEigBatch::cholqr_fallbacks() { return g_cholqr_fallbacks; }
 EigProfile& SymEigBatch::profile() { return g_prof; }
 
 int SymEigBatch::max_order() {
-  // rows of at least one matrix row plus three vectors in LDS, and at most
-  // 64 * 32 rows in the back-transformation's registers
-  return 2048;
+  // the largest order whose rows fit the LDS of at most (CUs - kCuMargin)
+  // co-resident workgroups next to three vectors each, and at most 64 * 32
+  // rows in the back-transformation's registers
+  static const int m = [] {
+    const int cus = device_cus() - kCuMargin;
+    for (int n = 2048; n > 1; --n) {
+      const int ldr = (n + 63) / 64 * 64;
+      const int rmax = (kTrdLdsDoubles - 3 * ldr) / ldr;
+      if (rmax >= 1 && (n + rmax - 1) / rmax <= cus) return n;
+    }
+    return 1;
+  }();
+  return m;
 }
 
 namespace {

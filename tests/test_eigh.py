@@ -5,13 +5,14 @@ against numpy's LAPACK eigh on the same matrices.  The matrices are Gram
 matrices X X^T whose singular values decay over ten decades, with exact
 degenerate pairs (as the entanglement spectra of the symmetric cylinders
 have), plus a random symmetric indefinite one; their orders range from 1 to
-the largest sector of the D=2048 centre bond, all in one batch."""
+beyond the largest sector of the D=2048 centre bond (1800: rows longer than
+the register-held slices of the reduction), all in one batch."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-ORDERS = [1, 2, 3, 17, 64, 200, 453, 961, 1255]
+ORDERS = [1, 2, 3, 17, 64, 200, 453, 961, 1255, 1800]
 
 
 def _gram(rng, n, pairs):
