@@ -16,7 +16,8 @@ import sadmrg_testlib as tl
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_reference_centre_step_child(tmp_path):
+def test_reference_centre_step_child(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)  # the reference's sweep writes its checkpoint files into the working directory
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refbind as rb
     if not rb.available():
