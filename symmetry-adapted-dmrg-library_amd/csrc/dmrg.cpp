@@ -517,12 +517,13 @@ void Dmrg::svd(const BTensor& vec, int a, int b, double p1, double p2) {
       std::fprintf(stderr,
                    "[sadmrg]   svd: gram=%lld gesdd=%lld redo=%lld max_sector=%lld t_gram=%.4f t_gesdd=%.4f "
                    "t_vectors=%.4f | eig: matrices=%lld launches=%lld max_order=%lld trd=%.4f bisect=%.4f "
-                   "invit=%.4f ortho=%.4f backt=%.4f vectors=%lld clusters=%lld max_cluster=%lld ns_steps=%lld\n",
+                   "invit=%.4f (kernel %.4f) ortho=%.4f backt=%.4f vectors=%lld clusters=%lld max_cluster=%lld ns_steps=%lld\n",
                    (long long)(s1.gram_sectors - s0.gram_sectors), (long long)(s1.svd_sectors - s0.svd_sectors),
                    (long long)(s1.gram_redo - s0.gram_redo), (long long)s1.max_sector, s1.t_gram - s0.t_gram,
                    s1.t_gesdd - s0.t_gesdd, s1.t_vectors - s0.t_vectors, (long long)(e1.matrices - e0.matrices),
                    (long long)(e1.launches - e0.launches), (long long)e1.max_order, e1.trd - e0.trd,
-                   e1.bisect - e0.bisect, e1.invit - e0.invit, e1.ortho - e0.ortho, e1.backt - e0.backt,
+                   e1.bisect - e0.bisect, e1.invit - e0.invit, e1.invit_kernel - e0.invit_kernel, e1.ortho - e0.ortho,
+                   e1.backt - e0.backt,
                    (long long)(e1.vectors - e0.vectors), (long long)(e1.clusters - e0.clusters),
                    (long long)e1.max_cluster, (long long)(e1.ns_steps - e0.ns_steps));
     }

@@ -61,7 +61,8 @@ struct TrdTask {
   double* e;
   double* tau;
   uint64_t* pub;  // 2 parities x n rows x 4 tagged words {p lo, p hi, col lo, col hi}
-  int n, lda, wg0, nwg, rows;
+  int n, lda, wg0, nwg, rows;  // rows: LDS-held rows per workgroup
+  int rreg;                     // register-held rows per workgroup (one per wave, waves 0 .. rreg-1)
   int ldr;        // LDS stride of rows and vectors: n rounded up to a multiple of 64
 };
 
@@ -131,6 +132,18 @@ __device__ inline void put_tagged(uint64_t* p, double x, uint32_t tag) {
   __hip_atomic_store(p + 1, (b >> 32) | (uint64_t(tag) << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Register-held rows (one per wave) fit next to the rest of the kernel's
+// registers for rows of up to 64 * 24 entries.
+constexpr int kTrdMaxRegNT = 24;
+template <int NT>
+constexpr bool kTrdRegRows = NT <= kTrdMaxRegNT;
+
+// The instantiation (NT) that holds rows of n entries: 64 NT >= n.
+inline int trd_template(int n) {
+  const int nt = (n + 63) / 64;
+  return nt <= 4 ? 4 : nt <= 8 ? 8 : nt <= 12 ? 12 : nt <= 16 ? 16 : nt <= 20 ? 20 : nt <= 24 ? 24 : 32;
+}
+
 template <int NT>
 __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restrict__ tasks, int ntask,
                                                           int* __restrict__ err, long long wait_ticks) {
@@ -165,6 +178,17 @@ __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restr
       rows[static_cast<size_t>(i) * ldr + j] =
           (r >= n || j >= n) ? 0.0 : (j >= r ? A[j + static_cast<size_t>(r) * lda] : A[r + static_cast<size_t>(j) * lda]);
   }
+  // a register-held row for waves below rreg: row index R + wave, lane l
+  // holding entries l + 64 m
+  const int rrow = (kTrdRegRows<NT> && wv < T.rreg) ? w + W * (R + wv) : n;  // n: none
+  double areg[kTrdRegRows<NT> ? NT : 1];
+#pragma unroll
+  for (int m = 0; m < (kTrdRegRows<NT> ? NT : 1); ++m) {
+    const int j = lane + 64 * m;
+    areg[m] = (rrow >= n || j >= n) ? 0.0
+                                    : (j >= rrow ? A[j + static_cast<size_t>(rrow) * lda]
+                                                 : A[rrow + static_cast<size_t>(j) * lda]);
+  }
   for (int j = tid; j < 3 * ldr; j += kTrdThreads) v[j] = 0.0;
   __syncthreads();
   for (int j = 1 + tid; j < n; j += kTrdThreads) v[j] = A[j];
@@ -185,7 +209,7 @@ __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restr
     // per row.  Each row is processed in chunks of kChunk slices: loaded,
     // updated, stored back.
     {
-      constexpr int NR = NT <= 16 ? NT : 12;
+      constexpr int NR = NT <= 16 ? NT : (kTrdRegRows<NT> ? 8 : 12);
       constexpr int kChunk = 4;
       static_assert(NR % kChunk == 0 && NT % kChunk == 0, "chunks must tile the register-held and LDS slices");
       double vs[NR], vps[NR], wps[NR];
@@ -256,6 +280,36 @@ __global__ __launch_bounds__(kTrdThreads) void trd_kernel(const TrdTask* __restr
         acc = wave_sum(acc);
         if (lane == 0) put_tagged(pub + 4 * static_cast<size_t>(r), tau * acc, tag);
         if (lane == ((k + 1) & 63)) put_tagged(pub + 4 * static_cast<size_t>(r) + 2, cn, tag);
+      }
+      // the wave's register-held row (wave-uniform condition)
+      if constexpr (kTrdRegRows<NT>) if (rrow < n && rrow > k) {
+        const double vr = k > 0 ? vp[rrow] : 0.0, wr = k > 0 ? wp[rrow] : 0.0;
+        double acc = 0.0, cn = 0.0;
+#pragma unroll
+        for (int m = 0; m < NT; ++m) {
+          const int j = lane + 64 * m;
+          double vsm, vpsm, wpsm;
+          if (m < NR) {
+            vsm = vs[m];
+            vpsm = vps[m];
+            wpsm = wps[m];
+          } else {
+            const int jc = min(j, ldr - 1);
+            const bool on = j > k && j < n;
+            const double a0 = v[jc], b0 = vp[jc], c0 = wp[jc];
+            vsm = on ? a0 : 0.0;
+            vpsm = (on && k > 0) ? b0 : 0.0;
+            wpsm = (on && k > 0) ? c0 : 0.0;
+          }
+          // zero vector slices leave entries j <= k and j >= n as they are
+          const double x = fma(-wr, vpsm, fma(-vr, wpsm, areg[m]));
+          acc = fma(x, vsm, acc);
+          if (j == k + 1) cn = x;
+          areg[m] = x;
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) put_tagged(pub + 4 * static_cast<size_t>(rrow), tau * acc, tag);
+        if (lane == ((k + 1) & 63)) put_tagged(pub + 4 * static_cast<size_t>(rrow) + 2, cn, tag);
       }
     }
     // reflector k to column k (every workgroup loaded its rows before it
@@ -469,23 +523,27 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
 // absolute accuracy), at most kMaxCluster members each.  For each member,
 // from the largest: three solves of (T - lambda I) x = y from a seeded
 // random start, each followed by Gram-Schmidt against the cluster's earlier
-// members and normalisation.  Eigenvalues further apart converge to vectors
-// whose overlaps are about eps |T| / gap (at most ~0.03), which the
-// orthonormalisation after this kernel removes; a vector's error is then a
-// rotation among eigenvectors of close eigenvalues, which leaves the span of
-// the kept set -- all the truncation uses -- unchanged to that order.  The
-// split of a longer run of exactly degenerate eigenvalues into chunks gives
-// chunks of random, hence well-conditioned, vectors of the same eigenspace.  Each solve runs the LU factorisation with partial pivoting
-// (dlagtf) fused with the forward elimination, then the back substitution.
-// Vectors of different clusters are orthogonal to about eps |T| / gap; the
-// Newton-Schulz steps after this kernel make the set orthonormal.  The
-// recurrences are sequential, so a lane's factors and iterate (4n doubles)
-// live in LDS, next to those of the wave's other lanes, and every step costs
-// LDS latency instead of an HBM round trip; a wave has as many active lanes
-// as its LDS holds (one wave per workgroup).
+// members and normalisation.  Each solve runs the LU factorisation with
+// partial pivoting (dlagtf) fused with the forward elimination, then the
+// back substitution.
+//
+// Eigenvalues further apart converge to vectors whose overlaps are about
+// eps |T| / gap (at most ~0.03), which the orthonormalisation after this
+// kernel removes; a vector's error is then a rotation among eigenvectors of
+// close eigenvalues, which leaves the span of the kept set -- all the
+// truncation uses -- unchanged to that order.  The split of a longer run of
+// exactly degenerate eigenvalues into chunks gives chunks of random, hence
+// well-conditioned, vectors of the same eigenspace.
+//
+// The recurrences are sequential along the matrix, so the parallelism is
+// across lanes: all 64 lanes of a wave serve clusters of one matrix.  A
+// lane's LU factors and iterate (4 n doubles) live in a global workspace
+// interleaved by lane -- entry j of array a of lane l at (a n + j) 64 + l --
+// so every access of the wave is one coalesced 512-byte line.  The forward
+// sweep only stores; the back substitution loads entry j-1 before it uses
+// entry j, so the load latency overlaps the division chain.
 constexpr double kClusterGap = 32 * DBL_EPSILON;
 constexpr int kMaxCluster = 8;
-constexpr int kInvLdsDoubles = kTrdLdsDoubles;
 
 struct InvTask {
   const double* d;
@@ -496,7 +554,8 @@ struct InvTask {
 };
 
 struct InvWave {
-  int task0, ntask, lanes, n;  // tasks [task0, task0 + ntask), lanes = LDS slots of the wave
+  int task0, ntask, n;
+  int64_t ws;  // workspace offset (doubles) of the wave: 4 n x 64
 };
 
 __device__ inline double seeded_uniform(uint64_t x) {
@@ -509,17 +568,26 @@ __device__ inline double seeded_uniform(uint64_t x) {
 }
 
 __global__ __launch_bounds__(64) void invit_kernel(const InvTask* __restrict__ tasks,
-                                                   const InvWave* __restrict__ waves) {
-  extern __shared__ double lds[];
+                                                   const InvWave* __restrict__ waves, double* __restrict__ work) {
+  // the wave's matrix (one per wave): diagonal and off-diagonal in LDS
+  __shared__ double sD[2048], sE[2048];
   const InvWave W = waves[blockIdx.x];
   const int lane = threadIdx.x;
+  {
+    const InvTask T0 = tasks[W.task0];
+    for (int j = lane; j < T0.n; j += 64) {
+      sD[j] = T0.d[j];
+      sE[j] = j + 1 < T0.n ? T0.e[j] : 0.0;
+    }
+  }
+  __syncthreads();
   if (lane >= W.ntask) return;
   const InvTask T = tasks[W.task0 + lane];
-  const int n = T.n, L = W.lanes;
-  const double* d = T.d;
-  const double* e = T.e;
-  auto at = [&](int arr, int j) -> double& { return lds[(static_cast<size_t>(arr) * n + j) * L + lane]; };
+  const int n = T.n;
+  double* ws = work + W.ws + lane;
+  auto at = [&](int arr, int j) -> double& { return ws[(static_cast<size_t>(arr) * n + j) * 64]; };
   constexpr int U0 = 0, U1 = 1, U2 = 2, Y = 3;
+  constexpr int kBlk = 8;  // entries loaded ahead of the recurrences
   const double tnorm = fmax(fabs(T.w[0]), fabs(T.w[n - 1]));
   const double tolp = fmax(DBL_EPSILON * tnorm, DBL_MIN);
   for (int a = 0; a < T.m; ++a) {
@@ -528,61 +596,78 @@ __global__ __launch_bounds__(64) void invit_kernel(const InvTask* __restrict__ t
     const uint64_t seed = (static_cast<uint64_t>(n) << 44) ^ (static_cast<uint64_t>(c) << 24);
     for (int j = 0; j < n; ++j) at(Y, j) = seeded_uniform(seed + j);
     for (int it = 0; it < 3; ++it) {
-      // factorise T - lambda I = P L U and apply L^-1 P to y in one sweep
-      double aj = d[0] - lam, bj = n > 1 ? e[0] : 0.0, cur = at(Y, 0);
-      double en = n > 1 ? e[0] : 0.0, dn = n > 1 ? d[1] : 0.0, en1 = n > 2 ? e[1] : 0.0;
-      for (int j = 0; j + 1 < n; ++j) {
-        const double cj = en, an = dn - lam, bn = en1;
-        if (j + 2 < n) {  // next step's operands, ahead of the division chain
-          en = en1;
-          dn = d[j + 2];
-          en1 = j + 3 < n ? e[j + 2] : 0.0;
+      // factorise T - lambda I = P L U and apply L^-1 P to y in one sweep;
+      // y[j+1] of a block of steps is loaded before the block (step j
+      // stores y[j] only, so the loaded entries are still the inputs)
+      double aj = sD[0] - lam, bj = sE[0], cur = at(Y, 0);
+      for (int j0 = 0; j0 + 1 < n; j0 += kBlk) {
+        double yb[kBlk];
+#pragma unroll
+        for (int q = 0; q < kBlk; ++q) yb[q] = j0 + q + 1 < n ? at(Y, j0 + q + 1) : 0.0;
+#pragma unroll
+        for (int q = 0; q < kBlk; ++q) {
+          const int j = j0 + q;
+          if (j + 1 >= n) break;
+          const double cj = sE[j], an = sD[j + 1] - lam, bn = sE[j + 1];
+          const double y1 = yb[q];
+          double mm, top, bot;
+          if (fabs(aj) >= fabs(cj)) {
+            mm = aj != 0.0 ? cj / aj : 0.0;
+            at(U0, j) = aj;
+            at(U1, j) = bj;
+            at(U2, j) = 0.0;
+            top = cur;
+            bot = y1;
+            aj = an - mm * bj;
+            bj = bn;
+          } else {
+            mm = aj / cj;
+            at(U0, j) = cj;
+            at(U1, j) = an;
+            at(U2, j) = bn;
+            top = y1;
+            bot = cur;
+            aj = bj - mm * an;
+            bj = -mm * bn;
+          }
+          at(Y, j) = top;
+          cur = bot - mm * top;
         }
-        const double y1 = at(Y, j + 1);
-        double mm, top, bot;
-        if (fabs(aj) >= fabs(cj)) {
-          mm = aj != 0.0 ? cj / aj : 0.0;
-          at(U0, j) = aj;
-          at(U1, j) = bj;
-          at(U2, j) = 0.0;
-          top = cur;
-          bot = y1;
-          aj = an - mm * bj;
-          bj = bn;
-        } else {
-          mm = aj / cj;
-          at(U0, j) = cj;
-          at(U1, j) = an;
-          at(U2, j) = bn;
-          top = y1;
-          bot = cur;
-          aj = bj - mm * an;
-          bj = -mm * bn;
-        }
-        at(Y, j) = top;
-        cur = bot - mm * top;
       }
       at(U0, n - 1) = aj;
       at(Y, n - 1) = cur;
       // back substitution (tiny pivots replaced by +-eps |T|), rescaling when
-      // the iterate grows large
+      // the iterate grows large; a block of entries is loaded before it is
+      // used (the rescaling touches only entries already solved)
       double x1 = 0.0, x2 = 0.0;
-      for (int j = n - 1; j >= 0; --j) {
-        double r = at(Y, j);
-        if (j + 1 < n) r -= at(U1, j) * x1;
-        if (j + 2 < n) r -= at(U2, j) * x2;
-        double u = at(U0, j);
-        if (fabs(u) < tolp) u = u < 0.0 ? -tolp : tolp;
-        double x = r / u;
-        if (fabs(x) > 1e150) {
-          for (int i = j + 1; i < n; ++i) at(Y, i) *= 1e-150;
-          x *= 1e-150;
-          x1 *= 1e-150;
-          x2 *= 1e-150;
+      for (int j1 = n - 1; j1 >= 0; j1 -= kBlk) {
+        double ry[kBlk], r0[kBlk], r1[kBlk], r2[kBlk];
+#pragma unroll
+        for (int q = 0; q < kBlk; ++q) {
+          const int j = j1 - q;
+          ry[q] = j >= 0 ? at(Y, j) : 0.0;
+          r0[q] = j >= 0 ? at(U0, j) : 1.0;
+          r1[q] = (j >= 0 && j + 1 < n) ? at(U1, j) : 0.0;
+          r2[q] = (j >= 0 && j + 2 < n) ? at(U2, j) : 0.0;
         }
-        at(Y, j) = x;
-        x2 = x1;
-        x1 = x;
+#pragma unroll
+        for (int q = 0; q < kBlk; ++q) {
+          const int j = j1 - q;
+          if (j < 0) break;
+          double r = ry[q] - r1[q] * x1 - r2[q] * x2;
+          double u = r0[q];
+          if (fabs(u) < tolp) u = u < 0.0 ? -tolp : tolp;
+          double x = r / u;
+          if (fabs(x) > 1e150) {
+            for (int i = j + 1; i < n; ++i) at(Y, i) *= 1e-150;
+            x *= 1e-150;
+            x1 *= 1e-150;
+            x2 *= 1e-150;
+          }
+          at(Y, j) = x;
+          x2 = x1;
+          x1 = x;
+        }
       }
       // Gram-Schmidt against the cluster's earlier members, then normalise
       for (int b = 0; b < a; ++b) {
@@ -612,28 +697,51 @@ __global__ __launch_bounds__(256) void ns_coef_kernel(const double* __restrict__
   B[i] = diag - 0.5 * M[i];
 }
 
-// max over every slot's leading k_s x k_s block of |M - I| (one workgroup per slot)
+// max over every slot's leading k_s x k_s block of |M - I|: blockIdx.y is
+// the slot, blockIdx.x a range of kOrthoCols columns.  Non-negative doubles
+// order like their bit patterns, so the workgroups' maxima meet in an integer
+// atomic max on out[s] (cleared to zero before the launch).
+constexpr int kOrthoCols = 16;
 __global__ __launch_bounds__(256) void ortho_err_kernel(const double* __restrict__ M, const int* __restrict__ ks,
-                                                        int k, double* __restrict__ out) {
-  const int s = blockIdx.x;
+                                                        int k, unsigned long long* __restrict__ out) {
+  const int s = blockIdx.y;
   const int kk = ks[s];
+  const int c0 = blockIdx.x * kOrthoCols;
+  if (c0 >= kk) return;
+  const int c1 = min(kk, c0 + kOrthoCols);
   const double* Ms = M + static_cast<size_t>(s) * k * k;
   double m = 0.0;
-  for (int64_t i = threadIdx.x; i < int64_t(kk) * kk; i += 256) {
-    const int r = static_cast<int>(i % kk), c = static_cast<int>(i / kk);
-    m = fmax(m, fabs(Ms[r + static_cast<size_t>(c) * k] - (r == c ? 1.0 : 0.0)));
-  }
+  for (int c = c0; c < c1; ++c)
+    for (int r = threadIdx.x; r < kk; r += 256)
+      m = fmax(m, fabs(Ms[r + static_cast<size_t>(c) * k] - (r == c ? 1.0 : 0.0)));
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   __shared__ double red[4];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) out[s] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  if (threadIdx.x == 0)
+    atomicMax(out + s, static_cast<unsigned long long>(
+                           __double_as_longlong(fmax(fmax(red[0], red[1]), fmax(red[2], red[3])))));
 }
 
 // ---------------------------------------------------------------------------
-// 4. Back-transformation Z <- H_0 H_1 ... H_{n-2} Z.  One wave per pair of
-// columns, held in registers (lane l keeps rows l + 64 m, m < NT); the
-// reflectors are applied from the last to the first.
+// 4. Back-transformation Z <- H_0 H_1 ... H_{n-2} Z, the reflectors applied
+// from the last to the first.  A workgroup of kBtWaves waves serves up to
+// two columns per wave, held in registers (lane l keeps rows l + 64 m,
+// m < NT).  The reflectors pass through LDS in blocks of kBtBlkOf<NT>,
+// double-buffered: the next block's global loads are issued before the
+// current block is applied and stored after it, one barrier per block, so
+// every reflector is read from memory once per workgroup.
+constexpr int kBtWaves = 8;
+constexpr int kBtThreads = 64 * kBtWaves;
+// the longest rows (NT > 24) take one column per wave and blocks of two
+// reflectors, so their registers fit
+template <int NT>
+constexpr int kBtBlkOf = NT > 24 ? 2 : 4;
+template <int NT>
+constexpr int kBtColsPerWave = NT > 24 ? 1 : 2;
+template <int NT>
+constexpr int kBtColsOf = kBtColsPerWave<NT> * kBtWaves;
+
 struct BtTask {
   const double* a;
   const double* tau;
@@ -642,57 +750,91 @@ struct BtTask {
 };
 
 template <int NT>
-__global__ __launch_bounds__(256) void bt_kernel(const BtTask* __restrict__ tasks, int ntask) {
+__global__ __launch_bounds__(kBtThreads) void bt_kernel(const BtTask* __restrict__ tasks, int ntask) {
+  extern __shared__ double sv[];  // 2 buffers x kBtBlkOf<NT> reflectors x 64 NT entries
+  constexpr int kBtBlk = kBtBlkOf<NT>;
+  __shared__ double stau[2][kBtBlk];
   int t = 0;
   while (t < ntask && static_cast<int>(blockIdx.x) >= tasks[t].blk0 + tasks[t].nblk) ++t;
   if (t >= ntask) return;
   const BtTask T = tasks[t];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int c0 = ((static_cast<int>(blockIdx.x) - T.blk0) * 4 + wv) * 2;
-  if (c0 >= T.k) return;
-  const bool two = c0 + 1 < T.k;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int L = 64 * NT;  // LDS stride of one reflector
   const int n = T.n;
-  double z0[NT], z1[NT];
+  constexpr bool kTwo = kBtColsPerWave<NT> == 2;
+  const int c0 = (static_cast<int>(blockIdx.x) - T.blk0) * kBtColsOf<NT> + kBtColsPerWave<NT> * wv;
+  const bool one = c0 < T.k, two = kTwo && c0 + 1 < T.k;  // wave-uniform
+  double z0[NT], z1[kTwo ? NT : 1];
   const double* Z0 = T.z + static_cast<size_t>(c0) * T.ldz;
   const double* Z1 = Z0 + T.ldz;
 #pragma unroll
   for (int m = 0; m < NT; ++m) {
     const int j = lane + 64 * m;
-    z0[m] = j < n ? Z0[j] : 0.0;
-    z1[m] = (two && j < n) ? Z1[j] : 0.0;
+    z0[m] = (one && j < n) ? Z0[j] : 0.0;
+    if constexpr (kTwo) z1[m] = (two && j < n) ? Z1[j] : 0.0;
   }
-  // reflector i's vector, loaded one reflector ahead
-  auto load_v = [&](int i, double (&vv)[NT]) {
-    const double* col = T.a + static_cast<size_t>(i) * T.lda;
+  // block b holds reflectors hi(b), hi(b)-1, ..., at most kBtBlk, hi(b) = n-2-b kBtBlk
+  const int nref = n - 1;
+  const int nblocks = nref > 0 ? (nref + kBtBlk - 1) / kBtBlk : 0;
+  constexpr int kPf = (kBtBlk * L + kBtThreads - 1) / kBtThreads;  // entries per thread per block
+  double pf[kPf];
+  auto fetch = [&](int b) {
+    const int hi = n - 2 - b * kBtBlk;
 #pragma unroll
-    for (int m = 0; m < NT; ++m) {
-      const int j = lane + 64 * m;
-      vv[m] = j == i + 1 ? 1.0 : ((j > i + 1 && j < n) ? col[j] : 0.0);
+    for (int p = 0; p < kPf; ++p) {
+      const int e = tid + kBtThreads * p;
+      const int r = e / L, j = e - r * L;
+      const int i = hi - r;
+      double x = 0.0;
+      if (r < kBtBlk && i >= 0) x = j == i + 1 ? 1.0 : ((j > i + 1 && j < n) ? T.a[j + static_cast<size_t>(i) * T.lda] : 0.0);
+      pf[p] = x;
     }
   };
-  double vn[NT];
-  if (n >= 2) load_v(n - 2, vn);
-  for (int i = n - 2; i >= 0; --i) {
-    double vv[NT];
+  auto stash = [&](int b, int buf) {
+    const int hi = n - 2 - b * kBtBlk;
 #pragma unroll
-    for (int m = 0; m < NT; ++m) vv[m] = vn[m];
-    if (i > 0) load_v(i - 1, vn);
-    const double ti = T.tau[i];
-    if (ti == 0.0) continue;
-    double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-    for (int m = 0; m < NT; ++m) {
-      s0 = fma(vv[m], z0[m], s0);
-      s1 = fma(vv[m], z1[m], s1);
+    for (int p = 0; p < kPf; ++p) {
+      const int e = tid + kBtThreads * p;
+      if (e < kBtBlk * L) sv[buf * kBtBlk * L + e] = pf[p];
     }
-    s0 = __shfl(wave_sum(s0), 0, 64) * ti;
-    s1 = __shfl(wave_sum(s1), 0, 64) * ti;
-#pragma unroll
-    for (int m = 0; m < NT; ++m) {
-      z0[m] = fma(-s0, vv[m], z0[m]);
-      z1[m] = fma(-s1, vv[m], z1[m]);
-    }
+    if (tid < kBtBlk) stau[buf][tid] = hi - tid >= 0 ? T.tau[hi - tid] : 0.0;
+  };
+  if (nblocks > 0) {
+    fetch(0);
+    stash(0, 0);
   }
+  __syncthreads();
+  for (int b = 0; b < nblocks; ++b) {
+    const int buf = b & 1;
+    if (b + 1 < nblocks) fetch(b + 1);
+    if (one) {
+      const double* vb = sv + buf * kBtBlk * L;
+#pragma unroll 1
+      for (int r = 0; r < kBtBlk; ++r) {
+        const double ti = stau[buf][r];
+        if (ti == 0.0) continue;  // also reflectors past the first
+        const double* vv = vb + r * L;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int m = 0; m < NT; ++m) {
+          const double x = vv[lane + 64 * m];
+          s0 = fma(x, z0[m], s0);
+          if constexpr (kTwo) s1 = fma(x, z1[m], s1);
+        }
+        s0 = wave_sum(s0) * ti;
+        if constexpr (kTwo) s1 = wave_sum(s1) * ti;
+#pragma unroll
+        for (int m = 0; m < NT; ++m) {
+          const double x = vv[lane + 64 * m];
+          z0[m] = fma(-s0, x, z0[m]);
+          if constexpr (kTwo) z1[m] = fma(-s1, x, z1[m]);
+        }
+      }
+    }
+    if (b + 1 < nblocks) stash(b + 1, buf ^ 1);
+    __syncthreads();
+  }
+  if (!one) return;
   double* W0 = T.z + static_cast<size_t>(c0) * T.ldz;
   double* W1 = W0 + T.ldz;
 #pragma unroll
@@ -700,14 +842,23 @@ __global__ __launch_bounds__(256) void bt_kernel(const BtTask* __restrict__ task
     const int j = lane + 64 * m;
     if (j < n) {
       W0[j] = z0[m];
-      if (two) W1[j] = z1[m];
+      if constexpr (kTwo)
+        if (two) W1[j] = z1[m];
     }
   }
 }
 
 template <int NT>
 hipError_t launch_bt(const BtTask* tasks, int ntask, int nblk, hipStream_t st) {
-  hipLaunchKernelGGL(bt_kernel<NT>, dim3(nblk), dim3(256), 0, st, tasks, ntask);
+  constexpr int lds = 2 * kBtBlkOf<NT> * 64 * NT * 8;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(bt_kernel<NT>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL(bt_kernel<NT>, dim3(nblk), dim3(kBtThreads), lds, st, tasks, ntask);
   return hipGetLastError();
 }
 
@@ -754,7 +905,8 @@ int SymEigBatch::max_order() {
     for (int n = 2048; n > 1; --n) {
       const int ldr = (n + 63) / 64 * 64;
       const int rmax = (kTrdLdsDoubles - 3 * ldr) / ldr;
-      if (rmax >= 1 && (n + rmax - 1) / rmax <= cus) return n;
+      const int cap = rmax + (trd_template(n) > kTrdMaxRegNT ? 0 : kTrdWaves);
+      if (rmax >= 1 && (n + cap - 1) / cap <= cus) return n;
     }
     return 1;
   }();
@@ -808,9 +960,11 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
   // publication words and the error flag: zero bits
   SADMRG_HIP(launch_fill((L.err + 8 - L.pubs) / 8, 0.0, reinterpret_cast<double*>(pubs), ctx->stream()));
   // workgroups per matrix: as many rows per workgroup as LDS holds next to
-  // three vectors, rows spread evenly
+  // three vectors, plus one register-held row per wave where the kernel
+  // instantiation has them; rows spread evenly, LDS first
   struct Plan {
-    int q, W, R;
+    int q, W, R, rreg;
+    bool big;  // rows longer than the register-row instantiations take
   };
   std::vector<Plan> plans;
   for (int q = 0; q < nj; ++q) {
@@ -819,10 +973,17 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
     const int ldr = (n + 63) / 64 * 64;
     const int rmax = (kTrdLdsDoubles - 3 * ldr) / ldr;
     require(rmax >= 1, "eigensolver: matrix too large for the LDS-resident reduction");
-    const int W = (n + rmax - 1) / rmax;
-    plans.push_back(Plan{q, W, (n + W - 1) / W});
+    const bool big = trd_template(n) > kTrdMaxRegNT;
+    const int reg = big ? 0 : kTrdWaves;
+    const int W = (n + rmax + reg - 1) / (rmax + reg);
+    const int Rt = (n + W - 1) / W;
+    const int R = std::min(Rt, rmax);
+    plans.push_back(Plan{q, W, R, Rt - R, big});
   }
-  std::stable_sort(plans.begin(), plans.end(), [](const Plan& x, const Plan& y) { return x.W > y.W; });
+  // launches hold one kind of rows only (the instantiation decides whether
+  // register rows exist); the widest matrices first
+  std::stable_sort(plans.begin(), plans.end(),
+                   [](const Plan& x, const Plan& y) { return x.big != y.big ? x.big : x.W > y.W; });
   // cooperative launches stay a margin below the CU count, so a grid never
   // depends on every CU being free at once
   const int cus = device_cus() - kCuMargin;
@@ -836,13 +997,27 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
   std::vector<char> host(L.total - L.tasks, 0);
   TrdTask* htasks = reinterpret_cast<TrdTask*>(host.data());
   BisTask* hbis = reinterpret_cast<BisTask*>(host.data() + (L.bis - L.tasks));
+  // first-fit decreasing: each plan, widest first, joins the first launch of
+  // its kind with room, or opens one
+  std::vector<std::vector<size_t>> bins;
+  std::vector<int> bin_wg;
+  for (size_t pi = 0; pi < plans.size(); ++pi) {
+    const Plan& P = plans[pi];
+    require(P.W <= cus, "eigensolver: matrix needs more workgroups than the device holds");
+    size_t b = 0;
+    while (b < bins.size() && !(plans[bins[b][0]].big == P.big && bin_wg[b] + P.W <= cus)) ++b;
+    if (b == bins.size()) {
+      bins.emplace_back();
+      bin_wg.push_back(0);
+    }
+    bins[b].push_back(pi);
+    bin_wg[b] += P.W;
+  }
   int nt_all = 0;
-  for (size_t p0 = 0; p0 < plans.size();) {
+  for (const std::vector<size_t>& bin : bins) {
     Launch l{nt_all, 0, 0, 0, 0};
-    size_t p1 = p0;
-    while (p1 < plans.size() && (l.wg + plans[p1].W <= cus || l.ntask == 0)) {
-      const Plan& P = plans[p1];
-      require(P.W <= cus, "eigensolver: matrix needs more workgroups than the device holds");
+    for (size_t pi : bin) {
+      const Plan& P = plans[pi];
       const EigJob& J = jobs[P.q];
       TrdTask t;
       t.a = J.a;
@@ -855,6 +1030,7 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
       t.wg0 = l.wg;
       t.nwg = P.W;
       t.rows = P.R;
+      t.rreg = P.rreg;
       t.ldr = (J.n + 63) / 64 * 64;
       std::memcpy(htasks + nt_all, &t, sizeof(TrdTask));
       ++nt_all;
@@ -862,10 +1038,8 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
       l.wg += P.W;
       l.lds = std::max(l.lds, (P.R + 3) * t.ldr * 8);
       l.nt = std::max(l.nt, (J.n + 63) / 64);
-      ++p1;
     }
     launches.push_back(l);
-    p0 = p1;
   }
   int total = 0, nb = 0;
   for (int q = 0; q < nj; ++q) {
@@ -1001,33 +1175,30 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
     }
   }
   {
-    // waves of lanes with the same order: as many lanes as LDS holds 4n doubles
+    // waves of up to 64 lanes (clusters) of one matrix, each wave with a
+    // workspace of 4 n x 64 doubles
     std::vector<InvWave> wv;
-    int lds = 0;
+    int64_t ws = 0;
     for (size_t t0 = 0; t0 < it.size();) {
       const int n = it[t0].n;
-      const int lanes = std::max(1, std::min(64, kInvLdsDoubles / (4 * n)));
       int cnt = 0;
-      while (t0 + cnt < it.size() && cnt < lanes && it[t0 + cnt].n == n && it[t0 + cnt].z == it[t0].z) ++cnt;
-      wv.push_back(InvWave{static_cast<int>(t0), cnt, lanes, n});
-      lds = std::max(lds, lanes * 4 * n * 8);
+      while (t0 + cnt < it.size() && cnt < 64 && it[t0 + cnt].z == it[t0].z) ++cnt;
+      wv.push_back(InvWave{static_cast<int>(t0), cnt, n, ws});
+      ws += 4 * int64_t(n) * 64;
       t0 += cnt;
     }
-    static bool attr = false;
-    if (!attr) {
-      SADMRG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(invit_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, kInvLdsDoubles * 8));
-      attr = true;
-    }
-    DevBuf dt(ctx, it.size() * sizeof(InvTask)), dw(ctx, wv.size() * sizeof(InvWave));
+    DevBuf dt(ctx, it.size() * sizeof(InvTask)), dw(ctx, wv.size() * sizeof(InvWave)), work(ctx, ws * 8);
     SADMRG_HIP(
         hipMemcpyAsync(dt.get(), it.data(), it.size() * sizeof(InvTask), hipMemcpyHostToDevice, ctx->stream()));
     SADMRG_HIP(
         hipMemcpyAsync(dw.get(), wv.data(), wv.size() * sizeof(InvWave), hipMemcpyHostToDevice, ctx->stream()));
-    hipLaunchKernelGGL(invit_kernel, dim3(wv.size()), dim3(64), static_cast<unsigned>(lds), ctx->stream(),
-                       dt.as<InvTask>(), dw.as<InvWave>());
+    ctx->sync();
+    const auto tk = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(invit_kernel, dim3(wv.size()), dim3(64), 0, ctx->stream(), dt.as<InvTask>(), dw.as<InvWave>(),
+                       work.as<double>());
     SADMRG_HIP(hipGetLastError());
     ctx->sync();
+    g_prof.invit_kernel += secs_since(tk);
   }
   g_prof.invit += secs_since(t0);
   t0 = std::chrono::steady_clock::now();
@@ -1053,8 +1224,9 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
                                       zcur, nmax, slot, zcur, nmax, slot, &zero, gm.as<double>(), kmax, sg, ns) !=
         rocblas_status_success)
       fail(kErrNumerics, "eigensolver: rocblas gemm failed");
-    hipLaunchKernelGGL(ortho_err_kernel, dim3(ns), dim3(256), 0, ctx->stream(), gm.as<double>(), dks.as<int>(), kmax,
-                       derr.as<double>());
+    SADMRG_HIP(launch_fill(ns, 0.0, derr.as<double>(), ctx->stream()));
+    hipLaunchKernelGGL(ortho_err_kernel, dim3((kmax + kOrthoCols - 1) / kOrthoCols, ns), dim3(256), 0, ctx->stream(),
+                       gm.as<double>(), dks.as<int>(), kmax, derr.as<unsigned long long>());
     SADMRG_HIP(hipGetLastError());
     std::vector<double> herr(ns);
     SADMRG_HIP(hipMemcpyAsync(herr.data(), derr.get(), ns * 8, hipMemcpyDeviceToHost, ctx->stream()));
@@ -1098,6 +1270,9 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
   // back-transformation on the padded slots, then the k columns to each job
   std::vector<BtTask> bts;
   int blk = 0;
+  // columns per workgroup of the instantiation the launch takes (NT of nmax)
+  const int bt_cols = (trd_template(nmax) > kTrdMaxRegNT ? 1 : 2) * kBtWaves;
+  static_assert(kBtColsPerWave<24> == 2 && kBtColsPerWave<32> == 1, "columns per wave follow the row length");
   for (int s = 0; s < ns; ++s) {
     const EigJob& J = jobs[sel[s]];
     BtTask b;
@@ -1109,7 +1284,7 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
     b.ldz = nmax;
     b.k = J.k;
     b.blk0 = blk;
-    b.nblk = (J.k + 7) / 8;
+    b.nblk = (J.k + bt_cols - 1) / bt_cols;
     bts.push_back(b);
     blk += b.nblk;
   }

@@ -30,7 +30,7 @@ namespace sadmrg {
 // Process-wide phase timers and counters of the solver (wall seconds; every
 // phase ends with a stream synchronisation, so the host clock times it).
 struct EigProfile {
-  double trd = 0, bisect = 0, invit = 0, ortho = 0, backt = 0;
+  double trd = 0, bisect = 0, invit = 0, invit_kernel = 0, ortho = 0, backt = 0;
   int64_t matrices = 0, launches = 0, vectors = 0, clusters = 0, max_cluster = 0, ns_steps = 0, max_order = 0;
 };
 
