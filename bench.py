@@ -522,7 +522,7 @@ def env_run(sad, args):
 
 
 def sweep_compare(sad, ctx, d, width, length, D, warm, j2=0.5, refine_tol=0.0, refine_krylov=6, check_every=(4,),
-                  bf16=False, after_fp64=None):
+                  bf16=False, after_fp64=None, fp64_every=()):
     """Mixed vs FP64 sweep on the warmed-up SU(2) J1-J2 cylinder (width x
     length, snake ordering; see warm_up): the state is cloned on the device
     and one sweep is timed in each precision from the identical start.  The
@@ -535,15 +535,19 @@ def sweep_compare(sad, ctx, d, width, length, D, warm, j2=0.5, refine_tol=0.0, r
     out = {}
     after = None
     memory = {}
-    legs = [("fp64", sad.FP64, 4), ("mixed", sad.FP32, check_every[0])]
-    legs += [(f"mixed_check{k}", sad.FP32, k) for k in check_every[1:]]
+    legs = [("fp64", sad.FP64, 4, 1), ("mixed", sad.FP32, check_every[0], 1)]
+    legs += [(f"mixed_check{k}", sad.FP32, k, 1) for k in check_every[1:]]
+    # FP64 energy at the sweep's last step only; the other steps stop on the
+    # FP32 recurrence residual and report FP32 Ritz values
+    legs += [(f"mixed_fp64_every{k}", sad.FP32, check_every[0], k) for k in fp64_every]
     if bf16:  # ablation: BF16 Krylov matvecs, same FP64 checks and refinement
-        legs.append(("bf16", sad.BF16, check_every[0]))
-    for name, prec, every in legs:
+        legs.append(("bf16", sad.BF16, check_every[0], 1))
+    for name, prec, every, e64 in legs:
         c = d.clone()
         c.set_precision(prec, refine=(prec != sad.FP64), refine_tol=refine_tol)
         c.set_refinement(refine_krylov, 6)
         c.set_check_interval(every)
+        c.set_fp64_energy_every(e64)
         ctx.sync()
         sad.svd_stats(reset=True)
         sad.lanczos_mode(reset=True)
@@ -563,6 +567,23 @@ def sweep_compare(sad, ctx, d, width, length, D, warm, j2=0.5, refine_tol=0.0, r
     e64 = np.asarray([r["energy"] for r in out["fp64"]["recs"]])
     emx = np.asarray([r["energy"] for r in out["mixed"]["recs"]])
     rel = np.abs(emx - e64) / np.abs(e64)
+    # the FP32 Krylov Ritz value each mixed step had before its FP64 check
+    r32 = np.asarray([r["ritz32"] for r in out["mixed"]["recs"]])
+    rel32 = np.abs(r32 - emx) / np.abs(emx)
+
+    def variant(name):
+        recs = out[name]["recs"]
+        e = np.asarray([r["energy"] for r in recs])
+        f64 = np.asarray([r["energy64"] for r in recs]) > 0
+        rv = np.abs(e - e64) / np.abs(e64)
+        return {"sweep_s": round(out[name]["seconds"], 3),
+                "speedup": round(out["fp64"]["seconds"] / out[name]["seconds"], 3),
+                "final_rel_dE": float(abs(out[name]["energy"] - out["fp64"]["energy"]) / abs(out["fp64"]["energy"])),
+                "max_rel_dE_per_step": float(rv.max()),
+                "fp64_energy_steps": int(f64.sum()),
+                "max_rel_dE_fp64_steps": float(rv[f64].max()) if f64.any() else None,
+                "split_s": {k: round(tsum(name, k), 3) for k in ("t_env", "t_lanczos", "t_svd")},
+                "matvecs": [int(tsum(name, "matvec32")), int(tsum(name, "matvec64"))]}
 
     def tsum(name, key):
         return float(sum(r[key] for r in out[name]["recs"]))
@@ -593,15 +614,10 @@ def sweep_compare(sad, ctx, d, width, length, D, warm, j2=0.5, refine_tol=0.0, r
         "centre_step": {"fp64": centre("fp64"), "mixed": centre("mixed")},
         "lanczos_graph_replays": {"fp64": out["fp64"]["graph_replays"], "mixed": out["mixed"]["graph_replays"]},
         "truncation": {"fp64": out["fp64"]["svd"], "mixed": out["mixed"]["svd"]},
-        "variants": {name: {"sweep_s": round(out[name]["seconds"], 3),
-                            "speedup": round(out["fp64"]["seconds"] / out[name]["seconds"], 3),
-                            "final_rel_dE": float(abs(out[name]["energy"] - out["fp64"]["energy"]) /
-                                                  abs(out["fp64"]["energy"])),
-                            "max_rel_dE_per_step": float(np.max(np.abs(
-                                np.asarray([r["energy"] for r in out[name]["recs"]]) - e64) / np.abs(e64))),
-                            "split_s": {k: round(tsum(name, k), 3) for k in ("t_env", "t_lanczos", "t_svd")},
-                            "matvecs": [int(tsum(name, "matvec32")), int(tsum(name, "matvec64"))]}
-                     for name, _, _ in legs[2:]},
+        "ritz32_vs_fp64_rayleigh": {"max_rel": float(rel32.max()), "median_rel": float(np.median(rel32)),
+                                    "note": "per mixed step: the FP32 Krylov Ritz value before the FP64 check "
+                                            "against that step's FP64 Rayleigh quotient"},
+        "variants": {name: variant(name) for name, _, _, _ in legs[2:]},
     }
 
 
@@ -740,6 +756,9 @@ def main():
                          "c1: U(1) L=32 m=64 four sweeps vs the reference; "
                          "c4: 64 U(1) complex chains at m=256, all centre bonds in one launch; "
                          "env: environment updates at the C3 centre bond")
+    ap.add_argument("--fp64-energy-every", default="0",
+                    help="mixed sweep variants computing the FP64 energy at every k-th step and the last "
+                         "(0: the last step only); comma list, '' for none")
     ap.add_argument("--sweep", default="6x24:2048:128,512",
                     help="WIDTHxLENGTH:D:warm-up Ds for the mixed vs FP64 sweep comparison ('none' skips it)")
     args = ap.parse_args()
@@ -893,6 +912,7 @@ def main():
         real, sweep = sweep_compare(
             sad, ctx, dr, w_, l_, int(dd), warm_d, refine_tol=args.refine_tol, refine_krylov=args.refine_krylov,
             check_every=tuple(int(x) for x in args.check_every.split(",")), bf16=args.bf16_sweep,
+            fp64_every=tuple(int(x) for x in args.fp64_energy_every.split(",") if x != ""),
             after_fp64=lambda c: real_centre(sad, tl, ctx, c, w_, l_, int(dd), hmap, coup, args, timer, real_path))
         sweep["warm_up_s"] = round(t_warm, 2)
         del dr

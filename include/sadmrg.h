@@ -313,6 +313,15 @@ int sadmrg_dmrg_set_refinement(sadmrg_dmrg* d, int krylov_steps, int max_restart
    reference checks every 4th (dmrg_inst.cpp:1334), the default.  A shorter
    interval lets the mixed path stop with fewer FP32 matvecs. */
 int sadmrg_dmrg_set_check_interval(sadmrg_dmrg* d, int every);
+/* Mixed precision with refinement (no reference counterpart): which sweep
+   steps compute the FP64 energy, i.e. the FP64 Rayleigh quotient and the
+   residual-correction restarts.  1 (default): every step.  k > 1: every
+   k-th step of a sweep and its last step.  0: the last step only.  The
+   other steps make no FP64 matvec: they stop when the recurrence residual
+   beta_m |y_m| of the FP32 Krylov Ritz pair is at most refine_tol |E| and
+   record that Ritz value (flag 0 in sadmrg_dmrg_records).  iDMRG steps
+   always compute the FP64 energy. */
+int sadmrg_dmrg_set_fp64_energy_every(sadmrg_dmrg* d, int every);
 /* 1: one line per sweep step on stderr (bond, energy, kept states, stage
    times), as the reference prints every step of DMRG::sweep to cout
    (dmrg_inst.cpp:881-897); 0 (default): silent. */
@@ -349,10 +358,13 @@ int sadmrg_dmrg_idmrg(sadmrg_dmrg* d);         /* do_idmrg (dmrg.hpp:107) */
 int sadmrg_dmrg_prepare_sweep(sadmrg_dmrg* d); /* prepare_sweep (dmrg.hpp:110) */
 int sadmrg_dmrg_sweep(sadmrg_dmrg* d);         /* sweep (dmrg.hpp:111) */
 int sadmrg_dmrg_energy(sadmrg_dmrg* d, double* e); /* get_enr (dmrg.hpp:102) */
-/* per-step records since the last call: 12 doubles each {il, ir, energy,
+/* per-step records since the last call: 14 doubles each {il, ir, energy,
    truncation error, kept, FP64 matvecs, FP32 matvecs, t_env, t_lanczos, t_svd,
-   H_eff flops, reduced elements of the wave function}; *n = number written;
-   records are consumed */
+   H_eff flops, reduced elements of the wave function, the last Ritz value of
+   the FP32 Krylov recurrence (mixed precision; 0 otherwise), 1 if energy is
+   an FP64 Rayleigh quotient (0: the FP32 Ritz value, see
+   sadmrg_dmrg_set_fp64_energy_every)}; *n = number written; records are
+   consumed */
 int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n);
 /* which: 0 uu[i], 1 hh[i], 2 opr[i][j]; for stored state j < max_exci:
    3 orth[j][i], 4 ovlp[j][i], 5 overlapvec[j] (i ignored) */

@@ -118,6 +118,7 @@ def lib():
         "sadmrg_dmrg_set_precision": (i, [vp, i, i, d]),
         "sadmrg_dmrg_set_refinement": (i, [vp, i, i]),
         "sadmrg_dmrg_set_check_interval": (i, [vp, i]),
+        "sadmrg_dmrg_set_fp64_energy_every": (i, [vp, i]),
         "sadmrg_dmrg_set_verbose": (i, [vp, i]),
         "sadmrg_dmrg_save_mps": (i, [vp, C.c_char_p, i]),
         "sadmrg_dmrg_save_ww": (i, [vp, C.c_char_p]),
@@ -553,7 +554,7 @@ def svd_gram_min(value=-1):
 
 
 REC_FIELDS = ("il", "ir", "energy", "trunc_err", "kept", "matvec64", "matvec32", "t_env", "t_lanczos", "t_svd",
-              "heff_flops", "ndata")
+              "heff_flops", "ndata", "ritz32", "energy64")
 
 
 class DMRG:
@@ -633,6 +634,11 @@ class DMRG:
         """Lanczos convergence check after every `every`-th step (reference: 4)."""
         check(lib().sadmrg_dmrg_set_check_interval(self.ptr, every))
 
+    def set_fp64_energy_every(self, every=1):
+        """Mixed precision: FP64 energies at every step (1), every k-th step
+        and the last (k > 1) or the last step of a sweep only (0)."""
+        check(lib().sadmrg_dmrg_set_fp64_energy_every(self.ptr, every))
+
     # checkpoint files in the reference's format (dmrg_inst.cpp:947-1177)
     def save_mps(self, directory=None, exci=0):
         """save_mps2(exci): uu-{ns}-{bondd}-{totspin}-{exci}-{i}.dat/.bin per site."""
@@ -689,11 +695,12 @@ class DMRG:
     def records(self):
         out = []
         while True:
-            buf = np.zeros(12 * 256)
+            nf = len(REC_FIELDS)
+            buf = np.zeros(nf * 256)
             n = C.c_int()
             check(lib().sadmrg_dmrg_records(self.ptr, _dptr(buf), 256, C.byref(n)))
             for k in range(n.value):
-                out.append(dict(zip(REC_FIELDS, buf[12 * k:12 * k + 12].tolist())))
+                out.append(dict(zip(REC_FIELDS, buf[nf * k:nf * k + nf].tolist())))
             if n.value < 256:
                 return out
 

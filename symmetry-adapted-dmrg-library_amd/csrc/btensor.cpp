@@ -66,6 +66,7 @@ void BTensor::layout(Context* c, bool complex_, const Struct& s, const std::vect
   tag.assign(n, tag0);
   nelem = 0;
   fuse_id = false;
+  host.clear();
   for (int b = 0; b < n; ++b)
     if (mask[b]) {
       off[b] = nelem;
@@ -81,6 +82,27 @@ void BTensor::clean() {
   tag.clear();
   buf.release();
   nelem = 0;
+  host.clear();
+}
+
+void BTensor::keep_host() {
+  host.assign(static_cast<size_t>(nelem) * w(), 0.0);
+  if (nelem > 0) {
+    SADMRG_HIP(hipMemcpyAsync(host.data(), dptr(), host.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream()));
+    ctx->sync();
+  }
+}
+
+const double* BTensor::host_values(std::vector<double>& tmp) const {
+  if (!host.empty() && host.size() == static_cast<size_t>(nelem) * w()) return host.data();
+  tmp.assign(static_cast<size_t>(nelem) * w(), 0.0);
+  if (nelem > 0) {
+    SADMRG_HIP(hipMemcpyAsync(tmp.data(), dptr(), tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream()));
+    ctx->sync();
+  }
+  return tmp.data();
 }
 
 Struct parse_header(SymKind sym, const int* h, int len) {
@@ -149,6 +171,7 @@ void copy_tensor(BTensor& dst, const BTensor& src) {
   dst.layout(src.ctx, src.cplx, src.st, mask, kCgcIdent);
   dst.tag = src.tag;
   dst.fuse_id = src.fuse_id;
+  dst.host = src.host;
   if (src.nelem > 0)
     SADMRG_HIP(hipMemcpyAsync(dst.dptr(), src.dptr(), src.nelem * sizeof(double) * src.w(),
                               hipMemcpyDeviceToDevice, src.ctx->stream()));
@@ -248,7 +271,6 @@ void op_hvm(BTensor& out, const BTensor& vec, const BTensor& op1, const BTensor&
   }
   run_batch(g1, ctx, vec.cplx, ar);
   run_batch(g2, ctx, vec.cplx, ar);
-  ctx->sync();
 }
 
 // ---------------------------------------------------------------------------
@@ -365,7 +387,6 @@ void op_contract(BTensor& out, const BTensor& t1, int i1, const BTensor& t2, int
     }
   }
   run_batch(g, ctx, t1.cplx, ar);
-  ctx->sync();
   out = std::move(res);
 }
 
@@ -443,12 +464,10 @@ struct ScatterBuilder {
       }
     }
     DevBuf d(ctx, tasks.size() * sizeof(ScatterTask));
-    SADMRG_HIP(hipMemcpyAsync(d.get(), tasks.data(), tasks.size() * sizeof(ScatterTask), hipMemcpyHostToDevice,
-                              ctx->stream()));
+    ctx->upload(d.get(), tasks.data(), tasks.size() * sizeof(ScatterTask));
     for (const auto& pr : pairs)
       SADMRG_HIP(launch_block_scatter(d.as<ScatterTask>(), static_cast<int>(tasks.size()), pr.first, pr.second, w,
                                       ctx->stream()));
-    ctx->sync();
   }
 };
 
@@ -513,7 +532,6 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
       }
     }
     run_batch(g2, ctx, cx, ar);
-    ctx->sync();
     return;
   }
 
@@ -643,7 +661,6 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
     }
     run_batch(g1, ctx, cx, ar);
     run_batch(g2, ctx, cx, ar);
-    ctx->sync();
     return;
   }
 
@@ -653,12 +670,8 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
     os.bonds = {uu.st.bonds[2], op.st.bonds[1], vv.st.bonds[2].inverted()};
     per.resize(os.nten());
     // operator values are needed on the host as scalar weights
-    std::vector<double> hop(op.nelem * op.w());
-    if (op.nelem) {
-      SADMRG_HIP(hipMemcpyAsync(hop.data(), op.dptr(), hop.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                ctx->stream()));
-      ctx->sync();
-    }
+    std::vector<double> hop_tmp;
+    const double* hop = op.host_values(hop_tmp);
     const int nt1 = op.nten(), nt2 = uu.nten();
     const int P2 = cgc ? cgc->p2() : 3;
     for (int i = 0; i < nt1 * nt2; ++i) {
@@ -705,7 +718,6 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
         g2.add_seg(p, view(0, uu, c.s1, c.K, true), view(1, vv, c.l, c.K, false, true), c.K, c.scale);
     }
     run_batch(g2, ctx, cx, ar);
-    ctx->sync();
     return;
   }
 
@@ -790,7 +802,6 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
         if (!g1.empty()) g1.run(ctx, prec_of(cx), ae.at);
         if (!g2.empty()) g2.run(ctx, prec_of(cx), ae.at);
       }
-    ctx->sync();
     return;
   }
 
@@ -803,12 +814,8 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
             "operator pairup wrong bb");
     os.bonds = {uu.st.bonds[2], vv.st.bonds[2].inverted()};
     per.resize(os.nten());
-    std::vector<double> hsg(sg.nelem * sg.w());
-    if (sg.nelem) {
-      SADMRG_HIP(hipMemcpyAsync(hsg.data(), sg.dptr(), hsg.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                ctx->stream()));
-      ctx->sync();
-    }
+    std::vector<double> hsg_tmp;
+    const double* hsg = sg.host_values(hsg_tmp);
     const int P2 = cgc ? cgc->p2() : 3;
     const int nt1 = op.nten(), nt2 = uu.nten();
     for (int i = 0; i < nt1 * nt2; ++i) {
@@ -881,7 +888,6 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
     }
     run_batch(g1, ctx, cx, ar);
     run_batch(g2, ctx, cx, ar);
-    ctx->sync();
     return;
   }
   fail(kErrInvalid, "environment update: unknown kind");
@@ -1121,8 +1127,7 @@ void op_axpy_channels(BTensor& y, double s, const dcmplex (&w)[3], const BTensor
     else
       SADMRG_HIP(launch_zscal(t.block_size(b), c.real(), c.imag(), t.block_ptr(b), t.block_ptr(b), t.ctx->stream()));
   }
-  op_axpy(y, 1.0, 0.0, t);
-  t.ctx->sync();  // t leaves scope
+  op_axpy(y, 1.0, 0.0, t);  // t's memory returns to the stream-ordered cache
 }
 
 void op_fill(BTensor& t, double v) {
@@ -1166,9 +1171,8 @@ void op_axpy(BTensor& y, double re, double im, const BTensor& x) {
     else {
       double* c = ctx->dscal();
       const double h[2] = {re, im};
-      SADMRG_HIP(hipMemcpyAsync(c, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream()));
+      ctx->upload(c, h, sizeof(h));
       SADMRG_HIP(launch_gemv_n_update(x.dptr(), 0, 1, c, y.dptr(), y.nelem, 1, ctx->stream()));
-      ctx->sync();
     }
     return;
   }
@@ -1180,9 +1184,8 @@ void op_axpy(BTensor& y, double re, double im, const BTensor& x) {
     } else {
       double* c = ctx->dscal();
       const double h[2] = {re, im};
-      SADMRG_HIP(hipMemcpyAsync(c, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream()));
+      ctx->upload(c, h, sizeof(h));
       SADMRG_HIP(launch_gemv_n_update(x.block_ptr(b), 0, 1, c, y.block_ptr(b), n, 1, ctx->stream()));
-      ctx->sync();
     }
   }
 }

@@ -35,6 +35,11 @@ struct BTensor {
   DevBuf buf;                  // nelem elements of (cplx ? 16 : 8) bytes
   int64_t nelem = 0;
   bool fuse_id = false;        // built by op_fuse: 0/1 identity-embedding blocks
+  // Host copy of the data (nelem * w doubles) kept for small constant tensors
+  // that are read as scalar weights (site operators); empty when not kept.
+  // layout() and clean() drop it; whoever writes the data afterwards must
+  // drop or refresh it (keep_host()).
+  std::vector<double> host;
 
   BTensor() = default;
   BTensor(BTensor&&) = default;
@@ -55,6 +60,10 @@ struct BTensor {
   // Lays out the blocks flagged in mask (zero-filled device memory).
   void layout(Context* c, bool complex_, const Struct& s, const std::vector<uint8_t>& mask, uint8_t tag0);
   void clean();
+  // Reads the data back into `host` (one synchronisation).
+  void keep_host();
+  // The data on the host: `host` when kept, otherwise read back into tmp.
+  const double* host_values(std::vector<double>& tmp) const;
 };
 
 // ---- exchange with host (the C ABI tensor format) ------------------------

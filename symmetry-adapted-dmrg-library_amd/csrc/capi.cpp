@@ -926,6 +926,14 @@ int sadmrg_dmrg_set_check_interval(sadmrg_dmrg* d, int every) {
   });
 }
 
+int sadmrg_dmrg_set_fp64_energy_every(sadmrg_dmrg* d, int every) {
+  return guard([&] {
+    need(d, "dmrg");
+    require(every >= 0, "set_fp64_energy_every: every must be >= 0");
+    d->d->set_fp64_energy_every(every);
+  });
+}
+
 extern "C++" {
 static std::string dir_arg(const char* dir) { return dir ? std::string(dir) : std::string(); }
 }
@@ -1028,7 +1036,7 @@ int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n) {
     const int m = std::min<int>(cap, static_cast<int>(r.size()));
     for (int i = 0; i < m; ++i) {
       const StepRecord& s = r[i];
-      double* p = rec + 12 * i;
+      double* p = rec + 14 * i;
       p[0] = s.il;
       p[1] = s.ir;
       p[2] = s.energy;
@@ -1041,6 +1049,8 @@ int sadmrg_dmrg_records(sadmrg_dmrg* d, double* rec, int cap, int* n) {
       p[9] = s.t_svd;
       p[10] = s.heff_flops;
       p[11] = static_cast<double>(s.ndata);
+      p[12] = s.ritz32;
+      p[13] = s.energy64;
     }
     r.erase(r.begin(), r.begin() + m);
     if (n) *n = m;

@@ -1,5 +1,6 @@
 // core.cpp -- device context and caching allocator.
 #include "core.hpp"
+#include <cstring>
 #include "eigen.hpp"
 #include <rocsolver/rocsolver.h>
 #include "kernels.hpp"
@@ -41,6 +42,9 @@ Context::Context(int device) : device_(device) {
 
 Context::~Context() {
   if (stream_) hipStreamSynchronize(stream_);
+  for (auto& e : staged_) hipEventDestroy(e.ev);
+  for (auto e : spare_events_) hipEventDestroy(e);
+  if (staging_) hipHostFree(staging_);
   if (solver_) rocblas_destroy_handle(static_cast<rocblas_handle>(solver_));
   partial_.release();
   dscal_.release();
@@ -50,6 +54,46 @@ Context::~Context() {
 }
 
 void Context::sync() { SADMRG_HIP(hipStreamSynchronize(stream_)); }
+
+void Context::upload(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  if (bytes > kStagingBytes / 4) {
+    SADMRG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+    sync();
+    return;
+  }
+  if (!staging_) SADMRG_HIP(hipHostMalloc(reinterpret_cast<void**>(&staging_), kStagingBytes, hipHostMallocDefault));
+  const size_t need = (bytes + 255) / 256 * 256;
+  if (staging_head_ + need > kStagingBytes) staging_head_ = 0;
+  const size_t b = staging_head_, e = b + need;
+  // Wait for the newest in-flight copy that still reads [b, e): copies on the
+  // one stream complete in order, so every older one has completed with it.
+  int last = -1;
+  for (size_t i = 0; i < staged_.size(); ++i)
+    if (staged_[i].begin < e && b < staged_[i].end) last = static_cast<int>(i);
+  if (last >= 0) SADMRG_HIP(hipEventSynchronize(staged_[last].ev));
+  for (int i = 0; i <= last; ++i) {
+    spare_events_.push_back(staged_.front().ev);
+    staged_.pop_front();
+  }
+  // drop completed entries from the front so the list stays short
+  while (staged_.size() > 64 && hipEventQuery(staged_.front().ev) == hipSuccess) {
+    spare_events_.push_back(staged_.front().ev);
+    staged_.pop_front();
+  }
+  std::memcpy(staging_ + b, src, bytes);
+  SADMRG_HIP(hipMemcpyAsync(dst, staging_ + b, bytes, hipMemcpyHostToDevice, stream_));
+  hipEvent_t ev;
+  if (spare_events_.empty()) {
+    SADMRG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  } else {
+    ev = spare_events_.back();
+    spare_events_.pop_back();
+  }
+  SADMRG_HIP(hipEventRecord(ev, stream_));
+  staged_.push_back(Staged{b, e, ev});
+  staging_head_ = e;
+}
 
 static size_t round_cap(size_t bytes) {
   const size_t a = 256;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime_api.h>
 #include <complex>
 #include <cstdint>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -98,6 +99,15 @@ class Context {
   hipStream_t stream() const { return stream_; }
   void sync();
 
+  // Host-to-device copy on the stream through pinned staging memory: the
+  // source is copied into a staging ring before the call returns, so it may
+  // be reused or freed at once, and the host does not wait for the device
+  // (a copy from pageable memory would).  The ring (kStagingBytes) is
+  // recycled in stream order: a region is reused once the copy that read it
+  // has completed.  Larger copies go straight from the source and wait.
+  void upload(void* dst, const void* src, size_t bytes);
+  static constexpr size_t kStagingBytes = size_t(64) << 20;
+
   void* raw_alloc(size_t bytes, size_t* cap);
   void raw_free(void* p, size_t cap);
   void trim();
@@ -120,6 +130,16 @@ class Context {
   size_t in_use_ = 0, cached_ = 0;
   DevBuf partial_, dscal_;
   void* solver_ = nullptr;
+  // staging ring of upload(): in-flight regions [begin, end) with the event
+  // recorded after the copy that reads them, oldest first
+  struct Staged {
+    size_t begin, end;
+    hipEvent_t ev;
+  };
+  char* staging_ = nullptr;
+  size_t staging_head_ = 0;
+  std::deque<Staged> staged_;
+  std::vector<hipEvent_t> spare_events_;
 };
 
 void set_last_error(const std::string& msg);

@@ -127,6 +127,7 @@ void Dmrg::copy_state_from(const Dmrg& o) {
   totspin_ = o.totspin_;
   max_dcut_ = o.max_dcut_;
   lopt_ = o.lopt_;
+  fp64_energy_every_ = o.fp64_energy_every_;
   gs_enr_ = o.gs_enr_;
   wout_ = o.wout_;
   for (int i = 0; i < ns_; ++i) {
@@ -166,7 +167,9 @@ void Dmrg::initialize_local_operators() {
     op_shift(t2, 1, 0);
     sigma_[1] = std::move(t2);
   }
-  ctx_->sync();
+  // the environment updates read the site operators' values as scalar weights
+  sigma_[0].keep_host();
+  sigma_[1].keep_host();
 }
 
 void Dmrg::boundary_tensors() {
@@ -478,9 +481,13 @@ double Dmrg::lanczos_solve(int il, int ir, BTensor& vec) {
   auto plan = make_heff(il, ir, vec, nullptr);
   LanczosOpts o = lopt_;
   o.mlanc = mlanc;
+  o.fp64_energy = step_fp64_energy_;
+  step_fp64_energy_ = true;
   const LanczosOut r = ::sadmrg::lanczos_solve(*plan, vec.dptr(), o);
   ctx_->sync();
   gs_enr_ = r.energy;
+  cur_.ritz32 = r.ritz32;
+  cur_.energy64 = r.energy64 ? 1 : 0;
   cur_.il = il;
   cur_.ir = ir;
   cur_.energy = r.energy;
@@ -707,12 +714,23 @@ void Dmrg::prepare_sweep() {
 // dmrg_inst.cpp:874-943 (checkpoint files only when a directory is set)
 void Dmrg::sweep() {
   BTensor vec;
+  // FP64-energy policy of the mixed path (set_fp64_energy_every): step s of
+  // nsteps computes the FP64 energy when every == 1, when s is a multiple of
+  // every (every > 1), and at the last step
+  const int nsteps = (ns_ / 2 - nfree_ - 1) + (ns_ - 2 * nfree_ - 2) + (ns_ - nfree_ - 1 - ns_ / 2);
+  int step = 0;
+  auto policy = [&]() {
+    const int k = fp64_energy_every_;
+    ++step;
+    step_fp64_energy_ = k == 1 || step >= nsteps || (k > 1 && step % k == 0);
+  };
   for (int i = ns_ / 2; i >= nfree_ + 2; --i) {
     wavefunc_transformation(i - 1, 1);
     prepare_site_operator_from_right(i, 1);
     prepare_input_vector(i - 2, i - 1, vec);
     prepare_site_operator_from_left(i - 2);
     prepare_site_operator_from_right(i - 1);
+    policy();
     lanczos_solve(i - 2, i - 1, vec);
     if (i > nfree_ + 2) svd(vec, i - 2, i - 1, 1.0, 0.0);
     else svd(vec, i - 2, i - 1, 0.0, 1.0);
@@ -723,6 +741,7 @@ void Dmrg::sweep() {
     prepare_input_vector(i + 1, i + 2, vec);
     prepare_site_operator_from_left(i + 1);
     prepare_site_operator_from_right(i + 2);
+    policy();
     lanczos_solve(i + 1, i + 2, vec);
     if (i < ns_ - nfree_ - 3) svd(vec, i + 1, i + 2, 0.0, 1.0);
     else svd(vec, i + 1, i + 2, 1.0, 0.0);
@@ -733,6 +752,7 @@ void Dmrg::sweep() {
     prepare_input_vector(i - 2, i - 1, vec);
     prepare_site_operator_from_left(i - 2);
     prepare_site_operator_from_right(i - 1);
+    policy();
     lanczos_solve(i - 2, i - 1, vec);
     svd(vec, i - 2, i - 1, 1.0, 0.0);
   }

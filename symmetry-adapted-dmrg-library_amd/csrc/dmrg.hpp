@@ -25,6 +25,8 @@ struct StepRecord {
   double t_env, t_lanczos, t_svd;  // seconds
   double heff_flops;  // algorithmic flops per H_eff application at this step
   int64_t ndata;      // reduced elements of the two-site wave function
+  double ritz32;      // mixed precision: the last Ritz value of the FP32 Krylov recurrence
+  int energy64;       // 1: energy is an FP64 Rayleigh quotient; 0: the FP32 Ritz value
 };
 
 class Dmrg {
@@ -49,6 +51,14 @@ class Dmrg {
   // 1: one line per sweep step on stderr (bond, energy, kept states, stage
   // times), as the reference prints each step of DMRG::sweep to cout.
   void set_verbose(int v) { verbose_ = v; }
+  // Mixed precision with refinement: which sweep steps compute the FP64
+  // energy (the FP64 Rayleigh quotient and the residual-correction restarts).
+  // 1 (default): every step.  k > 1: every k-th step of a sweep and its last
+  // step.  0: the last step of a sweep only.  The other steps stop on the
+  // FP32 recurrence residual (LanczosOpts::fp64_energy = false) and record
+  // the FP32 Ritz value.  iDMRG steps always compute the FP64 energy.
+  void set_fp64_energy_every(int k) { fp64_energy_every_ = k; }
+  int fp64_energy_every() const { return fp64_energy_every_; }
   const LanczosOpts& lanczos_opts() const { return lopt_; }
 
   // Deep copy of another driver's lattice maps, MPS, environments, settings
@@ -149,6 +159,8 @@ class Dmrg {
   std::unique_ptr<CgcTables> cgc_;
   LanczosOpts lopt_;
   int verbose_ = 0;
+  int fp64_energy_every_ = 1;
+  bool step_fp64_energy_ = true;  // policy for the next lanczos_solve
   double gs_enr_ = 0;
   BTensor last_vec_;  // Ritz vector of the last Lanczos solve
   int last_il_ = -1, last_ir_ = -1;

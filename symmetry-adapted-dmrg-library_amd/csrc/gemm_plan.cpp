@@ -221,12 +221,8 @@ void GemmBatch::finalize(Context* ctx, bool complex) {
   if (hp_.empty()) return;
   d_probs_ = DevBuf(ctx, hp_.size() * sizeof(GemmProb));
   d_segs_ = DevBuf(ctx, std::max<size_t>(hs_.size(), 1) * sizeof(GemmSeg));
-  SADMRG_HIP(hipMemcpyAsync(d_probs_.get(), hp_.data(), hp_.size() * sizeof(GemmProb), hipMemcpyHostToDevice,
-                            ctx->stream()));
-  if (!hs_.empty())
-    SADMRG_HIP(hipMemcpyAsync(d_segs_.get(), hs_.data(), hs_.size() * sizeof(GemmSeg), hipMemcpyHostToDevice,
-                              ctx->stream()));
-  ctx->sync();
+  ctx->upload(d_probs_.get(), hp_.data(), hp_.size() * sizeof(GemmProb));
+  if (!hs_.empty()) ctx->upload(d_segs_.get(), hs_.data(), hs_.size() * sizeof(GemmSeg));
 }
 
 int GemmBatch::default_cfg(GemmPrec prec) const {
@@ -311,19 +307,15 @@ GemmBatch::TileSet& GemmBatch::tileset(Context* ctx, int cfg) {
   ts.nparts = nparts;
   if (ts.ntiles == 0) return ts;
   ts.tiles = DevBuf(ctx, ht.size() * sizeof(GemmTile));
-  SADMRG_HIP(hipMemcpyAsync(ts.tiles.get(), ht.data(), ht.size() * sizeof(GemmTile), hipMemcpyHostToDevice,
-                            ctx->stream()));
+  ctx->upload(ts.tiles.get(), ht.data(), ht.size() * sizeof(GemmTile));
   if (ts.nreduce > 0) {
     ts.reduce = DevBuf(ctx, hr.size() * sizeof(GemmReduce));
-    SADMRG_HIP(hipMemcpyAsync(ts.reduce.get(), hr.data(), hr.size() * sizeof(GemmReduce), hipMemcpyHostToDevice,
-                              ctx->stream()));
+    ctx->upload(ts.reduce.get(), hr.data(), hr.size() * sizeof(GemmReduce));
     // room for double partials (the widest output type)
     ts.partial = DevBuf(ctx, int64_t(nparts) * tm * tn * sizeof(double));
     ts.counters = DevBuf(ctx, hr.size() * sizeof(int));
     SADMRG_HIP(hipMemsetAsync(ts.counters.get(), 0, hr.size() * sizeof(int), ctx->stream()));
   }
-  // host vectors go out of scope: make the uploads complete first
-  ctx->sync();
   return ts;
 }
 
@@ -378,9 +370,7 @@ void GemmBatch::run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& a
     pt.begin[hp_.size()] = static_cast<int>(ht.size());
     if (!ht.empty()) {
       pt.tiles = DevBuf(ctx, ht.size() * sizeof(GemmTile));
-      SADMRG_HIP(hipMemcpyAsync(pt.tiles.get(), ht.data(), ht.size() * sizeof(GemmTile), hipMemcpyHostToDevice,
-                                ctx->stream()));
-      ctx->sync();
+      ctx->upload(pt.tiles.get(), ht.data(), ht.size() * sizeof(GemmTile));
     }
   }
   for (size_t pi = 0; pi < hp_.size(); ++pi) {

@@ -108,13 +108,12 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
     if (!ti.empty()) {
       const size_t tb_bytes = ti.size() * sizeof(ScatterTask);
       DevBuf tb(ctx, tb_bytes + bases.size() * sizeof(double*));
-      SADMRG_HIP(hipMemcpyAsync(tb.get(), ti.data(), tb_bytes, hipMemcpyHostToDevice, ctx->stream()));
+      ctx->upload(tb.get(), ti.data(), tb_bytes);
       const double* const* dbases = reinterpret_cast<const double* const*>(static_cast<char*>(tb.get()) + tb_bytes);
-      SADMRG_HIP(hipMemcpyAsync(const_cast<const double**>(dbases), bases.data(), bases.size() * sizeof(double*),
-                                hipMemcpyHostToDevice, ctx->stream()));
+      ctx->upload(const_cast<const double**>(dbases), bases.data(), bases.size() * sizeof(double*));
       SADMRG_HIP(launch_repack(tb.as<ScatterTask>(), static_cast<int>(ti.size()), nullptr, dbases, pack64_.get(), w, 0,
                                ctx->stream()));
-      ctx->sync();  // tb, ti and bases leave scope
+      // tb returns to the stream-ordered cache; the staged sources are copies
     }
   }
   auto pk = [&](const BTensor* t, int b) -> Operand {
@@ -269,8 +268,7 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
   in64_ = DevBuf(ctx, std::max<int64_t>(in_elems_, 1) * 8 * w);
   if (!in_tasks_.empty()) {
     in_tasks_dev_ = DevBuf(ctx, in_tasks_.size() * sizeof(ScatterTask));
-    SADMRG_HIP(hipMemcpyAsync(in_tasks_dev_.get(), in_tasks_.data(), in_tasks_.size() * sizeof(ScatterTask),
-                              hipMemcpyHostToDevice, ctx->stream()));
+    ctx->upload(in_tasks_dev_.get(), in_tasks_.data(), in_tasks_.size() * sizeof(ScatterTask));
   }
   stats_.n = n_;
   stats_.flops = g1_.flops(cplx_) + g2_.flops(cplx_);
@@ -285,7 +283,6 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
   stats_.tiles1 = g1_.ntiles(ctx, stats_.tile_cfg1);
   stats_.tiles2 = g2_.ntiles(ctx, stats_.tile_cfg2);
   stats_.split_tiles = g1_.nsplit_tiles(ctx, stats_.tile_cfg1) + g2_.nsplit_tiles(ctx, stats_.tile_cfg2);
-  ctx->sync();
 }
 
 std::unique_ptr<HeffPlan> HeffPlan::merge(const std::vector<const HeffPlan*>& parts) {
@@ -332,8 +329,7 @@ std::unique_ptr<HeffPlan> HeffPlan::merge(const std::vector<const HeffPlan*>& pa
   m->in64_ = DevBuf(ctx, std::max<int64_t>(m->in_elems_, 1) * 8 * w);
   if (!m->in_tasks_.empty()) {
     m->in_tasks_dev_ = DevBuf(ctx, m->in_tasks_.size() * sizeof(ScatterTask));
-    SADMRG_HIP(hipMemcpyAsync(m->in_tasks_dev_.get(), m->in_tasks_.data(), m->in_tasks_.size() * sizeof(ScatterTask),
-                              hipMemcpyHostToDevice, ctx->stream()));
+    ctx->upload(m->in_tasks_dev_.get(), m->in_tasks_.data(), m->in_tasks_.size() * sizeof(ScatterTask));
   }
   for (GemmBatch* g : {&m->g1_, &m->g2_}) {
     g->request_vec();
@@ -359,7 +355,6 @@ std::unique_ptr<HeffPlan> HeffPlan::merge(const std::vector<const HeffPlan*>& pa
   st.tiles2 = m->g2_.ntiles(ctx, st.tile_cfg2);
   st.split_tiles = m->g1_.nsplit_tiles(ctx, st.tile_cfg1) + m->g2_.nsplit_tiles(ctx, st.tile_cfg2);
   for (const HeffPlan* p : parts) st.nsec_in += p->stats_.nsec_in;
-  ctx->sync();
   return m;
 }
 

@@ -218,6 +218,16 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
   // With refinement the check matvec is FP64: the stopping test then sees the
   // exact operator, and hv = H v is reused as the first residual below.
   const Prec check_prec = (bulk != Prec::FP64 && opt.refine) ? Prec::FP64 : bulk;
+  // FP32 Ritz pairs judged by their recurrence residual, no FP64 matvec
+  const bool est_only = bulk != Prec::FP64 && opt.refine && !opt.fp64_energy;
+  out.energy64 = !est_only;
+  // |r| = beta_m |y_m| of the lowest Ritz pair of the m-step recurrence
+  // (needs diatridiag(m) and nnl[m])
+  auto est_converged = [&](int m) {
+    const double th = eig[0], rr = nnl[m] * std::fabs(z[m - 1]);
+    out.ritz32 = th;
+    return rr <= opt.refine_tol * std::fabs(th);
+  };
   auto check = [&]() {
     const double nor = D.nrm(d_vec);
     D.scal(1.0 / nor, d_vec);
@@ -335,13 +345,24 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
         out.steps = k;
         diatridiag(k);
         ritz(k);
-        check();
+        out.ritz32 = eig[0];
+        if (!est_only) check();
         done = true;
         break;
       }
       if (m % opt.check_every != 0) continue;
       diatridiag(m);
+      if (est_only) {
+        out.trace.push_back(eig[0]);
+        if (est_converged(m)) {
+          ritz(m);
+          done = true;
+          break;
+        }
+        continue;
+      }
       ritz(m);
+      out.ritz32 = eig[0];
       const double ov = check();
       out.trace.push_back(eig[0]);
       if (std::fabs(ov - 1.0) < 1e-8) {
@@ -363,12 +384,21 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
     diatridiag(m);
     if (nnl[m] < tol) {
       ritz(m);
-      check();
+      out.ritz32 = eig[0];
+      if (!est_only) check();
       done = true;
       break;
     }
-    if (m % opt.check_every == 0) {
+    if (m % opt.check_every == 0 && est_only) {
+      out.trace.push_back(eig[0]);
+      if (est_converged(m)) {
+        ritz(m);
+        done = true;
+        break;
+      }
+    } else if (m % opt.check_every == 0) {
       ritz(m);
+      out.ritz32 = eig[0];
       const double ov = check();
       out.trace.push_back(eig[0]);
       if (std::fabs(ov - 1.0) < 1e-8) {
@@ -382,12 +412,13 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
     aal[ml - 1] = D.dot(col(ml), col(ml - 1)).real();
     diatridiag(ml);
     ritz(ml);
-    check();
+    out.ritz32 = eig[0];
+    if (!est_only) check();
     out.steps = ml;
   }
   out.energy = eig[0];
 
-  if (bulk != Prec::FP64 && opt.refine) {
+  if (bulk != Prec::FP64 && opt.refine && !est_only) {
     // FP64 residual-correction restarts on the Ritz vector in d_vec.
     const int k = std::max(1, std::min(opt.refine_krylov, ml - 1));
     double* x = d_vec;

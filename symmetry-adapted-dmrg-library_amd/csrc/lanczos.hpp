@@ -35,6 +35,15 @@ struct LanczosOpts {
   // host-driven step.
   bool device_steps = true;
   bool graph = true;
+  // Mixed precision with refinement only.  false: the solve makes no FP64
+  // matvec.  After every check_every-th step the Ritz pair (theta, y) of the
+  // FP32 Krylov recurrence is tested by its recurrence residual
+  // beta_m |y_m| (no matvec); the solve stops when that is at most
+  // refine_tol |theta| (the target the FP64 refinement would reach) and
+  // returns theta, the Ritz value of the FP32-operand operator.  The vector
+  // is that of an FP64 solve to within the FP32 operator error (about
+  // 1e-7 |H| / gap); the returned energy carries the FP32 operator error.
+  bool fp64_energy = true;
 };
 
 // Process-wide switches (default on): a solve uses device steps / graphs
@@ -53,6 +62,8 @@ struct LanczosOut {
   int graph_launches = 0;   // Krylov steps replayed from a captured graph
   int restarts = 0;
   double residual64 = -1;     // |H x - E x| of the returned vector (FP64), if measured
+  double ritz32 = 0;          // mixed precision: the last Ritz value of the FP32 Krylov recurrence
+  bool energy64 = true;       // energy is an FP64 Rayleigh quotient (false: the FP32 Ritz value)
   std::vector<double> trace;  // Ritz values at each check
 };
 

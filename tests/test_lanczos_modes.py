@@ -110,3 +110,60 @@ def test_full_run_step_energies_agree_across_modes(ctx, sad, restore_mode, name)
         assert np.array_equal(e, e0), (mode, float(np.max(np.abs(e - e0))))
         assert np.array_equal(m, m0), (mode, m, m0)
     assert runs["graph"][2] > 0
+
+
+@pytest.mark.parametrize("every", [0, 3])
+def test_fp64_energy_policy(ctx, sad, restore_mode, every):
+    """Mixed sweeps computing the FP64 energy at every `every`-th step and the
+    last (every = 0: the last only).  The other steps make no FP64 matvec
+    and record the FP32 Ritz value; the last step's FP64 energy matches the
+    FP64 sweep's to 1e-9, the FP32 Ritz values to the FP32 operator error.
+    Host-driven and graph-replayed steps agree bit for bit."""
+    tens, extra = _capture("dmrg_su2_obc12")
+    sym, cplx, xx, yy, D, ns = (int(x) for x in extra["params"])
+
+    def run(prec, mode, e64every):
+        sad.lanczos_mode(*MODES[mode], reset=True)
+        sad.lib().sadmrg_rng_seed(42)
+        d = sad.DMRG(ctx, sym, cplx, xx, yy, D)
+        d.set_couplings(extra["hmap"], extra["coup"])
+        d.set_max_dcut(D)
+        if prec == "mixed":
+            d.set_precision(sad.FP32, refine=True, refine_tol=1e-6)
+            d.set_check_interval(2)
+            d.set_fp64_energy_every(e64every)
+        d.idmrg()
+        d.sweep()
+        d.sweep()
+        return d.records()
+
+    ref = run("fp64", "graph", 1)
+    got = run("mixed", "graph", every)
+    host = run("mixed", "host", every)
+    assert len(got) == len(ref) == len(host)
+    e_ref = np.asarray([r["energy"] for r in ref])
+    e = np.asarray([r["energy"] for r in got])
+    assert np.array_equal(e, np.asarray([r["energy"] for r in host]))
+    # records: the iDMRG steps, then the two sweeps' steps
+    nsweep = 2 * ns - 12
+    n_idmrg = len(got) - 2 * nsweep
+    flags = np.asarray([int(r["energy64"]) for r in got])
+    assert np.all(flags[:n_idmrg] == 1)
+    for s in range(2):
+        f = flags[n_idmrg + s * nsweep:n_idmrg + (s + 1) * nsweep]
+        want = np.zeros(nsweep, int)
+        want[-1] = 1
+        if every > 1:
+            want[every - 1::every] = 1
+        assert np.array_equal(f, want), (s, f.tolist())
+    m64 = np.asarray([int(r["matvec64"]) for r in got])
+    assert np.all(m64[flags == 0] == 0) and np.all(m64[flags == 1] >= 1)
+    rel = np.abs(e - e_ref) / np.abs(e_ref)
+    # the converged second sweep: FP64-energy steps to 1e-9, the others to
+    # the FP32 operator error
+    last = slice(n_idmrg + nsweep, None)
+    f2 = flags[last] == 1
+    assert np.all(rel[last][f2] <= 1e-9), rel[last][f2]
+    assert np.all(rel[last][~f2] <= 1e-5), rel[last][~f2].max()
+    r32 = np.asarray([r["ritz32"] for r in got])
+    assert np.all(np.abs(r32[flags == 0] - e[flags == 0]) == 0)
