@@ -355,10 +355,20 @@ def c1_run(sad, args):
         secs, es, trs, t_lan, replays = run(dev, graph)
         modes[name] = {"seconds": round(secs, 3), "sweep_lanczos_seconds": round(t_lan, 3),
                        "graph_replays": replays, "last_energy": es[-1]}
+    sad.persistent(1)
+    p_s, p_e, _, p_lan, _ = run(True, True)
+    sad.persistent(0)
+    o_s, o_e, _, o_lan, _ = run(True, True)
+    sad.persistent(2)
     gpu_s, gpu_e, gpu_tr, _, _ = run(True, True)
     line = {"workload": "C1: U(1) open chain L=32, m=64 kept states, iDMRG + 4 sweeps, FP64",
             "gpu_seconds": round(gpu_s, 3), "gpu_sweep_energies": gpu_e, "gpu_last_step_trunc": gpu_tr,
-            "lanczos_step_modes": modes}
+            "lanczos_step_modes": modes,
+            "launch_kind": {"persistent": {"seconds": round(p_s, 3), "sweep_lanczos_seconds": round(p_lan, 3),
+                                           "same_energies": p_e == gpu_e},
+                            "one_tile_per_workgroup": {"seconds": round(o_s, 3),
+                                                       "sweep_lanczos_seconds": round(o_lan, 3),
+                                                       "same_energies": o_e == gpu_e}}}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
@@ -819,10 +829,15 @@ def main():
           flush=True)
 
     res = {}
-    legs = [("fp64", sad.FP64, True), ("fp32", sad.FP32, True), ("bf16", sad.BF16, True),
-            ("fp64_elem", sad.FP64, False), ("fp32_elem", sad.FP32, False)]
-    for name, prec, vec in legs:
+    # persistent launch mode: 2 the autotuned default, 0 one workgroup per
+    # tile everywhere, 1 persistent everywhere (same tuned tile plans)
+    legs = [("fp64", sad.FP64, True, 2), ("fp32", sad.FP32, True, 2), ("bf16", sad.BF16, True, 2),
+            ("fp64_elem", sad.FP64, False, 2), ("fp32_elem", sad.FP32, False, 2),
+            ("fp64_one_tile", sad.FP64, True, 0), ("fp32_one_tile", sad.FP32, True, 0),
+            ("fp64_persistent", sad.FP64, True, 1), ("fp32_persistent", sad.FP32, True, 1)]
+    for name, prec, vec, pers in legs:
         sad.vec_staging(vec)
+        sad.persistent(pers)
         for _ in range(args.warmup):
             plan.apply_device(d_in, d_out, prec)
         ctx.sync()
@@ -841,6 +856,7 @@ def main():
         print(f"[bench] {name}: {wall_ms:.3f} ms/apply {res[name]['tflops']:.2f} TFLOP/s "
               f"(stage1 {s1:.3f} ms, stage2 {s2:.3f} ms)", file=sys.stderr, flush=True)
     sad.vec_staging(True)
+    sad.persistent(2)
 
     # ablation: the uniform split policy (every stage-2 problem cut into the
     # same number of reduction parts), on a plan built and tuned under it
@@ -983,8 +999,9 @@ def main():
         "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp64"].items()},
         "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp32"].items()},
         "tile_configs": "launch plan = tile configuration (0: 64x64, 1: 128x128, 2: 128x64 output tiles) + 3 x "
-                        "split variant (0: about 3, 1: about 8 reduction parts per resident workgroup), "
-                        "autotuned per stage",
+                        "split variant (0: about 3, 1: about 8 reduction parts per resident workgroup) + 6 x "
+                        "launch kind (0: one workgroup per tile, 1: persistent grid taking tiles from a device "
+                        "ticket counter), autotuned per stage",
         "mixed_vs_fp64_speedup": round(res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"], 3),
         "element_staging": {"fp32_ms": round(res["fp32_elem"]["ms_per_step"], 4),
                             "fp64_ms": round(res["fp64_elem"]["ms_per_step"], 4),
@@ -992,6 +1009,14 @@ def main():
                                               round(res["fp32_elem"]["stage2_ms"], 4)],
                             "note": "the same kernels staging one element per lane instead of 16-byte vectors"},
         "uniform_split": uniform,
+        "launch_kind": {k: {"one_tile_per_workgroup_ms": round(res[f"{k}_one_tile"]["ms_per_step"], 4),
+                            "one_tile_stage_ms": [round(res[f"{k}_one_tile"]["stage1_ms"], 4),
+                                                  round(res[f"{k}_one_tile"]["stage2_ms"], 4)],
+                            "persistent_ms": round(res[f"{k}_persistent"]["ms_per_step"], 4),
+                            "persistent_stage_ms": [round(res[f"{k}_persistent"]["stage1_ms"], 4),
+                                                    round(res[f"{k}_persistent"]["stage2_ms"], 4)],
+                            "autotuned_ms": round(res[k]["ms_per_step"], 4)}
+                        for k in ("fp64", "fp32")},
         "fp32_rel_err_vs_fp64": rel32,
         "bf16_ablation": {"ms_per_step": round(res["bf16"]["ms_per_step"], 4),
                           "tflops": round(res["bf16"]["tflops"], 3),

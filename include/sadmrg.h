@@ -167,6 +167,19 @@ int sadmrg_autotune(int enable, int clear, int* entries);
    leaves the setting; *previous receives the old value.  Off stages single
    elements (the comparison baseline).  No reference counterpart. */
 int sadmrg_gemm_vec_staging(int enable, int* previous);
+/* Persistent launches of the real grouped GEMMs (process-wide): a grid of
+   about the resident workgroup count (one or two per CU) whose workgroups
+   take output tiles from a device ticket counter until the table is
+   exhausted, instead of one workgroup per tile placed by the hardware
+   dispatcher; the last workgroup to leave resets the counter, so launches
+   replay from HIP graphs unchanged.  Results are identical.  mode 0: never;
+   1: every real launch; 2 (default): the H_eff autotuner times both launch
+   kinds for each stage and keeps the faster (other tables launch one
+   workgroup per tile); -1 leaves the setting.  *previous receives the old
+   mode.  Replaces the reference's loop of per-block GEMMs over sector
+   triples (dmrg_inst.cpp:320-347, block_tensor_inst.cpp:2249-2279) like the
+   default launch; no reference counterpart of its own. */
+int sadmrg_gemm_persistent(int mode, int* previous);
 /* Split-reduction policy of the grouped GEMMs (process-wide, for plans built
    afterwards).  A table with fewer output tiles than the device has room for
    splits its reductions.  Balanced (1, the default): each problem's

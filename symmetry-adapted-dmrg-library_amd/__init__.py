@@ -102,6 +102,7 @@ def lib():
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
         "sadmrg_lanczos_mode": (i, [i, i, ip, C.POINTER(C.c_longlong), i]),
         "sadmrg_gemm_vec_staging": (i, [i, ip]),
+        "sadmrg_gemm_persistent": (i, [i, ip]),
         "sadmrg_gemm_balanced_split": (i, [i, ip]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_create_exci": (i, [vp, i, i, i, i, i, i, i, i, pp]),
@@ -477,6 +478,15 @@ def vec_staging(enable=None):
     prev = C.c_int()
     check(lib().sadmrg_gemm_vec_staging(-1 if enable is None else int(bool(enable)), C.byref(prev)))
     return bool(prev.value)
+
+
+def persistent(mode=None):
+    """Persistent tile-queue launches of the real grouped GEMMs
+    (process-wide): 0 never, 1 always, 2 autotuned per H_eff stage (the
+    default); returns the previous mode."""
+    prev = C.c_int()
+    check(lib().sadmrg_gemm_persistent(-1 if mode is None else int(mode), C.byref(prev)))
+    return prev.value
 
 
 def balanced_split(enable=None):

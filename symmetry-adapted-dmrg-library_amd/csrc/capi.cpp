@@ -601,6 +601,14 @@ int sadmrg_gemm_vec_staging(int enable, int* previous) {
   });
 }
 
+int sadmrg_gemm_persistent(int mode, int* previous) {
+  return guard([&] {
+    if (previous) *previous = GemmBatch::persistent();
+    if (mode > 2) fail(kErrInvalid, "gemm_persistent: mode must be -1, 0, 1 or 2");
+    if (mode >= 0) GemmBatch::set_persistent(mode);
+  });
+}
+
 int sadmrg_gemm_balanced_split(int enable, int* previous) {
   return guard([&] {
     if (previous) *previous = GemmBatch::balanced_split() ? 1 : 0;

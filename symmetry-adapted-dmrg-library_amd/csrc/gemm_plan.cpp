@@ -9,11 +9,14 @@ namespace sadmrg {
 
 namespace {
 std::atomic<bool> g_vec_staging{true};
+std::atomic<int> g_persistent{2};
 std::atomic<bool> g_balanced_split{true};
 }
 
 void GemmBatch::set_vec_staging(bool on) { g_vec_staging = on; }
 bool GemmBatch::vec_staging() { return g_vec_staging; }
+void GemmBatch::set_persistent(int mode) { g_persistent = mode; }
+int GemmBatch::persistent() { return g_persistent; }
 void GemmBatch::set_balanced_split(bool on) { g_balanced_split = on; }
 bool GemmBatch::balanced_split() { return g_balanced_split; }
 
@@ -316,6 +319,9 @@ GemmBatch::TileSet& GemmBatch::tileset(Context* ctx, int cfg) {
     ts.counters = DevBuf(ctx, hr.size() * sizeof(int));
     SADMRG_HIP(hipMemsetAsync(ts.counters.get(), 0, hr.size() * sizeof(int), ctx->stream()));
   }
+  // ticket and leaver counters of persistent launches (left zero by every launch)
+  ts.queue = DevBuf(ctx, 2 * sizeof(int));
+  SADMRG_HIP(hipMemsetAsync(ts.queue.get(), 0, 2 * sizeof(int), ctx->stream()));
   return ts;
 }
 
@@ -326,6 +332,10 @@ void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int c
   require(finalized_, "GemmBatch: run before finalize");
   if (hp_.empty()) return;
   if (cfg < 0) cfg = default_cfg(prec);
+  require(cfg < 2 * kPlans, "GemmBatch: bad launch plan");
+  const int mode = g_persistent;
+  const bool pers = !complex_ && (mode == 1 || (mode == 2 && cfg >= kPlans));
+  cfg %= kPlans;
   const bool zprec = prec == GemmPrec::Z64 || prec == GemmPrec::Z32 || prec == GemmPrec::Z32_OUT64;
   require(zprec == complex_, "GemmBatch: precision and table disagree on complex data");
   const TileSet& ts = tileset(ctx, cfg);
@@ -342,6 +352,7 @@ void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int c
   L.nreduce = ts.nreduce;
   L.vec = vec_ && g_vec_staging && !complex_ &&
           (prec == GemmPrec::F64 || prec == GemmPrec::F32 || prec == GemmPrec::F32_OUT64);
+  L.queue = pers ? ts.queue.as<int>() : nullptr;
   SADMRG_HIP(launch_grouped_gemm(prec, cfg % kTileConfigs, L, ctx->stream()));
 }
 
@@ -349,8 +360,8 @@ void GemmBatch::run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& a
   require(finalized_, "GemmBatch: run before finalize");
   if (hp_.empty()) return;
   if (cfg < 0) cfg = default_cfg(prec);
-  require(cfg >= 0 && cfg < kPlans, "GemmBatch: bad launch plan");
-  cfg %= kTileConfigs;  // no split reductions: the tile configuration only
+  require(cfg >= 0 && cfg < 2 * kPlans, "GemmBatch: bad launch plan");
+  cfg %= kTileConfigs;  // no split reductions or persistence: the tile configuration only
   require(!(complex_ && cfg != 0), "GemmBatch: complex tables use tile configuration 0");
   const bool zprec = prec == GemmPrec::Z64 || prec == GemmPrec::Z32 || prec == GemmPrec::Z32_OUT64;
   require(zprec == complex_, "GemmBatch: precision and table disagree on complex data");

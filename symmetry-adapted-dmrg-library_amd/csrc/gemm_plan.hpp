@@ -94,6 +94,14 @@ class GemmBatch {
   // (default on; off: element staging, for comparison).
   static void set_vec_staging(bool on);
   static bool vec_staging();
+  // Persistent launches of real tables: a grid of about the resident
+  // workgroup count taking tiles from a device ticket counter, instead of
+  // one workgroup per tile.  A launch plan id cfg >= kPlans is the
+  // persistent launch of plan cfg - kPlans.  Process-wide mode: 0 never
+  // (plans >= kPlans run as cfg - kPlans), 1 every launch, 2 (default) the
+  // autotuner times both launch kinds per stage (persistent(2) tells it to).
+  static void set_persistent(int mode);
+  static int persistent();
 
  private:
   struct P {
@@ -102,7 +110,7 @@ class GemmBatch {
   };
   struct TileSet {
     bool built = false;
-    DevBuf tiles, reduce, partial, counters;
+    DevBuf tiles, reduce, partial, counters, queue;
     int ntiles = 0, nreduce = 0, nparts = 0;
   };
   TileSet& tileset(Context* ctx, int cfg);

@@ -402,9 +402,12 @@ int HeffPlan::cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& 
   SADMRG_HIP(hipEventCreate(&e1));
   int best = g.default_cfg(prec);
   float best_ms = 0;
-  for (int cfg = 0; cfg < GemmBatch::kPlans; ++cfg) {
+  // plans >= kPlans: the persistent launch of the same plan (mode 2 only)
+  const int nplans = GemmBatch::persistent() == 2 ? 2 * GemmBatch::kPlans : GemmBatch::kPlans;
+  for (int cfg = 0; cfg < nplans; ++cfg) {
     // split variants only differ for tables that split their reductions
-    if (cfg >= kTileConfigs && g.nsplit_tiles(ctx_, cfg % kTileConfigs) == 0) continue;
+    const int plan = cfg % GemmBatch::kPlans;
+    if (plan >= kTileConfigs && g.nsplit_tiles(ctx_, plan % kTileConfigs) == 0) continue;
     g.run(ctx_, prec, at, cfg);  // builds the tile set and warms caches
     SADMRG_HIP(hipEventRecord(e0, ctx_->stream()));
     g.run(ctx_, prec, at, cfg);

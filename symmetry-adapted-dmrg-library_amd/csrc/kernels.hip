@@ -327,14 +327,14 @@ __device__ inline void mfma_step(const T* sa, const T* sb, Acc (&acc)[FM][FN], i
 // per CU, 256 registers per lane; 1: one workgroup, 512 registers).
 // PF: K steps that global loads run ahead of the LDS stage (1 or 2; 2 holds
 // two register sets of staged operands).
-template <class T, class TOut, int TM, int TN, int TK, int WPE, int PF, bool VEC = false>
-__global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
-                                                              const GemmProb* __restrict__ probs,
-                                                              const GemmSeg* __restrict__ segs,
-                                                              ArenaTable arenas,
-                                                              typename PartT<TOut>::type* __restrict__ partial,
-                                                              const GemmReduce* __restrict__ reduce,
-                                                              int* __restrict__ counters, int tile_base) {
+// One output tile (or one split part of it) of the grouped GEMM, executed by
+// the whole workgroup.
+template <class T, class TOut, int TM, int TN, int TK, int PF, bool VEC>
+__device__ __forceinline__ void gemm_tile(const GemmTile tile, const GemmProb* __restrict__ probs,
+                                          const GemmSeg* __restrict__ segs, const ArenaTable& arenas,
+                                          typename PartT<TOut>::type* __restrict__ partial,
+                                          const GemmReduce* __restrict__ reduce, int* __restrict__ counters,
+                                          const int tid) {
   using TA = OperandTile<T, TM, TK>;
   using TB = OperandTile<T, TN, TK>;
   constexpr int FM = TM / 32;  // 16x16 fragments per wave (2 x 2 waves)
@@ -345,9 +345,7 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
 
   __shared__ __align__(16) T lds[2][TA::ELEMS + TB::ELEMS];
 
-  const GemmTile tile = tiles[tile_base + blockIdx.x];
   const GemmProb prob = probs[tile.prob];
-  const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave & 1;
@@ -571,6 +569,55 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
     }
 }
 
+// The grouped GEMM launch.  queue == nullptr: one workgroup per tile
+// (tiles[tile_base + blockIdx.x]), placed by the hardware dispatcher.
+// Persistent (PERSIST): a grid of about the resident workgroup count; each
+// workgroup takes tile tickets from queue[0] until they exceed ntiles, and
+// the last workgroup to leave (queue[1] counts the leavers) resets both
+// counters, so the next launch (or graph replay) starts from zero without a
+// host-side fill.  Every workgroup takes exactly one failing ticket before it
+// signs out, so no ticket is taken after the reset.
+template <class T, class TOut, int TM, int TN, int TK, int WPE, int PF, bool VEC = false, bool PERSIST = false>
+__global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* __restrict__ tiles,
+                                                              const GemmProb* __restrict__ probs,
+                                                              const GemmSeg* __restrict__ segs,
+                                                              ArenaTable arenas,
+                                                              typename PartT<TOut>::type* __restrict__ partial,
+                                                              const GemmReduce* __restrict__ reduce,
+                                                              int* __restrict__ counters, int tile_base,
+                                                              int* __restrict__ queue, int ntiles) {
+  if constexpr (!PERSIST) {
+    gemm_tile<T, TOut, TM, TN, TK, PF, VEC>(tiles[tile_base + blockIdx.x], probs, segs, arenas, partial, reduce,
+                                            counters, static_cast<int>(threadIdx.x));
+  } else {
+    __shared__ int s_ticket;
+    for (;;) {
+      __syncthreads();  // the previous tile is done with LDS and s_ticket
+      if (threadIdx.x == 0)
+        s_ticket = __hip_atomic_fetch_add(&queue[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      // uniform across the workgroup: the tile and everything derived from it
+      // stay in scalar registers, as with blockIdx.x
+      const int t = __builtin_amdgcn_readfirstlane(s_ticket);
+      if (t >= ntiles) break;
+      // an opaque copy of the thread index per tile: nothing derived from it
+      // is hoisted out of the loop and kept live across tiles (the body needs
+      // the registers of the one-tile kernel)
+      int tid = static_cast<int>(threadIdx.x);
+      asm volatile("" : "+v"(tid));
+      gemm_tile<T, TOut, TM, TN, TK, PF, VEC>(tiles[tile_base + t], probs, segs, arenas, partial, reduce, counters,
+                                              tid);
+    }
+    if (threadIdx.x == 0) {
+      const int left = __hip_atomic_fetch_add(&queue[1], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (left == static_cast<int>(gridDim.x) - 1) {
+        __hip_atomic_store(&queue[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&queue[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
 // Complex grouped GEMM on interleaved (re, im) data, FP64 or FP32 operands
 // (FP32 accumulation, FP32 or FP64 output).  Four real MFMA products per
 // step; conjugation flags select the signs.
@@ -696,18 +743,41 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 template <class T, class TOut, int TM, int TN, int TK, int WPE = 2, int PF = 2>
 static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   typename PartT<TOut>::type* part = static_cast<typename PartT<TOut>::type*>(L.partial);
+  // Persistent variants exist where the tile loop compiles without scratch:
+  // FP64 128 x 64 tiles stage one K step ahead (two steps need the registers
+  // the loop keeps), and the FP32 128 x 128 tiles only with 16-byte staging;
+  // other tables launch one workgroup per tile.
+  constexpr int PFP = (sizeof(T) == 8 && TM == 128 && TN == 64) ? 1 : PF;
+  constexpr bool kElemPersist = !(sizeof(T) == 4 && TM == 128 && TN == 128);
+  if (L.queue && (L.vec || kElemPersist)) {
+    // persistent: about the resident workgroup count (WPE per CU)
+    const int grid = L.ntiles < kPersistCUs * WPE ? L.ntiles : kPersistCUs * WPE;
+    if constexpr (sizeof(T) >= 4) {
+      if (L.vec) {
+        hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PFP, true, true>), dim3(grid), dim3(256),
+                           0, st, L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, 0, L.queue,
+                           L.ntiles);
+        return hipGetLastError();
+      }
+    }
+    if constexpr (kElemPersist) {
+      hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PFP, false, true>), dim3(grid), dim3(256), 0,
+                         st, L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, 0, L.queue, L.ntiles);
+      return hipGetLastError();
+    }
+  }
   const int chunk = 1 << 30;
   for (int base = 0; base < L.ntiles; base += chunk) {
     const int n = L.ntiles - base < chunk ? L.ntiles - base : chunk;
     if constexpr (sizeof(T) >= 4) {
       if (L.vec) {
         hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF, true>), dim3(n), dim3(256), 0, st,
-                           L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base);
+                           L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base, nullptr, n);
         continue;
       }
     }
     hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF>), dim3(n), dim3(256), 0, st,
-                       L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base);
+                       L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base, nullptr, n);
   }
   return hipGetLastError();
 }

@@ -36,7 +36,13 @@ struct GemmLaunch {
   // contiguous extent and that rounded up to 16 bytes: operands are then
   // staged in 16-byte vectors (FP64 and FP32 tables).
   bool vec = false;
+  // Persistent launch (real tables): two zeroed ints, the tile ticket and
+  // leaver counters; the kernel leaves them zero.  nullptr: one workgroup
+  // per tile.
+  int* queue = nullptr;
 };
+// CUs the persistent grid is sized for (MI355X)
+constexpr int kPersistCUs = 256;
 
 // Output tile per configuration: 0: 64x64, 1: 128x128, 2: 128x64 (K steps of
 // 16 for FP64, 32 for FP32, 64 for BF16 operands).  Complex tables use
