@@ -274,6 +274,25 @@ typedef struct sadmrg_lanczos_options {
 int sadmrg_lanczos_default_options(sadmrg_lanczos_options* opt);
 int sadmrg_lanczos_ex(sadmrg_heff* h, sadmrg_tensor* psi, const sadmrg_lanczos_options* opt, double* energy,
                       int* counters, double* residual);
+/* Lanczos::lanczos1 (lanczos.hpp:70, dmrg_inst.cpp:1286-1348) of every
+   instance of a merged plan (sadmrg_heff_merge, n_inst parts; a plain plan
+   is one instance) at once.  Each Krylov step applies the merged plan once
+   (both stages of every instance in one launch each); inner products, the
+   recurrence, re-orthogonalisation, normalisation and Ritz vectors are
+   segmented per instance, and every instance keeps its own alpha/beta,
+   basis size min(mlanc, n_i + 1), breakdown stop and convergence test
+   (after every check_every-th step; the check matvec is shared by the
+   instances checking at that step).  precision FP32 with refine: FP32
+   Krylov matvecs, FP64 checks, then FP64 residual-correction restarts of
+   the instances with |r| > refine_tol |E|, batched; energies are FP64
+   Rayleigh quotients.  d_psi: the instances' start vectors back to back
+   (the layout of sadmrg_heff_apply_device), overwritten by their Ritz
+   vectors.  energies[n_inst]; steps, restarts, residuals (n_inst each,
+   optional); counters[2] (optional) = merged FP64 / FP32 applications.
+   mlanc in 2..62; device_steps and graph are ignored (host-driven steps).
+   No reference counterpart beyond lanczos1 per instance. */
+int sadmrg_lanczos_batched(sadmrg_heff* h, void* d_psi, int n_inst, const sadmrg_lanczos_options* opt,
+                           double* energies, int* steps, int* restarts, double* residuals, int* counters);
 
 /* ---- DMRG driver (DMRG<Scalar,Sym>, dmrg.hpp:29-133) --------------------- */
 /* DMRG(xx, yy, bondd, physpn, totspin, 0) (dmrg.hpp:99) */

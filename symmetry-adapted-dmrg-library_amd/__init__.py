@@ -97,6 +97,7 @@ def lib():
         "sadmrg_eigvalsh_batched_noalloc": (i, [vp, i, ip, pp, ip, pp]),
         "sadmrg_svd_eigensolver": (i, [i, ip]),
         "sadmrg_lanczos_ex": (i, [vp, vp, C.POINTER(LanczosOptions), dp, ip, dp]),
+        "sadmrg_lanczos_batched": (i, [vp, vp, i, C.POINTER(LanczosOptions), dp, ip, ip, dp, ip]),
         "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_svd_gram_min": (i, [i, ip]),
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
@@ -454,6 +455,26 @@ class Heff:
         check(lib().sadmrg_lanczos_ex(self.ptr, psi.ptr, C.byref(o), C.byref(e), _iptr(cnt), C.byref(res)))
         return e.value, {"matvec64": int(cnt[0]), "matvec32": int(cnt[1]), "steps": int(cnt[2]),
                          "restarts": int(cnt[3]), "residual64": res.value}
+
+    def lanczos_batched(self, d_psi, n_inst, **options):
+        """sadmrg_lanczos_batched on a merged plan: d_psi (device pointer) holds
+        the instances' start vectors back to back and receives their Ritz
+        vectors; options as lanczos_ex.  Returns (energies, info)."""
+        o = LanczosOptions()
+        check(lib().sadmrg_lanczos_default_options(C.byref(o)))
+        for k, v in options.items():
+            if k not in dict(LanczosOptions._fields_):
+                raise TypeError(f"unknown Lanczos option {k}")
+            setattr(o, k, type(getattr(o, k))(v))
+        e = np.zeros(n_inst)
+        steps = np.zeros(n_inst, dtype=np.int32)
+        rst = np.zeros(n_inst, dtype=np.int32)
+        res = np.zeros(n_inst)
+        cnt = np.zeros(2, dtype=np.int32)
+        check(lib().sadmrg_lanczos_batched(self.ptr, C.c_void_p(d_psi), n_inst, C.byref(o), _dptr(e), _iptr(steps),
+                                           _iptr(rst), _dptr(res), _iptr(cnt)))
+        return e, {"steps": steps, "restarts": rst, "residual64": res, "matvec64": int(cnt[0]),
+                   "matvec32": int(cnt[1])}
 
     def lanczos(self, psi, mlanc=40, precision=FP64, refine=False, refine_tol=0.0):
         e = C.c_double()

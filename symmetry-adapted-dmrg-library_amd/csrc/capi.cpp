@@ -784,6 +784,43 @@ int sadmrg_lanczos_ex(sadmrg_heff* h, sadmrg_tensor* psi, const sadmrg_lanczos_o
   });
 }
 
+int sadmrg_lanczos_batched(sadmrg_heff* h, void* d_psi, int n_inst, const sadmrg_lanczos_options* opt,
+                           double* energies, int* steps, int* restarts, double* residuals, int* counters) {
+  return guard([&] {
+    need(h, "heff");
+    need(d_psi, "psi");
+    need(opt, "opt");
+    need(energies, "energies");
+    const std::vector<int64_t> seg = h->p->part_offsets();
+    if (n_inst != static_cast<int>(seg.size()) - 1)
+      fail(kErrInvalid, "lanczos_batched: n_inst differs from the plan's instance count");
+    if (opt->mlanc < 2 || opt->mlanc > 62) fail(kErrInvalid, "lanczos_batched: mlanc must be in 2..62");
+    if (opt->check_every < 1) fail(kErrInvalid, "lanczos_batched: check_every must be >= 1");
+    if (opt->refine_krylov < 1 || opt->max_restarts < 0) fail(kErrInvalid, "lanczos_batched: bad refinement options");
+    if (!(opt->refine_tol > 0)) fail(kErrInvalid, "lanczos_batched: refine_tol must be > 0");
+    if (h->p->projectors() > 0) fail(kErrInvalid, "lanczos_batched: plans with projectors are not batched");
+    LanczosOpts o;
+    o.mlanc = opt->mlanc;
+    o.prec = prec_of(opt->precision);
+    o.refine = opt->refine != 0;
+    o.refine_tol = opt->refine_tol;
+    o.refine_krylov = opt->refine_krylov;
+    o.max_restarts = opt->max_restarts;
+    o.check_every = opt->check_every;
+    const BatchedLanczosOut r = lanczos_batched(*h->p, seg, static_cast<double*>(d_psi), o);
+    for (int i = 0; i < n_inst; ++i) {
+      energies[i] = r.energy[i];
+      if (steps) steps[i] = r.steps[i];
+      if (restarts) restarts[i] = r.restarts[i];
+      if (residuals) residuals[i] = r.residual64[i];
+    }
+    if (counters) {
+      counters[0] = r.matvec64;
+      counters[1] = r.matvec32;
+    }
+  });
+}
+
 int sadmrg_dmrg_create(sadmrg_ctx* ctx, int sym, int cplx, int xx, int yy, int bondd, int physpn, int totspin,
                        sadmrg_dmrg** out) {
   return guard([&] {

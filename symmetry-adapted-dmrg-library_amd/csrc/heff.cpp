@@ -293,6 +293,8 @@ std::unique_ptr<HeffPlan> HeffPlan::merge(const std::vector<const HeffPlan*>& pa
   std::unique_ptr<HeffPlan> m(new HeffPlan(ctx, cx));
   m->merged_ = true;
   int64_t pack = 0, scratch = 0, n = 0;
+  m->part_off_.assign(1, 0);
+  for (const HeffPlan* p : parts) m->part_off_.push_back(m->part_off_.back() + p->n_);
   for (const HeffPlan* p : parts) {
     require(p && p->ctx_ == ctx && p->cplx_ == cx, "H_eff merge: plans differ in context or scalar type");
     require(p->projectors() == 0, "H_eff merge: plans with projectors are not merged");

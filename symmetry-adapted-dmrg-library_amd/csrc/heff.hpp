@@ -78,6 +78,9 @@ class HeffPlan {
   void add_projector(const BTensor& phi, double weight);
   int projectors() const { return static_cast<int>(proj_w_.size()); }
   bool merged() const { return merged_; }
+  // merged plans: element offsets of the instances' vectors (parts + 1
+  // entries); a plain plan: {0, n}
+  std::vector<int64_t> part_offsets() const { return merged_ ? part_off_ : std::vector<int64_t>{0, n_}; }
   double stage_flops(int stage) const;
   const HeffStats& stats() const { return stats_; }
   Context* ctx() const { return ctx_; }
@@ -115,6 +118,7 @@ class HeffPlan {
   DevBuf proj_, proj_dot_;
   std::vector<double> proj_w_;
   bool merged_ = false;
+  std::vector<int64_t> part_off_;
   void apply_projectors(const double* d_in, double* d_out);
 };
 

@@ -1146,6 +1146,115 @@ static int grid_for(int64_t n, int block = 256) {
   return static_cast<int>(g);
 }
 
+// ---- segmented vector kernels (batched solves) ----------------------------
+// One workgroup per SegBlock; the columns are walked in order and each
+// block's partial sums are reduced in a fixed order, so results do not
+// depend on scheduling.
+__global__ __launch_bounds__(kRedBlock) void seg_dot_kernel(const double* __restrict__ X, int64_t ldx, int ncol,
+                                                            const double* __restrict__ y,
+                                                            const SegBlock* __restrict__ blocks, int cplx,
+                                                            double* __restrict__ partial) {
+  __shared__ double red[kRedBlock / 64][2];
+  const SegBlock b = blocks[blockIdx.x];
+  const int w = cplx ? 2 : 1;
+  for (int c = 0; c < ncol; ++c) {
+    const double* x = X + c * ldx * w;
+    double sr = 0, si = 0;
+    if (!cplx) {
+      for (int64_t i = b.begin + threadIdx.x; i < b.end; i += kRedBlock) sr = fma(x[i], y[i], sr);
+    } else {
+      for (int64_t i = b.begin + threadIdx.x; i < b.end; i += kRedBlock) {
+        const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+        sr += xr * yr + xi * yi;
+        si += xr * yi - xi * yr;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      sr += __shfl_down(sr, off);
+      si += __shfl_down(si, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[threadIdx.x >> 6][0] = sr;
+      red[threadIdx.x >> 6][1] = si;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tr = 0, ti = 0;
+      for (int q = 0; q < kRedBlock / 64; ++q) {
+        tr += red[q][0];
+        ti += red[q][1];
+      }
+      partial[(int64_t(blockIdx.x) * ncol + c) * 2] = tr;
+      partial[(int64_t(blockIdx.x) * ncol + c) * 2 + 1] = ti;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void seg_reduce_kernel(const double* __restrict__ partial, const int* __restrict__ first, int ninst,
+                                  int ncol, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ninst * ncol) return;
+  const int i = t / ncol, c = t % ncol;
+  double sr = 0, si = 0;
+  for (int b = first[i]; b < first[i + 1]; ++b) {
+    sr += partial[(int64_t(b) * ncol + c) * 2];
+    si += partial[(int64_t(b) * ncol + c) * 2 + 1];
+  }
+  out[int64_t(t) * 2] = sr;
+  out[int64_t(t) * 2 + 1] = si;
+}
+
+__global__ __launch_bounds__(256) void seg_combine_kernel(const double* __restrict__ X, int64_t ldx, int ncol,
+                                                          const double* __restrict__ coef,
+                                                          const double* __restrict__ beta, double* __restrict__ y,
+                                                          const SegBlock* __restrict__ blocks, int cplx) {
+  const SegBlock b = blocks[blockIdx.x];
+  const double* cf = coef + int64_t(b.inst) * ncol * 2;
+  const double bt = beta[b.inst];
+  if (!cplx) {
+    for (int64_t i = b.begin + threadIdx.x; i < b.end; i += 256) {
+      double acc = bt == 0.0 ? 0.0 : bt * y[i];
+      for (int c = 0; c < ncol; ++c) acc = fma(cf[2 * c], X[c * ldx + i], acc);
+      y[i] = acc;
+    }
+  } else {
+    for (int64_t i = b.begin + threadIdx.x; i < b.end; i += 256) {
+      double ar = bt == 0.0 ? 0.0 : bt * y[2 * i], ai = bt == 0.0 ? 0.0 : bt * y[2 * i + 1];
+      for (int c = 0; c < ncol; ++c) {
+        const double xr = X[(c * ldx + i) * 2], xi = X[(c * ldx + i) * 2 + 1];
+        const double cr = cf[2 * c], ci = cf[2 * c + 1];
+        ar = fma(cr, xr, fma(-ci, xi, ar));
+        ai = fma(cr, xi, fma(ci, xr, ai));
+      }
+      y[2 * i] = ar;
+      y[2 * i + 1] = ai;
+    }
+  }
+}
+
+hipError_t launch_seg_dot(const double* X, int64_t ldx, int ncol, const double* y, const SegBlock* blocks, int nblocks,
+                          int cplx, double* partial, hipStream_t st) {
+  if (nblocks <= 0 || ncol <= 0) return hipSuccess;
+  hipLaunchKernelGGL(seg_dot_kernel, dim3(nblocks), dim3(kRedBlock), 0, st, X, ldx, ncol, y, blocks, cplx, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_reduce(const double* partial, const int* first, int ninst, int ncol, double* out,
+                             hipStream_t st) {
+  if (ninst <= 0 || ncol <= 0) return hipSuccess;
+  const int n = ninst * ncol;
+  hipLaunchKernelGGL(seg_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, st, partial, first, ninst, ncol, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_combine(const double* X, int64_t ldx, int ncol, const double* coef, const double* beta,
+                              double* y, const SegBlock* blocks, int nblocks, int cplx, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(seg_combine_kernel, dim3(nblocks), dim3(256), 0, st, X, ldx, ncol, coef, beta, y, blocks, cplx);
+  return hipGetLastError();
+}
+
 hipError_t launch_gemv_t(const double* X, int64_t ldx, int ncol, const double* y, int64_t n, int cplx,
                          double* partial, double sign, double* out, hipStream_t st) {
   if (ncol <= 0) return hipSuccess;

@@ -110,6 +110,27 @@ constexpr int kPartialGrid = 1024;
 // One rectangular piece of a block scatter: dst[r + c*ldd] (+)= scale * src[r + c*lds]
 // for r < rows, c < cols (kind 0), or dst[t*(ldd+1)] = scale for t < rows
 // (kind 1: a scaled identity stripe).  Offsets are in elements.
+// Segmented vector operations of batched solves: a vector holds the
+// instances' vectors back to back; SegBlock b covers elements [begin, end)
+// of instance `inst` (blocks of one instance consecutive, in element order).
+struct SegBlock {
+  int64_t begin, end;
+  int32_t inst, pad;
+};
+// partial[(b * ncol + c) * 2 + {0, 1}] = conj(X[:, c]) . y over block b
+// (ldx in elements; w = 2 for complex data, conjugate-linear in X).
+hipError_t launch_seg_dot(const double* X, int64_t ldx, int ncol, const double* y, const SegBlock* blocks, int nblocks,
+                          int cplx, double* partial, hipStream_t st);
+// out[(i * ncol + c) * 2 + {0, 1}] = sum of the partials of instance i's
+// blocks [first[i], first[i + 1]) in block order (deterministic).
+hipError_t launch_seg_reduce(const double* partial, const int* first, int ninst, int ncol, double* out,
+                             hipStream_t st);
+// y = beta[i] y + sum_c coef[(i * ncol + c) * 2 + {re, im}] X[:, c] on the
+// elements of instance i (complex products when cplx); X may be null when
+// ncol == 0.
+hipError_t launch_seg_combine(const double* X, int64_t ldx, int ncol, const double* coef, const double* beta,
+                              double* y, const SegBlock* blocks, int nblocks, int cplx, hipStream_t st);
+
 struct ScatterTask {
   int64_t src_off, dst_off, lds, ldd;
   int32_t rows, cols, kind, pad;

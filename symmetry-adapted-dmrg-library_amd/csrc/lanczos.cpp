@@ -466,4 +466,369 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt) {
   return out;
 }
 
+
+// ---- batched solves over the instances of a merged plan -------------------
+namespace {
+
+// Segmented vector operations on the instances' vectors stored back to back.
+struct SegOps {
+  static constexpr int kMaxCol = 64;
+  static constexpr int64_t kChunk = 16384;  // elements per workgroup block
+  Context* ctx;
+  bool cx;
+  int ninst;
+  int64_t n;
+  int nb = 0;
+  DevBuf blocks, first, partial, out, coef, beta;
+  SegOps(Context* c, bool cx_, const std::vector<int64_t>& seg) : ctx(c), cx(cx_) {
+    ninst = static_cast<int>(seg.size()) - 1;
+    n = seg.back();
+    std::vector<SegBlock> hb;
+    std::vector<int> hf(ninst + 1, 0);
+    for (int i = 0; i < ninst; ++i) {
+      hf[i] = static_cast<int>(hb.size());
+      for (int64_t b = seg[i]; b < seg[i + 1]; b += kChunk)
+        hb.push_back(SegBlock{b, std::min(seg[i + 1], b + kChunk), i, 0});
+    }
+    hf[ninst] = static_cast<int>(hb.size());
+    nb = static_cast<int>(hb.size());
+    blocks = DevBuf(ctx, std::max<size_t>(hb.size(), 1) * sizeof(SegBlock));
+    first = DevBuf(ctx, hf.size() * sizeof(int));
+    if (nb) ctx->upload(blocks.get(), hb.data(), hb.size() * sizeof(SegBlock));
+    ctx->upload(first.get(), hf.data(), hf.size() * sizeof(int));
+    partial = DevBuf(ctx, std::max(nb, 1) * kMaxCol * 2 * sizeof(double));
+    out = DevBuf(ctx, ninst * kMaxCol * 2 * sizeof(double));
+    coef = DevBuf(ctx, ninst * kMaxCol * 2 * sizeof(double));
+    beta = DevBuf(ctx, ninst * sizeof(double));
+  }
+  // conj(X[:, c]) . y per instance: result[i * ncol + c]
+  std::vector<dcmplex> dots(const double* X, int64_t ldx, int ncol, const double* y) {
+    require(ncol >= 1 && ncol <= kMaxCol, "batched lanczos: too many columns");
+    SADMRG_HIP(launch_seg_dot(X, ldx, ncol, y, blocks.as<SegBlock>(), nb, cx ? 1 : 0, partial.as<double>(),
+                              ctx->stream()));
+    SADMRG_HIP(launch_seg_reduce(partial.as<double>(), first.as<int>(), ninst, ncol, out.as<double>(), ctx->stream()));
+    std::vector<double> h(static_cast<size_t>(ninst) * ncol * 2);
+    SADMRG_HIP(hipMemcpyAsync(h.data(), out.get(), h.size() * 8, hipMemcpyDeviceToHost, ctx->stream()));
+    ctx->sync();
+    std::vector<dcmplex> r(static_cast<size_t>(ninst) * ncol);
+    for (size_t k = 0; k < r.size(); ++k) r[k] = dcmplex(h[2 * k], h[2 * k + 1]);
+    return r;
+  }
+  std::vector<double> norms(const double* y) {
+    const std::vector<dcmplex> d = dots(y, 0, 1, y);
+    std::vector<double> r(ninst);
+    for (int i = 0; i < ninst; ++i) r[i] = std::sqrt(std::fabs(d[i].real()));
+    return r;
+  }
+  // y = b[i] y + sum_c c[i * ncol + c] X[:, c] on instance i's elements
+  void combine(const double* X, int64_t ldx, int ncol, const std::vector<dcmplex>& c, const std::vector<double>& b,
+               double* y) {
+    require(ncol >= 0 && ncol <= kMaxCol, "batched lanczos: too many columns");
+    if (ncol > 0) {
+      std::vector<double> h(static_cast<size_t>(ninst) * ncol * 2);
+      for (size_t k = 0; k < c.size(); ++k) {
+        h[2 * k] = c[k].real();
+        h[2 * k + 1] = c[k].imag();
+      }
+      ctx->upload(coef.get(), h.data(), h.size() * 8);
+    }
+    ctx->upload(beta.get(), b.data(), b.size() * 8);
+    SADMRG_HIP(launch_seg_combine(X, ldx, ncol, coef.as<double>(), beta.as<double>(), y, blocks.as<SegBlock>(), nb,
+                                  cx ? 1 : 0, ctx->stream()));
+  }
+  void scale(const std::vector<double>& a, double* y) { combine(nullptr, 0, 0, {}, a, y); }
+};
+
+}  // namespace
+
+BatchedLanczosOut lanczos_batched(HeffPlan& H, const std::vector<int64_t>& seg, double* d_vec, const LanczosOpts& opt) {
+  Context* ctx = H.ctx();
+  const bool cx = H.complex();
+  const int w = cx ? 2 : 1;
+  require(seg.size() >= 2 && seg.front() == 0 && seg.back() == H.n(), "batched lanczos: bad instance offsets");
+  for (size_t i = 0; i + 1 < seg.size(); ++i) require(seg[i + 1] > seg[i], "batched lanczos: empty instance");
+  const int ni = static_cast<int>(seg.size()) - 1;
+  const int64_t nw = H.n() * w;
+  std::vector<int> ml(ni);
+  int ML = 2;
+  for (int i = 0; i < ni; ++i) {
+    ml[i] = static_cast<int>(std::max<int64_t>(2, std::min<int64_t>(opt.mlanc, seg[i + 1] - seg[i] + 1)));
+    ML = std::max(ML, ml[i]);
+  }
+  require(ML + 1 <= SegOps::kMaxCol, "batched lanczos: mlanc must be at most 63");
+  SegOps S(ctx, cx, seg);
+  BatchedLanczosOut out;
+  out.energy.assign(ni, 0.0);
+  out.residual64.assign(ni, -1.0);
+  out.steps.assign(ni, 0);
+  out.restarts.assign(ni, 0);
+  DevBuf Vb(ctx, std::max<int64_t>(nw, 1) * 8 * (ML + 1)), hvb(ctx, std::max<int64_t>(nw, 1) * 8);
+  double* V = Vb.as<double>();
+  double* hv = hvb.as<double>();
+  auto col = [&](int j) { return V + static_cast<int64_t>(j) * nw; };
+  const Prec bulk = opt.prec;
+  const bool mixed = bulk != Prec::FP64 && opt.refine;
+  const Prec check_prec = mixed ? Prec::FP64 : bulk;
+  auto matvec = [&](const double* x, double* y, Prec p) {
+    H.apply(x, y, p);
+    if (p == Prec::FP64) out.matvec64++;
+    else out.matvec32++;
+  };
+  const double tol = 1e-6;
+  std::vector<std::vector<double>> aal(ni, std::vector<double>(ML + 1, 0.0)), nnl(ni, std::vector<double>(ML + 2, 0.0));
+  std::vector<uint8_t> active(ni, 1);
+  const std::vector<double> ones(ni, 1.0), zeros(ni, 0.0);
+  std::vector<dcmplex> zc;
+
+  // start vectors (initialize_lanczos, dmrg_inst.cpp:1201-1212)
+  SADMRG_HIP(hipMemcpyAsync(col(0), d_vec, nw * 8, hipMemcpyDeviceToDevice, ctx->stream()));
+  {
+    std::vector<double> nor = S.norms(col(0)), a(ni);
+    for (int i = 0; i < ni; ++i) a[i] = nor[i] > 1e-12 ? 1.0 / nor[i] : 1.0;
+    S.scale(a, col(0));
+  }
+  // x (d_vec) holds each instance's current Ritz vector; tridiagonal
+  // eigenvectors per instance for the Ritz combinations
+  auto tridiag = [&](int i, int m, std::vector<double>& d, std::vector<double>& z) {
+    d.assign(aal[i].begin(), aal[i].begin() + m);
+    std::vector<double> e(std::max(m - 1, 0));
+    for (int k = 0; k + 1 < m; ++k) e[k] = nnl[i][k + 1];
+    tridiag_eig(m, d, e, z);
+  };
+  // Ritz vectors of the instances in `set` (k[i] basis vectors), the
+  // others' x left as is; then, if check, normalisation and one shared
+  // check matvec: returns |<x,Hx>| / |Hx| per instance
+  auto ritz_and_check = [&](const std::vector<int>& set, const std::vector<int>& kk,
+                            const std::vector<std::vector<double>>& zz, bool check) {
+    int K = 1;
+    for (int i : set) K = std::max(K, kk[i]);
+    std::vector<dcmplex> c(static_cast<size_t>(ni) * K, dcmplex(0, 0));
+    std::vector<double> b(ni, 1.0);
+    for (int i : set) {
+      b[i] = 0.0;
+      for (int j = 0; j < kk[i]; ++j) c[static_cast<size_t>(i) * K + j] = zz[i][j];
+    }
+    S.combine(V, H.n(), K, c, b, d_vec);
+    std::vector<double> ov(ni, 0.0);
+    if (!check) return ov;
+    std::vector<double> nor = S.norms(d_vec), a(ni, 1.0);
+    for (int i : set) a[i] = nor[i] > 0 ? 1.0 / nor[i] : 1.0;
+    S.scale(a, d_vec);
+    matvec(d_vec, hv, check_prec);
+    const std::vector<double> hn = S.norms(hv);
+    const std::vector<dcmplex> d = S.dots(d_vec, 0, 1, hv);
+    for (int i : set) ov[i] = hn[i] > 0 ? std::abs(d[i]) / hn[i] : 1.0;
+    return ov;
+  };
+  std::vector<int> kk(ni, 0);
+  std::vector<std::vector<double>> zz(ni);
+  std::vector<double> dtmp, ztmp;
+  auto finish = [&](int i, int k, std::vector<int>& set) {
+    tridiag(i, k, dtmp, ztmp);
+    out.energy[i] = dtmp[0];
+    zz[i].assign(ztmp.begin(), ztmp.begin() + k);  // eigenvector 0 (leading dimension k)
+    kk[i] = k;
+    set.push_back(i);
+  };
+
+  // step 1
+  matvec(col(0), col(1), bulk);
+  {
+    const std::vector<dcmplex> a = S.dots(col(0), 0, 1, col(1));
+    std::vector<dcmplex> c(ni);
+    for (int i = 0; i < ni; ++i) {
+      aal[i][0] = a[i].real();
+      c[i] = -aal[i][0];
+    }
+    S.combine(col(0), 0, 1, c, ones, col(1));
+    const std::vector<double> b = S.norms(col(1));
+    std::vector<double> sc(ni);
+    std::vector<int> fin;
+    for (int i = 0; i < ni; ++i) {
+      nnl[i][1] = b[i];
+      out.steps[i] = 1;
+      sc[i] = b[i] >= tol ? 1.0 / b[i] : 0.0;
+      if (b[i] < tol) {  // Krylov space exhausted after one vector
+        finish(i, 1, fin);
+        active[i] = 0;
+      }
+    }
+    S.scale(sc, col(1));
+    if (!fin.empty()) ritz_and_check(fin, kk, zz, !mixed ? true : true);
+  }
+  for (int m = 2; m <= ML; ++m) {
+    bool any = false;
+    for (int i = 0; i < ni; ++i) any = any || active[i];
+    if (!any) break;
+    matvec(col(m - 1), col(m), bulk);
+    const std::vector<dcmplex> a = S.dots(col(m - 1), 0, 1, col(m));
+    // instances at their last basis vector (m == ml): the reference's final
+    // step takes alpha and stops (dmrg_inst.cpp:1341-1347)
+    std::vector<int> fin, chk;
+    std::vector<uint8_t> step(ni, 0);
+    for (int i = 0; i < ni; ++i) {
+      if (!active[i]) continue;
+      aal[i][m - 1] = a[i].real();
+      if (m == ml[i]) {
+        out.steps[i] = m;
+        finish(i, m, fin);
+        active[i] = 0;
+      } else {
+        step[i] = 1;
+      }
+    }
+    // three-term recurrence, two re-orthogonalisation passes and the
+    // normalisation for the stepping instances; every other instance's
+    // column m is zeroed
+    {
+      std::vector<dcmplex> c(static_cast<size_t>(ni) * 2, dcmplex(0, 0));
+      std::vector<double> b(ni, 0.0);
+      for (int i = 0; i < ni; ++i)
+        if (step[i]) {
+          c[2 * i] = -nnl[i][m - 1];
+          c[2 * i + 1] = -aal[i][m - 1];
+          b[i] = 1.0;
+        }
+      S.combine(col(m - 2), nw / w, 2, c, b, col(m));
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<dcmplex> c = S.dots(V, H.n(), m, col(m));
+      for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < m; ++j) c[static_cast<size_t>(i) * m + j] = step[i] ? -c[static_cast<size_t>(i) * m + j] : 0.0;
+      S.combine(V, H.n(), m, c, ones, col(m));
+    }
+    const std::vector<double> bn = S.norms(col(m));
+    std::vector<double> sc(ni, 0.0);
+    for (int i = 0; i < ni; ++i) {
+      if (!step[i]) continue;
+      nnl[i][m] = bn[i];
+      out.steps[i] = m;
+      if (bn[i] < tol) {  // breakdown: stop at this step (dmrg_inst.cpp:1322-1328)
+        finish(i, m, fin);
+        active[i] = 0;
+        continue;
+      }
+      sc[i] = 1.0 / bn[i];
+      if (m % opt.check_every == 0) finish(i, m, chk);
+    }
+    S.scale(sc, col(m));
+    std::vector<int> set = fin;
+    set.insert(set.end(), chk.begin(), chk.end());
+    if (set.empty()) continue;
+    const std::vector<double> ov = ritz_and_check(set, kk, zz, true);
+    for (int i : chk)
+      if (std::fabs(ov[i] - 1.0) < 1e-8) active[i] = 0;
+  }
+  ctx->sync();
+  if (!mixed) return out;
+
+  // FP64 residual-correction restarts (batched over the instances that need one)
+  const int k = std::max(1, std::min(opt.refine_krylov, ML - 1));
+  auto residual = [&](const std::vector<int>& set, std::vector<double>& a0, std::vector<double>& b1) {
+    // x normalised; hv = H x (FP64); col(1) = hv - a0 x for the instances in set
+    std::vector<double> nor = S.norms(d_vec), a(ni, 1.0);
+    for (int i : set) a[i] = nor[i] > 0 ? 1.0 / nor[i] : 1.0;
+    S.scale(a, d_vec);
+    matvec(d_vec, hv, Prec::FP64);
+    const std::vector<dcmplex> d = S.dots(d_vec, 0, 1, hv);
+    SADMRG_HIP(hipMemcpyAsync(col(1), hv, nw * 8, hipMemcpyDeviceToDevice, ctx->stream()));
+    std::vector<dcmplex> c(ni, dcmplex(0, 0));
+    std::vector<double> b(ni, 0.0);
+    for (int i : set) {
+      a0[i] = d[i].real();
+      c[i] = -a0[i];
+      b[i] = 1.0;
+    }
+    S.combine(d_vec, 0, 1, c, b, col(1));
+    const std::vector<double> bn = S.norms(col(1));
+    for (int i : set) {
+      b1[i] = bn[i];
+      out.energy[i] = a0[i];
+      out.residual64[i] = b1[i];
+    }
+  };
+  std::vector<int> all(ni);
+  for (int i = 0; i < ni; ++i) all[i] = i;
+  std::vector<double> a0(ni, 0.0), b1(ni, 0.0);
+  residual(all, a0, b1);
+  for (int r = 0; r < opt.max_restarts; ++r) {
+    std::vector<int> need;
+    for (int i = 0; i < ni; ++i)
+      if (b1[i] > opt.refine_tol * std::fabs(a0[i])) need.push_back(i);
+    if (need.empty()) break;
+    std::vector<uint8_t> on(ni, 0);
+    for (int i : need) {
+      on[i] = 1;
+      out.restarts[i]++;
+    }
+    // col(0) = x, col(1) = r / |r| on the instances refined; the others zero
+    SADMRG_HIP(hipMemcpyAsync(col(0), d_vec, nw * 8, hipMemcpyDeviceToDevice, ctx->stream()));
+    {
+      std::vector<double> z0(ni, 0.0), s1(ni, 0.0);
+      for (int i : need) {
+        z0[i] = 1.0;
+        s1[i] = 1.0 / b1[i];
+      }
+      S.scale(z0, col(0));
+      S.scale(s1, col(1));
+    }
+    std::vector<std::vector<double>> ta(ni), tb(ni);
+    std::vector<uint8_t> live = on;
+    for (int i : need) {
+      ta[i] = {a0[i]};
+      tb[i] = {0.0, b1[i]};
+    }
+    for (int j = 1; j <= k; ++j) {
+      matvec(col(j), col(j + 1), bulk);
+      const std::vector<dcmplex> a = S.dots(col(j), 0, 1, col(j + 1));
+      for (int i : need)
+        if (live[i]) ta[i].push_back(a[i].real());
+      if (j == k) break;
+      std::vector<dcmplex> c(static_cast<size_t>(ni) * 2, dcmplex(0, 0));
+      std::vector<double> b(ni, 0.0);
+      for (int i : need)
+        if (live[i]) {
+          c[2 * i] = -tb[i][j];
+          c[2 * i + 1] = -ta[i][j];
+          b[i] = 1.0;
+        }
+      S.combine(col(j - 1), nw / w, 2, c, b, col(j + 1));
+      for (int pass = 0; pass < 2; ++pass) {
+        std::vector<dcmplex> cc = S.dots(V, H.n(), j + 1, col(j + 1));
+        for (int i = 0; i < ni; ++i)
+          for (int q = 0; q <= j; ++q)
+            cc[static_cast<size_t>(i) * (j + 1) + q] = live[i] ? -cc[static_cast<size_t>(i) * (j + 1) + q] : 0.0;
+        S.combine(V, H.n(), j + 1, cc, ones, col(j + 1));
+      }
+      const std::vector<double> bn = S.norms(col(j + 1));
+      std::vector<double> sc(ni, 0.0);
+      for (int i : need) {
+        if (!live[i]) continue;
+        if (bn[i] < 1e-12 * std::fabs(a0[i])) {
+          live[i] = 0;
+          continue;
+        }
+        sc[i] = 1.0 / bn[i];
+        tb[i].push_back(bn[i]);
+      }
+      S.scale(sc, col(j + 1));
+    }
+    // Ritz vector of each refined instance in its small Krylov space
+    std::vector<int> kk2(ni, 0);
+    std::vector<std::vector<double>> z2(ni);
+    for (int i : need) {
+      const int mk = static_cast<int>(ta[i].size());
+      std::vector<double> d(ta[i].begin(), ta[i].end()), e(std::max(mk - 1, 0)), z;
+      for (int q = 0; q + 1 < mk; ++q) e[q] = tb[i][q + 1];
+      tridiag_eig(mk, d, e, z);
+      z2[i].assign(z.begin(), z.begin() + mk);
+      kk2[i] = mk;
+    }
+    ritz_and_check(need, kk2, z2, false);
+    residual(need, a0, b1);
+  }
+  ctx->sync();
+  return out;
+}
+
 }  // namespace sadmrg

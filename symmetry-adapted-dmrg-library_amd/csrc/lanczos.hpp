@@ -76,4 +76,26 @@ void tridiag_eig(int n, std::vector<double>& d, std::vector<double> e, std::vect
 // normalised Ritz vector.
 LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt);
 
+// Lanczos::lanczos1 of every instance of a merged plan (HeffPlan::merge) at
+// once: one merged H_eff application per Krylov step serves every instance,
+// and the vector operations (inner products, the three-term recurrence,
+// re-orthogonalisation, normalisation, Ritz vectors) are segmented per
+// instance, each instance with its own alpha/beta, basis size
+// min(mlanc, n_i + 1), breakdown stop and convergence test (after every
+// check_every-th step, |<v,Hv>| / |Hv| within 1e-8 of 1, the check matvec
+// shared by the instances checking at that step).  Instances that finish
+// stop updating; the merged matvec still covers them.  Mixed precision
+// (prec FP32 with refine): FP32 Krylov matvecs, FP64 checks, then FP64
+// residual-correction restarts of every instance whose |r| exceeds
+// refine_tol |E| (batched over those instances); energies are FP64
+// Rayleigh quotients.  seg: instance offsets in elements (n_inst + 1
+// entries, seg[0] = 0, seg.back() = H.n()).  Host-driven steps; mlanc <= 64.
+struct BatchedLanczosOut {
+  std::vector<double> energy, residual64;
+  std::vector<int> steps, restarts;
+  int matvec64 = 0, matvec32 = 0;  // merged applications
+};
+BatchedLanczosOut lanczos_batched(HeffPlan& H, const std::vector<int64_t>& seg, double* d_vec,
+                                  const LanczosOpts& opt);
+
 }  // namespace sadmrg

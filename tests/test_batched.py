@@ -142,3 +142,88 @@ def test_merged_real_plans(ctx, sad):
         ref = tl.from_dev(plan.apply(psi, sad.FP64))["data"]
         assert np.max(np.abs(got[off:off + ref.size] - ref)) <= 1e-13 * np.max(np.abs(ref))
         off += ref.size
+
+
+def _chain_instances(ctx, sad, lengths, m, seed=11):
+    """XXZ + Dzyaloshinskii-Moriya chains (U(1) complex drivers), iDMRG and
+    one sweep at m kept states each; the centre-bond H_eff plan of each and a
+    seeded start vector (the sweep's input vector plus Gaussian noise)."""
+    rng = np.random.default_rng(seed)
+    plans, starts, psis = [], [], []
+    for L in lengths:
+        hmap = np.asarray(sad.open_chain(L), dtype=np.int32)
+        coup = np.zeros(L + 1)
+        coup[1] = 1.0
+        d = sad.DMRG(ctx, sad.U1, 1, 1, L, m)
+        d.set_couplings(hmap, coup)
+        d.set_channel_weights(sad.xxz_dm_weights(L + 1, {1: (1.0, rng.uniform(0.5, 1.5), rng.uniform(0, 0.5))}))
+        d.set_max_dcut(m)
+        d.idmrg()
+        d.sweep()
+        il, ir = L // 2 - 1, L // 2
+        psi = d.input_vector(il, ir)
+        h = tl.from_dev(psi)
+        h["data"] = h["data"] + 0.3 * np.abs(h["data"]).max() * rng.standard_normal(h["data"].size)
+        starts.append(h)
+        psis.append(psi)
+        plans.append(d.heff_plan(il, ir, psi))
+    return plans, starts, psis
+
+
+def _dense(ctx, sad, plan, layout):
+    """The plan's H_eff as a dense complex matrix (applied to unit vectors)."""
+    h = tl.from_dev(layout)
+    n = h["data"].size // 2
+    cols = []
+    for k in range(n):
+        e = dict(h)
+        e["data"] = np.zeros_like(h["data"])
+        e["data"][2 * k] = 1.0
+        o = tl.from_dev(plan.apply(tl.to_dev(ctx, e), sad.FP64))["data"]
+        cols.append(o[0::2] + 1j * o[1::2])
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_lanczos_matches_single_solves(ctx, sad, precision):
+    """sadmrg_lanczos_batched on the merged plan of four chains of different
+    lengths.  FP64: every instance follows its own single solve (the same
+    lanczos1 schedule): energies to 1e-12, Ritz vectors to 1e-8 up to a
+    phase.  Mixed (FP32 Krylov, FP64 checks, residual-correction restarts to
+    |r| <= 1e-9 |E|): every energy equals the exact lowest eigenvalue of the
+    instance's H_eff (dense, Hermitian) to 1e-11."""
+    lengths = [8, 12, 10, 12]
+    plans, starts, psis = _chain_instances(ctx, sad, lengths, m=16)
+    merged = sad.Heff.merge(plans)
+    ref_e, ref_v = [], []
+    for p, s, psi in zip(plans, starts, psis):
+        v = tl.to_dev(ctx, s)
+        e, _ = p.lanczos(v, mlanc=40, precision=sad.FP64)
+        ref_e.append(e)
+        ref_v.append(tl.from_dev(v)["data"])
+    data = np.concatenate([s["data"] for s in starts])
+    n = data.size // 2
+    vin = tl.to_dev(ctx, dict(sym=0, cplx=1, header=tl.make_header(0, [(1, [0], [n]), (-1, [0], [1])]),
+                              present=np.ones(1, np.uint8), data=data))
+    opts = dict(mlanc=40, precision=sad.FP64)
+    if precision == "mixed":
+        opts.update(precision=sad.FP32, refine=1, refine_tol=1e-9, refine_krylov=4, max_restarts=8)
+    e, info = merged.lanczos_batched(vin.device_ptr(), len(plans), **opts)
+    ref_e = np.asarray(ref_e)
+    if precision == "fp64":
+        assert np.all(np.abs(e - ref_e) <= 1e-12 * np.abs(ref_e)), (e - ref_e)
+        got = tl.from_dev(vin)["data"]
+        off = 0
+        for i, r in enumerate(ref_v):
+            g = got[off:off + r.size]
+            off += r.size
+            zg, zr = g[0::2] + 1j * g[1::2], r[0::2] + 1j * r[1::2]
+            ph = np.vdot(zr, zg)
+            ph /= abs(ph)
+            assert np.max(np.abs(zg - ph * zr)) <= 1e-8, (i, float(np.max(np.abs(zg - ph * zr))))
+    else:
+        assert info["matvec32"] > 0
+        assert np.all(info["residual64"] <= 1e-9 * np.abs(e) * (1 + 1e-6)), info["residual64"]
+        exact = np.asarray([np.linalg.eigvalsh(_dense(ctx, sad, p, psi))[0] for p, psi in zip(plans, psis)])
+        assert np.all(np.abs(e - exact) <= 1e-11 * np.abs(exact)), (e - exact)
