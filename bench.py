@@ -315,6 +315,58 @@ def c2_run(sad, args):
     return line
 
 
+def c2_sweeps(sad, ctx, L=100, D=512, nsweep=6):
+    """C2 as whole runs: SU(2) open Heisenberg chain L=100, total spin 0,
+    iDMRG and 6 sweeps at D=512 multiplets, FP64 throughout, once with the
+    grouped H_eff launch and once with one launch per output block (every
+    H_eff of the run, sadmrg_heff_per_block); per-sweep energies and
+    truncation errors, and the centre bond's multiplet histogram."""
+    import sadmrg_testlib as tl
+    hmap = np.asarray(sad.open_chain(L), dtype=np.int32)
+    coup = np.zeros(L + 1)
+    coup[1] = 1.0
+    out = {}
+    for name, per_block in (("grouped", False), ("per_block", True)):
+        sad.heff_per_block(per_block)
+        sad.lib().sadmrg_rng_seed(42)
+        d = sad.DMRG(ctx, sad.SU2, 0, 1, L, D)
+        d.set_couplings(hmap, coup)
+        d.set_max_dcut(D)
+        ctx.sync()
+        t0 = time.perf_counter()
+        d.idmrg()
+        d.records()
+        t_idmrg = time.perf_counter() - t0
+        es, trs, split = [], [], {"t_env": 0.0, "t_lanczos": 0.0, "t_svd": 0.0}
+        mv = 0
+        for s_ in range(nsweep):
+            d.sweep()
+            recs = d.records()
+            es.append(d.energy())
+            trs.append(max(r["trunc_err"] for r in recs))
+            mv += int(sum(r["matvec64"] for r in recs))
+            for k in split:
+                split[k] += sum(r[k] for r in recs)
+            print(f"[bench] C2 {name}: sweep {s_ + 1} E={d.energy():.12f} ({time.perf_counter() - t0:.1f} s)",
+                  file=sys.stderr, flush=True)
+        ctx.sync()
+        secs = time.perf_counter() - t0
+        _, _, ub = tl.parse_header(d.tensor("uu", L // 2 - 1).header())
+        out[name] = {"seconds": round(secs, 3), "idmrg_seconds": round(t_idmrg, 3),
+                     "sweep_seconds": round(secs - t_idmrg, 3), "sweep_energies": es, "max_trunc_err": trs,
+                     "matvecs": mv, "split_s": {k: round(v, 3) for k, v in split.items()}}
+        if name == "grouped":
+            out["centre_bond_histogram"] = {"two_j": [int(x) for x in ub[-1][1]],
+                                            "multiplets": [int(x) for x in ub[-1][2]]}
+        del d
+    sad.heff_per_block(False)
+    out["workload"] = f"C2: SU(2) open chain L={L}, S=0, iDMRG + {nsweep} sweeps at D={D} multiplets, FP64"
+    out["grouped_over_per_block"] = round(out["per_block"]["seconds"] / out["grouped"]["seconds"], 3)
+    out["same_energies"] = bool(np.allclose(out["grouped"]["sweep_energies"], out["per_block"]["sweep_energies"],
+                                            rtol=1e-12, atol=0))
+    return out
+
+
 def c1_run(sad, args):
     """C1 (U(1) open chain L=32, m=64 kept states, 4 sweeps, FP64): the
     engine's iDMRG + 4 sweeps against the reference library's own step
@@ -853,6 +905,9 @@ def main():
                          "c1: U(1) L=32 m=64 four sweeps vs the reference; "
                          "c4: 64 U(1) complex chains at m=256, all centre bonds in one launch; "
                          "env: environment updates at the C3 centre bond")
+    ap.add_argument("--extras", default="c2,c5",
+                    help="default workload: further configurations reported in the same line (c2: whole runs, "
+                         "grouped vs per-block; c5: the L=48 D=1024 gradient); '' for none")
     ap.add_argument("--c4-solves", type=int, default=1,
                     help="c4 workload: also run the 64 batched ground-state solves on real drivers (0: skip)")
     ap.add_argument("--fp64-energy-every", default="0",
@@ -1022,6 +1077,20 @@ def main():
         sweep["warm_up_s"] = round(t_warm, 2)
         del dr
 
+    # further configurations in the same line: C2 whole runs (grouped vs
+    # per-block launches, plus the centre-bond H_eff against the reference
+    # on the host) and the C5 gradient with a finite-difference check
+    extras = {}
+    want = [x for x in args.extras.split(",") if x]
+    if "c2" in want:
+        print("[bench] C2 runs", file=sys.stderr, flush=True)
+        extras["c2"] = c2_sweeps(sad, ctx)
+        extras["c2"]["centre_bond_heff"] = c2_run(sad, args)
+    if "c5" in want:
+        print("[bench] C5 gradient", file=sys.stderr, flush=True)
+        extras["c5"] = {"workload": "C5: coupling gradient, SU(2) open chain L=48, D=1024",
+                        **gradient_run(sad, ctx, 48, 1024, [256, 512], fd_bonds=(None,))}
+
     # the reference on the host cores: both BLAS settings, each in a child
     # process; the synthetic bond for `value`, the real centre step for the
     # sweep-step estimate
@@ -1123,6 +1192,7 @@ def main():
                                    "frac": round(res["fp64"]["tflops"] / PEAK_FP64, 4)}},
         "real_centre_bond": real,
         "sigma_variant": sig24,
+        **extras,
         "cpu_baseline": cpu_summary(cpu, flops, real, sweep),
     }
     print(json.dumps(line))

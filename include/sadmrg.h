@@ -53,6 +53,12 @@ int sadmrg_ctx_create(int device, sadmrg_ctx** out);
 int sadmrg_ctx_destroy(sadmrg_ctx* ctx);
 int sadmrg_ctx_sync(sadmrg_ctx* ctx);
 int sadmrg_ctx_memory(sadmrg_ctx* ctx, long long* bytes_in_use, long long* bytes_cached);
+/* Host-side counters of the context since its creation: device allocations
+   that missed the allocator cache (hipMalloc calls) and stream
+   synchronisations, each with the wall seconds spent in them (any pointer
+   may be NULL).  No reference counterpart. */
+int sadmrg_ctx_stats(sadmrg_ctx* ctx, long long* mallocs, double* malloc_seconds, long long* syncs,
+                     double* sync_seconds);
 /* the context's HIP stream (hipStream_t), for callers that time or order work on it */
 int sadmrg_ctx_stream(sadmrg_ctx* ctx, void** stream);
 
@@ -141,6 +147,12 @@ int sadmrg_heff_apply_stage(sadmrg_heff* h, const void* d_in, void* d_out, int p
    the grouped launch, as the reference issues one GEMM per block.  No
    reference counterpart. */
 int sadmrg_heff_apply_unbatched(sadmrg_heff* h, const void* d_in, void* d_out, int precision);
+/* Process-wide: every H_eff application, the Lanczos solves of the DMRG
+   driver included, runs as sadmrg_heff_apply_unbatched does (one launch per
+   output block of each stage), so whole sweeps can be timed against the
+   grouped launch.  enable 0/1 (default 0), -1 leaves the setting;
+   *previous receives the old value.  No reference counterpart. */
+int sadmrg_heff_per_block(int enable, int* previous);
 /* One plan over n independent plans (instances, e.g. many chains of
    different lengths): their block tables merged, so each H_eff stage of
    every instance runs in a single launch.  Apply it with

@@ -54,6 +54,7 @@ def lib():
         "sadmrg_ctx_destroy": (i, [vp]),
         "sadmrg_ctx_sync": (i, [vp]),
         "sadmrg_ctx_memory": (i, [vp, llp, llp]),
+        "sadmrg_ctx_stats": (i, [vp, C.POINTER(C.c_longlong), dp, C.POINTER(C.c_longlong), dp]),
         "sadmrg_ctx_stream": (i, [vp, pp]),
         "sadmrg_clebsch_gordan": (i, [i, i, i, i, i, i, dp]),
         "sadmrg_cgc_create": (i, [i, pp]),
@@ -104,6 +105,7 @@ def lib():
         "sadmrg_lanczos_mode": (i, [i, i, ip, C.POINTER(C.c_longlong), i]),
         "sadmrg_gemm_vec_staging": (i, [i, ip]),
         "sadmrg_gemm_persistent": (i, [i, ip]),
+        "sadmrg_heff_per_block": (i, [i, ip]),
         "sadmrg_gemm_balanced_split": (i, [i, ip]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_create_exci": (i, [vp, i, i, i, i, i, i, i, i, pp]),
@@ -206,6 +208,12 @@ class Context:
         a, b = C.c_longlong(), C.c_longlong()
         check(lib().sadmrg_ctx_memory(self.ptr, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def host_stats(self):
+        """(hipMalloc calls, their seconds, stream synchronisations, their seconds)."""
+        a, b, c, d = C.c_longlong(), C.c_double(), C.c_longlong(), C.c_double()
+        check(lib().sadmrg_ctx_stats(self.ptr, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"mallocs": a.value, "malloc_s": b.value, "syncs": c.value, "sync_s": d.value}
 
 
 class Cgc:
@@ -498,6 +506,14 @@ def vec_staging(enable=None):
     previous setting."""
     prev = C.c_int()
     check(lib().sadmrg_gemm_vec_staging(-1 if enable is None else int(bool(enable)), C.byref(prev)))
+    return bool(prev.value)
+
+
+def heff_per_block(enable=None):
+    """Every H_eff application as one launch per output block (process-wide);
+    returns the previous setting."""
+    prev = C.c_int()
+    check(lib().sadmrg_heff_per_block(-1 if enable is None else int(bool(enable)), C.byref(prev)))
     return bool(prev.value)
 
 

@@ -8,14 +8,15 @@
 #include <cmath>
 #include <cstring>
 #include <rocsolver/rocsolver.h>
+#include <array>
 #include <map>
+#include <unordered_map>
 #include "kernels.hpp"
 
 namespace sadmrg {
 
 namespace {
 
-constexpr int kMaxLeg = 8;
 
 GemmPrec prec_of(bool cplx) { return cplx ? GemmPrec::Z64 : GemmPrec::F64; }
 
@@ -502,13 +503,14 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
     const int nto = os.nten();
     struct C { int j, k; };
     std::vector<std::vector<C>> per(nto);
-    for (int j = 0; j < uu.nten(); ++j) {
-      if (!uu.st.allowed(j) || !uu.present(j)) continue;
-      labels_of(uu, j, a1);
-      for (int k = 0; k < vv.nten(); ++k) {
-        if (!vv.st.allowed(k) || !vv.present(k)) continue;
-        labels_of(vv, k, a2);
-        if (a1[0] != a2[0] || a1[1] != a2[1]) continue;
+    const BlockList ul(uu), vl(vv);
+    const LabelIndex vix(vl, 0, 1);
+    for (int pj = 0; pj < ul.size(); ++pj) {
+      const int j = ul.b[pj];
+      for (int q = 0; q < 3; ++q) a1[q] = ul.lab[pj][q];
+      for (const int pk : vix.get(a1[0], a1[1])) {
+        const int k = vl.b[pk];
+        for (int q = 0; q < 3; ++q) a2[q] = vl.lab[pk][q];
         const int lab[2] = {a1[2], a2[2]};
         const int m = os.index_of_labels(lab);
         // blocks outside the symmetry-allowed set are never read back
@@ -600,19 +602,18 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
             "overlap transformation wrong bb");
     os.bonds = {uu.st.bonds[2], vv.st.bonds[2].inverted()};
     per.resize(os.nten());
-    const int nt1 = op.nten(), nt2 = uu.nten();
-    for (int i = 0; i < nt1 * nt2; ++i) {
-      const int j = i % nt1, k = (i / nt1) % nt2;
-      if (!op.st.allowed(j) || !uu.st.allowed(k)) continue;
-      labels_of(op, j, a1);
-      labels_of(uu, k, a2);
-      if (a1[0] != a2[flag]) continue;
-      if (!op.present(j) || !uu.present(k)) continue;
+    const BlockList ol(op), ul(uu), vl(vv);
+    const LabelIndex oix(ol, 0, -1), vix(vl, flag, 1 - flag);
+    for (int pk = 0; pk < ul.size(); ++pk) {
+      const int k = ul.b[pk];
+      for (int q = 0; q < 3; ++q) a2[q] = ul.lab[pk][q];
+      for (const int pj : oix.get(a2[flag])) {
+      const int j = ol.b[pj];
+      for (int q = 0; q < 3; ++q) a1[q] = ol.lab[pj][q];
       int si = -1;
-      for (int l = 0; l < vv.nten(); ++l) {
-        if (!vv.st.allowed(l) || !vv.present(l)) continue;
-        labels_of(vv, l, a3);
-        if (a1[1] != a3[flag] || a2[1 - flag] != a3[1 - flag]) continue;
+      for (const int pl : vix.get(a1[1], a2[1 - flag])) {
+        const int l = vl.b[pl];
+        for (int q = 0; q < 3; ++q) a3[q] = vl.lab[pl][q];
         const int lab[2] = {a2[2], a3[2]};
         if (!os.allowed_labels(lab)) continue;
         const int m = os.index_of_labels(lab);
@@ -626,6 +627,7 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
           scratch += int64_t(y) * r;
         }
         per[m].push_back(Con{si, l, 1.0, s1[si].M});
+      }
       }
     }
     std::vector<uint8_t> mask(os.nten());
@@ -672,20 +674,19 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
     // operator values are needed on the host as scalar weights
     std::vector<double> hop_tmp;
     const double* hop = op.host_values(hop_tmp);
-    const int nt1 = op.nten(), nt2 = uu.nten();
     const int P2 = cgc ? cgc->p2() : 3;
-    for (int i = 0; i < nt1 * nt2; ++i) {
-      const int j = i % nt1, k = (i / nt1) % nt2;
-      if (!op.st.allowed(j) || !uu.st.allowed(k)) continue;
-      labels_of(op, j, a1);
-      labels_of(uu, k, a2);
-      if (a1[0] != a2[1 - flag]) continue;
-      if (!op.present(j) || !uu.present(k)) continue;
+    const BlockList ol(op), ul(uu), vl(vv);
+    const LabelIndex oix(ol, 0, -1), vix(vl, 1 - flag, flag);
+    for (int pk = 0; pk < ul.size(); ++pk) {
+      const int k = ul.b[pk];
+      for (int q = 0; q < 3; ++q) a2[q] = ul.lab[pk][q];
+      for (const int pj : oix.get(a2[1 - flag])) {
+      const int j = ol.b[pj];
+      for (int q = 0; q < 3; ++q) a1[q] = ol.lab[pj][q];
       require(op.block_size(j) == 1, "operator initial: multi-element site operators are unsupported");
-      for (int l = 0; l < vv.nten(); ++l) {
-        if (!vv.st.allowed(l) || !vv.present(l)) continue;
-        labels_of(vv, l, a3);
-        if (a1[2] != a3[1 - flag] || a2[flag] != a3[flag]) continue;
+      for (const int pl : vix.get(a1[2], a2[flag])) {
+        const int l = vl.b[pl];
+        for (int q = 0; q < 3; ++q) a3[q] = vl.lab[pl][q];
         const int lab[3] = {a2[2], a1[1], a3[2]};
         if (!os.allowed_labels(lab)) continue;
         const int m = os.index_of_labels(lab);
@@ -698,6 +699,7 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
         const double sv = hop[op.off[j] * op.w()];
         const int K = dims3(uu, k, 0) * dims3(uu, k, 1);
         per[m].push_back(Con{k, l, fac * sv, K});
+      }
       }
     }
     std::vector<uint8_t> mask(os.nten());
@@ -726,19 +728,18 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
             "operator transformation wrong bb");
     os.bonds = {uu.st.bonds[2], op.st.bonds[1], vv.st.bonds[2].inverted()};
     per.resize(os.nten());
-    const int nt1 = op.nten(), nt2 = uu.nten();
-    for (int i = 0; i < nt1 * nt2; ++i) {
-      const int j = i % nt1, k = (i / nt1) % nt2;
-      if (!op.st.allowed(j) || !uu.st.allowed(k)) continue;
-      labels_of(op, j, a1);
-      labels_of(uu, k, a2);
-      if (a1[0] != a2[flag]) continue;
-      if (!op.present(j) || !uu.present(k)) continue;
+    const BlockList ol(op), ul(uu), vl(vv);
+    const LabelIndex oix(ol, 0, -1), vix(vl, flag, 1 - flag);
+    for (int pk = 0; pk < ul.size(); ++pk) {
+      const int k = ul.b[pk];
+      for (int q = 0; q < 3; ++q) a2[q] = ul.lab[pk][q];
+      for (const int pj : oix.get(a2[flag])) {
+      const int j = ol.b[pj];
+      for (int q = 0; q < 3; ++q) a1[q] = ol.lab[pj][q];
       int si = -1;
-      for (int l = 0; l < vv.nten(); ++l) {
-        if (!vv.st.allowed(l) || !vv.present(l)) continue;
-        labels_of(vv, l, a3);
-        if (a1[2] != a3[flag] || a2[1 - flag] != a3[1 - flag]) continue;
+      for (const int pl : vix.get(a1[2], a2[1 - flag])) {
+        const int l = vl.b[pl];
+        for (int q = 0; q < 3; ++q) a3[q] = vl.lab[pl][q];
         const int lab[3] = {a2[2], a1[1], a3[2]};
         if (!os.allowed_labels(lab)) continue;
         const int m = os.index_of_labels(lab);
@@ -753,6 +754,7 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
           s1.push_back(stage1_uu_op(j, k));
         }
         per[m].push_back(Con{si, l, fac, dims3(op, j, 2)});
+      }
       }
     }
     std::vector<uint8_t> mask(os.nten());
@@ -817,24 +819,22 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
     std::vector<double> hsg_tmp;
     const double* hsg = sg.host_values(hsg_tmp);
     const int P2 = cgc ? cgc->p2() : 3;
-    const int nt1 = op.nten(), nt2 = uu.nten();
-    for (int i = 0; i < nt1 * nt2; ++i) {
-      const int j = i % nt1, k = (i / nt1) % nt2;
-      if (!op.st.allowed(j) || !uu.st.allowed(k)) continue;
-      labels_of(op, j, a1);
-      labels_of(uu, k, a2);
-      if (a1[0] != a2[flag]) continue;
-      if (!op.present(j) || !uu.present(k)) continue;
+    const BlockList ol(op), ul(uu), vl(vv), gl(sg);
+    const LabelIndex oix(ol, 0, -1), vix(vl, flag, -1);
+    for (int pk = 0; pk < ul.size(); ++pk) {
+      const int k = ul.b[pk];
+      for (int q = 0; q < 3; ++q) a2[q] = ul.lab[pk][q];
+      for (const int pj : oix.get(a2[flag])) {
+      const int j = ol.b[pj];
+      for (int q = 0; q < 3; ++q) a1[q] = ol.lab[pj][q];
       int si = -1;
-      for (int l = 0; l < vv.nten(); ++l) {
-        if (!vv.st.allowed(l) || !vv.present(l)) continue;
-        labels_of(vv, l, a3);
-        if (a1[2] != a3[flag]) continue;
-        for (int n = 0; n < sg.nten(); ++n) {
-          if (!sg.st.allowed(n)) continue;
-          labels_of(sg, n, a4);
+      for (const int pl : vix.get(a1[2])) {
+        const int l = vl.b[pl];
+        for (int q = 0; q < 3; ++q) a3[q] = vl.lab[pl][q];
+        for (int pn = 0; pn < gl.size(); ++pn) {
+          const int n = gl.b[pn];
+          for (int q = 0; q < 3; ++q) a4[q] = gl.lab[pn][q];
           if (a4[0] != a2[1 - flag] || a4[2] != a3[1 - flag] || a4[1] != a1[1]) continue;
-          if (!sg.present(n)) continue;
           const int lab[2] = {a2[2], a3[2]};
           if (!os.allowed_labels(lab)) continue;
           const int m = os.index_of_labels(lab);
@@ -854,6 +854,7 @@ void op_env_impl(BTensor& out, int kind, const BTensor& uu, const BTensor& vv, c
           }
           per[m].push_back(Con{si, l, fac * sv, dims3(op, j, 2)});
         }
+      }
       }
     }
     std::vector<uint8_t> mask(os.nten());

@@ -12,8 +12,10 @@
 // thresholds, and lowers the per-block GEMMs into grouped block tables
 // (gemm_plan.hpp) executed by one or two device launches.
 #pragma once
+#include <array>
 #include <cstdint>
 #include <random>
+#include <unordered_map>
 #include <vector>
 #include "cgc.hpp"
 #include "core.hpp"
@@ -21,6 +23,9 @@
 #include "symmetry.hpp"
 
 namespace sadmrg {
+
+// Largest number of legs of a block tensor.
+constexpr int kMaxLeg = 8;
 
 // Symbolic CGC block for SU(2) 2-leg tensors (3-leg tensors always carry the
 // canonical <j0 j1 | j2> block in the paths used here).
@@ -64,6 +69,40 @@ struct BTensor {
   void keep_host();
   // The data on the host: `host` when kept, otherwise read back into tmp.
   const double* host_values(std::vector<double>& tmp) const;
+};
+
+// The present, allowed blocks of a tensor with their labels, in block order
+// (the order in which the reference's loops visit them), and indexes of
+// those blocks by one or two labels.  Block loop nests (environment
+// updates, H_eff plans) walk only the matching blocks, in the same order as
+// the reference's full loop nests.
+struct BlockList {
+  std::vector<int> b;
+  std::vector<std::array<int, 3>> lab;
+  explicit BlockList(const BTensor& t) {
+    int l[kMaxLeg];
+    for (int x = 0; x < t.nten(); ++x) {
+      if (!t.present(x) || !t.st.allowed(x)) continue;
+      t.st.labels(x, l);
+      b.push_back(x);
+      lab.push_back({l[0], t.nbond() > 1 ? l[1] : 0, t.nbond() > 2 ? l[2] : 0});
+    }
+  }
+  int size() const { return static_cast<int>(b.size()); }
+};
+
+struct LabelIndex {
+  std::unordered_map<int64_t, std::vector<int>> m;
+  static int64_t key(int a, int c) { return (static_cast<int64_t>(a) << 32) ^ static_cast<uint32_t>(c); }
+  // positions p of list (ascending) keyed by (lab[p][i0], lab[p][i1]); i1 < 0: one label
+  LabelIndex(const BlockList& list, int i0, int i1) {
+    for (int p = 0; p < list.size(); ++p) m[key(list.lab[p][i0], i1 < 0 ? 0 : list.lab[p][i1])].push_back(p);
+  }
+  const std::vector<int>& get(int a, int c = 0) const {
+    static const std::vector<int> none;
+    auto it = m.find(key(a, c));
+    return it == m.end() ? none : it->second;
+  }
 };
 
 // ---- exchange with host (the C ABI tensor format) ------------------------

@@ -127,6 +127,17 @@ int sadmrg_ctx_sync(sadmrg_ctx* ctx) {
   });
 }
 
+int sadmrg_ctx_stats(sadmrg_ctx* ctx, long long* mallocs, double* malloc_seconds, long long* syncs,
+                     double* sync_seconds) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (mallocs) *mallocs = ctx->c->mallocs();
+    if (malloc_seconds) *malloc_seconds = ctx->c->malloc_seconds();
+    if (syncs) *syncs = ctx->c->syncs();
+    if (sync_seconds) *sync_seconds = ctx->c->sync_seconds();
+  });
+}
+
 int sadmrg_ctx_memory(sadmrg_ctx* ctx, long long* in_use, long long* cached) {
   return guard([&] {
     need(ctx, "ctx");
@@ -565,6 +576,13 @@ int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n
       info[6] = s.tile_cfg2;
       info[7] = s.split_tiles;
     }
+  });
+}
+
+int sadmrg_heff_per_block(int enable, int* previous) {
+  return guard([&] {
+    if (previous) *previous = heff_per_block() ? 1 : 0;
+    if (enable >= 0) set_heff_per_block(enable != 0);
   });
 }
 

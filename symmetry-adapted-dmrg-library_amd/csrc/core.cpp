@@ -1,5 +1,6 @@
 // core.cpp -- device context and caching allocator.
 #include "core.hpp"
+#include <chrono>
 #include <cstring>
 #include "eigen.hpp"
 #include <rocsolver/rocsolver.h>
@@ -53,7 +54,18 @@ Context::~Context() {
   if (stream_) hipStreamDestroy(stream_);
 }
 
-void Context::sync() { SADMRG_HIP(hipStreamSynchronize(stream_)); }
+namespace {
+double wall_now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+void Context::sync() {
+  const double t0 = wall_now();
+  SADMRG_HIP(hipStreamSynchronize(stream_));
+  ++n_sync_;
+  t_sync_ += wall_now() - t0;
+}
 
 void Context::upload(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return;
@@ -119,12 +131,15 @@ void* Context::raw_alloc(size_t bytes, size_t* cap) {
     return p;
   }
   void* p = nullptr;
+  const double t0 = wall_now();
   hipError_t e = hipMalloc(&p, c);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     trim();
     SADMRG_HIP(hipMalloc(&p, c));
   }
+  ++n_malloc_;
+  t_malloc_ += wall_now() - t0;
   *cap = c;
   in_use_ += c;
   return p;

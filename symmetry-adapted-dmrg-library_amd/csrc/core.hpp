@@ -122,12 +122,20 @@ class Context {
 
   size_t bytes_in_use() const { return in_use_; }
   size_t bytes_cached() const { return cached_; }
+  // host-side counters: device allocations that missed the cache (hipMalloc)
+  // and stream synchronisations, with the wall seconds spent in each
+  long long mallocs() const { return n_malloc_; }
+  double malloc_seconds() const { return t_malloc_; }
+  long long syncs() const { return n_sync_; }
+  double sync_seconds() const { return t_sync_; }
 
  private:
   int device_;
   hipStream_t stream_ = nullptr;
   std::multimap<size_t, void*> free_;
   size_t in_use_ = 0, cached_ = 0;
+  long long n_malloc_ = 0, n_sync_ = 0;
+  double t_malloc_ = 0, t_sync_ = 0;
   DevBuf partial_, dscal_;
   void* solver_ = nullptr;
   // staging ring of upload(): in-flight regions [begin, end) with the event

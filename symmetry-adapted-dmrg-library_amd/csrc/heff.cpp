@@ -1,5 +1,6 @@
 // heff.cpp -- see heff.hpp.
 #include "heff.hpp"
+#include <atomic>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -192,6 +193,7 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
   }
 
   // bond terms
+  const BlockList lp(psi);
   for (size_t t = 0; t < o1.size(); ++t) {
     const BTensor* op1 = o1[t];
     const BTensor* op2 = o2[t];
@@ -200,27 +202,29 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
     require(op1->st.bonds[0] == ps.bonds[0].inverted() && op2->st.bonds[0] == ps.bonds[1].inverted() &&
                 op1->st.bonds[1] == ch2 && op1->st.bonds[2] == ps.bonds[0] && op2->st.bonds[2] == ps.bonds[1],
             "H_eff: bond-term operator bonds do not match psi");
-    const int nt1 = op1->nten(), nt3 = op2->nten();
     int a1[3], a3[3], d1[3], d3[3];
-    for (int i = 0; i < nt1 * ntp; ++i) {
-      const int j = i % nt1, k = (i / nt1) % ntp;
-      if (!op1->st.allowed(j) || !ps.allowed(k)) continue;
-      op1->st.labels(j, a1);
-      ps.labels(k, lab);
-      if (a1[0] != lab[0]) continue;
-      if (!op1->present(j) || !psi.present(k)) continue;
+    // the reference's loop nest (k over psi, j over op1, l over op2) over
+    // matching blocks only
+    const BlockList l1(*op1), l3(*op2);
+    const LabelIndex ix1(l1, 0, -1), ix3(l3, 0, 1);
+    for (int ip = 0; ip < lp.size(); ++ip) {
+      const int k = lp.b[ip];
+      lab[0] = lp.lab[ip][0];
+      lab[1] = lp.lab[ip][1];
+      for (const int pj : ix1.get(lab[0])) {
+      const int j = l1.b[pj];
+      for (int q = 0; q < 3; ++q) a1[q] = l1.lab[pj][q];
       op1->st.block_dims(j, d1);
       ps.block_dims(k, d);
       int64_t xoff = -1;
       int xm = 0;
-      for (int l = 0; l < nt3; ++l) {
-        if (!op2->st.allowed(l)) continue;
-        op2->st.labels(l, a3);
-        if (a3[0] != lab[1] || a1[1] != a3[1]) continue;
+      for (const int pl : ix3.get(lab[1], a1[1])) {
+        const int l = l3.b[pl];
+        for (int q = 0; q < 3; ++q) a3[q] = l3.lab[pl][q];
         const int ol[2] = {a1[2], a3[2]};
         if (!ps.allowed_labels(ol)) continue;
         const int m = ps.index_of_labels(ol);
-        if (m < 0 || !op2->present(l)) continue;
+        if (m < 0) continue;
         double fac = 1.0;
         if (su2) {
           fac = cgc->hamilt_vec(a1[0], a1[1], a1[2]);
@@ -248,6 +252,7 @@ HeffPlan::HeffPlan(Context* ctx, CgcTables* cgc, const BTensor& hl, const BTenso
         A.trans = true;
         g2_.add_seg(prob_of[m], A, B, kk, fac);
         contributions[m]++;
+      }
       }
     }
   }
@@ -435,7 +440,15 @@ double HeffPlan::stage_flops(int stage) const {
   return g1_.flops(cplx_) + g2_.flops(cplx_);
 }
 
+namespace {
+std::atomic<bool> g_per_block{false};
+}  // namespace
+
+void set_heff_per_block(bool on) { g_per_block = on; }
+bool heff_per_block() { return g_per_block; }
+
 void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bool unbatched) {
+  unbatched = unbatched || g_per_block;
   auto exec = [&](GemmBatch& g, GemmPrec p, const ArenaTable& at, int cfg) {
     if (unbatched) g.run_per_problem(ctx_, p, at, cfg);
     else g.run(ctx_, p, at, cfg);

@@ -49,6 +49,12 @@ bool autotune();
 int autotune_entries();
 void autotune_clear();
 
+// Process-wide: every H_eff application (Lanczos included) runs one launch
+// per output block of each stage instead of the grouped launch (the per-block
+// baseline, as the reference issues one GEMM per block).  Default off.
+void set_heff_per_block(bool on);
+bool heff_per_block();
+
 class HeffPlan {
  public:
   // hl / hr may be null tensors.  o1[t] / o2[t] are the bond-term operators
