@@ -153,6 +153,21 @@ int sadmrg_heff_apply_unbatched(sadmrg_heff* h, const void* d_in, void* d_out, i
    grouped launch.  enable 0/1 (default 0), -1 leaves the setting;
    *previous receives the old value.  No reference counterpart. */
 int sadmrg_heff_per_block(int enable, int* previous);
+/* Process-wide: every H_eff stage runs launch plan `plan` (0 .. 15: tile
+   configuration + 4 x split variant + 8 x launch kind, as in
+   sadmrg_heff_stats; complex tables accept tile configurations 0 and 3 and
+   ignore the rest) instead of the autotuned or default plan; -1 restores
+   them, below -1 leaves the setting.  *previous receives the old value.  For
+   tests and measurements of single plans.  No reference counterpart. */
+int sadmrg_heff_launch_plan(int plan, int* previous);
+/* Process-wide: complex H_eff plans run as real tables on planar operands
+   (1: each complex block product becomes four real ones on the re and im
+   planes, through the real grouped kernel with its 16-byte staging, split
+   reductions, persistent launches and autotuner) or on the complex kernel
+   over interleaved data (0, the default: one load of each operand panel
+   serves its four MFMA products, faster on small blocks).  -1 leaves the
+   setting; *previous receives the old value.  No reference counterpart. */
+int sadmrg_heff_complex_planar(int enable, int* previous);
 /* One plan over n independent plans (instances, e.g. many chains of
    different lengths): their block tables merged, so each H_eff stage of
    every instance runs in a single launch.  Apply it with
@@ -166,9 +181,16 @@ int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops);
 /* info[8] = {psi blocks, stage-1 problems, stage-2 problems, stage-1 tiles,
    stage-2 tiles, stage-1 launch plan, stage-2 launch plan, output tiles with
    a split reduction}.  Launch plan = tile configuration (0: 64x64, 1:
-   128x128, 2: 128x64 output tiles) + 3 x split variant (0: about 3, 1: about
-   8 reduction parts per resident workgroup). */
+   128x128, 2: 128x64, 3: 32x32 output tiles; complex tables 0 and 3) + 4 x
+   split variant (0: about 3, 1: about 8 reduction parts per resident
+   workgroup) + 8 x launch kind (1: persistent). */
 int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n, int* info);
+/* MFMA flops issued by the launch plans of the plan's last application for
+   operands of `precision` (stage 1 or 2, 0: both): every output tile is
+   computed whole over its problem's segments, K rounded up to the K step.
+   The algorithmic flops over this figure is the share of MFMA work that is
+   not edge-tile padding.  No reference counterpart. */
+int sadmrg_heff_issued_flops(sadmrg_heff* h, int stage, int precision, double* issued);
 /* tile-configuration autotuner of the H_eff stages (process-wide, on by
    default): enable 0/1 (-1 leaves the setting), clear forgets the tuned
    entries, *entries receives the table size.  No reference counterpart (the
@@ -294,10 +316,11 @@ int sadmrg_lanczos_ex(sadmrg_heff* h, sadmrg_tensor* psi, const sadmrg_lanczos_o
    segmented per instance, and every instance keeps its own alpha/beta,
    basis size min(mlanc, n_i + 1), breakdown stop and convergence test
    (after every check_every-th step; the check matvec is shared by the
-   instances checking at that step).  precision FP32 with refine: FP32
-   Krylov matvecs, FP64 checks, then FP64 residual-correction restarts of
-   the instances with |r| > refine_tol |E|, batched; energies are FP64
-   Rayleigh quotients.  d_psi: the instances' start vectors back to back
+   instances checking at that step).  refine: FP64 checks, then FP64
+   residual-correction restarts of the instances with |r| > refine_tol |E|,
+   batched, their Krylov steps in the bulk precision (FP32 operands for the
+   mixed path; FP64 gives a refined reference); energies are FP64 Rayleigh
+   quotients.  d_psi: the instances' start vectors back to back
    (the layout of sadmrg_heff_apply_device), overwritten by their Ritz
    vectors.  energies[n_inst]; steps, restarts, residuals (n_inst each,
    optional); counters[2] (optional) = merged FP64 / FP32 applications.

@@ -90,6 +90,7 @@ def lib():
         "sadmrg_heff_apply_stage": (i, [vp, vp, vp, i, i]),
         "sadmrg_heff_apply_unbatched": (i, [vp, vp, vp, i]),
         "sadmrg_heff_stage_flops": (i, [vp, i, dp]),
+        "sadmrg_heff_issued_flops": (i, [vp, i, i, dp]),
         "sadmrg_heff_stats": (i, [vp, dp, dp, llp, ip]),
         "sadmrg_heff_merge": (i, [C.POINTER(vp), i, pp]),
         "sadmrg_lanczos": (i, [vp, vp, i, i, i, d, dp, ip, dp]),
@@ -106,6 +107,8 @@ def lib():
         "sadmrg_gemm_vec_staging": (i, [i, ip]),
         "sadmrg_gemm_persistent": (i, [i, ip]),
         "sadmrg_heff_per_block": (i, [i, ip]),
+        "sadmrg_heff_complex_planar": (i, [i, ip]),
+        "sadmrg_heff_launch_plan": (i, [i, ip]),
         "sadmrg_gemm_balanced_split": (i, [i, ip]),
         "sadmrg_dmrg_create": (i, [vp, i, i, i, i, i, i, i, pp]),
         "sadmrg_dmrg_create_exci": (i, [vp, i, i, i, i, i, i, i, i, pp]),
@@ -437,6 +440,13 @@ class Heff:
         check(lib().sadmrg_heff_stage_flops(self.ptr, stage, C.byref(f)))
         return f.value
 
+    def issued_flops(self, precision=FP64, stage=0):
+        """MFMA flops the last application's launch plans issue (edge tiles
+        whole, K rounded up to the K step)."""
+        f = C.c_double()
+        check(lib().sadmrg_heff_issued_flops(self.ptr, stage, precision, C.byref(f)))
+        return f.value
+
     def stats(self):
         f, b = C.c_double(), C.c_double()
         n = C.c_longlong()
@@ -506,6 +516,22 @@ def vec_staging(enable=None):
     previous setting."""
     prev = C.c_int()
     check(lib().sadmrg_gemm_vec_staging(-1 if enable is None else int(bool(enable)), C.byref(prev)))
+    return bool(prev.value)
+
+
+def heff_launch_plan(plan=None):
+    """Force every H_eff stage onto one launch plan (-1: tuned/default);
+    returns the previous setting."""
+    prev = C.c_int()
+    check(lib().sadmrg_heff_launch_plan(-2 if plan is None else int(plan), C.byref(prev)))
+    return prev.value
+
+
+def heff_complex_planar(enable=None):
+    """Complex H_eff plans as real tables on planar operands (process-wide,
+    default off: the complex kernel); returns the previous setting."""
+    prev = C.c_int()
+    check(lib().sadmrg_heff_complex_planar(-1 if enable is None else int(bool(enable)), C.byref(prev)))
     return bool(prev.value)
 
 

@@ -559,6 +559,15 @@ int sadmrg_heff_stage_flops(sadmrg_heff* h, int stage, double* flops) {
   });
 }
 
+int sadmrg_heff_issued_flops(sadmrg_heff* h, int stage, int precision, double* issued) {
+  return guard([&] {
+    need(h, "heff");
+    need(issued, "issued");
+    require(stage >= 0 && stage <= 2, "heff_issued_flops: stage must be 0, 1 or 2");
+    *issued = h->p->issued_flops(stage, prec_of(precision));
+  });
+}
+
 int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n, int* info) {
   return guard([&] {
     need(h, "heff");
@@ -576,6 +585,20 @@ int sadmrg_heff_stats(sadmrg_heff* h, double* flops, double* bytes, long long* n
       info[6] = s.tile_cfg2;
       info[7] = s.split_tiles;
     }
+  });
+}
+
+int sadmrg_heff_launch_plan(int plan, int* previous) {
+  return guard([&] {
+    if (previous) *previous = heff_launch_plan();
+    if (plan >= -1) set_heff_launch_plan(plan);
+  });
+}
+
+int sadmrg_heff_complex_planar(int enable, int* previous) {
+  return guard([&] {
+    if (previous) *previous = heff_complex_planar() ? 1 : 0;
+    if (enable >= 0) set_heff_complex_planar(enable != 0);
   });
 }
 

@@ -88,6 +88,26 @@ double GemmBatch::flops(bool cplx) const {
   return cplx ? 4.0 * f : f;
 }
 
+double GemmBatch::issued_flops(int cfg, GemmPrec prec) const {
+  require(finalized_, "GemmBatch::issued_flops: table not finalized");
+  const int tc = cfg < 0 ? default_cfg(prec) % kTileConfigs : (cfg % kPlans) % kTileConfigs;
+  const int tm = tile_m(tc), tn = tile_n(tc);
+  int tk = 16;
+  double per = 2.0;
+  if (complex_)
+    per = 8.0;  // four real products per complex step, K step 16
+  else
+    tk = (prec == GemmPrec::F64) ? 16 : (prec == GemmPrec::BF16 || prec == GemmPrec::BF16_OUT64) ? 64 : 32;
+  double f = 0;
+  for (const auto& p : hp_) {
+    double k = 0;
+    for (int s = p.seg_begin; s < p.seg_end; ++s) k += double((hs_[s].K + tk - 1) / tk) * tk;
+    const double tiles = double((p.M + tm - 1) / tm) * double((p.N + tn - 1) / tn);
+    f += per * tiles * tm * tn * k;
+  }
+  return f;
+}
+
 // Launch order of the tiles.  Workgroups are dealt to the XCDs round-robin
 // (workgroup w runs on XCD w % kXcds), and each XCD has its own L2.  Tiles
 // that share operand panels -- the tiles of one problem and one reduction
@@ -170,6 +190,7 @@ void GemmBatch::finalize(Context* ctx, bool complex) {
       const int c = mn < 64 ? 0 : mn < 128 ? 1 : mn < 256 ? 2 : mn < 512 ? 3 : 4;
       cls[c] += double(p.p.M) * p.p.N * double(k);
     }
+    small_ = all > 0 && cls[0] / all > 0.5;
     uint64_t key = 0;
     for (int c = 0; c < 5; ++c) key = key * 9 + (all > 0 ? static_cast<uint64_t>(8.0 * cls[c] / all + 0.5) : 0);
     int lg = 0;
@@ -228,8 +249,52 @@ void GemmBatch::finalize(Context* ctx, bool complex) {
   if (!hs_.empty()) ctx->upload(d_segs_.get(), hs_.data(), hs_.size() * sizeof(GemmSeg));
 }
 
+void GemmBatch::planar_from(const GemmBatch& cx, const int64_t (&n4)[kMaxArenas]) {
+  require(cx.finalized_ && cx.complex_, "GemmBatch::planar_from: needs a finalized complex table");
+  require(!finalized_ && probs_.empty(), "GemmBatch::planar_from: target must be empty");
+  const int64_t mask = (int64_t(1) << kArenaShift) - 1;
+  auto opnd = [&](int64_t aoff, int64_t ld, bool trans, bool im) {
+    Operand o;
+    o.arena = static_cast<int>(static_cast<uint64_t>(aoff) >> kArenaShift);
+    o.off = (aoff & mask) + (im ? n4[o.arena] : 0);
+    o.ld = ld;
+    o.trans = trans;
+    o.conj = false;
+    return o;
+  };
+  for (const GemmProb& p : cx.hp_) {
+    const int ca = static_cast<int>(static_cast<uint64_t>(p.c_off) >> kArenaShift);
+    const int64_t co = p.c_off & mask;
+    const int pr = add_problem(ca, co, p.ldc, p.M, p.N, p.accumulate != 0);
+    const int pi = add_problem(ca, n4[ca] + co, p.ldc, p.M, p.N, p.accumulate != 0);
+    for (int si = p.seg_begin; si < p.seg_end; ++si) {
+      const GemmSeg& g = cx.hs_[si];
+      const bool ta = (g.flags & kTransA) != 0, tb = (g.flags & kTransB) != 0;
+      const double sa = (g.flags & kConjA) ? -1.0 : 1.0, sb = (g.flags & kConjB) ? -1.0 : 1.0;
+      const Operand Ar = opnd(g.a_off, g.lda, ta, false), Ai = opnd(g.a_off, g.lda, ta, true);
+      const Operand Br = opnd(g.b_off, g.ldb, tb, false), Bi = opnd(g.b_off, g.ldb, tb, true);
+      // s A B with A = Ar + i sa Ai, B = Br + i sb Bi, s = sr + i sm:
+      //   re = sr (Ar Br - sa sb Ai Bi) - sm (sa Ai Br + sb Ar Bi)
+      //   im = sr (sa Ai Br + sb Ar Bi) + sm (Ar Br - sa sb Ai Bi)
+      const double sr = g.scale, sm = g.scale_im;
+      if (sr != 0.0) {
+        add_seg(pr, Ar, Br, g.K, sr);
+        add_seg(pr, Ai, Bi, g.K, -sr * sa * sb);
+        add_seg(pi, Ai, Br, g.K, sr * sa);
+        add_seg(pi, Ar, Bi, g.K, sr * sb);
+      }
+      if (sm != 0.0) {
+        add_seg(pr, Ai, Br, g.K, -sm * sa);
+        add_seg(pr, Ar, Bi, g.K, -sm * sb);
+        add_seg(pi, Ar, Br, g.K, sm);
+        add_seg(pi, Ai, Bi, g.K, -sm * sa * sb);
+      }
+    }
+  }
+}
+
 int GemmBatch::default_cfg(GemmPrec prec) const {
-  if (complex_) return 0;
+  if (complex_) return small_ ? 3 : 0;
   // FP64 parts take about twice as long per K step as FP32 ones, so FP64
   // tables balance best with the finer split variant
   const int variant = prec == GemmPrec::F64 ? kTileConfigs : 0;
@@ -240,7 +305,7 @@ int GemmBatch::default_cfg(GemmPrec prec) const {
 GemmBatch::TileSet& GemmBatch::tileset(Context* ctx, int cfg) {
   require(finalized_, "GemmBatch: run before finalize");
   require(cfg >= 0 && cfg < kPlans, "GemmBatch: bad launch plan");
-  require(!(complex_ && cfg != 0), "GemmBatch: complex tables use tile configuration 0");
+  require(!complex_ || complex_tile_cfg(cfg % kTileConfigs), "GemmBatch: complex tables use tile configurations 0 and 3");
   TileSet& ts = sets_[cfg];
   if (ts.built) return ts;
   ts.built = true;
@@ -362,7 +427,7 @@ void GemmBatch::run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& a
   if (cfg < 0) cfg = default_cfg(prec);
   require(cfg >= 0 && cfg < 2 * kPlans, "GemmBatch: bad launch plan");
   cfg %= kTileConfigs;  // no split reductions or persistence: the tile configuration only
-  require(!(complex_ && cfg != 0), "GemmBatch: complex tables use tile configuration 0");
+  require(!complex_ || complex_tile_cfg(cfg), "GemmBatch: complex tables use tile configurations 0 and 3");
   const bool zprec = prec == GemmPrec::Z64 || prec == GemmPrec::Z32 || prec == GemmPrec::Z32_OUT64;
   require(zprec == complex_, "GemmBatch: precision and table disagree on complex data");
   ProbTiles& pt = pp_[cfg];

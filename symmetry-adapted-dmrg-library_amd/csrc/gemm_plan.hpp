@@ -39,6 +39,14 @@ class GemmBatch {
   // Algorithmic flops (real multiply-adds x2; complex x4) of the table.
   double flops(bool cplx) const;
 
+  // The real table of a finalized complex table on planar operands: an
+  // arena of n complex elements becomes two real planes, re at the complex
+  // offsets and im at n4[arena] plus them (n4 >= n, a multiple of 4).  Each
+  // complex problem becomes two real ones (re and im planes of C), each
+  // complex segment s * op(A) op(B) (conjugations and a complex scale
+  // included) four real segments, eight when the scale is complex.  This
+  // batch must be empty; it is left unfinalized.
+  void planar_from(const GemmBatch& cx, const int64_t (&n4)[kMaxArenas]);
   // Appends every problem and segment of a finalized batch, each element
   // offset moved by shift[arena] (independent tables merged into one launch).
   void absorb(const GemmBatch& o, const int64_t (&shift)[kMaxArenas]);
@@ -50,7 +58,8 @@ class GemmBatch {
   // vectors.
   void request_vec() { vec_req_ = true; }
   bool vec() const { return vec_; }
-  // complex: complex tables (configuration 0 only, no split reductions).
+  // complex: complex tables (tile configurations 0 and 3, no split
+  // reductions).
   void finalize(Context* ctx, bool complex);
   // cfg: a launch plan id (tile configuration + split variant, see kPlans);
   // cfg < 0: the default configuration for prec (default_cfg).
@@ -63,6 +72,8 @@ class GemmBatch {
   // >= 256 x 256 (128 x 128 for FP32 operands, 128 x 64 for FP64, whose wider
   // accumulators would otherwise cost occupancy).  Split variant: FP64 the
   // finer one (about 8 parts per workgroup slot), otherwise about 3.
+  // Complex tables: 32 x 32 tiles when most of the work is in problems with
+  // min(M, N) < 64, else 64 x 64.
   int default_cfg(GemmPrec prec) const;
   int big_tiles() const { return big_; }
   // Coarse signature of the problem shapes (work fractions per size class of
@@ -70,6 +81,11 @@ class GemmBatch {
   // one tuned tile configuration serves.
   uint64_t shape_key() const { return key_; }
   bool complex() const { return complex_; }
+  // MFMA flops a launch plan issues (finalized tables): each output tile
+  // computes its whole TM x TN block over every segment of its problem, K
+  // rounded up to the precision's K step.  flops() / issued_flops() is the
+  // share of issued MFMA work that is not edge-tile padding.
+  double issued_flops(int cfg, GemmPrec prec) const;
   // Tiles / split output tiles of a configuration (builds it if needed).
   int ntiles(Context* ctx, int cfg);
   int nsplit_tiles(Context* ctx, int cfg);
@@ -132,6 +148,7 @@ class GemmBatch {
   TileSet sets_[kPlans];
   bool complex_ = false;
   bool big_ = false;
+  bool small_ = false;  // most work in problems with min(M, N) < 64
   bool vec_req_ = false, vec_ = false;
   uint64_t key_ = 0;
   bool finalized_ = false;

@@ -396,8 +396,24 @@ void autotune_clear() {
 }
 
 // Tile configuration for one stage: tuned (cached per key) or the default.
+namespace {
+std::atomic<int> g_forced_plan{-1};
+}  // namespace
+
+void set_heff_launch_plan(int plan) {
+  require(plan >= -1 && plan < 2 * GemmBatch::kPlans, "H_eff launch plan out of range");
+  g_forced_plan = plan;
+}
+int heff_launch_plan() { return g_forced_plan; }
+
 int HeffPlan::cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& at) {
-  if (g.complex() || !autotune()) return g.default_cfg(prec);
+  const int forced = g_forced_plan;
+  if (forced >= 0) {
+    require(!g.complex() || complex_tile_cfg(forced % GemmBatch::kPlans % kTileConfigs),
+            "H_eff launch plan: complex tables have tile configurations 0 and 3 only");
+    return forced;
+  }
+  if (!autotune()) return g.default_cfg(prec);
   const TuneKey key{static_cast<int>(prec), stage, g.shape_key()};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -414,6 +430,9 @@ int HeffPlan::cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& 
   for (int cfg = 0; cfg < nplans; ++cfg) {
     // split variants only differ for tables that split their reductions
     const int plan = cfg % GemmBatch::kPlans;
+    // complex tables: no split reductions or persistent launches, two tile
+    // configurations
+    if (g.complex() && (cfg != plan || plan >= kTileConfigs || !complex_tile_cfg(plan))) continue;
     if (plan >= kTileConfigs && g.nsplit_tiles(ctx_, plan % kTileConfigs) == 0) continue;
     g.run(ctx_, prec, at, cfg);  // builds the tile set and warms caches
     SADMRG_HIP(hipEventRecord(e0, ctx_->stream()));
@@ -440,15 +459,109 @@ double HeffPlan::stage_flops(int stage) const {
   return g1_.flops(cplx_) + g2_.flops(cplx_);
 }
 
+double HeffPlan::issued_flops(int stage, Prec prec) const {
+  const bool planar = cplx_ && planar_built_ && heff_complex_planar();
+  const GemmBatch& a = planar ? g1p_ : g1_;
+  const GemmBatch& b = planar ? g2p_ : g2_;
+  GemmPrec p1, p2;
+  if (prec == Prec::FP64) {
+    p1 = p2 = (cplx_ && !planar) ? GemmPrec::Z64 : GemmPrec::F64;
+  } else if (prec == Prec::FP32) {
+    p1 = GemmPrec::F32;
+    p2 = GemmPrec::F32_OUT64;
+  } else {
+    p1 = GemmPrec::BF16;
+    p2 = GemmPrec::BF16_OUT64;
+  }
+  double f = 0;
+  if (stage != 2) f += a.issued_flops(stats_.tile_cfg1, p1);
+  if (stage != 1) f += b.issued_flops(stats_.tile_cfg2, p2);
+  return f;
+}
+
 namespace {
 std::atomic<bool> g_per_block{false};
+std::atomic<bool> g_complex_planar{false};
+int64_t round4(int64_t n) { return (n + 3) / 4 * 4; }
 }  // namespace
 
 void set_heff_per_block(bool on) { g_per_block = on; }
 bool heff_per_block() { return g_per_block; }
+void set_heff_complex_planar(bool on) { g_complex_planar = on; }
+bool heff_complex_planar() { return g_complex_planar; }
+
+void HeffPlan::build_planar() {
+  planar_built_ = true;
+  for (auto& x : n4_) x = 0;
+  n4_[kArIn] = round4(in_elems_);
+  n4_[kArOut] = round4(n_);
+  n4_[kArScratch] = round4(scratch_elems_);
+  n4_[kArPack] = round4(pack_elems_);
+  for (GemmBatch* g : {&g1p_, &g2p_}) {
+    g->planar_from(g == &g1p_ ? g1_ : g2_, n4_);
+    g->request_vec();
+    for (int a : {kArIn, kArOut, kArScratch, kArPack}) g->limit(a, 2 * n4_[a]);
+    g->finalize(ctx_, false);
+  }
+  op64_ = DevBuf(ctx_, std::max<int64_t>(2 * n4_[kArOut], 1) * 8);
+  zero_bytes(ctx_, op64_.get(), op64_.bytes());
+}
+
+// Complex H_eff on planar operands: the input is repacked (interleaved, the
+// padded layout of the complex tables) and split into planes, both stages
+// run as real tables, and the planar output is interleaved into d_out.
+void HeffPlan::apply_planar(const double* d_in, double* d_out, Prec prec, int stage, bool unbatched) {
+  require(prec == Prec::FP64 || prec == Prec::FP32, "H_eff: the BF16 path is real-only");
+  if (!planar_built_) build_planar();
+  const bool f32 = prec == Prec::FP32;
+  const int64_t np = 2 * n4_[kArPack], ns = 2 * n4_[kArScratch], ni = 2 * n4_[kArIn];
+  DevBuf& pp = f32 ? pp32_ : pp64_;
+  DevBuf& sp = f32 ? sp32_ : sp64_;
+  DevBuf& ip = f32 ? ip32_ : ip64_;
+  const size_t es = f32 ? 4 : 8;
+  if (!pp.get()) {
+    pp = DevBuf(ctx_, std::max<int64_t>(np, 1) * es);
+    sp = DevBuf(ctx_, std::max<int64_t>(ns, 1) * es);
+    ip = DevBuf(ctx_, std::max<int64_t>(ni, 1) * es);
+    zero_bytes(ctx_, sp.get(), sp.bytes());  // padding rows stay zero
+    SADMRG_HIP(launch_deinterleave(pack_elems_, n4_[kArPack], pack64_.as<double>(), pp.get(), f32 ? 1 : 0,
+                                   ctx_->stream()));
+  }
+  if (stage != 2) {
+    repack_input(d_in, in64_.get(), 0);
+    SADMRG_HIP(launch_deinterleave(in_elems_, n4_[kArIn], in64_.as<double>(), ip.get(), f32 ? 1 : 0,
+                                   ctx_->stream()));
+  }
+  ArenaTable at;
+  std::memset(&at, 0, sizeof(at));
+  at.base[kArIn] = ip.get();
+  at.base[kArOut] = op64_.get();
+  at.base[kArScratch] = sp.get();
+  at.base[kArPack] = pp.get();
+  auto exec = [&](GemmBatch& g, GemmPrec p, int cfg) {
+    if (unbatched) g.run_per_problem(ctx_, p, at, cfg);
+    else g.run(ctx_, p, at, cfg);
+  };
+  if (stage != 2) {
+    const GemmPrec p = f32 ? GemmPrec::F32 : GemmPrec::F64;
+    stats_.tile_cfg1 = cfg_for(g1p_, p, 1, at);
+    exec(g1p_, p, stats_.tile_cfg1);
+  }
+  if (stage != 1) {
+    const GemmPrec p = f32 ? GemmPrec::F32_OUT64 : GemmPrec::F64;
+    stats_.tile_cfg2 = cfg_for(g2p_, p, 2, at);
+    exec(g2p_, p, stats_.tile_cfg2);
+    SADMRG_HIP(launch_interleave(n_, n4_[kArOut], op64_.as<double>(), d_out, ctx_->stream()));
+  }
+  if (stage == 0) apply_projectors(d_in, d_out);
+}
 
 void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bool unbatched) {
   unbatched = unbatched || g_per_block;
+  if (cplx_ && g_complex_planar) {
+    apply_planar(d_in, d_out, prec, stage, unbatched);
+    return;
+  }
   auto exec = [&](GemmBatch& g, GemmPrec p, const ArenaTable& at, int cfg) {
     if (unbatched) g.run_per_problem(ctx_, p, at, cfg);
     else g.run(ctx_, p, at, cfg);
@@ -470,12 +583,12 @@ void HeffPlan::apply(const double* d_in, double* d_out, Prec prec, int stage, bo
     at.base[kArScratch] = scratch32_.get();
     at.base[kArPack] = pack32_.get();
     if (stage != 2) {
-      stats_.tile_cfg1 = 0;
-      exec(g1_, GemmPrec::Z32, at, 0);
+      stats_.tile_cfg1 = cfg_for(g1_, GemmPrec::Z32, 1, at);
+      exec(g1_, GemmPrec::Z32, at, stats_.tile_cfg1);
     }
     if (stage != 1) {
-      stats_.tile_cfg2 = 0;
-      exec(g2_, GemmPrec::Z32_OUT64, at, 0);
+      stats_.tile_cfg2 = cfg_for(g2_, GemmPrec::Z32_OUT64, 2, at);
+      exec(g2_, GemmPrec::Z32_OUT64, at, stats_.tile_cfg2);
     }
     if (stage == 0) apply_projectors(d_in, d_out);
     return;

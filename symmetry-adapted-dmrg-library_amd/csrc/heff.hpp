@@ -49,6 +49,22 @@ bool autotune();
 int autotune_entries();
 void autotune_clear();
 
+// Process-wide: every H_eff stage runs launch plan `plan` (a GemmBatch plan
+// id, persistent ids included) instead of the tuned or default one; -1
+// (default) restores tuning.  For tests and measurements of single plans.
+void set_heff_launch_plan(int plan);
+int heff_launch_plan();
+
+// Process-wide: complex plans run as real tables on planar operands
+// (GemmBatch::planar_from; the real kernel's 16-byte staging, split
+// reductions, persistent launches and autotuner) instead of the complex
+// kernel on interleaved data.  Default off: each complex product becomes
+// four real ones that load every operand panel twice, and on small blocks
+// the complex kernel, which loads each once for its four MFMA products, is
+// faster.
+void set_heff_complex_planar(bool on);
+bool heff_complex_planar();
+
 // Process-wide: every H_eff application (Lanczos included) runs one launch
 // per output block of each stage instead of the grouped launch (the per-block
 // baseline, as the reference issues one GEMM per block).  Default off.
@@ -88,6 +104,10 @@ class HeffPlan {
   // entries); a plain plan: {0, n}
   std::vector<int64_t> part_offsets() const { return merged_ ? part_off_ : std::vector<int64_t>{0, n_}; }
   double stage_flops(int stage) const;
+  // MFMA flops issued by the launch plans of the last application (stage 1
+  // or 2, 0: both) for operands of precision prec: edge tiles computed whole,
+  // K rounded up to the K step (GemmBatch::issued_flops).
+  double issued_flops(int stage, Prec prec) const;
   const HeffStats& stats() const { return stats_; }
   Context* ctx() const { return ctx_; }
   const BTensor& layout() const { return layout_; }
@@ -118,6 +138,14 @@ class HeffPlan {
   void repack_input(const double* d_in, void* dst, int out_type);
   GemmBatch g1_, g2_;
   int cfg_for(GemmBatch& g, GemmPrec prec, int stage, const ArenaTable& at);
+  // complex plans on planar operands: the real tables, the planes' offsets
+  // (n4 per arena) and planar copies of the pack, input, scratch and output
+  GemmBatch g1p_, g2p_;
+  bool planar_built_ = false;
+  int64_t n4_[kMaxArenas] = {};
+  DevBuf pp64_, pp32_, ip64_, ip32_, sp64_, sp32_, op64_;
+  void build_planar();
+  void apply_planar(const double* d_in, double* d_out, Prec prec, int stage, bool unbatched);
   HeffStats stats_;
   std::vector<uint8_t> out_null_;
   // projector columns: phi_0..phi_{P-1}, then weight_j * phi_j (n elements each)

@@ -782,45 +782,57 @@ static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <int TM, int TN>
+static hipError_t launch_complex(GemmPrec prec, const GemmLaunch& L, hipStream_t st) {
+  if (prec == GemmPrec::Z64)
+    hipLaunchKernelGGL((grouped_zgemm_kernel<double, double, TM, TN, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st,
+                       L.tiles, L.probs, L.segs, L.arenas, 0);
+  else if (prec == GemmPrec::Z32)
+    hipLaunchKernelGGL((grouped_zgemm_kernel<float, float, TM, TN, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st,
+                       L.tiles, L.probs, L.segs, L.arenas, 0);
+  else
+    hipLaunchKernelGGL((grouped_zgemm_kernel<float, double, TM, TN, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st,
+                       L.tiles, L.probs, L.segs, L.arenas, 0);
+  return hipGetLastError();
+}
+
 hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L, hipStream_t st) {
   if (L.ntiles <= 0) return hipSuccess;
   if (L.nreduce > 0 && (L.partial == nullptr || L.reduce == nullptr || L.counters == nullptr))
     return hipErrorInvalidValue;
   switch (prec) {
     case GemmPrec::F64:
+      if (tile_cfg == 3) return launch_real<double, double, 32, 32, 16>(L, st);
       if (tile_cfg == 1) return launch_real<double, double, 128, 128, 16, 1>(L, st);
       if (tile_cfg == 2) return launch_real<double, double, 128, 64, 16>(L, st);
       return launch_real<double, double, 64, 64, 16>(L, st);
     case GemmPrec::F32:
+      if (tile_cfg == 3) return launch_real<float, float, 32, 32, 32>(L, st);
       if (tile_cfg == 1) return launch_real<float, float, 128, 128, 32, 2, 1>(L, st);
       if (tile_cfg == 2) return launch_real<float, float, 128, 64, 32>(L, st);
       return launch_real<float, float, 64, 64, 32>(L, st);
     case GemmPrec::F32_OUT64:
+      if (tile_cfg == 3) return launch_real<float, double, 32, 32, 32>(L, st);
       if (tile_cfg == 1) return launch_real<float, double, 128, 128, 32, 2, 1>(L, st);
       if (tile_cfg == 2) return launch_real<float, double, 128, 64, 32>(L, st);
       return launch_real<float, double, 64, 64, 32>(L, st);
     case GemmPrec::BF16:
+      if (tile_cfg == 3) return launch_real<bf16_t, bf16_t, 32, 32, 64, 2, 1>(L, st);
       if (tile_cfg == 1) return launch_real<bf16_t, bf16_t, 128, 128, 64, 1, 1>(L, st);
       if (tile_cfg == 2) return launch_real<bf16_t, bf16_t, 128, 64, 64, 2, 1>(L, st);
       return launch_real<bf16_t, bf16_t, 64, 64, 64, 2, 1>(L, st);
     case GemmPrec::BF16_OUT64:
+      if (tile_cfg == 3) return launch_real<bf16_t, double, 32, 32, 64, 2, 1>(L, st);
       if (tile_cfg == 1) return launch_real<bf16_t, double, 128, 128, 64, 1, 1>(L, st);
       if (tile_cfg == 2) return launch_real<bf16_t, double, 128, 64, 64, 2, 1>(L, st);
       return launch_real<bf16_t, double, 64, 64, 64, 2, 1>(L, st);
     case GemmPrec::Z64:
     case GemmPrec::Z32:
     case GemmPrec::Z32_OUT64: {
-      if (L.nreduce > 0 || tile_cfg != 0) return hipErrorInvalidValue;  // no split or large tiles for complex
-      if (prec == GemmPrec::Z64)
-        hipLaunchKernelGGL((grouped_zgemm_kernel<double, double, 64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0,
-                           st, L.tiles, L.probs, L.segs, L.arenas, 0);
-      else if (prec == GemmPrec::Z32)
-        hipLaunchKernelGGL((grouped_zgemm_kernel<float, float, 64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0,
-                           st, L.tiles, L.probs, L.segs, L.arenas, 0);
-      else
-        hipLaunchKernelGGL((grouped_zgemm_kernel<float, double, 64, 64, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0,
-                           st, L.tiles, L.probs, L.segs, L.arenas, 0);
-      return hipGetLastError();
+      // no split reductions or large tiles for complex tables
+      if (L.nreduce > 0 || !complex_tile_cfg(tile_cfg)) return hipErrorInvalidValue;
+      if (tile_cfg == 3) return launch_complex<32, 32>(prec, L, st);
+      return launch_complex<64, 64>(prec, L, st);
     }
   }
   return hipErrorInvalidValue;
@@ -1146,6 +1158,25 @@ static int grid_for(int64_t n, int block = 256) {
   return static_cast<int>(g);
 }
 
+// ---- planar complex layout ------------------------------------------------
+template <class TOut>
+__global__ void deinterleave_kernel(int64_t n, int64_t n4, const double* __restrict__ src, TOut* __restrict__ dst) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n4; i += stride) {
+    const bool in = i < n;
+    dst[i] = static_cast<TOut>(in ? src[2 * i] : 0.0);
+    dst[n4 + i] = static_cast<TOut>(in ? src[2 * i + 1] : 0.0);
+  }
+}
+
+__global__ void interleave_kernel(int64_t n, int64_t n4, const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += stride) {
+    dst[2 * i] = src[i];
+    dst[2 * i + 1] = src[n4 + i];
+  }
+}
+
 // ---- segmented vector kernels (batched solves) ----------------------------
 // One workgroup per SegBlock; the columns are walked in order and each
 // block's partial sums are reduced in a fixed order, so results do not
@@ -1231,6 +1262,25 @@ __global__ __launch_bounds__(256) void seg_combine_kernel(const double* __restri
       y[2 * i + 1] = ai;
     }
   }
+}
+
+hipError_t launch_deinterleave(int64_t n, int64_t n4, const double* src, void* dst, int out_type, hipStream_t st) {
+  if (n4 <= 0) return hipSuccess;
+  if (n > n4) return hipErrorInvalidValue;
+  if (out_type == 0)
+    hipLaunchKernelGGL(deinterleave_kernel<double>, dim3(grid_for(n4)), dim3(256), 0, st, n, n4, src,
+                       static_cast<double*>(dst));
+  else
+    hipLaunchKernelGGL(deinterleave_kernel<float>, dim3(grid_for(n4)), dim3(256), 0, st, n, n4, src,
+                       static_cast<float*>(dst));
+  return hipGetLastError();
+}
+
+hipError_t launch_interleave(int64_t n, int64_t n4, const double* src, double* dst, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (n > n4) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(interleave_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, n4, src, dst);
+  return hipGetLastError();
 }
 
 hipError_t launch_seg_dot(const double* X, int64_t ldx, int ncol, const double* y, const SegBlock* blocks, int nblocks,

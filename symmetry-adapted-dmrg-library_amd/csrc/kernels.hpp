@@ -44,15 +44,17 @@ struct GemmLaunch {
 // CUs the persistent grid is sized for (MI355X)
 constexpr int kPersistCUs = 256;
 
-// Output tile per configuration: 0: 64x64, 1: 128x128, 2: 128x64 (K steps of
-// 16 for FP64, 32 for FP32, 64 for BF16 operands).  Complex tables use
-// configuration 0.
+// Output tile per configuration: 0: 64x64, 1: 128x128, 2: 128x64, 3: 32x32
+// (K steps of 16 for FP64, 32 for FP32, 64 for BF16 operands).  Complex
+// tables use configurations 0 and 3 (K step 16).
 // One launch, split reductions included.
-constexpr int kTileConfigs = 3;
+constexpr int kTileConfigs = 4;
 hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& launch, hipStream_t st);
 
-inline int tile_m(int tile_cfg) { return tile_cfg == 0 ? 64 : 128; }
-inline int tile_n(int tile_cfg) { return tile_cfg == 1 ? 128 : 64; }
+inline int tile_m(int tile_cfg) { return tile_cfg == 0 ? 64 : tile_cfg == 3 ? 32 : 128; }
+inline int tile_n(int tile_cfg) { return tile_cfg == 1 ? 128 : tile_cfg == 3 ? 32 : 64; }
+// tile configurations the complex kernel has
+inline bool complex_tile_cfg(int tile_cfg) { return tile_cfg == 0 || tile_cfg == 3; }
 
 // out[c] = sign * sum_i conj(X[i,c]) * y[i] for c < ncol (complex pairs, 2*ncol doubles).
 // `partial` must hold 2 * 1024 * ncol doubles.
@@ -130,6 +132,14 @@ hipError_t launch_seg_reduce(const double* partial, const int* first, int ninst,
 // ncol == 0.
 hipError_t launch_seg_combine(const double* X, int64_t ldx, int ncol, const double* coef, const double* beta,
                               double* y, const SegBlock* blocks, int nblocks, int cplx, hipStream_t st);
+
+// Planar complex layout (complex tables lowered onto the real grouped GEMM):
+// n complex elements as two real planes, re at [0, n) and im at [n4, n4 + n),
+// n4 = n rounded up to a multiple of 4, the tails zero.
+// deinterleave: interleaved complex (double) -> planar (out_type 0: double,
+// 1: float); interleave: planar double -> interleaved complex.
+hipError_t launch_deinterleave(int64_t n, int64_t n4, const double* src, void* dst, int out_type, hipStream_t st);
+hipError_t launch_interleave(int64_t n, int64_t n4, const double* src, double* dst, hipStream_t st);
 
 struct ScatterTask {
   int64_t src_off, dst_off, lds, ldd;

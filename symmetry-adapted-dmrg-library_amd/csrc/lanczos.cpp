@@ -567,8 +567,10 @@ BatchedLanczosOut lanczos_batched(HeffPlan& H, const std::vector<int64_t>& seg, 
   double* hv = hvb.as<double>();
   auto col = [&](int j) { return V + static_cast<int64_t>(j) * nw; };
   const Prec bulk = opt.prec;
-  const bool mixed = bulk != Prec::FP64 && opt.refine;
-  const Prec check_prec = mixed ? Prec::FP64 : bulk;
+  // refinement restarts (FP64 residuals, bulk-precision Krylov steps) in
+  // either precision; FP32 bulk checks in FP64
+  const bool mixed = opt.refine;
+  const Prec check_prec = opt.refine ? Prec::FP64 : bulk;
   auto matvec = [&](const double* x, double* y, Prec p) {
     H.apply(x, y, p);
     if (p == Prec::FP64) out.matvec64++;

@@ -84,11 +84,11 @@ LanczosOut lanczos_solve(HeffPlan& H, double* d_vec, const LanczosOpts& opt);
 // min(mlanc, n_i + 1), breakdown stop and convergence test (after every
 // check_every-th step, |<v,Hv>| / |Hv| within 1e-8 of 1, the check matvec
 // shared by the instances checking at that step).  Instances that finish
-// stop updating; the merged matvec still covers them.  Mixed precision
-// (prec FP32 with refine): FP32 Krylov matvecs, FP64 checks, then FP64
-// residual-correction restarts of every instance whose |r| exceeds
-// refine_tol |E| (batched over those instances); energies are FP64
-// Rayleigh quotients.  seg: instance offsets in elements (n_inst + 1
+// stop updating; the merged matvec still covers them.  With refine: FP64
+// checks, then FP64 residual-correction restarts of every instance whose
+// |r| exceeds refine_tol |E| (batched over those instances, Krylov steps in
+// the bulk precision: FP32 for the mixed path, FP64 for a refined FP64
+// reference); energies are FP64 Rayleigh quotients.  seg: instance offsets in elements (n_inst + 1
 // entries, seg[0] = 0, seg.back() = H.n()).  Host-driven steps; mlanc <= 64.
 struct BatchedLanczosOut {
   std::vector<double> energy, residual64;

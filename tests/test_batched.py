@@ -65,9 +65,11 @@ def _flat(t):
 @pytest.mark.gpu
 def test_merged_plan_equals_individual_plans(ctx, sad):
     """One merged launch per stage gives, instance by instance, the output of
-    that instance's own plan bit for bit (FP64, complex: every output tile is
-    reduced by one workgroup in segment order either way); the complex64
-    operand path stays within 1e-5 of FP64 and differs from it."""
+    that instance's own plan: bit for bit on the complex kernel (every output
+    tile is reduced by one workgroup in segment order either way), to 1e-13
+    on the planar real tables (the merged table may split long reductions
+    differently); the complex64 operand path stays within 1e-5 of FP64 and
+    differs from it."""
     wls = _instances(6, 64)
     plans, psis = _plans(ctx, sad, wls)
     merged = sad.Heff.merge(plans)
@@ -77,10 +79,14 @@ def test_merged_plan_equals_individual_plans(ctx, sad):
     merged.apply_device(vin.device_ptr(), out.device_ptr(), sad.FP64)
     got = _flat(out)
     off = 0
+    planar = sad.heff_complex_planar()
     for plan, psi, wl in zip(plans, psis, wls):
         ref = tl.from_dev(plan.apply(psi, sad.FP64))["data"]
         k = ref.size
-        assert np.array_equal(got[off:off + k], ref)
+        if planar:
+            assert np.max(np.abs(got[off:off + k] - ref)) <= 1e-13 * np.max(np.abs(ref))
+        else:
+            assert np.array_equal(got[off:off + k], ref)
         off += k
     out32 = vin.copy()
     merged.apply_device(vin.device_ptr(), out32.device_ptr(), sad.FP32)
@@ -227,3 +233,39 @@ def test_batched_lanczos_matches_single_solves(ctx, sad, precision):
         assert np.all(info["residual64"] <= 1e-9 * np.abs(e) * (1 + 1e-6)), info["residual64"]
         exact = np.asarray([np.linalg.eigvalsh(_dense(ctx, sad, p, psi))[0] for p, psi in zip(plans, psis)])
         assert np.all(np.abs(e - exact) <= 1e-11 * np.abs(exact)), (e - exact)
+
+
+@pytest.mark.gpu
+def test_complex_planar_matches_complex_kernel(ctx, sad):
+    """Complex plans lowered to real tables on planar operands (the default)
+    against the complex kernel on interleaved data, single and merged plans,
+    complex128 (1e-13) and complex64 operands (each within 1e-5 of complex128,
+    and the two complex64 paths within 2e-6 of each other); the interleaved
+    path stays bitwise reproducible across the switch."""
+    wls = _instances(4, 64, seed=5)
+    plans, psis = _plans(ctx, sad, wls)
+    merged = sad.Heff.merge(plans)
+    vin, n = _concat(ctx, sad, wls)
+    prev = sad.heff_complex_planar()
+    try:
+        res = {}
+        for planar in (False, True):
+            sad.heff_complex_planar(planar)
+            for prec in (sad.FP64, sad.FP32):
+                out = vin.copy()
+                merged.apply_device(vin.device_ptr(), out.device_ptr(), prec)
+                single = np.concatenate([tl.from_dev(p.apply(a, prec))["data"] for p, a in zip(plans, psis)])
+                res[(planar, prec)] = (_flat(out), single)
+        ref = res[(False, sad.FP64)][1]
+        scale = np.max(np.abs(ref))
+        for planar in (False, True):
+            for k in (0, 1):
+                assert np.max(np.abs(res[(planar, sad.FP64)][k] - ref)) <= 1e-13 * scale
+                assert np.max(np.abs(res[(planar, sad.FP32)][k] - ref)) <= 1e-5 * scale
+        assert np.max(np.abs(res[(True, sad.FP32)][0] - res[(False, sad.FP32)][0])) <= 2e-6 * scale
+        sad.heff_complex_planar(False)
+        out = vin.copy()
+        merged.apply_device(vin.device_ptr(), out.device_ptr(), sad.FP64)
+        assert np.array_equal(_flat(out), res[(False, sad.FP64)][0])
+    finally:
+        sad.heff_complex_planar(prev)
