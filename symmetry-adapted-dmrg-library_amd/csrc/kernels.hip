@@ -620,18 +620,29 @@ __global__ __launch_bounds__(256, WPE) void grouped_gemm_kernel(const GemmTile* 
 
 // Complex grouped GEMM on interleaved (re, im) data, FP64 or FP32 operands
 // (FP32 accumulation, FP32 or FP64 output).  Four real MFMA products per
-// step; conjugation flags select the signs.
+// step; conjugation flags and the segment scale are applied when a K step is
+// written to LDS.  One K step is held in registers ahead of the MFMAs: the
+// loads of step s+1 (the next K block, or the first of the next segment) are
+// issued before the products of step s, so their latency overlaps the
+// MFMAs.  Each complex element is one 8- or 16-byte load.
+template <class T> struct Cplx;
+template <> struct Cplx<float> { typedef float2 type; };
+template <> struct Cplx<double> { typedef double2 type; };
+
 template <class T, class TOut, int TM, int TN, int TK, int WM, int WN>
-__global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __restrict__ tiles,
-                                                            const GemmProb* __restrict__ probs,
-                                                            const GemmSeg* __restrict__ segs,
-                                                            ArenaTable arenas, int tile_base) {
+__global__ __launch_bounds__(64 * WM * WN) void grouped_zgemm_kernel(const GemmTile* __restrict__ tiles,
+                                                                     const GemmProb* __restrict__ probs,
+                                                                     const GemmSeg* __restrict__ segs,
+                                                                     ArenaTable arenas, int tile_base) {
+  constexpr int NT = 64 * WM * WN;  // threads: WM x WN waves
   constexpr int FM = TM / WM / 16;
   constexpr int FN = TN / WN / 16;
   constexpr int PAD = 2;
-  constexpr int A_ELEMS = TM * TK / 256;
-  constexpr int B_ELEMS = TN * TK / 256;
+  constexpr int A_ELEMS = TM * TK / NT;
+  constexpr int B_ELEMS = TN * TK / NT;
+  static_assert(A_ELEMS >= 1 && B_ELEMS >= 1 && NT % TM == 0 && NT % TN == 0 && NT % TK == 0, "tile shape");
   typedef typename Acc4<T>::type acc_t;
+  typedef typename Cplx<T>::type V;
   __shared__ T sAr[TK][TM + PAD], sAi[TK][TM + PAD];
   __shared__ T sBr[TK][TN + PAD], sBi[TK][TN + PAD];
 
@@ -649,76 +660,112 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
 #pragma unroll
     for (int j = 0; j < FN; ++j) { accr[i][j] = acc_t{0, 0, 0, 0}; acci[i][j] = acc_t{0, 0, 0, 0}; }
 
-  for (int si = tile.seg_lo; si < tile.seg_hi; ++si) {
+  // loader: the segment and K block whose elements are in ra / rb
+  const V* A = nullptr;
+  const V* B = nullptr;
+  int64_t lda = 0, ldb = 0;
+  int K = 0, k0 = 0;
+  bool ta = false, tb = false;
+  T ca = 1, cb = 1, sr = 1, sim = 0;
+  int ls = tile.seg_lo;
+  auto enter = [&](int si) {
     const GemmSeg sg = segs[si];
-    const T* A = static_cast<const T*>(arena_ptr(arenas, sg.a_off, 2 * sizeof(T)));
-    const T* B = static_cast<const T*>(arena_ptr(arenas, sg.b_off, 2 * sizeof(T)));
-    const bool ta = sg.flags & kTransA, tb = sg.flags & kTransB;
-    const T ca = (sg.flags & kConjA) ? T(-1) : T(1);
-    const T cb = (sg.flags & kConjB) ? T(-1) : T(1);
-    const T sr = static_cast<T>(sg.scale), sim = static_cast<T>(sg.scale_im);
-    for (int k0 = 0; k0 < sg.K; k0 += TK) {
-      __syncthreads();
+    A = static_cast<const V*>(arena_ptr(arenas, sg.a_off, 2 * sizeof(T)));
+    B = static_cast<const V*>(arena_ptr(arenas, sg.b_off, 2 * sizeof(T)));
+    lda = sg.lda;
+    ldb = sg.ldb;
+    K = sg.K;
+    ta = (sg.flags & kTransA) != 0;
+    tb = (sg.flags & kTransB) != 0;
+    ca = (sg.flags & kConjA) ? T(-1) : T(1);
+    cb = (sg.flags & kConjB) ? T(-1) : T(1);
+    sr = static_cast<T>(sg.scale);
+    sim = static_cast<T>(sg.scale_im);
+    k0 = 0;
+  };
+  // next segment with K > 0 at or after si; false past the tile's segments
+  auto seek = [&](int si) {
+    while (si < tile.seg_hi && segs[si].K <= 0) ++si;
+    ls = si;
+    if (si >= tile.seg_hi) return false;
+    enter(si);
+    return true;
+  };
+  V ra[A_ELEMS], rb[B_ELEMS];
+  auto fetch = [&]() {
 #pragma unroll
-      for (int e = 0; e < A_ELEMS; ++e) {
-        int m, k;
-        const int idx = tid + e * 256;
-        if (!ta) { m = idx % TM; k = idx / TM; } else { k = idx % TK; m = idx / TK; }
-        const int gm = tile.m0 + m, gk = k0 + k;
-        T xr = 0, xi = 0;
-        if (gm < prob.M && gk < sg.K) {
-          const int64_t o = ta ? (gk + int64_t(gm) * sg.lda) : (gm + int64_t(gk) * sg.lda);
-          xr = A[2 * o];
-          xi = ca * A[2 * o + 1];
-        }
-        sAr[k][m] = xr * sr - xi * sim;
-        sAi[k][m] = xr * sim + xi * sr;
+    for (int e = 0; e < A_ELEMS; ++e) {
+      const int idx = tid + e * NT;
+      const int m = ta ? idx / TK : idx % TM, k = ta ? idx % TK : idx / TM;
+      const int gm = tile.m0 + m, gk = k0 + k;
+      const bool ok = gm < prob.M && gk < K;
+      const int64_t o = ta ? (gk + int64_t(gm) * lda) : (gm + int64_t(gk) * lda);
+      ra[e] = ok ? A[o] : V{0, 0};
+    }
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e) {
+      const int idx = tid + e * NT;
+      const int n = tb ? idx % TN : idx / TK, k = tb ? idx / TN : idx % TK;
+      const int gn = tile.n0 + n, gk = k0 + k;
+      const bool ok = gn < prob.N && gk < K;
+      const int64_t o = tb ? (gn + int64_t(gk) * ldb) : (gk + int64_t(gn) * ldb);
+      rb[e] = ok ? B[o] : V{0, 0};
+    }
+  };
+
+  bool have = seek(tile.seg_lo);
+  if (have) fetch();
+  while (have) {
+    __syncthreads();  // the previous step's products are done with LDS
+#pragma unroll
+    for (int e = 0; e < A_ELEMS; ++e) {
+      const int idx = tid + e * NT;
+      const int m = ta ? idx / TK : idx % TM, k = ta ? idx % TK : idx / TM;
+      const T xr = ra[e].x, xi = ca * ra[e].y;
+      sAr[k][m] = xr * sr - xi * sim;
+      sAi[k][m] = xr * sim + xi * sr;
+    }
+#pragma unroll
+    for (int e = 0; e < B_ELEMS; ++e) {
+      const int idx = tid + e * NT;
+      const int n = tb ? idx % TN : idx / TK, k = tb ? idx / TN : idx % TK;
+      sBr[k][n] = rb[e].x;
+      sBi[k][n] = cb * rb[e].y;
+    }
+    __syncthreads();
+    // issue the next step's loads before this step's products
+    k0 += TK;
+    have = k0 < K || seek(ls + 1);
+    if (have) fetch();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      T ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        ar[i] = sAr[kr][(wm * FM + i) * 16 + (lane & 15)];
+        ai[i] = sAi[kr][(wm * FM + i) * 16 + (lane & 15)];
+        nai[i] = -ai[i];
       }
 #pragma unroll
-      for (int e = 0; e < B_ELEMS; ++e) {
-        int n, k;
-        const int idx = tid + e * 256;
-        if (tb) { n = idx % TN; k = idx / TN; } else { k = idx % TK; n = idx / TK; }
-        const int gn = tile.n0 + n, gk = k0 + k;
-        T xr = 0, xi = 0;
-        if (gn < prob.N && gk < sg.K) {
-          const int64_t o = tb ? (gn + int64_t(gk) * sg.ldb) : (gk + int64_t(gn) * sg.ldb);
-          xr = B[2 * o];
-          xi = cb * B[2 * o + 1];
-        }
-        sBr[k][n] = xr;
-        sBi[k][n] = xi;
+      for (int j = 0; j < FN; ++j) {
+        br[j] = sBr[kr][(wn * FN + j) * 16 + (lane & 15)];
+        bi[j] = sBi[kr][(wn * FN + j) * 16 + (lane & 15)];
       }
-      __syncthreads();
 #pragma unroll
-      for (int kk = 0; kk < TK; kk += 4) {
-        const int kr = kk + (lane >> 4);
-        T ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
-          ar[i] = sAr[kr][(wm * FM + i) * 16 + (lane & 15)];
-          ai[i] = sAi[kr][(wm * FM + i) * 16 + (lane & 15)];
-          nai[i] = -ai[i];
-        }
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          br[j] = sBr[kr][(wn * FN + j) * 16 + (lane & 15)];
-          bi[j] = sBi[kr][(wn * FN + j) * 16 + (lane & 15)];
+          accr[i][j] = mfma_16x16x4(ar[i], br[j], accr[i][j]);
+          accr[i][j] = mfma_16x16x4(nai[i], bi[j], accr[i][j]);
+          acci[i][j] = mfma_16x16x4(ar[i], bi[j], acci[i][j]);
+          acci[i][j] = mfma_16x16x4(ai[i], br[j], acci[i][j]);
         }
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) {
-            accr[i][j] = mfma_16x16x4(ar[i], br[j], accr[i][j]);
-            accr[i][j] = mfma_16x16x4(nai[i], bi[j], accr[i][j]);
-            acci[i][j] = mfma_16x16x4(ar[i], bi[j], acci[i][j]);
-            acci[i][j] = mfma_16x16x4(ai[i], br[j], acci[i][j]);
-          }
-      }
     }
   }
 
-  TOut* C = static_cast<TOut*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, 2 * sizeof(TOut))));
+  typedef typename Cplx<TOut>::type VO;
+  VO* C = static_cast<VO*>(const_cast<void*>(arena_ptr(arenas, prob.c_off, 2 * sizeof(TOut))));
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -729,10 +776,14 @@ __global__ __launch_bounds__(256) void grouped_zgemm_kernel(const GemmTile* __re
       for (int r = 0; r < 4; ++r) {
         const int gm = tile.m0 + (wm * FM + i) * 16 + acc_row<T>(lane >> 4, r);
         if (gm >= prob.M) continue;
-        TOut* p = C + 2 * (gm + int64_t(gn) * prob.ldc);
-        const TOut vr = static_cast<TOut>(accr[i][j][r]), vi = static_cast<TOut>(acci[i][j][r]);
-        if (prob.accumulate) { p[0] += vr; p[1] += vi; }
-        else { p[0] = vr; p[1] = vi; }
+        VO* p = C + (gm + int64_t(gn) * prob.ldc);
+        VO v{static_cast<TOut>(accr[i][j][r]), static_cast<TOut>(acci[i][j][r])};
+        if (prob.accumulate) {
+          const VO old = *p;
+          v.x += old.x;
+          v.y += old.y;
+        }
+        *p = v;
       }
     }
 }
@@ -782,17 +833,19 @@ static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int TM, int TN>
+// W x W waves per workgroup (a 32 x 32 tile on one wave, W = 1, measured
+// slower: fewer resident waves for the same LDS)
+template <int TM, int TN, int W>
 static hipError_t launch_complex(GemmPrec prec, const GemmLaunch& L, hipStream_t st) {
   if (prec == GemmPrec::Z64)
-    hipLaunchKernelGGL((grouped_zgemm_kernel<double, double, TM, TN, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st,
-                       L.tiles, L.probs, L.segs, L.arenas, 0);
+    hipLaunchKernelGGL((grouped_zgemm_kernel<double, double, TM, TN, 16, W, W>), dim3(L.ntiles), dim3(64 * W * W), 0,
+                       st, L.tiles, L.probs, L.segs, L.arenas, 0);
   else if (prec == GemmPrec::Z32)
-    hipLaunchKernelGGL((grouped_zgemm_kernel<float, float, TM, TN, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st,
-                       L.tiles, L.probs, L.segs, L.arenas, 0);
+    hipLaunchKernelGGL((grouped_zgemm_kernel<float, float, TM, TN, 16, W, W>), dim3(L.ntiles), dim3(64 * W * W), 0,
+                       st, L.tiles, L.probs, L.segs, L.arenas, 0);
   else
-    hipLaunchKernelGGL((grouped_zgemm_kernel<float, double, TM, TN, 16, 2, 2>), dim3(L.ntiles), dim3(256), 0, st,
-                       L.tiles, L.probs, L.segs, L.arenas, 0);
+    hipLaunchKernelGGL((grouped_zgemm_kernel<float, double, TM, TN, 16, W, W>), dim3(L.ntiles), dim3(64 * W * W), 0,
+                       st, L.tiles, L.probs, L.segs, L.arenas, 0);
   return hipGetLastError();
 }
 
@@ -831,8 +884,8 @@ hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L,
     case GemmPrec::Z32_OUT64: {
       // no split reductions or large tiles for complex tables
       if (L.nreduce > 0 || !complex_tile_cfg(tile_cfg)) return hipErrorInvalidValue;
-      if (tile_cfg == 3) return launch_complex<32, 32>(prec, L, st);
-      return launch_complex<64, 64>(prec, L, st);
+      if (tile_cfg == 3) return launch_complex<32, 32, 2>(prec, L, st);
+      return launch_complex<64, 64, 2>(prec, L, st);
     }
   }
   return hipErrorInvalidValue;
