@@ -394,9 +394,14 @@ def c1_run(sad, args):
         d.set_couplings(hmap, coup)
         d.set_max_dcut(m)
         ctx.sync()
+        h0 = ctx.host_stats()
         t0 = time.perf_counter()
         d.idmrg()
         es, trs, t_lan = [], [], 0.0
+        split = {"t_env": 0.0, "t_lanczos": 0.0, "t_svd": 0.0}
+        for r in d.records():
+            for k in split:
+                split[k] += r[k]
         for _ in range(nsweep):
             d.records()
             d.sweep()
@@ -404,9 +409,16 @@ def c1_run(sad, args):
             es.append(d.energy())
             trs.append(recs[-1]["trunc_err"])
             t_lan += sum(r["t_lanczos"] for r in recs)
+            for r in recs:
+                for k in split:
+                    split[k] += r[k]
         ctx.sync()
         secs = time.perf_counter() - t0
+        h1 = ctx.host_stats()
         replays = sad.lanczos_mode(prev[0], prev[1])[2]
+        run.split = {k: round(v, 3) for k, v in split.items()}
+        run.host = {"syncs": h1["syncs"] - h0["syncs"], "sync_s": round(h1["sync_s"] - h0["sync_s"], 3),
+                    "mallocs": h1["mallocs"] - h0["mallocs"]}
         return secs, es, trs, t_lan, replays
 
     run(True, True)  # first use: autotuner entries, rocSOLVER and allocator warm-up
@@ -423,7 +435,8 @@ def c1_run(sad, args):
     sad.persistent(2)
     gpu_s, gpu_e, gpu_tr, _, _ = run(True, True)
     line = {"workload": "C1: U(1) open chain L=32, m=64 kept states, iDMRG + 4 sweeps, FP64",
-            "gpu_seconds": round(gpu_s, 3), "gpu_sweep_energies": gpu_e, "gpu_last_step_trunc": gpu_tr,
+            "gpu_seconds": round(gpu_s, 3), "gpu_split_s": run.split, "gpu_host": run.host,
+            "gpu_sweep_energies": gpu_e, "gpu_last_step_trunc": gpu_tr,
             "lanczos_step_modes": modes,
             "launch_kind": {"persistent": {"seconds": round(p_s, 3), "sweep_lanczos_seconds": round(p_lan, 3),
                                            "same_energies": p_e == gpu_e},

@@ -231,6 +231,18 @@ int sadmrg_gemm_balanced_split(int enable, int* previous);
    the setting; *previous receives the old value.  Process-wide, default 256.
    No reference counterpart (the reference always calls gesvd). */
 int sadmrg_svd_gram_min(int value, int* previous);
+/* Small-sector SVD of the truncation: in sweep steps (one side absorbs S),
+   real sectors with min(m, n) <= max_order whose working set fits 64 KB of
+   LDS are decomposed by a
+   one-sided Jacobi kernel (one workgroup per sector, all of a step's small
+   sectors in one launch; relatively accurate singular values), in place of
+   rocSOLVER gesdd, which replaces the reference's gesvd
+   (dense_inst.cpp:486) for the others.  max_order 0 turns it off, -1 leaves
+   the setting (process-wide, default 64, at most 88); *previous receives the
+   old value; counts[2] = {sectors decomposed by it, sectors handed to gesdd
+   after 60 sweeps without convergence} since the last sadmrg_svd_stats
+   reset.  No reference counterpart. */
+int sadmrg_svd_jacobi(int max_order, int* previous, long long* counts);
 /* Batched symmetric eigensolver of the truncation's Gram route: for each of
    `count` real symmetric matrices a[q] (n[q] x n[q], column-major, leading
    dimension lda[q], lower triangle read, contents destroyed; device

@@ -102,6 +102,7 @@ def lib():
         "sadmrg_lanczos_batched": (i, [vp, vp, i, C.POINTER(LanczosOptions), dp, ip, ip, dp, ip]),
         "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_svd_gram_min": (i, [i, ip]),
+        "sadmrg_svd_jacobi": (i, [i, ip, llp]),
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
         "sadmrg_lanczos_mode": (i, [i, i, ip, C.POINTER(C.c_longlong), i]),
         "sadmrg_gemm_vec_staging": (i, [i, ip]),
@@ -615,6 +616,16 @@ def svd_stats(reset=False):
 
 def _path(directory):
     return b"" if directory is None else os.fsencode(str(directory))
+
+
+def svd_jacobi(max_order=-1):
+    """Largest min(m, n) of the real sectors the truncation decomposes by the
+    one-sided Jacobi kernel (0 off, -1 only reads; process-wide).  Returns
+    (previous value, sectors decomposed by it, sectors handed to gesdd)."""
+    prev = C.c_int()
+    cnt = (C.c_longlong * 2)()
+    check(lib().sadmrg_svd_jacobi(int(max_order), C.byref(prev), cnt))
+    return prev.value, int(cnt[0]), int(cnt[1])
 
 
 def svd_gram_min(value=-1):

@@ -1356,10 +1356,16 @@ void op_env_adjoint(BTensor& out, const BTensor& lam, const BTensor& uu, const B
 
 namespace {
 std::atomic<int> g_gram_min{256};
+std::atomic<int> g_jacobi_max{64};
+// sweeps before a Jacobi sector is handed to gesdd (quadratic convergence:
+// about 6 to 12 are needed)
+constexpr int kJacobiSweeps = 60;
 std::atomic<int> g_svd_eig{1};
 SvdStats g_svd_stats;
 }
 void set_gram_min(int n) { g_gram_min.store(n); }
+void set_svd_jacobi_max(int q) { g_jacobi_max.store(q); }
+int svd_jacobi_max() { return g_jacobi_max.load(); }
 int gram_min() { return g_gram_min.load(); }
 void set_svd_eigensolver(int v) { g_svd_eig.store(v); }
 int svd_eigensolver() { return g_svd_eig.load(); }
@@ -1401,7 +1407,23 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
   DevBuf du(ctx, std::max<int64_t>(u_tot, 1) * 8 * w), dvt(ctx, std::max<int64_t>(vt_tot, 1) * 8 * w);
   DevBuf ds(ctx, std::max<int64_t>(s_tot, 1) * 8), dwork(ctx, std::max<int64_t>(a_max, 1) * 8 * w);
   DevBuf dinfo(ctx, sizeof(int) * std::max<size_t>(secs.size(), 1));
+  SADMRG_HIP(hipMemsetAsync(dinfo.get(), 0, sizeof(int) * std::max<size_t>(secs.size(), 1), ctx->stream()));
   rocblas_handle h = static_cast<rocblas_handle>(ctx->solver());
+  // small real sectors of sweep steps (one side absorbs S): one-sided
+  // Jacobi in LDS, every such sector in one launch (jacobi_svd_kernel); the
+  // rest by gesdd or the Gram route below.  iDMRG steps, (p1, p2) = (0, 0),
+  // stay on gesdd: their blocks are rank deficient, the kept rule's 1e-10
+  // threshold then decides on round-off values (gesdd's sqrt(eps) S_max
+  // against the kernel's exact zeros), and the growth trajectory (the start
+  // vectors drawn per kept block) follows that decision.
+  std::vector<size_t> jac, rest;
+  const int jmax = svd_jacobi_max();
+  const bool sweep_step = p1 + p2 > 0.0;
+  for (size_t k = 0; k < secs.size(); ++k)
+    (!cx && sweep_step && jmax > 0 && jacobi_svd_fits(secs[k].m, secs[k].n, jmax) ? jac : rest).push_back(k);
+  DevBuf jjobs, jflags;
+  std::vector<int> jf(jac.size(), 0);
+  bool jac_pending = false;
   // One sector, unpadded, straight into its U / V' / S slots.
   auto single = [&](size_t si) {
     const Sec& s = secs[si];
@@ -1420,14 +1442,6 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
                             dvt.as<rocblas_double_complex>() + s.vt_off, s.mn, dinfo.as<int>() + si);
     if (st != rocblas_status_success) fail(kErrNumerics, "rocsolver gesdd failed");
   };
-  // Sectors of similar size share one strided-batched gesdd, each padded with
-  // zeros to the batch's largest (m, n).  Zero padding appends zero singular
-  // values and leaves the leading singular triplets unchanged, their vectors
-  // vanishing on the padding, so a sector's U (m x mn) and V' (mn x n) are the
-  // leading blocks of its padded factors.  A sector whose largest singular
-  // value is round-off (kept as one state by the reference's rule) is redone
-  // unpadded, because its singular vectors are then not determined by the
-  // data and a padded solve may return one that lives on the padding.
   // Sectors of similar size share one strided-batched gesdd, each padded with
   // zeros to the batch's largest (m, n).  Zero padding appends zero singular
   // values and leaves the leading singular triplets unchanged, their vectors
@@ -1743,7 +1757,19 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
       if (padded[si])
         SADMRG_HIP(hipMemcpyAsync(&lead[si], ds.as<double>() + secs[si].s_off, 8, hipMemcpyDeviceToHost,
                                   ctx->stream()));
+    if (jac_pending)
+      SADMRG_HIP(hipMemcpyAsync(jf.data(), jflags.get(), jac.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream()));
     ctx->sync();
+    if (jac_pending) {
+      // a Jacobi sector that did not converge is decomposed by gesdd
+      jac_pending = false;
+      for (size_t t = 0; t < jac.size(); ++t)
+        if (jf[t]) {
+          single(jac[t]);
+          g_svd_stats.jacobi_redo++;
+        }
+    }
     for (size_t si = 0; si < secs.size(); ++si)
       if (padded[si] && lead[si] <= 1e-16) {
         padded[si] = 0;
@@ -1790,10 +1816,49 @@ void op_svd(const BTensor& vec, BTensor& tu, double p1, BTensor& tv, double p2, 
     }
     // global descending order; ties broken by sector then index (deterministic)
     std::stable_sort(all.begin(), all.end(), [](const SV& a, const SV& b) { return a.v > b.v; });
+    // Values equal to within the decompositions' rounding (1e-12 relative)
+    // are ordered as exact ties, by sector then index, so that the cut
+    // through a degenerate multiplet (symmetry-related sectors) does not
+    // depend on which SVD routine produced the last bits.
+    for (size_t a = 0; a < all.size();) {
+      size_t b = a + 1;
+      while (b < all.size() && all[a].v - all[b].v <= 1e-12 * all[a].v) ++b;
+      if (b - a > 1)
+        std::stable_sort(all.begin() + a, all.begin() + b, [&](const SV& x, const SV& y) {
+          return sec_of[x.sec] != sec_of[y.sec] ? sec_of[x.sec] < sec_of[y.sec] : x.idx < y.idx;
+        });
+      a = b;
+    }
   };
-  std::vector<size_t> every(secs.size());
-  for (size_t k = 0; k < every.size(); ++k) every[k] = k;
-  decompose(every, true);
+  if (!jac.empty()) {
+    const auto tj = std::chrono::steady_clock::now();
+    std::vector<JacobiSvdJob> jobs(jac.size());
+    int words = 0;
+    for (size_t t = 0; t < jac.size(); ++t) {
+      const Sec& sc = secs[jac[t]];
+      JacobiSvdJob& J = jobs[t];
+      J.a = vec.block_ptr(sc.i + sc.j * nm1);
+      J.m = sc.m;
+      J.n = sc.n;
+      J.lda = sc.m;
+      J.s = ds.as<double>() + sc.s_off;
+      J.u = du.as<double>() + sc.u_off;
+      J.ldu = sc.m;
+      J.vt = dvt.as<double>() + sc.vt_off;
+      J.ldvt = sc.mn;
+      const int qe = sc.mn + (sc.mn & 1);
+      words = std::max(words, std::max(sc.m, sc.n) * qe + qe * qe);
+    }
+    jjobs = DevBuf(ctx, jobs.size() * sizeof(JacobiSvdJob));
+    ctx->upload(jjobs.get(), jobs.data(), jobs.size() * sizeof(JacobiSvdJob));
+    jflags = DevBuf(ctx, jac.size() * sizeof(int));
+    SADMRG_HIP(launch_jacobi_svd(jjobs.as<JacobiSvdJob>(), static_cast<int>(jac.size()), words, kJacobiSweeps,
+                                 jflags.as<int>(), ctx->stream()));
+    jac_pending = true;
+    g_svd_stats.jacobi_sectors += static_cast<int64_t>(jac.size());
+    g_svd_stats.t_jacobi += std::chrono::duration<double>(std::chrono::steady_clock::now() - tj).count();
+  }
+  decompose(rest, true);
   finish();
   select();
   {

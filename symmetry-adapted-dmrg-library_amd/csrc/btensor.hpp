@@ -166,8 +166,14 @@ int svd_eigensolver();
 // route's vectors, and the largest sector (max of m, n).
 struct SvdStats {
   int64_t calls = 0, gram_sectors = 0, svd_sectors = 0, gram_redo = 0, max_sector = 0;
-  double seconds = 0, t_gram = 0, t_gesdd = 0, t_vectors = 0;
+  int64_t jacobi_sectors = 0, jacobi_redo = 0;
+  double seconds = 0, t_gram = 0, t_gesdd = 0, t_vectors = 0, t_jacobi = 0;
 };
+// In sweep steps, sectors of real blocks with min(m, n) <= q (and a Jacobi
+// working set that fits 64 KB of LDS) are decomposed by the one-sided Jacobi
+// kernel (launch_jacobi_svd); 0 turns it off.  Process-wide, default 64.
+void set_svd_jacobi_max(int q);
+int svd_jacobi_max();
 SvdStats& svd_stats();
 
 // BlockTensor::fuse(BondType, BondType) (block_tensor_inst.cpp:708-743)

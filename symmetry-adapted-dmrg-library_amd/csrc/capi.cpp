@@ -738,6 +738,19 @@ int sadmrg_svd_eigensolver(int value, int* previous) {
   });
 }
 
+int sadmrg_svd_jacobi(int max_order, int* previous, long long* counts) {
+  return guard([&] {
+    if (max_order < -1 || max_order > kJacobiMaxOrder)
+      fail(kErrInvalid, "svd_jacobi: max_order must be -1 .. " + std::to_string(kJacobiMaxOrder));
+    if (previous) *previous = svd_jacobi_max();
+    if (counts) {
+      counts[0] = svd_stats().jacobi_sectors;
+      counts[1] = svd_stats().jacobi_redo;
+    }
+    if (max_order >= 0) set_svd_jacobi_max(max_order);
+  });
+}
+
 int sadmrg_svd_gram_min(int value, int* previous) {
   return guard([&] {
     if (value < -1) fail(kErrInvalid, "svd_gram_min: value must be >= -1");

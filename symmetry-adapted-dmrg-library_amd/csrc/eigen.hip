@@ -1315,4 +1315,144 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
   g_prof.backt += secs_since(t0);
 }
 
+// ---- small-sector SVD: one-sided Jacobi in LDS ------------------------------
+bool jacobi_svd_fits(int m, int n, int max_order) {
+  const int q = std::min(m, n), p = std::max(m, n);
+  const int qe = q + (q & 1);
+  return q >= 1 && q <= std::min(max_order, kJacobiMaxOrder) &&
+         int64_t(p) * qe + int64_t(qe) * qe <= kJacobiLdsWords;
+}
+
+__global__ __launch_bounds__(256) void jacobi_svd_kernel(const JacobiSvdJob* __restrict__ jobs, int max_sweeps,
+                                                         int* __restrict__ flags) {
+  constexpr int NT = 256;
+  extern __shared__ double lds[];
+  __shared__ int s_rot[2];
+  __shared__ double s_sig[kJacobiMaxOrder + 8];
+  __shared__ int s_rank[kJacobiMaxOrder + 8];
+  const JacobiSvdJob J = jobs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const bool tr = J.m < J.n;  // W = A^T
+  const int p = tr ? J.n : J.m, q = tr ? J.m : J.n;
+  const int qe = q + (q & 1);  // an odd count gets a zero column
+  double* W = lds;             // p x qe, column-major
+  double* V = lds + int64_t(p) * qe;  // qe x qe
+  if (!tr) {
+    for (int idx = tid; idx < p * q; idx += NT) {
+      const int r = idx % p, c = idx / p;
+      W[c * p + r] = J.a[r + int64_t(c) * J.lda];
+    }
+  } else {
+    for (int idx = tid; idx < p * q; idx += NT) {
+      const int i = idx % J.m, j = idx / J.m;  // A(i, j) = W(j, i), read along A's columns
+      W[i * p + j] = J.a[i + int64_t(j) * J.lda];
+    }
+  }
+  for (int r = tid; r < p; r += NT)
+    if (qe > q) W[q * p + r] = 0.0;
+  for (int idx = tid; idx < qe * qe; idx += NT) V[idx] = (idx % qe == idx / qe) ? 1.0 : 0.0;
+  if (tid < 2) s_rot[tid] = 0;
+  __syncthreads();
+
+  const int npairs = qe / 2;
+  int T = 64;
+  while (T > 1 && npairs * T > NT) T >>= 1;
+  const int pk = tid / T, pt = tid % T;
+  const double tol = 2.220446049250313e-16 * sqrt(static_cast<double>(p));
+  int converged = qe < 2 ? 1 : 0;
+  for (int sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
+    for (int r = 0; r < qe - 1; ++r) {
+      if (pk < npairs) {
+        // round-robin pairing: qe - 1 rounds meet every pair once
+        const int i = pk == 0 ? qe - 1 : (r + pk) % (qe - 1);
+        const int j = pk == 0 ? r : (r + qe - 1 - pk) % (qe - 1);
+        double* wi = W + int64_t(i) * p;
+        double* wj = W + int64_t(j) * p;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int k = pt; k < p; k += T) {
+          const double x = wi[k], y = wj[k];
+          al = fma(x, x, al);
+          be = fma(y, y, be);
+          ga = fma(x, y, ga);
+        }
+        for (int o = T / 2; o > 0; o >>= 1) {
+          al += __shfl_xor(al, o, 64);
+          be += __shfl_xor(be, o, 64);
+          ga += __shfl_xor(ga, o, 64);
+        }
+        if (i < q && j < q && al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
+          // rotation that makes columns i and j orthogonal (the smaller root
+          // of t^2 + 2 zeta t - 1 = 0)
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
+          const double c = 1.0 / sqrt(fma(t, t, 1.0)), sn = c * t;
+          for (int k = pt; k < p; k += T) {
+            const double x = wi[k], y = wj[k];
+            wi[k] = c * x - sn * y;
+            wj[k] = sn * x + c * y;
+          }
+          double* vi = V + int64_t(i) * qe;
+          double* vj = V + int64_t(j) * qe;
+          for (int k = pt; k < qe; k += T) {
+            const double x = vi[k], y = vj[k];
+            vi[k] = c * x - sn * y;
+            vj[k] = sn * x + c * y;
+          }
+          if (pt == 0) s_rot[sweep & 1] = 1;
+        }
+      }
+      __syncthreads();
+    }
+    const int rot = s_rot[sweep & 1];
+    __syncthreads();  // every thread has read the flag before it is cleared
+    if (tid == 0) s_rot[sweep & 1] = 0;
+    converged = rot == 0;
+  }
+  if (tid == 0) flags[blockIdx.x] = converged ? 0 : 1;
+
+  // singular values and their descending order (ties by column index)
+  for (int c = tid; c < q; c += NT) {
+    double x = 0.0;
+    for (int k = 0; k < p; ++k) x = fma(W[c * p + k], W[c * p + k], x);
+    s_sig[c] = sqrt(x);
+  }
+  __syncthreads();
+  for (int c = tid; c < q; c += NT) {
+    int rank = 0;
+    for (int k = 0; k < q; ++k) rank += (s_sig[k] > s_sig[c] || (s_sig[k] == s_sig[c] && k < c)) ? 1 : 0;
+    s_rank[c] = rank;
+    J.s[rank] = s_sig[c];
+  }
+  __syncthreads();
+  double smax = 0.0;
+  for (int k = 0; k < q; ++k) smax = fmax(smax, s_sig[k]);
+  const bool all_zero = smax == 0.0;
+  // W side: w_c / S_c (length p); rotation side: column c of V (length q)
+  for (int idx = tid; idx < p * q; idx += NT) {
+    const int k = idx % p, c = idx / p;
+    const double sg = s_sig[c];
+    double x = sg > 0.0 ? W[c * p + k] / sg : 0.0;
+    if (all_zero && s_rank[c] == 0) x = k == 0 ? 1.0 : 0.0;
+    const int rk = s_rank[c];
+    if (!tr) J.u[k + int64_t(rk) * J.ldu] = x;    // U(:, rk), m = p rows
+    else J.vt[rk + int64_t(k) * J.ldvt] = x;      // Vt(rk, :), n = p columns
+  }
+  for (int idx = tid; idx < q * q; idx += NT) {
+    const int k = idx % q, c = idx / q;
+    const int rk = s_rank[c];
+    const double x = V[c * qe + k];
+    if (!tr) J.vt[rk + int64_t(k) * J.ldvt] = x;  // Vt(rk, :), n = q columns
+    else J.u[k + int64_t(rk) * J.ldu] = x;        // U(:, rk), m = q rows
+  }
+}
+
+hipError_t launch_jacobi_svd(const JacobiSvdJob* jobs, int njobs, int lds_words, int max_sweeps, int* flags,
+                             hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  if (lds_words <= 0 || lds_words > kJacobiLdsWords || max_sweeps < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(jacobi_svd_kernel, dim3(njobs), dim3(256), size_t(lds_words) * sizeof(double), st, jobs,
+                     max_sweeps, flags);
+  return hipGetLastError();
+}
+
 }  // namespace sadmrg

@@ -23,6 +23,7 @@
 #pragma once
 #include <cstdint>
 #include <vector>
+#include <hip/hip_runtime.h>
 #include "core.hpp"
 
 namespace sadmrg {
@@ -68,6 +69,35 @@ class SymEigBatch {
   std::vector<int64_t> doff_;  // per job: offset of d, e, tau (3n doubles) in trd_
   DevBuf trd_;
 };
+
+// Small-sector SVD (one-sided Jacobi, Hestenes): A = U S V^T of a real
+// m x n block (column-major, leading dimension lda), S descending,
+// U m x mn (ldu), Vt mn x n (ldvt), mn = min(m, n), each matrix in the LDS
+// of one workgroup.  The longer side is the column length: the q = mn
+// columns of W = A (or A^T when m < n) are rotated in pairs, q/2 disjoint
+// pairs per round in round-robin order, until a whole sweep needs no
+// rotation (|w_i . w_j| <= sqrt(p) eps |w_i| |w_j| for every pair).  S_j =
+// |w_j|; the W side's vectors are w_j / S_j, the other side accumulates the
+// rotations.  Column orthogonality is enforced relative to each column's
+// norm, so small singular values keep their relative accuracy (unlike the
+// Gram matrix route and gesdd).  A column with S_j = 0 gets a zero vector
+// (e_0 when the whole block is zero); the truncation never keeps such a
+// column except that one.
+struct JacobiSvdJob {
+  const double* a = nullptr;
+  double* s = nullptr;
+  double* u = nullptr;
+  double* vt = nullptr;
+  int m = 0, n = 0, lda = 0, ldu = 0, ldvt = 0, pad = 0;
+};
+// Largest min(m, n) and LDS words (doubles) of a matrix the kernel takes.
+constexpr int kJacobiMaxOrder = 88;
+constexpr int kJacobiLdsWords = 8192;  // 64 KB
+bool jacobi_svd_fits(int m, int n, int max_order);
+// One workgroup per job; flags[j] = 0 converged, 1 not converged within
+// max_sweeps (the outputs are then those of the last sweep).
+hipError_t launch_jacobi_svd(const JacobiSvdJob* jobs, int njobs, int lds_words, int max_sweeps, int* flags,
+                             hipStream_t st);
 
 // Looks up the device address of the allocation-free scratch of `device`
 // (the current device), once per device.
