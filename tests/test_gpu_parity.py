@@ -257,9 +257,7 @@ def test_lanczos_and_svd_on_reference_state(ctx, sad, name):
 # reference fixtures, maximum per-step relative energy difference:
 #   su2_obc12   iDMRG 3.2e-11, sweeps 1.6e-10 / 3.0e-13
 #   su2_ring10  iDMRG 2.2e-8 (Lanczos stop, see above), sweeps 3.1e-11 / 1.2e-15
-#   u1_ring10   iDMRG 1.4e-8, sweeps 1.6e-9 / 1.1e-14 (sweep 1 4.1e-10 with
-#               gesdd sector SVDs; the ring's degenerate singular values are
-#               rotated differently by the Jacobi kernel); truncation 7.5e-10 rel
+#   u1_ring10   iDMRG 1.4e-8, sweeps 4.1e-10 / 1.1e-14; truncation 7.5e-10 rel
 #   u1c_ring10  iDMRG 1.0e-8, sweeps 1.2e-9 / 2.9e-14; truncation 2.4e-9 rel
 #   u1_obc32    (L=32 m=64, four sweeps) iDMRG 1.7e-3, sweeps 1.1e-3 / 2.3e-8 /
 #               1.4e-12 / 4.7e-15; truncation |diff| 1.3e-14 / 2.1e-15 in
@@ -272,7 +270,7 @@ def test_lanczos_and_svd_on_reference_state(ctx, sad, name):
 DMRG_PARITY = {
     "dmrg_su2_obc12": {"idmrg": 1e-9, "sweeps": [1e-9, 1e-9], "trunc": []},
     "dmrg_su2_ring10": {"idmrg": 1e-6, "sweeps": [1e-9, 1e-9], "trunc": []},
-    "dmrg_u1_ring10": {"idmrg": 1e-6, "sweeps": [3e-9, 1e-9], "trunc": [1]},
+    "dmrg_u1_ring10": {"idmrg": 1e-6, "sweeps": [1e-9, 1e-9], "trunc": [1]},
     "dmrg_u1c_ring10": {"idmrg": 1e-6, "sweeps": [3e-9, 1e-9], "trunc": [1]},
     "dmrg_su2_j1j2_4x3": {"idmrg": None, "sweeps": [None, None], "trunc": [], "best_last": 1e-6},
     "dmrg_u1_obc32": {"idmrg": 5e-3, "sweeps": [5e-3, 1e-7, 1e-9, 1e-9], "trunc": [2, 3]},
