@@ -1144,7 +1144,7 @@ def main():
     if "c5" in want:
         print("[bench] C5 gradient", file=sys.stderr, flush=True)
         extras["c5"] = {"workload": "C5: coupling gradient, SU(2) open chain L=48, D=1024",
-                        **gradient_run(sad, ctx, 48, 1024, [256, 512], fd_bonds=(None,))}
+                        **gradient_run(sad, ctx, 48, 1024, [256, 512], fd_bonds=(0, None, -1))}
 
     # the reference on the host cores: both BLAS settings, each in a child
     # process; the synthetic bond for `value`, the real centre step for the
