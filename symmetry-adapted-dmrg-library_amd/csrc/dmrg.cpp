@@ -478,7 +478,9 @@ double Dmrg::lanczos_solve(int il, int ir, BTensor& vec) {
   if (mlanc < 2) mlanc = 2;
   for (int b = 0; b < vec.nten(); ++b)
     require(!vec.st.allowed(b) || vec.present(b), "lanczos: input vector must hold every allowed block");
+  const auto tp = Clock::now();
   auto plan = make_heff(il, ir, vec, nullptr);
+  cur_.t_plan += since(tp);
   LanczosOpts o = lopt_;
   o.mlanc = mlanc;
   o.fp64_energy = step_fp64_energy_;
@@ -516,8 +518,9 @@ void Dmrg::svd(const BTensor& vec, int a, int b, double p1, double p2) {
   cur_.t_svd += since(t0);
   rec_.push_back(cur_);
   if (verbose_ > 0) {
-    std::fprintf(stderr, "[sadmrg] step %d-%d E=%.12f kept=%d trunc=%.3e env=%.4fs lanczos=%.4fs svd=%.4fs\n", a, b,
-                 cur_.energy, cur_.kept, te, cur_.t_env, cur_.t_lanczos, cur_.t_svd);
+    std::fprintf(stderr,
+                 "[sadmrg] step %d-%d E=%.12f kept=%d trunc=%.3e env=%.4fs lanczos=%.4fs (plan %.4fs) svd=%.4fs\n", a,
+                 b, cur_.energy, cur_.kept, te, cur_.t_env, cur_.t_lanczos, cur_.t_plan, cur_.t_svd);
     if (verbose_ > 1) {
       const SvdStats& s1 = svd_stats();
       const EigProfile& e1 = SymEigBatch::profile();

@@ -27,6 +27,7 @@ struct StepRecord {
   int64_t ndata;      // reduced elements of the two-site wave function
   double ritz32;      // mixed precision: the last Ritz value of the FP32 Krylov recurrence
   int energy64;       // 1: energy is an FP64 Rayleigh quotient; 0: the FP32 Ritz value
+  double t_plan = 0;  // seconds of t_lanczos spent building the step's H_eff plan
 };
 
 class Dmrg {
