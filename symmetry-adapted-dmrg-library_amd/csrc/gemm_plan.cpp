@@ -415,8 +415,7 @@ void GemmBatch::run(Context* ctx, GemmPrec prec, const ArenaTable& arenas, int c
   L.arenas = arenas;
   L.ntiles = ts.ntiles;
   L.nreduce = ts.nreduce;
-  L.vec = vec_ && g_vec_staging && !complex_ &&
-          (prec == GemmPrec::F64 || prec == GemmPrec::F32 || prec == GemmPrec::F32_OUT64);
+  L.vec = vec_ && g_vec_staging && !complex_;  // every real operand type
   L.queue = pers ? ts.queue.as<int>() : nullptr;
   SADMRG_HIP(launch_grouped_gemm(prec, cfg % kTileConfigs, L, ctx->stream()));
 }
@@ -462,8 +461,7 @@ void GemmBatch::run_per_problem(Context* ctx, GemmPrec prec, const ArenaTable& a
     L.arenas = arenas;
     L.ntiles = n;
     L.nreduce = 0;
-    L.vec = vec_ && g_vec_staging && !complex_ &&
-            (prec == GemmPrec::F64 || prec == GemmPrec::F32 || prec == GemmPrec::F32_OUT64);
+    L.vec = vec_ && g_vec_staging && !complex_;  // every real operand type
     SADMRG_HIP(launch_grouped_gemm(prec, cfg, L, ctx->stream()));
   }
 }

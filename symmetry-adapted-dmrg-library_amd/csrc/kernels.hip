@@ -22,7 +22,7 @@ namespace sadmrg {
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16_t;
-typedef short s4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 __device__ inline d4_t mfma_16x16x4(double a, double b, d4_t c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -156,18 +156,35 @@ __device__ inline void cursor_advance(OperandCursor<T>& c, int TK) {
   c.kk -= TK;
 }
 
-// ---- 16-byte staging (tables whose operands are 16-byte aligned and
-// zero-padded to a 16-byte leading dimension) --------------------------------
-// A thread stages VW = 16 / sizeof(T) consecutive elements along the
-// operand's contiguous index per vector, NV vectors per K step.
+// ---- vector staging (tables whose operands are aligned to 4 elements and
+// zero-padded to a leading dimension of 4) -----------------------------------
+// A thread stages VW consecutive elements along the operand's contiguous
+// index per vector, NV vectors per K step: 16 bytes (4 FP32 or 2 FP64
+// values), or 8 bytes (4 BF16 values, the 4-element alignment of the
+// layout).  Vec16 is the 16-byte view of an LDS row used by fragment reads.
 template <class T> struct Vec16;
 template <> struct Vec16<float> { typedef float4 type; };
 template <> struct Vec16<double> { typedef double2 type; };
-template <> struct Vec16<bf16_t> { typedef uint4 type; };  // BF16 tables stage elements (no VEC variant)
+template <> struct Vec16<bf16_t> { typedef uint4 type; };
+template <class T> struct VecS { typedef typename Vec16<T>::type type; };
+template <> struct VecS<bf16_t> { typedef uint2 type; };
+// the segment scale applied to a staged vector (BF16: in FP32, rounded once
+// per element, as scaled() does for single elements)
+template <class V, class S> __device__ inline V vscaled(const V& v, S s) { return v * s; }
+template <> __device__ inline uint2 vscaled<uint2, float>(const uint2& v, float s) {
+  auto one = [&](unsigned short h) {
+    const bf16_t x = static_cast<bf16_t>(static_cast<float>(__builtin_bit_cast(bf16_t, h)) * s);
+    return static_cast<unsigned>(__builtin_bit_cast(unsigned short, x));
+  };
+  uint2 r;
+  r.x = one(static_cast<unsigned short>(v.x & 0xffffu)) | (one(static_cast<unsigned short>(v.x >> 16)) << 16);
+  r.y = one(static_cast<unsigned short>(v.y & 0xffffu)) | (one(static_cast<unsigned short>(v.y >> 16)) << 16);
+  return r;
+}
 
 template <class T, int R, int TK>
 struct VecTile {
-  static constexpr int VW = 16 / sizeof(T);
+  static constexpr int VW = sizeof(typename VecS<T>::type) / sizeof(T);
   static constexpr int NV = R * TK / (256 * VW);  // vectors per thread and step
   static_assert(NV >= 1 && (R / VW) <= 256 && 256 % (R / VW) == 0 && 256 % (TK / VW) == 0, "vector tile shape");
 };
@@ -200,8 +217,8 @@ __device__ inline void vcursor_init(OperandCursor<T>& c, const T* base, int64_t 
 }
 
 template <class T, int NV>
-__device__ inline void vcursor_load(const OperandCursor<T>& c, typename Vec16<T>::type (&v)[NV]) {
-  typedef typename Vec16<T>::type V;
+__device__ inline void vcursor_load(const OperandCursor<T>& c, typename VecS<T>::type (&v)[NV]) {
+  typedef typename VecS<T>::type V;
 #pragma unroll
   for (int e = 0; e < NV; ++e) {
     const bool ok = c.kr ? (c.kk > 0 && e * c.epos < c.rr) : (c.rr > 0 && e * c.epos < c.kk);
@@ -211,10 +228,10 @@ __device__ inline void vcursor_load(const OperandCursor<T>& c, typename Vec16<T>
 }
 
 template <class T, int R, int TK>
-__device__ inline void vtile_store(T* lds, bool kr, const typename Vec16<T>::type (&v)[VecTile<T, R, TK>::NV],
+__device__ inline void vtile_store(T* lds, bool kr, const typename VecS<T>::type (&v)[VecTile<T, R, TK>::NV],
                                    typename ScaleT<T>::type scale, int tid) {
   using OT = OperandTile<T, R, TK>;
-  typedef typename Vec16<T>::type V;
+  typedef typename VecS<T>::type V;
   constexpr int VW = VecTile<T, R, TK>::VW;
   constexpr int NV = VecTile<T, R, TK>::NV;
   if (!kr) {
@@ -222,13 +239,15 @@ __device__ inline void vtile_store(T* lds, bool kr, const typename Vec16<T>::typ
     const int rv = (tid % NR) * VW, kt = tid / NR;
 #pragma unroll
     for (int e = 0; e < NV; ++e)
-      *reinterpret_cast<V*>(&lds[(kt + e * (256 / NR)) * OT::LR + rv]) = v[e] * scale;
+      *reinterpret_cast<V*>(&lds[(kt + e * (256 / NR)) * OT::LR + rv]) = vscaled(v[e], scale);
   } else {
+    // a vector of VW <= NQ elements at a multiple of VW stays inside one
+    // swizzled 16-byte group
     constexpr int NK = TK / VW;
     const int kv = (tid % NK) * VW, rt = tid / NK;
 #pragma unroll
     for (int e = 0; e < NV; ++e)
-      *reinterpret_cast<V*>(&lds[OT::kr_off(rt + e * (256 / NK), kv)]) = v[e] * scale;
+      *reinterpret_cast<V*>(&lds[OT::kr_off(rt + e * (256 / NK), kv)]) = vscaled(v[e], scale);
   }
 }
 
@@ -286,6 +305,43 @@ __device__ inline void mfma_step(const T* sa, const T* sb, Acc (&acc)[FM][FN], i
   constexpr int NF = FragW<T>::NF;
   constexpr int NGRP = (TK / 16) * (4 / NF);  // fragment groups per K step
   const int g = lane >> 4;
+  if constexpr (sizeof(T) == 2) {
+    // BF16: gfx950's 32-deep MFMA takes eight k-slot values per lane, the
+    // four of lane group g in the 16-deep chunks kc and kc + 16 (A and B
+    // alike, so the products sum over all 32 k of the pair of chunks)
+    static_assert(TK % 32 == 0, "BF16 K steps are multiples of 32");
+    // B fragments of the chunk pair held packed, A fragments one at a time
+    auto frag8 = [&](const T* lds, int row, int kc, bool is_a) {
+      T lo[NF], hi[NF];
+      if (is_a) {
+        tile_frag<T, TM, TK, KRA>(lds, row, kc, g, 0, lo);
+        tile_frag<T, TM, TK, KRA>(lds, row, kc + 16, g, 0, hi);
+      } else {
+        tile_frag<T, TN, TK, KRB>(lds, row, kc, g, 0, lo);
+        tile_frag<T, TN, TK, KRB>(lds, row, kc + 16, g, 0, hi);
+      }
+      bf8_t v;
+#pragma unroll
+      for (int q = 0; q < NF; ++q) {
+        v[q] = lo[q];
+        v[NF + q] = hi[q];
+      }
+      return v;
+    };
+#pragma unroll 1
+    for (int kc = 0; kc < TK; kc += 32) {
+      bf8_t pb[FN];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) pb[j] = frag8(sb, wn * (TN / 2) + j * 16 + (lane & 15), kc, false);
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const bf8_t pa = frag8(sa, wm * (TM / 2) + i * 16 + (lane & 15), kc, true);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pa, pb[j], acc[i][j], 0, 0, 0);
+      }
+    }
+    return;
+  }
   // one fragment group live at a time keeps the accumulators, the staged
   // next step and the fragments inside the 256-register budget
 #pragma unroll 1
@@ -296,30 +352,12 @@ __device__ inline void mfma_step(const T* sa, const T* sb, Acc (&acc)[FM][FN], i
     for (int i = 0; i < FM; ++i) tile_frag<T, TM, TK, KRA>(sa, wm * (TM / 2) + i * 16 + (lane & 15), kc, g, j0, af[i]);
 #pragma unroll
     for (int j = 0; j < FN; ++j) tile_frag<T, TN, TK, KRB>(sb, wn * (TN / 2) + j * 16 + (lane & 15), kc, g, j0, bf[j]);
-    if constexpr (sizeof(T) == 2) {
-      // BF16: the four k-slot values of lane group g feed one 16-deep MFMA
-      s4_t pa[FM], pb[FN];
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) pa[i][q] = __builtin_bit_cast(short, af[i][q]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) pb[j][q] = __builtin_bit_cast(short, bf[j][q]);
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pa[i], pb[j], acc[i][j], 0, 0, 0);
-    } else {
-#pragma unroll
-      for (int q = 0; q < NF; ++q)
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = mfma_16x16x4(af[i][q], bf[j][q], acc[i][j]);
-    }
+        for (int j = 0; j < FN; ++j) acc[i][j] = mfma_16x16x4(af[i][q], bf[j][q], acc[i][j]);
   }
 }
 
@@ -410,7 +448,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTile tile, const GemmProb* _
   };
   // one staged K step in registers, with the orientation and scale it was
   // loaded with (elements, or 16-byte vectors when VEC)
-  typedef typename Vec16<T>::type vec_t;
+  typedef typename VecS<T>::type vec_t;
   struct StagedS {
     T a[TA::E], b[TB::E];
     scale_t scale;
@@ -796,20 +834,17 @@ static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   typename PartT<TOut>::type* part = static_cast<typename PartT<TOut>::type*>(L.partial);
   // Persistent variants exist where the tile loop compiles without scratch:
   // FP64 128 x 64 tiles stage one K step ahead (two steps need the registers
-  // the loop keeps), and the FP32 128 x 128 tiles only with 16-byte staging;
-  // other tables launch one workgroup per tile.
+  // the loop keeps), and the FP32 128 x 128 tiles and every BF16 tile only
+  // with vector staging; other tables launch one workgroup per tile.
   constexpr int PFP = (sizeof(T) == 8 && TM == 128 && TN == 64) ? 1 : PF;
-  constexpr bool kElemPersist = !(sizeof(T) == 4 && TM == 128 && TN == 128);
+  constexpr bool kElemPersist = !(sizeof(T) == 4 && TM == 128 && TN == 128) && sizeof(T) != 2;
   if (L.queue && (L.vec || kElemPersist)) {
     // persistent: about the resident workgroup count (WPE per CU)
     const int grid = L.ntiles < kPersistCUs * WPE ? L.ntiles : kPersistCUs * WPE;
-    if constexpr (sizeof(T) >= 4) {
-      if (L.vec) {
-        hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PFP, true, true>), dim3(grid), dim3(256),
-                           0, st, L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, 0, L.queue,
-                           L.ntiles);
-        return hipGetLastError();
-      }
+    if (L.vec) {
+      hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PFP, true, true>), dim3(grid), dim3(256), 0,
+                         st, L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, 0, L.queue, L.ntiles);
+      return hipGetLastError();
     }
     if constexpr (kElemPersist) {
       hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PFP, false, true>), dim3(grid), dim3(256), 0,
@@ -820,12 +855,10 @@ static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   const int chunk = 1 << 30;
   for (int base = 0; base < L.ntiles; base += chunk) {
     const int n = L.ntiles - base < chunk ? L.ntiles - base : chunk;
-    if constexpr (sizeof(T) >= 4) {
-      if (L.vec) {
-        hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF, true>), dim3(n), dim3(256), 0, st,
-                           L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base, nullptr, n);
-        continue;
-      }
+    if (L.vec) {
+      hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF, true>), dim3(n), dim3(256), 0, st,
+                         L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base, nullptr, n);
+      continue;
     }
     hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF>), dim3(n), dim3(256), 0, st,
                        L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base, nullptr, n);
