@@ -837,6 +837,9 @@ static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
   // the loop keeps), and the FP32 128 x 128 tiles and every BF16 tile only
   // with vector staging; other tables launch one workgroup per tile.
   constexpr int PFP = (sizeof(T) == 8 && TM == 128 && TN == 64) ? 1 : PF;
+  // BF16 128 x 128 tiles fit two workgroups per CU with vector staging only;
+  // element staging keeps one
+  constexpr int WPEE = (sizeof(T) == 2 && TM == 128 && TN == 128) ? 1 : WPE;
   constexpr bool kElemPersist = !(sizeof(T) == 4 && TM == 128 && TN == 128) && sizeof(T) != 2;
   if (L.queue && (L.vec || kElemPersist)) {
     // persistent: about the resident workgroup count (WPE per CU)
@@ -860,7 +863,7 @@ static hipError_t launch_real(const GemmLaunch& L, hipStream_t st) {
                          L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base, nullptr, n);
       continue;
     }
-    hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPE, PF>), dim3(n), dim3(256), 0, st,
+    hipLaunchKernelGGL((grouped_gemm_kernel<T, TOut, TM, TN, TK, WPEE, PF>), dim3(n), dim3(256), 0, st,
                        L.tiles, L.probs, L.segs, L.arenas, part, L.reduce, L.counters, base, nullptr, n);
   }
   return hipGetLastError();
@@ -904,12 +907,12 @@ hipError_t launch_grouped_gemm(GemmPrec prec, int tile_cfg, const GemmLaunch& L,
       return launch_real<float, double, 64, 64, 32>(L, st);
     case GemmPrec::BF16:
       if (tile_cfg == 3) return launch_real<bf16_t, bf16_t, 32, 32, 64, 2, 1>(L, st);
-      if (tile_cfg == 1) return launch_real<bf16_t, bf16_t, 128, 128, 64, 1, 1>(L, st);
+      if (tile_cfg == 1) return launch_real<bf16_t, bf16_t, 128, 128, 64, 2, 1>(L, st);
       if (tile_cfg == 2) return launch_real<bf16_t, bf16_t, 128, 64, 64, 2, 1>(L, st);
       return launch_real<bf16_t, bf16_t, 64, 64, 64, 2, 1>(L, st);
     case GemmPrec::BF16_OUT64:
       if (tile_cfg == 3) return launch_real<bf16_t, double, 32, 32, 64, 2, 1>(L, st);
-      if (tile_cfg == 1) return launch_real<bf16_t, double, 128, 128, 64, 1, 1>(L, st);
+      if (tile_cfg == 1) return launch_real<bf16_t, double, 128, 128, 64, 2, 1>(L, st);
       if (tile_cfg == 2) return launch_real<bf16_t, double, 128, 64, 64, 2, 1>(L, st);
       return launch_real<bf16_t, double, 64, 64, 64, 2, 1>(L, st);
     case GemmPrec::Z64:
