@@ -422,14 +422,17 @@ std::vector<int64_t> fuse_positions(const BTensor& f) {
 struct ScatterBuilder {
   std::vector<ScatterTask> tasks;
   void copy(int64_t src_off, int64_t lds, int64_t dst_off, int64_t ldd, int rows, int cols, double scale) {
-    for (int c0 = 0; c0 < cols; c0 += 64) {
+    // about 16 K elements per task (one workgroup each), at most 64 columns:
+    // enough workgroups to spread a few large blocks over every CU
+    const int cw = std::max(1, std::min(64, 16384 / std::max(rows, 1)));
+    for (int c0 = 0; c0 < cols; c0 += cw) {
       ScatterTask t{};
       t.src_off = src_off + int64_t(c0) * lds;
       t.dst_off = dst_off + int64_t(c0) * ldd;
       t.lds = lds;
       t.ldd = ldd;
       t.rows = rows;
-      t.cols = std::min(64, cols - c0);
+      t.cols = std::min(cw, cols - c0);
       t.kind = 0;
       t.scale = scale;
       tasks.push_back(t);

@@ -1182,13 +1182,66 @@ __global__ __launch_bounds__(256) void block_scatter_kernel(const ScatterTask* _
     }
     return;
   }
-  const int64_t total = int64_t(t.rows) * t.cols;
-  for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
-    const int r = static_cast<int>(e % t.rows);
-    const int c = static_cast<int>(e / t.rows);
-    const double* s = src + (t.src_off + r + int64_t(c) * t.lds) * w;
-    double* d = dst + (t.dst_off + r + int64_t(c) * t.ldd) * w;
-    for (int k = 0; k < w; ++k) d[k] = t.scale * s[k];
+  // the rectangle in column-major order, 256 elements per pass: (r, c) of
+  // a thread's element advances by (256 mod rows, 256 / rows) with one
+  // carry, so the loop divides only once (at the start)
+  const int rows = t.rows, cols = t.cols;
+  if (rows <= 0) return;
+  const int dc = 256 / rows, dr = 256 - dc * rows;
+  int c = static_cast<int>(threadIdx.x) / rows;
+  int r = static_cast<int>(threadIdx.x) - c * rows;
+  const double sc = t.scale;
+  // U elements per thread in flight: their positions first, then the loads,
+  // then the stores
+  constexpr int U = 8;
+  auto advance = [&]() {
+    r += dr;
+    c += dc;
+    if (r >= rows) {
+      r -= rows;
+      ++c;
+    }
+  };
+  if (w == 1) {
+    const double* s0 = src + t.src_off;
+    double* d0 = dst + t.dst_off;
+    while (c < cols) {
+      int64_t so[U], dof[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ok[u] = c < cols;
+        so[u] = r + int64_t(c) * t.lds;
+        dof[u] = r + int64_t(c) * t.ldd;
+        advance();
+      }
+      double x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = ok[u] ? s0[so[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (ok[u]) d0[dof[u]] = sc * x[u];
+    }
+  } else {
+    const double2* s0 = reinterpret_cast<const double2*>(src) + t.src_off;
+    double2* d0 = reinterpret_cast<double2*>(dst) + t.dst_off;
+    while (c < cols) {
+      int64_t so[U], dof[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ok[u] = c < cols;
+        so[u] = r + int64_t(c) * t.lds;
+        dof[u] = r + int64_t(c) * t.ldd;
+        advance();
+      }
+      double2 x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = ok[u] ? s0[so[u]] : double2{0.0, 0.0};
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (ok[u]) d0[dof[u]] = double2{sc * x[u].x, sc * x[u].y};
+    }
   }
 }
 
