@@ -256,6 +256,15 @@ int sadmrg_svd_jacobi(int max_order, int* previous, long long* counts);
    which the reference calls one matrix at a time. */
 int sadmrg_eigh_batched(sadmrg_ctx* ctx, int count, const int* n, double* const* a, const int* lda,
                         double* const* w, const int* k, double* const* z, const int* ldz);
+/* Inverse iteration of sadmrg_eigh_batched over clusters of close
+   eigenvalues (closer than 32 eps |T|): mode 0 (default) iterates the
+   members in parallel lanes and orthonormalises each cluster afterwards,
+   running a cluster again serially when two members came out nearly
+   parallel; mode 1 runs every cluster serially in one lane (LAPACK dstein's
+   order, Gram-Schmidt after every solve); -1 only reads.  *previous gets the
+   mode before the call, *serial_redo the process-wide count of clusters run
+   again serially.  No reference counterpart. */
+int sadmrg_eigh_clusters(int mode, int* previous, long long* serial_redo);
 /* sadmrg_eigh_batched with every k[q] = 0 (eigenvalues only), allocating
    nothing: no hipMalloc, hipFree or allocating library call.  Its working
    memory (tridiagonal factors, publication words, task tables) is a static

@@ -103,6 +103,7 @@ def lib():
         "sadmrg_autotune": (i, [i, i, ip]),
         "sadmrg_svd_gram_min": (i, [i, ip]),
         "sadmrg_svd_jacobi": (i, [i, ip, llp]),
+        "sadmrg_eigh_clusters": (i, [i, ip, llp]),
         "sadmrg_svd_stats": (i, [C.POINTER(C.c_longlong), dp, i]),
         "sadmrg_lanczos_mode": (i, [i, i, ip, C.POINTER(C.c_longlong), i]),
         "sadmrg_gemm_vec_staging": (i, [i, ip]),
@@ -616,6 +617,16 @@ def svd_stats(reset=False):
 
 def _path(directory):
     return b"" if directory is None else os.fsencode(str(directory))
+
+
+def eigh_clusters(mode=-1):
+    """Inverse iteration over clusters of close eigenvalues: 0 members in
+    parallel lanes (default), 1 every cluster serially, -1 only reads.
+    Process-wide.  Returns (previous mode, clusters run again serially)."""
+    prev = C.c_int()
+    cnt = C.c_longlong()
+    check(lib().sadmrg_eigh_clusters(int(mode), C.byref(prev), C.byref(cnt)))
+    return prev.value, int(cnt.value)
 
 
 def svd_jacobi(max_order=-1):

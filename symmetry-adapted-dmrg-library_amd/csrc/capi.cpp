@@ -730,6 +730,15 @@ int sadmrg_eigvalsh_batched_noalloc(sadmrg_ctx* ctx, int count, const int* n, do
   });
 }
 
+int sadmrg_eigh_clusters(int mode, int* previous, long long* serial_redo) {
+  return guard([&] {
+    if (previous) *previous = SymEigBatch::cluster_serial() ? 1 : 0;
+    if (serial_redo) *serial_redo = static_cast<long long>(SymEigBatch::profile().serial_clusters);
+    if (mode == 0 || mode == 1) SymEigBatch::set_cluster_serial(mode == 1);
+    else if (mode != -1) fail(kErrInvalid, "eigh clusters: 0, 1 or -1");
+  });
+}
+
 int sadmrg_svd_eigensolver(int value, int* previous) {
   return guard([&] {
     if (previous) *previous = svd_eigensolver();

@@ -686,6 +686,66 @@ __global__ __launch_bounds__(64) void invit_kernel(const InvTask* __restrict__ t
   }
 }
 
+// Orthonormalisation inside clusters: the members of a cluster run their
+// inverse iterations in separate lanes (each from its own seeded start), so
+// each converges into the cluster's eigenspace independently; afterwards the
+// m member columns of the cluster are made orthonormal by two passes of
+// modified Gram-Schmidt, one workgroup per cluster.  A cluster is a run of
+// eigenvalues that inverse iteration cannot separate, so any orthonormal
+// basis of its span is as good as another -- provided the members came out
+// independent.  A member left with less than kGsKeep of its norm by the
+// first pass (two iterates nearly parallel, so the remainder would carry
+// their rounding errors magnified) flags the cluster, and the host runs that
+// cluster again the serial way: one lane, each member orthogonalised against
+// the earlier ones after every solve.
+constexpr double kGsKeep = 1e-2;
+bool g_cluster_serial = false;  // every cluster the serial way (process-wide)
+
+struct ClusterGs {
+  double* z;  // column c0 of the cluster; column j at z + j * ldz
+  int n, ldz, m, pad;
+};
+
+__device__ inline double block_sum256(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();  // red is free again
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void cluster_gs_kernel(const ClusterGs* __restrict__ cl, int* __restrict__ dep) {
+  __shared__ double red[4];
+  const ClusterGs C = cl[blockIdx.x];
+  int flag = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int a = 0; a < C.m; ++a) {
+      double* za = C.z + static_cast<int64_t>(a) * C.ldz;
+      double q0 = 0.0;
+      if (pass == 0) {
+        double q = 0.0;
+        for (int j = threadIdx.x; j < C.n; j += 256) q = fma(za[j], za[j], q);
+        q0 = block_sum256(q, red);
+      }
+      for (int b = 0; b < a; ++b) {
+        const double* zb = C.z + static_cast<int64_t>(b) * C.ldz;
+        double p = 0.0;
+        for (int j = threadIdx.x; j < C.n; j += 256) p = fma(za[j], zb[j], p);
+        const double dt = block_sum256(p, red);
+        for (int j = threadIdx.x; j < C.n; j += 256) za[j] = fma(-dt, zb[j], za[j]);
+        __syncthreads();
+      }
+      double q = 0.0;
+      for (int j = threadIdx.x; j < C.n; j += 256) q = fma(za[j], za[j], q);
+      const double nrm = block_sum256(q, red);
+      if (pass == 0 && !(nrm > kGsKeep * kGsKeep * q0)) flag = 1;
+      const double sc = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+      for (int j = threadIdx.x; j < C.n; j += 256) za[j] *= sc;
+      __syncthreads();
+    }
+  if (threadIdx.x == 0) dep[blockIdx.x] = flag;
+}
+
 // B = 1.5 I - 0.5 M on every k x k slot (M = Z^T Z): one Newton-Schulz step
 // Z <- Z B towards the orthonormal polar factor of Z.
 __global__ __launch_bounds__(256) void ns_coef_kernel(const double* __restrict__ M, double* __restrict__ B, int k,
@@ -895,6 +955,8 @@ long long wait_ticks() {
 
 long long SymEigBatch::cholqr_fallbacks() { return g_cholqr_fallbacks; }
 EigProfile& SymEigBatch::profile() { return g_prof; }
+void SymEigBatch::set_cluster_serial(bool serial) { g_cluster_serial = serial; }
+bool SymEigBatch::cluster_serial() { return g_cluster_serial; }
 
 int SymEigBatch::max_order() {
   // the largest order whose rows fit the LDS of at most (CUs - kCuMargin)
@@ -1148,6 +1210,8 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
   SADMRG_HIP(hipMemsetAsync(dinfo.get(), 0, sizeof(int) * 2 * ns, ctx->stream()));
   // clusters of eigenvalues closer than kClusterGap |T| (host copy of w)
   std::vector<InvTask> it;
+  std::vector<ClusterGs> cgs;
+  std::vector<InvTask> serial;  // the clusters of cgs as single serial tasks
   for (int s = 0; s < ns; ++s) {
     const EigJob& J = jobs[sel[s]];
     std::vector<double> w(J.n);
@@ -1165,38 +1229,82 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
       int m = 1;
       while (c + m < J.k && m < kMaxCluster && w[J.n - 1 - (c + m - 1)] - w[J.n - 1 - (c + m)] <= kClusterGap * tn)
         ++m;
-      v.c0 = c;
-      v.m = m;
-      it.push_back(v);
+      if (m > 1 && g_cluster_serial) {
+        v.c0 = c;
+        v.m = m;
+        it.push_back(v);
+        g_prof.max_cluster = std::max<int64_t>(g_prof.max_cluster, m);
+        g_prof.vectors += m;
+        g_prof.clusters++;
+        c += m;
+        continue;
+      }
+      // one lane per member: a cluster's members iterate in parallel and
+      // are made orthonormal afterwards (cluster_gs_kernel)
+      for (int a = 0; a < m; ++a) {
+        v.c0 = c + a;
+        v.m = 1;
+        it.push_back(v);
+      }
+      if (m > 1) {
+        cgs.push_back(ClusterGs{v.z + static_cast<int64_t>(c) * v.ldz, v.n, v.ldz, m, 0});
+        serial.push_back(v);
+        serial.back().c0 = c;
+        serial.back().m = m;
+      }
       g_prof.max_cluster = std::max<int64_t>(g_prof.max_cluster, m);
       g_prof.vectors += m;
       g_prof.clusters++;
       c += m;
     }
   }
-  {
-    // waves of up to 64 lanes (clusters) of one matrix, each wave with a
-    // workspace of 4 n x 64 doubles
+  // waves of up to 64 lanes (tasks) of one matrix, each wave with a
+  // workspace of 4 n x 64 doubles; the launch is left in flight
+  auto run_invit = [&](const std::vector<InvTask>& tasks, std::vector<DevBuf>& keep) {
     std::vector<InvWave> wv;
     int64_t ws = 0;
-    for (size_t t0 = 0; t0 < it.size();) {
-      const int n = it[t0].n;
+    for (size_t t0 = 0; t0 < tasks.size();) {
+      const int n = tasks[t0].n;
       int cnt = 0;
-      while (t0 + cnt < it.size() && cnt < 64 && it[t0 + cnt].z == it[t0].z) ++cnt;
+      while (t0 + cnt < tasks.size() && cnt < 64 && tasks[t0 + cnt].z == tasks[t0].z) ++cnt;
       wv.push_back(InvWave{static_cast<int>(t0), cnt, n, ws});
       ws += 4 * int64_t(n) * 64;
       t0 += cnt;
     }
-    DevBuf dt(ctx, it.size() * sizeof(InvTask)), dw(ctx, wv.size() * sizeof(InvWave)), work(ctx, ws * 8);
-    SADMRG_HIP(
-        hipMemcpyAsync(dt.get(), it.data(), it.size() * sizeof(InvTask), hipMemcpyHostToDevice, ctx->stream()));
+    DevBuf dt(ctx, tasks.size() * sizeof(InvTask)), dw(ctx, wv.size() * sizeof(InvWave)), work(ctx, ws * 8);
+    SADMRG_HIP(hipMemcpyAsync(dt.get(), tasks.data(), tasks.size() * sizeof(InvTask), hipMemcpyHostToDevice,
+                              ctx->stream()));
     SADMRG_HIP(
         hipMemcpyAsync(dw.get(), wv.data(), wv.size() * sizeof(InvWave), hipMemcpyHostToDevice, ctx->stream()));
-    ctx->sync();
-    const auto tk = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(invit_kernel, dim3(wv.size()), dim3(64), 0, ctx->stream(), dt.as<InvTask>(), dw.as<InvWave>(),
                        work.as<double>());
     SADMRG_HIP(hipGetLastError());
+    keep.push_back(std::move(dt));
+    keep.push_back(std::move(dw));
+    keep.push_back(std::move(work));
+  };
+  {
+    std::vector<DevBuf> keep;
+    const auto tk = std::chrono::steady_clock::now();
+    run_invit(it, keep);
+    if (!cgs.empty()) {
+      const int nc = static_cast<int>(cgs.size());
+      DevBuf dc(ctx, cgs.size() * sizeof(ClusterGs)), dd(ctx, sizeof(int) * nc);
+      SADMRG_HIP(
+          hipMemcpyAsync(dc.get(), cgs.data(), cgs.size() * sizeof(ClusterGs), hipMemcpyHostToDevice, ctx->stream()));
+      hipLaunchKernelGGL(cluster_gs_kernel, dim3(nc), dim3(256), 0, ctx->stream(), dc.as<ClusterGs>(), dd.as<int>());
+      SADMRG_HIP(hipGetLastError());
+      std::vector<int> dep(nc);
+      SADMRG_HIP(hipMemcpyAsync(dep.data(), dd.get(), sizeof(int) * nc, hipMemcpyDeviceToHost, ctx->stream()));
+      ctx->sync();
+      std::vector<InvTask> redo;
+      for (int q = 0; q < nc; ++q)
+        if (dep[q]) redo.push_back(serial[q]);
+      if (!redo.empty()) {
+        g_prof.serial_clusters += static_cast<int64_t>(redo.size());
+        run_invit(redo, keep);
+      }
+    }
     ctx->sync();
     g_prof.invit_kernel += secs_since(tk);
   }

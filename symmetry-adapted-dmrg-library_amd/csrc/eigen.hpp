@@ -33,6 +33,7 @@ namespace sadmrg {
 struct EigProfile {
   double trd = 0, bisect = 0, invit = 0, invit_kernel = 0, ortho = 0, backt = 0;
   int64_t matrices = 0, launches = 0, vectors = 0, clusters = 0, max_cluster = 0, ns_steps = 0, max_order = 0;
+  int64_t serial_clusters = 0;  // clusters whose members came out dependent and ran again serially
 };
 
 struct EigJob {
@@ -63,6 +64,11 @@ class SymEigBatch {
   // and fell back to Cholesky QR (process-wide count).
   static long long cholqr_fallbacks();
   static EigProfile& profile();
+  // Clusters of close eigenvalues: members iterate in parallel lanes and are
+  // orthonormalised afterwards (false, the default), or each cluster runs in
+  // one lane with Gram-Schmidt after every solve (true).  Process-wide.
+  static void set_cluster_serial(bool serial);
+  static bool cluster_serial();
 
  private:
   Context* ctx_;

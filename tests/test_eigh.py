@@ -1,12 +1,15 @@
 """The batched symmetric eigensolver behind the truncation's Gram route
 (sadmrg_eigh_batched: persistent tridiagonalisation, bisection, inverse
 iteration with Cholesky-QR re-orthonormalisation, back-transformation)
-against numpy's LAPACK eigh on the same matrices.  The matrices are Gram
+against numpy's LAPACK eigh on the same matrices, with the members of
+eigenvalue clusters iterated in parallel lanes and serially
+(sadmrg_eigh_clusters).  The matrices are Gram
 matrices X X^T whose singular values decay over ten decades, with exact
 degenerate pairs (as the entanglement spectra of the symmetric cylinders
 have), plus a random symmetric indefinite one; their orders range from 1 to
 beyond the largest sector of the D=2048 centre bond (1800: rows longer than
-the register-held slices of the reduction), all in one batch."""
+the register-held slices of the reduction), all in one batch, and one with
+runs of eigenvalues a few eps apart."""
 import numpy as np
 import pytest
 
@@ -49,11 +52,31 @@ def _solve(sad, ctx, mats, ks):
     return w, z
 
 
-def test_eigh_batched_matches_lapack(sad, ctx):
+def _near_ties(rng, n):
+    """Runs of eigenvalues a few eps apart (clusters inverse iteration
+    cannot separate, but not exactly degenerate) between well separated
+    ones."""
+    lam = np.linspace(1.0, 0.01, n)
+    eps = np.finfo(float).eps
+    for c in range(0, n - 8, 16):
+        lam[c:c + 6] = lam[c] * (1.0 - eps * np.array([0, 2, 3, 7, 8, 20]))
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (q * lam) @ q.T
+
+
+@pytest.fixture(params=[0, 1], ids=["parallel_members", "serial_clusters"])
+def cluster_mode(sad, request):
+    prev = sad.eigh_clusters(request.param)[0]
+    yield request.param
+    sad.eigh_clusters(prev)
+
+
+def test_eigh_batched_matches_lapack(sad, ctx, cluster_mode):
     rng = np.random.default_rng(2024)
     mats = [_gram(rng, x, pairs=True) for x in ORDERS]
     b = rng.standard_normal((300, 300))
     mats.append(0.5 * (b + b.T))
+    mats.append(_near_ties(rng, 240))
     ks = [max(0, (2 * m.shape[0]) // 5) for m in mats]
     ks[0] = 1
     w, z = _solve(sad, ctx, mats, ks)
@@ -87,3 +110,5 @@ def test_eigh_batched_validates(sad, ctx):
         sad.eigh_batched(ctx, [4], [a.device_ptr()], [4], [w.device_ptr()], [5], [a.device_ptr()], [4])  # k > n
     with pytest.raises(sad.SadmrgError):
         sad.eigh_batched(ctx, [5000], [a.device_ptr()], [5000], [w.device_ptr()])  # order beyond the solver
+    with pytest.raises(sad.SadmrgError):
+        sad.eigh_clusters(2)
