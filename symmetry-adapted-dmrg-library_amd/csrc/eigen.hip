@@ -517,15 +517,19 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// 3. Eigenvectors of T by inverse iteration (LAPACK dstein scheme): one
-// lane per cluster of eigenvalues that inverse iteration cannot separate
-// (neighbours closer than kClusterGap * |T|, a few times the bisection's
-// absolute accuracy), at most kMaxCluster members each.  For each member,
-// from the largest: three solves of (T - lambda I) x = y from a seeded
-// random start, each followed by Gram-Schmidt against the cluster's earlier
-// members and normalisation.  Each solve runs the LU factorisation with
-// partial pivoting (dlagtf) fused with the forward elimination, then the
-// back substitution.
+// 3. Eigenvectors of T by inverse iteration (LAPACK dstein scheme).  A task
+// is one lane: either one eigenvalue, or a cluster of eigenvalues that
+// inverse iteration cannot separate (neighbours closer than kClusterGap * |T|,
+// a few times the bisection's absolute accuracy; at most kMaxCluster
+// members) run serially.  By default the members of a cluster are separate
+// tasks and cluster_gs_kernel orthonormalises them afterwards.  For each
+// member, from the largest: three solves of (T - lambda I) x = y from a
+// seeded random start, each followed by Gram-Schmidt against the task's
+// earlier members and normalisation.  For a task's first member the norm is
+// summed by the back substitution and the scaling is applied as the next
+// solve loads y, so no serial pass over the iterate remains besides the
+// solves.  Each solve runs the LU factorisation with partial pivoting
+// (dlagtf) fused with the forward elimination, then the back substitution.
 //
 // Eigenvalues further apart converge to vectors whose overlaps are about
 // eps |T| / gap (at most ~0.03), which the orthonormalisation after this
@@ -536,7 +540,7 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
 // well-conditioned, vectors of the same eigenspace.
 //
 // The recurrences are sequential along the matrix, so the parallelism is
-// across lanes: all 64 lanes of a wave serve clusters of one matrix.  A
+// across lanes: all 64 lanes of a wave serve tasks of one matrix.  A
 // lane's LU factors and iterate (4 n doubles) live in a global workspace
 // interleaved by lane -- entry j of array a of lane l at (a n + j) 64 + l --
 // so every access of the wave is one coalesced 512-byte line.  The forward
@@ -595,15 +599,16 @@ __global__ __launch_bounds__(64) void invit_kernel(const InvTask* __restrict__ t
     const double lam = T.w[n - 1 - c];
     const uint64_t seed = (static_cast<uint64_t>(n) << 44) ^ (static_cast<uint64_t>(c) << 24);
     for (int j = 0; j < n; ++j) at(Y, j) = seeded_uniform(seed + j);
+    double pend = 1.0;  // normalisation of y not yet applied (applied as y is loaded)
     for (int it = 0; it < 3; ++it) {
       // factorise T - lambda I = P L U and apply L^-1 P to y in one sweep;
       // y[j+1] of a block of steps is loaded before the block (step j
       // stores y[j] only, so the loaded entries are still the inputs)
-      double aj = sD[0] - lam, bj = sE[0], cur = at(Y, 0);
+      double aj = sD[0] - lam, bj = sE[0], cur = at(Y, 0) * pend;
       for (int j0 = 0; j0 + 1 < n; j0 += kBlk) {
         double yb[kBlk];
 #pragma unroll
-        for (int q = 0; q < kBlk; ++q) yb[q] = j0 + q + 1 < n ? at(Y, j0 + q + 1) : 0.0;
+        for (int q = 0; q < kBlk; ++q) yb[q] = j0 + q + 1 < n ? at(Y, j0 + q + 1) * pend : 0.0;
 #pragma unroll
         for (int q = 0; q < kBlk; ++q) {
           const int j = j0 + q;
@@ -638,8 +643,9 @@ __global__ __launch_bounds__(64) void invit_kernel(const InvTask* __restrict__ t
       at(Y, n - 1) = cur;
       // back substitution (tiny pivots replaced by +-eps |T|), rescaling when
       // the iterate grows large; a block of entries is loaded before it is
-      // used (the rescaling touches only entries already solved)
-      double x1 = 0.0, x2 = 0.0;
+      // used (the rescaling touches only entries already solved); the
+      // squared norm accumulates on the way
+      double x1 = 0.0, x2 = 0.0, nrm = 0.0;
       for (int j1 = n - 1; j1 >= 0; j1 -= kBlk) {
         double ry[kBlk], r0[kBlk], r1[kBlk], r2[kBlk];
 #pragma unroll
@@ -663,26 +669,36 @@ __global__ __launch_bounds__(64) void invit_kernel(const InvTask* __restrict__ t
             x *= 1e-150;
             x1 *= 1e-150;
             x2 *= 1e-150;
+            nrm *= 1e-300;
           }
           at(Y, j) = x;
+          nrm = fma(x, x, nrm);
           x2 = x1;
           x1 = x;
         }
       }
-      // Gram-Schmidt against the cluster's earlier members, then normalise
-      for (int b = 0; b < a; ++b) {
-        const double* zb = T.z + static_cast<size_t>(T.c0 + b) * T.ldz;
-        double dt = 0.0;
-        for (int j = 0; j < n; ++j) dt = fma(zb[j], at(Y, j), dt);
-        for (int j = 0; j < n; ++j) at(Y, j) = fma(-dt, zb[j], at(Y, j));
+      if (a > 0) {
+        // Gram-Schmidt against the cluster's earlier members, then the norm
+        for (int b = 0; b < a; ++b) {
+          const double* zb = T.z + static_cast<size_t>(T.c0 + b) * T.ldz;
+          double dt = 0.0;
+          for (int j = 0; j < n; ++j) dt = fma(zb[j], at(Y, j), dt);
+          for (int j = 0; j < n; ++j) at(Y, j) = fma(-dt, zb[j], at(Y, j));
+        }
+        nrm = 0.0;
+        for (int j = 0; j < n; ++j) nrm = fma(at(Y, j), at(Y, j), nrm);
       }
-      double nrm = 0.0;
-      for (int j = 0; j < n; ++j) nrm = fma(at(Y, j), at(Y, j), nrm);
-      const double s = 1.0 / sqrt(nrm);
-      for (int j = 0; j < n; ++j) at(Y, j) *= s;
+      pend = 1.0 / sqrt(nrm);
     }
     double* zc = T.z + static_cast<size_t>(c) * T.ldz;
-    for (int j = 0; j < n; ++j) zc[j] = at(Y, j);
+    for (int j0 = 0; j0 < n; j0 += kBlk) {
+      double t[kBlk];
+#pragma unroll
+      for (int q = 0; q < kBlk; ++q) t[q] = j0 + q < n ? at(Y, j0 + q) : 0.0;
+#pragma unroll
+      for (int q = 0; q < kBlk; ++q)
+        if (j0 + q < n) zc[j0 + q] = t[q] * pend;
+    }
   }
 }
 
