@@ -440,43 +440,54 @@ hipError_t launch_trd(const TrdTask* tasks, int ntask, int* err, long long wait_
 // LAPACK dstebz, 65-way): one wave per eigenvalue, lane l evaluates the
 // count at the l-th of 64 interior points of the current interval, and the
 // interval shrinks to the sub-interval where the count passes the index, a
-// factor 65 per round (nine rounds to working precision).
+// factor 65 per round (nine rounds to working precision).  A workgroup
+// holds four eigenvalues of one matrix and first copies the diagonal and the
+// squared off-diagonal into LDS, so the count's per-step operands are LDS
+// broadcasts instead of global loads.
+constexpr int kBisOrder = 2048;  // LDS rows of the copy; no larger order reaches the solver
+
 struct BisTask {
   const double* d;
   const double* e;
   double* w;
-  int n, off;
+  int n, off;  // off: the matrix's first workgroup
 };
 
-__device__ inline int sturm_count(const double* __restrict__ d, const double* __restrict__ e, int n, double x,
-                                  double pivmin) {
+__device__ inline int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   int c = q < 0.0;
-  double dn = n > 1 ? d[1] : 0.0, en = n > 1 ? e[0] : 0.0;
+  double dn = n > 1 ? d[1] : 0.0, en = n > 1 ? e2[0] : 0.0;
   for (int j = 1; j < n; ++j) {
-    const double dj = dn, ej = en;
+    const double dj = dn, e2j = en;
     if (j + 1 < n) {  // operands of the next step, loaded ahead of the division chain
       dn = d[j + 1];
-      en = e[j];
+      en = e2[j];
     }
-    q = dj - x - ej * ej / q;
+    q = dj - x - e2j / q;
     if (fabs(q) < pivmin) q = -pivmin;
     c += q < 0.0;
   }
   return c;
 }
 
-__global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__ tasks, int ntask, int total) {
-  const int gid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // one wave per eigenvalue
-  const int lane = threadIdx.x & 63;
-  if (gid >= total) return;
+__global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__ tasks, int ntask) {
+  __shared__ double sD[kBisOrder], sE2[kBisOrder];
+  const int blk = blockIdx.x;
   int t = 0;
-  while (t + 1 < ntask && gid >= tasks[t + 1].off) ++t;
+  while (t + 1 < ntask && blk >= tasks[t + 1].off) ++t;
   const BisTask T = tasks[t];
-  const int i = gid - T.off, n = T.n;
+  const int n = T.n;
   const double* d = T.d;
   const double* e = T.e;
+  for (int j = threadIdx.x; j < n; j += 256) {
+    sD[j] = d[j];
+    sE2[j] = j + 1 < n ? e[j] * e[j] : 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int i = (blk - T.off) * 4 + (threadIdx.x >> 6);  // one wave per eigenvalue
+  if (i >= n) return;
   // Gershgorin interval and pivot threshold, lanes over the rows
   double lo = INFINITY, hi = -INFINITY, emax2 = 0.0;
   for (int j = lane; j < n; j += 64) {
@@ -501,7 +512,7 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
     if (hi - lo <= fmax(abstol, 2.0 * eps * fmax(fabs(lo), fabs(hi)))) break;
     const double step = (hi - lo) / 65.0;
     const double x = lo + step * (lane + 1);
-    const bool above = sturm_count(d, e, n, x, pivmin) > i;
+    const bool above = sturm_count(sD, sE2, n, x, pivmin) > i;
     const unsigned long long m = __ballot(above);
     if (m == 0) {
       lo = __shfl(x, 63, 64);
@@ -1119,9 +1130,10 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
     }
     launches.push_back(l);
   }
-  int total = 0, nb = 0;
+  int total = 0, nb = 0;  // total: workgroups of the bisection (four eigenvalues each)
   for (int q = 0; q < nj; ++q) {
     if (jobs[q].n == 0) continue;
+    if (jobs[q].n > kBisOrder) fail(kErrInvalid, "eigensolver: matrix order beyond the bisection's LDS copy");
     BisTask b;
     b.d = base + L.doff[q];
     b.e = b.d + jobs[q].n;
@@ -1130,7 +1142,7 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
     b.off = total;
     std::memcpy(hbis + nb, &b, sizeof(BisTask));
     ++nb;
-    total += jobs[q].n;
+    total += (jobs[q].n + 3) / 4;
   }
   if (total == 0) return;
   SADMRG_HIP(hipMemcpyAsync(mem + L.tasks, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream()));
@@ -1155,7 +1167,7 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
   t0 = std::chrono::steady_clock::now();
   g_prof.matrices += nj;
   for (const EigJob& J : jobs) g_prof.max_order = std::max<int64_t>(g_prof.max_order, J.n);
-  hipLaunchKernelGGL(bisect_kernel, dim3((total + 3) / 4), dim3(256), 0, ctx->stream(), dbis, nb, total);
+  hipLaunchKernelGGL(bisect_kernel, dim3(total), dim3(256), 0, ctx->stream(), dbis, nb);
   SADMRG_HIP(hipGetLastError());
   ctx->sync();
   g_prof.bisect += secs_since(t0);
