@@ -1240,11 +1240,16 @@ void SymEigBatch::top_vectors(const std::vector<EigJob>& jobs) {
   std::vector<InvTask> it;
   std::vector<ClusterGs> cgs;
   std::vector<InvTask> serial;  // the clusters of cgs as single serial tasks
+  std::vector<std::vector<double>> hw(ns);
   for (int s = 0; s < ns; ++s) {
     const EigJob& J = jobs[sel[s]];
-    std::vector<double> w(J.n);
-    SADMRG_HIP(hipMemcpyAsync(w.data(), J.w, J.n * 8, hipMemcpyDeviceToHost, ctx->stream()));
-    ctx->sync();
+    hw[s].resize(J.n);
+    SADMRG_HIP(hipMemcpyAsync(hw[s].data(), J.w, J.n * 8, hipMemcpyDeviceToHost, ctx->stream()));
+  }
+  ctx->sync();
+  for (int s = 0; s < ns; ++s) {
+    const EigJob& J = jobs[sel[s]];
+    const std::vector<double>& w = hw[s];
     const double tn = std::max(std::fabs(w[0]), std::fabs(w[J.n - 1]));
     InvTask v;
     v.d = base + doff_[sel[s]];
