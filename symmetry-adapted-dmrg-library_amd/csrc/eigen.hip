@@ -437,14 +437,17 @@ hipError_t launch_trd(const TrdTask* tasks, int ntask, int* err, long long wait_
 
 // ---------------------------------------------------------------------------
 // 2. Eigenvalues of T by multisection on the Sturm count (the bisection of
-// LAPACK dstebz, 65-way): one wave per eigenvalue, lane l evaluates the
-// count at the l-th of 64 interior points of the current interval, and the
-// interval shrinks to the sub-interval where the count passes the index, a
-// factor 65 per round (nine rounds to working precision).  A workgroup
-// holds four eigenvalues of one matrix and first copies the diagonal and the
-// squared off-diagonal into LDS, so the count's per-step operands are LDS
+// LAPACK dstebz, 17-way): 16 lanes per eigenvalue, lane l evaluating the
+// count at the l-th of 16 interior points of the current interval, which
+// shrinks to the sub-interval where the count passes the index, a factor 17
+// per round (about 13 rounds to working precision: 208 counts per
+// eigenvalue, where 64 lanes and 65-way cuts need 576 for nine rounds, and
+// the counts are what the kernel's time goes to).  A workgroup holds 16
+// eigenvalues of one matrix and first copies the diagonal and the squared
+// off-diagonal into LDS, so the count's per-step operands are LDS
 // broadcasts instead of global loads.
 constexpr int kBisOrder = 2048;  // LDS rows of the copy; no larger order reaches the solver
+constexpr int kBisPerBlock = 16;  // eigenvalues per workgroup: 16 lanes each
 
 struct BisTask {
   const double* d;
@@ -486,8 +489,10 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int i = (blk - T.off) * 4 + (threadIdx.x >> 6);  // one wave per eigenvalue
-  if (i >= n) return;
+  const int grp = lane >> 4, sub = lane & 15;  // 16 lanes per eigenvalue, four per wave
+  const int i0 = (blk - T.off) * kBisPerBlock + (threadIdx.x >> 6) * 4;
+  if (i0 >= n) return;  // the whole wave is past the matrix
+  const int i = i0 + grp;
   // Gershgorin interval and pivot threshold, lanes over the rows
   double lo = INFINITY, hi = -INFINITY, emax2 = 0.0;
   for (int j = lane; j < n; j += 64) {
@@ -508,23 +513,29 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
   lo -= fudge;
   hi += fudge;
   const double abstol = eps * tnorm;
+  // lo and hi are uniform over a group; a group stops when its interval has
+  // converged, the wave when every group has
+  bool done = i >= n;
   for (int it = 0; it < 64; ++it) {
-    if (hi - lo <= fmax(abstol, 2.0 * eps * fmax(fabs(lo), fabs(hi)))) break;
-    const double step = (hi - lo) / 65.0;
-    const double x = lo + step * (lane + 1);
-    const bool above = sturm_count(sD, sE2, n, x, pivmin) > i;
-    const unsigned long long m = __ballot(above);
-    if (m == 0) {
-      lo = __shfl(x, 63, 64);
-    } else {
-      const int l = __ffsll(static_cast<long long>(m)) - 1;
-      const double nhi = __shfl(x, l, 64);
-      const double nlo = l > 0 ? __shfl(x, l - 1, 64) : lo;
-      lo = nlo;
-      hi = nhi;
+    done = done || hi - lo <= fmax(abstol, 2.0 * eps * fmax(fabs(lo), fabs(hi)));
+    if (__ballot(!done) == 0) break;
+    const double step = (hi - lo) / 17.0;
+    const double x = lo + step * (sub + 1);
+    const bool above = !done && sturm_count(sD, sE2, n, x, pivmin) > i;
+    const unsigned gm = static_cast<unsigned>(__ballot(above) >> (grp * 16)) & 0xFFFFu;
+    const int l = gm ? __builtin_ctz(gm) : 15;
+    const double xl = __shfl(x, grp * 16 + l, 64);
+    const double xm = __shfl(x, grp * 16 + (l > 0 ? l - 1 : 0), 64);
+    if (!done) {
+      if (gm == 0) {
+        lo = xl;  // above the group's highest point
+      } else {
+        hi = xl;
+        if (l > 0) lo = xm;
+      }
     }
   }
-  if (lane == 0) T.w[i] = 0.5 * (lo + hi);
+  if (sub == 0 && i < n) T.w[i] = 0.5 * (lo + hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,7 +1141,7 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
     }
     launches.push_back(l);
   }
-  int total = 0, nb = 0;  // total: workgroups of the bisection (four eigenvalues each)
+  int total = 0, nb = 0;  // total: workgroups of the bisection (kBisPerBlock eigenvalues each)
   for (int q = 0; q < nj; ++q) {
     if (jobs[q].n == 0) continue;
     if (jobs[q].n > kBisOrder) fail(kErrInvalid, "eigensolver: matrix order beyond the bisection's LDS copy");
@@ -1142,7 +1153,7 @@ void phase1(Context* ctx, const std::vector<EigJob>& jobs, const Phase1Layout& L
     b.off = total;
     std::memcpy(hbis + nb, &b, sizeof(BisTask));
     ++nb;
-    total += (jobs[q].n + 3) / 4;
+    total += (jobs[q].n + kBisPerBlock - 1) / kBisPerBlock;
   }
   if (total == 0) return;
   SADMRG_HIP(hipMemcpyAsync(mem + L.tasks, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream()));
