@@ -441,13 +441,15 @@ hipError_t launch_trd(const TrdTask* tasks, int ntask, int* err, long long wait_
 // count at the l-th of 16 interior points of the current interval, which
 // shrinks to the sub-interval where the count passes the index, a factor 17
 // per round (about 13 rounds to working precision: 208 counts per
-// eigenvalue, where 64 lanes and 65-way cuts need 576 for nine rounds, and
-// the counts are what the kernel's time goes to).  A workgroup holds 16
+// eigenvalue, where 64 lanes and 65-way cuts need 576 for nine rounds; the
+// counts fill the machine, so their number sets the time, while 8 lanes and
+// 9-way cuts make the 17 dependent rounds the bound).  A workgroup holds 16
 // eigenvalues of one matrix and first copies the diagonal and the squared
 // off-diagonal into LDS, so the count's per-step operands are LDS
 // broadcasts instead of global loads.
 constexpr int kBisOrder = 2048;  // LDS rows of the copy; no larger order reaches the solver
-constexpr int kBisPerBlock = 16;  // eigenvalues per workgroup: 16 lanes each
+constexpr int kBisLanes = 16;                    // lanes (interior points) per eigenvalue
+constexpr int kBisPerBlock = 4 * 64 / kBisLanes;  // eigenvalues per workgroup of four waves
 
 struct BisTask {
   const double* d;
@@ -489,8 +491,8 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int grp = lane >> 4, sub = lane & 15;  // 16 lanes per eigenvalue, four per wave
-  const int i0 = (blk - T.off) * kBisPerBlock + (threadIdx.x >> 6) * 4;
+  const int grp = lane / kBisLanes, sub = lane % kBisLanes;  // kBisLanes lanes per eigenvalue
+  const int i0 = (blk - T.off) * kBisPerBlock + (threadIdx.x >> 6) * (64 / kBisLanes);
   if (i0 >= n) return;  // the whole wave is past the matrix
   const int i = i0 + grp;
   // Gershgorin interval and pivot threshold, lanes over the rows
@@ -519,13 +521,13 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
   for (int it = 0; it < 64; ++it) {
     done = done || hi - lo <= fmax(abstol, 2.0 * eps * fmax(fabs(lo), fabs(hi)));
     if (__ballot(!done) == 0) break;
-    const double step = (hi - lo) / 17.0;
+    const double step = (hi - lo) / (kBisLanes + 1);
     const double x = lo + step * (sub + 1);
     const bool above = !done && sturm_count(sD, sE2, n, x, pivmin) > i;
-    const unsigned gm = static_cast<unsigned>(__ballot(above) >> (grp * 16)) & 0xFFFFu;
-    const int l = gm ? __builtin_ctz(gm) : 15;
-    const double xl = __shfl(x, grp * 16 + l, 64);
-    const double xm = __shfl(x, grp * 16 + (l > 0 ? l - 1 : 0), 64);
+    const unsigned gm = static_cast<unsigned>(__ballot(above) >> (grp * kBisLanes)) & ((1u << kBisLanes) - 1u);
+    const int l = gm ? __builtin_ctz(gm) : kBisLanes - 1;
+    const double xl = __shfl(x, grp * kBisLanes + l, 64);
+    const double xm = __shfl(x, grp * kBisLanes + (l > 0 ? l - 1 : 0), 64);
     if (!done) {
       if (gm == 0) {
         lo = xl;  // above the group's highest point
@@ -536,6 +538,7 @@ __global__ __launch_bounds__(256) void bisect_kernel(const BisTask* __restrict__
     }
   }
   if (sub == 0 && i < n) T.w[i] = 0.5 * (lo + hi);
+  static_assert(64 % kBisLanes == 0 && kBisLanes <= 32, "lane groups tile a wave");
 }
 
 // ---------------------------------------------------------------------------
